@@ -1,0 +1,149 @@
+/*
+ * tc3 -- MI355X three-phonon scattering and lattice thermal conductivity engine.
+ *
+ * C ABI of libtc3.so.  Plain pointers and sizes, no C++ or torch types.  All
+ * quantities are in the reference's internal Hartree atomic units (lengths in
+ * Bohr, reciprocal vectors in 1/Bohr without 2*pi, masses in electron masses,
+ * IFCs in Hartree/Bohr^n, frequencies in Hartree).  Fortran column-major
+ * conventions are kept for 3-vectors and 3x3 matrices: v[3*i + c] is
+ * component c of vector i, and m[3*j + i] is element (i, j).
+ *
+ * Every function returns 0 on success or a positive status that follows the
+ * reference's exit codes (src/libolle/konstanter.f90:160-172):
+ *   1 bad dimension, 2 eigen-solver failure, 3 unphysical input,
+ *   4 symmetry failure, 5 bad parameter, 6 I/O, 7 device error.
+ * tc3_last_error() returns the message of the last failure on a context.
+ * No C++ exception crosses this boundary.  One context per host thread; calls
+ * on one context are ordered and not re-entrant.  Host arrays belong to the
+ * caller and are copied in; device memory belongs to the context and is
+ * released by tc3_destroy.  One GPU, no collectives.
+ */
+#ifndef TC3_H
+#define TC3_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tc3_ctx tc3_ctx;
+
+enum {
+    TC3_PREC_FP64 = 0,   /* |Psi|^2 contraction in FP64                       */
+    TC3_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulation                       */
+    TC3_PREC_BF16X3 = 2  /* split-BF16 x3 MFMA, FP32 accumulation              */
+};
+
+/* Create / destroy a context bound to one HIP device.  No counterpart in the
+ * reference (its state lives in Fortran derived types owned by main.f90). */
+int tc3_create(tc3_ctx **ctx, int device);
+void tc3_destroy(tc3_ctx *ctx);
+const char *tc3_last_error(const tc3_ctx *ctx);
+
+/* Crystal structure: replaces lo_crystalstructure%readfromfile + readisotopefromfile
+ * (type_crystalstructure_io.f90:9-350, :921-959).  latvec: 3x3 lattice vectors as
+ * columns (Bohr); rfrac: 3*na fractional positions; species: na species ids (atoms
+ * with equal ids are interchangeable by symmetry); mass: na masses (emu). */
+int tc3_set_structure(tc3_ctx *ctx, int na, const double *latvec, const double *rfrac,
+                      const int *species, const double *mass);
+
+/* Second-order IFCs: replaces lo_forceconstant_secondorder%readfromfile
+ * (type_forceconstant_secondorder_io.f90:109-247), non-polar.  a1, a2: 0-based atoms;
+ * lv2: 3*npair Cartesian lattice vectors (Bohr); m: 9*npair tensors (Ha/Bohr^2),
+ * m[9*n + 3*j + i] = Phi_ij. */
+int tc3_set_fc2(tc3_ctx *ctx, int npair, const int *a1, const int *a2, const double *lv2,
+                const double *m);
+
+/* Third-order IFCs: replaces lo_forceconstant_thirdorder%readfromfile
+ * (type_forceconstant_thirdorder.f90:243-311).  lv2, lv3: 3*ntrip Cartesian lattice
+ * vectors (Bohr); m: 27*ntrip tensors (Ha/Bohr^3), m[27*t + 9*k + 3*j + i] = Phi_ijk.
+ * The mass weighting (mwm, :277) is applied inside. */
+int tc3_set_fc3(tc3_ctx *ctx, int ntrip, const int *a1, const int *a2, const int *a3,
+                const double *lv2, const double *lv3, const double *m);
+
+/* Space group, Gamma-centred FFT q-mesh, irreducible points, little groups and the
+ * irreducible triplet lists: replaces uc%classify('wedge') (type_crystalstructure_symmetry.f90:7-91),
+ * lo_generate_qmesh (type_qpointmesh.f90:452-657, type_qpointmesh_gridgeneration.f90:8-545)
+ * and triplet_is_irreducible (scattering_threephonon.f90:319-448).  Host-side. */
+int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal);
+
+/* Sizes of what tc3_set_qmesh built. */
+int tc3_mesh_info(const tc3_ctx *ctx, int *nsym, int *nfull, int *nirr, int64_t *ntriplets,
+                  int *nrows);
+
+/* Mesh arrays: irreducible full indices (nirr), irreducible index of every full point
+ * (nfull), operation from irreducible (nfull, 1-based, negative = time reversal),
+ * integration weights (nirr), Cartesian q of every full point (3*nfull). */
+int tc3_get_mesh(const tc3_ctx *ctx, int *irr_full_index, int *irr_of_full, int *op_of_full,
+                 double *weight_irr, double *qcart);
+
+/* Symmetry operations: Cartesian rotations (9*nsym, column-major), Cartesian
+ * translations (3*nsym), atom maps (na*nsym). */
+int tc3_get_symmetry(const tc3_ctx *ctx, double *rot, double *tr, int *fmap);
+
+/* Harmonic properties on the mesh on the GPU: dynamical matrix, FP64 Hermitian
+ * eigen-solve, degenerate-subspace rotation, group velocities, unfolding to the full
+ * grid.  Replaces lo_phonon_dispersions%generate (type_phonon_dispersions_generation.f90:8-241)
+ * with fc%dynamicalmatrix / frequencies_eigenvectors_groupvelocities
+ * (type_forceconstant_secondorder_dynamicalmatrix.f90:13-481, :484-788).
+ * omega_min receives the smallest non-zero frequency (unstable if < -1e-9). */
+int tc3_harmonic(tc3_ctx *ctx, double *omega_min);
+
+/* Copy harmonic results to the host.  omega: nq*nb, vel: nq*nb*3, egv: nq*nb*nb
+ * complex as interleaved (re, im) pairs, egv[(q*nb + mode)*nb + component]
+ * (the reference's egv(:, mode)).  full = 0 for irreducible points, 1 for the full grid.
+ * Any pointer may be NULL. */
+int tc3_get_harmonic(const tc3_ctx *ctx, int full, double *omega, double *vel, double *egv);
+
+/* Three-phonon scattering rates and scattering matrix for nT temperatures at once:
+ * replaces lo_scattering_rates%generate (scattering.f90:58-358) with
+ * compute_threephonon_scattering (scattering_threephonon.f90:2-217) for
+ * qg3ph = qgrid, no isotope and no boundary terms.  integrationtype 1 (fixed sigma
+ * in THz) or 2 (adaptive); precision one of TC3_PREC_*.  The scattering matrix stays
+ * on the device. */
+int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
+                int integrationtype, double sigma, int classical);
+
+/* Scattering results for temperature iT: linewidth (nirr*nb), qs = 2*linewidth,
+ * F0 (nirr*nb*3), scalar mean free path (nirr*nb).  Any pointer may be NULL. */
+int tc3_get_scattering(const tc3_ctx *ctx, int iT, double *linewidth, double *F0,
+                       double *scalar_mfp);
+
+/* Copy rows [row0, row0+nrow) of the symmetrised scattering matrix Xi of temperature iT
+ * (each row nfull*nb doubles) and the (q_irr, band) of every row (2*nrows ints). */
+int tc3_get_xi(const tc3_ctx *ctx, int iT, int row0, int nrow, double *xi, int *row_qb);
+
+/* Thermal conductivity for every scattered temperature: SMA (get_kappa, kappa.f90:28-75),
+ * iterative BTE (iterative_solution, kappa.f90:326-495, FP64 on the GPU) and the
+ * off-diagonal coherence term (get_kappa_offdiag, kappa.f90:77-306), each symmetrised
+ * (symmetrize_kappa, kappa.f90:308-324).  Outputs nT*9 (atomic units, column-major 3x3);
+ * niter receives the iteration count per temperature (may be NULL); kappa_offdiag may be
+ * NULL to skip the coherence term.  maxiter <= 0 skips the iterative solution. */
+int tc3_kappa(tc3_ctx *ctx, int maxiter, double tol, int classical, double *kappa_sma,
+              double *kappa_iter, double *kappa_offdiag, int *niter);
+
+/* Per-iteration log of the last tc3_kappa call for temperature iT: up to maxlen
+ * records of (kxx, kyy, kzz, kxy, kxz, kyz, DeltaF/F) in atomic units
+ * (kappa.f90:377, 473-474).  Returns the number of records in *nlog. */
+int tc3_get_kappa_log(const tc3_ctx *ctx, int iT, int maxlen, double *log, int *nlog);
+
+/* Final per-mode quantities of temperature iT after tc3_kappa: Fn (nirr*nb*3, before
+ * the omega/T prefactor) and per-mode kappa (nirr*nb*9, atomic units). */
+int tc3_get_modes(const tc3_ctx *ctx, int iT, double *Fn, double *kappa_mode);
+
+/* Wall times (seconds) of the last calls: [0] harmonic, [1] scatter, [2] kappa total,
+ * [3] BTE only, [4] off-diagonal only, [5] scatter kernel time (HIP events).  n <= 8. */
+int tc3_get_timings(const tc3_ctx *ctx, double *t, int n);
+
+/* Write outfile.thermal_conductivity_grid.hdf5 for temperature iT (layout of
+ * lo_phonon_dispersions%write_to_hdf5, type_phonon_dispersions.f90:316-641, after the
+ * Fn*omega/T prefactor of main.f90:323-328). */
+int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TC3_H */

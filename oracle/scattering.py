@@ -1,0 +1,259 @@
+"""Three-phonon scattering rates and scattering matrix, restated in numpy.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Follows src/thermal_conductivity/scattering.f90 (lo_scattering_rates%generate
+:58-358) and scattering_threephonon.f90 (compute_threephonon_scattering :2-217,
+get_dirac :219-271, pretransform_phi3 :275-317, triplet_is_irreducible
+:319-448).  The per-row loop of the reference is kept, except that the |Psi|^2
+of one (q1, q2, q3) triplet is formed for all b1 at once (the triplet set of a
+row depends on q1 only); `psi_reference_form` keeps the reference's own
+zgeru/dot_product form for small cross-checks.
+"""
+import numpy as np
+
+from . import units as U
+
+THREEPHONON_PREFACTOR = U.PI / 16.0  # scattering.f90:30
+
+
+def third_grid_index(i1, i2, dims):
+    """fft_third_grid_index (lo_fftgrid_helper.f90:136-163), 0-based indices."""
+    n1, n2, n3 = dims
+    g1 = np.array([i1 // (n2 * n3), (i1 // n3) % n2, i1 % n3])
+    g2 = np.array([i2 // (n2 * n3), (i2 // n3) % n2, i2 % n3])
+    g3 = (-g1 - g2) % np.array(dims)
+    return int(g3[0] * n2 * n3 + g3[1] * n3 + g3[2])
+
+
+def rotation_table(p, mesh, ops):
+    """rot[o, q] = grid index of R_o q (reciprocal fractional), -1 if off grid."""
+    from .lattice import _on_grid, _index_from_coordinate
+    dims = mesh.dims
+    frac = mesh.qcart @ p.inv_reclat.T
+    rot = np.full((len(ops), mesh.nfull), -1, dtype=np.int64)
+    for o, op in enumerate(ops):
+        for q in range(mesh.nfull):
+            v1 = op.rfm @ frac[q]
+            if _on_grid(v1, dims):
+                rot[o, q] = mesh.gi2ind(_index_from_coordinate(v1, dims))
+    return rot
+
+
+def irreducible_triplets(mesh, rot, q1):
+    """Visited triplets of irreducible point q1 (compute_threephonon_scattering :72-84 with
+    triplet_is_irreducible :319-448): list of (q2, q3, perm, red_triplet)."""
+    lg = mesh.littlegroup_irr[q1]
+    q1f = mesh.irr_full_index[q1]
+    out = []
+    for q2 in range(mesh.nfull):
+        q3 = third_grid_index(q1f, q2, mesh.dims)
+        if q3 < q2:
+            continue
+        newqp = []
+        isred = False
+        for k in lg:
+            a = rot[k, q2]
+            if a < 0:
+                newqp.append(None)
+                continue
+            b = rot[k, q3]
+            if b < 0:
+                newqp.append(None)
+                continue
+            if min(a, b) > q2:
+                isred = True
+            newqp.append((a, b))
+        if isred:
+            continue
+        newqp = [(q2, q3) if x is None else x for x in newqp]
+        srt = [tuple(sorted(x)) for x in newqp]
+        red = []
+        n = len(newqp)
+        for j in range(n):
+            if not any(srt[j] == srt[k] for k in range(j + 1, n)):
+                red.append(newqp[j])
+        perm = 1.0 if q2 == q3 else 2.0
+        out.append((q2, q3, perm, red))
+    return out
+
+
+def pretransform_phi3(fc3, q2, q3, nb):
+    """pretransform_phi3 (scattering_threephonon.f90:275-317) as a (nb, nb, nb) array
+    indexed [ia, ib, ic]."""
+    ph = np.exp(-1j * U.TWOPI * (fc3["lv2"] @ q2 + fc3["lv3"] @ q3))
+    ptf = np.zeros((nb, nb, nb), dtype=np.complex128)
+    for t in range(len(ph)):
+        a1, a2, a3 = fc3["a1"][t], fc3["a2"][t], fc3["a3"][t]
+        ptf[3 * a1:3 * a1 + 3, 3 * a2:3 * a2 + 3, 3 * a3:3 * a3 + 3] += fc3["mwm"][t] * ph[t]
+    return ptf
+
+
+def psi_reference_form(egv1, egv2, egv3, ptf):
+    """One matrix element exactly as scattering_threephonon.f90:96-112 forms it:
+    evp1 = zgeru(egv2, egv1); evp2 = conj(zgeru(egv3, evp1)); c0 = dot_product(evp2, ptf)."""
+    nb = len(egv1)
+    evp1 = np.zeros(nb * nb, dtype=np.complex128)
+    for j in range(nb):            # zgeru: A(i,j) += x(i) y(j), column-major
+        evp1[j * nb:(j + 1) * nb] += egv2 * egv1[j]
+    evp2 = np.zeros(nb ** 3, dtype=np.complex128)
+    for j in range(nb * nb):
+        evp2[j * nb:(j + 1) * nb] += egv3 * evp1[j]
+    evp2 = np.conj(evp2)
+    # ptf flat index l = ia*nb^2 + ib*nb + ic ; evp2 flat index = ic + nb*(ib + nb*ia)
+    flat = ptf.reshape(-1)
+    return np.sum(np.conj(evp2) * flat)   # dot_product conjugates its first argument
+
+
+def psi_chain(E1, E2, E3, ptf):
+    """Psi[b1,b2,b3] = sum E1[ia,b1] E2[ib,b2] E3[ic,b3] ptf[ia,ib,ic] as three GEMM steps."""
+    t1 = np.tensordot(E1, ptf, axes=([0], [0]))          # [b1, ib, ic]
+    t2 = np.einsum("ib,aic->abc", E2, t1, optimize=True)  # [b1, b2, ic]
+    return np.einsum("abc,cd->abd", t2, E3, optimize=True)
+
+
+class ScatteringRates:
+    pass
+
+
+def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
+                     integrationtype=2, sigma=1.0, rows_only=None):
+    """lo_scattering_rates%generate (scattering.f90:58-358), three-phonon only,
+    qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q)."""
+    nb = disp.nb
+    nirr, nfull = mesh.nirr, mesh.nfull
+    dims = mesh.dims
+    sr = ScatteringRates()
+    # occupations and adaptive smearing (:143-167)
+    be = np.zeros((nirr, nb))
+    sigsq = np.zeros((nirr, nb))
+    sigavg = np.zeros(nb)
+    reclat = p.reclat / np.array(dims)[None, :]
+    for q in range(nirr):
+        for b in range(nb):
+            om = disp.om_irr[q, b]
+            if om < U.FREQTOL:
+                continue
+            if classical:
+                be[q, b] = U.KB_HARTREE * temperature / om
+            else:
+                be[q, b] = U.planck(temperature, om)
+            v0 = (np.abs(disp.vel_irr[q, b]) @ reclat) ** 2
+            s = np.max(v0 * 0.5)
+            sigsq[q, b] = s
+            sigavg[b] += s * mesh.weight_irr[q]
+    sigsq += np.max(sigavg) * 1e-4
+    sr.be, sr.sigsq = be, sigsq
+    sr.sigma_fixed = sigma * U.FREQ_THZ_TO_HARTREE
+    sr.thresh_sigma = np.max(sigavg) * 1e-4
+    rows = [(q, b) for q in range(nirr) for b in range(nb) if disp.om_irr[q, b] >= U.FREQTOL]
+    sr.rows = rows
+    row_of = {r: i for i, r in enumerate(rows)}
+    nrow = len(rows)
+    gamma_row = np.zeros(nrow)
+    Xi = np.zeros((nrow, nfull * nb))
+    rot = rotation_table(p, mesh, ops)
+    sr.rot = rot
+    mcw = 1.0 / nfull
+    ntrip = 0
+    for q1 in range(nirr):
+        myb = [b for b in range(nb) if (q1, b) in row_of]
+        if rows_only is not None:
+            myb = [b for b in myb if row_of[(q1, b)] in rows_only]
+        if not myb:
+            continue
+        om1 = disp.om_irr[q1]
+        E1 = disp.egv_irr[q1] / np.sqrt(np.where(om1 > 0, om1, 1.0))[None, :]
+        od = np.zeros((nb, nfull, nb))   # [b1, q, b]
+        trips = irreducible_triplets(mesh, rot, q1)
+        ntrip += len(trips)
+        for (q2, q3, perm, red) in trips:
+            om2 = disp.om_full[q2]; om3 = disp.om_full[q3]
+            E2 = disp.egv_full[q2] / np.sqrt(np.where(om2 > 0, om2, 1.0))[None, :]
+            E3 = disp.egv_full[q3] / np.sqrt(np.where(om3 > 0, om3, 1.0))[None, :]
+            ptf = pretransform_phi3(fc3, mesh.qcart[q2], mesh.qcart[q3], nb)
+            psi = psi_chain(E1, E2, E3, ptf)
+            psisq = THREEPHONON_PREFACTOR * np.abs(psi) ** 2 * mcw
+            i2, i3 = mesh.irr_of_full[q2], mesh.irr_of_full[q3]
+            n2 = be[i2][None, :, None]; n3 = be[i3][None, None, :]
+            w1 = om1[:, None, None]; w2 = om2[None, :, None]; w3 = om3[None, None, :]
+            if integrationtype == 2:
+                sig = np.sqrt(sigsq[q1][:, None, None] + sigsq[i2][None, :, None]
+                              + sigsq[i3][None, None, :])
+            elif integrationtype == 1:
+                sig = np.full((nb, nb, nb), sr.sigma_fixed)
+            else:
+                raise NotImplementedError("integrationtype 6")
+            d0 = U.gauss(w1, -w2 + w3, sig) - U.gauss(w1, w2 - w3, sig)
+            d1 = U.gauss(w1, w2 + w3, sig) - U.gauss(w1, -w2 - w3, sig)
+            f = perm * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1)
+            mask = ((w2 >= U.FREQTOL) & (np.abs(w1 - w2) >= U.FREQTOL)
+                    & (w3 >= U.FREQTOL) & (np.abs(w1 - w3) >= U.FREQTOL)
+                    & (np.abs(w2 - w3) >= U.FREQTOL))
+            f = np.where(mask, f, 0.0)
+            tot = f.sum(axis=(1, 2))
+            s2 = f.sum(axis=2)   # [b1, b2]
+            s3 = f.sum(axis=1)   # [b1, b3]
+            for b1 in myb:
+                gamma_row[row_of[(q1, b1)]] += tot[b1] * len(red)
+                for (q2p, q3p) in red:
+                    od[b1, q2p, :] += 2.0 * s2[b1] * om2 / om1[b1]
+                    od[b1, q3p, :] += 2.0 * s3[b1] * om3 / om1[b1]
+        # symmetrize_and_distribute (:143-205)
+        lg = mesh.littlegroup_irr[q1]
+        for q2 in range(nfull):
+            imgs = rot[lg, q2]
+            imgs = imgs[imgs >= 0]
+            if np.any(imgs < q2):
+                continue
+            for b1 in myb:
+                vals = od[b1][imgs]              # [n_img, nb]
+                pos = vals > 0.0
+                nn = pos.sum(axis=0)
+                s = np.where(pos, vals, 0.0).sum(axis=0)
+                upd = nn > 0
+                avg = np.where(upd, s / np.maximum(nn, 1), 0.0)
+                for x in imgs:
+                    od[b1, x, upd] = avg[upd]
+        for b1 in myb:
+            Xi[row_of[(q1, b1)]] += od[b1].reshape(-1)
+    sr.ntriplets = ntrip
+    # reduce + degeneracy average + F0 (:260-292)
+    lw = np.zeros((nirr, nb))
+    for i, (q, b) in enumerate(rows):
+        lw[q, b] = gamma_row[i]
+    for q in range(nirr):
+        for b in range(nb):
+            if disp.om_irr[q, b] < U.FREQTOL:
+                continue
+            dm = disp.degen_irr[q][b]
+            f0 = lw[q, dm].sum() / len(dm)
+            lw[q, dm] = f0
+    linewidth = np.zeros((nirr, nb)); qs = np.zeros((nirr, nb))
+    F0 = np.zeros((nirr, nb, 3)); mfp = np.zeros((nirr, nb, 3)); smfp = np.zeros((nirr, nb))
+    for q in range(nirr):
+        for b in range(nb):
+            if disp.om_irr[q, b] < U.FREQTOL:
+                continue
+            linewidth[q, b] = lw[q, b]
+            qs[q, b] = 2.0 * linewidth[q, b]
+            if linewidth[q, b] > U.FREQTOL:
+                v = disp.vel_irr[q, b]
+                mfp[q, b] = v / qs[q, b]
+                smfp[q, b] = np.linalg.norm(v) / qs[q, b]
+                F0[q, b] = mfp[q, b]
+    # symmetrize_scatmat (:296-357)
+    for i, (q1, b1) in enumerate(rows):
+        lg = mesh.littlegroup_irr[q1]
+        row = Xi[i].reshape(nfull, nb)
+        for q2 in range(nfull):
+            imgs = rot[lg, q2]
+            imgs = imgs[imgs >= 0]
+            if np.any(imgs < q2) or len(imgs) == 0:
+                continue
+            buf = row[imgs].sum(axis=0) / len(imgs)
+            row[imgs] = buf[None, :]
+    sr.Xi = Xi
+    sr.linewidth, sr.qs, sr.F0, sr.mfp, sr.scalar_mfp = linewidth, qs, F0, mfp, smfp
+    sr.gamma_row = gamma_row
+    return sr

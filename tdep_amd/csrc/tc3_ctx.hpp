@@ -1,0 +1,91 @@
+// The tc3 context: host-side model data, mesh/triplet tables and device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "tc3_device.hpp"
+#include "tc3_internal.hpp"
+
+struct tc3_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    bool have_structure = false, have_fc2 = false, have_fc3 = false, have_mesh = false,
+         have_harmonic = false, have_scatter = false, have_kappa = false;
+
+    tc3::Crystal cr;
+    std::vector<tc3::SymOp> ops;
+    int nb = 0;
+
+    // second order, sorted by (a1, a2); blk2_off[(a1*na + a2)] .. [+1]
+    std::vector<int> p_a1, p_a2, blk2_off;
+    std::vector<double> p_lv, p_m;  // 3 / 9 per pair (m row-major [i][j])
+
+    // third order, sorted by (a1, a2, a3); blk3_off over na^3 blocks; a1_off over atoms
+    int nfc3 = 0;
+    std::vector<int> t_a1, t_a2, t_a3, blk3_off, a1_off;
+    std::vector<double> t_lv2, t_lv3, t_mwm;  // 3 / 3 / 27 per triplet (mwm [i][j][k])
+
+    tc3::Mesh mesh;
+    tc3::TripletSet trip;
+    tc3::OrbitSet orb;
+
+    // harmonic results (host copies); egv stored [q][component][mode]
+    std::vector<double> om_irr, vel_irr, om_full, vel_full;
+    std::vector<tc3::cd> egv_irr, egv_full;
+    std::vector<std::vector<std::vector<int>>> degen;  // [q_irr][band] -> degenerate bands
+    double omega_min = 0;
+
+    // rows (q_irr, band) with omega >= freqtol, q-major
+    int nrows = 0;
+    std::vector<int> row_q, row_b, row_of;  // row_of[q*nb + b] = row or -1
+
+    // scattering results
+    int nT = 0;
+    std::vector<double> temps;
+    int precision = 0;
+    bool classical = false;
+    std::vector<double> be, sigsq;            // [T][q][b], [q][b]
+    std::vector<double> linewidth, qs, F0, mfp, smfp;  // [T][q][b](x3)
+
+    // kappa results
+    std::vector<double> kappa_sma, kappa_iter, kappa_od;  // [T][9]
+    std::vector<double> Fn, kappa_mode;                   // [T][q][b][3], [T][q][b][9]
+    std::vector<std::vector<double>> kappa_log;           // [T][iter*7]
+    std::vector<int> niter;
+
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // device buffers
+    double *d_om_irr = nullptr, *d_om_full = nullptr, *d_vel_irr = nullptr, *d_vel_full = nullptr;
+    tc3::cd *d_egv_irr = nullptr, *d_egv_full = nullptr;
+    double *d_xi = nullptr;     // [T][row][q*nb + b]
+    size_t xi_row = 0;          // nfull*nb
+    int *d_irr_of_full = nullptr;
+
+    ~tc3_ctx();
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+};
+
+namespace tc3 {
+
+int device_error(hipError_t e, const char *what, const char *file, int line);
+extern thread_local std::string g_device_error;
+
+// tc3_harmonic.hip
+int run_harmonic(tc3_ctx *c);
+// tc3_scatter.hip
+int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma);
+// tc3_kappa.hip
+int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag);
+// tc3_hdf5.cpp
+int write_grid_hdf5(const tc3_ctx *c, int iT, const char *path, std::string &err);
+
+}  // namespace tc3

@@ -1,0 +1,592 @@
+// Harmonic properties on the q-mesh, FP64 on the GPU.
+//
+//   k_dynmat     dynamical matrix and its q-gradient per irreducible point
+//                (type_forceconstant_secondorder_dynamicalmatrix.f90:484-788, non-polar)
+//   k_jacobi     Hermitian eigen-solve: parallel cyclic Jacobi, one workgroup per matrix
+//                (replaces zheev, …_dynamicalmatrix.f90:113-124)
+//   k_post       Gamma zeroing, sorting, degenerate-subspace rotation to a fixed gauge,
+//                group velocities with degenerate averaging (…:145-215, :287-374, :472)
+//   k_unfold     rotation of eigenvectors/velocities to the full grid
+//                (type_phonon_dispersions_generation.f90:117-173 with
+//                 lo_eigenvector_transformation_matrix, type_symmetryoperation.f90:243-293)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tc3_ctx.hpp"
+
+namespace tc3 {
+
+__device__ const double kWellDefinedDev[69] = {
+    0.0, 0.1, -0.1, 0.1111111111111111, -0.1111111111111111, 0.125, -0.125,
+    0.1428571428571428, -0.1428571428571428, 0.1666666666666667, -0.1666666666666667,
+    0.2, -0.2, 0.2222222222222222, -0.2222222222222222, 0.25, -0.25,
+    0.2857142857142857, -0.2857142857142857, 0.3, -0.3, 0.3333333333333333,
+    -0.3333333333333333, 0.375, -0.375, 0.4, -0.4, 0.4285714285714285,
+    -0.4285714285714285, 0.4444444444444444, -0.4444444444444444, 0.5, -0.5,
+    0.5555555555555556, -0.5555555555555556, 0.5714285714285714, -0.5714285714285714,
+    0.6, -0.6, 0.625, -0.625, 0.6666666666666667, -0.6666666666666667, 0.7, -0.7,
+    0.7071067811865475, -0.7071067811865475, 0.7142857142857143, -0.7142857142857143,
+    0.75, -0.75, 0.7777777777777778, -0.7777777777777778, 0.8, -0.8,
+    0.8333333333333334, -0.8333333333333334, 0.8571428571428571, -0.8571428571428571,
+    0.8660254037844386, -0.8660254037844386, 0.875, -0.875, 0.8888888888888888,
+    -0.8888888888888888, 0.9, -0.9, 1.0, -1.0};
+
+__device__ double chop_dev(double x, double tol) {
+    for (int i = 0; i < 69; ++i)
+        if (fabs(x - kWellDefinedDev[i]) < tol) return kWellDefinedDev[i];
+    return x;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kGold = 0.6180339887498949;
+constexpr double kPlastic = 0.7548776662466927;
+
+// D[q][(a2,j)][(a1,i)] = sum_pairs m_ij e^{i qv.R} / sqrt(m1 m2); gD likewise with i R_alpha.
+__global__ void k_dynmat(int na, const double *__restrict__ qv, const int *__restrict__ blk_off,
+                         const double *__restrict__ lv, const double *__restrict__ pm,
+                         const double *__restrict__ ism, cd *__restrict__ D, cd *__restrict__ gD) {
+    const int q = blockIdx.x;
+    const int nb = 3 * na;
+    const double q0 = qv[3 * q], q1 = qv[3 * q + 1], q2 = qv[3 * q + 2];
+    cd *Dq = D + (size_t)q * nb * nb;
+    cd *gq = gD + (size_t)q * 3 * nb * nb;
+    for (int blk = threadIdx.x; blk < na * na; blk += blockDim.x) {
+        const int a1 = blk / na, a2 = blk % na;
+        cd acc[9], gacc[3][9];
+        for (int k = 0; k < 9; ++k) {
+            acc[k] = {0, 0};
+            for (int al = 0; al < 3; ++al) gacc[al][k] = {0, 0};
+        }
+        for (int p = blk_off[blk]; p < blk_off[blk + 1]; ++p) {
+            const double r0 = lv[3 * p], r1 = lv[3 * p + 1], r2 = lv[3 * p + 2];
+            const double qr = r0 * q0 + r1 * q1 + r2 * q2;
+            double s, c;
+            sincos(qr, &s, &c);
+            const double R[3] = {r0, r1, r2};
+            for (int k = 0; k < 9; ++k) {
+                const double m = pm[9 * p + k];
+                acc[k].x += c * m;
+                acc[k].y += s * m;
+                for (int al = 0; al < 3; ++al) {
+                    // i R e^{iqr} m
+                    gacc[al][k].x += -R[al] * s * m;
+                    gacc[al][k].y += R[al] * c * m;
+                }
+            }
+        }
+        const double w = ism[a1] * ism[a2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const size_t idx = (size_t)(3 * a2 + j) * nb + (3 * a1 + i);
+                Dq[idx] = cscale(acc[3 * i + j], w);
+                for (int al = 0; al < 3; ++al) gq[(size_t)al * nb * nb + idx] = cscale(gacc[al][3 * i + j], w);
+            }
+    }
+}
+
+// Parallel cyclic Jacobi for one Hermitian matrix per workgroup.  Round-robin pairing
+// (circle method) gives nb/2 disjoint (p,q) rotations per round; all rotations of a
+// round are applied to the columns, then to the rows, then to V.
+__global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, double *__restrict__ lam,
+                         int maxsweep, int *__restrict__ status) {
+    const int q = blockIdx.x;
+    cd *A = Aall + (size_t)q * nb * nb;
+    cd *V = Vall + (size_t)q * nb * nb;
+    const int mpad = nb + (nb & 1);
+    const int half = mpad / 2;
+    __shared__ int sp[128], sq[128];
+    __shared__ double sc[128], ss[128], sphx[128], sphy[128];
+    __shared__ double red[kBlock];
+    __shared__ int done;
+    for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) V[i] = {(i / nb) == (i % nb) ? 1.0 : 0.0, 0.0};
+    __syncthreads();
+    int sweep = 0;
+    for (; sweep < maxsweep; ++sweep) {
+        // convergence test: off-diagonal vs diagonal Frobenius norm
+        double off = 0, dia = 0;
+        for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) {
+            const int r = i / nb, c = i % nb;
+            if (r == c) dia += cabs2(A[i]);
+            else off += cabs2(A[i]);
+        }
+        red[threadIdx.x] = off;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        const double offt = red[0];
+        __syncthreads();
+        red[threadIdx.x] = dia;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        const double diat = red[0];
+        if (threadIdx.x == 0) done = (offt <= 1e-30 * diat) || offt == 0.0;
+        __syncthreads();
+        if (done) break;
+        for (int r = 0; r < mpad - 1; ++r) {
+            // pairs of this round and their rotations
+            for (int k = threadIdx.x; k < half; k += blockDim.x) {
+                int p, qq;
+                if (k == 0) { p = r; qq = mpad - 1; }
+                else { p = (r + k) % (mpad - 1); qq = (r - k + mpad - 1) % (mpad - 1); }
+                if (p > qq) { int t = p; p = qq; qq = t; }
+                double c = 1, s = 0, ex = 1, ey = 0;
+                if (qq < nb) {
+                    const cd apq = A[(size_t)p * nb + qq];
+                    const double b = sqrt(cabs2(apq));
+                    if (b > 1e-300) {
+                        const double app = A[(size_t)p * nb + p].x, aqq = A[(size_t)qq * nb + qq].x;
+                        const double tau = (aqq - app) / (2.0 * b);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                        ex = apq.x / b;
+                        ey = apq.y / b;
+                    }
+                } else {
+                    qq = -1;
+                }
+                sp[k] = p; sq[k] = qq; sc[k] = c; ss[k] = s; sphx[k] = ex; sphy[k] = ey;
+            }
+            __syncthreads();
+            // columns: A <- A U, V <- V U with U = diag(1, e^{-i phi}) [[c, s], [-s, c]]
+            for (int idx = threadIdx.x; idx < half * nb; idx += blockDim.x) {
+                const int k = idx / nb, row = idx % nb;
+                const int p = sp[k], qq = sq[k];
+                if (qq < 0 || ss[k] == 0.0) continue;
+                const double c = sc[k], s = ss[k];
+                const cd em = {sphx[k], -sphy[k]};  // e^{-i phi}
+                cd ap = A[(size_t)row * nb + p], aq = A[(size_t)row * nb + qq];
+                cd emaq = cmul(em, aq);
+                A[(size_t)row * nb + p] = csub(cscale(ap, c), cscale(emaq, s));
+                A[(size_t)row * nb + qq] = cadd(cscale(ap, s), cscale(emaq, c));
+                cd vp = V[(size_t)row * nb + p], vq = V[(size_t)row * nb + qq];
+                cd emvq = cmul(em, vq);
+                V[(size_t)row * nb + p] = csub(cscale(vp, c), cscale(emvq, s));
+                V[(size_t)row * nb + qq] = cadd(cscale(vp, s), cscale(emvq, c));
+            }
+            __syncthreads();
+            // rows: A <- U^H A
+            for (int idx = threadIdx.x; idx < half * nb; idx += blockDim.x) {
+                const int k = idx / nb, col = idx % nb;
+                const int p = sp[k], qq = sq[k];
+                if (qq < 0 || ss[k] == 0.0) continue;
+                const double c = sc[k], s = ss[k];
+                const cd ep = {sphx[k], sphy[k]};  // e^{i phi}
+                cd ap = A[(size_t)p * nb + col], aq = A[(size_t)qq * nb + col];
+                cd epaq = cmul(ep, aq);
+                A[(size_t)p * nb + col] = csub(cscale(ap, c), cscale(epaq, s));
+                A[(size_t)qq * nb + col] = cadd(cscale(ap, s), cscale(epaq, c));
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) lam[(size_t)q * nb + i] = A[(size_t)i * nb + i].x;
+    if (threadIdx.x == 0) status[q] = (sweep >= maxsweep) ? 1 : 0;
+}
+
+// Serial Jacobi for a small Hermitian matrix (degenerate-subspace probe), ascending order.
+__device__ void small_eigh(int k, cd *M, cd *W, double *ev, int *ord) {
+    for (int i = 0; i < k * k; ++i) W[i] = {(i / k) == (i % k) ? 1.0 : 0.0, 0.0};
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, dia = 0;
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j) (i == j ? dia : off) += cabs2(M[i * k + j]);
+        if (off <= 1e-32 * dia || off == 0.0) break;
+        for (int p = 0; p < k; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                const cd apq = M[p * k + q];
+                const double b = sqrt(cabs2(apq));
+                if (b <= 1e-300) continue;
+                const double tau = (M[q * k + q].x - M[p * k + p].x) / (2.0 * b);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                const cd em = {apq.x / b, -apq.y / b}, ep = {apq.x / b, apq.y / b};
+                for (int r = 0; r < k; ++r) {
+                    cd ap = M[r * k + p], aq = cmul(em, M[r * k + q]);
+                    M[r * k + p] = csub(cscale(ap, c), cscale(aq, s));
+                    M[r * k + q] = cadd(cscale(ap, s), cscale(aq, c));
+                    cd wp = W[r * k + p], wq = cmul(em, W[r * k + q]);
+                    W[r * k + p] = csub(cscale(wp, c), cscale(wq, s));
+                    W[r * k + q] = cadd(cscale(wp, s), cscale(wq, c));
+                }
+                for (int r = 0; r < k; ++r) {
+                    cd ap = M[p * k + r], aq = cmul(ep, M[q * k + r]);
+                    M[p * k + r] = csub(cscale(ap, c), cscale(aq, s));
+                    M[q * k + r] = cadd(cscale(ap, s), cscale(aq, c));
+                }
+            }
+    }
+    for (int i = 0; i < k; ++i) { ev[i] = M[i * k + i].x; ord[i] = i; }
+    for (int i = 1; i < k; ++i) {  // stable insertion sort by eigenvalue
+        int o = ord[i];
+        int j = i - 1;
+        while (j >= 0 && ev[ord[j]] > ev[o]) { ord[j + 1] = ord[j]; --j; }
+        ord[j + 1] = o;
+    }
+}
+
+// Per irreducible point: Gamma zeroing, sort by signed frequency, canonical gauge of the
+// degenerate subspaces, phase fix, chop, group velocities.
+__global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__restrict__ Vin,
+                       const cd *__restrict__ gD, const int *__restrict__ at_gamma,
+                       double *__restrict__ omega, cd *__restrict__ egv, double *__restrict__ vel,
+                       cd *__restrict__ scratch, double veltol) {
+    const int q = blockIdx.x;
+    __shared__ double lam[128], om[128], hf[3][128];
+    __shared__ int ord[128], grp[129];
+    __shared__ int ngrp;
+    const cd *V = Vin + (size_t)q * nb * nb;
+    cd *E = egv + (size_t)q * nb * nb;
+    cd *scr = scratch + (size_t)q * (2 * nb * nb + nb);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < nb; ++i) lam[i] = lam_in[(size_t)q * nb + i];
+        if (at_gamma[q]) {
+            // three smallest |lambda| (stable) set to zero
+            int idx[128];
+            for (int i = 0; i < nb; ++i) idx[i] = i;
+            for (int i = 1; i < nb; ++i) {
+                int o = idx[i], j = i - 1;
+                while (j >= 0 && fabs(lam[idx[j]]) > fabs(lam[o])) { idx[j + 1] = idx[j]; --j; }
+                idx[j + 1] = o;
+            }
+            for (int i = 0; i < 3 && i < nb; ++i) lam[idx[i]] = 0.0;
+        }
+        for (int i = 0; i < nb; ++i) {
+            const double l = lam[i];
+            om[i] = sqrt(fabs(l)) * (l >= 0 ? 1.0 : -1.0);
+            ord[i] = i;
+        }
+        for (int i = 1; i < nb; ++i) {
+            int o = ord[i], j = i - 1;
+            while (j >= 0 && om[ord[j]] > om[o]) { ord[j + 1] = ord[j]; --j; }
+            ord[j + 1] = o;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) {
+        const int r = i / nb, c = i % nb;
+        E[i] = V[(size_t)r * nb + ord[c]];
+    }
+    if (threadIdx.x == 0) {
+        double tmp[128];
+        for (int i = 0; i < nb; ++i) tmp[i] = lam[ord[i]];
+        for (int i = 0; i < nb; ++i) {
+            lam[i] = tmp[i];
+            om[i] = sqrt(fabs(tmp[i])) * (tmp[i] >= 0 ? 1.0 : -1.0);
+        }
+        // groups of consecutive modes closer than lo_freqtol
+        int n = 0;
+        grp[0] = 0;
+        for (int i = 1; i < nb; ++i)
+            if (!(om[i] - om[i - 1] < 1e-9)) grp[++n] = i;
+        grp[++n] = nb;
+        ngrp = n;
+    }
+    __syncthreads();
+    // canonical gauge inside degenerate groups (serial; groups are small)
+    if (threadIdx.x == 0) {
+        for (int g = 0; g < ngrp; ++g) {
+            const int i0 = grp[g], k = grp[g + 1] - grp[g];
+            if (k < 2) continue;
+            cd *M = scr;
+            cd *W = scr + nb * nb;
+            double ev[128];
+            int so[128];
+            for (int a = 0; a < k; ++a)
+                for (int b = 0; b < k; ++b) {
+                    cd s = {0, 0};
+                    for (int r = 0; r < nb; ++r) {
+                        const double pr = fmod((r + 1) * kGold, 1.0);
+                        s = cadd(s, cscale(cmulc(E[(size_t)r * nb + i0 + a], E[(size_t)r * nb + i0 + b]), pr));
+                    }
+                    M[a * k + b] = s;
+                }
+            for (int a = 0; a < k; ++a)
+                for (int b = a + 1; b < k; ++b) {
+                    cd x = M[a * k + b], y = cconj(M[b * k + a]);
+                    M[a * k + b] = cscale(cadd(x, y), 0.5);
+                    M[b * k + a] = cconj(M[a * k + b]);
+                }
+            for (int a = 0; a < k; ++a) M[a * k + a].y = 0.0;
+            small_eigh(k, M, W, ev, so);
+            cd *tmp = scr + 2 * nb * nb;
+            for (int r = 0; r < nb; ++r) {
+                for (int b = 0; b < k; ++b) {
+                    cd s = {0, 0};
+                    for (int a = 0; a < k; ++a) s = cadd(s, cmul(E[(size_t)r * nb + i0 + a], W[a * k + so[b]]));
+                    tmp[b] = s;
+                }
+                for (int b = 0; b < k; ++b) E[(size_t)r * nb + i0 + b] = tmp[b];
+            }
+        }
+    }
+    __syncthreads();
+    // phase fix and chop
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        cd s = {0, 0};
+        for (int r = 0; r < nb; ++r) {
+            const double kk = (double)(r + 1);
+            const double amp = 1.0 + fmod(kk * kPlastic, 1.0);
+            double sn, cs;
+            sincos(0.9 * kk, &sn, &cs);
+            s = cadd(s, cmul({amp * cs, amp * sn}, E[(size_t)r * nb + b]));
+        }
+        const double a = sqrt(cabs2(s));
+        const cd ph = {s.x / a, -s.y / a};
+        for (int r = 0; r < nb; ++r) {
+            cd e = cmul(E[(size_t)r * nb + b], ph);
+            if (fabs(e.x) < 1e-13) e.x = 0.0;
+            if (fabs(e.y) < 1e-13) e.y = 0.0;
+            E[(size_t)r * nb + b] = e;
+        }
+    }
+    __syncthreads();
+    // Hellmann-Feynman: hf[al][b] = Re e_b^H gD_al e_b
+    const cd *g = gD + (size_t)q * 3 * nb * nb;
+    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) {
+        const int al = t / nb, b = t % nb;
+        double acc = 0;
+        for (int i = 0; i < nb; ++i) {
+            cd s = {0, 0};
+            for (int j = 0; j < nb; ++j) s = cadd(s, cmul(g[(size_t)al * nb * nb + (size_t)i * nb + j], E[(size_t)j * nb + b]));
+            const cd ei = E[(size_t)i * nb + b];
+            acc += ei.x * s.x + ei.y * s.y;
+        }
+        hf[al][b] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // return_degenerate_subspaces grouping (…_dynamicalmatrix.f90:911-980)
+        int sub[128];
+        int nsub = 0;
+        int first[128];
+        for (int i = 0; i < nb; ++i) {
+            int s = -1;
+            for (int j = 0; j < i && s < 0; ++j)
+                if (fabs(om[j] - om[i]) < 1e-9) s = sub[j];
+            if (s < 0) { s = nsub++; first[s] = i; }
+            sub[i] = s;
+        }
+        for (int s = 0; s < nsub; ++s) {
+            double m0 = 0, m1 = 0, m2 = 0;
+            int cnt = 0;
+            for (int i = 0; i < nb; ++i)
+                if (sub[i] == s) { m0 += hf[0][i]; m1 += hf[1][i]; m2 += hf[2][i]; ++cnt; }
+            for (int i = 0; i < nb; ++i)
+                if (sub[i] == s) {
+                    double v[3] = {m0 / cnt, m1 / cnt, m2 / cnt};
+                    for (int al = 0; al < 3; ++al) {
+                        double x = (om[i] > 1e-9) ? v[al] / (2.0 * om[i]) : 0.0;
+                        vel[((size_t)q * nb + i) * 3 + al] = chop_dev(x, veltol);
+                    }
+                }
+        }
+        for (int i = 0; i < nb; ++i) omega[(size_t)q * nb + i] = om[i];
+    }
+}
+
+// Full-grid eigenvectors e_full = Gamma(R) e_irr (conjugated under time reversal),
+// velocities +-R v_irr.
+__global__ void k_unfold(int nb, int na, const int *__restrict__ irr_of_full,
+                         const int *__restrict__ op_of_full, const double *__restrict__ opm,
+                         const int *__restrict__ src, const double *__restrict__ v0tab,
+                         const double *__restrict__ qirr, const cd *__restrict__ egv_irr,
+                         const double *__restrict__ vel_irr, const double *__restrict__ om_irr,
+                         cd *__restrict__ egv_full, double *__restrict__ vel_full,
+                         double *__restrict__ om_full) {
+    const int q = blockIdx.x;
+    const int k = irr_of_full[q];
+    const int iop = op_of_full[q];
+    const int o = (iop > 0 ? iop : -iop) - 1;
+    const bool tr = iop < 0;
+    const double *m = opm + 9 * o;
+    const double *qi = qirr + 3 * k;
+    const cd *Ei = egv_irr + (size_t)k * nb * nb;
+    cd *Ef = egv_full + (size_t)q * nb * nb;
+    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int row = idx / nb, b = idx % nb;
+        const int a1 = row / 3, al = row % 3;
+        const int a2 = src[o * na + a1];
+        const double *v0 = v0tab + 3 * (o * na + a2);
+        const double ph = -(v0[0] * qi[0] + v0[1] * qi[1] + v0[2] * qi[2]) * kTwoPi;
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        cd s = {0, 0};
+        for (int be = 0; be < 3; ++be) s = cadd(s, cscale(Ei[(size_t)(3 * a2 + be) * nb + b], m[3 * al + be]));
+        s = cmul(s, {cs, sn});
+        if (tr) s.y = -s.y;
+        Ef[idx] = s;
+    }
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        om_full[(size_t)q * nb + b] = om_irr[(size_t)k * nb + b];
+        const double *v = vel_irr + ((size_t)k * nb + b) * 3;
+        for (int al = 0; al < 3; ++al) {
+            double x = m[3 * al] * v[0] + m[3 * al + 1] * v[1] + m[3 * al + 2] * v[2];
+            vel_full[((size_t)q * nb + b) * 3 + al] = tr ? -x : x;
+        }
+    }
+}
+
+}  // namespace
+
+int run_harmonic(tc3_ctx *c) {
+    const int na = c->cr.na, nb = 3 * na;
+    const int nirr = c->mesh.nirr, nfull = c->mesh.nfull;
+    const int nsym = (int)c->ops.size();
+    if (nb > 128) return c->fail(1, "at most 42 atoms per cell are supported by the harmonic kernels");
+    hipStream_t st = c->stream;
+    // host tables
+    std::vector<double> qv(3 * nirr), qirr(3 * nirr);
+    std::vector<int> atg(nirr);
+    for (int k = 0; k < nirr; ++k) {
+        const double *q = &c->mesh.qcart[3 * c->mesh.irr_full_index[k]];
+        double n2 = 0;
+        for (int i = 0; i < 3; ++i) {
+            qv[3 * k + i] = chop_scalar(q[i] * kTwoPi, 1e-13);
+            qirr[3 * k + i] = q[i];
+            n2 += q[i] * q[i];
+        }
+        atg[k] = n2 < kSqTol ? 1 : 0;
+    }
+    std::vector<double> opm(9 * nsym), v0tab(3 * nsym * na);
+    std::vector<int> src(nsym * na);
+    for (int o = 0; o < nsym; ++o) {
+        const SymOp &op = c->ops[o];
+        std::memcpy(&opm[9 * o], op.m, 9 * sizeof(double));
+        for (int a2 = 0; a2 < na; ++a2) {
+            const int a1 = op.fmap[a2];
+            src[o * na + a1] = a2;
+            double d[3];
+            for (int i = 0; i < 3; ++i) d[i] = c->cr.rcart[3 * a1 + i] - op.tr[i];
+            // im = transpose(m)
+            for (int i = 0; i < 3; ++i)
+                v0tab[3 * (o * na + a2) + i] =
+                    op.m[0 * 3 + i] * d[0] + op.m[1 * 3 + i] * d[1] + op.m[2 * 3 + i] * d[2] - c->cr.rcart[3 * a2 + i];
+        }
+    }
+    // device buffers
+    double *d_qv = nullptr, *d_lv = nullptr, *d_pm = nullptr, *d_ism = nullptr, *d_lam = nullptr;
+    double *d_opm = nullptr, *d_v0 = nullptr, *d_qirr = nullptr;
+    int *d_blk = nullptr, *d_stat = nullptr, *d_atg = nullptr, *d_src = nullptr, *d_opf = nullptr;
+    cd *d_D = nullptr, *d_gD = nullptr, *d_V = nullptr, *d_scr = nullptr;
+    const size_t nn = (size_t)nb * nb;
+    const int npair = (int)c->p_a1.size();
+    auto cleanup = [&]() {
+        hipFree(d_qv); hipFree(d_lv); hipFree(d_pm); hipFree(d_ism); hipFree(d_lam);
+        hipFree(d_opm); hipFree(d_v0); hipFree(d_qirr); hipFree(d_blk); hipFree(d_stat);
+        hipFree(d_atg); hipFree(d_src); hipFree(d_opf); hipFree(d_D); hipFree(d_gD);
+        hipFree(d_V); hipFree(d_scr);
+    };
+    hipError_t e = hipSuccess;
+#define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
+    TRY(hipMalloc(&d_qv, sizeof(double) * 3 * nirr));
+    TRY(hipMalloc(&d_qirr, sizeof(double) * 3 * nirr));
+    TRY(hipMalloc(&d_lv, sizeof(double) * 3 * std::max(npair, 1)));
+    TRY(hipMalloc(&d_pm, sizeof(double) * 9 * std::max(npair, 1)));
+    TRY(hipMalloc(&d_ism, sizeof(double) * na));
+    TRY(hipMalloc(&d_blk, sizeof(int) * (na * na + 1)));
+    TRY(hipMalloc(&d_D, sizeof(cd) * nn * nirr));
+    TRY(hipMalloc(&d_gD, sizeof(cd) * 3 * nn * nirr));
+    TRY(hipMalloc(&d_V, sizeof(cd) * nn * nirr));
+    TRY(hipMalloc(&d_lam, sizeof(double) * nb * nirr));
+    TRY(hipMalloc(&d_stat, sizeof(int) * nirr));
+    TRY(hipMalloc(&d_atg, sizeof(int) * nirr));
+    TRY(hipMalloc(&d_scr, sizeof(cd) * (2 * nn + nb) * nirr));
+    TRY(hipMalloc(&d_opm, sizeof(double) * 9 * nsym));
+    TRY(hipMalloc(&d_v0, sizeof(double) * 3 * nsym * na));
+    TRY(hipMalloc(&d_src, sizeof(int) * nsym * na));
+    TRY(hipMalloc(&d_opf, sizeof(int) * nfull));
+    // persistent outputs
+    hipFree(c->d_om_irr); hipFree(c->d_om_full); hipFree(c->d_vel_irr); hipFree(c->d_vel_full);
+    hipFree(c->d_egv_irr); hipFree(c->d_egv_full); hipFree(c->d_irr_of_full);
+    c->d_om_irr = c->d_om_full = c->d_vel_irr = c->d_vel_full = nullptr;
+    c->d_egv_irr = c->d_egv_full = nullptr;
+    c->d_irr_of_full = nullptr;
+    TRY(hipMalloc(&c->d_om_irr, sizeof(double) * nb * nirr));
+    TRY(hipMalloc(&c->d_vel_irr, sizeof(double) * 3 * nb * nirr));
+    TRY(hipMalloc(&c->d_egv_irr, sizeof(cd) * nn * nirr));
+    TRY(hipMalloc(&c->d_om_full, sizeof(double) * nb * nfull));
+    TRY(hipMalloc(&c->d_vel_full, sizeof(double) * 3 * nb * nfull));
+    TRY(hipMalloc(&c->d_egv_full, sizeof(cd) * nn * nfull));
+    TRY(hipMalloc(&c->d_irr_of_full, sizeof(int) * nfull));
+    TRY(hipMemcpyAsync(d_qv, qv.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_qirr, qirr.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
+    if (npair) {
+        TRY(hipMemcpyAsync(d_lv, c->p_lv.data(), sizeof(double) * 3 * npair, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_pm, c->p_m.data(), sizeof(double) * 9 * npair, hipMemcpyHostToDevice, st));
+    }
+    TRY(hipMemcpyAsync(d_ism, c->cr.invsqrtmass.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_blk, c->blk2_off.data(), sizeof(int) * (na * na + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_atg, atg.data(), sizeof(int) * nirr, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_v0, v0tab.data(), sizeof(double) * 3 * nsym * na, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_src, src.data(), sizeof(int) * nsym * na, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(c->d_irr_of_full, c->mesh.irr_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
+
+    hipLaunchKernelGGL(k_dynmat, dim3(nirr), dim3(kBlock), 0, st, na, d_qv, d_blk, d_lv, d_pm, d_ism, d_D, d_gD);
+    TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_jacobi, dim3(nirr), dim3(kBlock), 0, st, nb, d_D, d_V, d_lam, 60, d_stat);
+    TRY(hipGetLastError());
+    const double veltol = 1e-9 / kGroupvelAuToMs;
+    hipLaunchKernelGGL(k_post, dim3(nirr), dim3(kBlock), 0, st, nb, d_lam, d_V, d_gD, d_atg,
+                       c->d_om_irr, c->d_egv_irr, c->d_vel_irr, d_scr, veltol);
+    TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_unfold, dim3(nfull), dim3(kBlock), 0, st, nb, na, c->d_irr_of_full, d_opf, d_opm,
+                       d_src, d_v0, d_qirr, c->d_egv_irr, c->d_vel_irr, c->d_om_irr, c->d_egv_full,
+                       c->d_vel_full, c->d_om_full);
+    TRY(hipGetLastError());
+    std::vector<int> stat(nirr);
+    TRY(hipMemcpyAsync(stat.data(), d_stat, sizeof(int) * nirr, hipMemcpyDeviceToHost, st));
+    c->om_irr.resize((size_t)nb * nirr);
+    c->vel_irr.resize((size_t)3 * nb * nirr);
+    c->egv_irr.resize(nn * nirr);
+    c->om_full.resize((size_t)nb * nfull);
+    c->vel_full.resize((size_t)3 * nb * nfull);
+    c->egv_full.resize(nn * nfull);
+    TRY(hipMemcpyAsync(c->om_irr.data(), c->d_om_irr, sizeof(double) * nb * nirr, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(c->vel_irr.data(), c->d_vel_irr, sizeof(double) * 3 * nb * nirr, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(c->egv_irr.data(), c->d_egv_irr, sizeof(cd) * nn * nirr, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(c->om_full.data(), c->d_om_full, sizeof(double) * nb * nfull, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(c->vel_full.data(), c->d_vel_full, sizeof(double) * 3 * nb * nfull, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(c->egv_full.data(), c->d_egv_full, sizeof(cd) * nn * nfull, hipMemcpyDeviceToHost, st));
+    TRY(hipStreamSynchronize(st));
+#undef TRY
+    cleanup();
+    for (int k = 0; k < nirr; ++k)
+        if (stat[k]) return c->fail(2, "Jacobi eigen-solver did not converge");
+    // degeneracy table (type_phonon_dispersions.f90:1190-1214)
+    c->degen.assign(nirr, std::vector<std::vector<int>>(nb));
+    c->omega_min = 1e300;
+    for (int k = 0; k < nirr; ++k)
+        for (int b = 0; b < nb; ++b) {
+            const double w = c->om_irr[(size_t)k * nb + b];
+            for (int b2 = 0; b2 < nb; ++b2)
+                if (std::fabs(c->om_irr[(size_t)k * nb + b2] - w) < kFreqTol) c->degen[k][b].push_back(b2);
+            if (std::fabs(w) > kFreqTol) c->omega_min = std::min(c->omega_min, w);
+        }
+    // rows
+    c->row_q.clear(); c->row_b.clear();
+    c->row_of.assign((size_t)nirr * nb, -1);
+    for (int k = 0; k < nirr; ++k)
+        for (int b = 0; b < nb; ++b)
+            if (!(c->om_irr[(size_t)k * nb + b] < kFreqTol)) {
+                c->row_of[(size_t)k * nb + b] = (int)c->row_q.size();
+                c->row_q.push_back(k);
+                c->row_b.push_back(b);
+            }
+    c->nrows = (int)c->row_q.size();
+    return 0;
+}
+
+}  // namespace tc3

@@ -1,0 +1,407 @@
+// Thermal conductivity: RTA (host), iterative BTE (FP64 on the GPU) and the
+// off-diagonal coherence term (host).
+//
+//   get_kappa          kappa.f90:28-75
+//   iterative_solution kappa.f90:326-495 (fold-out, Xi.F, update on the GPU)
+//   get_kappa_offdiag  kappa.f90:77-306
+//   symmetrize_kappa   kappa.f90:308-324
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "tc3_ctx.hpp"
+
+namespace tc3 {
+
+namespace {
+
+using zc = std::complex<double>;
+
+double cv_host(double T, double om) {
+    // lo_harmonic_oscillator_cv (gottochblandat_physics.f90:331-367)
+    if (om < kFreqTol) return 0.0;
+    if (T < kTemperatureTol) return 0.0;
+    const double x = om / T / kKbHartree;
+    if (x > 200.0) return 0.0;
+    if (x > 21.0) return kKbHartree * x * x * std::exp(-x);
+    if (x < 2e-2) return kKbHartree * (1.0 - x * x / 12.0 + x * x * x * x / 240.0);
+    const double ex = std::exp(x);
+    return kKbHartree * x * x * ex / ((ex - 1) * (ex - 1));
+}
+
+void chop_kappa(double *k) {
+    double s = 0;
+    for (int i = 0; i < 9; ++i) s += std::fabs(k[i]);
+    for (int i = 0; i < 9; ++i) k[i] = chop_scalar(k[i], s * 1e-6);
+}
+
+// get_kappa: kappa (column-major 3x3) and optional per-mode tensors.
+void get_kappa(const tc3_ctx *c, const double *Fn, double T, bool classical, double *kappa, double *kmode) {
+    const int nb = c->nb, nirr = c->mesh.nirr, nsym = (int)c->ops.size();
+    double k[9] = {0};
+    for (int q = 0; q < nirr; ++q)
+        for (int b = 0; b < nb; ++b) {
+            const double om = c->om_irr[(size_t)q * nb + b];
+            if (kmode) std::fill(kmode + ((size_t)q * nb + b) * 9, kmode + ((size_t)q * nb + b) * 9 + 9, 0.0);
+            if (om < kFreqTol) continue;
+            const double cv = classical ? kKbHartree : cv_host(T, om);
+            const double *F = Fn + ((size_t)q * nb + b) * 3;
+            const double *v = &c->vel_irr[((size_t)q * nb + b) * 3];
+            double v2[9] = {0};
+            for (int o = 0; o < nsym; ++o) {
+                const double *m = c->ops[o].m;
+                double v0[3], v1[3];
+                for (int i = 0; i < 3; ++i) {
+                    v0[i] = m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2];
+                    v1[i] = m[3 * i] * v[0] + m[3 * i + 1] * v[1] + m[3 * i + 2] * v[2];
+                }
+                for (int j = 0; j < 3; ++j)
+                    for (int i = 0; i < 3; ++i) v2[3 * j + i] += v0[i] * v1[j] / nsym;
+            }
+            for (int i = 0; i < 9; ++i) {
+                const double buf = cv * v2[i] / c->cr.volume;
+                if (kmode) kmode[((size_t)q * nb + b) * 9 + i] = buf;
+                k[i] += buf * c->mesh.weight_irr[q];
+            }
+        }
+    chop_kappa(k);
+    std::memcpy(kappa, k, sizeof(k));
+}
+
+void symmetrize_kappa(const tc3_ctx *c, double *k) {
+    const int nsym = (int)c->ops.size();
+    double t[9] = {0};
+    for (int o = 0; o < nsym; ++o) {
+        const double *m = c->ops[o].m;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                double s = 0;
+                for (int jj = 0; jj < 3; ++jj)
+                    for (int ii = 0; ii < 3; ++ii) s += k[3 * jj + ii] * m[3 * i + ii] * m[3 * j + jj];
+                t[3 * j + i] += s;
+            }
+    }
+    for (int i = 0; i < 9; ++i) k[i] = t[i] / nsym;
+    chop_kappa(k);
+}
+
+// get_kappa_offdiag with the coherence velocities formed as avg_op[egw^H G^T egw]
+// (entry for entry the product the reference forms through its Kronecker matrix).
+void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, double *kod) {
+    const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr;
+    double k[9] = {0};
+    std::vector<zc> G(3 * (size_t)nb * nb), egw((size_t)nb * nb), tmp((size_t)nb * nb), V(3 * (size_t)nb * nb);
+    std::vector<double> mw(nb);
+    for (int a = 0; a < na; ++a)
+        for (int i = 0; i < 3; ++i) mw[3 * a + i] = c->cr.invsqrtmass[a];
+    const double vtol = 1e-10 / (kGroupvelAuToMs / 1000.0);
+    for (int iq = 0; iq < nirr; ++iq) {
+        const double *q = &c->mesh.qcart[3 * c->mesh.irr_full_index[iq]];
+        double qv[3];
+        for (int i = 0; i < 3; ++i) qv[i] = chop_scalar(q[i] * kTwoPi, 1e-13);
+        // gradient of the dynamical matrix without masses, indexed G[al][(a2,j)][(a1,i)]
+        std::fill(G.begin(), G.end(), zc(0, 0));
+        for (size_t p = 0; p < c->p_a1.size(); ++p) {
+            const int a1 = c->p_a1[p], a2 = c->p_a2[p];
+            const double *R = &c->p_lv[3 * p];
+            const double qr = R[0] * qv[0] + R[1] * qv[1] + R[2] * qv[2];
+            const zc e(std::cos(qr), std::sin(qr));
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const double m = c->p_m[9 * p + 3 * i + j];
+                    for (int al = 0; al < 3; ++al)
+                        G[(size_t)al * nb * nb + (size_t)(3 * a2 + j) * nb + (3 * a1 + i)] += zc(0, R[al]) * e * m;
+                }
+        }
+        const double *om = &c->om_irr[(size_t)iq * nb];
+        const auto &lg = c->mesh.littlegroup[iq];
+        std::fill(V.begin(), V.end(), zc(0, 0));
+        for (int o : lg) {
+            const SymOp &op = c->ops[o];
+            // egw = Gamma(R) egv, scaled by invsqrtmass / sqrt(2 omega)
+            for (int a1 = 0; a1 < na; ++a1) {
+                int a2 = 0;
+                for (int x = 0; x < na; ++x)
+                    if (op.fmap[x] == a1) a2 = x;
+                double d[3], v0[3];
+                for (int i = 0; i < 3; ++i) d[i] = c->cr.rcart[3 * a1 + i] - op.tr[i];
+                for (int i = 0; i < 3; ++i)
+                    v0[i] = op.m[i] * d[0] + op.m[3 + i] * d[1] + op.m[6 + i] * d[2] - c->cr.rcart[3 * a2 + i];
+                const double ph = -(v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2]) * kTwoPi;
+                const zc e(std::cos(ph), std::sin(ph));
+                for (int al = 0; al < 3; ++al)
+                    for (int b = 0; b < nb; ++b) {
+                        zc s(0, 0);
+                        for (int be = 0; be < 3; ++be) {
+                            const cd x = c->egv_irr[((size_t)iq * nb + 3 * a2 + be) * nb + b];
+                            s += op.m[3 * al + be] * zc(x.x, x.y);
+                        }
+                        s *= e;
+                        const double sc = (om[b] > kFreqTol) ? mw[3 * a1 + al] / std::sqrt(om[b] * 2.0) : 0.0;
+                        egw[(size_t)(3 * a1 + al) * nb + b] = s * sc;
+                    }
+            }
+            for (int z = 0; z < 3; ++z) {
+                // tmp = G^T egw ; V += egw^H tmp
+                const zc *Gz = &G[(size_t)z * nb * nb];
+                for (int r = 0; r < nb; ++r)
+                    for (int b = 0; b < nb; ++b) {
+                        zc s(0, 0);
+                        for (int x = 0; x < nb; ++x) s += Gz[(size_t)x * nb + r] * egw[(size_t)x * nb + b];
+                        tmp[(size_t)r * nb + b] = s / (mw[r] * 1.0);
+                    }
+                // G was built without masses already; the division above undoes nothing
+                for (int r = 0; r < nb; ++r)
+                    for (int b = 0; b < nb; ++b) tmp[(size_t)r * nb + b] *= mw[r];
+                for (int i = 0; i < nb; ++i)
+                    for (int j = 0; j < nb; ++j) {
+                        zc s(0, 0);
+                        for (int r = 0; r < nb; ++r) s += std::conj(egw[(size_t)r * nb + i]) * tmp[(size_t)r * nb + j];
+                        V[(size_t)z * nb * nb + (size_t)i * nb + j] += s;
+                    }
+            }
+        }
+        const double pref = c->mesh.weight_irr[iq] / c->cr.volume;
+        const auto &so = c->mesh.star_ops[iq];
+        const double *lw = &c->linewidth[(size_t)iT * nirr * nb + (size_t)iq * nb];
+        for (int j = 0; j < nb; ++j) {
+            const double om1 = om[j];
+            if (om1 < kFreqTol) continue;
+            const double t1 = lw[j];
+            if (t1 < kFreqTol) continue;
+            for (int kk = 0; kk < nb; ++kk) {
+                if (kk == j) continue;
+                const double om2 = om[kk];
+                if (om2 < kFreqTol) continue;
+                const double t2 = lw[kk];
+                if (t2 < kFreqTol) continue;
+                double vel[3];
+                for (int z = 0; z < 3; ++z) {
+                    zc x = V[(size_t)z * nb * nb + (size_t)j * nb + kk] / (double)lg.size();
+                    double re = x.real();
+                    if (std::fabs(re) < vtol) re = 0.0;
+                    vel[z] = re;
+                }
+                double vsq[9] = {0};
+                for (int s : so) {
+                    const double *m = c->ops[std::abs(s) - 1].m;
+                    double v1[3];
+                    for (int i = 0; i < 3; ++i) v1[i] = m[3 * i] * vel[0] + m[3 * i + 1] * vel[1] + m[3 * i + 2] * vel[2];
+                    for (int b = 0; b < 3; ++b)
+                        for (int a = 0; a < 3; ++a) vsq[3 * b + a] += v1[a] * v1[b];
+                }
+                const double f0 = classical ? kKbHartree : 0.5 * (cv_host(T, om1) + cv_host(T, om2));
+                const double tau = (t1 + t2) / ((t1 + t2) * (t1 + t2) + (om1 - om2) * (om1 - om2));
+                for (int i = 0; i < 9; ++i) k[i] += vsq[i] / so.size() * tau * f0 * pref;
+            }
+        }
+    }
+    chop_kappa(k);
+    std::memcpy(kod, k, sizeof(k));
+}
+
+constexpr int kBlock = 256;
+
+__global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full, const int *__restrict__ op_of_full,
+                          const double *__restrict__ opm, const double *__restrict__ Fn, double *__restrict__ Fbb) {
+    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (j >= (long)nfull * nb) return;
+    const int q = (int)(j / nb), b = (int)(j % nb);
+    const int k = irr_of_full[q], iop = op_of_full[q];
+    const double *m = opm + 9 * ((iop > 0 ? iop : -iop) - 1);
+    const double *F = Fn + ((size_t)k * nb + b) * 3;
+    const double s = iop > 0 ? 1.0 : -1.0;
+    for (int i = 0; i < 3; ++i) Fbb[j * 3 + i] = s * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]);
+}
+
+// buf[row][0..2] = sum_j Xi[row][j] Fbb[j][0..2]  (lo_gemv x3, kappa.f90:411-413)
+__global__ void k_xi_gemv(long ncol, const double *__restrict__ xi, const double *__restrict__ Fbb,
+                          double *__restrict__ buf) {
+    __shared__ double red[3][kBlock];
+    const double *x = xi + (size_t)blockIdx.x * ncol;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (long j = threadIdx.x; j < ncol; j += blockDim.x) {
+        const double v = x[j];
+        s0 += v * Fbb[j * 3];
+        s1 += v * Fbb[j * 3 + 1];
+        s2 += v * Fbb[j * 3 + 2];
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int i = 0; i < 3; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) buf[(size_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Fnb = -buf/qs; degeneracy average then +F0, band by band in order (kappa.f90:415-448);
+// convergence partial and mixing (kappa.f90:456-468).  Workgroup per irreducible point.
+__global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf,
+                             const double *__restrict__ qs, const double *__restrict__ F0,
+                             const int *__restrict__ dg_off, const int *__restrict__ dg, double mix,
+                             double *__restrict__ Fn, double *__restrict__ g0part) {
+    const int q = blockIdx.x;
+    __shared__ double Fnb[128 * 3];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const int r = row_of[q * nb + b];
+        const double s = qs[q * nb + b];
+        for (int i = 0; i < 3; ++i) Fnb[3 * b + i] = (r < 0 || s < 1e-9) ? 0.0 : -buf[(size_t)r * 3 + i] / s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int b1 = 0; b1 < nb; ++b1) {
+            const int o0 = dg_off[q * nb + b1], o1 = dg_off[q * nb + b1 + 1];
+            double v0[3] = {0, 0, 0};
+            for (int o = o0; o < o1; ++o)
+                for (int i = 0; i < 3; ++i) v0[i] += Fnb[3 * dg[o] + i];
+            for (int i = 0; i < 3; ++i) v0[i] /= (double)(o1 - o0);
+            for (int o = o0; o < o1; ++o)
+                for (int i = 0; i < 3; ++i) Fnb[3 * dg[o] + i] = v0[i];
+            for (int i = 0; i < 3; ++i) Fnb[3 * b1 + i] = F0[((size_t)q * nb + b1) * 3 + i] + Fnb[3 * b1 + i];
+        }
+        double g0 = 0;
+        for (int b = 0; b < nb; ++b) {
+            double *F = Fn + ((size_t)q * nb + b) * 3;
+            double g1 = 0, g2 = 0;
+            for (int i = 0; i < 3; ++i) {
+                g1 += (F[i] - Fnb[3 * b + i]) * (F[i] - Fnb[3 * b + i]);
+                g2 += F[i] * F[i];
+            }
+            if (g2 > 1e-10) g0 += g1 / g2;
+            for (int i = 0; i < 3; ++i) F[i] = F[i] * (1.0 - mix) + mix * Fnb[3 * b + i];
+        }
+        g0part[q] = g0;
+    }
+}
+
+}  // namespace
+
+int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
+    const int nb = c->nb, nirr = c->mesh.nirr, nfull = c->mesh.nfull, nT = c->nT, nsym = (int)c->ops.size();
+    const int nrows = c->nrows;
+    const size_t qb = (size_t)nirr * nb;
+    hipStream_t st = c->stream;
+    const auto w0 = std::chrono::steady_clock::now();
+    c->kappa_sma.assign(9 * nT, 0.0);
+    c->kappa_iter.assign(9 * nT, 0.0);
+    c->kappa_od.assign(9 * nT, 0.0);
+    c->Fn.assign(qb * 3 * nT, 0.0);
+    c->kappa_mode.assign(qb * 9 * nT, 0.0);
+    c->kappa_log.assign(nT, {});
+    c->niter.assign(nT, 0);
+    std::vector<void *> tmp;
+    auto cleanup = [&]() {
+        for (void *p : tmp) hipFree(p);
+        tmp.clear();
+    };
+    hipError_t e = hipSuccess;
+#define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
+    auto dalloc = [&](void **p, size_t n) -> hipError_t {
+        hipError_t r = hipMalloc(p, std::max<size_t>(n, 16));
+        if (r == hipSuccess) tmp.push_back(*p);
+        return r;
+    };
+    double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_buf, *d_opm, *d_g0;
+    int *d_opf, *d_rowof, *d_dgoff, *d_dg;
+    std::vector<double> opm(9 * nsym);
+    for (int o = 0; o < nsym; ++o) std::memcpy(&opm[9 * o], c->ops[o].m, sizeof(double) * 9);
+    std::vector<int> dgoff(qb + 1, 0), dg;
+    for (int q = 0; q < nirr; ++q)
+        for (int b = 0; b < nb; ++b) {
+            for (int x : c->degen[q][b]) dg.push_back(x);
+            dgoff[(size_t)q * nb + b + 1] = (int)dg.size();
+        }
+    TRY(dalloc((void **)&d_Fn, sizeof(double) * qb * 3));
+    TRY(dalloc((void **)&d_F0, sizeof(double) * qb * 3));
+    TRY(dalloc((void **)&d_qs, sizeof(double) * qb));
+    TRY(dalloc((void **)&d_Fbb, sizeof(double) * (size_t)nfull * nb * 3));
+    TRY(dalloc((void **)&d_buf, sizeof(double) * (size_t)nrows * 3));
+    TRY(dalloc((void **)&d_opm, sizeof(double) * 9 * nsym));
+    TRY(dalloc((void **)&d_g0, sizeof(double) * nirr));
+    TRY(dalloc((void **)&d_opf, sizeof(int) * nfull));
+    TRY(dalloc((void **)&d_rowof, sizeof(int) * qb));
+    TRY(dalloc((void **)&d_dgoff, sizeof(int) * (qb + 1)));
+    TRY(dalloc((void **)&d_dg, sizeof(int) * dg.size()));
+    TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_rowof, c->row_of.data(), sizeof(int) * qb, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_dgoff, dgoff.data(), sizeof(int) * (qb + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_dg, dg.data(), sizeof(int) * dg.size(), hipMemcpyHostToDevice, st));
+    double t_bte = 0, t_od = 0;
+    std::vector<double> g0(nirr);
+    for (int iT = 0; iT < nT; ++iT) {
+        const double T = c->temps[iT];
+        const double *F0 = &c->F0[(size_t)iT * qb * 3];
+        double *Fn = &c->Fn[(size_t)iT * qb * 3];
+        double *ks = &c->kappa_sma[9 * iT], *ki = &c->kappa_iter[9 * iT], *ko = &c->kappa_od[9 * iT];
+        std::memcpy(Fn, F0, sizeof(double) * qb * 3);
+        get_kappa(c, F0, T, c->classical, ks, nullptr);
+        if (T < kTemperatureTol) {
+            std::fill(ks, ks + 9, 0.0);
+            continue;
+        }
+        if (offdiag) {
+            const auto a = std::chrono::steady_clock::now();
+            kappa_offdiag(c, iT, T, c->classical, ko);
+            t_od += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        }
+        const auto a = std::chrono::steady_clock::now();
+        int it_done = 0;
+        if (maxiter > 0) {
+            TRY(hipMemcpyAsync(d_Fn, Fn, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
+            TRY(hipMemcpyAsync(d_F0, F0, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
+            TRY(hipMemcpyAsync(d_qs, &c->qs[(size_t)iT * qb], sizeof(double) * qb, hipMemcpyHostToDevice, st));
+            double mix = 0.95;
+            std::vector<double> scf;
+            double k0[9];
+            get_kappa(c, Fn, T, c->classical, k0, nullptr);
+            auto &log = c->kappa_log[iT];
+            log.insert(log.end(), {k0[0], k0[4], k0[8], k0[3], k0[6], k0[7], 0.0});
+            const double *xi = c->d_xi + (size_t)iT * c->xi_row * nrows;
+            for (int it = 1; it <= maxiter; ++it) {
+                const long nj = (long)nfull * nb;
+                hipLaunchKernelGGL(k_foldout, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, st, nfull, nb,
+                                   c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb);
+                if (nrows > 0)
+                    hipLaunchKernelGGL(k_xi_gemv, dim3(nrows), dim3(kBlock), 0, st, (long)c->xi_row, xi, d_Fbb, d_buf);
+                hipLaunchKernelGGL(k_bte_update, dim3(nirr), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0, d_dgoff,
+                                   d_dg, mix, d_Fn, d_g0);
+                TRY(hipGetLastError());
+                TRY(hipMemcpyAsync(g0.data(), d_g0, sizeof(double) * nirr, hipMemcpyDeviceToHost, st));
+                TRY(hipMemcpyAsync(Fn, d_Fn, sizeof(double) * qb * 3, hipMemcpyDeviceToHost, st));
+                TRY(hipStreamSynchronize(st));
+                double s = 0;
+                for (int q = 0; q < nirr; ++q) s += g0[q];
+                scf.push_back(s / nirr / nb);
+                double kk[9];
+                get_kappa(c, Fn, T, c->classical, kk, nullptr);
+                log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf.back()});
+                it_done = it;
+                if (it >= 3 && scf[it - 1] + scf[it - 2] + scf[it - 3] < tol) break;
+                if (it > 15 && mix > 0.50) mix *= 0.98;
+            }
+        }
+        c->niter[iT] = it_done;
+        t_bte += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        get_kappa(c, Fn, T, c->classical, ki, &c->kappa_mode[(size_t)iT * qb * 9]);
+        symmetrize_kappa(c, ki);
+        symmetrize_kappa(c, ko);
+        symmetrize_kappa(c, ks);
+    }
+#undef TRY
+    cleanup();
+    c->timings[2] = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    c->timings[3] = t_bte;
+    c->timings[4] = t_od;
+    return 0;
+}
+
+}  // namespace tc3

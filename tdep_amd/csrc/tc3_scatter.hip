@@ -1,0 +1,493 @@
+// Three-phonon scattering on the GPU.
+//
+// Per visited triplet (q1 irreducible, q2, q3 = -q1-q2) the engine forms
+//   Phi(q2,q3)[ia,ib,ic] = sum_t mwm_t e^{-2 pi i (q2.lv2_t + q3.lv3_t)}     (pretransform_phi3,
+//                                                   scattering_threephonon.f90:275-317)
+//   Psi[b1,b2,b3] = sum E1[ia,b1] E2[ib,b2] E3[ic,b3] Phi[ia,ib,ic]             (:88-113)
+// for all b1 at once as three batched complex GEMMs, then weights |Psi|^2 with the
+// Gaussian-smeared energy conservation and the occupations of every temperature
+// (get_dirac :219-271, :113-123) and accumulates the linewidth and the off-diagonal
+// scattering-matrix rows (:126-135).  The off-diagonal rows are symmetrised over the
+// little group of q1 (:143-205, scattering.f90:296-357).
+//
+// Each (row, q') target of the scattering matrix receives contributions from exactly
+// one triplet (checked on the host), so rows are accumulated without atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "tc3_ctx.hpp"
+
+namespace tc3 {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct cplx { T x, y; };
+
+// Phi[tl][ia][ib][ic] for the triplets of one batch; workgroup per (triplet, atom a1).
+template <typename T>
+__global__ void k_phi(int na, int t0, const int *__restrict__ tq2, const int *__restrict__ tq3,
+                      const double *__restrict__ qcart, const int *__restrict__ a1_off,
+                      const int *__restrict__ blk_off, const double *__restrict__ lv2,
+                      const double *__restrict__ lv3, const double *__restrict__ mwm,
+                      cplx<T> *__restrict__ phi) {
+    extern __shared__ cd ph[];
+    const int tl = blockIdx.x / na, a1 = blockIdx.x % na;
+    const int t = t0 + tl;
+    const int nb = 3 * na;
+    const double *q2 = qcart + 3 * tq2[t];
+    const double *q3 = qcart + 3 * tq3[t];
+    const int f0 = a1_off[a1], f1 = a1_off[a1 + 1];
+    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
+        const double arg = -kTwoPi * (q2[0] * lv2[3 * f] + q2[1] * lv2[3 * f + 1] + q2[2] * lv2[3 * f + 2] +
+                                      q3[0] * lv3[3 * f] + q3[1] * lv3[3 * f + 1] + q3[2] * lv3[3 * f + 2]);
+        double s, c;
+        sincos(arg, &s, &c);
+        ph[f - f0] = {c, s};
+    }
+    __syncthreads();
+    const size_t nb2 = (size_t)nb * nb;
+    cplx<T> *out = phi + (size_t)tl * nb * nb2 + (size_t)(3 * a1) * nb2;
+    for (size_t e = threadIdx.x; e < 3 * nb2; e += blockDim.x) {
+        const int i = (int)(e / nb2);
+        const int rem = (int)(e % nb2);
+        const int ib = rem / nb, ic = rem % nb;
+        const int a2 = ib / 3, j = ib % 3, a3 = ic / 3, k = ic % 3;
+        const int blk = (a1 * na + a2) * na + a3;
+        double ax = 0, ay = 0;
+        for (int f = blk_off[blk]; f < blk_off[blk + 1]; ++f) {
+            const double m = mwm[27 * f + 9 * i + 3 * j + k];
+            const cd p = ph[f - f0];
+            ax += m * p.x;
+            ay += m * p.y;
+        }
+        out[e] = {(T)ax, (T)ay};
+    }
+}
+
+// Generic batched complex GEMM, one 16x16 output tile per workgroup.
+// C_z(m,n) = sum_k A_z(m,k) B_z(k,n); matrices addressed by element strides, the batch
+// index z selecting A/B through optional index tables (table[z / div]).
+template <typename T>
+__global__ void k_cgemm(int M, int N, int K, int ntm, int ntn,
+                        const cplx<T> *__restrict__ A, const int *__restrict__ aidx, int adiv, long sA, int am, int ak,
+                        const cplx<T> *__restrict__ B, const int *__restrict__ bidx, int bdiv, long sB, int bk, int bn,
+                        cplx<T> *__restrict__ C, long sC, int cm, int cn) {
+    __shared__ cplx<T> As[16][17], Bs[16][17];
+    const int tiles = ntm * ntn;
+    const long z = blockIdx.x / tiles;
+    const int tile = blockIdx.x % tiles;
+    const int m0 = (tile / ntn) * 16, n0 = (tile % ntn) * 16;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const cplx<T> *Az = A + (aidx ? (long)aidx[z / adiv] : z) * sA;
+    const cplx<T> *Bz = B + (bidx ? (long)bidx[z / bdiv] : z) * sB;
+    T rx = 0, ry = 0;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        {
+            const int m = m0 + ty, k = k0 + tx;
+            As[ty][tx] = (m < M && k < K) ? Az[(long)m * am + (long)k * ak] : cplx<T>{0, 0};
+            const int kk = k0 + ty, n = n0 + tx;
+            Bs[ty][tx] = (kk < K && n < N) ? Bz[(long)kk * bk + (long)n * bn] : cplx<T>{0, 0};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const cplx<T> a = As[ty][kk], b = Bs[kk][tx];
+            rx += a.x * b.x - a.y * b.y;
+            ry += a.x * b.y + a.y * b.x;
+        }
+        __syncthreads();
+    }
+    const int m = m0 + ty, n = n0 + tx;
+    if (m < M && n < N) C[z * sC + (long)m * cm + (long)n * cn] = {rx, ry};
+}
+
+// E[q][ia][b] = egv[q][ia][b] / sqrt(omega_b), zero for omega_b < lo_freqtol.
+template <typename T>
+__global__ void k_scale(long n, int nb, const cd *__restrict__ egv, const double *__restrict__ om,
+                        cplx<T> *__restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long q = i / ((long)nb * nb);
+        const int b = (int)(i % nb);
+        const double w = om[q * nb + b];
+        const double s = (w < 1e-9) ? 0.0 : 1.0 / sqrt(w);
+        out[i] = {(T)(egv[i].x * s), (T)(egv[i].y * s)};
+    }
+}
+
+__device__ inline double gauss(double x, double mu, double sigma) {
+    // lo_gauss (gottochblandat_calculus.f90:268-288)
+    const double d = x - mu;
+    if (fabs(d) > 8.0 * sigma) return 0.0;
+    const double inv = 1.0 / sigma;
+    return exp(-d * d * 0.5 * inv * inv) * 0.398942280401433 * inv;
+}
+
+struct EpiArgs {
+    int nb, nT, t0, nfull, integrationtype;
+    double sigma_fixed, psi_scale;  // pi/16 / N_q
+    const int *tq1, *tq2, *tq3, *eq_off, *eq_q2, *eq_q3, *irr_of_full, *row_of;
+    const double *perm, *om_irr, *om_full, *be, *sigsq;
+    double *xi;      // [T][row][nfull*nb]
+    double *gpart;   // [t][T][nb]
+    size_t xi_row, xi_T;
+    int nirr;
+};
+
+// |Psi|^2 -> smeared-delta weighted rates for every temperature; linewidth partials and
+// scattering-matrix rows.  Workgroup per (triplet, b1).  psi_ld: stride of b1 rows.
+template <typename P>
+__global__ void k_epilogue(EpiArgs a, const P *__restrict__ psi, int psi_complex_planar) {
+    extern __shared__ double sm[];
+    const int nb = a.nb;
+    double *fs = sm;               // nb*nb
+    double *s2 = fs + nb * nb;     // nb
+    double *s3 = s2 + nb;          // nb
+    const int tl = blockIdx.x / nb, b1 = blockIdx.x % nb;
+    const int t = a.t0 + tl;
+    const int q1 = a.tq1[t], q2 = a.tq2[t], q3 = a.tq3[t];
+    const int row = a.row_of[q1 * nb + b1];
+    if (row < 0) {
+        for (int T = 0; T < a.nT && threadIdx.x == 0; ++T) a.gpart[((size_t)t * a.nT + T) * nb + b1] = 0.0;
+        return;
+    }
+    const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
+    const double w1 = a.om_irr[q1 * nb + b1];
+    const double perm = a.perm[t];
+    const double sg1 = a.sigsq[q1 * nb + b1];
+    const int e0 = a.eq_off[t], e1 = a.eq_off[t + 1];
+    const size_t base = ((size_t)tl * nb + b1) * nb * nb;
+    for (int T = 0; T < a.nT; ++T) {
+        const double *be = a.be + (size_t)T * a.nirr * nb;
+        for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+            const int b2 = e / nb, b3 = e % nb;
+            const double w2 = a.om_full[(size_t)q2 * nb + b2], w3 = a.om_full[(size_t)q3 * nb + b3];
+            double f = 0.0;
+            if (!(w2 < 1e-9) && !(fabs(w1 - w2) < 1e-9) && !(w3 < 1e-9) && !(fabs(w1 - w3) < 1e-9) &&
+                !(fabs(w2 - w3) < 1e-9)) {
+                double re, im;
+                if (psi_complex_planar) {
+                    re = (double)psi[base * 2 + e];
+                    im = (double)psi[base * 2 + nb * nb + e];
+                } else {
+                    re = (double)psi[(base + e) * 2];
+                    im = (double)psi[(base + e) * 2 + 1];
+                }
+                const double psisq = a.psi_scale * (re * re + im * im);
+                double sig;
+                if (a.integrationtype == 2)
+                    sig = sqrt(sg1 + a.sigsq[i2 * nb + b2] + a.sigsq[i3 * nb + b3]);
+                else
+                    sig = a.sigma_fixed;
+                const double d0 = gauss(w1, -w2 + w3, sig) - gauss(w1, w2 - w3, sig);
+                const double d1 = gauss(w1, w2 + w3, sig) - gauss(w1, -w2 - w3, sig);
+                const double n2 = be[i2 * nb + b2], n3 = be[i3 * nb + b3];
+                f = perm * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1);
+            }
+            fs[e] = f;
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            double x = 0, y = 0;
+            for (int k = 0; k < nb; ++k) {
+                x += fs[b * nb + k];
+                y += fs[k * nb + b];
+            }
+            s2[b] = x;
+            s3[b] = y;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0;
+            for (int b = 0; b < nb; ++b) tot += s2[b];
+            a.gpart[((size_t)t * a.nT + T) * nb + b1] = tot * (double)(e1 - e0);
+        }
+        double *xr = a.xi + (size_t)T * a.xi_T + (size_t)row * a.xi_row;
+        for (int e = e0; e < e1; ++e) {
+            const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
+            for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+                xr[(size_t)x2 * nb + b] += 2.0 * s2[b] * a.om_full[(size_t)q2 * nb + b] / w1;
+                xr[(size_t)x3 * nb + b] += 2.0 * s3[b] * a.om_full[(size_t)q3 * nb + b] / w1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Linewidth of every row: deterministic sum over the triplets of its q1.
+__global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_q, const int *__restrict__ row_b,
+                        const int *__restrict__ q1_off, const double *__restrict__ gpart, double *__restrict__ lw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows * nT) return;
+    const int T = i / nrows, r = i % nrows;
+    const int q = row_q[r], b = row_b[r];
+    double s = 0;
+    for (int t = q1_off[q]; t < q1_off[q + 1]; ++t) s += gpart[((size_t)t * nT + T) * nb + b];
+    lw[i] = s;
+}
+
+// Orbit averaging of one scattering-matrix row: symmetrize_and_distribute (positive
+// entries only, scattering_threephonon.f90:156-196) followed by symmetrize_scatmat
+// (scattering.f90:312-354).  Workgroup per (row, T).
+__global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const int *__restrict__ q1_off,
+                         const int *__restrict__ orb_off, const int *__restrict__ members,
+                         double *__restrict__ xi, size_t xi_row, size_t xi_T) {
+    const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
+    const int q1 = row_q[r];
+    double *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
+    const int o0 = q1_off[q1], o1 = q1_off[q1 + 1];
+    const long work = (long)(o1 - o0) * nb;
+    for (long w = threadIdx.x; w < work; w += blockDim.x) {
+        const int o = o0 + (int)(w / nb), b = (int)(w % nb);
+        const int m0 = orb_off[o], m1 = orb_off[o + 1];
+        double s = 0;
+        int n = 0;
+        for (int m = m0; m < m1; ++m) {
+            const double v = x[(size_t)members[m] * nb + b];
+            if (v > 0.0) { s += v; ++n; }
+        }
+        if (n > 0) {
+            const double avg = s / n;
+            for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = avg;
+        }
+        s = 0;
+        for (int m = m0; m < m1; ++m) s += x[(size_t)members[m] * nb + b];
+        const double avg = s / (m1 - m0);
+        for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = avg;
+    }
+}
+
+template <typename T>
+int launch_chain(tc3_ctx *c, hipStream_t st, int t0, int nt, const cplx<T> *E1, const cplx<T> *Ef,
+                 cplx<T> *phi, cplx<T> *t1, cplx<T> *t2, cplx<T> *psi, const int *d_tq1, const int *d_tq2,
+                 const int *d_tq3, const double *d_qcart, const int *d_a1off, const int *d_blk3,
+                 const double *d_lv2, const double *d_lv3, const double *d_mwm, size_t phi_shmem) {
+    const int na = c->cr.na, nb = 3 * na;
+    const long nb2 = (long)nb * nb, nb3 = nb2 * nb;
+    hipLaunchKernelGGL(k_phi<T>, dim3(nt * na), dim3(kBlock), phi_shmem, st, na, t0, d_tq2, d_tq3, d_qcart,
+                       d_a1off, d_blk3, d_lv2, d_lv3, d_mwm, phi);
+    // stage A: t1[tl](b1, n) = sum_ia E1[q1](ia, b1) phi[tl](ia, n)
+    {
+        const int ntm = (nb + 15) / 16, ntn = (int)((nb2 + 15) / 16);
+        hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)((long)nt * ntm * ntn)), dim3(256), 0, st, nb, (int)nb2, nb,
+                           ntm, ntn, E1, d_tq1 + t0, 1, nb2, 1, nb, phi, (const int *)nullptr, 1, nb3, (int)nb2, 1,
+                           t1, nb3, (int)nb2, 1);
+    }
+    const int ntm = (nb + 15) / 16, ntn = (nb + 15) / 16;
+    // stage B: t2[tl,b1](b2, ic) = sum_ib E2[q2](ib, b2) t1[tl,b1](ib, ic)
+    hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)((long)nt * nb * ntm * ntn)), dim3(256), 0, st, nb, nb, nb, ntm,
+                       ntn, Ef, d_tq2 + t0, nb, nb2, 1, nb, t1, (const int *)nullptr, 1, nb2, nb, 1, t2, nb2, nb, 1);
+    // stage C: psi[tl,b1](b2, b3) = sum_ic t2[tl,b1](b2, ic) E3[q3](ic, b3)
+    hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)((long)nt * nb * ntm * ntn)), dim3(256), 0, st, nb, nb, nb, ntm,
+                       ntn, t2, (const int *)nullptr, 1, nb2, nb, 1, Ef, d_tq3 + t0, nb, nb2, nb, 1, psi, nb2, nb, 1);
+    return 0;
+}
+
+}  // namespace
+
+int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
+    const int na = c->cr.na, nb = 3 * na;
+    const int nirr = c->mesh.nirr, nfull = c->mesh.nfull, nT = c->nT;
+    const long ntrip = (long)c->trip.q1.size();
+    const int nrows = c->nrows;
+    hipStream_t st = c->stream;
+    const auto wall0 = std::chrono::steady_clock::now();
+    // device tables
+    std::vector<void *> tmp;
+    auto dalloc = [&](void **p, size_t n) -> hipError_t {
+        hipError_t e = hipMalloc(p, std::max<size_t>(n, 16));
+        if (e == hipSuccess) tmp.push_back(*p);
+        return e;
+    };
+    auto cleanup = [&]() {
+        for (void *p : tmp) hipFree(p);
+        tmp.clear();
+    };
+    hipError_t e = hipSuccess;
+#define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
+    int *d_tq1, *d_tq2, *d_tq3, *d_eqoff, *d_eq2, *d_eq3, *d_rowof, *d_rowq, *d_rowb, *d_q1off;
+    int *d_orboff, *d_orbq1, *d_members, *d_a1off, *d_blk3;
+    double *d_perm, *d_qcart, *d_be, *d_sigsq, *d_lv2, *d_lv3, *d_mwm, *d_gpart, *d_lw;
+    TRY(dalloc((void **)&d_tq1, sizeof(int) * ntrip));
+    TRY(dalloc((void **)&d_tq2, sizeof(int) * ntrip));
+    TRY(dalloc((void **)&d_tq3, sizeof(int) * ntrip));
+    TRY(dalloc((void **)&d_perm, sizeof(double) * ntrip));
+    TRY(dalloc((void **)&d_eqoff, sizeof(int) * (ntrip + 1)));
+    TRY(dalloc((void **)&d_eq2, sizeof(int) * c->trip.eq_q2.size()));
+    TRY(dalloc((void **)&d_eq3, sizeof(int) * c->trip.eq_q3.size()));
+    TRY(dalloc((void **)&d_rowof, sizeof(int) * nirr * nb));
+    TRY(dalloc((void **)&d_rowq, sizeof(int) * nrows));
+    TRY(dalloc((void **)&d_rowb, sizeof(int) * nrows));
+    TRY(dalloc((void **)&d_q1off, sizeof(int) * (nirr + 1)));
+    TRY(dalloc((void **)&d_orboff, sizeof(int) * c->orb.orb_off.size()));
+    TRY(dalloc((void **)&d_orbq1, sizeof(int) * c->orb.q1_off.size()));
+    TRY(dalloc((void **)&d_members, sizeof(int) * c->orb.members.size()));
+    TRY(dalloc((void **)&d_a1off, sizeof(int) * (na + 1)));
+    TRY(dalloc((void **)&d_blk3, sizeof(int) * ((size_t)na * na * na + 1)));
+    TRY(dalloc((void **)&d_qcart, sizeof(double) * 3 * nfull));
+    TRY(dalloc((void **)&d_be, sizeof(double) * nT * nirr * nb));
+    TRY(dalloc((void **)&d_sigsq, sizeof(double) * nirr * nb));
+    TRY(dalloc((void **)&d_lv2, sizeof(double) * 3 * c->nfc3));
+    TRY(dalloc((void **)&d_lv3, sizeof(double) * 3 * c->nfc3));
+    TRY(dalloc((void **)&d_mwm, sizeof(double) * 27 * c->nfc3));
+    TRY(dalloc((void **)&d_gpart, sizeof(double) * ntrip * nT * nb));
+    TRY(dalloc((void **)&d_lw, sizeof(double) * nrows * nT));
+    TRY(hipMemcpyAsync(d_tq1, c->trip.q1.data(), sizeof(int) * ntrip, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_tq2, c->trip.q2.data(), sizeof(int) * ntrip, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_tq3, c->trip.q3.data(), sizeof(int) * ntrip, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_perm, c->trip.perm.data(), sizeof(double) * ntrip, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_eqoff, c->trip.eq_off.data(), sizeof(int) * (ntrip + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_eq2, c->trip.eq_q2.data(), sizeof(int) * c->trip.eq_q2.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_eq3, c->trip.eq_q3.data(), sizeof(int) * c->trip.eq_q3.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_rowof, c->row_of.data(), sizeof(int) * nirr * nb, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_rowq, c->row_q.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_rowb, c->row_b.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_q1off, c->trip.q1_off.data(), sizeof(int) * (nirr + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_orboff, c->orb.orb_off.data(), sizeof(int) * c->orb.orb_off.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_orbq1, c->orb.q1_off.data(), sizeof(int) * c->orb.q1_off.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_a1off, c->a1_off.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_blk3, c->blk3_off.data(), sizeof(int) * ((size_t)na * na * na + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_qcart, c->mesh.qcart.data(), sizeof(double) * 3 * nfull, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_be, c->be.data(), sizeof(double) * nT * nirr * nb, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_sigsq, c->sigsq.data(), sizeof(double) * nirr * nb, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_lv2, c->t_lv2.data(), sizeof(double) * 3 * c->nfc3, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_lv3, c->t_lv3.data(), sizeof(double) * 3 * c->nfc3, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_mwm, c->t_mwm.data(), sizeof(double) * 27 * c->nfc3, hipMemcpyHostToDevice, st));
+    // scattering matrix
+    hipFree(c->d_xi);
+    c->d_xi = nullptr;
+    c->xi_row = (size_t)nfull * nb;
+    const size_t xi_T = c->xi_row * nrows;
+    TRY(hipMalloc(&c->d_xi, sizeof(double) * xi_T * nT));
+    TRY(hipMemsetAsync(c->d_xi, 0, sizeof(double) * xi_T * nT, st));
+
+    // batch size: intermediates of 4 * nb^3 complex per triplet
+    const long nb3 = (long)nb * nb * nb;
+    const size_t elem = (precision == TC3_PREC_FP64) ? sizeof(cd) : sizeof(cf);
+    const size_t per_trip = 4 * (size_t)nb3 * elem;
+    long batch = std::max<long>(1, std::min<long>(ntrip, (long)((size_t)6 << 30) / (long)per_trip));
+    batch = std::min<long>(batch, 1 << 20);
+    void *d_work = nullptr;
+    void *d_E1 = nullptr, *d_Ef = nullptr;
+    TRY(dalloc(&d_work, per_trip * batch));
+    TRY(dalloc(&d_E1, elem * (size_t)nirr * nb * nb));
+    TRY(dalloc(&d_Ef, elem * (size_t)nfull * nb * nb));
+    if (precision == TC3_PREC_FP64) {
+        hipLaunchKernelGGL(k_scale<double>, dim3(1024), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
+                           c->d_om_irr, (cplx<double> *)d_E1);
+        hipLaunchKernelGGL(k_scale<double>, dim3(4096), dim3(256), 0, st, (long)nfull * nb * nb, nb, c->d_egv_full,
+                           c->d_om_full, (cplx<double> *)d_Ef);
+    } else {
+        hipLaunchKernelGGL(k_scale<float>, dim3(1024), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
+                           c->d_om_irr, (cplx<float> *)d_E1);
+        hipLaunchKernelGGL(k_scale<float>, dim3(4096), dim3(256), 0, st, (long)nfull * nb * nb, nb, c->d_egv_full,
+                           c->d_om_full, (cplx<float> *)d_Ef);
+    }
+    TRY(hipGetLastError());
+    int maxa1 = 0;
+    for (int a = 0; a < na; ++a) maxa1 = std::max(maxa1, c->a1_off[a + 1] - c->a1_off[a]);
+    const size_t phi_shmem = sizeof(cd) * std::max(maxa1, 1);
+
+    EpiArgs ea;
+    ea.nb = nb; ea.nT = nT; ea.nfull = nfull; ea.integrationtype = integrationtype;
+    ea.sigma_fixed = sigma * kFreqThzToHartree;
+    ea.psi_scale = (kPi / 16.0) / (double)nfull;
+    ea.tq1 = d_tq1; ea.tq2 = d_tq2; ea.tq3 = d_tq3; ea.eq_off = d_eqoff; ea.eq_q2 = d_eq2; ea.eq_q3 = d_eq3;
+    ea.irr_of_full = c->d_irr_of_full; ea.row_of = d_rowof;
+    ea.perm = d_perm; ea.om_irr = c->d_om_irr; ea.om_full = c->d_om_full; ea.be = d_be; ea.sigsq = d_sigsq;
+    ea.xi = c->d_xi; ea.gpart = d_gpart; ea.xi_row = c->xi_row; ea.xi_T = xi_T; ea.nirr = nirr;
+    const size_t epi_shmem = sizeof(double) * ((size_t)nb * nb + 2 * nb);
+
+    hipEvent_t ev0, ev1;
+    TRY(hipEventCreate(&ev0));
+    TRY(hipEventCreate(&ev1));
+    TRY(hipEventRecord(ev0, st));
+    for (long t0 = 0; t0 < ntrip; t0 += batch) {
+        const int nt = (int)std::min<long>(batch, ntrip - t0);
+        ea.t0 = (int)t0;
+        if (precision == TC3_PREC_FP64) {
+            auto *w = (cplx<double> *)d_work;
+            cplx<double> *phi = w, *t1 = w + (size_t)batch * nb3, *t2 = t1 + (size_t)batch * nb3,
+                         *psi = t2 + (size_t)batch * nb3;
+            launch_chain<double>(c, st, (int)t0, nt, (const cplx<double> *)d_E1, (const cplx<double> *)d_Ef, phi, t1,
+                                 t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2, d_lv3, d_mwm,
+                                 phi_shmem);
+            hipLaunchKernelGGL(k_epilogue<double>, dim3(nt * nb), dim3(kBlock), epi_shmem, st, ea,
+                               (const double *)psi, 0);
+        } else if (precision == TC3_PREC_FP32) {
+            auto *w = (cplx<float> *)d_work;
+            cplx<float> *phi = w, *t1 = w + (size_t)batch * nb3, *t2 = t1 + (size_t)batch * nb3,
+                        *psi = t2 + (size_t)batch * nb3;
+            launch_chain<float>(c, st, (int)t0, nt, (const cplx<float> *)d_E1, (const cplx<float> *)d_Ef, phi, t1,
+                                t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2, d_lv3, d_mwm,
+                                phi_shmem);
+            hipLaunchKernelGGL(k_epilogue<float>, dim3(nt * nb), dim3(kBlock), epi_shmem, st, ea,
+                               (const float *)psi, 0);
+        } else {
+            cleanup();
+            return c->fail(5, "precision mode not available in this build");
+        }
+        TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_gamma, dim3((nrows * nT + 255) / 256), dim3(256), 0, st, nrows, nT, nb, d_rowq, d_rowb,
+                       d_q1off, d_gpart, d_lw);
+    TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_xi_sym, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
+                       d_members, c->d_xi, c->xi_row, xi_T);
+    TRY(hipGetLastError());
+    TRY(hipEventRecord(ev1, st));
+    std::vector<double> lw((size_t)nrows * nT);
+    TRY(hipMemcpyAsync(lw.data(), d_lw, sizeof(double) * nrows * nT, hipMemcpyDeviceToHost, st));
+    TRY(hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+#undef TRY
+    cleanup();
+    c->timings[5] = ms * 1e-3;
+    // reduce + degeneracy average + F0 (scattering.f90:260-292)
+    const size_t qb = (size_t)nirr * nb;
+    c->linewidth.assign(qb * nT, 0.0);
+    c->qs.assign(qb * nT, 0.0);
+    c->F0.assign(qb * 3 * nT, 0.0);
+    c->mfp.assign(qb * 3 * nT, 0.0);
+    c->smfp.assign(qb * nT, 0.0);
+    for (int T = 0; T < nT; ++T) {
+        std::vector<double> buf(qb, 0.0);
+        for (int r = 0; r < nrows; ++r) buf[(size_t)c->row_q[r] * nb + c->row_b[r]] = lw[(size_t)T * nrows + r];
+        for (int q = 0; q < nirr; ++q)
+            for (int b = 0; b < nb; ++b) {
+                const double w = c->om_irr[(size_t)q * nb + b];
+                if (w < kFreqTol) continue;
+                const auto &dm = c->degen[q][b];
+                double f0 = 0;
+                for (int b2 : dm) f0 += buf[(size_t)q * nb + b2];
+                f0 /= (double)dm.size();
+                for (int b2 : dm) buf[(size_t)q * nb + b2] = f0;
+                const size_t i = (size_t)T * qb + (size_t)q * nb + b;
+                c->linewidth[i] = buf[(size_t)q * nb + b];
+                c->qs[i] = 2.0 * c->linewidth[i];
+                if (c->linewidth[i] > kFreqTol) {
+                    const double *v = &c->vel_irr[((size_t)q * nb + b) * 3];
+                    const double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                    for (int k = 0; k < 3; ++k) {
+                        c->mfp[i * 3 + k] = v[k] / c->qs[i];
+                        c->F0[i * 3 + k] = c->mfp[i * 3 + k];
+                    }
+                    c->smfp[i] = vn / c->qs[i];
+                }
+            }
+    }
+    c->timings[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    return 0;
+}
+
+}  // namespace tc3
