@@ -36,8 +36,10 @@ enum {
     TC3_PREC_BF16X3 = 2  /* split-BF16 x3 MFMA, FP32 accumulation              */
 };
 
-/* Create / destroy a context bound to one HIP device.  No counterpart in the
- * reference (its state lives in Fortran derived types owned by main.f90). */
+/* Create / destroy a context bound to one HIP device.  device = -1 creates a
+ * host-only context that accepts the structure, IFCs and q-mesh calls and refuses
+ * every device computation.  No counterpart in the reference (its state lives in
+ * Fortran derived types owned by main.f90). */
 int tc3_create(tc3_ctx **ctx, int device);
 void tc3_destroy(tc3_ctx *ctx);
 const char *tc3_last_error(const tc3_ctx *ctx);

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/tc3.h"
 #include "tc3_device.hpp"
 #include "tc3_internal.hpp"
 

@@ -129,7 +129,7 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
             __syncthreads();
         }
         const double diat = red[0];
-        if (threadIdx.x == 0) done = (offt <= 1e-30 * diat) || offt == 0.0;
+        if (threadIdx.x == 0) done = (offt <= 1e-28 * diat) || offt == 0.0;
         __syncthreads();
         if (done) break;
         for (int r = 0; r < mpad - 1; ++r) {
@@ -536,7 +536,7 @@ int run_harmonic(tc3_ctx *c) {
 
     hipLaunchKernelGGL(k_dynmat, dim3(nirr), dim3(kBlock), 0, st, na, d_qv, d_blk, d_lv, d_pm, d_ism, d_D, d_gD);
     TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_jacobi, dim3(nirr), dim3(kBlock), 0, st, nb, d_D, d_V, d_lam, 60, d_stat);
+    hipLaunchKernelGGL(k_jacobi, dim3(nirr), dim3(kBlock), 0, st, nb, d_D, d_V, d_lam, 100, d_stat);
     TRY(hipGetLastError());
     const double veltol = 1e-9 / kGroupvelAuToMs;
     hipLaunchKernelGGL(k_post, dim3(nirr), dim3(kBlock), 0, st, nb, d_lam, d_V, d_gD, d_atg,

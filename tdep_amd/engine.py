@@ -1,0 +1,218 @@
+"""Python mirror of the reference driver for the three-phonon path.
+
+`Engine` owns one libtc3 context and exposes the reference's steps under their
+own names:
+
+  Engine.from_directory(...)      uc%readfromfile / fc%readfromfile / fct%readfromfile
+                                  (main.f90:94-133) and lo_generate_qmesh (main.f90:139)
+  engine.dispersions()            dr%generate (main.f90:146)
+  engine.scattering_rates(...)    sr%generate (main.f90:191) for several temperatures
+  engine.kappa(...)               get_kappa / get_kappa_offdiag / iterative_solution /
+                                  symmetrize_kappa (main.f90:219-244)
+  engine.outfile_text(iT)         outfile.thermal_conductivity (main.f90:248-294)
+
+Errors from the library raise Tc3Error with the reference's exit code.
+"""
+import ctypes
+
+import numpy as np
+
+from . import io as tio
+from .lib import Tc3Error, dptr, iptr, load
+
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+KAPPA_AU_TO_SI = (27.21138602 * 1.6021766208e-19) / (
+    (6.582119514e-16 * 1.6021766208e-19) / (27.21138602 * 1.6021766208e-19)) / 0.52917721067e-10
+
+
+def _fe(x, w=24, d=12):
+    """Fortran Ew.d edit descriptor (0.ddddE+xx)."""
+    if x == 0.0 or not np.isfinite(x):
+        s = "0." + "0" * d + "E+00"
+    else:
+        e = int(np.floor(np.log10(abs(x)))) + 1
+        ms = f"{abs(x) / 10.0 ** e:.{d}f}"
+        if ms.startswith("1."):
+            e += 1
+            ms = f"{abs(x) / 10.0 ** e:.{d}f}"
+        s = ("-" if x < 0 else "") + ms + ("E+" if e >= 0 else "E-") + f"{abs(e):02d}"
+    return s.rjust(w)
+
+
+class Engine:
+    def __init__(self, device=0):
+        self.lib = load()
+        self.ctx = ctypes.c_void_p()
+        r = self.lib.tc3_create(ctypes.byref(self.ctx), int(device))
+        if r:
+            raise Tc3Error(r, "could not create a context on device %d" % device)
+
+    def __del__(self):
+        try:
+            if self.ctx:
+                self.lib.tc3_destroy(self.ctx)
+                self.ctx = ctypes.c_void_p()
+        except Exception:
+            pass
+
+    def _check(self, r):
+        if r:
+            raise Tc3Error(r, self.lib.tc3_last_error(self.ctx).decode())
+
+    # ---------------------------------------------------------------- inputs
+    @classmethod
+    def from_directory(cls, indir, qgrid, device=0, timereversal=True):
+        eng = cls(device)
+        p, fc2, fc3 = tio.read_directory(indir)
+        eng.set_structure(p)
+        eng.set_fc2(fc2)
+        eng.set_fc3(fc3)
+        eng.set_qmesh(qgrid, timereversal)
+        return eng
+
+    def set_structure(self, p):
+        self.structure = p
+        lat = np.ascontiguousarray(p.latvec.T.reshape(-1))  # columns -> consecutive triples
+        rf = np.ascontiguousarray(p.rfrac.reshape(-1))
+        sp = np.ascontiguousarray(p.species, dtype=np.int32)
+        ms = np.ascontiguousarray(p.mass, dtype=np.float64)
+        self._check(self.lib.tc3_set_structure(self.ctx, p.na, dptr(lat), dptr(rf), iptr(sp), dptr(ms)))
+        self.na = p.na
+        self.nb = 3 * p.na
+
+    def set_fc2(self, fc2):
+        m = np.ascontiguousarray(np.transpose(fc2["m"], (0, 2, 1)).reshape(-1))  # m[9n+3j+i]
+        self._check(self.lib.tc3_set_fc2(self.ctx, len(fc2["a1"]), iptr(fc2["a1"]), iptr(fc2["a2"]),
+                                         dptr(np.ascontiguousarray(fc2["lv2"].reshape(-1))), dptr(m)))
+
+    def set_fc3(self, fc3):
+        m = np.ascontiguousarray(np.transpose(fc3["m"], (0, 3, 2, 1)).reshape(-1))  # m[27t+9k+3j+i]
+        self._check(self.lib.tc3_set_fc3(self.ctx, len(fc3["a1"]), iptr(fc3["a1"]), iptr(fc3["a2"]),
+                                         iptr(fc3["a3"]), dptr(np.ascontiguousarray(fc3["lv2"].reshape(-1))),
+                                         dptr(np.ascontiguousarray(fc3["lv3"].reshape(-1))), dptr(m)))
+
+    def set_qmesh(self, qgrid, timereversal=True):
+        d = np.ascontiguousarray(qgrid, dtype=np.int32)
+        self._check(self.lib.tc3_set_qmesh(self.ctx, iptr(d), int(bool(timereversal))))
+        self.qgrid = tuple(int(x) for x in qgrid)
+        self.mesh_info()
+
+    def mesh_info(self):
+        nsym, nfull, nirr, nrows = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ntrip = ctypes.c_int64()
+        self._check(self.lib.tc3_mesh_info(self.ctx, ctypes.byref(nsym), ctypes.byref(nfull), ctypes.byref(nirr),
+                                           ctypes.byref(ntrip), ctypes.byref(nrows)))
+        self.nsym, self.nfull, self.nirr, self.ntriplets = nsym.value, nfull.value, nirr.value, ntrip.value
+        self.nrows = nrows.value
+        return dict(nsym=self.nsym, nfull=self.nfull, nirr=self.nirr, ntriplets=self.ntriplets,
+                    nrows=self.nrows)
+
+    def mesh(self):
+        irr = np.zeros(self.nirr, np.int32)
+        iof = np.zeros(self.nfull, np.int32)
+        oof = np.zeros(self.nfull, np.int32)
+        w = np.zeros(self.nirr)
+        qc = np.zeros(3 * self.nfull)
+        self._check(self.lib.tc3_get_mesh(self.ctx, iptr(irr), iptr(iof), iptr(oof), dptr(w), dptr(qc)))
+        return dict(irr_full_index=irr, irr_of_full=iof, op_of_full=oof, weight_irr=w,
+                    qcart=qc.reshape(self.nfull, 3))
+
+    def symmetry(self):
+        rot = np.zeros(9 * self.nsym)
+        tr = np.zeros(3 * self.nsym)
+        fm = np.zeros(self.na * self.nsym, np.int32)
+        self._check(self.lib.tc3_get_symmetry(self.ctx, dptr(rot), dptr(tr), iptr(fm)))
+        return dict(rot=rot.reshape(self.nsym, 3, 3).transpose(0, 2, 1), tr=tr.reshape(self.nsym, 3),
+                    fmap=fm.reshape(self.nsym, self.na))
+
+    # ------------------------------------------------------------- harmonic
+    def dispersions(self):
+        om = ctypes.c_double()
+        self._check(self.lib.tc3_harmonic(self.ctx, ctypes.byref(om)))
+        self.omega_min = om.value
+        self.mesh_info()
+        return self.omega_min
+
+    def harmonic(self, full=False):
+        nq = self.nfull if full else self.nirr
+        nb = self.nb
+        om = np.zeros(nq * nb)
+        vel = np.zeros(nq * nb * 3)
+        eg = np.zeros(nq * nb * nb * 2)
+        self._check(self.lib.tc3_get_harmonic(self.ctx, int(full), dptr(om), dptr(vel), dptr(eg)))
+        eg = eg.reshape(nq, nb, nb, 2)
+        egv = (eg[..., 0] + 1j * eg[..., 1]).transpose(0, 2, 1)  # [q, component, mode]
+        return om.reshape(nq, nb), vel.reshape(nq, nb, 3), egv
+
+    # ----------------------------------------------------------- scattering
+    def scattering_rates(self, temperatures, precision="fp64", integrationtype=2, sigma=1.0,
+                         classical=False):
+        T = np.ascontiguousarray(np.atleast_1d(temperatures), dtype=np.float64)
+        self.temperatures = T
+        self.classical = bool(classical)
+        self._check(self.lib.tc3_scatter(self.ctx, len(T), dptr(T), PREC[precision], int(integrationtype),
+                                         float(sigma), int(bool(classical))))
+
+    def linewidths(self, iT=0):
+        qb = self.nirr * self.nb
+        lw = np.zeros(qb)
+        F0 = np.zeros(qb * 3)
+        sm = np.zeros(qb)
+        self._check(self.lib.tc3_get_scattering(self.ctx, iT, dptr(lw), dptr(F0), dptr(sm)))
+        return lw.reshape(self.nirr, self.nb), F0.reshape(self.nirr, self.nb, 3), sm.reshape(self.nirr, self.nb)
+
+    def scattering_matrix(self, iT=0, row0=0, nrow=None):
+        nrow = self.nrows - row0 if nrow is None else nrow
+        xi = np.zeros(nrow * self.nfull * self.nb)
+        rq = np.zeros(2 * self.nrows, np.int32)
+        self._check(self.lib.tc3_get_xi(self.ctx, iT, row0, nrow, dptr(xi), iptr(rq)))
+        return xi.reshape(nrow, self.nfull * self.nb), rq.reshape(self.nrows, 2)
+
+    # ---------------------------------------------------------------- kappa
+    def kappa(self, maxiter=200, tol=1e-5, offdiag=True):
+        nT = len(self.temperatures)
+        ks, ki, ko = np.zeros(9 * nT), np.zeros(9 * nT), np.zeros(9 * nT)
+        nit = np.zeros(nT, np.int32)
+        self._check(self.lib.tc3_kappa(self.ctx, int(maxiter), float(tol), int(self.classical), dptr(ks), dptr(ki),
+                                       dptr(ko) if offdiag else None, iptr(nit)))
+        f = lambda a: a.reshape(nT, 3, 3).transpose(0, 2, 1).copy()
+        self.kappa_sma, self.kappa_iter, self.kappa_offdiag = f(ks), f(ki), f(ko)
+        self.niter = nit
+        return self.kappa_sma, self.kappa_iter, self.kappa_offdiag
+
+    def kappa_log(self, iT=0):
+        buf = np.zeros(7 * 1024)
+        n = ctypes.c_int()
+        self._check(self.lib.tc3_get_kappa_log(self.ctx, iT, 1024, dptr(buf), ctypes.byref(n)))
+        return buf[:7 * min(n.value, 1024)].reshape(-1, 7)
+
+    def modes(self, iT=0):
+        qb = self.nirr * self.nb
+        Fn = np.zeros(qb * 3)
+        km = np.zeros(qb * 9)
+        self._check(self.lib.tc3_get_modes(self.ctx, iT, dptr(Fn), dptr(km)))
+        return Fn.reshape(self.nirr, self.nb, 3), km.reshape(self.nirr, self.nb, 3, 3).transpose(0, 1, 3, 2)
+
+    def timings(self):
+        t = np.zeros(8)
+        self._check(self.lib.tc3_get_timings(self.ctx, dptr(t), 8))
+        return dict(harmonic=t[0], scatter=t[1], kappa=t[2], bte=t[3], offdiag=t[4], scatter_kernels=t[5])
+
+    def write_grid_hdf5(self, path, iT=0):
+        self._check(self.lib.tc3_write_grid_hdf5(self.ctx, iT, path.encode()))
+
+    def outfile_text(self, iT=0):
+        """outfile.thermal_conductivity (main.f90:248-294)."""
+        def six(m):
+            m = m * KAPPA_AU_TO_SI
+            v = [m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[0, 2], m[1, 2]]
+            return " " + "".join(" " + _fe(x) for x in v) + "\n"
+        hdr = "#" + "".join(" " + s.rjust(24) for s in ("kxx", "kyy", "kzz", "kxy", "kxz", "kyz")) + "\n"
+        s, it, od = self.kappa_sma[iT], self.kappa_iter[iT], self.kappa_offdiag[iT]
+        t = "# Unit:" + " " * 15 + "W/m/K\n"
+        t += "# Temperature:" + " " * 8 + _fe(float(self.temperatures[iT]), 20, 12) + "\n"
+        t += "# Single mode approximation\n" + hdr + six(s)
+        t += "# Collective contribution\n" + hdr + six(it - s)
+        t += "# Off diagonal (coherence) contribution\n" + hdr + six(od)
+        t += "# Total thermal conductivity\n" + hdr + six(it + od)
+        return t
