@@ -98,7 +98,7 @@ def iterative_solution(p, mesh, disp, ops, sr, temperature, niter=200, tol=1e-5,
 def get_kappa_offdiag(p, fc2, mesh, disp, ops, linewidth, temperature, classical=False):
     """get_kappa_offdiag (kappa.f90:77-306).  The reference forms a (nb^2 x nb^2)
     Kronecker matrix and multiplies it by the flattened gradient; that product
-    equals, entry for entry, avg_op[ egw^H G^T egw ], which is what is formed here."""
+    equals, entry for entry, avg_op[ egw^H G egw ], which is what is formed here."""
     nb = disp.nb
     kod = np.zeros((3, 3))
     ism = p.invsqrtmass
@@ -119,7 +119,7 @@ def get_kappa_offdiag(p, fc2, mesh, disp, ops, linewidth, temperature, classical
             scale = np.where(om > U.FREQTOL, 1.0 / np.sqrt(np.where(om > 0, om, 1.0) * 2.0), 0.0)
             egw = egw * mw[:, None] * scale[None, :]
             for z in range(3):
-                V[z] += egw.conj().T @ G[z].T @ egw
+                V[z] += egw.conj().T @ G[z] @ egw
         V /= len(lg)
         V = U.chop_complex(V, 1e-10 / (U.GROUPVEL_HARTREEBOHR_TO_MS / 1000))
         vel = V.real  # [z, i, j]

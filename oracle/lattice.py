@@ -245,17 +245,14 @@ class Mesh:
 
 
 def _bz_fold(p, v0):
-    """Move a fractional q to the first Brillouin zone the way
-    type_qpointmesh_gridgeneration.f90:115-133 does: nearest reciprocal lattice
-    vector of q + lo_degenvector, only when |q| exceeds the BZ inscribed radius."""
+    """Move a fractional q to the first Brillouin zone as
+    type_qpointmesh_gridgeneration.f90:115-133 does: subtract the reciprocal lattice
+    vector nearest to q + lo_degenvector (the shift breaks ties on the zone boundary).
+    The reference skips points inside the inscribed sphere first; that shortcut only
+    changes which of two boundary-equivalent images a point on the sphere gets, so it is
+    left out to keep the choice independent of rounding."""
     B = p.reclat
-    # inscribed radius of the Wigner-Seitz cell of the reciprocal lattice
-    ns = np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1], indexing="ij")).reshape(3, -1).T
-    ns = ns[np.any(ns != 0, axis=1)]
-    rmin = 0.5 * np.min(np.linalg.norm(ns @ B.T, axis=1))
     q = B @ v0
-    if q @ q <= rmin * rmin:
-        return v0
     v1 = q + U.DEGENVECTOR
     n2 = np.array(np.meshgrid(*([np.arange(-2, 3)] * 3), indexing="ij")).reshape(3, -1).T
     Gs = n2 @ B.T

@@ -264,24 +264,12 @@ int third_grid_index(int i1, int i2, const int *dims) {
 namespace {
 
 void bz_fold(const Crystal &c, const double *v0, double *out) {
-    // type_qpointmesh_gridgeneration.f90:115-133
+    // type_qpointmesh_gridgeneration.f90:115-133: subtract the reciprocal lattice vector
+    // nearest to q + lo_degenvector.  The reference's inscribed-sphere shortcut is left
+    // out: it only decides between two zone-boundary images by rounding.
     const double *B = c.rec;
-    double rmin = 1e300;
-    for (int i = -1; i <= 1; ++i)
-        for (int j = -1; j <= 1; ++j)
-            for (int k = -1; k <= 1; ++k) {
-                if (!i && !j && !k) continue;
-                double n[3] = {(double)i, (double)j, (double)k}, g[3];
-                matvec3(B, n, g);
-                rmin = std::min(rmin, std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]));
-            }
-    rmin *= 0.5;
     double q[3];
     matvec3(B, v0, q);
-    if (q[0] * q[0] + q[1] * q[1] + q[2] * q[2] <= rmin * rmin) {
-        out[0] = v0[0]; out[1] = v0[1]; out[2] = v0[2];
-        return;
-    }
     double v1[3] = {q[0] + kDegenVector[0], q[1] + kDegenVector[1], q[2] + kDegenVector[2]};
     double best = 1e300;
     int bn[3] = {0, 0, 0};

@@ -89,7 +89,7 @@ void symmetrize_kappa(const tc3_ctx *c, double *k) {
     chop_kappa(k);
 }
 
-// get_kappa_offdiag with the coherence velocities formed as avg_op[egw^H G^T egw]
+// get_kappa_offdiag with the coherence velocities formed as avg_op[egw^H G egw]
 // (entry for entry the product the reference forms through its Kronecker matrix).
 void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, double *kod) {
     const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr;
@@ -146,17 +146,14 @@ void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, double *k
                     }
             }
             for (int z = 0; z < 3; ++z) {
-                // tmp = G^T egw ; V += egw^H tmp
+                // tmp = G egw ; V += egw^H tmp
                 const zc *Gz = &G[(size_t)z * nb * nb];
                 for (int r = 0; r < nb; ++r)
                     for (int b = 0; b < nb; ++b) {
                         zc s(0, 0);
-                        for (int x = 0; x < nb; ++x) s += Gz[(size_t)x * nb + r] * egw[(size_t)x * nb + b];
-                        tmp[(size_t)r * nb + b] = s / (mw[r] * 1.0);
+                        for (int x = 0; x < nb; ++x) s += Gz[(size_t)r * nb + x] * egw[(size_t)x * nb + b];
+                        tmp[(size_t)r * nb + b] = s;
                     }
-                // G was built without masses already; the division above undoes nothing
-                for (int r = 0; r < nb; ++r)
-                    for (int b = 0; b < nb; ++b) tmp[(size_t)r * nb + b] *= mw[r];
                 for (int i = 0; i < nb; ++i)
                     for (int j = 0; j < nb; ++j) {
                         zc s(0, 0);

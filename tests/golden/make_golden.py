@@ -1,0 +1,44 @@
+"""Write the oracle's FP64 results for small workloads as committed fixtures.
+
+Run from the repository root:  python tests/golden/make_golden.py
+The inputs are regenerated from tdep_amd.inputs with the default seed; each fixture
+stores the q-grid, temperature and the oracle's frequencies, velocities, linewidths
+and kappa tensors (atomic units)."""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import run as R  # noqa: E402
+from tdep_amd import inputs  # noqa: E402
+
+CASES = [("fcc_al", (6, 6, 6), (300.0,)), ("rocksalt", (4, 4, 4), (100.0, 300.0)),
+         ("sl16", (2, 2, 2), (300.0,))]
+
+
+def main():
+    out = os.path.join(ROOT, "tests", "golden")
+    for name, grid, temps in CASES:
+        with tempfile.TemporaryDirectory() as d:
+            inputs.make_config(name, d)
+            pre = R.setup(d, grid)
+            rec = dict(grid=np.array(grid), temperatures=np.array(temps), om_irr=pre[5].om_irr,
+                       vel_irr=pre[5].vel_irr, nirr=pre[4].nirr)
+            for iT, T in enumerate(temps):
+                r = R.run(d, grid, T, pre=pre)
+                rec[f"linewidth_{iT}"] = r.sr.linewidth
+                rec[f"kappa_sma_{iT}"] = r.kappa_sma
+                rec[f"kappa_iter_{iT}"] = r.kappa_iter
+                rec[f"kappa_offdiag_{iT}"] = r.kappa_offdiag
+                rec[f"niter_{iT}"] = len(r.scf)
+            fn = os.path.join(out, f"oracle_{name}_{'x'.join(map(str, grid))}.npz")
+            np.savez_compressed(fn, **rec)
+            print(fn, {k: np.shape(v) for k, v in rec.items()})
+
+
+if __name__ == "__main__":
+    main()

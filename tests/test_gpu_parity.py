@@ -1,0 +1,109 @@
+"""HIP path against the FP64 oracle and the committed oracle fixtures, through the C ABI.
+
+Tolerances:
+  FP64 mode:      frequencies/eigenvectors 1e-10, linewidths and Xi 1e-9 relative,
+                  kappa 1e-8 relative (summation order is the only difference).
+  FP32 / BF16x3:  kappa within 1e-4 relative of the FP64 GPU result, linewidths of the
+                  strongly scattered modes within 1e-3 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _engine(inputs_dir, name, grid):
+    from tdep_amd.engine import Engine
+    print(f"[case] {name} {grid}", flush=True)
+    e = Engine.from_directory(inputs_dir(name), grid, device=0)
+    e.dispersions()
+    return e
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4)), ("sl16", (2, 2, 2))])
+def test_harmonic_matches_oracle(inputs_dir, oracle_setup, name, grid):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
+    e = _engine(inputs_dir, name, grid)
+    om, vel, egv = e.harmonic()
+    assert np.abs(om - disp.om_irr).max() < 1e-10 * np.abs(disp.om_irr).max()
+    assert np.abs(vel - disp.vel_irr).max() < 1e-9 * np.abs(disp.vel_irr).max()
+    assert np.abs(egv - disp.egv_irr).max() < 1e-9
+    omf, velf, egvf = e.harmonic(full=True)
+    assert np.abs(egvf - disp.egv_full).max() < 1e-9
+    assert np.abs(velf - disp.vel_full).max() < 1e-9 * np.abs(disp.vel_full).max()
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4)), ("sl16", (2, 2, 2))])
+def test_fp64_scattering_and_kappa_match_golden(inputs_dir, name, grid):
+    g = np.load(os.path.join(GOLDEN, f"oracle_{name}_{'x'.join(map(str, grid))}.npz"))
+    e = _engine(inputs_dir, name, grid)
+    T = g["temperatures"]
+    e.scattering_rates(T, precision="fp64")
+    ks, ki, ko = e.kappa()
+    for iT in range(len(T)):
+        lw, F0, _ = e.linewidths(iT)
+        assert _rel(lw, g[f"linewidth_{iT}"]) < 1e-9
+        assert _rel(ks[iT], g[f"kappa_sma_{iT}"]) < 1e-9
+        assert _rel(ki[iT], g[f"kappa_iter_{iT}"]) < 1e-7
+        assert _rel(ko[iT], g[f"kappa_offdiag_{iT}"]) < 1e-8
+        assert e.niter[iT] == int(g[f"niter_{iT}"])
+
+
+@pytest.mark.parametrize("name,grid,T", [("rocksalt", (4, 4, 4), 300.0), ("sl16", (2, 2, 2), 300.0)])
+def test_fp64_scattering_matrix_matches_oracle(inputs_dir, oracle_setup, name, grid, T):
+    from oracle import scattering as S
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, T)
+    e = _engine(inputs_dir, name, grid)
+    e.scattering_rates([T], precision="fp64")
+    xi, rq = e.scattering_matrix(0)
+    assert [tuple(x) for x in rq] == list(sr.rows)
+    assert _rel(xi, sr.Xi) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_kappa_within_1e4(inputs_dir, prec):
+    e = _engine(inputs_dir, "sl16", (4, 4, 2))
+    T = [100.0, 300.0, 700.0]
+    e.scattering_rates(T, precision="fp64")
+    k64 = e.kappa()
+    lw64 = [e.linewidths(i)[0] for i in range(3)]
+    e.scattering_rates(T, precision=prec)
+    kmx = e.kappa()
+    for a, b in zip(kmx, k64):
+        assert _rel(a, b) < 1e-4
+    for i in range(3):
+        lw = e.linewidths(i)[0]
+        big = lw64[i] > 1e-3 * lw64[i].max()
+        assert np.abs(lw[big] / lw64[i][big] - 1).max() < 1e-3
+
+
+def test_full_size_superlattice_properties(inputs_dir):
+    """16-atom superlattice at its full 10x10x4 mesh: linewidths are non-negative and
+    equal on degenerate sets, the scattering matrix is non-negative, kappa is tetragonal
+    (xx = yy, no off-diagonal part) and mixed precision stays within 1e-4 of FP64."""
+    e = _engine(inputs_dir, "sl16", (10, 10, 4))
+    om, vel, egv = e.harmonic()
+    e.scattering_rates([300.0], precision="fp64")
+    k64 = e.kappa(offdiag=False)
+    lw, F0, _ = e.linewidths(0)
+    assert lw.min() >= 0
+    for q in range(e.nirr):
+        for b in range(e.nb):
+            same = np.abs(om[q] - om[q, b]) < 1e-9
+            assert np.ptp(lw[q, same]) <= 1e-12 * max(lw[q, same].max(), 1e-300)
+    xi, _ = e.scattering_matrix(0, 0, 64)
+    assert xi.min() >= 0
+    ki = k64[1][0]
+    assert abs(ki[0, 0] - ki[1, 1]) < 1e-10 * ki[0, 0] and np.abs(ki - np.diag(np.diag(ki))).max() == 0
+    e.scattering_rates([300.0], precision="bf16x3")
+    kb = e.kappa(offdiag=False)
+    assert _rel(kb[1][0], ki) < 1e-4

@@ -1,0 +1,101 @@
+"""libtc3.so on the CPU: exported ABI, input readers and the host-side preprocessing
+(space group, q-mesh, irreducible triplets) against the oracle.  No device calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import lattice as L
+from oracle import scattering as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _lib():
+    from tdep_amd.lib import load
+    return load()
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib()
+    hdr = open(os.path.join(ROOT, "include", "tc3.h")).read()
+    names = re.findall(r"^\s*(?:int|void|const char \*)\s*(tc3_\w+)\(", hdr, re.M)
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    from tdep_amd.lib import SIGNATURES
+    assert set(names) == set(SIGNATURES)
+
+
+def test_device_calls_fail_cleanly_on_host_only_context(inputs_dir):
+    from tdep_amd.engine import Engine
+    from tdep_amd.lib import Tc3Error
+    e = Engine.from_directory(inputs_dir("fcc_al"), (4, 4, 4), device=-1)
+    with pytest.raises(Tc3Error) as ei:
+        e.dispersions()
+    assert ei.value.code == 7
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (12, 12, 12)), ("rocksalt", (8, 8, 8)),
+                                       ("sl16", (10, 10, 4))])
+def test_host_mesh_and_triplets_equal_oracle(inputs_dir, name, grid):
+    from tdep_amd.engine import Engine
+    d = inputs_dir(name)
+    e = Engine.from_directory(d, grid, device=-1)
+    p = L.read_ucposcar(os.path.join(d, "infile.ucposcar"))
+    L.read_isotopes(p, os.path.join(d, "infile.isotopes"))
+    ops = L.spacegroup(p)
+    m = L.generate_mesh(p, ops, grid)
+    mm = e.mesh()
+    sy = e.symmetry()
+    assert e.nsym == len(ops) and e.nirr == m.nirr and e.nfull == m.nfull
+    assert np.array_equal(mm["irr_full_index"], m.irr_full_index)
+    assert np.array_equal(mm["irr_of_full"], m.irr_of_full)
+    assert np.array_equal(mm["op_of_full"], m.op_of_full)
+    assert np.allclose(mm["weight_irr"], m.weight_irr, rtol=0, atol=1e-15)
+    assert np.abs(mm["qcart"] - m.qcart).max() < 1e-14
+    for o, r, t, f in zip(ops, sy["rot"], sy["tr"], sy["fmap"]):
+        assert np.array_equal(o.m, r) and np.allclose(o.tr, t, atol=1e-14) and np.array_equal(o.fmap, f)
+    rot = S.rotation_table(p, m, ops)
+    nt = sum(len(S.irreducible_triplets(m, rot, q)) for q in range(m.nirr))
+    assert e.ntriplets == nt
+
+
+def test_readers_on_reference_example_fixture():
+    """examples/example_1_fcc_al infiles of the reference (a real TDEP fit: 43 pairs,
+    85 triplets including three-body clusters)."""
+    from tdep_amd import io as tio
+    d = os.path.join(GOLDEN, "fcc_al_example")
+    p = tio.read_ucposcar(os.path.join(d, "infile.ucposcar"))
+    p.mass = np.array([26.98 * tio.AMU_TO_EMU])
+    fc2 = tio.read_fc2(p, os.path.join(d, "infile.forceconstant"))
+    fc3 = tio.read_fc3(p, os.path.join(d, "infile.forceconstant_thirdorder"))
+    assert len(fc2["a1"]) == 43 and len(fc3["a1"]) == 85
+    # acoustic sum rules of a fitted model hold to the fit's precision
+    assert np.abs(fc2["m"].sum(axis=0)).max() < 1e-6 * np.abs(fc2["m"]).max()
+    # the oracle's reader agrees with the product reader
+    po = L.read_ucposcar(os.path.join(d, "infile.ucposcar"))
+    po.invsqrtmass = np.array([1 / np.sqrt(p.mass[0])])
+    f3o = L.read_fc3(po, os.path.join(d, "infile.forceconstant_thirdorder"))
+    assert np.allclose(f3o["lv2"], fc3["lv2"]) and np.allclose(f3o["m"], fc3["m"], rtol=1e-15)
+    f2o = L.read_fc2(po, os.path.join(d, "infile.forceconstant"))
+    assert np.allclose(f2o["m"], fc2["m"], rtol=1e-15) and np.allclose(f2o["lv2"], fc2["lv2"])
+
+
+def test_outfile_format_matches_reference_fixture():
+    """The reference's own outfile.thermal_conductivity (tests/thermal_conductivity/reference)
+    fixes the layout: header lines, four titled blocks of six E24.12 numbers."""
+    from tdep_amd.engine import _fe
+    ref = open(os.path.join(GOLDEN, "outfile.thermal_conductivity.reference")).read().splitlines()
+    assert ref[0].startswith("# Unit:") and ref[1].startswith("# Temperature:")
+    vals = [ln for ln in ref if not ln.startswith("#")]
+    assert len(vals) == 4
+    for ln in vals:
+        nums = [float(x) for x in ln.split()]
+        assert len(nums) == 6
+        assert ln == " " + "".join(" " + _fe(x) for x in nums)
+    t = float(ref[1].split()[-1])
+    assert ref[1] == "# Temperature:" + " " * 8 + _fe(t, 20, 12)

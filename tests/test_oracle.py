@@ -1,0 +1,339 @@
+"""Pins of the FP64 oracle against closed forms, invariants and brute force.
+
+Every check here is chosen so that a plausible restatement mistake (a dropped 2*pi,
+a transposed dynamical matrix, a wrong phase sign, a wrong triplet multiplicity, a
+wrong Xi normalisation) fails it.
+"""
+import numpy as np
+import pytest
+
+from oracle import kappa as K
+from oracle import lattice as L
+from oracle import phonons as P
+from oracle import scattering as S
+from oracle import units as U
+
+
+# --------------------------------------------------------------------- units & helpers
+def test_unit_constants_against_independent_values():
+    # Hartree energy as a frequency: E_h/h = 6.579683920e15 Hz (CODATA); the reference's
+    # frequency unit is angular (E/hbar), so Hartree->THz = E_h/h in THz.
+    assert abs(U.FREQ_HARTREE_TO_THZ / 6579.683920 - 1) < 1e-7
+    # kappa atomic unit E_h^2 / (hbar a0) in W/m/K
+    Eh, hbar, a0 = 4.3597447222e-18, 1.054571817e-34, 0.529177210903e-10
+    assert abs(U.KAPPA_AU_TO_SI / (Eh * Eh / hbar / a0) - 1) < 1e-6
+    assert abs(U.KB_HARTREE * 300 / (1.380649e-23 * 300 / Eh) - 1) < 1e-5
+
+
+def test_gauss_normalised_and_cut_at_8_sigma():
+    s = 0.37
+    x = np.linspace(-12 * s, 12 * s, 200001)
+    g = U.gauss(x, 0.0, s)
+    assert abs(np.trapezoid(g, x) - 1.0) < 1e-9
+    assert U.gauss(8.0001 * s, 0.0, s) == 0.0 and U.gauss(7.999 * s, 0.0, s) > 0
+
+
+def test_cv_is_temperature_derivative_of_energy():
+    # c_v = d/dT [ omega n(omega, T) ] ; covers the x>21, middle and x<0.02 branches
+    for om, T in ((1e-3, 1.0), (2e-3, 100.0), (1e-4, 300.0), (1e-6, 900.0)):
+        h = T * 1e-5
+        e = lambda t: om * U.planck(t, om)
+        fd = (e(T + h) - e(T - h)) / (2 * h)
+        cv = U.harmonic_oscillator_cv(T, om)
+        assert abs(cv - fd) <= 1e-6 * abs(fd) + 1e-30
+
+
+def test_chop_snaps_to_listed_values_only():
+    x = np.array([0.5 + 1e-12, 0.49, 1e-14, 0.8660254037844386 - 1e-13, 0.333])
+    y = U.chop(x, 1e-10)
+    assert y[0] == 0.5 and y[1] == 0.49 and y[2] == 0.0 and y[3] == 0.8660254037844386 and y[4] == 0.333
+
+
+# --------------------------------------------------------------------- symmetry & mesh
+@pytest.mark.parametrize("name,order", [("fcc_al", 48), ("rocksalt", 48), ("sl16", 8)])
+def test_space_group_is_a_group(oracle_setup, name, order):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, (2, 2, 2))
+    assert len(ops) == order
+    Ms = [o.m for o in ops]
+    for a in Ms:
+        assert np.allclose(a @ a.T, np.eye(3), atol=1e-12)
+        for b in Ms:
+            assert any(np.allclose(a @ b, c, atol=1e-9) for c in Ms)
+    for o in ops:
+        assert sorted(o.fmap) == list(range(p.na))
+        # R r_a + t lands on the mapped atom modulo a lattice vector
+        for a in range(p.na):
+            d = o.fm @ p.rfrac[a] + o.ftr - p.rfrac[o.fmap[a]]
+            assert np.allclose(d, np.round(d), atol=1e-8)
+
+
+def test_fcc_mesh_counts_match_independent_count(oracle_setup):
+    # 12x12x12 FCC with Oh and time reversal: 72 irreducible points and 33,490 visited
+    # triplets, as counted by an independent script for the workload definition.
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("fcc_al", (12, 12, 12))
+    assert mesh.nfull == 1728 and mesh.nirr == 72
+    rot = S.rotation_table(p, mesh, ops)
+    assert sum(len(S.irreducible_triplets(mesh, rot, q)) for q in range(mesh.nirr)) == 33490
+    assert abs(mesh.weight_irr.sum() - 1.0) < 1e-14
+
+
+def test_mesh_maps_irreducible_points_to_full_points(oracle_setup):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("sl16", (4, 4, 2))
+    for q in range(mesh.nfull):
+        k = mesh.irr_of_full[q]
+        iop = mesh.op_of_full[q]
+        v = ops[abs(iop) - 1].rfm @ (p.inv_reclat @ mesh.qcart[mesh.irr_full_index[k]])
+        v = v if iop > 0 else -v
+        d = v - p.inv_reclat @ mesh.qcart[q]
+        assert np.allclose(d, np.round(d), atol=1e-9)
+    # irreducible point = smallest index of its orbit
+    for k, qi in enumerate(mesh.irr_full_index):
+        assert qi == min(np.nonzero(mesh.irr_of_full == k)[0])
+
+
+# --------------------------------------------------------------------- harmonic
+def _textbook_dynmat_monatomic(p, fc2, q):
+    """D(q) = (1/m) sum_R Phi(R) cos(2 pi q.R) for a one-atom cell (Phi(0) included)."""
+    D = np.zeros((3, 3))
+    for n in range(len(fc2["a1"])):
+        D += fc2["m"][n] * np.cos(U.TWOPI * fc2["lv2"][n] @ q)
+    return D / p.mass[0]
+
+
+def test_monatomic_frequencies_match_textbook_form(oracle_setup):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("fcc_al", (6, 6, 6))
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        q = rng.normal(size=3) * 0.05
+        om, e, v = P.harmonic_point(p, fc2, q, False)
+        ref = np.sort(np.sqrt(np.linalg.eigvalsh(_textbook_dynmat_monatomic(p, fc2, q))))
+        assert np.allclose(om, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_acoustic_sum_rule_and_hermiticity(oracle_setup):
+    for name in ("rocksalt", "sl16"):
+        p, fc2, fc3, ops, mesh, disp = oracle_setup(name, (2, 2, 2))
+        D, gD = P.dynamical_matrix(p, fc2, np.zeros(3))
+        assert np.allclose(D, D.conj().T, atol=1e-14)
+        u = np.zeros((3 * p.na, 3))
+        for a in range(p.na):
+            u[3 * a:3 * a + 3] = np.eye(3) / p.invsqrtmass[a]
+        assert np.abs(D @ u).max() < 1e-12 * np.abs(D).max()
+        assert np.all(np.abs(disp.om_irr[0, :3]) == 0.0)
+
+
+def test_group_velocity_is_frequency_gradient(oracle_setup):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (2, 2, 2))
+    q = np.array([0.031, 0.017, -0.011])
+    om, e, v = P.harmonic_point(p, fc2, q, False)
+    h = 1e-6
+    for al in range(3):
+        dq = np.zeros(3); dq[al] = h
+        op_, _, _ = P.harmonic_point(p, fc2, q + dq, False)
+        om_, _, _ = P.harmonic_point(p, fc2, q - dq, False)
+        # velocity is d omega / d(2 pi q)
+        fd = (op_ - om_) / (2 * h) / U.TWOPI
+        assert np.allclose(v[:, al], fd, rtol=1e-5, atol=1e-10)
+
+
+def test_unfolded_eigenvectors_diagonalise_full_grid_matrices(oracle_setup):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("sl16", (4, 4, 2))
+    for q in range(mesh.nfull):
+        D, _ = P.dynamical_matrix(p, fc2, mesh.qcart[q])
+        E = disp.egv_full[q]
+        lam = np.sign(disp.om_full[q]) * disp.om_full[q] ** 2
+        assert np.abs(D @ E - E * lam[None, :]).max() < 1e-10 * np.abs(D).max()
+        assert np.abs(E.conj().T @ E - np.eye(len(lam))).max() < 1e-10
+
+
+def test_canonical_gauge_is_basis_independent(oracle_setup):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("fcc_al", (2, 2, 2))
+    om = disp.om_irr[0]
+    e = disp.egv_irr[0]
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    Uq, _ = np.linalg.qr(X)
+    e2 = e.copy()
+    e2[:, :3] = e[:, :3] @ Uq          # rotate the degenerate acoustic subspace
+    g1 = P.canonical_gauge(om, e)
+    g2 = P.canonical_gauge(om, e2)
+    assert np.abs(g1 - g2).max() < 1e-12
+
+
+# --------------------------------------------------------------------- |Psi|^2
+def test_gemm_chain_equals_reference_zgeru_form():
+    rng = np.random.default_rng(1)
+    nb = 6
+    E = [rng.normal(size=(nb, nb)) + 1j * rng.normal(size=(nb, nb)) for _ in range(3)]
+    ptf = rng.normal(size=(nb, nb, nb)) + 1j * rng.normal(size=(nb, nb, nb))
+    chain = S.psi_chain(E[0], E[1], E[2], ptf)
+    for b1, b2, b3 in ((0, 1, 2), (5, 3, 0), (2, 2, 4)):
+        ref = S.psi_reference_form(E[0][:, b1], E[1][:, b2], E[2][:, b3], ptf)
+        assert abs(chain[b1, b2, b3] - ref) < 1e-12 * abs(ref)
+
+
+def test_pretransform_obeys_acoustic_sum_rule(oracle_setup):
+    # Sum over the second atom, weighted by sqrt(m), of Phi(q2 = 0, q3) vanishes:
+    # a uniform translation of the second phonon costs nothing.
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (2, 2, 2))
+    nb = 3 * p.na
+    q3 = np.array([0.013, -0.021, 0.007])
+    ptf = S.pretransform_phi3(fc3, np.zeros(3), q3, nb)
+    w = np.repeat(1.0 / p.invsqrtmass, 3)
+    for j in range(3):
+        sel = np.arange(j, nb, 3)
+        s = np.einsum("abc,b->ac", ptf[:, sel, :], w[sel])
+        assert np.abs(s).max() < 1e-10 * np.abs(ptf).max()
+
+
+def _brute_force_rates(p, fc3, mesh, disp, T):
+    """Linewidth and scattering-matrix rows of every row by summing over all ordered
+    (q2, q3 = -q1 - q2) pairs of the full grid, with no symmetry reduction at all."""
+    nb = disp.nb
+    be = np.zeros((mesh.nirr, nb))
+    for q in range(mesh.nirr):
+        be[q] = np.where(disp.om_irr[q] >= U.FREQTOL, U.planck(T, disp.om_irr[q]), 0.0)
+    sr = S.scattering_rates(p, fc3, mesh, disp, L.spacegroup(p), T)
+    gam = np.zeros((mesh.nirr, nb))
+    xi = np.zeros((mesh.nirr, nb, mesh.nfull, nb))
+    for q1 in range(mesh.nirr):
+        om1 = disp.om_irr[q1]
+        E1 = disp.egv_irr[q1] / np.sqrt(np.where(om1 > 0, om1, 1))[None]
+        for q2 in range(mesh.nfull):
+            q3 = S.third_grid_index(mesh.irr_full_index[q1], q2, mesh.dims)
+            om2, om3 = disp.om_full[q2], disp.om_full[q3]
+            E2 = disp.egv_full[q2] / np.sqrt(np.where(om2 > 0, om2, 1))[None]
+            E3 = disp.egv_full[q3] / np.sqrt(np.where(om3 > 0, om3, 1))[None]
+            psi = S.psi_chain(E1, E2, E3, S.pretransform_phi3(fc3, mesh.qcart[q2], mesh.qcart[q3], nb))
+            psisq = U.PI / 16 * np.abs(psi) ** 2 / mesh.nfull
+            i2, i3 = mesh.irr_of_full[q2], mesh.irr_of_full[q3]
+            w1, w2, w3 = om1[:, None, None], om2[None, :, None], om3[None, None, :]
+            sig = np.sqrt(sr.sigsq[q1][:, None, None] + sr.sigsq[i2][None, :, None] + sr.sigsq[i3][None, None, :])
+            d0 = U.gauss(w1, -w2 + w3, sig) - U.gauss(w1, w2 - w3, sig)
+            d1 = U.gauss(w1, w2 + w3, sig) - U.gauss(w1, -w2 - w3, sig)
+            n2, n3 = be[i2][None, :, None], be[i3][None, None, :]
+            f = psisq * ((n2 - n3) * d0 + (n2 + n3 + 1) * d1)
+            ok = ((w1 >= U.FREQTOL) & (w2 >= U.FREQTOL) & (w3 >= U.FREQTOL) & (abs(w1 - w2) >= U.FREQTOL)
+                  & (abs(w1 - w3) >= U.FREQTOL) & (abs(w2 - w3) >= U.FREQTOL))
+            f = np.where(ok, f, 0.0)
+            gam[q1] += f.sum(axis=(1, 2))
+            with np.errstate(divide="ignore", invalid="ignore"):
+                r = np.where(om1 > 0, 1.0 / om1, 0.0)
+            xi[q1, :, q2, :] += 2 * f.sum(axis=2) * om2[None, :] * r[:, None]
+            xi[q1, :, q3, :] += 2 * f.sum(axis=1) * om3[None, :] * r[:, None]
+    return sr, gam, xi
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (4, 4, 4)), ("rocksalt", (3, 3, 3))])
+def test_symmetry_reduced_rates_equal_brute_force(oracle_setup, name, grid):
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
+    sr, gam, xi = _brute_force_rates(p, fc3, mesh, disp, 300.0)
+    nb = disp.nb
+    for q in range(mesh.nirr):
+        for b in range(nb):
+            if disp.om_irr[q, b] < U.FREQTOL:
+                continue
+            dm = disp.degen_irr[q][b]
+            ref = gam[q, dm].mean()
+            assert abs(sr.linewidth[q, b] - ref) <= 1e-10 * abs(ref) + 1e-30
+    # scattering matrix: compare sums over degenerate sets of (row band, column band)
+    row_of = {r: i for i, r in enumerate(sr.rows)}
+    for q1 in range(mesh.nirr):
+        groups1 = {tuple(disp.degen_irr[q1][b]) for b in range(nb) if disp.om_irr[q1, b] >= U.FREQTOL}
+        for g1 in groups1:
+            a = sum(sr.Xi[row_of[(q1, b)]].reshape(mesh.nfull, nb) for b in g1)
+            bref = xi[q1, list(g1)].sum(axis=0)
+            for x in range(mesh.nfull):
+                kx = mesh.irr_of_full[x]
+                for g2 in {tuple(disp.degen_irr[kx][b]) for b in range(nb)}:
+                    assert abs(a[x, list(g2)].sum() - bref[x, list(g2)].sum()) <= 1e-9 * np.abs(bref).max()
+
+
+# --------------------------------------------------------------------- kappa
+def test_cubic_kappa_is_isotropic(oracle_setup):
+    from oracle import run as R
+    pre = oracle_setup("fcc_al", (4, 4, 4))
+    r = R.run(None, (4, 4, 4), 300.0, pre=pre)
+    for k in (r.kappa_sma, r.kappa_iter, r.kappa_offdiag):
+        assert np.allclose(k, k[0, 0] * np.eye(3), rtol=1e-10, atol=1e-30)
+    assert r.kappa_iter[0, 0] > r.kappa_sma[0, 0] > 0
+
+
+def test_iteration_converges_to_linear_solve_without_degeneracies(oracle_setup):
+    """With singleton degenerate sets the iteration's fixed point is the solution of
+    qs F + Xi Fold(F) = qs F0 (kappa.f90:380-468), solved here densely."""
+    from oracle import run as R
+    pre = oracle_setup("rocksalt", (3, 3, 3))
+    p, fc2, fc3, ops, mesh, disp = pre
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
+    nb = disp.nb
+    disp1 = P.Dispersions()
+    disp1.__dict__.update(disp.__dict__)
+    disp1.degen_irr = [[np.array([b]) for b in range(nb)] for _ in range(mesh.nirr)]
+    Fn, scf, hist = K.iterative_solution(p, mesh, disp1, ops, sr, 300.0, niter=400, tol=1e-24)
+    n = mesh.nirr * nb * 3
+
+    def apply(F):
+        F = F.reshape(mesh.nirr, nb, 3)
+        Fbb = np.zeros((mesh.nfull, nb, 3))
+        for q in range(mesh.nfull):
+            iop = mesh.op_of_full[q]
+            v = F[mesh.irr_of_full[q]] @ ops[abs(iop) - 1].m.T
+            Fbb[q] = v if iop > 0 else -v
+        out = np.zeros((mesh.nirr, nb, 3))
+        buf = sr.Xi @ Fbb.reshape(-1, 3)
+        for i, (q, b) in enumerate(sr.rows):
+            if sr.qs[q, b] >= U.FREQTOL:
+                out[q, b] = buf[i] / sr.qs[q, b]
+        return (F + out).reshape(-1)
+    A = np.stack([apply(e) for e in np.eye(n)], axis=1)
+    F = np.linalg.lstsq(A, sr.F0.reshape(-1), rcond=None)[0].reshape(mesh.nirr, nb, 3)
+    k_it = K.get_kappa(p, mesh, disp, ops, Fn, 300.0)
+    k_ls = K.get_kappa(p, mesh, disp, ops, F, 300.0)
+    assert np.allclose(k_it, k_ls, rtol=1e-8)
+
+
+def test_offdiag_velocities_equal_reference_kronecker_form(oracle_setup):
+    """kappa.f90:159-213 builds kronegv (nb^2 x nb^2) and multiplies it by the flattened
+    gradient; the oracle uses the equivalent similarity transform.  Literal form here."""
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (2, 2, 2))
+    nb = disp.nb
+    iq = 1
+    q = mesh.qcart[mesh.irr_full_index[iq]]
+    _, gD = P.dynamical_matrix(p, fc2, q)
+    mw = np.repeat(p.invsqrtmass, 3)
+    om = disp.om_irr[iq]
+    cm1 = np.zeros((nb * nb, 3), dtype=complex)
+    for a1 in range(p.na):
+        for a2 in range(p.na):
+            for ix in range(3):
+                for iy in range(3):
+                    ib, ia = 3 * a1 + ix, 3 * a2 + iy
+                    cm1[ia * nb + ib] = gD[:, ia, ib] / (p.invsqrtmass[a1] * p.invsqrtmass[a2])
+    kron = np.zeros((nb * nb, nb * nb), dtype=complex)
+    lg = mesh.littlegroup_irr[iq]
+    egws = []
+    for k in lg:
+        egw = P.eigenvector_transformation(p, q, ops[k]) @ disp.egv_irr[iq]
+        sc = np.where(om > U.FREQTOL, 1 / np.sqrt(np.where(om > 0, om, 1) * 2), 0.0)
+        egw = egw * mw[:, None] * sc[None, :]
+        egws.append(egw)
+        for i in range(nb):
+            for j in range(nb):
+                be = egw * np.conj(egw[j, i])
+                for ii in range(nb):
+                    for jj in range(nb):
+                        kron[i * nb + ii, j * nb + jj] += be[jj, ii]
+    kron /= len(lg)
+    cm2 = kron @ cm1
+    V = np.zeros((3, nb, nb), dtype=complex)
+    G = gD / (mw[:, None] * mw[None, :])[None]
+    for egw in egws:
+        for z in range(3):
+            V[z] += egw.conj().T @ G[z] @ egw
+    V /= len(lg)
+    # the little-group average cancels most entries: compare at the scale of one term
+    scale = max(np.abs(egw.conj().T @ G[z] @ egw).max() for egw in egws for z in range(3))
+    for i in range(nb):
+        for j in range(nb):
+            assert np.allclose(cm2[i * nb + j], V[:, i, j], atol=1e-12 * scale)
