@@ -139,6 +139,14 @@ int tc3_get_modes(const tc3_ctx *ctx, int iT, double *Fn, double *kappa_mode);
  * [3] BTE only, [4] off-diagonal only, [5] scatter kernel time (HIP events).  n <= 8. */
 int tc3_get_timings(const tc3_ctx *ctx, double *t, int n);
 
+/* Device time (ms, HIP events on the context's stream) and launch count of one kernel
+ * class in the last tc3_scatter / tc3_kappa call: 0 Phi + first two contractions,
+ * 1 last contraction + smeared-delta epilogue, 2 scattering-matrix symmetrisation,
+ * 3 BTE scattering-matrix product.  Also the triplet batch of the last scatter call.
+ * No counterpart in the reference (its timers are host wall-clock buckets). */
+int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches,
+                         int64_t *batch);
+
 /* Write outfile.thermal_conductivity_grid.hdf5 for temperature iT (layout of
  * lo_phonon_dispersions%write_to_hdf5, type_phonon_dispersions.f90:316-641, after the
  * Fn*omega/T prefactor of main.f90:323-328). */

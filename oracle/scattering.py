@@ -116,15 +116,9 @@ class ScatteringRates:
     pass
 
 
-def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
-                     integrationtype=2, sigma=1.0, rows_only=None):
-    """lo_scattering_rates%generate (scattering.f90:58-358), three-phonon only,
-    qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q)."""
-    nb = disp.nb
-    nirr, nfull = mesh.nirr, mesh.nfull
-    dims = mesh.dims
-    sr = ScatteringRates()
-    # occupations and adaptive smearing (:143-167)
+def occupations_and_smearing(p, mesh, disp, temperature, classical=False):
+    """Bose-Einstein occupations and adaptive smearing widths (scattering.f90:131-167)."""
+    nb, nirr, dims = disp.nb, mesh.nirr, mesh.dims
     be = np.zeros((nirr, nb))
     sigsq = np.zeros((nirr, nb))
     sigavg = np.zeros(nb)
@@ -143,6 +137,17 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
             sigsq[q, b] = s
             sigavg[b] += s * mesh.weight_irr[q]
     sigsq += np.max(sigavg) * 1e-4
+    return be, sigsq, sigavg
+
+
+def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
+                     integrationtype=2, sigma=1.0, rows_only=None):
+    """lo_scattering_rates%generate (scattering.f90:58-358), three-phonon only,
+    qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q)."""
+    nb = disp.nb
+    nirr, nfull = mesh.nirr, mesh.nfull
+    sr = ScatteringRates()
+    be, sigsq, sigavg = occupations_and_smearing(p, mesh, disp, temperature, classical)
     sr.be, sr.sigsq = be, sigsq
     sr.sigma_fixed = sigma * U.FREQ_THZ_TO_HARTREE
     sr.thresh_sigma = np.max(sigavg) * 1e-4

@@ -395,6 +395,14 @@ int tc3_get_timings(const tc3_ctx *ctx, double *t, int n) {
     return 0;
 }
 
+int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches, int64_t *batch) {
+    if (!ctx || which < 0 || which > 3) return 5;
+    if (total_ms) *total_ms = ctx->kstat_ms[which];
+    if (launches) *launches = ctx->kstat_n[which];
+    if (batch) *batch = ctx->batch;
+    return 0;
+}
+
 int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path) {
     if (!ctx || !ctx->have_kappa || iT < 0 || iT >= ctx->nT || !path) return 5;
     std::string err;

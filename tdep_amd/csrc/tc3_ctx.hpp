@@ -60,6 +60,11 @@ struct tc3_ctx {
     std::vector<int> niter;
 
     double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // per-kernel-class device time of the last calls (HIP events): 0 Phi+U+W kernel,
+    // 1 Psi + smeared-delta epilogue kernel, 2 scattering-matrix symmetrisation, 3 BTE Xi.F
+    double kstat_ms[4] = {0, 0, 0, 0};
+    int kstat_n[4] = {0, 0, 0, 0};
+    long batch = 0;
 
     // device buffers
     double *d_om_irr = nullptr, *d_om_full = nullptr, *d_vel_irr = nullptr, *d_vel_full = nullptr;

@@ -17,9 +17,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "tc3_ctx.hpp"
+#include "tc3_mfma.hpp"
 
 namespace tc3 {
 
@@ -408,6 +410,54 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     hipEvent_t ev0, ev1;
     TRY(hipEventCreate(&ev0));
     TRY(hipEventCreate(&ev1));
+    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
+    if (precision == TC3_PREC_BF16X3 && !use_mfma) {
+        cleanup();
+        return c->fail(5, "split-BF16x3 needs a band count that is a multiple of 16");
+    }
+    std::vector<hipEvent_t> kev;
+    for (double &x : c->kstat_ms) x = 0;
+    for (int &x : c->kstat_n) x = 0;
+    if (use_mfma) {
+        // W of one triplet: 2 * nb^3 reals; size the batch so W stays within the
+        // last-level cache, with at least 64 triplets per launch
+        const size_t es = mfma_elem_size(precision);
+        const size_t wtrip = 2 * (size_t)nb3 * es;
+        long mb = 192l << 20;
+        if (const char *env = std::getenv("TC3_BATCH_MB")) mb = std::max(1l, std::atol(env)) << 20;
+        long mbatch = std::max<long>(64, mb / (long)wtrip);
+        mbatch = std::min<long>(mbatch, ntrip);
+        mbatch = std::min<long>(mbatch, (long)((size_t)8 << 30) / (long)wtrip);
+        mbatch = std::max<long>(mbatch, 1);
+        MfmaPlan mp;
+        mp.na = na; mp.nb = nb; mp.nirr = nirr; mp.nfull = nfull; mp.maxa1 = std::max(maxa1, 1);
+        mp.ntrip = ntrip; mp.batch = mbatch;
+        mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
+        TRY(dalloc(&mp.E1re, es * (size_t)nirr * nb * nb));
+        TRY(dalloc(&mp.E1im, es * (size_t)nirr * nb * nb));
+        TRY(dalloc(&mp.Efre, es * (size_t)nfull * nb * nb));
+        TRY(dalloc(&mp.Efim, es * (size_t)nfull * nb * nb));
+        TRY(dalloc(&mp.work, wtrip * mbatch));
+        mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off; mp.blk3 = d_blk3;
+        mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
+        MfmaEpiArgs &me = mp.epi;
+        me.nb = nb; me.nT = nT; me.nirr = nirr; me.integrationtype = integrationtype;
+        me.cb = mfma_choose_chunk(nb, nT, &mp.psi_shmem);
+        if (me.cb == 0) { cleanup(); return c->fail(5, "too many temperatures for one pass"); }
+        me.sigma_fixed = ea.sigma_fixed; me.psi_scale = ea.psi_scale;
+        me.tq1 = d_tq1; me.tq2 = d_tq2; me.tq3 = d_tq3; me.eq_off = d_eqoff; me.eq_q2 = d_eq2; me.eq_q3 = d_eq3;
+        me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
+        me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.be = d_be; me.sigsq = d_sigsq;
+        me.xi = c->d_xi; me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
+        const long nbatch = (ntrip + mbatch - 1) / mbatch;
+        kev.resize(3 * nbatch);
+        for (auto &x : kev) TRY(hipEventCreate(&x));
+        mp.ev = kev.data();
+        TRY(hipEventRecord(ev0, st));
+        int r = run_mfma_chain(mp, precision, st);
+        if (r) { cleanup(); return r; }
+        c->batch = mbatch;
+    } else {
     TRY(hipEventRecord(ev0, st));
     for (long t0 = 0; t0 < ntrip; t0 += batch) {
         const int nt = (int)std::min<long>(batch, ntrip - t0);
@@ -436,12 +486,19 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         }
         TRY(hipGetLastError());
     }
+    c->batch = batch;
+    }
+    hipEvent_t evs0, evs1;
+    TRY(hipEventCreate(&evs0));
+    TRY(hipEventCreate(&evs1));
     hipLaunchKernelGGL(k_gamma, dim3((nrows * nT + 255) / 256), dim3(256), 0, st, nrows, nT, nb, d_rowq, d_rowb,
                        d_q1off, d_gpart, d_lw);
     TRY(hipGetLastError());
+    TRY(hipEventRecord(evs0, st));
     hipLaunchKernelGGL(k_xi_sym, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
                        d_members, c->d_xi, c->xi_row, xi_T);
     TRY(hipGetLastError());
+    TRY(hipEventRecord(evs1, st));
     TRY(hipEventRecord(ev1, st));
     std::vector<double> lw((size_t)nrows * nT);
     TRY(hipMemcpyAsync(lw.data(), d_lw, sizeof(double) * nrows * nT, hipMemcpyDeviceToHost, st));
@@ -450,6 +507,21 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     hipEventElapsedTime(&ms, ev0, ev1);
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
+    for (size_t b = 0; b + 2 < kev.size(); b += 3) {
+        float m1 = 0, m2 = 0;
+        hipEventElapsedTime(&m1, kev[b], kev[b + 1]);
+        hipEventElapsedTime(&m2, kev[b + 1], kev[b + 2]);
+        c->kstat_ms[0] += m1; c->kstat_n[0] += 1;
+        c->kstat_ms[1] += m2; c->kstat_n[1] += 1;
+    }
+    for (auto &x : kev) hipEventDestroy(x);
+    {
+        float m3 = 0;
+        hipEventElapsedTime(&m3, evs0, evs1);
+        c->kstat_ms[2] += m3; c->kstat_n[2] += 1;
+    }
+    hipEventDestroy(evs0);
+    hipEventDestroy(evs1);
 #undef TRY
     cleanup();
     c->timings[5] = ms * 1e-3;

@@ -36,6 +36,8 @@ SIGNATURES = {
     "tc3_get_kappa_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_int_p]),
     "tc3_get_modes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p]),
     "tc3_get_timings": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int]),
+    "tc3_get_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_int_p,
+                                            ctypes.POINTER(ctypes.c_int64)]),
     "tc3_write_grid_hdf5": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]),
 }
 

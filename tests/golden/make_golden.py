@@ -17,7 +17,7 @@ from oracle import run as R  # noqa: E402
 from tdep_amd import inputs  # noqa: E402
 
 CASES = [("fcc_al", (6, 6, 6), (300.0,)), ("rocksalt", (4, 4, 4), (100.0, 300.0)),
-         ("sl16", (2, 2, 2), (300.0,))]
+         ("sl16", (4, 4, 2), (300.0,))]
 
 
 def main():

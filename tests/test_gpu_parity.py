@@ -28,20 +28,20 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4)), ("sl16", (2, 2, 2))])
+@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2))])
 def test_harmonic_matches_oracle(inputs_dir, oracle_setup, name, grid):
     p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
     e = _engine(inputs_dir, name, grid)
     om, vel, egv = e.harmonic()
     assert np.abs(om - disp.om_irr).max() < 1e-10 * np.abs(disp.om_irr).max()
-    assert np.abs(vel - disp.vel_irr).max() < 1e-9 * np.abs(disp.vel_irr).max()
+    assert np.abs(vel - disp.vel_irr).max() < 1e-9 * np.abs(disp.vel_irr).max() + 1e-12
     assert np.abs(egv - disp.egv_irr).max() < 1e-9
     omf, velf, egvf = e.harmonic(full=True)
     assert np.abs(egvf - disp.egv_full).max() < 1e-9
-    assert np.abs(velf - disp.vel_full).max() < 1e-9 * np.abs(disp.vel_full).max()
+    assert np.abs(velf - disp.vel_full).max() < 1e-9 * np.abs(disp.vel_full).max() + 1e-12
 
 
-@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4)), ("sl16", (2, 2, 2))])
+@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2))])
 def test_fp64_scattering_and_kappa_match_golden(inputs_dir, name, grid):
     g = np.load(os.path.join(GOLDEN, f"oracle_{name}_{'x'.join(map(str, grid))}.npz"))
     e = _engine(inputs_dir, name, grid)
@@ -57,7 +57,7 @@ def test_fp64_scattering_and_kappa_match_golden(inputs_dir, name, grid):
         assert e.niter[iT] == int(g[f"niter_{iT}"])
 
 
-@pytest.mark.parametrize("name,grid,T", [("rocksalt", (4, 4, 4), 300.0), ("sl16", (2, 2, 2), 300.0)])
+@pytest.mark.parametrize("name,grid,T", [("rocksalt", (4, 4, 4), 300.0), ("sl16", (4, 4, 2), 300.0)])
 def test_fp64_scattering_matrix_matches_oracle(inputs_dir, oracle_setup, name, grid, T):
     from oracle import scattering as S
     p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)

@@ -337,3 +337,14 @@ def test_offdiag_velocities_equal_reference_kronecker_form(oracle_setup):
     for i in range(nb):
         for j in range(nb):
             assert np.allclose(cm2[i * nb + j], V[:, i, j], atol=1e-12 * scale)
+
+
+def test_c_port_of_reference_row_loop_matches_oracle(oracle_setup):
+    """oracle/scatter_ref.c follows the reference's zgeru/dot_product per-element loop;
+    its per-row linewidths (before degeneracy averaging) equal the oracle's."""
+    from oracle import ref_port
+    pre = oracle_setup("rocksalt", (3, 3, 3))
+    p, fc2, fc3, ops, mesh, disp = pre
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
+    gam, dt, nthr, nel = ref_port.run_rows(pre, 300.0, sr.rows)
+    assert np.allclose(gam, sr.gamma_row, rtol=1e-10, atol=1e-14 * np.abs(sr.gamma_row).max())
