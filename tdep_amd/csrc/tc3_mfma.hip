@@ -222,14 +222,20 @@ __device__ inline double gaussd(double x, double mu, double sigma) {
     return exp(-d * d * 0.5 * inv * inv) * 0.398942280401433 * inv;
 }
 
+// Staging/accumulation type of the epilogue: FP64 in FP64 mode, FP32 otherwise.
+template <int PREC> struct Stage { typedef float T; };
+template <> struct Stage<0> { typedef double T; };
+
 template <int PREC>
 __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Traits<PREC>::T *__restrict__ E1_re,
                                              const typename Traits<PREC>::T *__restrict__ E1_im,
                                              const typename Traits<PREC>::T *__restrict__ W) {
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
+    typedef typename Stage<PREC>::T S;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nb = a.nb, nT = a.nT, CB = a.cb;
+    const int nbp = nb + 1;                         // padded row stride of the staging arrays
     const int nbb = nb / 16;
     // XCD-aware order: the b1 blocks of one triplet share blockIdx % 8
     const int g = blockIdx.x;
@@ -244,13 +250,12 @@ __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Trait
     const int e0 = a.eq_off[t], e1 = a.eq_off[t + 1];
     const size_t nb2 = (size_t)nb * nb;
     // LDS layout
-    double *A = (double *)smem;                     // 16*CB*nb  psisq*d0 (masked)
-    double *B = A + 16 * CB * nb;                   // 16*CB*nb  psisq*d1
-    double *S3 = B + 16 * CB * nb;                  // nT*16*nb
-    double *tot = S3 + nT * 16 * nb;                // nT*16
-    double *s2 = tot + nT * 16;                     // 16*CB
-    double *n2t = s2 + 16 * CB;                     // nT*nb
-    double *n3t = n2t + nT * nb;                    // nT*nb
+    S *A = (S *)smem;                               // [16][CB][nbp]  psisq*d0 (masked)
+    S *B = A + 16 * CB * nbp;                       // [16][CB][nbp]  psisq*d1
+    S *S2 = B + 16 * CB * nbp;                      // [nT][16][nb]   sum over b3
+    S *S3 = S2 + nT * 16 * nb;                      // [nT][16][nb]   sum over b2
+    double *n2t = (double *)(((size_t)(S3 + nT * 16 * nb) + 15) & ~(size_t)15);  // [nT][nb]
+    double *n3t = n2t + nT * nb;                    // [nT][nb]
     double *w1 = n3t + nT * nb;                     // 16
     double *sg1 = w1 + 16;                          // 16
     int *row = (int *)(sg1 + 16);                   // 16
@@ -259,8 +264,7 @@ __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Trait
         n2t[x] = a.be[((size_t)T_ * a.nirr + i2) * nb + b];
         n3t[x] = a.be[((size_t)T_ * a.nirr + i3) * nb + b];
     }
-    for (int x = threadIdx.x; x < nT * 16 * nb; x += blockDim.x) S3[x] = 0.0;
-    for (int x = threadIdx.x; x < nT * 16; x += blockDim.x) tot[x] = 0.0;
+    for (int x = threadIdx.x; x < nT * 16 * nb; x += blockDim.x) S3[x] = 0;
     if (threadIdx.x < 16) {
         const int b1 = 16 * bb + threadIdx.x;
         w1[threadIdx.x] = a.om_irr[(size_t)q1 * nb + b1];
@@ -273,10 +277,9 @@ __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Trait
     const T *Wi = Wr + nb * nb2;
     const double *om2 = a.om_full + (size_t)q2 * nb, *om3 = a.om_full + (size_t)q3 * nb;
     const double *sgq2 = a.sigsq + (size_t)i2 * nb, *sgq3 = a.sigsq + (size_t)i3 * nb;
-    const int ncol = CB * nb;       // columns of one chunk
-    const int ntile = ncol / 16;
+    const int ntile = CB * nb / 16;
     for (int c0 = 0; c0 < nb; c0 += CB) {
-        // Psi[b1][(b2,b3)] for b2 in [c0, c0+CB): tiles of 16 columns
+        // Psi[b1][(b2,b3)] for b2 in [c0, c0+CB): tiles of 16 consecutive b3
         for (int tile = wave; tile < ntile; tile += 4) {
             const int n0 = c0 * nb + tile * 16;
             Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
@@ -289,12 +292,13 @@ __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Trait
                              br, bi);
                 cmma<PREC>(cr, ci, ar, ai, br, bi);
             }
+            const int n = n0 + (lane & 15);
+            const int b2 = n / nb, b3 = n % nb;
+            const double w2 = om2[b2], w3 = om3[b3];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = 4 * (lane >> 4) + r;          // local b1
-                const int n = n0 + (lane & 15);              // global column b2*nb + b3
-                const int b2 = n / nb, b3 = n % nb;
-                const double w1v = w1[m], w2 = om2[b2], w3 = om3[b3];
+                const double w1v = w1[m];
                 double va = 0.0, vb = 0.0;
                 if (row[m] >= 0 && !(w2 < 1e-9) && !(fabs(w1v - w2) < 1e-9) && !(w3 < 1e-9) &&
                     !(fabs(w1v - w3) < 1e-9) && !(fabs(w2 - w3) < 1e-9)) {
@@ -306,65 +310,68 @@ __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Trait
                     va = psisq * d0;
                     vb = psisq * d1;
                 }
-                const int loc = (m * CB + (b2 - c0)) * nb + b3;
-                A[loc] = va;
-                B[loc] = vb;
+                const int loc = (m * CB + (b2 - c0)) * nbp + b3;
+                A[loc] = (S)va;
+                B[loc] = (S)vb;
             }
         }
         __syncthreads();
-        for (int T_ = 0; T_ < nT; ++T_) {
-            const double *n2 = n2t + T_ * nb, *n3 = n3t + T_ * nb;
-            // S2[b1][b2] = sum_b3 f  (thread per (b1, b2) of the chunk)
-            for (int x = threadIdx.x; x < 16 * CB; x += blockDim.x) {
-                const int b2 = c0 + x % CB;
-                const double *pa = A + (size_t)x * nb, *pb = B + (size_t)x * nb;
-                double s = 0;
-                for (int b3 = 0; b3 < nb; ++b3)
-                    s += (n2[b2] - n3[b3]) * pa[b3] + (n2[b2] + n3[b3] + 1.0) * pb[b3];
-                s2[x] = s;
-            }
-            // S3[b1][b3] += sum_{b2 in chunk} f  (thread per (b1, b3))
-            for (int x = threadIdx.x; x < 16 * nb; x += blockDim.x) {
-                const int m = x / nb, b3 = x % nb;
-                double s = 0;
-                for (int j = 0; j < CB; ++j) {
-                    const int b2 = c0 + j;
-                    const int loc = (m * CB + j) * nb + b3;
-                    s += (n2[b2] - n3[b3]) * A[loc] + (n2[b2] + n3[b3] + 1.0) * B[loc];
-                }
-                S3[(size_t)T_ * 16 * nb + x] += s;
-            }
-            __syncthreads();
-            if (threadIdx.x < 16) {
-                double s = 0;
-                for (int j = 0; j < CB; ++j) s += s2[threadIdx.x * CB + j];
-                tot[T_ * 16 + threadIdx.x] += s;
-            }
-            // scattering-matrix contributions of this chunk's b2 (scattering_threephonon.f90:133)
-            for (int x = threadIdx.x; x < 16 * CB; x += blockDim.x) {
-                const int m = x / CB, b2 = c0 + x % CB;
-                const int r = row[m];
-                if (r < 0) continue;
-                double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row;
-                const double v = 2.0 * s2[x] * om2[b2] / w1[m];
-                for (int e = e0; e < e1; ++e) xr[(size_t)a.eq_q2[e] * nb + b2] += v;
-            }
-            __syncthreads();
+        // S2[T][b1][b2] = sum_b3 f  (work item per (T, b1, b2 of the chunk))
+        for (int x = threadIdx.x; x < nT * 16 * CB; x += blockDim.x) {
+            const int T_ = x / (16 * CB), mj = x % (16 * CB);
+            const int m = mj / CB, j = mj % CB, b2 = c0 + j;
+            const double *n3 = n3t + T_ * nb;
+            const double n2 = n2t[T_ * nb + b2];
+            const S *pa = A + (size_t)mj * nbp, *pb = B + (size_t)mj * nbp;
+            S s = 0;
+            for (int b3 = 0; b3 < nb; ++b3)
+                s += (S)(n2 - n3[b3]) * pa[b3] + (S)(n2 + n3[b3] + 1.0) * pb[b3];
+            S2[(T_ * 16 + m) * nb + b2] = s;
         }
+        // S3[T][b1][b3] += sum_{b2 in chunk} f  (work item per (T, b1, b3))
+        for (int x = threadIdx.x; x < nT * 16 * nb; x += blockDim.x) {
+            const int T_ = x / (16 * nb), mb = x % (16 * nb);
+            const int m = mb / nb, b3 = mb % nb;
+            const double *n2 = n2t + T_ * nb;
+            const double n3 = n3t[T_ * nb + b3];
+            S s = 0;
+            for (int j = 0; j < CB; ++j) {
+                const int loc = (m * CB + j) * nbp + b3;
+                s += (S)(n2[c0 + j] - n3) * A[loc] + (S)(n2[c0 + j] + n3 + 1.0) * B[loc];
+            }
+            S3[x] += s;
+        }
+        __syncthreads();
     }
-    // (:134) and the linewidth partial of every row of the block
-    for (int T_ = 0; T_ < nT; ++T_) {
-        for (int x = threadIdx.x; x < 16 * nb; x += blockDim.x) {
-            const int m = x / nb, b3 = x % nb;
-            const int r = row[m];
-            if (r < 0) continue;
-            double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row;
-            const double v = 2.0 * S3[(size_t)T_ * 16 * nb + x] * om3[b3] / w1[m];
-            for (int e = e0; e < e1; ++e) xr[(size_t)a.eq_q3[e] * nb + b3] += v;
+    // linewidth partials (g0 += f0 + f1 per equivalent triplet, :126-128)
+    for (int x = threadIdx.x; x < nT * 16; x += blockDim.x) {
+        const int T_ = x / 16, m = x % 16;
+        double s = 0;
+        for (int b2 = 0; b2 < nb; ++b2) s += (double)S2[(T_ * 16 + m) * nb + b2];
+        a.gpart[((size_t)t * nT + T_) * nb + 16 * bb + m] = (row[m] >= 0) ? s * (double)(e1 - e0) : 0.0;
+    }
+    // scattering-matrix entries of this (triplet, b1 block) (:133-134).  Every target
+    // (row, q', band) belongs to exactly one triplet, and inside a triplet q' appears in
+    // one equivalent pair only, so the values are stored, not accumulated.
+    const int neq = e1 - e0;
+    for (long x = threadIdx.x; x < (long)nT * 16 * neq * nb; x += blockDim.x) {
+        const int b = (int)(x % nb);
+        long y = x / nb;
+        const int e = e0 + (int)(y % neq);
+        y /= neq;
+        const int m = (int)(y % 16), T_ = (int)(y / 16);
+        const int r = row[m];
+        if (r < 0) continue;
+        double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row;
+        const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
+        const double v2 = 2.0 * (double)S2[(T_ * 16 + m) * nb + b] * om2[b] / w1[m];
+        const double v3 = 2.0 * (double)S3[(T_ * 16 + m) * nb + b] * om3[b] / w1[m];
+        if (x2 == x3) {
+            xr[(size_t)x2 * nb + b] = v2 + v3;
+        } else {
+            xr[(size_t)x2 * nb + b] = v2;
+            xr[(size_t)x3 * nb + b] = v3;
         }
-        if (threadIdx.x < 16)
-            a.gpart[((size_t)t * nT + T_) * nb + 16 * bb + threadIdx.x] =
-                (row[threadIdx.x] >= 0) ? tot[T_ * 16 + threadIdx.x] * (double)(e1 - e0) : 0.0;
     }
 }
 
@@ -426,16 +433,20 @@ bool mfma_supported(int nb, int precision) {
     return 4 * (size_t)nb * nb * es + 16 * 4096 <= 160 * 1024;
 }
 
-int mfma_choose_chunk(int nb, int nT, size_t *shmem) {
-    const size_t fixed = sizeof(double) * ((size_t)nT * 16 * nb + nT * 16 + 2 * (size_t)nT * nb + 32) + 16 * sizeof(int);
-    for (int cb : {16, 8, 4, 2, 1}) {
-        if (nb % cb) continue;
-        const size_t s = fixed + sizeof(double) * (2 * 16 * (size_t)cb * nb + 16 * cb);
-        if (s <= 150 * 1024) {
-            *shmem = s;
-            return cb;
+int mfma_choose_chunk(int nb, int nT, int precision, size_t *shmem) {
+    const size_t es = (precision == TC3_PREC_FP64) ? sizeof(double) : sizeof(float);
+    const size_t fixed = es * (2 * (size_t)nT * 16 * nb) + 16 +
+                         sizeof(double) * (2 * (size_t)nT * nb + 32) + 16 * sizeof(int);
+    // largest chunk of b2 that keeps the workgroup within 80 KB (two per CU), else 150 KB
+    for (size_t cap : {(size_t)80 * 1024, (size_t)150 * 1024})
+        for (int cb : {16, 8, 4, 2, 1}) {
+            if (nb % cb) continue;
+            const size_t s = fixed + es * (2 * 16 * (size_t)cb * (nb + 1));
+            if (s <= cap) {
+                *shmem = s;
+                return cb;
+            }
         }
-    }
     return 0;
 }
 

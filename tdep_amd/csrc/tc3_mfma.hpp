@@ -31,7 +31,7 @@ struct MfmaPlan {
 
 size_t mfma_elem_size(int precision);
 bool mfma_supported(int nb, int precision);
-int mfma_choose_chunk(int nb, int nT, size_t *shmem);
+int mfma_choose_chunk(int nb, int nT, int precision, size_t *shmem);
 int run_mfma_chain(const MfmaPlan &p, int precision, hipStream_t st);
 
 }  // namespace tc3

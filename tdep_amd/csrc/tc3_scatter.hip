@@ -210,12 +210,20 @@ __global__ void k_epilogue(EpiArgs a, const P *__restrict__ psi, int psi_complex
             for (int b = 0; b < nb; ++b) tot += s2[b];
             a.gpart[((size_t)t * a.nT + T) * nb + b1] = tot * (double)(e1 - e0);
         }
+        // every (row, q', band) target belongs to one triplet and, inside it, to one
+        // equivalent pair: store the values (q2' == q3' receives both terms)
         double *xr = a.xi + (size_t)T * a.xi_T + (size_t)row * a.xi_row;
         for (int e = e0; e < e1; ++e) {
             const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
             for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-                xr[(size_t)x2 * nb + b] += 2.0 * s2[b] * a.om_full[(size_t)q2 * nb + b] / w1;
-                xr[(size_t)x3 * nb + b] += 2.0 * s3[b] * a.om_full[(size_t)q3 * nb + b] / w1;
+                const double v2 = 2.0 * s2[b] * a.om_full[(size_t)q2 * nb + b] / w1;
+                const double v3 = 2.0 * s3[b] * a.om_full[(size_t)q3 * nb + b] / w1;
+                if (x2 == x3) {
+                    xr[(size_t)x2 * nb + b] = v2 + v3;
+                } else {
+                    xr[(size_t)x2 * nb + b] = v2;
+                    xr[(size_t)x3 * nb + b] = v3;
+                }
             }
         }
         __syncthreads();
@@ -410,7 +418,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     hipEvent_t ev0, ev1;
     TRY(hipEventCreate(&ev0));
     TRY(hipEventCreate(&ev1));
-    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
+    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr &&
+                          (precision != TC3_PREC_FP64 || std::getenv("TC3_FP64_MFMA") != nullptr);
     if (precision == TC3_PREC_BF16X3 && !use_mfma) {
         cleanup();
         return c->fail(5, "split-BF16x3 needs a band count that is a multiple of 16");
@@ -442,7 +451,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
         MfmaEpiArgs &me = mp.epi;
         me.nb = nb; me.nT = nT; me.nirr = nirr; me.integrationtype = integrationtype;
-        me.cb = mfma_choose_chunk(nb, nT, &mp.psi_shmem);
+        me.cb = mfma_choose_chunk(nb, nT, precision, &mp.psi_shmem);
         if (me.cb == 0) { cleanup(); return c->fail(5, "too many temperatures for one pass"); }
         me.sigma_fixed = ea.sigma_fixed; me.psi_scale = ea.psi_scale;
         me.tq1 = d_tq1; me.tq2 = d_tq2; me.tq3 = d_tq3; me.eq_off = d_eqoff; me.eq_q2 = d_eq2; me.eq_q3 = d_eq3;
@@ -459,8 +468,12 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         c->batch = mbatch;
     } else {
     TRY(hipEventRecord(ev0, st));
+    kev.resize(3 * ((ntrip + batch - 1) / batch));
+    for (auto &x : kev) TRY(hipEventCreate(&x));
     for (long t0 = 0; t0 < ntrip; t0 += batch) {
         const int nt = (int)std::min<long>(batch, ntrip - t0);
+        hipEvent_t *ev = &kev[3 * (t0 / batch)];
+        TRY(hipEventRecord(ev[0], st));
         ea.t0 = (int)t0;
         if (precision == TC3_PREC_FP64) {
             auto *w = (cplx<double> *)d_work;
@@ -469,6 +482,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             launch_chain<double>(c, st, (int)t0, nt, (const cplx<double> *)d_E1, (const cplx<double> *)d_Ef, phi, t1,
                                  t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2, d_lv3, d_mwm,
                                  phi_shmem);
+            TRY(hipEventRecord(ev[1], st));
             hipLaunchKernelGGL(k_epilogue<double>, dim3(nt * nb), dim3(kBlock), epi_shmem, st, ea,
                                (const double *)psi, 0);
         } else if (precision == TC3_PREC_FP32) {
@@ -478,12 +492,14 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             launch_chain<float>(c, st, (int)t0, nt, (const cplx<float> *)d_E1, (const cplx<float> *)d_Ef, phi, t1,
                                 t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2, d_lv3, d_mwm,
                                 phi_shmem);
+            TRY(hipEventRecord(ev[1], st));
             hipLaunchKernelGGL(k_epilogue<float>, dim3(nt * nb), dim3(kBlock), epi_shmem, st, ea,
                                (const float *)psi, 0);
         } else {
             cleanup();
             return c->fail(5, "precision mode not available in this build");
         }
+        TRY(hipEventRecord(ev[2], st));
         TRY(hipGetLastError());
     }
     c->batch = batch;
