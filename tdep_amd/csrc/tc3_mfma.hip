@@ -19,7 +19,8 @@
 //           a.b ~ ah.bh + ah.bl + al.bh, FP32 accumulation
 // Lane layouts (16x16 output tile, 64 lanes): A[m = l%16][k], B[k][n = l%16] with
 // k = k0 + l/16 (K-step 4) or k = k0 + 8*(l/16) + j (K-step 32, j < 8), and
-// D[m = 4*(l/16) + r][n = l%16], r < 4.
+// D[m][n = l%16] in register r < 4 with m = 4*(l/16) + r (FP32 accumulation) or
+// m = 4*r + l/16 (FP64); both checked on the device by an in-bounds probe.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,6 +63,14 @@ __device__ inline void mma<2>(v4f &c, const Frag<2> &a, const Frag<2> &b) {
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.l, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.l, b.h, c, 0, 0, 0);
+}
+
+// Output row of register r in lane l: 4*(l/16) + r for the FP32-accumulating
+// instructions, 4*r + l/16 for v_mfma_f64_16x16x4_f64.
+template <int PREC>
+__device__ inline int out_row(int lane, int r) {
+    if constexpr (PREC == 0) return 4 * r + (lane >> 4);
+    else return 4 * (lane >> 4) + r;
 }
 
 template <int PREC>
@@ -152,7 +161,7 @@ __device__ inline void cgemm_nb(int nb, FA fa, FB fb, FC fc) {
             cmma<PREC>(cr, ci, ar, ai, br, bi);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) fc(m0 + 4 * (lane >> 4) + r, n0 + (lane & 15), cr[r], ci[r]);
+        for (int r = 0; r < 4; ++r) fc(m0 + out_row<PREC>(lane, r), n0 + (lane & 15), cr[r], ci[r]);
     }
 }
 
@@ -297,7 +306,7 @@ __global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Trait
             const double w2 = om2[b2], w3 = om3[b3];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = 4 * (lane >> 4) + r;          // local b1
+                const int m = out_row<PREC>(lane, r);       // local b1
                 const double w1v = w1[m];
                 double va = 0.0, vb = 0.0;
                 if (row[m] >= 0 && !(w2 < 1e-9) && !(fabs(w1v - w2) < 1e-9) && !(w3 < 1e-9) &&

@@ -418,8 +418,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     hipEvent_t ev0, ev1;
     TRY(hipEventCreate(&ev0));
     TRY(hipEventCreate(&ev1));
-    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr &&
-                          (precision != TC3_PREC_FP64 || std::getenv("TC3_FP64_MFMA") != nullptr);
+    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
     if (precision == TC3_PREC_BF16X3 && !use_mfma) {
         cleanup();
         return c->fail(5, "split-BF16x3 needs a band count that is a multiple of 16");
