@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the timed mixed-precision pass (for profiler runs); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: the scattering engine has no multi-GPU path "
@@ -111,6 +113,8 @@ def main():
         ks_mx = kernel_stats(eng)
         tk = eng.timings()
         log(f"[bench] {args.precision}: {t_mx * 1e3:.2f} ms/step  kernels {ks_mx}")
+        if args.quick:
+            return
         # FP64 GPU form of the same pass
         t_64 = scatter_pass(eng, temps, "fp64", max(1, args.steps // 2), 1)
         ks_64 = kernel_stats(eng)
