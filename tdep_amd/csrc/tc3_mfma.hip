@@ -1,26 +1,36 @@
-// |Psi|^2 contraction on MFMA for band counts that are multiples of 16.
+// |Psi|^2 contraction on MFMA for band counts nb in {16, 32, 48, 64, 96}.
 //
-// Two kernels per batch of triplets (t = q1, q2, q3):
+// Per batch of visited triplets t = (q1, q2, q3) two kernels run:
 //
-//   k_uw   workgroup per (t, ia):  builds the row Phi(q2,q3)[ia][ib][ic] in LDS from the
-//          third-order IFC list (pretransform_phi3, scattering_threephonon.f90:275-317),
-//          then U = Phi_ia E3 and W_ia = E2^T U on MFMA; writes W[t][ia][b2][b3].
-//   k_psi  workgroup per (t, 16 values of b1): Psi[b1][b2][b3] = sum_ia E1[ia][b1] W[ia][b2][b3]
-//          on MFMA, chunk by chunk of b2, fused with the smeared energy conservation and
-//          occupations of every temperature (get_dirac :219-271, :113-123) and the
-//          linewidth / scattering-matrix accumulation (:126-135).  Reductions are done in
-//          a fixed order in LDS, so results do not depend on scheduling.
+//   k_uw   workgroup per (t, ia).  Phi(q2,q3) is block sparse in the atoms (a1, a2, a3):
+//          only the (a2, a3) blocks of a1 = ia/3 that hold IFCs are summed
+//          (pretransform_phi3, scattering_threephonon.f90:275-317).  U[ib][b3] =
+//          sum_ic Phi[ia][ib][ic] E3[ic][b3] is formed block by block on the vector ALUs,
+//          W[b2][b3] = sum_ib E2[ib][b2] U[ib][b3] on MFMA, and W[t][ia][b2][b3] is stored.
+//   k_psi  workgroup per (t, 16 values of b1) with one wave per 16 values of b3.  For each
+//          b2 a wave forms the tile Psi^T[(b2, b3)][b1] = sum_ia W[ia][b2][b3] E1[ia][b1]
+//          on MFMA (:88-113) and weights |Psi|^2 with the smeared energy conservation
+//          (get_dirac :219-271).  With P = psisq (d0 + d1) and M = psisq (d1 - d0) the
+//          per-temperature rate f0 + f1 of :116-123 is
+//              f_T = P (n2_T + 1/2) + M (n3_T + 1/2),
+//          so the occupations enter only through the two sums taken afterwards:
+//            * over b3 (linewidth, q2' entries of the scattering matrix, :126-133): one
+//              more MFMA per tile whose output columns are the temperatures;
+//            * over b2 (q3' entries, :134): per-lane registers, since a wave keeps its b3
+//              range for the whole triplet.
+//          Cross-wave sums go through LDS in a fixed order, so results do not depend on
+//          scheduling.
 //
-// Complex products are formed from four real MFMA products (re = ar br - ai bi,
-// im = ar bi + ai br).  Precision modes:
-//   FP64    v_mfma_f64_16x16x4 on FP64 operands
-//   FP32    v_mfma_f32_16x16x4 on FP32 operands, FP32 accumulation
-//   BF16X3  v_mfma_f32_16x16x32_bf16 with every FP32 operand split as hi + lo (two bf16);
-//           a.b ~ ah.bh + ah.bl + al.bh, FP32 accumulation
-// Lane layouts (16x16 output tile, 64 lanes): A[m = l%16][k], B[k][n = l%16] with
-// k = k0 + l/16 (K-step 4) or k = k0 + 8*(l/16) + j (K-step 32, j < 8), and
-// D[m][n = l%16] in register r < 4 with m = 4*(l/16) + r (FP32 accumulation) or
-// m = 4*r + l/16 (FP64); both checked on the device by an in-bounds probe.
+// Complex products are four real products (re = ar br - ai bi, im = ar bi + ai br).
+// Precision modes:
+//   FP64    v_mfma_f64_16x16x4_f64, FP64 epilogue
+//   FP32    v_mfma_f32_16x16x4_f32, FP32 epilogue
+//   BF16X3  v_mfma_f32_16x16x32_bf16 with every FP32 operand split as hi + lo (two
+//           bf16), a.b ~ ah.bh + ah.bl + al.bh, FP32 accumulation and epilogue.
+// Lane layouts (16x16 output tile, 64 lanes, g = lane/16, c = lane%16): A[m = c][k],
+// B[k][n = c] with k = k0 + g (K-step 4) or k = k0 + 8g + j, j < 8 (K-step 32); D[m][n = c]
+// in register r < 4 with m = 4g + r (FP32 accumulation) or m = 4r + g (FP64).  Checked on
+// the device by an in-bounds probe.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,433 +47,656 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
+constexpr int kPlanes = 6;   // split planes: re hi, re lo, im hi, im lo, -im hi, -im lo
+constexpr int kChunk = 8;    // b2 values between cross-wave reductions in k_psi
+constexpr double kTol = 1e-9;
+
+constexpr int kpad(int nb) { return (nb + 31) / 32 * 32; }
+
 template <int PREC> struct Traits;
-template <> struct Traits<0> { typedef double T; typedef v4d Acc; static constexpr int KS = 4; };
-template <> struct Traits<1> { typedef float T; typedef v4f Acc; static constexpr int KS = 4; };
-template <> struct Traits<2> { typedef float T; typedef v4f Acc; static constexpr int KS = 32; };
+template <> struct Traits<0> { typedef double T; typedef v4d Acc; };
+template <> struct Traits<1> { typedef float T; typedef v4f Acc; };
+template <> struct Traits<2> { typedef float T; typedef v4f Acc; };
 
-// Fragment of one real operand for one K-step.
-template <int PREC> struct Frag;
-template <> struct Frag<0> { double v; };
-template <> struct Frag<1> { float v; };
-template <> struct Frag<2> { v8bf h, l; };
+// K-steps of a contraction over nb: 4 per step (FP64, FP32) or 32, zero padded (BF16).
+template <int PREC, int NB> struct KSteps { static constexpr int n = NB / 4; };
+template <int NB> struct KSteps<2, NB> { static constexpr int n = (NB + 31) / 32; };
 
-template <int PREC>
-__device__ inline void mma(typename Traits<PREC>::Acc &c, const Frag<PREC> &a, const Frag<PREC> &b);
-template <>
-__device__ inline void mma<0>(v4d &c, const Frag<0> &a, const Frag<0> &b) {
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v, b.v, c, 0, 0, 0);
+// Epilogue type: FP64 in FP64 mode, FP32 otherwise.
+template <int PREC> struct Stage { typedef float T; };
+template <> struct Stage<0> { typedef double T; };
+
+struct Split { v8bf h, l; };
+
+__device__ inline void split8(const float *x, Split &s) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const __bf16 h = (__bf16)x[j];
+        s.h[j] = h;
+        s.l[j] = (__bf16)(x[j] - (float)h);
+    }
 }
-template <>
-__device__ inline void mma<1>(v4f &c, const Frag<1> &a, const Frag<1> &b) {
-    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v, b.v, c, 0, 0, 0);
-}
-template <>
-__device__ inline void mma<2>(v4f &c, const Frag<2> &a, const Frag<2> &b) {
+
+__device__ inline v8bf ld8(const __bf16 *p) { return *(const v8bf *)p; }
+
+__device__ inline void mma3(v4f &c, const Split &a, const Split &b) {
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.l, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.l, b.h, c, 0, 0, 0);
 }
+__device__ inline void mma1(v4f &c, float a, float b) { c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline void mma1(v4d &c, double a, double b) { c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-// Output row of register r in lane l: 4*(l/16) + r for the FP32-accumulating
-// instructions, 4*r + l/16 for v_mfma_f64_16x16x4_f64.
 template <int PREC>
 __device__ inline int out_row(int lane, int r) {
     if constexpr (PREC == 0) return 4 * r + (lane >> 4);
     else return 4 * (lane >> 4) + r;
 }
 
-template <int PREC>
-__device__ inline Frag<PREC> neg(const Frag<PREC> &a) {
-    Frag<PREC> r;
-    if constexpr (PREC == 2) {
-        r.h = -a.h;
-        r.l = -a.l;
-    } else {
-        r.v = -a.v;
-    }
-    return r;
-}
+__device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Load the A fragment of a K-step: element (m, k) from accessor fa(m, k) -> (re, im).
-template <int PREC, typename F>
-__device__ inline void load_a(int lane, int m0, int k0, int K, F fa, Frag<PREC> &re, Frag<PREC> &im) {
-    const int m = m0 + (lane & 15);
-    if constexpr (PREC == 2) {
-        const int kb = k0 + 8 * (lane >> 4);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            float x = 0, y = 0;
-            if (kb + j < K) fa(m, kb + j, x, y);
-            const __bf16 xh = (__bf16)x, yh = (__bf16)y;
-            re.h[j] = xh;
-            re.l[j] = (__bf16)(x - (float)xh);
-            im.h[j] = yh;
-            im.l[j] = (__bf16)(y - (float)yh);
+// e[q][component][mode] / sqrt(omega), transposed to [q][plane][mode][k] and split into
+// bf16 hi/lo planes of re, im and -im; k padded with zeros to a multiple of 32.
+__global__ void k_split_t(long nq, int nb, int kp, const cd *__restrict__ egv, const double *__restrict__ om,
+                          __bf16 *__restrict__ et) {
+    const long n = nq * nb * kp;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % kp);
+        const long qm = i / kp;
+        const int mode = (int)(qm % nb);
+        const long q = qm / nb;
+        float re = 0.f, im = 0.f;
+        if (k < nb) {
+            const double w = om[q * nb + mode];
+            const double s = (w < kTol) ? 0.0 : 1.0 / sqrt(w);
+            const cd e = egv[(q * nb + k) * nb + mode];
+            re = (float)(e.x * s);
+            im = (float)(e.y * s);
         }
-    } else {
-        const int k = k0 + (lane >> 4);
-        typename Traits<PREC>::T x = 0, y = 0;
-        if (k < K) fa(m, k, x, y);
-        re.v = x;
-        im.v = y;
+        const __bf16 rh = (__bf16)re, ih = (__bf16)im;
+        const __bf16 rl = (__bf16)(re - (float)rh), il = (__bf16)(im - (float)ih);
+        const size_t pstride = (size_t)nb * kp;
+        __bf16 *o = et + (size_t)q * kPlanes * pstride + (size_t)mode * kp + k;
+        o[0] = rh;
+        o[pstride] = rl;
+        o[2 * pstride] = ih;
+        o[3 * pstride] = il;
+        o[4 * pstride] = -ih;
+        o[5 * pstride] = -il;
     }
 }
 
-template <int PREC, typename F>
-__device__ inline void load_b(int lane, int k0, int n0, int K, F fb, Frag<PREC> &re, Frag<PREC> &im) {
-    const int n = n0 + (lane & 15);
-    if constexpr (PREC == 2) {
-        const int kb = k0 + 8 * (lane >> 4);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            float x = 0, y = 0;
-            if (kb + j < K) fb(kb + j, n, x, y);
-            const __bf16 xh = (__bf16)x, yh = (__bf16)y;
-            re.h[j] = xh;
-            re.l[j] = (__bf16)(x - (float)xh);
-            im.h[j] = yh;
-            im.l[j] = (__bf16)(y - (float)yh);
-        }
-    } else {
-        const int k = k0 + (lane >> 4);
-        typename Traits<PREC>::T x = 0, y = 0;
-        if (k < K) fb(k, n, x, y);
-        re.v = x;
-        im.v = y;
+// e[q][component][mode] / sqrt(omega) as planar re / im in the contraction precision.
+template <typename T>
+__global__ void k_scale_planar(long n, int nb, const cd *__restrict__ egv, const double *__restrict__ om,
+                               T *__restrict__ re, T *__restrict__ im) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long q = i / ((long)nb * nb);
+        const int b = (int)(i % nb);
+        const double w = om[q * nb + b];
+        const double s = (w < kTol) ? 0.0 : 1.0 / sqrt(w);
+        re[i] = (T)(egv[i].x * s);
+        im[i] = (T)(egv[i].y * s);
     }
 }
 
-// acc_re += ar br - ai bi ; acc_im += ar bi + ai br
-template <int PREC>
-__device__ inline void cmma(typename Traits<PREC>::Acc &cr, typename Traits<PREC>::Acc &ci, const Frag<PREC> &ar,
-                            const Frag<PREC> &ai, const Frag<PREC> &br, const Frag<PREC> &bi) {
-    mma<PREC>(cr, ar, br);
-    mma<PREC>(cr, neg<PREC>(ai), bi);
-    mma<PREC>(ci, ar, bi);
-    mma<PREC>(ci, ai, br);
-}
+struct UwArgs {
+    int t0, maxblk;
+    const int *tq2, *tq3;
+    const double *qcart, *lv2, *lv3, *mwm;
+    const int *a1_off;      // IFC entries of atom a1: [a1_off[a1], a1_off[a1 + 1])
+    const int4 *blist;      // (a2, a3, first entry, end entry) of every nonzero block, by (a1, a2, a3)
+    const int *a1_boff;     // blocks of a1: [a1_boff[a1], a1_boff[a1 + 1])
+    const int *a12_boff;    // blocks of (a1, a2): [a12_boff[a1 na + a2], a12_boff[a1 na + a2 + 1])
+    const void *Ef_re, *Ef_im;
+    const __bf16 *Et;
+    void *W;                // [tl][re/im][ia][b2 nb + b3]
+};
 
-// C(nb x nb) = A(nb x nb) B(nb x nb), complex, 16x16 tiles spread over the 4 waves.
-// fa/fb are element accessors, fc(m, n, re, im) stores one result.
-template <int PREC, typename FA, typename FB, typename FC>
-__device__ inline void cgemm_nb(int nb, FA fa, FB fb, FC fc) {
-    typedef typename Traits<PREC>::Acc Acc;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nt = nb / 16;
-    for (int tile = wave; tile < nt * nt; tile += 4) {
-        const int m0 = (tile / nt) * 16, n0 = (tile % nt) * 16;
-        Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-        for (int k0 = 0; k0 < nb; k0 += Traits<PREC>::KS) {
-            Frag<PREC> ar, ai, br, bi;
-            load_a<PREC>(lane, m0, k0, nb, fa, ar, ai);
-            load_b<PREC>(lane, k0, n0, nb, fb, br, bi);
-            cmma<PREC>(cr, ci, ar, ai, br, bi);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) fc(m0 + out_row<PREC>(lane, r), n0 + (lane & 15), cr[r], ci[r]);
-    }
-}
-
-template <int PREC>
-__global__ void __launch_bounds__(256) k_uw(int na, int t0, const int *__restrict__ tq2, const int *__restrict__ tq3,
-                                            const double *__restrict__ qcart, const int *__restrict__ a1_off,
-                                            const int *__restrict__ blk_off, const double *__restrict__ lv2,
-                                            const double *__restrict__ lv3, const double *__restrict__ mwm,
-                                            const typename Traits<PREC>::T *__restrict__ Ef_re,
-                                            const typename Traits<PREC>::T *__restrict__ Ef_im,
-                                            typename Traits<PREC>::T *__restrict__ W) {
+template <int PREC, int NB>
+size_t uw_lds(int maxblk, int maxa1) {
     typedef typename Traits<PREC>::T T;
+    constexpr int RS = kpad(NB) + 8;
+    const size_t ub = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB * NB;
+    size_t s = (ub + 15) & ~(size_t)15;
+    s += sizeof(cd) * 9 * (size_t)maxblk + sizeof(int) * (size_t)maxblk;
+    s = (s + 15) & ~(size_t)15;
+    return s + sizeof(cd) * (size_t)std::max(maxa1, 1);
+}
+
+template <int PREC, int NB>
+__global__ void __launch_bounds__(256) k_uw(UwArgs u) {
+    typedef typename Traits<PREC>::T T;
+    typedef typename Traits<PREC>::Acc Acc;
+    constexpr int NA = NB / 3, NB2 = NB * NB, NT = NB / 16;
+    constexpr int KP = kpad(NB), RS = KP + 8, KS = KSteps<PREC, NB>::n;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nb = 3 * na;
-    const int tl = blockIdx.x / nb, ia = blockIdx.x % nb;
-    const int t = t0 + tl;
+    const int tl = blockIdx.x / NB, ia = blockIdx.x % NB;
+    const int t = u.t0 + tl;
     const int a1 = ia / 3, i = ia % 3;
-    const size_t nb2 = (size_t)nb * nb;
-    T *phr = (T *)smem, *phi = phr + nb2, *ur = phi + nb2, *ui = ur + nb2;
-    cd *ph = (cd *)(ui + nb2);
-    const int q2 = tq2[t], q3 = tq3[t];
-    const double *Q2 = qcart + 3 * q2, *Q3 = qcart + 3 * q3;
-    const int f0 = a1_off[a1], f1 = a1_off[a1 + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    // LDS: U, then the Phi blocks of (a1, i), their a3, then the phases of a1's IFCs
+    constexpr size_t ub = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB2;
+    __bf16 *ut = (__bf16 *)smem;   // BF16X3: [plane re hi/lo, im hi/lo][b3][ib], row stride RS
+    T *us = (T *)smem;             // FP64/FP32: [re/im][ib][b3]
+    cd *pb = (cd *)(smem + ((ub + 15) & ~(size_t)15));
+    int *pa3 = (int *)(pb + 9 * u.maxblk);
+    cd *ph = (cd *)(smem + align16((size_t)((unsigned char *)(pa3 + u.maxblk) - smem)));
+    const int q2 = u.tq2[t], q3 = u.tq3[t];
+    const double *Q2 = u.qcart + 3 * q2, *Q3 = u.qcart + 3 * q3;
+    const int f0 = u.a1_off[a1], f1 = u.a1_off[a1 + 1];
     for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
-        const double arg = -kTwoPi * (Q2[0] * lv2[3 * f] + Q2[1] * lv2[3 * f + 1] + Q2[2] * lv2[3 * f + 2] +
-                                      Q3[0] * lv3[3 * f] + Q3[1] * lv3[3 * f + 1] + Q3[2] * lv3[3 * f + 2]);
+        const double arg = -kTwoPi * (Q2[0] * u.lv2[3 * f] + Q2[1] * u.lv2[3 * f + 1] + Q2[2] * u.lv2[3 * f + 2] +
+                                      Q3[0] * u.lv3[3 * f] + Q3[1] * u.lv3[3 * f + 1] + Q3[2] * u.lv3[3 * f + 2]);
         double s, c;
         sincos(arg, &s, &c);
         ph[f - f0] = {c, s};
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < (int)nb2; e += blockDim.x) {
-        const int ib = e / nb, ic = e % nb;
-        const int blk = (a1 * na + ib / 3) * na + ic / 3;
-        const int j = ib % 3, k = ic % 3;
-        double ax = 0, ay = 0;
-        for (int f = blk_off[blk]; f < blk_off[blk + 1]; ++f) {
-            const double m = mwm[27 * f + 9 * i + 3 * j + k];
-            ax += m * ph[f - f0].x;
-            ay += m * ph[f - f0].y;
-        }
-        phr[e] = (T)ax;
-        phi[e] = (T)ay;
+    if constexpr (PREC == 2) {
+        // zero U including the K padding read by the MFMA fragments
+        uint4 *z = (uint4 *)ut;
+        for (int x = threadIdx.x; x < 4 * NB * RS / 8; x += blockDim.x) z[x] = make_uint4(0, 0, 0, 0);
     }
     __syncthreads();
-    const T *E3r = Ef_re + (size_t)q3 * nb2, *E3i = Ef_im + (size_t)q3 * nb2;
-    const T *E2r = Ef_re + (size_t)q2 * nb2, *E2i = Ef_im + (size_t)q2 * nb2;
-    // U[ib][b3] = sum_ic Phi[ib][ic] E3[ic][b3]
-    cgemm_nb<PREC>(
-        nb, [&](int m, int k, T &x, T &y) { x = phr[m * nb + k]; y = phi[m * nb + k]; },
-        [&](int k, int n, T &x, T &y) { x = E3r[k * nb + n]; y = E3i[k * nb + n]; },
-        [&](int m, int n, T x, T y) { ur[m * nb + n] = x; ui[m * nb + n] = y; });
+    const int bo0 = u.a1_boff[a1], nblk = u.a1_boff[a1 + 1] - bo0;
+    for (int it = threadIdx.x; it < 9 * nblk; it += blockDim.x) {
+        const int bl = it / 9, jk = it % 9, j = jk / 3, k = jk % 3;
+        const int4 b = u.blist[bo0 + bl];
+        double ax = 0, ay = 0;
+        for (int f = b.z; f < b.w; ++f) {
+            const double m = u.mwm[27 * f + 9 * i + 3 * j + k];
+            const cd p = ph[f - f0];
+            ax += m * p.x;
+            ay += m * p.y;
+        }
+        pb[it] = {ax, ay};
+        if (jk == 0) pa3[bl] = b.y;
+    }
+    __syncthreads();
+    // U[ib][b3] = sum over the blocks (a1, a2 = ib/3, a3) of sum_k Phi[j][k] E3[3 a3 + k][b3]
+    const T *E3r = (const T *)u.Ef_re + (size_t)q3 * NB2, *E3i = (const T *)u.Ef_im + (size_t)q3 * NB2;
+    for (int e = threadIdx.x; e < NB2; e += blockDim.x) {
+        const int ib = e / NB, b3 = e % NB, a2 = ib / 3, j = ib % 3;
+        const int s0 = u.a12_boff[a1 * NA + a2] - bo0, s1 = u.a12_boff[a1 * NA + a2 + 1] - bo0;
+        T ur = 0, ui = 0;
+        for (int bl = s0; bl < s1; ++bl) {
+            const int a3 = pa3[bl];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const cd p = pb[bl * 9 + 3 * j + k];
+                const T er = E3r[(3 * a3 + k) * NB + b3], ei = E3i[(3 * a3 + k) * NB + b3];
+                ur += (T)p.x * er - (T)p.y * ei;
+                ui += (T)p.x * ei + (T)p.y * er;
+            }
+        }
+        if constexpr (PREC == 2) {
+            const __bf16 rh = (__bf16)ur, ih = (__bf16)ui;
+            __bf16 *o = ut + (size_t)b3 * RS + ib;
+            o[0] = rh;
+            o[(size_t)NB * RS] = (__bf16)(ur - (float)rh);
+            o[(size_t)2 * NB * RS] = ih;
+            o[(size_t)3 * NB * RS] = (__bf16)(ui - (float)ih);
+        } else {
+            us[ib * NB + b3] = ur;
+            us[NB2 + ib * NB + b3] = ui;
+        }
+    }
     __syncthreads();
     // W[b2][b3] = sum_ib E2[ib][b2] U[ib][b3]
-    T *Wr = W + (size_t)tl * 2 * nb * nb2 + (size_t)ia * nb2;
-    T *Wi = Wr + nb * nb2;
-    cgemm_nb<PREC>(
-        nb, [&](int m, int k, T &x, T &y) { x = E2r[k * nb + m]; y = E2i[k * nb + m]; },
-        [&](int k, int n, T &x, T &y) { x = ur[k * nb + n]; y = ui[k * nb + n]; },
-        [&](int m, int n, T x, T y) { Wr[m * nb + n] = x; Wi[m * nb + n] = y; });
+    T *Wr = (T *)u.W + (size_t)tl * 2 * NB * NB2 + (size_t)ia * NB2;
+    T *Wi = Wr + (size_t)NB * NB2;
+    for (int tile = wave; tile < NT * NT; tile += 4) {
+        const int m0 = (tile / NT) * 16, n0 = (tile % NT) * 16;
+        Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        if constexpr (PREC == 2) {
+            const size_t ps = (size_t)NB * KP;
+            const __bf16 *pa = u.Et + (size_t)q2 * kPlanes * ps + (size_t)(m0 + c16) * KP + 8 * g;
+            const __bf16 *pu = ut + (size_t)(n0 + c16) * RS + 8 * g;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                Split ar, ai, an, br, bi;
+                const int kb = 32 * ks;
+                ar.h = ld8(pa + kb);
+                ar.l = ld8(pa + ps + kb);
+                ai.h = ld8(pa + 2 * ps + kb);
+                ai.l = ld8(pa + 3 * ps + kb);
+                an.h = ld8(pa + 4 * ps + kb);
+                an.l = ld8(pa + 5 * ps + kb);
+                br.h = ld8(pu + kb);
+                br.l = ld8(pu + (size_t)NB * RS + kb);
+                bi.h = ld8(pu + (size_t)2 * NB * RS + kb);
+                bi.l = ld8(pu + (size_t)3 * NB * RS + kb);
+                mma3(cr, ar, br);
+                mma3(cr, an, bi);
+                mma3(ci, ar, bi);
+                mma3(ci, ai, br);
+            }
+        } else {
+            const T *E2r = (const T *)u.Ef_re + (size_t)q2 * NB2, *E2i = (const T *)u.Ef_im + (size_t)q2 * NB2;
+#pragma unroll 4
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = 4 * ks + g;
+                const T ar = E2r[k * NB + m0 + c16], ai = E2i[k * NB + m0 + c16];
+                const T br = us[k * NB + n0 + c16], bi = us[NB2 + k * NB + n0 + c16];
+                mma1(cr, ar, br);
+                mma1(cr, -ai, bi);
+                mma1(ci, ar, bi);
+                mma1(ci, ai, br);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int b2 = m0 + out_row<PREC>(lane, r);
+            Wr[b2 * NB + n0 + c16] = cr[r];
+            Wi[b2 * NB + n0 + c16] = ci[r];
+        }
+    }
 }
 
 __device__ inline double gaussd(double x, double mu, double sigma) {
+    // lo_gauss (gottochblandat_calculus.f90:268-288)
     const double d = x - mu;
     if (fabs(d) > 8.0 * sigma) return 0.0;
     const double inv = 1.0 / sigma;
     return exp(-d * d * 0.5 * inv * inv) * 0.398942280401433 * inv;
 }
 
-// Staging/accumulation type of the epilogue: FP64 in FP64 mode, FP32 otherwise.
-template <int PREC> struct Stage { typedef float T; };
-template <> struct Stage<0> { typedef double T; };
+// d0 = g(w1; w3 - w2) - g(w1; w2 - w3), d1 = g(w1; w2 + w3) - g(w1; -w2 - w3) (get_dirac,
+// scattering_threephonon.f90:247-270) for sigma^2 = sig2, with g zero beyond 8 sigma.
+__device__ inline void delta_pair(double w1, double w2, double w3, double sig2, double &d0, double &d1) {
+    const double sig = sqrt(sig2);
+    d0 = gaussd(w1, -w2 + w3, sig) - gaussd(w1, w2 - w3, sig);
+    d1 = gaussd(w1, w2 + w3, sig) - gaussd(w1, -w2 - w3, sig);
+}
+__device__ inline void delta_pair(double w1, double w2, double w3, float sig2, float &d0, float &d1) {
+    const float inv = __builtin_amdgcn_rsqf(sig2);
+    const float ce = -0.72134752044448170f * inv * inv;  // -log2(e) / (2 sigma^2)
+    const float lim = 64.0f * sig2;                       // (8 sigma)^2
+    const float x1 = (float)w1, x2 = (float)w2, x3 = (float)w3;
+    const float dd = x2 - x3, ss = x2 + x3;
+    auto g = [&](float d) {
+        const float d2 = d * d;
+        return d2 > lim ? 0.0f : __builtin_amdgcn_exp2f(ce * d2);
+    };
+    const float nrm = 0.398942280401433f * inv;
+    d0 = nrm * (g(x1 + dd) - g(x1 - dd));
+    d1 = nrm * (g(x1 - ss) - g(x1 + ss));
+}
 
-template <int PREC>
-__global__ void __launch_bounds__(256) k_psi(MfmaEpiArgs a, const typename Traits<PREC>::T *__restrict__ E1_re,
-                                             const typename Traits<PREC>::T *__restrict__ E1_im,
-                                             const typename Traits<PREC>::T *__restrict__ W) {
+template <int PREC, int NB, int NTY>
+size_t psi_lds() {
+    typedef typename Stage<PREC>::T S;
+    constexpr int NW = NB / 16;
+    size_t s = sizeof(S) * (2 * NTY * NB + NB + NTY * 16 * NB + NW * kChunk * 16 * NTY);
+    s = (s + 15) & ~(size_t)15;
+    return s + sizeof(double) * (NB + 16) + sizeof(int) * 16;
+}
+
+template <int PREC, int NB, int NTY>
+__global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_, const void *E1im_,
+                                                const __bf16 *__restrict__ Et1, const void *W_) {
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
     typedef typename Stage<PREC>::T S;
+    constexpr int NW = NB / 16, NB2 = NB * NB, KS = KSteps<PREC, NB>::n, KP = kpad(NB);
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nb = a.nb, nT = a.nT, CB = a.cb;
-    const int nbp = nb + 1;                         // padded row stride of the staging arrays
-    const int nbb = nb / 16;
+    const int nT = a.nT;
     // XCD-aware order: the b1 blocks of one triplet share blockIdx % 8
-    const int g = blockIdx.x;
-    const int tl = (g / (8 * nbb)) * 8 + (g % 8);
-    const int bb = (g / 8) % nbb;
+    const int gid = blockIdx.x;
+    const int tl = (gid / (8 * NW)) * 8 + (gid % 8);
+    const int bb = (gid / 8) % NW;
     if (tl >= a.nt) return;
     const int t = a.t0 + tl;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
     const int q1 = a.tq1[t], q2 = a.tq2[t], q3 = a.tq3[t];
     const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
-    const double perm = a.perm[t];
-    const int e0 = a.eq_off[t], e1 = a.eq_off[t + 1];
-    const size_t nb2 = (size_t)nb * nb;
-    // LDS layout
-    S *A = (S *)smem;                               // [16][CB][nbp]  psisq*d0 (masked)
-    S *B = A + 16 * CB * nbp;                       // [16][CB][nbp]  psisq*d1
-    S *S2 = B + 16 * CB * nbp;                      // [nT][16][nb]   sum over b3
-    S *S3 = S2 + nT * 16 * nb;                      // [nT][16][nb]   sum over b2
-    double *n2t = (double *)(((size_t)(S3 + nT * 16 * nb) + 15) & ~(size_t)15);  // [nT][nb]
-    double *n3t = n2t + nT * nb;                    // [nT][nb]
-    double *w1 = n3t + nT * nb;                     // 16
-    double *sg1 = w1 + 16;                          // 16
-    int *row = (int *)(sg1 + 16);                   // 16
-    for (int x = threadIdx.x; x < nT * nb; x += blockDim.x) {
-        const int T_ = x / nb, b = x % nb;
-        n2t[x] = a.be[((size_t)T_ * a.nirr + i2) * nb + b];
-        n3t[x] = a.be[((size_t)T_ * a.nirr + i3) * nb + b];
+    const int e0 = a.eq_off[t], e1 = a.eq_off[t + 1], neq = e1 - e0;
+    // LDS
+    S *n2h = (S *)smem;                    // [NTY][NB]     n2 + 1/2
+    S *n3h = n2h + NTY * NB;               // [NTY][NB]     n3 + 1/2
+    S *sg2s = n3h + NTY * NB;              // [NB]          sigma^2 of q2 modes
+    S *S2 = sg2s + NB;                     // [NTY][16][NB] sum over b3 of f_T
+    S *Xp = S2 + NTY * 16 * NB;            // [NW][kChunk][16][NTY] per-wave partial sums
+    double *w2s = (double *)(smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + NB + NTY * 16 * NB +
+                                                                   NW * kChunk * 16 * NTY)));
+    double *w1s = w2s + NB;                // [16]
+    int *rows = (int *)(w1s + 16);         // [16]
+    for (int x = threadIdx.x; x < nT * NB; x += blockDim.x) {
+        const int T_ = x / NB, b = x % NB;
+        n2h[T_ * NB + b] = (S)(a.be[((size_t)T_ * a.nirr + i2) * NB + b] + 0.5);
+        n3h[T_ * NB + b] = (S)(a.be[((size_t)T_ * a.nirr + i3) * NB + b] + 0.5);
     }
-    for (int x = threadIdx.x; x < nT * 16 * nb; x += blockDim.x) S3[x] = 0;
+    for (int b = threadIdx.x; b < NB; b += blockDim.x) {
+        w2s[b] = a.om_full[(size_t)q2 * NB + b];
+        sg2s[b] = (S)a.sigsq[(size_t)i2 * NB + b];
+    }
     if (threadIdx.x < 16) {
         const int b1 = 16 * bb + threadIdx.x;
-        w1[threadIdx.x] = a.om_irr[(size_t)q1 * nb + b1];
-        sg1[threadIdx.x] = a.sigsq[(size_t)q1 * nb + b1];
-        row[threadIdx.x] = a.row_of[(size_t)q1 * nb + b1];
+        w1s[threadIdx.x] = a.om_irr[(size_t)q1 * NB + b1];
+        rows[threadIdx.x] = a.row_of[(size_t)q1 * NB + b1];
     }
     __syncthreads();
-    const T *E1r = E1_re + (size_t)q1 * nb2, *E1i = E1_im + (size_t)q1 * nb2;
-    const T *Wr = W + (size_t)tl * 2 * nb * nb2;
-    const T *Wi = Wr + nb * nb2;
-    const double *om2 = a.om_full + (size_t)q2 * nb, *om3 = a.om_full + (size_t)q3 * nb;
-    const double *sgq2 = a.sigsq + (size_t)i2 * nb, *sgq3 = a.sigsq + (size_t)i3 * nb;
-    const int ntile = CB * nb / 16;
-    for (int c0 = 0; c0 < nb; c0 += CB) {
-        // Psi[b1][(b2,b3)] for b2 in [c0, c0+CB): tiles of 16 consecutive b3
-        for (int tile = wave; tile < ntile; tile += 4) {
-            const int n0 = c0 * nb + tile * 16;
-            Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-            for (int k0 = 0; k0 < nb; k0 += Traits<PREC>::KS) {
-                Frag<PREC> ar, ai, br, bi;
-                load_a<PREC>(lane, 16 * bb, k0, nb,
-                             [&](int m, int k, T &x, T &y) { x = E1r[k * nb + m]; y = E1i[k * nb + m]; }, ar, ai);
-                load_b<PREC>(lane, k0, n0, nb,
-                             [&](int k, int n, T &x, T &y) { x = Wr[(size_t)k * nb2 + n]; y = Wi[(size_t)k * nb2 + n]; },
-                             br, bi);
-                cmma<PREC>(cr, ci, ar, ai, br, bi);
-            }
-            const int n = n0 + (lane & 15);
-            const int b2 = n / nb, b3 = n % nb;
-            const double w2 = om2[b2], w3 = om3[b3];
+    // per-lane constants: b1 = 16 bb + c16 and four values of b3
+    const double w1 = w1s[c16];
+    const int row1 = rows[c16];
+    const S sg1 = (S)a.sigsq[(size_t)q1 * NB + 16 * bb + c16];
+    const double *om3 = a.om_full + (size_t)q3 * NB;
+    int b3r[4];
+    double w3[4];
+    S sg3[4], bm[4];
+    bool ok13[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = out_row<PREC>(lane, r);       // local b1
-                const double w1v = w1[m];
-                double va = 0.0, vb = 0.0;
-                if (row[m] >= 0 && !(w2 < 1e-9) && !(fabs(w1v - w2) < 1e-9) && !(w3 < 1e-9) &&
-                    !(fabs(w1v - w3) < 1e-9) && !(fabs(w2 - w3) < 1e-9)) {
-                    const double re = (double)cr[r], im = (double)ci[r];
-                    const double psisq = a.psi_scale * (re * re + im * im) * perm;
-                    const double sig = (a.integrationtype == 2) ? sqrt(sg1[m] + sgq2[b2] + sgq3[b3]) : a.sigma_fixed;
-                    const double d0 = gaussd(w1v, -w2 + w3, sig) - gaussd(w1v, w2 - w3, sig);
-                    const double d1 = gaussd(w1v, w2 + w3, sig) - gaussd(w1v, -w2 - w3, sig);
-                    va = psisq * d0;
-                    vb = psisq * d1;
-                }
-                const int loc = (m * CB + (b2 - c0)) * nbp + b3;
-                A[loc] = (S)va;
-                B[loc] = (S)vb;
-            }
-        }
-        __syncthreads();
-        // S2[T][b1][b2] = sum_b3 f  (work item per (T, b1, b2 of the chunk))
-        for (int x = threadIdx.x; x < nT * 16 * CB; x += blockDim.x) {
-            const int T_ = x / (16 * CB), mj = x % (16 * CB);
-            const int m = mj / CB, j = mj % CB, b2 = c0 + j;
-            const double *n3 = n3t + T_ * nb;
-            const double n2 = n2t[T_ * nb + b2];
-            const S *pa = A + (size_t)mj * nbp, *pb = B + (size_t)mj * nbp;
-            S s = 0;
-            for (int b3 = 0; b3 < nb; ++b3)
-                s += (S)(n2 - n3[b3]) * pa[b3] + (S)(n2 + n3[b3] + 1.0) * pb[b3];
-            S2[(T_ * 16 + m) * nb + b2] = s;
-        }
-        // S3[T][b1][b3] += sum_{b2 in chunk} f  (work item per (T, b1, b3))
-        for (int x = threadIdx.x; x < nT * 16 * nb; x += blockDim.x) {
-            const int T_ = x / (16 * nb), mb = x % (16 * nb);
-            const int m = mb / nb, b3 = mb % nb;
-            const double *n2 = n2t + T_ * nb;
-            const double n3 = n3t[T_ * nb + b3];
-            S s = 0;
-            for (int j = 0; j < CB; ++j) {
-                const int loc = (m * CB + j) * nbp + b3;
-                s += (S)(n2[c0 + j] - n3) * A[loc] + (S)(n2[c0 + j] + n3 + 1.0) * B[loc];
-            }
-            S3[x] += s;
-        }
-        __syncthreads();
+    for (int r = 0; r < 4; ++r) {
+        b3r[r] = 16 * w + out_row<PREC>(lane, r);
+        w3[r] = om3[b3r[r]];
+        sg3[r] = (S)a.sigsq[(size_t)i3 * NB + b3r[r]];
+        ok13[r] = row1 >= 0 && !(w3[r] < kTol) && !(fabs(w1 - w3[r]) < kTol);
+        bm[r] = (c16 < nT) ? n3h[c16 * NB + b3r[r]] : (S)0;
     }
-    // linewidth partials (g0 += f0 + f1 per equivalent triplet, :126-128)
+    // E1 operand (B[k = ia][n = b1]) held in registers for the whole triplet
+    Split e1r[PREC == 2 ? KS : 1], e1i[PREC == 2 ? KS : 1], e1n[PREC == 2 ? KS : 1];
+    const T *E1r = (const T *)E1re_ + (size_t)q1 * NB2 + 16 * bb + c16;
+    const T *E1i = (const T *)E1im_ + (size_t)q1 * NB2 + 16 * bb + c16;
+    if constexpr (PREC == 2) {
+        const size_t ps = (size_t)NB * KP;
+        const __bf16 *pe = Et1 + (size_t)q1 * kPlanes * ps + (size_t)(16 * bb + c16) * KP + 8 * g;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            e1r[ks].h = ld8(pe + 32 * ks);
+            e1r[ks].l = ld8(pe + ps + 32 * ks);
+            e1i[ks].h = ld8(pe + 2 * ps + 32 * ks);
+            e1i[ks].l = ld8(pe + 3 * ps + 32 * ks);
+            e1n[ks].h = ld8(pe + 4 * ps + 32 * ks);
+            e1n[ks].l = ld8(pe + 5 * ps + 32 * ks);
+        }
+    }
+    // W operand (A[m = (b2, b3)][k = ia]), prefetched one tile ahead
+    const T *Wr = (const T *)W_ + (size_t)tl * 2 * NB * NB2 + 16 * w + c16;
+    const T *Wi = Wr + (size_t)NB * NB2;
+    constexpr int RW = (PREC == 2) ? 8 : 1;
+    T rr[KS][RW], ri[KS][RW];
+    auto load_w = [&](int b2) {
+        const size_t n = (size_t)b2 * NB;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            if constexpr (PREC == 2) {
+                const int kb = 32 * ks + 8 * g;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    rr[ks][j] = (kb < NB) ? Wr[(size_t)(kb + j) * NB2 + n] : 0.f;
+                    ri[ks][j] = (kb < NB) ? Wi[(size_t)(kb + j) * NB2 + n] : 0.f;
+                }
+            } else {
+                const int k = 4 * ks + g;
+                rr[ks][0] = Wr[(size_t)k * NB2 + n];
+                ri[ks][0] = Wi[(size_t)k * NB2 + n];
+            }
+        }
+    };
+    S y[NTY][4], yM[4];
+#pragma unroll
+    for (int T_ = 0; T_ < NTY; ++T_)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y[T_][r] = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) yM[r] = 0;
+    const S sc = (S)(a.psi_scale * a.perm[t]);
+    const S sfix2 = (S)(a.sigma_fixed * a.sigma_fixed);
+    load_w(0);
+    for (int b2 = 0; b2 < NB; ++b2) {
+        Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        if constexpr (PREC == 2) {
+            Split ar[KS], ai[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                split8(rr[ks], ar[ks]);
+                split8(ri[ks], ai[ks]);
+            }
+            if (b2 + 1 < NB) load_w(b2 + 1);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                mma3(cr, ar[ks], e1r[ks]);
+                mma3(cr, ai[ks], e1n[ks]);
+                mma3(ci, ar[ks], e1i[ks]);
+                mma3(ci, ai[ks], e1r[ks]);
+            }
+        } else {
+            T ar[KS], ai[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                ar[ks] = rr[ks][0];
+                ai[ks] = ri[ks][0];
+            }
+            if (b2 + 1 < NB) load_w(b2 + 1);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = 4 * ks + g;
+                const T br = E1r[k * NB], bi = E1i[k * NB];
+                mma1(cr, ar[ks], br);
+                mma1(cr, ai[ks], -bi);
+                mma1(ci, ar[ks], bi);
+                mma1(ci, ai[ks], br);
+            }
+        }
+        // smeared energy conservation for the 4 elements (b1, b2, b3r[r]) of this lane
+        const double w2 = w2s[b2];
+        const bool ok2 = row1 >= 0 && !(w2 < kTol) && !(fabs(w1 - w2) < kTol);
+        const S sg12 = sg1 + sg2s[b2];
+        S P[4], M[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            P[r] = 0;
+            M[r] = 0;
+            if (ok2 && ok13[r] && !(fabs(w2 - w3[r]) < kTol)) {
+                const S re = (S)cr[r], im = (S)ci[r];
+                const S psisq = sc * (re * re + im * im);
+                const S sig2 = (a.integrationtype == 2) ? sg12 + sg3[r] : sfix2;
+                S d0, d1;
+                delta_pair(w1, w2, w3[r], sig2, d0, d1);
+                const S A = psisq * d0, B = psisq * d1;
+                P[r] = A + B;
+                M[r] = B - A;
+            }
+        }
+        // sum over this wave's b3 of f_T, temperature T in output column c16
+        const S bp = (c16 < nT) ? n2h[c16 * NB + b2] : (S)0;
+        Acc xa = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            mma1(xa, M[r], bm[r]);
+            mma1(xa, P[r], bp);
+        }
+        const int cs = b2 % kChunk;
+        if (c16 < nT) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Xp[((w * kChunk + cs) * 16 + out_row<PREC>(lane, r)) * NTY + c16] = (S)xa[r];
+        }
+        // sums over b2, kept per lane
+#pragma unroll
+        for (int T_ = 0; T_ < NTY; ++T_) {
+            if (T_ < nT) {
+                const S nh = n2h[T_ * NB + b2];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) y[T_][r] += P[r] * nh;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) yM[r] += M[r];
+        if (cs == kChunk - 1) {
+            __syncthreads();
+            const int cb = b2 - cs;
+            for (int x = threadIdx.x; x < kChunk * 16 * nT; x += blockDim.x) {
+                const int T_ = x % nT, mc = x / nT, m = mc % 16, c = mc / 16;
+                S s = 0;
+#pragma unroll
+                for (int ww = 0; ww < NW; ++ww) s += Xp[((ww * kChunk + c) * 16 + m) * NTY + T_];
+                S2[(T_ * 16 + m) * NB + cb + c] = s;
+            }
+            __syncthreads();
+        }
+    }
+    // linewidth partials (g0 += f0 + f1 for every equivalent triplet, :126-128)
     for (int x = threadIdx.x; x < nT * 16; x += blockDim.x) {
         const int T_ = x / 16, m = x % 16;
         double s = 0;
-        for (int b2 = 0; b2 < nb; ++b2) s += (double)S2[(T_ * 16 + m) * nb + b2];
-        a.gpart[((size_t)t * nT + T_) * nb + 16 * bb + m] = (row[m] >= 0) ? s * (double)(e1 - e0) : 0.0;
+        for (int b2 = 0; b2 < NB; ++b2) s += (double)S2[(T_ * 16 + m) * NB + b2];
+        a.gpart[((size_t)t * a.gT + a.T0 + T_) * NB + 16 * bb + m] = (rows[m] >= 0) ? s * (double)neq : 0.0;
     }
-    // scattering-matrix entries of this (triplet, b1 block) (:133-134).  Every target
-    // (row, q', band) belongs to exactly one triplet, and inside a triplet q' appears in
-    // one equivalent pair only, so the values are stored, not accumulated.
-    const int neq = e1 - e0;
-    for (long x = threadIdx.x; x < (long)nT * 16 * neq * nb; x += blockDim.x) {
-        const int b = (int)(x % nb);
-        long y = x / nb;
-        const int e = e0 + (int)(y % neq);
-        y /= neq;
-        const int m = (int)(y % 16), T_ = (int)(y / 16);
-        const int r = row[m];
-        if (r < 0) continue;
-        double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row;
-        const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
-        const double v2 = 2.0 * (double)S2[(T_ * 16 + m) * nb + b] * om2[b] / w1[m];
-        const double v3 = 2.0 * (double)S3[(T_ * 16 + m) * nb + b] * om3[b] / w1[m];
-        if (x2 == x3) {
-            xr[(size_t)x2 * nb + b] = v2 + v3;
-        } else {
-            xr[(size_t)x2 * nb + b] = v2;
-            xr[(size_t)x3 * nb + b] = v3;
+    // scattering-matrix entries (:133-134).  Every target (row, q', band) belongs to one
+    // triplet and, inside it, to one equivalent pair, so values are stored, not added.
+    // q3' entries from this lane's registers; q2' == q3' also takes the q2' term.
+    if (row1 >= 0) {
+        const double iw1 = 2.0 / w1;
+        for (int e = e0; e < e1; ++e) {
+            const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
+#pragma unroll
+            for (int T_ = 0; T_ < NTY; ++T_) {
+                if (T_ >= nT) continue;
+                double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)row1 * a.xi_row + (size_t)x3 * NB;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int b3 = b3r[r];
+                    const S s3 = y[T_][r] + n3h[T_ * NB + b3] * yM[r];
+                    double v = (double)s3 * w3[r] * iw1;
+                    if (x2 == x3) v += (double)S2[(T_ * 16 + c16) * NB + b3] * w2s[b3] * iw1;
+                    xr[b3] = v;
+                }
+            }
         }
     }
-}
-
-template <int PREC>
-__global__ void k_scale_planar(long n, int nb, const cd *__restrict__ egv, const double *__restrict__ om,
-                               typename Traits<PREC>::T *__restrict__ re, typename Traits<PREC>::T *__restrict__ im) {
-    typedef typename Traits<PREC>::T T;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const long q = i / ((long)nb * nb);
-        const int b = (int)(i % nb);
-        const double w = om[q * nb + b];
-        const double s = (w < 1e-9) ? 0.0 : 1.0 / sqrt(w);
-        re[i] = (T)(egv[i].x * s);
-        im[i] = (T)(egv[i].y * s);
+    // q2' entries from LDS
+    const long nx = (long)nT * 16 * neq * NB;
+    for (long x = threadIdx.x; x < nx; x += blockDim.x) {
+        const int b = (int)(x % NB);
+        long yy = x / NB;
+        const int e = e0 + (int)(yy % neq);
+        yy /= neq;
+        const int m = (int)(yy % 16), T_ = (int)(yy / 16);
+        const int r = rows[m];
+        if (r < 0) continue;
+        const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
+        if (x2 == x3) continue;
+        a.xi[(size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)x2 * NB + b] =
+            2.0 * (double)S2[(T_ * 16 + m) * NB + b] * w2s[b] / w1s[m];
     }
 }
 
-template <int PREC>
-int run_mfma(const MfmaPlan &p, hipStream_t st) {
-    typedef typename Traits<PREC>::T T;
-    const int nb = p.nb;
-    const size_t nb2 = (size_t)nb * nb;
-    hipLaunchKernelGGL(k_scale_planar<PREC>, dim3(1024), dim3(256), 0, st, (long)p.nirr * nb2, nb, p.egv_irr, p.om_irr,
-                       (T *)p.E1re, (T *)p.E1im);
-    hipLaunchKernelGGL(k_scale_planar<PREC>, dim3(4096), dim3(256), 0, st, (long)p.nfull * nb2, nb, p.egv_full, p.om_full,
-                       (T *)p.Efre, (T *)p.Efim);
-    const size_t sh_uw = sizeof(T) * 4 * nb2 + sizeof(cd) * p.maxa1;
-    const size_t sh_psi = p.psi_shmem;
-    hipFuncSetAttribute((const void *)k_uw<PREC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh_uw);
-    hipFuncSetAttribute((const void *)k_psi<PREC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh_psi);
+template <int PREC, int NB, int NTY>
+int run_chain(const MfmaPlan &p, hipStream_t st) {
+    constexpr int NW = NB / 16;
+    UwArgs u;
+    u.maxblk = std::max(p.maxblk, 1);
+    u.tq2 = p.tq2; u.tq3 = p.tq3; u.qcart = p.qcart; u.lv2 = p.lv2; u.lv3 = p.lv3; u.mwm = p.mwm;
+    u.a1_off = p.a1off; u.blist = p.blist; u.a1_boff = p.a1_boff; u.a12_boff = p.a12_boff;
+    u.Ef_re = p.Efre; u.Ef_im = p.Efim; u.Et = (const __bf16 *)p.Etf; u.W = p.work;
+    const size_t sh_uw = uw_lds<PREC, NB>(u.maxblk, p.maxa1);
+    const size_t sh_psi = psi_lds<PREC, NB, NTY>();
+    hipError_t e = hipFuncSetAttribute((const void *)k_uw<PREC, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sh_uw);
+    if (e != hipSuccess) return device_error(e, "k_uw LDS size", __FILE__, __LINE__);
+    e = hipFuncSetAttribute((const void *)k_psi<PREC, NB, NTY>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sh_psi);
+    if (e != hipSuccess) return device_error(e, "k_psi LDS size", __FILE__, __LINE__);
     MfmaEpiArgs ea = p.epi;
-    const int nbb = nb / 16;
     for (long t0 = 0; t0 < p.ntrip; t0 += p.batch) {
         const int nt = (int)std::min<long>(p.batch, p.ntrip - t0);
         hipEvent_t *ev = p.ev ? p.ev + 3 * (t0 / p.batch) : nullptr;
         if (ev) hipEventRecord(ev[0], st);
-        hipLaunchKernelGGL(k_uw<PREC>, dim3(nt * nb), dim3(256), sh_uw, st, p.na, (int)t0, p.tq2, p.tq3, p.qcart,
-                           p.a1off, p.blk3, p.lv2, p.lv3, p.mwm, (const T *)p.Efre, (const T *)p.Efim, (T *)p.work);
+        u.t0 = (int)t0;
+        hipLaunchKernelGGL((k_uw<PREC, NB>), dim3(nt * NB), dim3(256), sh_uw, st, u);
         if (ev) hipEventRecord(ev[1], st);
         ea.t0 = (int)t0;
         ea.nt = nt;
         const int ngrp = (nt + 7) / 8;
-        hipLaunchKernelGGL(k_psi<PREC>, dim3(ngrp * 8 * nbb), dim3(256), sh_psi, st, ea, (const T *)p.E1re,
-                           (const T *)p.E1im, (const T *)p.work);
+        hipLaunchKernelGGL((k_psi<PREC, NB, NTY>), dim3(ngrp * 8 * NW), dim3(64 * NW), sh_psi, st, ea, p.E1re, p.E1im,
+                           (const __bf16 *)p.Eti, (const void *)p.work);
         if (ev) hipEventRecord(ev[2], st);
-        hipError_t e = hipGetLastError();
+        e = hipGetLastError();
         if (e != hipSuccess) return device_error(e, "MFMA contraction launch", __FILE__, __LINE__);
     }
     return 0;
+}
+
+template <int PREC>
+int prepare(const MfmaPlan &p, hipStream_t st) {
+    typedef typename Traits<PREC>::T T;
+    const int nb = p.nb;
+    const size_t nb2 = (size_t)nb * nb;
+    hipLaunchKernelGGL(k_scale_planar<T>, dim3(1024), dim3(256), 0, st, (long)p.nirr * nb2, nb, p.egv_irr, p.om_irr,
+                       (T *)p.E1re, (T *)p.E1im);
+    hipLaunchKernelGGL(k_scale_planar<T>, dim3(4096), dim3(256), 0, st, (long)p.nfull * nb2, nb, p.egv_full,
+                       p.om_full, (T *)p.Efre, (T *)p.Efim);
+    if (PREC == 2) {
+        hipLaunchKernelGGL(k_split_t, dim3(1024), dim3(256), 0, st, (long)p.nirr, nb, kpad(nb), p.egv_irr, p.om_irr,
+                           (__bf16 *)p.Eti);
+        hipLaunchKernelGGL(k_split_t, dim3(4096), dim3(256), 0, st, (long)p.nfull, nb, kpad(nb), p.egv_full,
+                           p.om_full, (__bf16 *)p.Etf);
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : device_error(e, "operand preparation", __FILE__, __LINE__);
+}
+
+template <int PREC, int NB>
+int dispatch_nty(const MfmaPlan &p, hipStream_t st) {
+    switch (p.nty) {
+    case 4: return run_chain<PREC, NB, 4>(p, st);
+    case 8: return run_chain<PREC, NB, 8>(p, st);
+    default: return run_chain<PREC, NB, 16>(p, st);
+    }
+}
+
+template <int PREC>
+int dispatch_nb(const MfmaPlan &p, hipStream_t st) {
+    int r = prepare<PREC>(p, st);
+    if (r) return r;
+    switch (p.nb) {
+    case 16: return dispatch_nty<PREC, 16>(p, st);
+    case 32: return dispatch_nty<PREC, 32>(p, st);
+    case 48: return dispatch_nty<PREC, 48>(p, st);
+    case 64: return dispatch_nty<PREC, 64>(p, st);
+    case 96: return dispatch_nty<PREC, 96>(p, st);
+    default: return 5;
+    }
+}
+
+// LDS bytes of the two kernels for band count nb and temperature block nty.
+size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
+    const size_t es = (precision == TC3_PREC_FP64) ? 8 : 4;
+    const size_t kp = kpad(nb), rs = kp + 8, nw = nb / 16;
+    size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * rs : es * 2 * nb * nb;
+    uw = ((uw + 15) & ~(size_t)15) + 16 * 9 * (size_t)maxblk + 4 * (size_t)maxblk;
+    uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
+    size_t ps = es * (2 * nty * nb + nb + nty * 16 * nb + nw * kChunk * 16 * nty);
+    ps = ((ps + 15) & ~(size_t)15) + 8 * (nb + 16) + 4 * 16;
+    return std::max(uw, ps);
 }
 
 }  // namespace
 
 size_t mfma_elem_size(int precision) { return precision == TC3_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
+size_t mfma_split_elems(int nb) { return (size_t)kPlanes * nb * kpad(nb); }
+
 bool mfma_supported(int nb, int precision) {
-    if (nb % 16 != 0) return false;
-    const size_t es = mfma_elem_size(precision);
-    return 4 * (size_t)nb * nb * es + 16 * 4096 <= 160 * 1024;
+    if (nb != 16 && nb != 32 && nb != 48 && nb != 64 && nb != 96) return false;
+    // IFC blocks and entries per atom are bounded separately by the caller
+    return lds_need(nb, 4, precision, 0, 0) + 32 * 1024 <= 160 * 1024;
 }
 
-int mfma_choose_chunk(int nb, int nT, int precision, size_t *shmem) {
-    const size_t es = (precision == TC3_PREC_FP64) ? sizeof(double) : sizeof(float);
-    const size_t fixed = es * (2 * (size_t)nT * 16 * nb) + 16 +
-                         sizeof(double) * (2 * (size_t)nT * nb + 32) + 16 * sizeof(int);
-    // largest chunk of b2 that keeps the workgroup within 80 KB (two per CU), else 150 KB
-    for (size_t cap : {(size_t)80 * 1024, (size_t)150 * 1024})
-        for (int cb : {16, 8, 4, 2, 1}) {
-            if (nb % cb) continue;
-            const size_t s = fixed + es * (2 * 16 * (size_t)cb * (nb + 1));
-            if (s <= cap) {
-                *shmem = s;
-                return cb;
-            }
-        }
+int mfma_temperature_block(int nb, int nT, int precision) {
+    for (int nty : {4, 8, 16})
+        if (nty >= nT && lds_need(nb, nty, precision, 0, 0) + 32 * 1024 <= 160 * 1024) return nty;
+    for (int nty : {16, 8, 4})
+        if (lds_need(nb, nty, precision, 0, 0) + 32 * 1024 <= 160 * 1024) return nty;
     return 0;
 }
 
 int run_mfma_chain(const MfmaPlan &p, int precision, hipStream_t st) {
+    if (lds_need(p.nb, p.nty, precision, p.maxblk, p.maxa1) > 160 * 1024) return 5;
     switch (precision) {
-    case TC3_PREC_FP64: return run_mfma<0>(p, st);
-    case TC3_PREC_FP32: return run_mfma<1>(p, st);
-    default: return run_mfma<2>(p, st);
+    case TC3_PREC_FP64: return dispatch_nb<0>(p, st);
+    case TC3_PREC_FP32: return dispatch_nb<1>(p, st);
+    default: return dispatch_nb<2>(p, st);
     }
 }
 

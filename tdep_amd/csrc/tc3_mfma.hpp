@@ -8,30 +8,36 @@
 namespace tc3 {
 
 struct MfmaEpiArgs {
-    int nb, nT, t0, nt, nirr, integrationtype, cb;
+    int nb, nT, t0, nt, nirr, integrationtype;
+    int gT, T0;              // temperature stride and offset of gpart (temperature chunks)
     double sigma_fixed, psi_scale;
     const int *tq1, *tq2, *tq3, *eq_off, *eq_q2, *eq_q3, *irr_of_full, *row_of;
-    const double *perm, *om_irr, *om_full, *be, *sigsq;
-    double *xi, *gpart;
+    const double *perm, *om_irr, *om_full, *be, *sigsq;   // be: first temperature of the chunk
+    double *xi, *gpart;      // xi: first temperature of the chunk
     size_t xi_row, xi_T;
 };
 
 struct MfmaPlan {
-    int na, nb, nirr, nfull, maxa1;
+    int na, nb, nirr, nfull, maxa1, maxblk, nty;
     long ntrip, batch;
     const cd *egv_irr, *egv_full;
     const double *om_irr, *om_full;
-    void *E1re, *E1im, *Efre, *Efim, *work;
-    const int *tq2, *tq3, *a1off, *blk3;
+    void *E1re, *E1im, *Efre, *Efim, *work;   // planar e/sqrt(w), [q][component][mode]
+    void *Eti, *Etf;                          // split bf16 planes, [q][plane][mode][k] (BF16X3)
+    const int *tq2, *tq3, *a1off;
+    const int4 *blist;                        // nonzero (a2, a3) blocks of every a1
+    const int *a1_boff, *a12_boff;
     const double *qcart, *lv2, *lv3, *mwm;
-    size_t psi_shmem;
     MfmaEpiArgs epi;
-    hipEvent_t *ev;   // 3 events per batch (before k_uw, between, after k_psi), or null
+    hipEvent_t *ev;   // 3 events per launch pair (before k_uw, between, after k_psi), or null
 };
 
-size_t mfma_elem_size(int precision);
+// Whether the MFMA kernels handle this band count and precision, and the temperature
+// block (4, 8 or 16) a pass uses for nT temperatures (0 if none fits the LDS).
 bool mfma_supported(int nb, int precision);
-int mfma_choose_chunk(int nb, int nT, int precision, size_t *shmem);
+int mfma_temperature_block(int nb, int nT, int precision);
+size_t mfma_elem_size(int precision);
+size_t mfma_split_elems(int nb);   // bf16 elements of one q-point's split planes
 int run_mfma_chain(const MfmaPlan &p, int precision, hipStream_t st);
 
 }  // namespace tc3

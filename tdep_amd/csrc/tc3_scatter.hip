@@ -427,8 +427,34 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     for (double &x : c->kstat_ms) x = 0;
     for (int &x : c->kstat_n) x = 0;
     if (use_mfma) {
-        // W of one triplet: 2 * nb^3 reals; size the batch so W stays within the
-        // last-level cache, with at least 64 triplets per launch
+        // nonzero (a2, a3) IFC blocks of every atom a1, in (a1, a2, a3) order
+        std::vector<int4> blist;
+        std::vector<int> a1_boff(na + 1, 0), a12_boff((size_t)na * na + 1, 0);
+        int maxblk = 0;
+        for (int a1 = 0; a1 < na; ++a1) {
+            a1_boff[a1] = (int)blist.size();
+            for (int a2 = 0; a2 < na; ++a2) {
+                a12_boff[(size_t)a1 * na + a2] = (int)blist.size();
+                for (int a3 = 0; a3 < na; ++a3) {
+                    const size_t bk = ((size_t)a1 * na + a2) * na + a3;
+                    if (c->blk3_off[bk + 1] > c->blk3_off[bk])
+                        blist.push_back(make_int4(a2, a3, c->blk3_off[bk], c->blk3_off[bk + 1]));
+                }
+            }
+            maxblk = std::max(maxblk, (int)blist.size() - a1_boff[a1]);
+        }
+        a1_boff[na] = (int)blist.size();
+        a12_boff[(size_t)na * na] = (int)blist.size();
+        int4 *d_blist;
+        int *d_a1boff, *d_a12boff;
+        TRY(dalloc((void **)&d_blist, sizeof(int4) * std::max<size_t>(blist.size(), 1)));
+        TRY(dalloc((void **)&d_a1boff, sizeof(int) * a1_boff.size()));
+        TRY(dalloc((void **)&d_a12boff, sizeof(int) * a12_boff.size()));
+        if (!blist.empty())
+            TRY(hipMemcpyAsync(d_blist, blist.data(), sizeof(int4) * blist.size(), hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_a1boff, a1_boff.data(), sizeof(int) * a1_boff.size(), hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_a12boff, a12_boff.data(), sizeof(int) * a12_boff.size(), hipMemcpyHostToDevice, st));
+        // W of one triplet: 2 * nb^3 reals; batches of ~192 MB keep it in the last-level cache
         const size_t es = mfma_elem_size(precision);
         const size_t wtrip = 2 * (size_t)nb3 * es;
         long mb = 192l << 20;
@@ -439,31 +465,47 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         mbatch = std::max<long>(mbatch, 1);
         MfmaPlan mp;
         mp.na = na; mp.nb = nb; mp.nirr = nirr; mp.nfull = nfull; mp.maxa1 = std::max(maxa1, 1);
+        mp.maxblk = std::max(maxblk, 1);
+        mp.nty = mfma_temperature_block(nb, nT, precision);
+        if (mp.nty == 0) { cleanup(); return c->fail(5, "band count too large for the MFMA kernels' LDS"); }
         mp.ntrip = ntrip; mp.batch = mbatch;
         mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
         TRY(dalloc(&mp.E1re, es * (size_t)nirr * nb * nb));
         TRY(dalloc(&mp.E1im, es * (size_t)nirr * nb * nb));
         TRY(dalloc(&mp.Efre, es * (size_t)nfull * nb * nb));
         TRY(dalloc(&mp.Efim, es * (size_t)nfull * nb * nb));
+        mp.Eti = mp.Etf = nullptr;
+        if (precision == TC3_PREC_BF16X3) {
+            TRY(dalloc(&mp.Eti, 2 * mfma_split_elems(nb) * nirr));
+            TRY(dalloc(&mp.Etf, 2 * mfma_split_elems(nb) * nfull));
+        }
         TRY(dalloc(&mp.work, wtrip * mbatch));
-        mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off; mp.blk3 = d_blk3;
+        mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off;
+        mp.blist = d_blist; mp.a1_boff = d_a1boff; mp.a12_boff = d_a12boff;
         mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
         MfmaEpiArgs &me = mp.epi;
-        me.nb = nb; me.nT = nT; me.nirr = nirr; me.integrationtype = integrationtype;
-        me.cb = mfma_choose_chunk(nb, nT, precision, &mp.psi_shmem);
-        if (me.cb == 0) { cleanup(); return c->fail(5, "too many temperatures for one pass"); }
+        me.nb = nb; me.nirr = nirr; me.integrationtype = integrationtype; me.gT = nT;
         me.sigma_fixed = ea.sigma_fixed; me.psi_scale = ea.psi_scale;
         me.tq1 = d_tq1; me.tq2 = d_tq2; me.tq3 = d_tq3; me.eq_off = d_eqoff; me.eq_q2 = d_eq2; me.eq_q3 = d_eq3;
         me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
-        me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.be = d_be; me.sigsq = d_sigsq;
-        me.xi = c->d_xi; me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
+        me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
+        me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
         const long nbatch = (ntrip + mbatch - 1) / mbatch;
-        kev.resize(3 * nbatch);
+        const int ntchunk = (nT + mp.nty - 1) / mp.nty;
+        kev.resize(3 * nbatch * ntchunk);
         for (auto &x : kev) TRY(hipEventCreate(&x));
-        mp.ev = kev.data();
         TRY(hipEventRecord(ev0, st));
-        int r = run_mfma_chain(mp, precision, st);
-        if (r) { cleanup(); return r; }
+        // temperatures in blocks of mp.nty; each block recomputes the contraction
+        for (int tc = 0; tc < ntchunk; ++tc) {
+            const int T0 = tc * mp.nty;
+            me.T0 = T0;
+            me.nT = std::min(mp.nty, nT - T0);
+            me.be = d_be + (size_t)T0 * nirr * nb;
+            me.xi = c->d_xi + (size_t)T0 * xi_T;
+            mp.ev = kev.data() + 3 * nbatch * tc;
+            int r = run_mfma_chain(mp, precision, st);
+            if (r) { cleanup(); return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r; }
+        }
         c->batch = mbatch;
     } else {
     TRY(hipEventRecord(ev0, st));
