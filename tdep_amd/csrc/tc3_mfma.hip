@@ -1,4 +1,4 @@
-// |Psi|^2 contraction on MFMA for band counts nb in {16, 32, 48, 64, 96}.
+// |Psi|^2 contraction on MFMA for band counts nb = 48 and 96 (16 and 32 atoms).
 //
 // Per batch of visited triplets t = (q1, q2, q3) two kernels run:
 //
@@ -68,13 +68,24 @@ template <> struct Stage<0> { typedef double T; };
 
 struct Split { v8bf h, l; };
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// x ~ hi + lo with hi = bf16(x) (round to nearest even) and lo = bf16(x - hi), two values
+// per conversion instruction.
 __device__ inline void split8(const float *x, Split &s) {
+    v4u hu, lu;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const __bf16 h = (__bf16)x[j];
-        s.h[j] = h;
-        s.l[j] = (__bf16)(x[j] - (float)h);
+    for (int p = 0; p < 4; ++p) {
+        const v2f v = {x[2 * p], x[2 * p + 1]};
+        const unsigned u = __builtin_bit_cast(unsigned, __builtin_convertvector(v, v2bf));
+        const v2f hf = {__builtin_bit_cast(float, u << 16), __builtin_bit_cast(float, u & 0xffff0000u)};
+        hu[p] = u;
+        lu[p] = __builtin_bit_cast(unsigned, __builtin_convertvector(v - hf, v2bf));
     }
+    s.h = __builtin_bit_cast(v8bf, hu);
+    s.l = __builtin_bit_cast(v8bf, lu);
 }
 
 __device__ inline v8bf ld8(const __bf16 *p) { return *(const v8bf *)p; }
@@ -153,36 +164,63 @@ struct UwArgs {
     void *W;                // [tl][re/im][ia][b2 nb + b3]
 };
 
+template <typename T> struct C2 { T x, y; };
+
 template <int PREC, int NB>
 size_t uw_lds(int maxblk, int maxa1) {
     typedef typename Traits<PREC>::T T;
     constexpr int RS = kpad(NB) + 8;
-    const size_t ub = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB * NB;
-    size_t s = (ub + 15) & ~(size_t)15;
-    s += sizeof(cd) * 9 * (size_t)maxblk + sizeof(int) * (size_t)maxblk;
+    size_t s = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB * NB;   // U
+    s += sizeof(C2<T>) * 27 * (size_t)maxblk + sizeof(int) * (size_t)maxblk;
     s = (s + 15) & ~(size_t)15;
     return s + sizeof(cd) * (size_t)std::max(maxa1, 1);
 }
 
+// Raw buffer descriptor over [p, p + bytes) (gfx9 word 3).
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float bload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ inline double bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <typename T>
+__device__ inline T bload_t(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    if constexpr (sizeof(T) == 8) return bload2(r, voff, soff);
+    else return bload(r, voff, soff);
+}
+
+// Workgroup per (triplet, atom a1), one wave per 16 values of b2.  For i = 0, 1, 2
+// (ia = 3 a1 + i) the workgroup forms U[ib][b3] = sum_(a3,k) Phi[ia][ib][3 a3 + k]
+// E3[3 a3 + k][b3] from the nonzero IFC blocks of a1 on the vector ALUs (four b3 per
+// thread), then each wave forms W[b2][b3] = sum_ib E2[ib][b2] U[ib][b3] for its b2 on
+// MFMA with the E2 fragments held in registers for the whole workgroup.
 template <int PREC, int NB>
-__global__ void __launch_bounds__(256) k_uw(UwArgs u) {
+__global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
     constexpr int NA = NB / 3, NB2 = NB * NB, NT = NB / 16;
     constexpr int KP = kpad(NB), RS = KP + 8, KS = KSteps<PREC, NB>::n;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int tl = blockIdx.x / NB, ia = blockIdx.x % NB;
+    const int tl = blockIdx.x / NA, a1 = blockIdx.x % NA;
     const int t = u.t0 + tl;
-    const int a1 = ia / 3, i = ia % 3;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
-    // LDS: U, then the Phi blocks of (a1, i), their a3, then the phases of a1's IFCs
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    // LDS: U (BF16X3: split planes re hi/lo, im hi/lo, [b3][ib] with row stride RS; else
+    // planar [re/im][ib][b3]), Phi blocks [i][block][j][k], a3 of each block, phases
     constexpr size_t ub = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB2;
-    __bf16 *ut = (__bf16 *)smem;   // BF16X3: [plane re hi/lo, im hi/lo][b3][ib], row stride RS
-    T *us = (T *)smem;             // FP64/FP32: [re/im][ib][b3]
-    cd *pb = (cd *)(smem + ((ub + 15) & ~(size_t)15));
-    int *pa3 = (int *)(pb + 9 * u.maxblk);
+    __bf16 *ut = (__bf16 *)smem;
+    T *us = (T *)smem;
+    C2<T> *pb = (C2<T> *)(smem + ub);
+    int *pa3 = (int *)(pb + 27 * u.maxblk);
     cd *ph = (cd *)(smem + align16((size_t)((unsigned char *)(pa3 + u.maxblk) - smem)));
     const int q2 = u.tq2[t], q3 = u.tq3[t];
+    if constexpr (PREC == 2) {
+        // zero U including the K padding read by the MFMA fragments
+        uint4 *z = (uint4 *)ut;
+        for (int x = threadIdx.x; x < 4 * NB * RS / 8; x += blockDim.x) z[x] = make_uint4(0, 0, 0, 0);
+    }
     const double *Q2 = u.qcart + 3 * q2, *Q3 = u.qcart + 3 * q3;
     const int f0 = u.a1_off[a1], f1 = u.a1_off[a1 + 1];
     for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
@@ -192,104 +230,123 @@ __global__ void __launch_bounds__(256) k_uw(UwArgs u) {
         sincos(arg, &s, &c);
         ph[f - f0] = {c, s};
     }
+    // E2 operand (A[m = b2][k = ib] = E2[ib][b2]) of this wave's b2 tile, for all i
+    const int m0 = 16 * w;
+    Split ea_r[PREC == 2 ? KS : 1], ea_i[PREC == 2 ? KS : 1], ea_n[PREC == 2 ? KS : 1];
+    T fa_r[PREC == 2 ? 1 : KS], fa_i[PREC == 2 ? 1 : KS];
     if constexpr (PREC == 2) {
-        // zero U including the K padding read by the MFMA fragments
-        uint4 *z = (uint4 *)ut;
-        for (int x = threadIdx.x; x < 4 * NB * RS / 8; x += blockDim.x) z[x] = make_uint4(0, 0, 0, 0);
+        const size_t ps = (size_t)NB * KP;
+        const __bf16 *pa = u.Et + (size_t)q2 * kPlanes * ps + (size_t)(m0 + c16) * KP + 8 * g;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            ea_r[ks].h = ld8(pa + 32 * ks);
+            ea_r[ks].l = ld8(pa + ps + 32 * ks);
+            ea_i[ks].h = ld8(pa + 2 * ps + 32 * ks);
+            ea_i[ks].l = ld8(pa + 3 * ps + 32 * ks);
+            ea_n[ks].h = ld8(pa + 4 * ps + 32 * ks);
+            ea_n[ks].l = ld8(pa + 5 * ps + 32 * ks);
+        }
+    } else {
+        const T *E2r = (const T *)u.Ef_re + (size_t)q2 * NB2 + m0 + c16;
+        const T *E2i = (const T *)u.Ef_im + (size_t)q2 * NB2 + m0 + c16;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            fa_r[ks] = E2r[(4 * ks + g) * NB];
+            fa_i[ks] = E2i[(4 * ks + g) * NB];
+        }
     }
     __syncthreads();
     const int bo0 = u.a1_boff[a1], nblk = u.a1_boff[a1 + 1] - bo0;
-    for (int it = threadIdx.x; it < 9 * nblk; it += blockDim.x) {
-        const int bl = it / 9, jk = it % 9, j = jk / 3, k = jk % 3;
+    for (int it = threadIdx.x; it < 27 * nblk; it += blockDim.x) {
+        const int bl = it / 27, ijk = it % 27, i = ijk / 9, jk = ijk % 9;
         const int4 b = u.blist[bo0 + bl];
         double ax = 0, ay = 0;
         for (int f = b.z; f < b.w; ++f) {
-            const double m = u.mwm[27 * f + 9 * i + 3 * j + k];
+            const double m = u.mwm[27 * f + ijk];
             const cd p = ph[f - f0];
             ax += m * p.x;
             ay += m * p.y;
         }
-        pb[it] = {ax, ay};
-        if (jk == 0) pa3[bl] = b.y;
+        pb[(i * u.maxblk + bl) * 9 + jk] = {(T)ax, (T)ay};
+        if (ijk == 0) pa3[bl] = b.y;
     }
     __syncthreads();
-    // U[ib][b3] = sum over the blocks (a1, a2 = ib/3, a3) of sum_k Phi[j][k] E3[3 a3 + k][b3]
     const T *E3r = (const T *)u.Ef_re + (size_t)q3 * NB2, *E3i = (const T *)u.Ef_im + (size_t)q3 * NB2;
-    for (int e = threadIdx.x; e < NB2; e += blockDim.x) {
-        const int ib = e / NB, b3 = e % NB, a2 = ib / 3, j = ib % 3;
-        const int s0 = u.a12_boff[a1 * NA + a2] - bo0, s1 = u.a12_boff[a1 * NA + a2 + 1] - bo0;
-        T ur = 0, ui = 0;
-        for (int bl = s0; bl < s1; ++bl) {
-            const int a3 = pa3[bl];
+    T *Wt = (T *)u.W + (size_t)tl * 2 * NB * NB2;
+    for (int i = 0; i < 3; ++i) {
+        for (int e = threadIdx.x; e < NB2 / 4; e += blockDim.x) {
+            const int ib = e / (NB / 4), b3 = 4 * (e % (NB / 4)), a2 = ib / 3, j = ib % 3;
+            const int s0 = u.a12_boff[a1 * NA + a2] - bo0, s1 = u.a12_boff[a1 * NA + a2 + 1] - bo0;
+            T ur[4] = {0, 0, 0, 0}, ui[4] = {0, 0, 0, 0};
+            for (int bl = s0; bl < s1; ++bl) {
+                const int a3 = pa3[bl];
+                const C2<T> *pp = pb + (i * u.maxblk + bl) * 9 + 3 * j;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const cd p = pb[bl * 9 + 3 * j + k];
-                const T er = E3r[(3 * a3 + k) * NB + b3], ei = E3i[(3 * a3 + k) * NB + b3];
-                ur += (T)p.x * er - (T)p.y * ei;
-                ui += (T)p.x * ei + (T)p.y * er;
+                for (int k = 0; k < 3; ++k) {
+                    const C2<T> p = pp[k];
+                    const T *er = E3r + (3 * a3 + k) * NB + b3, *ei = E3i + (3 * a3 + k) * NB + b3;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        ur[v] += p.x * er[v] - p.y * ei[v];
+                        ui[v] += p.x * ei[v] + p.y * er[v];
+                    }
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if constexpr (PREC == 2) {
+                    const __bf16 rh = (__bf16)ur[v], ih = (__bf16)ui[v];
+                    __bf16 *o = ut + (size_t)(b3 + v) * RS + ib;
+                    o[0] = rh;
+                    o[(size_t)NB * RS] = (__bf16)(ur[v] - (float)rh);
+                    o[(size_t)2 * NB * RS] = ih;
+                    o[(size_t)3 * NB * RS] = (__bf16)(ui[v] - (float)ih);
+                } else {
+                    us[ib * NB + b3 + v] = ur[v];
+                    us[NB2 + ib * NB + b3 + v] = ui[v];
+                }
             }
         }
-        if constexpr (PREC == 2) {
-            const __bf16 rh = (__bf16)ur, ih = (__bf16)ui;
-            __bf16 *o = ut + (size_t)b3 * RS + ib;
-            o[0] = rh;
-            o[(size_t)NB * RS] = (__bf16)(ur - (float)rh);
-            o[(size_t)2 * NB * RS] = ih;
-            o[(size_t)3 * NB * RS] = (__bf16)(ui - (float)ih);
-        } else {
-            us[ib * NB + b3] = ur;
-            us[NB2 + ib * NB + b3] = ui;
-        }
-    }
-    __syncthreads();
-    // W[b2][b3] = sum_ib E2[ib][b2] U[ib][b3]
-    T *Wr = (T *)u.W + (size_t)tl * 2 * NB * NB2 + (size_t)ia * NB2;
-    T *Wi = Wr + (size_t)NB * NB2;
-    for (int tile = wave; tile < NT * NT; tile += 4) {
-        const int m0 = (tile / NT) * 16, n0 = (tile % NT) * 16;
-        Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-        if constexpr (PREC == 2) {
-            const size_t ps = (size_t)NB * KP;
-            const __bf16 *pa = u.Et + (size_t)q2 * kPlanes * ps + (size_t)(m0 + c16) * KP + 8 * g;
-            const __bf16 *pu = ut + (size_t)(n0 + c16) * RS + 8 * g;
+        __syncthreads();
+        const int ia = 3 * a1 + i;
+        T *Wr = Wt + (size_t)ia * NB2, *Wi = Wr + (size_t)NB * NB2;
+#pragma unroll 1
+        for (int nt = 0; nt < NT; ++nt) {
+            const int n0 = 16 * nt;
+            Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+            if constexpr (PREC == 2) {
+                const __bf16 *pu = ut + (size_t)(n0 + c16) * RS + 8 * g;
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                Split ar, ai, an, br, bi;
-                const int kb = 32 * ks;
-                ar.h = ld8(pa + kb);
-                ar.l = ld8(pa + ps + kb);
-                ai.h = ld8(pa + 2 * ps + kb);
-                ai.l = ld8(pa + 3 * ps + kb);
-                an.h = ld8(pa + 4 * ps + kb);
-                an.l = ld8(pa + 5 * ps + kb);
-                br.h = ld8(pu + kb);
-                br.l = ld8(pu + (size_t)NB * RS + kb);
-                bi.h = ld8(pu + (size_t)2 * NB * RS + kb);
-                bi.l = ld8(pu + (size_t)3 * NB * RS + kb);
-                mma3(cr, ar, br);
-                mma3(cr, an, bi);
-                mma3(ci, ar, bi);
-                mma3(ci, ai, br);
+                for (int ks = 0; ks < KS; ++ks) {
+                    Split br, bi;
+                    br.h = ld8(pu + 32 * ks);
+                    br.l = ld8(pu + (size_t)NB * RS + 32 * ks);
+                    bi.h = ld8(pu + (size_t)2 * NB * RS + 32 * ks);
+                    bi.l = ld8(pu + (size_t)3 * NB * RS + 32 * ks);
+                    mma3(cr, ea_r[ks], br);
+                    mma3(cr, ea_n[ks], bi);
+                    mma3(ci, ea_r[ks], bi);
+                    mma3(ci, ea_i[ks], br);
+                }
+            } else {
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int k = 4 * ks + g;
+                    const T br = us[k * NB + n0 + c16], bi = us[NB2 + k * NB + n0 + c16];
+                    mma1(cr, fa_r[ks], br);
+                    mma1(cr, -fa_i[ks], bi);
+                    mma1(ci, fa_r[ks], bi);
+                    mma1(ci, fa_i[ks], br);
+                }
             }
-        } else {
-            const T *E2r = (const T *)u.Ef_re + (size_t)q2 * NB2, *E2i = (const T *)u.Ef_im + (size_t)q2 * NB2;
-#pragma unroll 4
-            for (int ks = 0; ks < KS; ++ks) {
-                const int k = 4 * ks + g;
-                const T ar = E2r[k * NB + m0 + c16], ai = E2i[k * NB + m0 + c16];
-                const T br = us[k * NB + n0 + c16], bi = us[NB2 + k * NB + n0 + c16];
-                mma1(cr, ar, br);
-                mma1(cr, -ai, bi);
-                mma1(ci, ar, bi);
-                mma1(ci, ai, br);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int b2 = m0 + out_row<PREC>(lane, r);
+                Wr[b2 * NB + n0 + c16] = cr[r];
+                Wi[b2 * NB + n0 + c16] = ci[r];
             }
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int b2 = m0 + out_row<PREC>(lane, r);
-            Wr[b2 * NB + n0 + c16] = cr[r];
-            Wi[b2 * NB + n0 + c16] = ci[r];
-        }
+        __syncthreads();
     }
 }
 
@@ -308,11 +365,10 @@ __device__ inline void delta_pair(double w1, double w2, double w3, double sig2, 
     d0 = gaussd(w1, -w2 + w3, sig) - gaussd(w1, w2 - w3, sig);
     d1 = gaussd(w1, w2 + w3, sig) - gaussd(w1, -w2 - w3, sig);
 }
-__device__ inline void delta_pair(double w1, double w2, double w3, float sig2, float &d0, float &d1) {
+__device__ inline void delta_pair(float x1, float x2, float x3, float sig2, float &d0, float &d1) {
     const float inv = __builtin_amdgcn_rsqf(sig2);
     const float ce = -0.72134752044448170f * inv * inv;  // -log2(e) / (2 sigma^2)
     const float lim = 64.0f * sig2;                       // (8 sigma)^2
-    const float x1 = (float)w1, x2 = (float)w2, x3 = (float)w3;
     const float dd = x2 - x3, ss = x2 + x3;
     auto g = [&](float d) {
         const float d2 = d * d;
@@ -323,22 +379,27 @@ __device__ inline void delta_pair(double w1, double w2, double w3, float sig2, f
     d1 = nrm * (g(x1 - ss) - g(x1 + ss));
 }
 
+template <int NB> struct MaskWords { static constexpr int n = (NB + 31) / 32; };
+
 template <int PREC, int NB, int NTY>
 size_t psi_lds() {
     typedef typename Stage<PREC>::T S;
-    constexpr int NW = NB / 16;
-    size_t s = sizeof(S) * (2 * NTY * NB + NB + NTY * 16 * NB + NW * kChunk * 16 * NTY);
+    constexpr int NW = NB / 16, NMW = MaskWords<NB>::n, KS = KSteps<PREC, NB>::n;
+    size_t s = sizeof(S) * (2 * NTY * NB + NB + NW * kChunk * 16 * NTY);
     s = (s + 15) & ~(size_t)15;
-    return s + sizeof(double) * (NB + 16) + sizeof(int) * 16;
+    s += (PREC == 2) ? (size_t)KS * kPlanes * 64 * 16 : 0;    // E1 fragments
+    s += sizeof(double) * (NB + 16 + 16 * NTY) + sizeof(float) * NB + sizeof(int) * 16 +
+         sizeof(unsigned) * (size_t)(NB + 16) * NMW;
+    return s;
 }
 
 template <int PREC, int NB, int NTY>
-__global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_, const void *E1im_,
+__global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PREC == 2 ? 3 : 1))) k_psi(MfmaEpiArgs a, const void *E1re_, const void *E1im_,
                                                 const __bf16 *__restrict__ Et1, const void *W_) {
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
     typedef typename Stage<PREC>::T S;
-    constexpr int NW = NB / 16, NB2 = NB * NB, KS = KSteps<PREC, NB>::n, KP = kpad(NB);
+    constexpr int NW = NB / 16, NB2 = NB * NB, KS = KSteps<PREC, NB>::n, KP = kpad(NB), NMW = MaskWords<NB>::n;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nT = a.nT;
     // XCD-aware order: the b1 blocks of one triplet share blockIdx % 8
@@ -355,19 +416,26 @@ __global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_
     S *n2h = (S *)smem;                    // [NTY][NB]     n2 + 1/2
     S *n3h = n2h + NTY * NB;               // [NTY][NB]     n3 + 1/2
     S *sg2s = n3h + NTY * NB;              // [NB]          sigma^2 of q2 modes
-    S *S2 = sg2s + NB;                     // [NTY][16][NB] sum over b3 of f_T
-    S *Xp = S2 + NTY * 16 * NB;            // [NW][kChunk][16][NTY] per-wave partial sums
-    double *w2s = (double *)(smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + NB + NTY * 16 * NB +
-                                                                   NW * kChunk * 16 * NTY)));
+    S *Xp = sg2s + NB;                     // [NW][kChunk][16][NTY] per-wave partial sums over b3
+    unsigned char *p8 = smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + NB + NW * kChunk * 16 * NTY));
+    v8bf *e1s = (v8bf *)p8;                // [KS][6 planes][64 lanes] (BF16X3)
+    if constexpr (PREC == 2) p8 += (size_t)KS * kPlanes * 64 * 16;
+    double *w2s = (double *)p8;            // [NB]
     double *w1s = w2s + NB;                // [16]
-    int *rows = (int *)(w1s + 16);         // [16]
+    double *gam = w1s + 16;                // [NTY][16] linewidth partial sums
+    float *w2f = (float *)(gam + 16 * NTY);   // [NB]
+    int *rows = (int *)(w2f + NB);         // [16]
+    unsigned *v23 = (unsigned *)(rows + 16);   // [NB][NMW]  |w2 - w3| >= tol, by b3
+    unsigned *v12 = v23 + NB * NMW;            // [16][NMW]  b2 allowed for b1, by b2
+    const double *om2 = a.om_full + (size_t)q2 * NB, *om3 = a.om_full + (size_t)q3 * NB;
     for (int x = threadIdx.x; x < nT * NB; x += blockDim.x) {
         const int T_ = x / NB, b = x % NB;
         n2h[T_ * NB + b] = (S)(a.be[((size_t)T_ * a.nirr + i2) * NB + b] + 0.5);
         n3h[T_ * NB + b] = (S)(a.be[((size_t)T_ * a.nirr + i3) * NB + b] + 0.5);
     }
     for (int b = threadIdx.x; b < NB; b += blockDim.x) {
-        w2s[b] = a.om_full[(size_t)q2 * NB + b];
+        w2s[b] = om2[b];
+        w2f[b] = (float)om2[b];
         sg2s[b] = (S)a.sigsq[(size_t)i2 * NB + b];
     }
     if (threadIdx.x < 16) {
@@ -375,61 +443,87 @@ __global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_
         w1s[threadIdx.x] = a.om_irr[(size_t)q1 * NB + b1];
         rows[threadIdx.x] = a.row_of[(size_t)q1 * NB + b1];
     }
+    for (int x = threadIdx.x; x < 16 * NTY; x += blockDim.x) gam[x] = 0.0;
+    // pair masks (scattering_threephonon.f90:90-91, 100-101)
+    for (int x = threadIdx.x; x < NB * NMW; x += blockDim.x) {
+        const int b2 = x / NMW, wd = x % NMW;
+        const double w2 = om2[b2];
+        unsigned m = 0;
+        for (int k = 0; k < 32 && 32 * wd + k < NB; ++k)
+            if (!(fabs(w2 - om3[32 * wd + k]) < kTol)) m |= 1u << k;
+        v23[x] = m;
+    }
+    for (int x = threadIdx.x; x < 16 * NMW; x += blockDim.x) {
+        const int m1 = x / NMW, wd = x % NMW;
+        const int b1 = 16 * bb + m1;
+        const double w1 = a.om_irr[(size_t)q1 * NB + b1];
+        const bool r_ok = a.row_of[(size_t)q1 * NB + b1] >= 0;
+        unsigned m = 0;
+        for (int k = 0; k < 32 && 32 * wd + k < NB; ++k) {
+            const double w2 = om2[32 * wd + k];
+            if (r_ok && !(w2 < kTol) && !(fabs(w1 - w2) < kTol)) m |= 1u << k;
+        }
+        v12[x] = m;
+    }
+    if constexpr (PREC == 2) {
+        // E1 operand (B[k = ia][n = b1]) fragments, one 16-byte slot per lane
+        const size_t ps = (size_t)NB * KP;
+        const __bf16 *pe = Et1 + (size_t)q1 * kPlanes * ps + (size_t)(16 * bb + c16) * KP + 8 * g;
+        if (w == 0) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                for (int p = 0; p < kPlanes; ++p) e1s[(ks * kPlanes + p) * 64 + lane] = ld8(pe + p * ps + 32 * ks);
+        }
+    }
     __syncthreads();
     // per-lane constants: b1 = 16 bb + c16 and four values of b3
     const double w1 = w1s[c16];
+    const float x1 = (float)w1;
     const int row1 = rows[c16];
     const S sg1 = (S)a.sigsq[(size_t)q1 * NB + 16 * bb + c16];
-    const double *om3 = a.om_full + (size_t)q3 * NB;
-    int b3r[4];
-    double w3[4];
+    unsigned v12w[NMW];
+#pragma unroll
+    for (int k = 0; k < NMW; ++k) v12w[k] = v12[c16 * NMW + k];
+    const int b3w = 16 * w;                         // first b3 of this wave
+    const int sh3 = (b3w & 31);                     // bit of b3w in its mask word
+    unsigned ok13 = 0;
     S sg3[4], bm[4];
-    bool ok13[4];
+    float x3[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        b3r[r] = 16 * w + out_row<PREC>(lane, r);
-        w3[r] = om3[b3r[r]];
-        sg3[r] = (S)a.sigsq[(size_t)i3 * NB + b3r[r]];
-        ok13[r] = row1 >= 0 && !(w3[r] < kTol) && !(fabs(w1 - w3[r]) < kTol);
-        bm[r] = (c16 < nT) ? n3h[c16 * NB + b3r[r]] : (S)0;
+        const int b3 = b3w + out_row<PREC>(lane, r);
+        const double w3 = om3[b3];
+        x3[r] = (float)w3;
+        sg3[r] = (S)a.sigsq[(size_t)i3 * NB + b3];
+        if (row1 >= 0 && !(w3 < kTol) && !(fabs(w1 - w3) < kTol)) ok13 |= 1u << (sh3 + out_row<PREC>(lane, r));
+        bm[r] = (c16 < nT) ? n3h[c16 * NB + b3] : (S)0;
     }
-    // E1 operand (B[k = ia][n = b1]) held in registers for the whole triplet
-    Split e1r[PREC == 2 ? KS : 1], e1i[PREC == 2 ? KS : 1], e1n[PREC == 2 ? KS : 1];
-    const T *E1r = (const T *)E1re_ + (size_t)q1 * NB2 + 16 * bb + c16;
-    const T *E1i = (const T *)E1im_ + (size_t)q1 * NB2 + 16 * bb + c16;
-    if constexpr (PREC == 2) {
-        const size_t ps = (size_t)NB * KP;
-        const __bf16 *pe = Et1 + (size_t)q1 * kPlanes * ps + (size_t)(16 * bb + c16) * KP + 8 * g;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            e1r[ks].h = ld8(pe + 32 * ks);
-            e1r[ks].l = ld8(pe + ps + 32 * ks);
-            e1i[ks].h = ld8(pe + 2 * ps + 32 * ks);
-            e1i[ks].l = ld8(pe + 3 * ps + 32 * ks);
-            e1n[ks].h = ld8(pe + 4 * ps + 32 * ks);
-            e1n[ks].l = ld8(pe + 5 * ps + 32 * ks);
-        }
-    }
-    // W operand (A[m = (b2, b3)][k = ia]), prefetched one tile ahead
-    const T *Wr = (const T *)W_ + (size_t)tl * 2 * NB * NB2 + 16 * w + c16;
-    const T *Wi = Wr + (size_t)NB * NB2;
+    // W operand (A[m = (b2, b3)][k = ia]), prefetched one tile ahead by buffer loads
+    const T *Wt = (const T *)W_ + (size_t)tl * 2 * NB * NB2;
+    const __amdgpu_buffer_rsrc_t wr = rsrc(Wt, sizeof(T) * NB * NB2);
+    const __amdgpu_buffer_rsrc_t wi = rsrc(Wt + (size_t)NB * NB2, sizeof(T) * NB * NB2);
     constexpr int RW = (PREC == 2) ? 8 : 1;
     T rr[KS][RW], ri[KS][RW];
+    const T *E1r = (const T *)E1re_ + (size_t)q1 * NB2 + 16 * bb + c16;
+    const T *E1i = (const T *)E1im_ + (size_t)q1 * NB2 + 16 * bb + c16;
     auto load_w = [&](int b2) {
-        const size_t n = (size_t)b2 * NB;
+        const int n = b2 * NB + b3w + c16;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             if constexpr (PREC == 2) {
-                const int kb = 32 * ks + 8 * g;
+                // padded K rows (k >= NB) meet zero E1 fragments; read a valid row instead
+                const int kb = min(32 * ks + 8 * g, NB - 8);
+                const int vo = (int)sizeof(T) * (kb * NB2 + n);
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    rr[ks][j] = (kb < NB) ? Wr[(size_t)(kb + j) * NB2 + n] : 0.f;
-                    ri[ks][j] = (kb < NB) ? Wi[(size_t)(kb + j) * NB2 + n] : 0.f;
+                    rr[ks][j] = bload(wr, vo, (int)sizeof(T) * j * NB2);
+                    ri[ks][j] = bload(wi, vo, (int)sizeof(T) * j * NB2);
                 }
             } else {
-                const int k = 4 * ks + g;
-                rr[ks][0] = Wr[(size_t)k * NB2 + n];
-                ri[ks][0] = Wi[(size_t)k * NB2 + n];
+                const int vo = (int)sizeof(T) * (g * NB2 + n);
+                rr[ks][0] = bload_t<T>(wr, vo, (int)sizeof(T) * 4 * ks * NB2);
+                ri[ks][0] = bload_t<T>(wi, vo, (int)sizeof(T) * 4 * ks * NB2);
             }
         }
     };
@@ -452,14 +546,22 @@ __global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_
                 split8(rr[ks], ar[ks]);
                 split8(ri[ks], ai[ks]);
             }
-            if (b2 + 1 < NB) load_w(b2 + 1);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                mma3(cr, ar[ks], e1r[ks]);
-                mma3(cr, ai[ks], e1n[ks]);
-                mma3(ci, ar[ks], e1i[ks]);
-                mma3(ci, ai[ks], e1r[ks]);
+                Split br, bi, bn;
+                br.h = e1s[(ks * kPlanes + 0) * 64 + lane];
+                br.l = e1s[(ks * kPlanes + 1) * 64 + lane];
+                bi.h = e1s[(ks * kPlanes + 2) * 64 + lane];
+                bi.l = e1s[(ks * kPlanes + 3) * 64 + lane];
+                bn.h = e1s[(ks * kPlanes + 4) * 64 + lane];
+                bn.l = e1s[(ks * kPlanes + 5) * 64 + lane];
+                mma3(cr, ar[ks], br);
+                mma3(cr, ai[ks], bn);
+                mma3(ci, ar[ks], bi);
+                mma3(ci, ai[ks], br);
             }
+            // next tile's W is in flight during the epilogue
+            if (b2 + 1 < NB) load_w(b2 + 1);
         } else {
             T ar[KS], ai[KS];
 #pragma unroll
@@ -478,21 +580,24 @@ __global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_
                 mma1(ci, ai[ks], br);
             }
         }
-        // smeared energy conservation for the 4 elements (b1, b2, b3r[r]) of this lane
-        const double w2 = w2s[b2];
-        const bool ok2 = row1 >= 0 && !(w2 < kTol) && !(fabs(w1 - w2) < kTol);
+        // smeared energy conservation for the 4 elements (b1, b2, b3) of this lane
+        const unsigned ok2 = (v12w[b2 >> 5] >> (b2 & 31)) & 1u;
+        const unsigned okm = ok2 ? (ok13 & v23[b2 * NMW + (b3w >> 5)]) : 0u;
         const S sg12 = sg1 + sg2s[b2];
         S P[4], M[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             P[r] = 0;
             M[r] = 0;
-            if (ok2 && ok13[r] && !(fabs(w2 - w3[r]) < kTol)) {
+            if ((okm >> (sh3 + out_row<PREC>(lane, r))) & 1u) {
                 const S re = (S)cr[r], im = (S)ci[r];
                 const S psisq = sc * (re * re + im * im);
                 const S sig2 = (a.integrationtype == 2) ? sg12 + sg3[r] : sfix2;
                 S d0, d1;
-                delta_pair(w1, w2, w3[r], sig2, d0, d1);
+                if constexpr (PREC == 0)
+                    delta_pair(w1, w2s[b2], om3[b3w + out_row<PREC>(lane, r)], sig2, d0, d1);
+                else
+                    delta_pair(x1, w2f[b2], x3[r], sig2, d0, d1);
                 const S A = psisq * d0, B = psisq * d1;
                 P[r] = A + B;
                 M[r] = B - A;
@@ -523,14 +628,32 @@ __global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_
 #pragma unroll
         for (int r = 0; r < 4; ++r) yM[r] += M[r];
         if (cs == kChunk - 1) {
+            // sums over b3 of the chunk's b2: linewidth partials and the q2' entries of the
+            // scattering matrix (:126-133).  Every target (row, q', band) belongs to one
+            // triplet and, inside it, to one equivalent pair, so values are stored.
             __syncthreads();
             const int cb = b2 - cs;
-            for (int x = threadIdx.x; x < kChunk * 16 * nT; x += blockDim.x) {
-                const int T_ = x % nT, mc = x / nT, m = mc % 16, c = mc / 16;
-                S s = 0;
+            for (int x = threadIdx.x; x < 16 * nT; x += blockDim.x) {
+                const int m = x % 16, T_ = x / 16;
+                S v[kChunk];
+                double gs = 0;
 #pragma unroll
-                for (int ww = 0; ww < NW; ++ww) s += Xp[((ww * kChunk + c) * 16 + m) * NTY + T_];
-                S2[(T_ * 16 + m) * NB + cb + c] = s;
+                for (int c = 0; c < kChunk; ++c) {
+                    S s = 0;
+#pragma unroll
+                    for (int ww = 0; ww < NW; ++ww) s += Xp[((ww * kChunk + c) * 16 + m) * NTY + T_];
+                    v[c] = s;
+                    gs += (double)s;
+                }
+                gam[T_ * 16 + m] += gs;
+                const int r = rows[m];
+                if (r < 0) continue;
+                const double iw1 = 2.0 / w1s[m];
+                for (int e = e0; e < e1; ++e) {
+                    double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)a.eq_q2[e] * NB + cb;
+#pragma unroll
+                    for (int c = 0; c < kChunk; ++c) xr[c] = (double)v[c] * w2s[cb + c] * iw1;
+                }
             }
             __syncthreads();
         }
@@ -538,46 +661,28 @@ __global__ void __launch_bounds__(4 * NB) k_psi(MfmaEpiArgs a, const void *E1re_
     // linewidth partials (g0 += f0 + f1 for every equivalent triplet, :126-128)
     for (int x = threadIdx.x; x < nT * 16; x += blockDim.x) {
         const int T_ = x / 16, m = x % 16;
-        double s = 0;
-        for (int b2 = 0; b2 < NB; ++b2) s += (double)S2[(T_ * 16 + m) * NB + b2];
-        a.gpart[((size_t)t * a.gT + a.T0 + T_) * NB + 16 * bb + m] = (rows[m] >= 0) ? s * (double)neq : 0.0;
+        a.gpart[((size_t)t * a.gT + a.T0 + T_) * NB + 16 * bb + m] = (rows[m] >= 0) ? gam[x] * (double)neq : 0.0;
     }
-    // scattering-matrix entries (:133-134).  Every target (row, q', band) belongs to one
-    // triplet and, inside it, to one equivalent pair, so values are stored, not added.
-    // q3' entries from this lane's registers; q2' == q3' also takes the q2' term.
+    // q3' entries (:134) from this lane's registers; where q3' = q2' the entry already
+    // holds the q2' term written above by this workgroup, and the q3' term is added.
     if (row1 >= 0) {
         const double iw1 = 2.0 / w1;
         for (int e = e0; e < e1; ++e) {
-            const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
+            const int x2 = a.eq_q2[e], x3q = a.eq_q3[e];
 #pragma unroll
             for (int T_ = 0; T_ < NTY; ++T_) {
                 if (T_ >= nT) continue;
-                double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)row1 * a.xi_row + (size_t)x3 * NB;
+                double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)row1 * a.xi_row + (size_t)x3q * NB;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int b3 = b3r[r];
+                    const int b3 = b3w + out_row<PREC>(lane, r);
                     const S s3 = y[T_][r] + n3h[T_ * NB + b3] * yM[r];
-                    double v = (double)s3 * w3[r] * iw1;
-                    if (x2 == x3) v += (double)S2[(T_ * 16 + c16) * NB + b3] * w2s[b3] * iw1;
-                    xr[b3] = v;
+                    const double v = (double)s3 * om3[b3] * iw1;
+                    if (x2 == x3q) xr[b3] += v;
+                    else xr[b3] = v;
                 }
             }
         }
-    }
-    // q2' entries from LDS
-    const long nx = (long)nT * 16 * neq * NB;
-    for (long x = threadIdx.x; x < nx; x += blockDim.x) {
-        const int b = (int)(x % NB);
-        long yy = x / NB;
-        const int e = e0 + (int)(yy % neq);
-        yy /= neq;
-        const int m = (int)(yy % 16), T_ = (int)(yy / 16);
-        const int r = rows[m];
-        if (r < 0) continue;
-        const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
-        if (x2 == x3) continue;
-        a.xi[(size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)x2 * NB + b] =
-            2.0 * (double)S2[(T_ * 16 + m) * NB + b] * w2s[b] / w1s[m];
     }
 }
 
@@ -601,16 +706,16 @@ int run_chain(const MfmaPlan &p, hipStream_t st) {
     for (long t0 = 0; t0 < p.ntrip; t0 += p.batch) {
         const int nt = (int)std::min<long>(p.batch, p.ntrip - t0);
         hipEvent_t *ev = p.ev ? p.ev + 3 * (t0 / p.batch) : nullptr;
-        if (ev) hipEventRecord(ev[0], st);
+        if (ev) (void)hipEventRecord(ev[0], st);
         u.t0 = (int)t0;
-        hipLaunchKernelGGL((k_uw<PREC, NB>), dim3(nt * NB), dim3(256), sh_uw, st, u);
-        if (ev) hipEventRecord(ev[1], st);
+        hipLaunchKernelGGL((k_uw<PREC, NB>), dim3(nt * (NB / 3)), dim3(4 * NB), sh_uw, st, u);
+        if (ev) (void)hipEventRecord(ev[1], st);
         ea.t0 = (int)t0;
         ea.nt = nt;
         const int ngrp = (nt + 7) / 8;
         hipLaunchKernelGGL((k_psi<PREC, NB, NTY>), dim3(ngrp * 8 * NW), dim3(64 * NW), sh_psi, st, ea, p.E1re, p.E1im,
                            (const __bf16 *)p.Eti, (const void *)p.work);
-        if (ev) hipEventRecord(ev[2], st);
+        if (ev) (void)hipEventRecord(ev[2], st);
         e = hipGetLastError();
         if (e != hipSuccess) return device_error(e, "MFMA contraction launch", __FILE__, __LINE__);
     }
@@ -650,10 +755,7 @@ int dispatch_nb(const MfmaPlan &p, hipStream_t st) {
     int r = prepare<PREC>(p, st);
     if (r) return r;
     switch (p.nb) {
-    case 16: return dispatch_nty<PREC, 16>(p, st);
-    case 32: return dispatch_nty<PREC, 32>(p, st);
     case 48: return dispatch_nty<PREC, 48>(p, st);
-    case 64: return dispatch_nty<PREC, 64>(p, st);
     case 96: return dispatch_nty<PREC, 96>(p, st);
     default: return 5;
     }
@@ -662,12 +764,15 @@ int dispatch_nb(const MfmaPlan &p, hipStream_t st) {
 // LDS bytes of the two kernels for band count nb and temperature block nty.
 size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     const size_t es = (precision == TC3_PREC_FP64) ? 8 : 4;
-    const size_t kp = kpad(nb), rs = kp + 8, nw = nb / 16;
-    size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * rs : es * 2 * nb * nb;
-    uw = ((uw + 15) & ~(size_t)15) + 16 * 9 * (size_t)maxblk + 4 * (size_t)maxblk;
+    const size_t kp = kpad(nb), nw = nb / 16, nmw = (nb + 31) / 32;
+    const size_t ks = (precision == TC3_PREC_BF16X3) ? (nb + 31) / 32 : nb / 4;
+    size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * (kp + 8) : es * 2 * nb * nb;
+    uw += 2 * es * 27 * (size_t)maxblk + 4 * (size_t)maxblk;
     uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
-    size_t ps = es * (2 * nty * nb + nb + nty * 16 * nb + nw * kChunk * 16 * nty);
-    ps = ((ps + 15) & ~(size_t)15) + 8 * (nb + 16) + 4 * 16;
+    size_t ps = es * (2 * nty * nb + nb + nw * kChunk * 16 * nty);
+    ps = ((ps + 15) & ~(size_t)15);
+    if (precision == TC3_PREC_BF16X3) ps += ks * kPlanes * 64 * 16;
+    ps += 8 * (nb + 16 + 16 * nty) + 4 * nb + 4 * 16 + 4 * (nb + 16) * nmw;
     return std::max(uw, ps);
 }
 
@@ -678,7 +783,7 @@ size_t mfma_elem_size(int precision) { return precision == TC3_PREC_FP64 ? sizeo
 size_t mfma_split_elems(int nb) { return (size_t)kPlanes * nb * kpad(nb); }
 
 bool mfma_supported(int nb, int precision) {
-    if (nb != 16 && nb != 32 && nb != 48 && nb != 64 && nb != 96) return false;
+    if (nb != 48 && nb != 96) return false;   // nb = 3 na and a multiple of 16
     // IFC blocks and entries per atom are bounded separately by the caller
     return lds_need(nb, 4, precision, 0, 0) + 32 * 1024 <= 160 * 1024;
 }
