@@ -35,6 +35,7 @@ tc3_ctx::~tc3_ctx() {
     hipFree(d_egv_full);
     hipFree(d_xi);
     hipFree(d_irr_of_full);
+    for (auto &kv : pool) hipFree(kv.second.p);
     if (stream) hipStreamDestroy(stream);
 }
 
@@ -114,6 +115,7 @@ int tc3_set_structure(tc3_ctx *ctx, int na, const double *latvec, const double *
         ctx->nb = 3 * na;
         ctx->have_structure = true;
         ctx->have_fc2 = ctx->have_fc3 = ctx->have_mesh = ctx->have_harmonic = ctx->have_scatter = false;
+        ++ctx->gen_model;
         return 0;
     });
 }
@@ -143,6 +145,7 @@ int tc3_set_fc2(tc3_ctx *ctx, int npair, const int *a1, const int *a2, const dou
         }
         for (int b = 0; b < na * na; ++b) ctx->blk2_off[b + 1] += ctx->blk2_off[b];
         ctx->have_fc2 = true;
+        ++ctx->gen_model;
         ctx->have_harmonic = ctx->have_scatter = false;
         return 0;
     });
@@ -187,6 +190,7 @@ int tc3_set_fc3(tc3_ctx *ctx, int ntrip, const int *a1, const int *a2, const int
         for (size_t b = 0; b < (size_t)na * na * na; ++b) ctx->blk3_off[b + 1] += ctx->blk3_off[b];
         for (int a = 0; a < na; ++a) ctx->a1_off[a + 1] += ctx->a1_off[a];
         ctx->have_fc3 = true;
+        ++ctx->gen_model;
         ctx->have_scatter = false;
         return 0;
     });
@@ -204,6 +208,7 @@ int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal) {
         s = build_orbits(ctx->mesh, ctx->orb);
         if (s.code) return ctx->fail(s.code, s.msg);
         ctx->have_mesh = true;
+        ++ctx->gen_model;
         ctx->have_harmonic = ctx->have_scatter = false;
         return 0;
     });

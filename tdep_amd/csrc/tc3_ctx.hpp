@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -71,7 +72,32 @@ struct tc3_ctx {
     tc3::cd *d_egv_irr = nullptr, *d_egv_full = nullptr;
     double *d_xi = nullptr;     // [T][row][q*nb + b]
     size_t xi_row = 0;          // nfull*nb
+    size_t xi_bytes = 0;        // allocated size of d_xi
+    long xi_gen = -1;           // model generation d_xi was zeroed for
     int *d_irr_of_full = nullptr;
+
+    // Device buffers kept across calls, by name (grown on demand, freed with the context),
+    // and the model generation: bumped by every structure / IFC / mesh change so that
+    // tables uploaded for an older model are uploaded again.
+    struct DevBuf { void *p = nullptr; size_t bytes = 0; };
+    std::map<std::string, DevBuf> pool;
+    long gen_model = 0;
+    long scatter_tables_gen = -1;
+    hipError_t pool_get(const std::string &name, size_t bytes, void **p, bool *fresh = nullptr) {
+        DevBuf &b = pool[name];
+        if (fresh) *fresh = false;
+        if (b.bytes < bytes || b.p == nullptr) {
+            if (b.p) (void)hipFree(b.p);
+            b.p = nullptr;
+            b.bytes = 0;
+            const hipError_t e = hipMalloc(&b.p, bytes < 16 ? 16 : bytes);
+            if (e != hipSuccess) { b.p = nullptr; return e; }
+            b.bytes = bytes;
+            if (fresh) *fresh = true;
+        }
+        *p = b.p;
+        return hipSuccess;
+    }
 
     ~tc3_ctx();
     int fail(int code, const std::string &msg) {

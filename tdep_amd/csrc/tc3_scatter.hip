@@ -242,9 +242,11 @@ __global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_q
     lw[i] = s;
 }
 
-// Orbit averaging of one scattering-matrix row: symmetrize_and_distribute (positive
-// entries only, scattering_threephonon.f90:156-196) followed by symmetrize_scatmat
-// (scattering.f90:312-354).  Workgroup per (row, T).
+// Orbit averaging of one scattering-matrix row: symmetrize_and_distribute (mean of the
+// positive entries broadcast to the orbit, scattering_threephonon.f90:156-196) followed by
+// symmetrize_scatmat (plain mean, scattering.f90:312-354).  After the first step an orbit
+// with a positive entry holds one value, so the second step is evaluated on that value in
+// registers; each entry is read once and written once.  Workgroup per (row, T).
 __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const int *__restrict__ q1_off,
                          const int *__restrict__ orb_off, const int *__restrict__ members,
                          double *__restrict__ xi, size_t xi_row, size_t xi_T) {
@@ -256,20 +258,21 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
     for (long w = threadIdx.x; w < work; w += blockDim.x) {
         const int o = o0 + (int)(w / nb), b = (int)(w % nb);
         const int m0 = orb_off[o], m1 = orb_off[o + 1];
-        double s = 0;
+        double sp = 0, sa = 0;
         int n = 0;
         for (int m = m0; m < m1; ++m) {
             const double v = x[(size_t)members[m] * nb + b];
-            if (v > 0.0) { s += v; ++n; }
+            sa += v;
+            if (v > 0.0) { sp += v; ++n; }
         }
+        double s = sa;
         if (n > 0) {
-            const double avg = s / n;
-            for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = avg;
+            const double avg = sp / n;
+            s = 0;
+            for (int m = m0; m < m1; ++m) s += avg;
         }
-        s = 0;
-        for (int m = m0; m < m1; ++m) s += x[(size_t)members[m] * nb + b];
-        const double avg = s / (m1 - m0);
-        for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = avg;
+        const double out = s / (m1 - m0);
+        for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = out;
     }
 }
 
@@ -308,99 +311,100 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     const int nrows = c->nrows;
     hipStream_t st = c->stream;
     const auto wall0 = std::chrono::steady_clock::now();
-    // device tables
+    // device tables: kept in the context's pool, uploaded again only when the model changed
     std::vector<void *> tmp;
-    auto dalloc = [&](void **p, size_t n) -> hipError_t {
-        hipError_t e = hipMalloc(p, std::max<size_t>(n, 16));
-        if (e == hipSuccess) tmp.push_back(*p);
-        return e;
-    };
     auto cleanup = [&]() {
         for (void *p : tmp) hipFree(p);
         tmp.clear();
     };
     hipError_t e = hipSuccess;
 #define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
+    const bool upload = c->scatter_tables_gen != c->gen_model;
+    auto table = [&](const char *name, void **p, const void *src, size_t bytes) -> hipError_t {
+        bool fresh = false;
+        hipError_t r = c->pool_get(name, bytes, p, &fresh);
+        if (r == hipSuccess && (upload || fresh) && bytes > 0)
+            r = hipMemcpyAsync(*p, src, bytes, hipMemcpyHostToDevice, st);
+        return r;
+    };
     int *d_tq1, *d_tq2, *d_tq3, *d_eqoff, *d_eq2, *d_eq3, *d_rowof, *d_rowq, *d_rowb, *d_q1off;
     int *d_orboff, *d_orbq1, *d_members, *d_a1off, *d_blk3;
     double *d_perm, *d_qcart, *d_be, *d_sigsq, *d_lv2, *d_lv3, *d_mwm, *d_gpart, *d_lw;
-    TRY(dalloc((void **)&d_tq1, sizeof(int) * ntrip));
-    TRY(dalloc((void **)&d_tq2, sizeof(int) * ntrip));
-    TRY(dalloc((void **)&d_tq3, sizeof(int) * ntrip));
-    TRY(dalloc((void **)&d_perm, sizeof(double) * ntrip));
-    TRY(dalloc((void **)&d_eqoff, sizeof(int) * (ntrip + 1)));
-    TRY(dalloc((void **)&d_eq2, sizeof(int) * c->trip.eq_q2.size()));
-    TRY(dalloc((void **)&d_eq3, sizeof(int) * c->trip.eq_q3.size()));
-    TRY(dalloc((void **)&d_rowof, sizeof(int) * nirr * nb));
-    TRY(dalloc((void **)&d_rowq, sizeof(int) * nrows));
-    TRY(dalloc((void **)&d_rowb, sizeof(int) * nrows));
-    TRY(dalloc((void **)&d_q1off, sizeof(int) * (nirr + 1)));
-    TRY(dalloc((void **)&d_orboff, sizeof(int) * c->orb.orb_off.size()));
-    TRY(dalloc((void **)&d_orbq1, sizeof(int) * c->orb.q1_off.size()));
-    TRY(dalloc((void **)&d_members, sizeof(int) * c->orb.members.size()));
-    TRY(dalloc((void **)&d_a1off, sizeof(int) * (na + 1)));
-    TRY(dalloc((void **)&d_blk3, sizeof(int) * ((size_t)na * na * na + 1)));
-    TRY(dalloc((void **)&d_qcart, sizeof(double) * 3 * nfull));
-    TRY(dalloc((void **)&d_be, sizeof(double) * nT * nirr * nb));
-    TRY(dalloc((void **)&d_sigsq, sizeof(double) * nirr * nb));
-    TRY(dalloc((void **)&d_lv2, sizeof(double) * 3 * c->nfc3));
-    TRY(dalloc((void **)&d_lv3, sizeof(double) * 3 * c->nfc3));
-    TRY(dalloc((void **)&d_mwm, sizeof(double) * 27 * c->nfc3));
-    TRY(dalloc((void **)&d_gpart, sizeof(double) * ntrip * nT * nb));
-    TRY(dalloc((void **)&d_lw, sizeof(double) * nrows * nT));
-    TRY(hipMemcpyAsync(d_tq1, c->trip.q1.data(), sizeof(int) * ntrip, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_tq2, c->trip.q2.data(), sizeof(int) * ntrip, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_tq3, c->trip.q3.data(), sizeof(int) * ntrip, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_perm, c->trip.perm.data(), sizeof(double) * ntrip, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_eqoff, c->trip.eq_off.data(), sizeof(int) * (ntrip + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_eq2, c->trip.eq_q2.data(), sizeof(int) * c->trip.eq_q2.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_eq3, c->trip.eq_q3.data(), sizeof(int) * c->trip.eq_q3.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_rowof, c->row_of.data(), sizeof(int) * nirr * nb, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_rowq, c->row_q.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_rowb, c->row_b.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_q1off, c->trip.q1_off.data(), sizeof(int) * (nirr + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_orboff, c->orb.orb_off.data(), sizeof(int) * c->orb.orb_off.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_orbq1, c->orb.q1_off.data(), sizeof(int) * c->orb.q1_off.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_a1off, c->a1_off.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_blk3, c->blk3_off.data(), sizeof(int) * ((size_t)na * na * na + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_qcart, c->mesh.qcart.data(), sizeof(double) * 3 * nfull, hipMemcpyHostToDevice, st));
+    TRY(table("tq1", (void **)&d_tq1, c->trip.q1.data(), sizeof(int) * ntrip));
+    TRY(table("tq2", (void **)&d_tq2, c->trip.q2.data(), sizeof(int) * ntrip));
+    TRY(table("tq3", (void **)&d_tq3, c->trip.q3.data(), sizeof(int) * ntrip));
+    TRY(table("perm", (void **)&d_perm, c->trip.perm.data(), sizeof(double) * ntrip));
+    TRY(table("eqoff", (void **)&d_eqoff, c->trip.eq_off.data(), sizeof(int) * (ntrip + 1)));
+    TRY(table("eq2", (void **)&d_eq2, c->trip.eq_q2.data(), sizeof(int) * c->trip.eq_q2.size()));
+    TRY(table("eq3", (void **)&d_eq3, c->trip.eq_q3.data(), sizeof(int) * c->trip.eq_q3.size()));
+    TRY(table("rowof", (void **)&d_rowof, c->row_of.data(), sizeof(int) * nirr * nb));
+    TRY(table("rowq", (void **)&d_rowq, c->row_q.data(), sizeof(int) * nrows));
+    TRY(table("rowb", (void **)&d_rowb, c->row_b.data(), sizeof(int) * nrows));
+    TRY(table("q1off", (void **)&d_q1off, c->trip.q1_off.data(), sizeof(int) * (nirr + 1)));
+    TRY(table("orboff", (void **)&d_orboff, c->orb.orb_off.data(), sizeof(int) * c->orb.orb_off.size()));
+    TRY(table("orbq1", (void **)&d_orbq1, c->orb.q1_off.data(), sizeof(int) * c->orb.q1_off.size()));
+    TRY(table("members", (void **)&d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size()));
+    TRY(table("a1off", (void **)&d_a1off, c->a1_off.data(), sizeof(int) * (na + 1)));
+    TRY(table("blk3", (void **)&d_blk3, c->blk3_off.data(), sizeof(int) * ((size_t)na * na * na + 1)));
+    TRY(table("qcart", (void **)&d_qcart, c->mesh.qcart.data(), sizeof(double) * 3 * nfull));
+    TRY(table("lv2", (void **)&d_lv2, c->t_lv2.data(), sizeof(double) * 3 * c->nfc3));
+    TRY(table("lv3", (void **)&d_lv3, c->t_lv3.data(), sizeof(double) * 3 * c->nfc3));
+    TRY(table("mwm", (void **)&d_mwm, c->t_mwm.data(), sizeof(double) * 27 * c->nfc3));
+    // per call: occupations, smearing, partial sums
+    TRY(c->pool_get("be", sizeof(double) * nT * nirr * nb, (void **)&d_be));
+    TRY(c->pool_get("sigsq", sizeof(double) * nirr * nb, (void **)&d_sigsq));
+    TRY(c->pool_get("gpart", sizeof(double) * ntrip * nT * nb, (void **)&d_gpart));
+    TRY(c->pool_get("lw", sizeof(double) * nrows * nT, (void **)&d_lw));
     TRY(hipMemcpyAsync(d_be, c->be.data(), sizeof(double) * nT * nirr * nb, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_sigsq, c->sigsq.data(), sizeof(double) * nirr * nb, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_lv2, c->t_lv2.data(), sizeof(double) * 3 * c->nfc3, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_lv3, c->t_lv3.data(), sizeof(double) * 3 * c->nfc3, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_mwm, c->t_mwm.data(), sizeof(double) * 27 * c->nfc3, hipMemcpyHostToDevice, st));
-    // scattering matrix
-    hipFree(c->d_xi);
-    c->d_xi = nullptr;
+    // scattering matrix: every entry that any triplet owns is rewritten by every pass, so
+    // the buffer is zeroed only when it is new or the model changed
     c->xi_row = (size_t)nfull * nb;
     const size_t xi_T = c->xi_row * nrows;
-    TRY(hipMalloc(&c->d_xi, sizeof(double) * xi_T * nT));
-    TRY(hipMemsetAsync(c->d_xi, 0, sizeof(double) * xi_T * nT, st));
+    if (c->xi_bytes < sizeof(double) * xi_T * nT) {
+        hipFree(c->d_xi);
+        c->d_xi = nullptr;
+        c->xi_bytes = 0;
+        TRY(hipMalloc(&c->d_xi, sizeof(double) * xi_T * nT));
+        c->xi_bytes = sizeof(double) * xi_T * nT;
+        c->xi_gen = -1;
+    }
+    if (c->xi_gen != c->gen_model) {
+        TRY(hipMemsetAsync(c->d_xi, 0, c->xi_bytes, st));
+        c->xi_gen = c->gen_model;
+    }
+    c->scatter_tables_gen = c->gen_model;
 
-    // batch size: intermediates of 4 * nb^3 complex per triplet
+    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
+    if (precision == TC3_PREC_BF16X3 && !use_mfma) {
+        cleanup();
+        return c->fail(5, "split-BF16x3 needs 16 or 32 atoms (48 or 96 bands)");
+    }
+    // generic path: batch of intermediates of 4 * nb^3 complex per triplet
     const long nb3 = (long)nb * nb * nb;
     const size_t elem = (precision == TC3_PREC_FP64) ? sizeof(cd) : sizeof(cf);
     const size_t per_trip = 4 * (size_t)nb3 * elem;
-    long batch = std::max<long>(1, std::min<long>(ntrip, (long)((size_t)6 << 30) / (long)per_trip));
+    long batch = std::max<long>(1, std::min<long>(ntrip, (long)((size_t)2 << 30) / (long)per_trip));
     batch = std::min<long>(batch, 1 << 20);
     void *d_work = nullptr;
     void *d_E1 = nullptr, *d_Ef = nullptr;
-    TRY(dalloc(&d_work, per_trip * batch));
-    TRY(dalloc(&d_E1, elem * (size_t)nirr * nb * nb));
-    TRY(dalloc(&d_Ef, elem * (size_t)nfull * nb * nb));
-    if (precision == TC3_PREC_FP64) {
-        hipLaunchKernelGGL(k_scale<double>, dim3(1024), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
-                           c->d_om_irr, (cplx<double> *)d_E1);
-        hipLaunchKernelGGL(k_scale<double>, dim3(4096), dim3(256), 0, st, (long)nfull * nb * nb, nb, c->d_egv_full,
-                           c->d_om_full, (cplx<double> *)d_Ef);
-    } else {
-        hipLaunchKernelGGL(k_scale<float>, dim3(1024), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
-                           c->d_om_irr, (cplx<float> *)d_E1);
-        hipLaunchKernelGGL(k_scale<float>, dim3(4096), dim3(256), 0, st, (long)nfull * nb * nb, nb, c->d_egv_full,
-                           c->d_om_full, (cplx<float> *)d_Ef);
+    if (!use_mfma) {
+        TRY(c->pool_get("g_work", per_trip * batch, &d_work));
+        TRY(c->pool_get("g_E1", elem * (size_t)nirr * nb * nb, &d_E1));
+        TRY(c->pool_get("g_Ef", elem * (size_t)nfull * nb * nb, &d_Ef));
+        if (precision == TC3_PREC_FP64) {
+            hipLaunchKernelGGL(k_scale<double>, dim3(1024), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
+                               c->d_om_irr, (cplx<double> *)d_E1);
+            hipLaunchKernelGGL(k_scale<double>, dim3(4096), dim3(256), 0, st, (long)nfull * nb * nb, nb, c->d_egv_full,
+                               c->d_om_full, (cplx<double> *)d_Ef);
+        } else {
+            hipLaunchKernelGGL(k_scale<float>, dim3(1024), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
+                               c->d_om_irr, (cplx<float> *)d_E1);
+            hipLaunchKernelGGL(k_scale<float>, dim3(4096), dim3(256), 0, st, (long)nfull * nb * nb, nb, c->d_egv_full,
+                               c->d_om_full, (cplx<float> *)d_Ef);
+        }
+        TRY(hipGetLastError());
     }
-    TRY(hipGetLastError());
     int maxa1 = 0;
     for (int a = 0; a < na; ++a) maxa1 = std::max(maxa1, c->a1_off[a + 1] - c->a1_off[a]);
     const size_t phi_shmem = sizeof(cd) * std::max(maxa1, 1);
@@ -418,11 +422,6 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     hipEvent_t ev0, ev1;
     TRY(hipEventCreate(&ev0));
     TRY(hipEventCreate(&ev1));
-    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
-    if (precision == TC3_PREC_BF16X3 && !use_mfma) {
-        cleanup();
-        return c->fail(5, "split-BF16x3 needs a band count that is a multiple of 16");
-    }
     std::vector<hipEvent_t> kev;
     for (double &x : c->kstat_ms) x = 0;
     for (int &x : c->kstat_n) x = 0;
@@ -447,13 +446,9 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         a12_boff[(size_t)na * na] = (int)blist.size();
         int4 *d_blist;
         int *d_a1boff, *d_a12boff;
-        TRY(dalloc((void **)&d_blist, sizeof(int4) * std::max<size_t>(blist.size(), 1)));
-        TRY(dalloc((void **)&d_a1boff, sizeof(int) * a1_boff.size()));
-        TRY(dalloc((void **)&d_a12boff, sizeof(int) * a12_boff.size()));
-        if (!blist.empty())
-            TRY(hipMemcpyAsync(d_blist, blist.data(), sizeof(int4) * blist.size(), hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_a1boff, a1_boff.data(), sizeof(int) * a1_boff.size(), hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_a12boff, a12_boff.data(), sizeof(int) * a12_boff.size(), hipMemcpyHostToDevice, st));
+        TRY(table("blist", (void **)&d_blist, blist.data(), sizeof(int4) * blist.size()));
+        TRY(table("a1boff", (void **)&d_a1boff, a1_boff.data(), sizeof(int) * a1_boff.size()));
+        TRY(table("a12boff", (void **)&d_a12boff, a12_boff.data(), sizeof(int) * a12_boff.size()));
         // W of one triplet: 2 * nb^3 reals; batches of ~192 MB keep it in the last-level cache
         const size_t es = mfma_elem_size(precision);
         const size_t wtrip = 2 * (size_t)nb3 * es;
@@ -470,16 +465,16 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         if (mp.nty == 0) { cleanup(); return c->fail(5, "band count too large for the MFMA kernels' LDS"); }
         mp.ntrip = ntrip; mp.batch = mbatch;
         mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
-        TRY(dalloc(&mp.E1re, es * (size_t)nirr * nb * nb));
-        TRY(dalloc(&mp.E1im, es * (size_t)nirr * nb * nb));
-        TRY(dalloc(&mp.Efre, es * (size_t)nfull * nb * nb));
-        TRY(dalloc(&mp.Efim, es * (size_t)nfull * nb * nb));
+        TRY(c->pool_get("m_E1re", es * (size_t)nirr * nb * nb, &mp.E1re));
+        TRY(c->pool_get("m_E1im", es * (size_t)nirr * nb * nb, &mp.E1im));
+        TRY(c->pool_get("m_Efre", es * (size_t)nfull * nb * nb, &mp.Efre));
+        TRY(c->pool_get("m_Efim", es * (size_t)nfull * nb * nb, &mp.Efim));
         mp.Eti = mp.Etf = nullptr;
         if (precision == TC3_PREC_BF16X3) {
-            TRY(dalloc(&mp.Eti, 2 * mfma_split_elems(nb) * nirr));
-            TRY(dalloc(&mp.Etf, 2 * mfma_split_elems(nb) * nfull));
+            TRY(c->pool_get("m_Eti", 2 * mfma_split_elems(nb) * nirr, &mp.Eti));
+            TRY(c->pool_get("m_Etf", 2 * mfma_split_elems(nb) * nfull, &mp.Etf));
         }
-        TRY(dalloc(&mp.work, wtrip * mbatch));
+        TRY(c->pool_get("m_work", wtrip * mbatch, &mp.work));
         mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off;
         mp.blist = d_blist; mp.a1_boff = d_a1boff; mp.a12_boff = d_a12boff;
         mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
