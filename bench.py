@@ -56,12 +56,17 @@ def scatter_pass(eng, temps, prec, steps, warmup):
 
 
 def kappa_wall(eng, temps, prec):
-    """harmonic + scattering + RTA + iterative BTE for all temperatures (no file I/O)."""
+    """harmonic + scattering + RTA + iterative BTE for all temperatures (no file I/O).
+    Returns the wall time, kappa (SMA, iterative) and the engine's own stage times."""
     t0 = time.perf_counter()
     eng.dispersions()
     eng.scattering_rates(temps, precision=prec)
     ks, ki, _ = eng.kappa(offdiag=False)
-    return time.perf_counter() - t0, ks, ki
+    wall = time.perf_counter() - t0
+    tm = eng.timings()
+    stages = {k: float(tm[k]) for k in ("harmonic", "scatter", "kappa", "bte")}
+    stages["bte_iterations"] = [int(x) for x in eng.niter]
+    return wall, ks, ki, stages
 
 
 def cpu_baseline(indir, grid, temperature, budget_s=15.0):
@@ -120,8 +125,8 @@ def main():
         ks_64 = kernel_stats(eng)
         log(f"[bench] fp64: {t_64 * 1e3:.2f} ms/step  kernels {ks_64}")
         # kappa wall time, both modes
-        kw_mx, ksma_mx, kit_mx = kappa_wall(eng, temps, args.precision)
-        kw_64, ksma_64, kit_64 = kappa_wall(eng, temps, "fp64")
+        kw_mx, ksma_mx, kit_mx, st_mx = kappa_wall(eng, temps, args.precision)
+        kw_64, ksma_64, kit_64, st_64 = kappa_wall(eng, temps, "fp64")
         err = float(np.max(np.abs(kit_mx - kit_64)) / np.max(np.abs(kit_64)))
         err_sma = float(np.max(np.abs(ksma_mx - ksma_64)) / np.max(np.abs(ksma_64)))
         log(f"[bench] kappa wall {args.precision} {kw_mx:.3f}s fp64 {kw_64:.3f}s rel err {err:.2e}")
@@ -159,7 +164,8 @@ def main():
                        "precision": args.precision},
             "fp64": {"value": elements / t_64, "ms_per_step": t_64 * 1e3},
             "speedup_vs_fp64": t_64 / t_mx,
-            "kappa_wall_s": {"mixed": kw_mx, "fp64": kw_64, "speedup": kw_64 / kw_mx},
+            "kappa_wall_s": {"mixed": kw_mx, "fp64": kw_64, "speedup": kw_64 / kw_mx,
+                             "stages_mixed": st_mx, "stages_fp64": st_64},
             "kappa_rel_err_vs_fp64": {"iterative": err, "sma": err_sma},
             "kappa_xx_W_mK_300K": {"mixed": float(kit_mx[2][0, 0] * KAPPA_AU_TO_SI),
                                    "fp64": float(kit_64[2][0, 0] * KAPPA_AU_TO_SI)},

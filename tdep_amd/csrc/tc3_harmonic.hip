@@ -93,10 +93,14 @@ __global__ void k_dynmat(int na, const double *__restrict__ qv, const int *__res
 // (circle method) gives nb/2 disjoint (p,q) rotations per round; all rotations of a
 // round are applied to the columns, then to the rows, then to V.
 __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, double *__restrict__ lam,
-                         int maxsweep, int *__restrict__ status) {
+                         int maxsweep, int *__restrict__ status, int use_lds) {
     const int q = blockIdx.x;
-    cd *A = Aall + (size_t)q * nb * nb;
-    cd *V = Vall + (size_t)q * nb * nb;
+    // A and V live in LDS when they fit (use_lds), else in global memory
+    extern __shared__ cd jsm[];
+    cd *A = use_lds ? jsm : Aall + (size_t)q * nb * nb;
+    cd *V = use_lds ? jsm + (size_t)nb * nb : Vall + (size_t)q * nb * nb;
+    if (use_lds)
+        for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) A[i] = Aall[(size_t)q * nb * nb + i];
     const int mpad = nb + (nb & 1);
     const int half = mpad / 2;
     __shared__ int sp[128], sq[128];
@@ -158,9 +162,11 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
                 sp[k] = p; sq[k] = qq; sc[k] = c; ss[k] = s; sphx[k] = ex; sphy[k] = ey;
             }
             __syncthreads();
-            // columns: A <- A U, V <- V U with U = diag(1, e^{-i phi}) [[c, s], [-s, c]]
-            for (int idx = threadIdx.x; idx < half * nb; idx += blockDim.x) {
-                const int k = idx / nb, row = idx % nb;
+            // columns: A <- A U, V <- V U with U = diag(1, e^{-i phi}) [[c, s], [-s, c]];
+            // wave w takes the pairs k = w, w + 4, ..., its lanes the rows
+            const int lane = threadIdx.x & 63, nwv = blockDim.x >> 6;
+            for (int k = threadIdx.x >> 6; k < half; k += nwv)
+            for (int row = lane; row < nb; row += 64) {
                 const int p = sp[k], qq = sq[k];
                 if (qq < 0 || ss[k] == 0.0) continue;
                 const double c = sc[k], s = ss[k];
@@ -176,8 +182,8 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
             }
             __syncthreads();
             // rows: A <- U^H A
-            for (int idx = threadIdx.x; idx < half * nb; idx += blockDim.x) {
-                const int k = idx / nb, col = idx % nb;
+            for (int k = threadIdx.x >> 6; k < half; k += nwv)
+            for (int col = lane; col < nb; col += 64) {
                 const int p = sp[k], qq = sq[k];
                 if (qq < 0 || ss[k] == 0.0) continue;
                 const double c = sc[k], s = ss[k];
@@ -191,6 +197,8 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
         }
     }
     for (int i = threadIdx.x; i < nb; i += blockDim.x) lam[(size_t)q * nb + i] = A[(size_t)i * nb + i].x;
+    if (use_lds)
+        for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) Vall[(size_t)q * nb * nb + i] = V[i];
     if (threadIdx.x == 0) status[q] = (sweep >= maxsweep) ? 1 : 0;
 }
 
@@ -240,13 +248,16 @@ __device__ void small_eigh(int k, cd *M, cd *W, double *ev, int *ord) {
 __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__restrict__ Vin,
                        const cd *__restrict__ gD, const int *__restrict__ at_gamma,
                        double *__restrict__ omega, cd *__restrict__ egv, double *__restrict__ vel,
-                       cd *__restrict__ scratch, double veltol) {
+                       cd *__restrict__ scratch, double veltol, int use_lds) {
     const int q = blockIdx.x;
     __shared__ double lam[128], om[128], hf[3][128];
     __shared__ int ord[128], grp[129];
     __shared__ int ngrp;
+    // eigenvectors in LDS when they fit (use_lds), else in the output array
+    extern __shared__ cd psm[];
     const cd *V = Vin + (size_t)q * nb * nb;
-    cd *E = egv + (size_t)q * nb * nb;
+    cd *Eg = egv + (size_t)q * nb * nb;
+    cd *E = use_lds ? psm : Eg;
     cd *scr = scratch + (size_t)q * (2 * nb * nb + nb);
     if (threadIdx.x == 0) {
         for (int i = 0; i < nb; ++i) lam[i] = lam_in[(size_t)q * nb + i];
@@ -351,19 +362,42 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
         }
     }
     __syncthreads();
-    // Hellmann-Feynman: hf[al][b] = Re e_b^H gD_al e_b
+    // Hellmann-Feynman: hf[al][b] = Re e_b^H gD_al e_b, partial sums over rows i per thread
+    // (work items (i-block, b)) then a fixed-order sum; gD_al staged in LDS when E is
     const cd *g = gD + (size_t)q * 3 * nb * nb;
-    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) {
-        const int al = t / nb, b = t % nb;
-        double acc = 0;
-        for (int i = 0; i < nb; ++i) {
-            cd s = {0, 0};
-            for (int j = 0; j < nb; ++j) s = cadd(s, cmul(g[(size_t)al * nb * nb + (size_t)i * nb + j], E[(size_t)j * nb + b]));
-            const cd ei = E[(size_t)i * nb + b];
-            acc += ei.x * s.x + ei.y * s.y;
+    constexpr int kIb = 8;
+    __shared__ double hpart[kIb][128];
+    const int nib = (nb + kIb - 1) / kIb;
+    cd *gs = use_lds ? psm + (size_t)nb * nb : nullptr;
+    for (int al = 0; al < 3; ++al) {
+        const cd *ga = g + (size_t)al * nb * nb;
+        if (use_lds) {
+            for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) gs[i] = ga[i];
+            ga = gs;
         }
-        hf[al][b] = acc;
+        __syncthreads();
+        for (int t = threadIdx.x; t < kIb * nb; t += blockDim.x) {
+            const int b = t % nb, ib = t / nb;
+            double acc = 0;
+            for (int i = ib * nib; i < min(nb, (ib + 1) * nib); ++i) {
+                cd s = {0, 0};
+                const cd *gr = ga + (size_t)i * nb;
+                for (int j = 0; j < nb; ++j) s = cadd(s, cmul(gr[j], E[(size_t)j * nb + b]));
+                const cd ei = E[(size_t)i * nb + b];
+                acc += ei.x * s.x + ei.y * s.y;
+            }
+            hpart[ib][b] = acc;
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            double acc = 0;
+            for (int ib = 0; ib < kIb; ++ib) acc += hpart[ib][b];
+            hf[al][b] = acc;
+        }
+        __syncthreads();
     }
+    if (use_lds)
+        for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) Eg[i] = E[i];
     __syncthreads();
     if (threadIdx.x == 0) {
         // return_degenerate_subspaces grouping (…_dynamicalmatrix.f90:911-980)
@@ -536,11 +570,17 @@ int run_harmonic(tc3_ctx *c) {
 
     hipLaunchKernelGGL(k_dynmat, dim3(nirr), dim3(kBlock), 0, st, na, d_qv, d_blk, d_lv, d_pm, d_ism, d_D, d_gD);
     TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_jacobi, dim3(nirr), dim3(kBlock), 0, st, nb, d_D, d_V, d_lam, 100, d_stat);
+    // matrices in LDS when they fit next to the kernels' static arrays
+    const size_t jac_lds = 2 * sizeof(cd) * (size_t)nb * nb, post_lds = 2 * sizeof(cd) * (size_t)nb * nb;
+    const int jac_use = jac_lds <= 144 * 1024, post_use = post_lds <= 96 * 1024;
+    if (jac_use) TRY(hipFuncSetAttribute((const void *)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jac_lds));
+    if (post_use) TRY(hipFuncSetAttribute((const void *)k_post, hipFuncAttributeMaxDynamicSharedMemorySize, (int)post_lds));
+    hipLaunchKernelGGL(k_jacobi, dim3(nirr), dim3(kBlock), jac_use ? jac_lds : 0, st, nb, d_D, d_V, d_lam, 100, d_stat,
+                       jac_use);
     TRY(hipGetLastError());
     const double veltol = 1e-9 / kGroupvelAuToMs;
-    hipLaunchKernelGGL(k_post, dim3(nirr), dim3(kBlock), 0, st, nb, d_lam, d_V, d_gD, d_atg,
-                       c->d_om_irr, c->d_egv_irr, c->d_vel_irr, d_scr, veltol);
+    hipLaunchKernelGGL(k_post, dim3(nirr), dim3(kBlock), post_use ? post_lds : 0, st, nb, d_lam, d_V, d_gD, d_atg,
+                       c->d_om_irr, c->d_egv_irr, c->d_vel_irr, d_scr, veltol, post_use);
     TRY(hipGetLastError());
     hipLaunchKernelGGL(k_unfold, dim3(nfull), dim3(kBlock), 0, st, nb, na, c->d_irr_of_full, d_opf, d_opm,
                        d_src, d_v0, d_qirr, c->d_egv_irr, c->d_vel_irr, c->d_om_irr, c->d_egv_full,

@@ -215,68 +215,148 @@ __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full
     for (int i = 0; i < 3; ++i) Fbb[j * 3 + i] = s * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]);
 }
 
-// buf[row][0..2] = sum_j Xi[row][j] Fbb[j][0..2]  (lo_gemv x3, kappa.f90:411-413)
-__global__ void k_xi_gemv(long ncol, const double *__restrict__ xi, const double *__restrict__ Fbb,
-                          double *__restrict__ buf) {
-    __shared__ double red[3][kBlock];
-    const double *x = xi + (size_t)blockIdx.x * ncol;
-    double s0 = 0, s1 = 0, s2 = 0;
-    for (long j = threadIdx.x; j < ncol; j += blockDim.x) {
-        const double v = x[j];
-        s0 += v * Fbb[j * 3];
-        s1 += v * Fbb[j * 3 + 1];
-        s2 += v * Fbb[j * 3 + 2];
+// buf[row][0..2] = sum_j Xi[row][j] Fbb[j][0..2]  (lo_gemv x3, kappa.f90:411-413).
+// Workgroup per (kRows rows, column slice): every Fbb element fetched serves kRows rows,
+// and the grid holds enough workgroups to keep HBM busy.  Partial sums per slice are
+// added in slice order by k_gemv_sum, so the result does not depend on scheduling.
+constexpr int kRows = 4, kSlices = 8;
+__global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const double *__restrict__ xi,
+                                                    const double *__restrict__ Fbb, double *__restrict__ part) {
+    __shared__ double red[3 * kRows][kBlock / 64];
+    const int r0 = blockIdx.x * kRows, sl = blockIdx.y;
+    const int nr = min(kRows, nrows - r0);
+    const long c0 = ncol * sl / kSlices, c1 = ncol * (sl + 1) / kSlices;
+    const double *x = xi + (size_t)r0 * ncol;
+    double s[kRows][3];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
+    for (long j = c0 + threadIdx.x; j < c1; j += blockDim.x) {
+        const double f0 = Fbb[j * 3], f1 = Fbb[j * 3 + 1], f2 = Fbb[j * 3 + 2];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (r < nr) {
+                const double v = x[(size_t)r * ncol + j];
+                s[r][0] += v * f0;
+                s[r][1] += v * f1;
+                s[r][2] += v * f2;
+            }
+        }
     }
-    red[0][threadIdx.x] = s0;
-    red[1][threadIdx.x] = s1;
-    red[2][threadIdx.x] = s2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double v = s[r][i];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[3 * r + i][wave] = v;
+        }
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int i = 0; i < 3; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x < 3 * nr) {
+        double v = 0;
+        for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
+        part[((size_t)sl * nrows + r0) * 3 + threadIdx.x] = v;
     }
-    if (threadIdx.x < 3) buf[(size_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_gemv_sum(int n3, const double *__restrict__ part, double *__restrict__ buf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double v = 0;
+    for (int sl = 0; sl < kSlices; ++sl) v += part[(size_t)sl * n3 + i];
+    buf[i] = v;
 }
 
 // Fnb = -buf/qs; degeneracy average then +F0, band by band in order (kappa.f90:415-448);
-// convergence partial and mixing (kappa.f90:456-468).  Workgroup per irreducible point.
+// convergence partial and mixing (kappa.f90:456-468).  Workgroup per irreducible point;
+// the ordered degeneracy pass runs on one thread over LDS copies.
 __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf,
                              const double *__restrict__ qs, const double *__restrict__ F0,
                              const int *__restrict__ dg_off, const int *__restrict__ dg, double mix,
                              double *__restrict__ Fn, double *__restrict__ g0part) {
     const int q = blockIdx.x;
-    __shared__ double Fnb[128 * 3];
+    __shared__ double Fnb[128 * 3], F0s[128 * 3], g1s[128];
+    __shared__ int dgo[129], dgl[128 * 8];
+    const int o_base = dg_off[q * nb];
+    const int ndg = dg_off[q * nb + nb] - o_base;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         const int r = row_of[q * nb + b];
         const double s = qs[q * nb + b];
-        for (int i = 0; i < 3; ++i) Fnb[3 * b + i] = (r < 0 || s < 1e-9) ? 0.0 : -buf[(size_t)r * 3 + i] / s;
+        for (int i = 0; i < 3; ++i) {
+            Fnb[3 * b + i] = (r < 0 || s < 1e-9) ? 0.0 : -buf[(size_t)r * 3 + i] / s;
+            F0s[3 * b + i] = F0[((size_t)q * nb + b) * 3 + i];
+        }
     }
+    for (int b = threadIdx.x; b <= nb; b += blockDim.x) dgo[b] = dg_off[q * nb + b] - o_base;
+    for (int x = threadIdx.x; x < ndg; x += blockDim.x) dgl[x] = dg[o_base + x];
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int b1 = 0; b1 < nb; ++b1) {
-            const int o0 = dg_off[q * nb + b1], o1 = dg_off[q * nb + b1 + 1];
+            const int o0 = dgo[b1], o1 = dgo[b1 + 1];
             double v0[3] = {0, 0, 0};
             for (int o = o0; o < o1; ++o)
-                for (int i = 0; i < 3; ++i) v0[i] += Fnb[3 * dg[o] + i];
+                for (int i = 0; i < 3; ++i) v0[i] += Fnb[3 * dgl[o] + i];
             for (int i = 0; i < 3; ++i) v0[i] /= (double)(o1 - o0);
             for (int o = o0; o < o1; ++o)
-                for (int i = 0; i < 3; ++i) Fnb[3 * dg[o] + i] = v0[i];
-            for (int i = 0; i < 3; ++i) Fnb[3 * b1 + i] = F0[((size_t)q * nb + b1) * 3 + i] + Fnb[3 * b1 + i];
+                for (int i = 0; i < 3; ++i) Fnb[3 * dgl[o] + i] = v0[i];
+            for (int i = 0; i < 3; ++i) Fnb[3 * b1 + i] = F0s[3 * b1 + i] + Fnb[3 * b1 + i];
         }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        double *F = Fn + ((size_t)q * nb + b) * 3;
+        double g1 = 0, g2 = 0, f[3];
+        for (int i = 0; i < 3; ++i) {
+            f[i] = F[i];
+            g1 += (f[i] - Fnb[3 * b + i]) * (f[i] - Fnb[3 * b + i]);
+            g2 += f[i] * f[i];
+        }
+        g1s[b] = (g2 > 1e-10) ? g1 / g2 : 0.0;
+        for (int i = 0; i < 3; ++i) F[i] = f[i] * (1.0 - mix) + mix * Fnb[3 * b + i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         double g0 = 0;
-        for (int b = 0; b < nb; ++b) {
-            double *F = Fn + ((size_t)q * nb + b) * 3;
-            double g1 = 0, g2 = 0;
-            for (int i = 0; i < 3; ++i) {
-                g1 += (F[i] - Fnb[3 * b + i]) * (F[i] - Fnb[3 * b + i]);
-                g2 += F[i] * F[i];
-            }
-            if (g2 > 1e-10) g0 += g1 / g2;
-            for (int i = 0; i < 3; ++i) F[i] = F[i] * (1.0 - mix) + mix * Fnb[3 * b + i];
-        }
+        for (int b = 0; b < nb; ++b) g0 += g1s[b];
         g0part[q] = g0;
     }
+}
+
+// kappa of the current Fn for the iteration log (get_kappa, kappa.f90:28-75, before the
+// chop): workgroup per slice of modes, each writing its 9 partial sums; the host adds
+// the slices in order.  cvw[q][b] = cv(omega) * weight_q / volume (0 below lo_freqtol).
+constexpr int kKapSlices = 32;
+__global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const double *__restrict__ cvw,
+                                                   const double *__restrict__ Fn, const double *__restrict__ vel,
+                                                   const double *__restrict__ opm, double *__restrict__ out) {
+    __shared__ double red[9][256];
+    double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int m0 = (int)((long)nmode * blockIdx.x / gridDim.x), m1 = (int)((long)nmode * (blockIdx.x + 1) / gridDim.x);
+    for (int x = m0 + threadIdx.x; x < m1; x += blockDim.x) {
+        const double w = cvw[x];
+        if (w == 0.0) continue;
+        const double *F = Fn + 3 * (size_t)x, *v = vel + 3 * (size_t)x;
+        double v2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int o = 0; o < nsym; ++o) {
+            const double *m = opm + 9 * o;
+            double v0[3], v1[3];
+            for (int i = 0; i < 3; ++i) {
+                v0[i] = m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2];
+                v1[i] = m[3 * i] * v[0] + m[3 * i + 1] * v[1] + m[3 * i + 2] * v[2];
+            }
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) v2[3 * j + i] += v0[i] * v1[j] / nsym;
+        }
+        for (int i = 0; i < 9; ++i) k[i] += w * v2[i];
+    }
+    for (int i = 0; i < 9; ++i) red[i][threadIdx.x] = k[i];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int i = 0; i < 9; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) out[(size_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 }  // namespace
@@ -306,16 +386,20 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
         if (r == hipSuccess) tmp.push_back(*p);
         return r;
     };
-    double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_buf, *d_opm, *d_g0;
+    double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_buf, *d_opm, *d_g0, *d_cvw, *d_vel, *d_klog;
     int *d_opf, *d_rowof, *d_dgoff, *d_dg;
     std::vector<double> opm(9 * nsym);
     for (int o = 0; o < nsym; ++o) std::memcpy(&opm[9 * o], c->ops[o].m, sizeof(double) * 9);
     std::vector<int> dgoff(qb + 1, 0), dg;
-    for (int q = 0; q < nirr; ++q)
+    for (int q = 0; q < nirr; ++q) {
         for (int b = 0; b < nb; ++b) {
             for (int x : c->degen[q][b]) dg.push_back(x);
             dgoff[(size_t)q * nb + b + 1] = (int)dg.size();
         }
+        // k_bte_update holds one point's degenerate-set lists in LDS (1024 entries)
+        if (dgoff[(size_t)q * nb + nb] - dgoff[(size_t)q * nb] > 1024)
+            return c->fail(1, "degenerate sets of one q-point exceed the BTE kernel's capacity");
+    }
     TRY(dalloc((void **)&d_Fn, sizeof(double) * qb * 3));
     TRY(dalloc((void **)&d_F0, sizeof(double) * qb * 3));
     TRY(dalloc((void **)&d_qs, sizeof(double) * qb));
@@ -327,6 +411,12 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     TRY(dalloc((void **)&d_rowof, sizeof(int) * qb));
     TRY(dalloc((void **)&d_dgoff, sizeof(int) * (qb + 1)));
     TRY(dalloc((void **)&d_dg, sizeof(int) * dg.size()));
+    TRY(dalloc((void **)&d_cvw, sizeof(double) * qb));
+    TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
+    TRY(dalloc((void **)&d_klog, sizeof(double) * 9 * kKapSlices * (size_t)std::max(maxiter, 1)));
+    double *d_part;
+    TRY(dalloc((void **)&d_part, sizeof(double) * 3 * (size_t)std::max(nrows, 1) * kSlices));
+    TRY(hipMemcpyAsync(d_vel, c->vel_irr.data(), sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_rowof, c->row_of.data(), sizeof(int) * qb, hipMemcpyHostToDevice, st));
@@ -353,6 +443,15 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
         const auto a = std::chrono::steady_clock::now();
         int it_done = 0;
         if (maxiter > 0) {
+            std::vector<double> cvw(qb, 0.0);
+            for (int q = 0; q < nirr; ++q)
+                for (int b = 0; b < nb; ++b) {
+                    const double om = c->om_irr[(size_t)q * nb + b];
+                    if (om < kFreqTol) continue;
+                    const double cv = c->classical ? kKbHartree : cv_host(T, om);
+                    cvw[(size_t)q * nb + b] = cv * c->mesh.weight_irr[q] / c->cr.volume;
+                }
+            TRY(hipMemcpyAsync(d_cvw, cvw.data(), sizeof(double) * qb, hipMemcpyHostToDevice, st));
             TRY(hipMemcpyAsync(d_Fn, Fn, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
             TRY(hipMemcpyAsync(d_F0, F0, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
             TRY(hipMemcpyAsync(d_qs, &c->qs[(size_t)iT * qb], sizeof(double) * qb, hipMemcpyHostToDevice, st));
@@ -367,23 +466,38 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
                 const long nj = (long)nfull * nb;
                 hipLaunchKernelGGL(k_foldout, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, st, nfull, nb,
                                    c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb);
-                if (nrows > 0)
-                    hipLaunchKernelGGL(k_xi_gemv, dim3(nrows), dim3(kBlock), 0, st, (long)c->xi_row, xi, d_Fbb, d_buf);
+                if (nrows > 0) {
+                    hipLaunchKernelGGL(k_xi_gemv, dim3((nrows + kRows - 1) / kRows, kSlices), dim3(kBlock), 0, st,
+                                       nrows, (long)c->xi_row, xi, d_Fbb, d_part);
+                    hipLaunchKernelGGL(k_gemv_sum, dim3((3 * nrows + 255) / 256), dim3(256), 0, st, 3 * nrows, d_part,
+                                       d_buf);
+                }
                 hipLaunchKernelGGL(k_bte_update, dim3(nirr), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0, d_dgoff,
                                    d_dg, mix, d_Fn, d_g0);
+                hipLaunchKernelGGL(k_kappa_sum, dim3(kKapSlices), dim3(256), 0, st, (int)qb, nsym, d_cvw, d_Fn, d_vel,
+                                   d_opm, d_klog + 9 * kKapSlices * (size_t)(it - 1));
                 TRY(hipGetLastError());
                 TRY(hipMemcpyAsync(g0.data(), d_g0, sizeof(double) * nirr, hipMemcpyDeviceToHost, st));
-                TRY(hipMemcpyAsync(Fn, d_Fn, sizeof(double) * qb * 3, hipMemcpyDeviceToHost, st));
                 TRY(hipStreamSynchronize(st));
                 double s = 0;
                 for (int q = 0; q < nirr; ++q) s += g0[q];
                 scf.push_back(s / nirr / nb);
-                double kk[9];
-                get_kappa(c, Fn, T, c->classical, kk, nullptr);
-                log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf.back()});
                 it_done = it;
                 if (it >= 3 && scf[it - 1] + scf[it - 2] + scf[it - 3] < tol) break;
                 if (it > 15 && mix > 0.50) mix *= 0.98;
+            }
+            // final Fn and the per-iteration kappa log (kappa.f90:473-474)
+            std::vector<double> klog(9 * kKapSlices * (size_t)it_done);
+            TRY(hipMemcpyAsync(Fn, d_Fn, sizeof(double) * qb * 3, hipMemcpyDeviceToHost, st));
+            if (it_done > 0)
+                TRY(hipMemcpyAsync(klog.data(), d_klog, sizeof(double) * klog.size(), hipMemcpyDeviceToHost, st));
+            TRY(hipStreamSynchronize(st));
+            for (int it = 0; it < it_done; ++it) {
+                double kk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int sl = 0; sl < kKapSlices; ++sl)
+                    for (int i = 0; i < 9; ++i) kk[i] += klog[(9 * kKapSlices) * (size_t)it + 9 * sl + i];
+                chop_kappa(kk);
+                log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf[it]});
             }
         }
         c->niter[iT] = it_done;
