@@ -353,8 +353,16 @@ int tc3_get_xi(const tc3_ctx *ctx, int iT, int row0, int nrow, double *xi, int *
     if (xi && nrow > 0) {
         if (row0 < 0 || row0 + nrow > ctx->nrows) return 1;
         hipSetDevice(ctx->device);
-        const double *src = ctx->d_xi + (size_t)iT * ctx->xi_row * ctx->nrows + (size_t)row0 * ctx->xi_row;
-        if (hipMemcpy(xi, src, sizeof(double) * ctx->xi_row * nrow, hipMemcpyDeviceToHost) != hipSuccess) return 7;
+        const size_t n = ctx->xi_row * (size_t)nrow;
+        const char *src = (const char *)ctx->d_xi + (size_t)ctx->xi_es * ((size_t)iT * ctx->xi_row * ctx->nrows +
+                                                                          (size_t)row0 * ctx->xi_row);
+        if (ctx->xi_es == 8) {
+            if (hipMemcpy(xi, src, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) return 7;
+        } else {
+            std::vector<float> tmp(n);
+            if (hipMemcpy(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess) return 7;
+            for (size_t i = 0; i < n; ++i) xi[i] = (double)tmp[i];
+        }
     }
     return 0;
 }

@@ -70,10 +70,12 @@ struct tc3_ctx {
     // device buffers
     double *d_om_irr = nullptr, *d_om_full = nullptr, *d_vel_irr = nullptr, *d_vel_full = nullptr;
     tc3::cd *d_egv_irr = nullptr, *d_egv_full = nullptr;
-    double *d_xi = nullptr;     // [T][row][q*nb + b]
+    void *d_xi = nullptr;       // [T][row][q*nb + b]: FP64 in FP64 mode, FP32 in the mixed modes
+    int xi_es = 8;              // bytes per entry of d_xi (8 or 4)
     size_t xi_row = 0;          // nfull*nb
     size_t xi_bytes = 0;        // allocated size of d_xi
     long xi_gen = -1;           // model generation d_xi was zeroed for
+    int xi_es_zeroed = 0;       // entry size d_xi was zeroed for
     int *d_irr_of_full = nullptr;
 
     // Device buffers kept across calls, by name (grown on demand, freed with the context),

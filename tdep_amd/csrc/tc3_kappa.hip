@@ -203,8 +203,21 @@ void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, double *k
 
 constexpr int kBlock = 256;
 
+// Iteration state of the BTE solve, kept on the device so that chunks of iterations run
+// without a host round trip: iterations past convergence leave everything unchanged.
+struct BteState {
+    int it;        // iterations done
+    int done;      // converged or maxiter reached
+    int maxiter;
+    int pad;
+    double mix;    // mixing factor (kappa.f90:364, 487-489)
+    double tol;
+};
+
 __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full, const int *__restrict__ op_of_full,
-                          const double *__restrict__ opm, const double *__restrict__ Fn, double *__restrict__ Fbb) {
+                          const double *__restrict__ opm, const double *__restrict__ Fn, double *__restrict__ Fbb,
+                          const BteState *__restrict__ st) {
+    if (st->done) return;
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (j >= (long)nfull * nb) return;
     const int q = (int)(j / nb), b = (int)(j % nb);
@@ -220,13 +233,16 @@ __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full
 // and the grid holds enough workgroups to keep HBM busy.  Partial sums per slice are
 // added in slice order by k_gemv_sum, so the result does not depend on scheduling.
 constexpr int kRows = 4, kSlices = 8;
-__global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const double *__restrict__ xi,
-                                                    const double *__restrict__ Fbb, double *__restrict__ part) {
+template <typename XT>
+__global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const XT *__restrict__ xi,
+                                                    const double *__restrict__ Fbb, double *__restrict__ part,
+                                                    const BteState *__restrict__ st) {
+    if (st->done) return;
     __shared__ double red[3 * kRows][kBlock / 64];
     const int r0 = blockIdx.x * kRows, sl = blockIdx.y;
     const int nr = min(kRows, nrows - r0);
     const long c0 = ncol * sl / kSlices, c1 = ncol * (sl + 1) / kSlices;
-    const double *x = xi + (size_t)r0 * ncol;
+    const XT *x = xi + (size_t)r0 * ncol;
     double s[kRows][3];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
@@ -235,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const 
 #pragma unroll
         for (int r = 0; r < kRows; ++r) {
             if (r < nr) {
-                const double v = x[(size_t)r * ncol + j];
+                const double v = (double)x[(size_t)r * ncol + j];
                 s[r][0] += v * f0;
                 s[r][1] += v * f1;
                 s[r][2] += v * f2;
@@ -259,7 +275,9 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const 
     }
 }
 
-__global__ void k_gemv_sum(int n3, const double *__restrict__ part, double *__restrict__ buf) {
+__global__ void k_gemv_sum(int n3, const double *__restrict__ part, double *__restrict__ buf,
+                           const BteState *__restrict__ st) {
+    if (st->done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n3) return;
     double v = 0;
@@ -272,8 +290,10 @@ __global__ void k_gemv_sum(int n3, const double *__restrict__ part, double *__re
 // the ordered degeneracy pass runs on one thread over LDS copies.
 __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf,
                              const double *__restrict__ qs, const double *__restrict__ F0,
-                             const int *__restrict__ dg_off, const int *__restrict__ dg, double mix,
-                             double *__restrict__ Fn, double *__restrict__ g0part) {
+                             const int *__restrict__ dg_off, const int *__restrict__ dg,
+                             double *__restrict__ Fn, double *__restrict__ g0part, const BteState *__restrict__ st) {
+    if (st->done) return;
+    const double mix = st->mix;
     const int q = blockIdx.x;
     __shared__ double Fnb[128 * 3], F0s[128 * 3], g1s[128];
     __shared__ int dgo[129], dgl[128 * 8];
@@ -328,7 +348,10 @@ __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const doubl
 constexpr int kKapSlices = 32;
 __global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const double *__restrict__ cvw,
                                                    const double *__restrict__ Fn, const double *__restrict__ vel,
-                                                   const double *__restrict__ opm, double *__restrict__ out) {
+                                                   const double *__restrict__ opm, double *__restrict__ klog,
+                                                   const BteState *__restrict__ st) {
+    if (st->done) return;
+    double *out = klog + (size_t)9 * gridDim.x * st->it;
     __shared__ double red[9][256];
     double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int m0 = (int)((long)nmode * blockIdx.x / gridDim.x), m1 = (int)((long)nmode * (blockIdx.x + 1) / gridDim.x);
@@ -357,6 +380,23 @@ __global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const do
         __syncthreads();
     }
     if (threadIdx.x < 9) out[(size_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Convergence test and mixing schedule after one iteration (kappa.f90:456-489):
+// scfcheck = mean over modes of |Fn - Fnb|^2 / |Fn|^2 (partials per point added in order),
+// stop when the last three add up to less than tol; after iteration 15 the mixing factor
+// shrinks by 2% per iteration while above 0.5.
+__global__ void k_bte_check(int nirr, int nb, const double *__restrict__ g0part, double *__restrict__ scf,
+                            BteState *__restrict__ st) {
+    if (threadIdx.x != 0 || st->done) return;
+    double s = 0;
+    for (int q = 0; q < nirr; ++q) s += g0part[q];
+    const int it = st->it + 1;   // 1-based number of the iteration just done
+    scf[it - 1] = s / nirr / nb;
+    st->it = it;
+    if (it >= 3 && scf[it - 1] + scf[it - 2] + scf[it - 3] < st->tol) st->done = 1;
+    else if (it >= st->maxiter) st->done = 1;
+    else if (it > 15 && st->mix > 0.50) st->mix *= 0.98;
 }
 
 }  // namespace
@@ -414,8 +454,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     TRY(dalloc((void **)&d_cvw, sizeof(double) * qb));
     TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
     TRY(dalloc((void **)&d_klog, sizeof(double) * 9 * kKapSlices * (size_t)std::max(maxiter, 1)));
-    double *d_part;
+    double *d_part, *d_scf;
+    BteState *d_st;
     TRY(dalloc((void **)&d_part, sizeof(double) * 3 * (size_t)std::max(nrows, 1) * kSlices));
+    TRY(dalloc((void **)&d_scf, sizeof(double) * (size_t)std::max(maxiter, 1)));
+    TRY(dalloc((void **)&d_st, sizeof(BteState)));
     TRY(hipMemcpyAsync(d_vel, c->vel_irr.data(), sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
@@ -455,37 +498,64 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             TRY(hipMemcpyAsync(d_Fn, Fn, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
             TRY(hipMemcpyAsync(d_F0, F0, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
             TRY(hipMemcpyAsync(d_qs, &c->qs[(size_t)iT * qb], sizeof(double) * qb, hipMemcpyHostToDevice, st));
-            double mix = 0.95;
-            std::vector<double> scf;
             double k0[9];
             get_kappa(c, Fn, T, c->classical, k0, nullptr);
             auto &log = c->kappa_log[iT];
             log.insert(log.end(), {k0[0], k0[4], k0[8], k0[3], k0[6], k0[7], 0.0});
-            const double *xi = c->d_xi + (size_t)iT * c->xi_row * nrows;
-            for (int it = 1; it <= maxiter; ++it) {
+            const char *xi = (const char *)c->d_xi + (size_t)c->xi_es * iT * c->xi_row * nrows;
+            BteState hs = {0, 0, maxiter, 0, 0.95, tol};
+            TRY(hipMemcpyAsync(d_st, &hs, sizeof(BteState), hipMemcpyHostToDevice, st));
+            // one iteration: fold out, Xi.F, update, kappa log, convergence test
+            auto iteration = [&]() {
                 const long nj = (long)nfull * nb;
                 hipLaunchKernelGGL(k_foldout, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, st, nfull, nb,
-                                   c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb);
+                                   c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb, d_st);
                 if (nrows > 0) {
-                    hipLaunchKernelGGL(k_xi_gemv, dim3((nrows + kRows - 1) / kRows, kSlices), dim3(kBlock), 0, st,
-                                       nrows, (long)c->xi_row, xi, d_Fbb, d_part);
+                    if (c->xi_es == 8)
+                        hipLaunchKernelGGL(k_xi_gemv<double>, dim3((nrows + kRows - 1) / kRows, kSlices), dim3(kBlock),
+                                           0, st, nrows, (long)c->xi_row, (const double *)xi, d_Fbb, d_part, d_st);
+                    else
+                        hipLaunchKernelGGL(k_xi_gemv<float>, dim3((nrows + kRows - 1) / kRows, kSlices), dim3(kBlock),
+                                           0, st, nrows, (long)c->xi_row, (const float *)xi, d_Fbb, d_part, d_st);
                     hipLaunchKernelGGL(k_gemv_sum, dim3((3 * nrows + 255) / 256), dim3(256), 0, st, 3 * nrows, d_part,
-                                       d_buf);
+                                       d_buf, d_st);
                 }
                 hipLaunchKernelGGL(k_bte_update, dim3(nirr), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0, d_dgoff,
-                                   d_dg, mix, d_Fn, d_g0);
+                                   d_dg, d_Fn, d_g0, d_st);
                 hipLaunchKernelGGL(k_kappa_sum, dim3(kKapSlices), dim3(256), 0, st, (int)qb, nsym, d_cvw, d_Fn, d_vel,
-                                   d_opm, d_klog + 9 * kKapSlices * (size_t)(it - 1));
-                TRY(hipGetLastError());
-                TRY(hipMemcpyAsync(g0.data(), d_g0, sizeof(double) * nirr, hipMemcpyDeviceToHost, st));
-                TRY(hipStreamSynchronize(st));
-                double s = 0;
-                for (int q = 0; q < nirr; ++q) s += g0[q];
-                scf.push_back(s / nirr / nb);
-                it_done = it;
-                if (it >= 3 && scf[it - 1] + scf[it - 2] + scf[it - 3] < tol) break;
-                if (it > 15 && mix > 0.50) mix *= 0.98;
+                                   d_opm, d_klog, d_st);
+                hipLaunchKernelGGL(k_bte_check, dim3(1), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st);
+            };
+            // chunks of kChunkIt iterations captured once as a graph and replayed until the
+            // device reports convergence; direct launches if capture is not available
+            constexpr int kChunkIt = 8;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t gexec = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                for (int k = 0; k < kChunkIt; ++k) iteration();
+                if (hipStreamEndCapture(st, &graph) != hipSuccess ||
+                    hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    graph = nullptr;
+                    gexec = nullptr;
+                }
             }
+            (void)hipGetLastError();
+            for (int done = 0; !done;) {
+                if (gexec) TRY(hipGraphLaunch(gexec, st));
+                else
+                    for (int k = 0; k < kChunkIt; ++k) iteration();
+                TRY(hipGetLastError());
+                TRY(hipMemcpyAsync(&hs, d_st, sizeof(BteState), hipMemcpyDeviceToHost, st));
+                TRY(hipStreamSynchronize(st));
+                done = hs.done;
+            }
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+            it_done = hs.it;
+            std::vector<double> scf((size_t)std::max(it_done, 1));
+            if (it_done > 0)
+                TRY(hipMemcpyAsync(scf.data(), d_scf, sizeof(double) * it_done, hipMemcpyDeviceToHost, st));
             // final Fn and the per-iteration kappa log (kappa.f90:473-474)
             std::vector<double> klog(9 * kKapSlices * (size_t)it_done);
             TRY(hipMemcpyAsync(Fn, d_Fn, sizeof(double) * qb * 3, hipMemcpyDeviceToHost, st));

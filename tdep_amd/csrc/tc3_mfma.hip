@@ -399,6 +399,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
     typedef typename Stage<PREC>::T S;
+    typedef S XT;                          // scattering-matrix entries: FP64 in FP64 mode, else FP32
+    XT *const xi = (XT *)a.xi;
     constexpr int NW = NB / 16, NB2 = NB * NB, KS = KSteps<PREC, NB>::n, KP = kpad(NB), NMW = MaskWords<NB>::n;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nT = a.nT;
@@ -650,9 +652,9 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                 if (r < 0) continue;
                 const double iw1 = 2.0 / w1s[m];
                 for (int e = e0; e < e1; ++e) {
-                    double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)a.eq_q2[e] * NB + cb;
+                    XT *xr = xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)a.eq_q2[e] * NB + cb;
 #pragma unroll
-                    for (int c = 0; c < kChunk; ++c) xr[c] = (double)v[c] * w2s[cb + c] * iw1;
+                    for (int c = 0; c < kChunk; ++c) xr[c] = (XT)((double)v[c] * w2s[cb + c] * iw1);
                 }
             }
             __syncthreads();
@@ -672,14 +674,14 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
 #pragma unroll
             for (int T_ = 0; T_ < NTY; ++T_) {
                 if (T_ >= nT) continue;
-                double *xr = a.xi + (size_t)T_ * a.xi_T + (size_t)row1 * a.xi_row + (size_t)x3q * NB;
+                XT *xr = xi + (size_t)T_ * a.xi_T + (size_t)row1 * a.xi_row + (size_t)x3q * NB;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int b3 = b3w + out_row<PREC>(lane, r);
                     const S s3 = y[T_][r] + n3h[T_ * NB + b3] * yM[r];
                     const double v = (double)s3 * om3[b3] * iw1;
-                    if (x2 == x3q) xr[b3] += v;
-                    else xr[b3] = v;
+                    if (x2 == x3q) xr[b3] = (XT)((double)xr[b3] + v);
+                    else xr[b3] = (XT)v;
                 }
             }
         }

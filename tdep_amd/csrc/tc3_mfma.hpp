@@ -13,7 +13,8 @@ struct MfmaEpiArgs {
     double sigma_fixed, psi_scale;
     const int *tq1, *tq2, *tq3, *eq_off, *eq_q2, *eq_q3, *irr_of_full, *row_of;
     const double *perm, *om_irr, *om_full, *be, *sigsq;   // be: first temperature of the chunk
-    double *xi, *gpart;      // xi: first temperature of the chunk
+    void *xi;                // first temperature of the chunk; FP64 entries in FP64 mode, else FP32
+    double *gpart;
     size_t xi_row, xi_T;
 };
 

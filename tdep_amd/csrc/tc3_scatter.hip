@@ -136,7 +136,7 @@ struct EpiArgs {
     double sigma_fixed, psi_scale;  // pi/16 / N_q
     const int *tq1, *tq2, *tq3, *eq_off, *eq_q2, *eq_q3, *irr_of_full, *row_of;
     const double *perm, *om_irr, *om_full, *be, *sigsq;
-    double *xi;      // [T][row][nfull*nb]
+    void *xi;        // [T][row][nfull*nb], entries of the epilogue's type P
     double *gpart;   // [t][T][nb]
     size_t xi_row, xi_T;
     int nirr;
@@ -212,17 +212,17 @@ __global__ void k_epilogue(EpiArgs a, const P *__restrict__ psi, int psi_complex
         }
         // every (row, q', band) target belongs to one triplet and, inside it, to one
         // equivalent pair: store the values (q2' == q3' receives both terms)
-        double *xr = a.xi + (size_t)T * a.xi_T + (size_t)row * a.xi_row;
+        P *xr = (P *)a.xi + (size_t)T * a.xi_T + (size_t)row * a.xi_row;
         for (int e = e0; e < e1; ++e) {
             const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
             for (int b = threadIdx.x; b < nb; b += blockDim.x) {
                 const double v2 = 2.0 * s2[b] * a.om_full[(size_t)q2 * nb + b] / w1;
                 const double v3 = 2.0 * s3[b] * a.om_full[(size_t)q3 * nb + b] / w1;
                 if (x2 == x3) {
-                    xr[(size_t)x2 * nb + b] = v2 + v3;
+                    xr[(size_t)x2 * nb + b] = (P)(v2 + v3);
                 } else {
-                    xr[(size_t)x2 * nb + b] = v2;
-                    xr[(size_t)x3 * nb + b] = v3;
+                    xr[(size_t)x2 * nb + b] = (P)v2;
+                    xr[(size_t)x3 * nb + b] = (P)v3;
                 }
             }
         }
@@ -247,12 +247,13 @@ __global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_q
 // symmetrize_scatmat (plain mean, scattering.f90:312-354).  After the first step an orbit
 // with a positive entry holds one value, so the second step is evaluated on that value in
 // registers; each entry is read once and written once.  Workgroup per (row, T).
+template <typename XT>
 __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const int *__restrict__ q1_off,
                          const int *__restrict__ orb_off, const int *__restrict__ members,
-                         double *__restrict__ xi, size_t xi_row, size_t xi_T) {
+                         XT *__restrict__ xi, size_t xi_row, size_t xi_T) {
     const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
     const int q1 = row_q[r];
-    double *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
+    XT *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
     const int o0 = q1_off[q1], o1 = q1_off[q1 + 1];
     const long work = (long)(o1 - o0) * nb;
     for (long w = threadIdx.x; w < work; w += blockDim.x) {
@@ -261,7 +262,7 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
         double sp = 0, sa = 0;
         int n = 0;
         for (int m = m0; m < m1; ++m) {
-            const double v = x[(size_t)members[m] * nb + b];
+            const double v = (double)x[(size_t)members[m] * nb + b];
             sa += v;
             if (v > 0.0) { sp += v; ++n; }
         }
@@ -271,7 +272,7 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
             s = 0;
             for (int m = m0; m < m1; ++m) s += avg;
         }
-        const double out = s / (m1 - m0);
+        const XT out = (XT)(s / (m1 - m0));
         for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = out;
     }
 }
@@ -361,17 +362,19 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     // the buffer is zeroed only when it is new or the model changed
     c->xi_row = (size_t)nfull * nb;
     const size_t xi_T = c->xi_row * nrows;
-    if (c->xi_bytes < sizeof(double) * xi_T * nT) {
+    c->xi_es = (precision == TC3_PREC_FP64) ? 8 : 4;
+    if (c->xi_bytes < (size_t)c->xi_es * xi_T * nT) {
         hipFree(c->d_xi);
         c->d_xi = nullptr;
         c->xi_bytes = 0;
-        TRY(hipMalloc(&c->d_xi, sizeof(double) * xi_T * nT));
-        c->xi_bytes = sizeof(double) * xi_T * nT;
+        TRY(hipMalloc(&c->d_xi, (size_t)c->xi_es * xi_T * nT));
+        c->xi_bytes = (size_t)c->xi_es * xi_T * nT;
         c->xi_gen = -1;
     }
-    if (c->xi_gen != c->gen_model) {
+    if (c->xi_gen != c->gen_model || c->xi_es_zeroed != c->xi_es) {
         TRY(hipMemsetAsync(c->d_xi, 0, c->xi_bytes, st));
         c->xi_gen = c->gen_model;
+        c->xi_es_zeroed = c->xi_es;
     }
     c->scatter_tables_gen = c->gen_model;
 
@@ -496,7 +499,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             me.T0 = T0;
             me.nT = std::min(mp.nty, nT - T0);
             me.be = d_be + (size_t)T0 * nirr * nb;
-            me.xi = c->d_xi + (size_t)T0 * xi_T;
+            me.xi = (char *)c->d_xi + (size_t)c->xi_es * T0 * xi_T;
             mp.ev = kev.data() + 3 * nbatch * tc;
             int r = run_mfma_chain(mp, precision, st);
             if (r) { cleanup(); return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r; }
@@ -547,8 +550,12 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
                        d_q1off, d_gpart, d_lw);
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs0, st));
-    hipLaunchKernelGGL(k_xi_sym, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
-                       d_members, c->d_xi, c->xi_row, xi_T);
+    if (c->xi_es == 8)
+        hipLaunchKernelGGL(k_xi_sym<double>, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1,
+                           d_orboff, d_members, (double *)c->d_xi, c->xi_row, xi_T);
+    else
+        hipLaunchKernelGGL(k_xi_sym<float>, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1,
+                           d_orboff, d_members, (float *)c->d_xi, c->xi_row, xi_T);
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs1, st));
     TRY(hipEventRecord(ev1, st));
