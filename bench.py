@@ -34,6 +34,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (two rocprofv3
+    --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "scatter_sl16_bf16x3_counters.md")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 5 and cells[0] == kernel:
+                return float(cells[4]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def kernel_stats(eng):
     import ctypes
     out = {}
@@ -96,7 +110,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed mixed-precision pass (for profiler runs); prints no JSON")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 256 MiB device-to-device copy (the reference kernel for scaling the HBM "
+                         "byte counters); prints no JSON")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {a.numel() * 4 >> 20} MiB")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: the scattering engine has no multi-GPU path "
               f"(requested --gpus {args.gpus})", file=sys.stderr)
@@ -124,8 +150,11 @@ def main():
         t_64 = scatter_pass(eng, temps, "fp64", max(1, args.steps // 2), 1)
         ks_64 = kernel_stats(eng)
         log(f"[bench] fp64: {t_64 * 1e3:.2f} ms/step  kernels {ks_64}")
-        # kappa wall time, both modes
+        # kappa wall time, both modes; one untimed run each first (kernel code loading and
+        # graph instantiation on first use are process start-up, not part of the solve)
+        kappa_wall(eng, temps, args.precision)
         kw_mx, ksma_mx, kit_mx, st_mx = kappa_wall(eng, temps, args.precision)
+        kappa_wall(eng, temps, "fp64")
         kw_64, ksma_64, kit_64, st_64 = kappa_wall(eng, temps, "fp64")
         err = float(np.max(np.abs(kit_mx - kit_64)) / np.max(np.abs(kit_64)))
         err_sma = float(np.max(np.abs(ksma_mx - ksma_64)) / np.max(np.abs(ksma_64)))
@@ -142,12 +171,17 @@ def main():
         # kernel statistics cover the last tc3_scatter call, i.e. one step
         achieved = fl / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
         peak = PEAK_TFLOPS[args.precision]
+        traffic, tsrc = (None, None)
+        if args.precision == "bf16x3":
+            traffic, tsrc = profile_traffic("k_uw" if dom.startswith("k_uw") else "k_psi")
         roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak,
-                    traffic=None, kernel=dom, launches_per_step=n,
+                    traffic=traffic, traffic_source=tsrc, kernel=dom, launches_per_step=n,
                     avg_launch_ms=ms / max(1, n), triplets_per_launch=batch,
                     flop_per_triplet=(16.0 if dom.startswith("k_uw") else 8.0) * nb ** 4)
         if args.precision == "bf16x3":
             roof["mfma_issue_frac"] = 3 * achieved / peak
+        if traffic:
+            roof["hbm_GBps"] = traffic / (ms / max(1, n) * 1e-3) / 1e9
         res = {
             "metric": "3ph matrix elements/s + κ wall-time, mixed vs FP64 GPU @≤1e-4 rel κ err",
             "value": elements / t_mx,
