@@ -171,7 +171,7 @@ size_t uw_lds(int maxblk, int maxa1) {
     typedef typename Traits<PREC>::T T;
     constexpr int RS = kpad(NB) + 8;
     size_t s = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB * NB;   // U
-    s += sizeof(C2<T>) * 27 * (size_t)maxblk + sizeof(int) * (size_t)maxblk;
+    s += sizeof(C2<T>) * 9 * (size_t)maxblk + sizeof(int) * (size_t)maxblk;
     s = (s + 15) & ~(size_t)15;
     return s + sizeof(cd) * (size_t)std::max(maxa1, 1);
 }
@@ -208,12 +208,12 @@ __global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
     const int t = u.t0 + tl;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
     // LDS: U (BF16X3: split planes re hi/lo, im hi/lo, [b3][ib] with row stride RS; else
-    // planar [re/im][ib][b3]), Phi blocks [i][block][j][k], a3 of each block, phases
+    // planar [re/im][ib][b3]), Phi blocks of one i [block][j][k], a3 of each block, phases
     constexpr size_t ub = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB2;
     __bf16 *ut = (__bf16 *)smem;
     T *us = (T *)smem;
     C2<T> *pb = (C2<T> *)(smem + ub);
-    int *pa3 = (int *)(pb + 27 * u.maxblk);
+    int *pa3 = (int *)(pb + 9 * u.maxblk);
     cd *ph = (cd *)(smem + align16((size_t)((unsigned char *)(pa3 + u.maxblk) - smem)));
     const int q2 = u.tq2[t], q3 = u.tq3[t];
     if constexpr (PREC == 2) {
@@ -257,30 +257,31 @@ __global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
     }
     __syncthreads();
     const int bo0 = u.a1_boff[a1], nblk = u.a1_boff[a1 + 1] - bo0;
-    for (int it = threadIdx.x; it < 27 * nblk; it += blockDim.x) {
-        const int bl = it / 27, ijk = it % 27, i = ijk / 9, jk = ijk % 9;
-        const int4 b = u.blist[bo0 + bl];
-        double ax = 0, ay = 0;
-        for (int f = b.z; f < b.w; ++f) {
-            const double m = u.mwm[27 * f + ijk];
-            const cd p = ph[f - f0];
-            ax += m * p.x;
-            ay += m * p.y;
-        }
-        pb[(i * u.maxblk + bl) * 9 + jk] = {(T)ax, (T)ay};
-        if (ijk == 0) pa3[bl] = b.y;
-    }
-    __syncthreads();
+    for (int bl = threadIdx.x; bl < nblk; bl += blockDim.x) pa3[bl] = u.blist[bo0 + bl].y;
     const T *E3r = (const T *)u.Ef_re + (size_t)q3 * NB2, *E3i = (const T *)u.Ef_im + (size_t)q3 * NB2;
     T *Wt = (T *)u.W + (size_t)tl * 2 * NB * NB2;
     for (int i = 0; i < 3; ++i) {
+        // Phi blocks of (a1, i): [block][j][k]
+        for (int it = threadIdx.x; it < 9 * nblk; it += blockDim.x) {
+            const int bl = it / 9, jk = it % 9;
+            const int4 b = u.blist[bo0 + bl];
+            double ax = 0, ay = 0;
+            for (int f = b.z; f < b.w; ++f) {
+                const double m = u.mwm[27 * f + 9 * i + jk];
+                const cd p = ph[f - f0];
+                ax += m * p.x;
+                ay += m * p.y;
+            }
+            pb[it] = {(T)ax, (T)ay};
+        }
+        __syncthreads();
         for (int e = threadIdx.x; e < NB2 / 4; e += blockDim.x) {
             const int ib = e / (NB / 4), b3 = 4 * (e % (NB / 4)), a2 = ib / 3, j = ib % 3;
             const int s0 = u.a12_boff[a1 * NA + a2] - bo0, s1 = u.a12_boff[a1 * NA + a2 + 1] - bo0;
             T ur[4] = {0, 0, 0, 0}, ui[4] = {0, 0, 0, 0};
             for (int bl = s0; bl < s1; ++bl) {
                 const int a3 = pa3[bl];
-                const C2<T> *pp = pb + (i * u.maxblk + bl) * 9 + 3 * j;
+                const C2<T> *pp = pb + bl * 9 + 3 * j;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const C2<T> p = pp[k];
@@ -769,7 +770,7 @@ size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     const size_t kp = kpad(nb), nw = nb / 16, nmw = (nb + 31) / 32;
     const size_t ks = (precision == TC3_PREC_BF16X3) ? (nb + 31) / 32 : nb / 4;
     size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * (kp + 8) : es * 2 * nb * nb;
-    uw += 2 * es * 27 * (size_t)maxblk + 4 * (size_t)maxblk;
+    uw += 2 * es * 9 * (size_t)maxblk + 4 * (size_t)maxblk;
     uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
     size_t ps = es * (2 * nty * nb + nb + nw * kChunk * 16 * nty);
     ps = ((ps + 15) & ~(size_t)15);
@@ -787,14 +788,14 @@ size_t mfma_split_elems(int nb) { return (size_t)kPlanes * nb * kpad(nb); }
 bool mfma_supported(int nb, int precision) {
     if (nb != 48 && nb != 96) return false;   // nb = 3 na and a multiple of 16
     // IFC blocks and entries per atom are bounded separately by the caller
-    return lds_need(nb, 4, precision, 0, 0) + 32 * 1024 <= 160 * 1024;
+    return lds_need(nb, 4, precision, 0, 0) + 8 * 1024 <= 160 * 1024;
 }
 
 int mfma_temperature_block(int nb, int nT, int precision) {
     for (int nty : {4, 8, 16})
-        if (nty >= nT && lds_need(nb, nty, precision, 0, 0) + 32 * 1024 <= 160 * 1024) return nty;
+        if (nty >= nT && lds_need(nb, nty, precision, 0, 0) + 8 * 1024 <= 160 * 1024) return nty;
     for (int nty : {16, 8, 4})
-        if (lds_need(nb, nty, precision, 0, 0) + 32 * 1024 <= 160 * 1024) return nty;
+        if (lds_need(nb, nty, precision, 0, 0) + 8 * 1024 <= 160 * 1024) return nty;
     return 0;
 }
 

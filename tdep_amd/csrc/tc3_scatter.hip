@@ -73,10 +73,19 @@ __global__ void k_phi(int na, int t0, const int *__restrict__ tq2, const int *__
     }
 }
 
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi), returned as FP32 values.
+__device__ inline void split_bf16(float x, float &h, float &l) {
+    h = (float)(__bf16)x;
+    l = (float)(__bf16)(x - h);
+}
+// a.b ~ ah.bh + ah.bl + al.bh: the split-BF16x3 product (each term exact in FP32).
+__device__ inline float mul3(float ah, float al, float bh, float bl) { return fmaf(ah, bh, fmaf(ah, bl, al * bh)); }
+
 // Generic batched complex GEMM, one 16x16 output tile per workgroup.
 // C_z(m,n) = sum_k A_z(m,k) B_z(k,n); matrices addressed by element strides, the batch
-// index z selecting A/B through optional index tables (table[z / div]).
-template <typename T>
+// index z selecting A/B through optional index tables (table[z / div]).  SPLIT: every
+// FP32 product taken as the split-BF16x3 product, as the matrix cores form it.
+template <typename T, bool SPLIT = false>
 __global__ void k_cgemm(int M, int N, int K, int ntm, int ntn,
                         const cplx<T> *__restrict__ A, const int *__restrict__ aidx, int adiv, long sA, int am, int ak,
                         const cplx<T> *__restrict__ B, const int *__restrict__ bidx, int bdiv, long sB, int bk, int bn,
@@ -101,8 +110,18 @@ __global__ void k_cgemm(int M, int N, int K, int ntm, int ntn,
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
             const cplx<T> a = As[ty][kk], b = Bs[kk][tx];
-            rx += a.x * b.x - a.y * b.y;
-            ry += a.x * b.y + a.y * b.x;
+            if constexpr (SPLIT) {
+                float arh, arl, aih, ail, brh, brl, bih, bil;
+                split_bf16(a.x, arh, arl);
+                split_bf16(a.y, aih, ail);
+                split_bf16(b.x, brh, brl);
+                split_bf16(b.y, bih, bil);
+                rx += mul3(arh, arl, brh, brl) - mul3(aih, ail, bih, bil);
+                ry += mul3(arh, arl, bih, bil) + mul3(aih, ail, brh, brl);
+            } else {
+                rx += a.x * b.x - a.y * b.y;
+                ry += a.x * b.y + a.y * b.x;
+            }
         }
         __syncthreads();
     }
@@ -277,7 +296,7 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
     }
 }
 
-template <typename T>
+template <typename T, bool SPLIT = false>
 int launch_chain(tc3_ctx *c, hipStream_t st, int t0, int nt, const cplx<T> *E1, const cplx<T> *Ef,
                  cplx<T> *phi, cplx<T> *t1, cplx<T> *t2, cplx<T> *psi, const int *d_tq1, const int *d_tq2,
                  const int *d_tq3, const double *d_qcart, const int *d_a1off, const int *d_blk3,
@@ -289,16 +308,16 @@ int launch_chain(tc3_ctx *c, hipStream_t st, int t0, int nt, const cplx<T> *E1, 
     // stage A: t1[tl](b1, n) = sum_ia E1[q1](ia, b1) phi[tl](ia, n)
     {
         const int ntm = (nb + 15) / 16, ntn = (int)((nb2 + 15) / 16);
-        hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)((long)nt * ntm * ntn)), dim3(256), 0, st, nb, (int)nb2, nb,
+        hipLaunchKernelGGL((k_cgemm<T, SPLIT>), dim3((unsigned)((long)nt * ntm * ntn)), dim3(256), 0, st, nb, (int)nb2, nb,
                            ntm, ntn, E1, d_tq1 + t0, 1, nb2, 1, nb, phi, (const int *)nullptr, 1, nb3, (int)nb2, 1,
                            t1, nb3, (int)nb2, 1);
     }
     const int ntm = (nb + 15) / 16, ntn = (nb + 15) / 16;
     // stage B: t2[tl,b1](b2, ic) = sum_ib E2[q2](ib, b2) t1[tl,b1](ib, ic)
-    hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)((long)nt * nb * ntm * ntn)), dim3(256), 0, st, nb, nb, nb, ntm,
+    hipLaunchKernelGGL((k_cgemm<T, SPLIT>), dim3((unsigned)((long)nt * nb * ntm * ntn)), dim3(256), 0, st, nb, nb, nb, ntm,
                        ntn, Ef, d_tq2 + t0, nb, nb2, 1, nb, t1, (const int *)nullptr, 1, nb2, nb, 1, t2, nb2, nb, 1);
     // stage C: psi[tl,b1](b2, b3) = sum_ic t2[tl,b1](b2, ic) E3[q3](ic, b3)
-    hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)((long)nt * nb * ntm * ntn)), dim3(256), 0, st, nb, nb, nb, ntm,
+    hipLaunchKernelGGL((k_cgemm<T, SPLIT>), dim3((unsigned)((long)nt * nb * ntm * ntn)), dim3(256), 0, st, nb, nb, nb, ntm,
                        ntn, t2, (const int *)nullptr, 1, nb2, nb, 1, Ef, d_tq3 + t0, nb, nb2, nb, 1, psi, nb2, nb, 1);
     return 0;
 }
@@ -379,10 +398,6 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     c->scatter_tables_gen = c->gen_model;
 
     const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
-    if (precision == TC3_PREC_BF16X3 && !use_mfma) {
-        cleanup();
-        return c->fail(5, "split-BF16x3 needs 16 or 32 atoms (48 or 96 bands)");
-    }
     // generic path: batch of intermediates of 4 * nb^3 complex per triplet
     const long nb3 = (long)nb * nb * nb;
     const size_t elem = (precision == TC3_PREC_FP64) ? sizeof(cd) : sizeof(cf);
@@ -524,13 +539,18 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             TRY(hipEventRecord(ev[1], st));
             hipLaunchKernelGGL(k_epilogue<double>, dim3(nt * nb), dim3(kBlock), epi_shmem, st, ea,
                                (const double *)psi, 0);
-        } else if (precision == TC3_PREC_FP32) {
+        } else if (precision == TC3_PREC_FP32 || precision == TC3_PREC_BF16X3) {
             auto *w = (cplx<float> *)d_work;
             cplx<float> *phi = w, *t1 = w + (size_t)batch * nb3, *t2 = t1 + (size_t)batch * nb3,
                         *psi = t2 + (size_t)batch * nb3;
-            launch_chain<float>(c, st, (int)t0, nt, (const cplx<float> *)d_E1, (const cplx<float> *)d_Ef, phi, t1,
-                                t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2, d_lv3, d_mwm,
-                                phi_shmem);
+            if (precision == TC3_PREC_FP32)
+                launch_chain<float>(c, st, (int)t0, nt, (const cplx<float> *)d_E1, (const cplx<float> *)d_Ef, phi,
+                                    t1, t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2, d_lv3, d_mwm,
+                                    phi_shmem);
+            else
+                launch_chain<float, true>(c, st, (int)t0, nt, (const cplx<float> *)d_E1, (const cplx<float> *)d_Ef,
+                                          phi, t1, t2, psi, d_tq1, d_tq2, d_tq3, d_qcart, d_a1off, d_blk3, d_lv2,
+                                          d_lv3, d_mwm, phi_shmem);
             TRY(hipEventRecord(ev[1], st));
             hipLaunchKernelGGL(k_epilogue<float>, dim3(nt * nb), dim3(kBlock), epi_shmem, st, ea,
                                (const float *)psi, 0);

@@ -69,9 +69,13 @@ def test_fp64_scattering_matrix_matches_oracle(inputs_dir, oracle_setup, name, g
     assert _rel(xi, sr.Xi) < 1e-9
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-def test_mixed_precision_kappa_within_1e4(inputs_dir, prec):
-    e = _engine(inputs_dir, "sl16", (4, 4, 2))
+@pytest.mark.parametrize("name,grid,prec", [("sl16", (4, 4, 2), "fp32"), ("sl16", (4, 4, 2), "bf16x3"),
+                                            ("rocksalt", (6, 6, 6), "fp32"), ("rocksalt", (6, 6, 6), "bf16x3"),
+                                            ("sl32", (4, 4, 1), "bf16x3")])
+def test_mixed_precision_kappa_within_1e4(inputs_dir, name, grid, prec):
+    """Mixed-precision contraction against the FP64 GPU path: 2-atom rocksalt (small-band
+    path), 16- and 32-atom superlattices (MFMA kernels for 48 and 96 bands)."""
+    e = _engine(inputs_dir, name, grid)
     T = [100.0, 300.0, 700.0]
     e.scattering_rates(T, precision="fp64")
     k64 = e.kappa()
