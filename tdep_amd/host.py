@@ -1,0 +1,66 @@
+"""Command-line host for the three-phonon thermal-conductivity path.
+
+Run in a directory holding infile.ucposcar, infile.forceconstant,
+infile.forceconstant_thirdorder (and infile.isotopes): reads them, runs the harmonic
+step, three-phonon scattering, RTA + iterative BTE (+ off-diagonal) kappa on the GPU, and
+writes outfile.thermal_conductivity and outfile.thermal_conductivity_grid.hdf5 like the
+reference program (src/thermal_conductivity/main.f90:246-296, 323-328), for every
+requested temperature (one pass; files of temperatures after the first get a suffix).
+
+Options follow the reference's (src/thermal_conductivity/options.f90:44-237) for the
+parts of the path built here:
+
+    python -m tdep_amd.host -qg 12 12 12 --temperature 300 [--precision bf16x3]
+"""
+import argparse
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def parse(argv=None):
+    ap = argparse.ArgumentParser(prog="thermal_conductivity (MI355X)", description=__doc__.split("\n\n")[0])
+    ap.add_argument("-qg", "--qpoint_grid", type=int, nargs=3, default=[26, 26, 26])
+    ap.add_argument("--temperature", type=float, nargs="+", default=[300.0],
+                    help="one or more temperatures (K); all share one scattering pass")
+    ap.add_argument("-it", "--integrationtype", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--sigma", type=float, default=1.0, help="fixed smearing (THz) for integrationtype 1")
+    ap.add_argument("--iterative_tolerance", type=float, default=1e-5)
+    ap.add_argument("--iterative_maxsteps", type=int, default=200)
+    ap.add_argument("--classical", action="store_true")
+    ap.add_argument("--readiso", action="store_true", help="read infile.isotopes (required: masses come from it)")
+    ap.add_argument("--noisotope", action="store_true",
+                    help="accepted for compatibility; isotope scattering is not part of this path")
+    ap.add_argument("--notr", action="store_true", help="do not use time-reversal symmetry")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="precision of the |Psi|^2 contraction")
+    ap.add_argument("--no-offdiag", action="store_true", help="skip the off-diagonal (coherence) term")
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--directory", default=".")
+    return ap.parse_args(argv)
+
+
+def main(argv=None):
+    a = parse(argv)
+    from tdep_amd.engine import Engine
+    eng = Engine.from_directory(a.directory, tuple(a.qpoint_grid), device=a.device, timereversal=not a.notr)
+    eng.dispersions()
+    eng.scattering_rates(a.temperature, precision=a.precision, integrationtype=a.integrationtype,
+                         sigma=a.sigma, classical=a.classical)
+    eng.kappa(maxiter=a.iterative_maxsteps, tol=a.iterative_tolerance, offdiag=not a.no_offdiag)
+    for iT, T in enumerate(a.temperature):
+        sfx = "" if iT == 0 else f".T{T:g}"
+        with open(os.path.join(a.directory, "outfile.thermal_conductivity" + sfx), "w") as f:
+            f.write(eng.outfile_text(iT))
+        eng.write_grid_hdf5(os.path.join(a.directory, "outfile.thermal_conductivity_grid" + sfx + ".hdf5"), iT)
+    t = eng.timings()
+    print(f"harmonic {t['harmonic']:.3f} s, scattering {t['scatter']:.3f} s, kappa {t['kappa']:.3f} s "
+          f"(BTE iterations {list(int(x) for x in eng.niter)})")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

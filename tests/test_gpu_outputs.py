@@ -1,0 +1,113 @@
+"""Output files of the path against the engine's own results, through the C ABI.
+
+outfile.thermal_conductivity_grid.hdf5 (lo_phonon_dispersions%write_to_hdf5,
+type_phonon_dispersions.f90:316-641): layout, units and attributes, and the reference's own
+consistency check (tests/thermal_conductivity/test_thermal_conductivity.py:25-40): the
+per-mode conductivity summed over the grid and divided by the number of q-points equals
+the printed kappa.  The file is read back with h5py from the image's conda interpreter.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CONDA_PY = "/opt/conda/bin/python3.9"
+HZ = 1.0 / (6.582119514e-16 / 27.21138602)   # lo_frequency_Hartree_to_Hz
+
+_DUMP = r"""
+import json, sys, h5py, numpy as np
+out = {"attrs": {}, "data": {}}
+with h5py.File(sys.argv[1], "r") as f:
+    for k, v in f.attrs.items():
+        v = v.decode() if isinstance(v, bytes) else np.asarray(v).tolist()
+        out["attrs"][k] = v
+    for k in f:
+        d = f[k]
+        a = d[()]
+        out["data"][k] = {"shape": list(a.shape), "unit": d.attrs["unit"].decode().strip(),
+                          "dimensions": d.attrs["dimensions"].decode().strip(), "values": a.tolist()}
+json.dump(out, sys.stdout)
+"""
+
+
+def _read(path):
+    if not os.path.exists(CONDA_PY):
+        pytest.skip("no interpreter with h5py in this image")
+    r = subprocess.run([CONDA_PY, "-s", "-c", _DUMP, path], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0 and "No module named 'h5py'" in r.stderr:
+        pytest.skip("h5py not importable")
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("rocksalt", (4, 4, 4))])
+def test_grid_hdf5(inputs_dir, tmp_path, name, grid):
+    from tdep_amd.engine import Engine, KAPPA_AU_TO_SI
+    print(f"[case] grid hdf5 {name} {grid}", flush=True)
+    e = Engine.from_directory(inputs_dir(name), grid, device=0)
+    e.dispersions()
+    e.scattering_rates([300.0], precision="fp64")
+    ks, ki, ko = e.kappa()
+    path = str(tmp_path / "outfile.thermal_conductivity_grid.hdf5")
+    e.write_grid_hdf5(path, 0)
+    h = _read(path)
+    nq = int(np.prod(grid))
+    nb = e.nb
+    assert h["attrs"]["number_of_atoms"] == [nb // 3]
+    assert h["attrs"]["number_of_bands"] == [nb]
+    assert h["attrs"]["number_of_qpoints"] == [nq]
+    assert h["attrs"]["meshtype"].strip() == "FFT"
+    assert h["attrs"]["temperature"] == [300.0]
+    d = h["data"]
+    expect = {"qpoints": ([nq, 3], "1/A", "q-vector,xyz"),
+              "frequencies": ([nq, nb], "Hz (angular)", "q-vector,mode"),
+              "group_velocities": ([nq, nb, 3], "m/s", "q-vector,mode,xyz"),
+              "eigenvectors_re": ([nq, nb, nb], "dimensionless", "q-vector,mode,atom-xyz"),
+              "eigenvectors_im": ([nq, nb, nb], "dimensionless", "q-vector,mode,atom-xyz"),
+              "lifetimes": ([nq, nb], "s", "q-vector,mode"),
+              "lattice_vectors": ([3, 3], "A", "xyz,xyz"),
+              "linewidths": ([nq, nb], "Hz", "q-vector,mode"),
+              "mean_free_paths": ([nq, nb], "m", "q-vector,mode"),
+              "thermal_conductivity": ([nq, nb, 3, 3], "W/mK", "q-vector,mode,xyz,xyz"),
+              "harmonic_heat_capacity": ([nq, nb], "J/K", "q-vector,mode")}
+    assert set(d) == set(expect)
+    for k, (shape, unit, dims) in expect.items():
+        assert d[k]["shape"] == shape and d[k]["unit"] == unit and d[k]["dimensions"] == dims, k
+    om, vel, egv = e.harmonic(full=True)
+    assert np.allclose(np.array(d["frequencies"]["values"]), om * HZ, rtol=1e-13, atol=0)
+    # Fortran egv(component, mode) per q-point appears as [q][mode][component]
+    egf = np.array(d["eigenvectors_re"]["values"]) + 1j * np.array(d["eigenvectors_im"]["values"])
+    assert np.abs(egf - egv.transpose(0, 2, 1)).max() == 0.0
+    lw_irr = e.linewidths(0)[0]
+    irr_of_full = e.mesh()["irr_of_full"]
+    lw = lw_irr[irr_of_full]
+    assert np.allclose(np.array(d["linewidths"]["values"]), lw * HZ, rtol=1e-13, atol=0)
+    # the reference's own check: sum of the per-mode kappa over the grid / N_q = kappa
+    kgrid = np.array(d["thermal_conductivity"]["values"]).sum(axis=(0, 1)) / nq
+    kref = ki[0] * KAPPA_AU_TO_SI
+    assert np.allclose(np.diag(kgrid), np.diag(kref), rtol=1e-7, atol=1e-12 * np.abs(kref).max())
+
+
+def test_host_writes_both_outfiles(inputs_dir, tmp_path):
+    """The command-line host in a directory of infiles writes the reference's two outfiles,
+    and the printed kappa equals the grid file's per-mode sum."""
+    import shutil
+    from tdep_amd import host
+    print("[case] host fcc_al 6x6x6", flush=True)
+    src = inputs_dir("fcc_al")
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    assert host.main(["-qg", "6", "6", "6", "--temperature", "300", "--readiso",
+                      "--directory", str(tmp_path)]) == 0
+    rows = np.loadtxt(tmp_path / "outfile.thermal_conductivity")
+    assert rows.ndim == 2 and rows.shape[1] == 6
+    h = _read(str(tmp_path / "outfile.thermal_conductivity_grid.hdf5"))
+    kgrid = np.array(h["data"]["thermal_conductivity"]["values"]).sum(axis=(0, 1)) / 216
+    # rows: SMA, collective correction, off-diagonal, total (main.f90:256-293); SMA +
+    # collective is the iterative kappa the grid file holds per mode
+    assert rows.shape[0] == 4
+    assert np.allclose(np.diag(kgrid), rows[0][:3] + rows[1][:3], rtol=1e-6)
