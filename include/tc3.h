@@ -65,6 +65,14 @@ int tc3_set_fc2(tc3_ctx *ctx, int npair, const int *a1, const int *a2, const dou
 int tc3_set_fc3(tc3_ctx *ctx, int ntrip, const int *a1, const int *a2, const int *a3,
                 const double *lv2, const double *lv3, const double *m);
 
+/* Isotope scattering (compute_isotope_scattering, scattering_isotope.f90:2-92): disorder
+ * holds the mass-disorder parameter g of every atom (mass_disorder_parameter,
+ * type_crystalstructure.f90:1505-1519, from infile.isotopes); enable = 1 adds the isotope
+ * term to the linewidths and the scattering matrix of every later tc3_scatter call (the
+ * reference's default; its --noisotope corresponds to enable = 0, the state after
+ * tc3_set_structure).  The term is zero when every g is zero. */
+int tc3_set_isotopes(tc3_ctx *ctx, const double *disorder, int enable);
+
 /* Space group, Gamma-centred FFT q-mesh, irreducible points, little groups and the
  * irreducible triplet lists: replaces uc%classify('wedge') (type_crystalstructure_symmetry.f90:7-91),
  * lo_generate_qmesh (type_qpointmesh.f90:452-657, type_qpointmesh_gridgeneration.f90:8-545)

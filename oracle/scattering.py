@@ -140,10 +140,51 @@ def occupations_and_smearing(p, mesh, disp, temperature, classical=False):
     return be, sigsq, sigavg
 
 
+ISOTOPE_PREFACTOR = U.PI / 4.0  # scattering.f90:29
+
+
+def isotope_strength(e1, e2, g):
+    """isotope_scattering_strength (scattering_isotope.f90:72-92): sum over atoms of
+    g_a (sum_x |conj(e1_ax) e2_ax|)^2.  e1: [component]; e2: [..., component]."""
+    na = len(g)
+    e2a = e2.reshape(e2.shape[:-1] + (na, 3))             # [..., atom, xyz]
+    e1a = np.conj(e1).reshape(na, 3)
+    f0 = np.abs(e1a * e2a).sum(axis=-1) ** 2              # [..., atom]
+    return f0 @ np.asarray(g, float)
+
+
+def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0):
+    """compute_isotope_scattering (scattering_isotope.f90:2-70) for one row (q1, b1):
+    returns the linewidth term and the row of scattering-matrix terms [q2, b2]."""
+    nb, nfull = disp.nb, mesh.nfull
+    om1 = disp.om_irr[q1, b1]
+    e1 = disp.egv_irr[q1][:, b1]
+    gam = 0.0
+    xi = np.zeros((nfull, nb))
+    for q2 in range(nfull):
+        om2 = disp.om_full[q2]
+        e2 = disp.egv_full[q2]                      # [component, mode]
+        st = isotope_strength(e1, e2.T, g)          # [mode]
+        i2 = mesh.irr_of_full[q2]
+        for b2 in range(nb):
+            if om2[b2] < U.FREQTOL:
+                continue
+            if integrationtype == 2:
+                sig = np.sqrt(sigsq[q1, b1] + sigsq[i2, b2])
+            else:
+                sig = sigma_fixed
+            psisq = st[b2] * ISOTOPE_PREFACTOR / nfull
+            f0 = psisq * om1 * om2[b2] * U.gauss(om1, om2[b2], sig)
+            gam += f0
+            xi[q2, b2] += f0 * om2[b2] / om1
+    return gam, xi
+
+
 def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
-                     integrationtype=2, sigma=1.0, rows_only=None):
-    """lo_scattering_rates%generate (scattering.f90:58-358), three-phonon only,
-    qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q)."""
+                     integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None):
+    """lo_scattering_rates%generate (scattering.f90:58-358): three-phonon scattering with
+    qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q), plus
+    isotope scattering (:229-232) when per-atom mass-disorder parameters are given."""
     nb = disp.nb
     nirr, nfull = mesh.nirr, mesh.nfull
     sr = ScatteringRates()
@@ -223,6 +264,13 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
         for b1 in myb:
             Xi[row_of[(q1, b1)]] += od[b1].reshape(-1)
     sr.ntriplets = ntrip
+    if isotope_g is not None:
+        for i, (q1, b1) in enumerate(rows):
+            if rows_only is not None and i not in rows_only:
+                continue
+            gi, xi = isotope_row(mesh, disp, sigsq, q1, b1, isotope_g, integrationtype, sr.sigma_fixed)
+            gamma_row[i] += gi
+            Xi[i] += xi.reshape(-1)
     # reduce + degeneracy average + F0 (:260-292)
     lw = np.zeros((nirr, nb))
     for i, (q, b) in enumerate(rows):

@@ -113,6 +113,8 @@ int tc3_set_structure(tc3_ctx *ctx, int na, const double *latvec, const double *
         Status s = build_crystal(ctx->cr, na, latvec, rfrac, species, mass);
         if (s.code) return ctx->fail(s.code, s.msg);
         ctx->nb = 3 * na;
+        ctx->iso_g.assign(na, 0.0);
+        ctx->iso_on = false;
         ctx->have_structure = true;
         ctx->have_fc2 = ctx->have_fc3 = ctx->have_mesh = ctx->have_harmonic = ctx->have_scatter = false;
         ++ctx->gen_model;
@@ -210,6 +212,22 @@ int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal) {
         ctx->have_mesh = true;
         ++ctx->gen_model;
         ctx->have_harmonic = ctx->have_scatter = false;
+        return 0;
+    });
+}
+
+int tc3_set_isotopes(tc3_ctx *ctx, const double *disorder, int enable) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->have_structure) return ctx->fail(5, "set the structure first");
+        if (enable && !disorder) return ctx->fail(5, "isotope scattering needs the mass-disorder parameters");
+        ctx->iso_g.assign(ctx->cr.na, 0.0);
+        if (disorder)
+            for (int a = 0; a < ctx->cr.na; ++a) {
+                if (!(disorder[a] >= 0.0)) return ctx->fail(3, "negative mass-disorder parameter");
+                ctx->iso_g[a] = disorder[a];
+            }
+        ctx->iso_on = enable != 0;
+        ctx->have_scatter = false;
         return 0;
     });
 }

@@ -52,6 +52,8 @@ struct tc3_ctx {
     int precision = 0;
     bool classical = false;
     std::vector<double> be, sigsq;            // [T][q][b], [q][b]
+    std::vector<double> iso_g;                // mass-disorder parameter per atom
+    bool iso_on = false;                      // isotope scattering requested
     std::vector<double> linewidth, qs, F0, mfp, smfp;  // [T][q][b](x3)
 
     // kappa results

@@ -261,20 +261,55 @@ __global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_q
     lw[i] = s;
 }
 
+// Isotope scattering inputs (compute_isotope_scattering, scattering_isotope.f90:2-70): the
+// mass-disorder parameter of every atom; null g means no isotope term.
+struct IsoArgs {
+    const double *g;               // na
+    const cd *egv_irr, *egv_full;  // [q][component][mode]
+    const double *om_irr, *om_full, *sigsq;
+    const int *row_b, *irr_of_full;
+    double sigma_fixed, pref;      // pi/4 / N_q
+    int na, integrationtype;
+    double *gam;                   // [row] linewidth term (written by the T = 0 workgroup)
+};
+
+__device__ inline double gauss_iso(double x, double mu, double sigma) {
+    // lo_gauss (gottochblandat_calculus.f90:268-288)
+    const double d = x - mu;
+    if (fabs(d) > 8.0 * sigma) return 0.0;
+    const double inv = 1.0 / sigma;
+    return exp(-d * d * 0.5 * inv * inv) * 0.398942280401433 * inv;
+}
+
 // Orbit averaging of one scattering-matrix row: symmetrize_and_distribute (mean of the
-// positive entries broadcast to the orbit, scattering_threephonon.f90:156-196) followed by
-// symmetrize_scatmat (plain mean, scattering.f90:312-354).  After the first step an orbit
-// with a positive entry holds one value, so the second step is evaluated on that value in
-// registers; each entry is read once and written once.  Workgroup per (row, T).
-template <typename XT>
+// positive three-phonon entries broadcast to the orbit, scattering_threephonon.f90:156-196),
+// then the isotope term added (it enters the row directly, scattering_isotope.f90:62-66),
+// then symmetrize_scatmat (plain mean, scattering.f90:312-354).  Without isotopes the first
+// step leaves an orbit with a positive entry holding one value, so the mean is evaluated
+// on that value in registers; each entry is read once and written once.  Workgroup per
+// (row, T).
+template <typename XT, bool ISO>
 __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const int *__restrict__ q1_off,
                          const int *__restrict__ orb_off, const int *__restrict__ members,
-                         XT *__restrict__ xi, size_t xi_row, size_t xi_T) {
+                         XT *__restrict__ xi, size_t xi_row, size_t xi_T, IsoArgs iso) {
     const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
     const int q1 = row_q[r];
     XT *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
     const int o0 = q1_off[q1], o1 = q1_off[q1 + 1];
     const long work = (long)(o1 - o0) * nb;
+    __shared__ cd e1[128];
+    __shared__ double gsh[64], red[kBlock];
+    int b1 = 0;
+    double om1 = 0, sg1 = 0;
+    if constexpr (ISO) {
+        b1 = iso.row_b[r];
+        om1 = iso.om_irr[(size_t)q1 * nb + b1];
+        sg1 = iso.sigsq[(size_t)q1 * nb + b1];
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) e1[k] = iso.egv_irr[((size_t)q1 * nb + k) * nb + b1];
+        for (int a = threadIdx.x; a < iso.na; a += blockDim.x) gsh[a] = iso.g[a];
+        __syncthreads();
+    }
+    double gsum = 0;
     for (long w = threadIdx.x; w < work; w += blockDim.x) {
         const int o = o0 + (int)(w / nb), b = (int)(w % nb);
         const int m0 = orb_off[o], m1 = orb_off[o + 1];
@@ -286,13 +321,51 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
             if (v > 0.0) { sp += v; ++n; }
         }
         double s = sa;
-        if (n > 0) {
-            const double avg = sp / n;
+        if constexpr (!ISO) {
+            if (n > 0) {
+                const double avg = sp / n;
+                s = 0;
+                for (int m = m0; m < m1; ++m) s += avg;
+            }
+        } else {
+            const double avg = (n > 0) ? sp / n : 0.0;
             s = 0;
-            for (int m = m0; m < m1; ++m) s += avg;
+            for (int m = m0; m < m1; ++m) {
+                const int q2 = members[m];
+                double v = (n > 0) ? avg : (double)x[(size_t)q2 * nb + b];
+                const double om2 = iso.om_full[(size_t)q2 * nb + b];
+                if (!(om2 < kFreqTol)) {
+                    double st = 0;
+                    for (int a = 0; a < iso.na; ++a) {
+                        double f0 = 0;
+                        for (int j = 0; j < 3; ++j) {
+                            const cd p = e1[3 * a + j], e2 = iso.egv_full[((size_t)q2 * nb + 3 * a + j) * nb + b];
+                            const double re = p.x * e2.x + p.y * e2.y, im = p.x * e2.y - p.y * e2.x;
+                            f0 += sqrt(re * re + im * im);
+                        }
+                        st += f0 * f0 * gsh[a];
+                    }
+                    const double sig = (iso.integrationtype == 2)
+                                           ? sqrt(sg1 + iso.sigsq[(size_t)iso.irr_of_full[q2] * nb + b])
+                                           : iso.sigma_fixed;
+                    const double f = st * iso.pref * om1 * om2 * gauss_iso(om1, om2, sig);
+                    gsum += f;
+                    v += f * om2 / om1;
+                }
+                s += v;
+            }
         }
         const XT out = (XT)(s / (m1 - m0));
         for (int m = m0; m < m1; ++m) x[(size_t)members[m] * nb + b] = out;
+    }
+    if (ISO && T == 0) {
+        red[threadIdx.x] = gsum;
+        __syncthreads();
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) iso.gam[r] = red[0];
     }
 }
 
@@ -570,18 +643,52 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
                        d_q1off, d_gpart, d_lw);
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs0, st));
-    if (c->xi_es == 8)
-        hipLaunchKernelGGL(k_xi_sym<double>, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1,
-                           d_orboff, d_members, (double *)c->d_xi, c->xi_row, xi_T);
-    else
-        hipLaunchKernelGGL(k_xi_sym<float>, dim3(nrows * nT), dim3(kBlock), 0, st, nrows, nb, d_rowq, d_orbq1,
-                           d_orboff, d_members, (float *)c->d_xi, c->xi_row, xi_T);
+    // isotope scattering (scattering.f90:229-232): on when requested and some atom has mass
+    // disorder
+    IsoArgs iso{};
+    bool use_iso = false;
+    for (double g : c->iso_g) use_iso = use_iso || (c->iso_on && g > 0.0);
+    if (use_iso) {
+        double *d_g;
+        TRY(c->pool_get("iso_g", sizeof(double) * na, (void **)&d_g));
+        TRY(c->pool_get("iso_gam", sizeof(double) * std::max(nrows, 1), (void **)&iso.gam));
+        TRY(hipMemcpyAsync(d_g, c->iso_g.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
+        iso.g = d_g;
+        iso.egv_irr = c->d_egv_irr; iso.egv_full = c->d_egv_full;
+        iso.om_irr = c->d_om_irr; iso.om_full = c->d_om_full; iso.sigsq = d_sigsq;
+        iso.row_b = d_rowb; iso.irr_of_full = c->d_irr_of_full;
+        iso.sigma_fixed = ea.sigma_fixed; iso.pref = (kPi / 4.0) / (double)nfull;
+        iso.na = na; iso.integrationtype = integrationtype;
+    }
+    {
+        const dim3 g(nrows * nT), b(kBlock);
+        if (c->xi_es == 8) {
+            if (use_iso)
+                hipLaunchKernelGGL((k_xi_sym<double, true>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
+                                   d_members, (double *)c->d_xi, c->xi_row, xi_T, iso);
+            else
+                hipLaunchKernelGGL((k_xi_sym<double, false>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
+                                   d_members, (double *)c->d_xi, c->xi_row, xi_T, iso);
+        } else {
+            if (use_iso)
+                hipLaunchKernelGGL((k_xi_sym<float, true>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
+                                   d_members, (float *)c->d_xi, c->xi_row, xi_T, iso);
+            else
+                hipLaunchKernelGGL((k_xi_sym<float, false>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
+                                   d_members, (float *)c->d_xi, c->xi_row, xi_T, iso);
+        }
+    }
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs1, st));
     TRY(hipEventRecord(ev1, st));
-    std::vector<double> lw((size_t)nrows * nT);
+    std::vector<double> lw((size_t)nrows * nT), lw_iso(use_iso ? nrows : 0);
     TRY(hipMemcpyAsync(lw.data(), d_lw, sizeof(double) * nrows * nT, hipMemcpyDeviceToHost, st));
+    if (use_iso) TRY(hipMemcpyAsync(lw_iso.data(), iso.gam, sizeof(double) * nrows, hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
+    // isotope linewidth first, three-phonon added (scattering.f90:226-236)
+    if (use_iso)
+        for (int T = 0; T < nT; ++T)
+            for (int r = 0; r < nrows; ++r) lw[(size_t)T * nrows + r] = lw_iso[r] + lw[(size_t)T * nrows + r];
     float ms = 0;
     hipEventElapsedTime(&ms, ev0, ev1);
     hipEventDestroy(ev0);

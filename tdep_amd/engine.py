@@ -70,6 +70,12 @@ class Engine:
         eng.set_qmesh(qgrid, timereversal)
         return eng
 
+    def set_isotopes(self, disorder, enable=True):
+        """Isotope scattering with per-atom mass-disorder parameters (scattering_isotope.f90);
+        enable=False is the reference's --noisotope."""
+        g = np.ascontiguousarray(disorder, dtype=np.float64)
+        self._check(self.lib.tc3_set_isotopes(self.ctx, dptr(g), int(bool(enable))))
+
     def set_structure(self, p):
         self.structure = p
         lat = np.ascontiguousarray(p.latvec.T.reshape(-1))  # columns -> consecutive triples

@@ -32,8 +32,7 @@ def parse(argv=None):
     ap.add_argument("--iterative_maxsteps", type=int, default=200)
     ap.add_argument("--classical", action="store_true")
     ap.add_argument("--readiso", action="store_true", help="read infile.isotopes (required: masses come from it)")
-    ap.add_argument("--noisotope", action="store_true",
-                    help="accepted for compatibility; isotope scattering is not part of this path")
+    ap.add_argument("--noisotope", action="store_true", help="switch off isotope scattering")
     ap.add_argument("--notr", action="store_true", help="do not use time-reversal symmetry")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"],
                     help="precision of the |Psi|^2 contraction")
@@ -47,6 +46,7 @@ def main(argv=None):
     a = parse(argv)
     from tdep_amd.engine import Engine
     eng = Engine.from_directory(a.directory, tuple(a.qpoint_grid), device=a.device, timereversal=not a.notr)
+    eng.set_isotopes(eng.structure.mass_disorder, enable=not a.noisotope)
     eng.dispersions()
     eng.scattering_rates(a.temperature, precision=a.precision, integrationtype=a.integrationtype,
                          sigma=a.sigma, classical=a.classical)

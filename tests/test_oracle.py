@@ -348,3 +348,41 @@ def test_c_port_of_reference_row_loop_matches_oracle(oracle_setup):
     sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
     gam, dt, nthr, nel = ref_port.run_rows(pre, 300.0, sr.rows)
     assert np.allclose(gam, sr.gamma_row, rtol=1e-10, atol=1e-14 * np.abs(sr.gamma_row).max())
+
+
+def test_isotope_strength_closed_form():
+    """isotope_scattering_strength: sum_a g_a (sum_x |conj(e1_ax) e2_ax|)^2 on hand-made
+    two-atom vectors (scattering_isotope.f90:72-92)."""
+    from oracle import scattering as S
+    e1 = np.array([1, 0, 0, 0, 1j, 0]) / np.sqrt(2)
+    e2 = np.array([[1, 0, 0, 0, 0, 1], [0, 1, 0, 1j, 1, 0]]) / np.sqrt(2)
+    st = S.isotope_strength(e1, e2, [0.1, 0.2])
+    assert np.allclose(st, [0.1 * 0.25, 0.2 * 0.25])
+
+
+def test_isotope_rates_linear_and_detailed_balance(oracle_setup):
+    """Isotope linewidth terms scale linearly with the disorder parameters, and the rate
+    f(l1 -> l2) = Xi(l1, l2) om1 / om2 equals f(l2 -> l1) between two irreducible points
+    (the Gaussian and the strength are symmetric in the pair)."""
+    from oracle import scattering as S
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (4, 4, 4))
+    be, sigsq, _ = S.occupations_and_smearing(p, mesh, disp, 300.0)
+    g = np.array([2e-3, 1e-3])
+    q1, q2 = 1, 3                                   # two irreducible points
+    f1, f2 = mesh.irr_full_index[q1], mesh.irr_full_index[q2]
+    gam_a, xa = S.isotope_row(mesh, disp, sigsq, q1, 2, g)
+    gam_b, xb = S.isotope_row(mesh, disp, sigsq, q1, 2, 3.0 * g)
+    assert gam_a > 0 and np.isclose(gam_b, 3.0 * gam_a, rtol=1e-13)
+    worst = 0.0
+    for b1 in range(disp.nb):
+        if disp.om_irr[q1, b1] < 1e-9:
+            continue
+        _, x1 = S.isotope_row(mesh, disp, sigsq, q1, b1, g)
+        for b2 in range(disp.nb):
+            if disp.om_irr[q2, b2] < 1e-9:
+                continue
+            _, x2 = S.isotope_row(mesh, disp, sigsq, q2, b2, g)
+            r12 = x1[f2, b2] * disp.om_irr[q1, b1] / disp.om_irr[q2, b2]
+            r21 = x2[f1, b1] * disp.om_irr[q2, b2] / disp.om_irr[q1, b1]
+            worst = max(worst, abs(r12 - r21) / max(abs(r12), 1e-300))
+    assert worst < 1e-12
