@@ -110,7 +110,8 @@ int tc3_get_harmonic(const tc3_ctx *ctx, int full, double *omega, double *vel, d
 /* Three-phonon scattering rates and scattering matrix for nT temperatures at once:
  * replaces lo_scattering_rates%generate (scattering.f90:58-358) with
  * compute_threephonon_scattering (scattering_threephonon.f90:2-217) for
- * qg3ph = qgrid, no isotope and no boundary terms.  integrationtype 1 (fixed sigma
+ * qg3ph = qgrid, plus isotope scattering when tc3_set_isotopes enabled it
+ * (compute_isotope_scattering, scattering_isotope.f90:2-70); no boundary term.  integrationtype 1 (fixed sigma
  * in THz) or 2 (adaptive); precision one of TC3_PREC_*.  The scattering matrix stays
  * on the device. */
 int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
@@ -144,7 +145,8 @@ int tc3_get_kappa_log(const tc3_ctx *ctx, int iT, int maxlen, double *log, int *
 int tc3_get_modes(const tc3_ctx *ctx, int iT, double *Fn, double *kappa_mode);
 
 /* Wall times (seconds) of the last calls: [0] harmonic, [1] scatter, [2] kappa total,
- * [3] BTE only, [4] off-diagonal only, [5] scatter kernel time (HIP events).  n <= 8. */
+ * [3] BTE only, [4] off-diagonal only, [5] scatter kernel time (HIP events),
+ * [6] adjoint total, [7] adjoint kernel time (HIP events).  n <= 8. */
 int tc3_get_timings(const tc3_ctx *ctx, double *t, int n);
 
 /* Device time (ms, HIP events on the context's stream) and launch count of one kernel
@@ -154,6 +156,26 @@ int tc3_get_timings(const tc3_ctx *ctx, double *t, int n);
  * No counterpart in the reference (its timers are host wall-clock buckets). */
 int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches,
                          int64_t *batch);
+
+/* Adjoint of the linewidths with respect to the third-order force constants: for a
+ * cotangent dL_dlinewidth (nT*nirr*nb, of the linewidths tc3_get_scattering returns, all
+ * temperatures of the last tc3_scatter call) writes dL_dfc3 (27*ntrip), the gradient with
+ * respect to the tensors m of tc3_set_fc3, in the caller's triplet order and layout
+ * m[27 t + 9 k + 3 j + i].  Differentiates the three-phonon part of the linewidth
+ * (compute_threephonon_scattering, scattering_threephonon.f90:88-135, pretransform_phi3
+ * :275-317) through the degeneracy average (scattering.f90:260-277); the isotope term
+ * does not depend on the force constants.  |Psi|^2 is recomputed, with the contraction in
+ * the given precision (TC3_PREC_*) and FP64 accumulation of the gradient.  Replaces nothing
+ * in the reference, which has no derivatives; the forward it differentiates is
+ * lo_scattering_rates%generate (scattering.f90:58-358).  Needs tc3_scatter first. */
+int tc3_adjoint_gamma(tc3_ctx *ctx, const double *dL_dlinewidth, int precision, double *dL_dfc3);
+
+/* Adjoint of the single-mode (RTA) thermal conductivity with respect to the third-order
+ * force constants: dL_dkappa (nT*9, column-major like tc3_kappa's kappa_sma) -> dL_dfc3
+ * as in tc3_adjoint_gamma.  kappa_sma = symmetrize_kappa(get_kappa(F0)), F0 = v / (2
+ * linewidth) (kappa.f90:28-75, :308-324, scattering.f90:278-292); lo_chop is taken as the
+ * identity.  Needs tc3_scatter first (not tc3_kappa). */
+int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, double *dL_dfc3);
 
 /* Write outfile.thermal_conductivity_grid.hdf5 for temperature iT (layout of
  * lo_phonon_dispersions%write_to_hdf5, type_phonon_dispersions.f90:316-641, after the

@@ -170,6 +170,7 @@ int tc3_set_fc3(tc3_ctx *ctx, int ntrip, const int *a1, const int *a2, const int
         ctx->t_a1.resize(ntrip); ctx->t_a2.resize(ntrip); ctx->t_a3.resize(ntrip);
         ctx->t_lv2.resize(3 * (size_t)ntrip); ctx->t_lv3.resize(3 * (size_t)ntrip);
         ctx->t_mwm.resize(27 * (size_t)ntrip);
+        ctx->t_src = idx;
         ctx->blk3_off.assign((size_t)na * na * na + 1, 0);
         ctx->a1_off.assign(na + 1, 0);
         const auto &ism = ctx->cr.invsqrtmass;
@@ -320,6 +321,8 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
         ctx->temps.assign(temperatures, temperatures + nT);
         ctx->precision = precision;
         ctx->classical = classical != 0;
+        ctx->integrationtype = integrationtype;
+        ctx->sigma = sigma;
         // occupations and adaptive smearing (scattering.f90:131-167)
         ctx->be.assign((size_t)nT * nirr * nb, 0.0);
         ctx->sigsq.assign((size_t)nirr * nb, 0.0);
@@ -418,6 +421,66 @@ int tc3_get_modes(const tc3_ctx *ctx, int iT, double *Fn, double *kappa_mode) {
     if (Fn) std::copy(&ctx->Fn[iT * qb * 3], &ctx->Fn[iT * qb * 3] + qb * 3, Fn);
     if (kappa_mode) std::copy(&ctx->kappa_mode[iT * qb * 9], &ctx->kappa_mode[iT * qb * 9] + qb * 9, kappa_mode);
     return 0;
+}
+
+// dL/dm for a cotangent of the (degeneracy-averaged) linewidths of every temperature.
+static int adjoint_from_linewidths(tc3_ctx *ctx, const double *dlw, int precision, double *dL_dfc3) {
+    const int nb = ctx->nb, nirr = ctx->mesh.nirr, nT = ctx->nT, nrows = ctx->nrows;
+    const size_t qb = (size_t)nirr * nb;
+    // The forward averages the raw linewidths over degenerate bands band by band, in place
+    // (scattering.f90:264-277); each step is a symmetric averaging map, so the cotangent
+    // goes through the same steps in reverse order.
+    std::vector<double> g((size_t)nT * nrows, 0.0), buf(qb);
+    for (int T = 0; T < nT; ++T) {
+        for (size_t i = 0; i < qb; ++i) buf[i] = (ctx->om_irr[i] < kFreqTol) ? 0.0 : dlw[(size_t)T * qb + i];
+        for (int q = 0; q < nirr; ++q)
+            for (int b = nb - 1; b >= 0; --b) {
+                if (ctx->om_irr[(size_t)q * nb + b] < kFreqTol) continue;
+                const auto &dm = ctx->degen[q][b];
+                double f0 = 0;
+                for (int b2 : dm) f0 += buf[(size_t)q * nb + b2];
+                f0 /= (double)dm.size();
+                for (int b2 : dm) buf[(size_t)q * nb + b2] = f0;
+            }
+        for (int r = 0; r < nrows; ++r) g[(size_t)T * nrows + r] = buf[(size_t)ctx->row_q[r] * nb + ctx->row_b[r]];
+    }
+    std::vector<double> grad;
+    int r = run_adjoint_gamma(ctx, precision, g.data(), grad);
+    if (r) return r;
+    // sorted mass-weighted [i][j][k] -> caller's order and layout m[27 t + 9 k + 3 j + i];
+    // mwm = m / sqrt(m1 m2 m3) (type_forceconstant_thirdorder.f90:277)
+    const auto &ism = ctx->cr.invsqrtmass;
+    std::fill(dL_dfc3, dL_dfc3 + 27 * (size_t)ctx->nfc3, 0.0);
+    for (int n = 0; n < ctx->nfc3; ++n) {
+        const int t = ctx->t_src[n];
+        const double w = ism[ctx->t_a1[n]] * ism[ctx->t_a2[n]] * ism[ctx->t_a3[n]];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k)
+                    dL_dfc3[27 * (size_t)t + 9 * k + 3 * j + i] = grad[27 * (size_t)n + 9 * i + 3 * j + k] * w;
+    }
+    return 0;
+}
+
+int tc3_adjoint_gamma(tc3_ctx *ctx, const double *dL_dlinewidth, int precision, double *dL_dfc3) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->have_scatter) return ctx->fail(5, "run tc3_scatter first");
+        if (!dL_dlinewidth || !dL_dfc3) return ctx->fail(5, "null argument");
+        if (precision < 0 || precision > 2) return ctx->fail(5, "unknown precision mode");
+        return adjoint_from_linewidths(ctx, dL_dlinewidth, precision, dL_dfc3);
+    });
+}
+
+int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, double *dL_dfc3) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->have_scatter) return ctx->fail(5, "run tc3_scatter first");
+        if (!dL_dkappa || !dL_dfc3) return ctx->fail(5, "null argument");
+        if (precision < 0 || precision > 2) return ctx->fail(5, "unknown precision mode");
+        const size_t qb = (size_t)ctx->mesh.nirr * ctx->nb;
+        std::vector<double> dlw(qb * ctx->nT);
+        for (int T = 0; T < ctx->nT; ++T) kappa_rta_cotangent(ctx, T, dL_dkappa + 9 * (size_t)T, &dlw[qb * T]);
+        return adjoint_from_linewidths(ctx, dlw.data(), precision, dL_dfc3);
+    });
 }
 
 int tc3_get_timings(const tc3_ctx *ctx, double *t, int n) {

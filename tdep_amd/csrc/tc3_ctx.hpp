@@ -31,6 +31,7 @@ struct tc3_ctx {
     int nfc3 = 0;
     std::vector<int> t_a1, t_a2, t_a3, blk3_off, a1_off;
     std::vector<double> t_lv2, t_lv3, t_mwm;  // 3 / 3 / 27 per triplet (mwm [i][j][k])
+    std::vector<int> t_src;                   // caller's index of each sorted triplet
 
     tc3::Mesh mesh;
     tc3::TripletSet trip;
@@ -51,6 +52,8 @@ struct tc3_ctx {
     std::vector<double> temps;
     int precision = 0;
     bool classical = false;
+    int integrationtype = 2;
+    double sigma = 1.0;                       // fixed smearing (THz) of integrationtype 1
     std::vector<double> be, sigsq;            // [T][q][b], [q][b]
     std::vector<double> iso_g;                // mass-disorder parameter per atom
     bool iso_on = false;                      // isotope scattering requested
@@ -87,6 +90,7 @@ struct tc3_ctx {
     std::map<std::string, DevBuf> pool;
     long gen_model = 0;
     long scatter_tables_gen = -1;
+    long adjoint_tables_gen = -1;
     hipError_t pool_get(const std::string &name, size_t bytes, void **p, bool *fresh = nullptr) {
         DevBuf &b = pool[name];
         if (fresh) *fresh = false;
@@ -121,6 +125,9 @@ int run_harmonic(tc3_ctx *c);
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma);
 // tc3_kappa.hip
 int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag);
+void kappa_rta_cotangent(const tc3_ctx *c, int iT, const double *G, double *dlw);
+// tc3_adjoint.hip
+int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<double> &grad);
 // tc3_hdf5.cpp
 int write_grid_hdf5(const tc3_ctx *c, int iT, const char *path, std::string &err);
 

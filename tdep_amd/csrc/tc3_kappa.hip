@@ -401,6 +401,54 @@ __global__ void k_bte_check(int nirr, int nb, const double *__restrict__ g0part,
 
 }  // namespace
 
+// Cotangent of the linewidths for L = sum_ab G_ab kappa_RTA_ab at temperature iT, kappa_RTA
+// the symmetrised single-mode kappa from F0 = v / (2 linewidth) (get_kappa kappa.f90:28-75,
+// symmetrize_kappa :308-324, F0 from scattering.f90:278-292).  G and kappa column-major
+// (G[3 j + i] pairs with kappa_ij); lo_chop is taken as the identity.  The symmetrisation
+// averages R k R^T, so its adjoint averages R^T G R; each mode's kappa is linear in
+// 1 / linewidth, so d kappa / d linewidth = -w_q kappa_mode / linewidth.
+void kappa_rta_cotangent(const tc3_ctx *c, int iT, const double *G, double *dlw) {
+    const int nb = c->nb, nirr = c->mesh.nirr, nsym = (int)c->ops.size();
+    const size_t qb = (size_t)nirr * nb;
+    std::fill(dlw, dlw + qb, 0.0);
+    const double T = c->temps[iT];
+    if (T < kTemperatureTol) return;
+    double Gs[9] = {0};
+    for (int o = 0; o < nsym; ++o) {
+        const double *m = c->ops[o].m;  // row-major R
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+                double s = 0;
+                for (int j = 0; j < 3; ++j)
+                    for (int i = 0; i < 3; ++i) s += m[3 * i + a] * G[3 * j + i] * m[3 * j + b];
+                Gs[3 * b + a] += s / nsym;
+            }
+    }
+    for (int q = 0; q < nirr; ++q)
+        for (int b = 0; b < nb; ++b) {
+            const size_t i = (size_t)q * nb + b;
+            const double om = c->om_irr[i];
+            const double lw = c->linewidth[(size_t)iT * qb + i];
+            if (om < kFreqTol || !(lw > kFreqTol)) continue;
+            const double cv = c->classical ? kKbHartree : cv_host(T, om);
+            const double *F = &c->F0[((size_t)iT * qb + i) * 3];
+            const double *v = &c->vel_irr[i * 3];
+            double s = 0;
+            for (int o = 0; o < nsym; ++o) {
+                const double *m = c->ops[o].m;
+                double v0[3], v1[3];
+                for (int x = 0; x < 3; ++x) {
+                    v0[x] = m[3 * x] * F[0] + m[3 * x + 1] * F[1] + m[3 * x + 2] * F[2];
+                    v1[x] = m[3 * x] * v[0] + m[3 * x + 1] * v[1] + m[3 * x + 2] * v[2];
+                }
+                for (int y = 0; y < 3; ++y)
+                    for (int x = 0; x < 3; ++x) s += Gs[3 * y + x] * v0[x] * v1[y];
+            }
+            const double kmode = cv * s / nsym / c->cr.volume;
+            dlw[i] = -c->mesh.weight_irr[q] * kmode / lw;
+        }
+}
+
 int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     const int nb = c->nb, nirr = c->mesh.nirr, nfull = c->mesh.nfull, nT = c->nT, nsym = (int)c->ops.size();
     const int nrows = c->nrows;

@@ -92,6 +92,8 @@ class Engine:
                                          dptr(np.ascontiguousarray(fc2["lv2"].reshape(-1))), dptr(m)))
 
     def set_fc3(self, fc3):
+        self.fc3 = fc3
+        self.nfc3 = len(fc3["a1"])
         m = np.ascontiguousarray(np.transpose(fc3["m"], (0, 3, 2, 1)).reshape(-1))  # m[27t+9k+3j+i]
         self._check(self.lib.tc3_set_fc3(self.ctx, len(fc3["a1"]), iptr(fc3["a1"]), iptr(fc3["a2"]),
                                          iptr(fc3["a3"]), dptr(np.ascontiguousarray(fc3["lv2"].reshape(-1))),
@@ -199,10 +201,32 @@ class Engine:
         self._check(self.lib.tc3_get_modes(self.ctx, iT, dptr(Fn), dptr(km)))
         return Fn.reshape(self.nirr, self.nb, 3), km.reshape(self.nirr, self.nb, 3, 3).transpose(0, 1, 3, 2)
 
+    # -------------------------------------------------------------- adjoint
+    def adjoint_gamma(self, dL_dlinewidth, precision="fp64"):
+        """dL/dm [ntrip, 3, 3, 3] (m as in fc3["m"], Ha/Bohr^3) for a cotangent of the
+        linewidths [nT, nirr, nb] of the last scattering_rates call; |Psi|^2 recomputed."""
+        nT = len(self.temperatures)
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(dL_dlinewidth, np.float64),
+                                                 (nT, self.nirr, self.nb)).reshape(-1))
+        out = np.zeros(27 * self.nfc3)
+        self._check(self.lib.tc3_adjoint_gamma(self.ctx, dptr(g), PREC[precision], dptr(out)))
+        return out.reshape(self.nfc3, 3, 3, 3).transpose(0, 3, 2, 1).copy()
+
+    def adjoint_kappa_rta(self, dL_dkappa, precision="fp64"):
+        """dL/dm [ntrip, 3, 3, 3] for a cotangent of kappa_sma [nT, 3, 3] (as kappa() returns
+        it, atomic units); needs scattering_rates, not kappa."""
+        nT = len(self.temperatures)
+        d = np.broadcast_to(np.asarray(dL_dkappa, np.float64), (nT, 3, 3))
+        d = np.ascontiguousarray(d.transpose(0, 2, 1).reshape(-1))   # column-major per T
+        out = np.zeros(27 * self.nfc3)
+        self._check(self.lib.tc3_adjoint_kappa_rta(self.ctx, dptr(d), PREC[precision], dptr(out)))
+        return out.reshape(self.nfc3, 3, 3, 3).transpose(0, 3, 2, 1).copy()
+
     def timings(self):
         t = np.zeros(8)
         self._check(self.lib.tc3_get_timings(self.ctx, dptr(t), 8))
-        return dict(harmonic=t[0], scatter=t[1], kappa=t[2], bte=t[3], offdiag=t[4], scatter_kernels=t[5])
+        return dict(harmonic=t[0], scatter=t[1], kappa=t[2], bte=t[3], offdiag=t[4], scatter_kernels=t[5],
+                    adjoint=t[6], adjoint_kernels=t[7])
 
     def write_grid_hdf5(self, path, iT=0):
         self._check(self.lib.tc3_write_grid_hdf5(self.ctx, iT, path.encode()))

@@ -1,0 +1,152 @@
+"""Adjoint of the linewidths and of the RTA kappa with respect to the third-order force
+constants, through the C ABI and the torch.autograd.Function wrappers.
+
+Checks:
+  * against the FP64 oracle gradient (oracle/adjoint.py) on small meshes: relative error of
+    the gradient vector (2-norm) below 1e-10 in FP64;
+  * at the full rocksalt 16x16x16 workload: Euler's identity <m, dL/dm> = 2 L (the
+    linewidths are quadratic in Phi3), central finite differences of the GPU forward with a
+    step of 1e-4 relative: the linewidth functional within 1e-8 (a quadratic has no
+    truncation error), kappa_xx (RTA) within 1e-5;
+  * the FP32 adjoint against the FP64 one (2-norm relative error below 1e-4).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(inputs_dir, name, grid):
+    from tdep_amd.engine import Engine
+    print(f"[case] {name} {grid}", flush=True)
+    e = Engine.from_directory(inputs_dir(name), grid, device=0)
+    e.dispersions()
+    return e
+
+
+def _nrel(a, b):
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+def _oracle_grad(pre, temps, dl):
+    from oracle import adjoint as A
+    p, fc2, fc3, ops, mesh, disp = pre
+    g = 0.0
+    for iT, T in enumerate(temps):
+        g = g + A.gamma_adjoint(p, fc3, mesh, disp, ops, T, A.raw_cotangent(disp, dl[iT]))
+    return A.mwm_to_m(p, fc3, g)
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (4, 4, 4)), ("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2))])
+def test_adjoint_gamma_matches_oracle(inputs_dir, oracle_setup, name, grid):
+    pre = oracle_setup(name, grid)
+    e = _engine(inputs_dir, name, grid)
+    temps = [100.0, 300.0]
+    e.scattering_rates(temps, precision="fp64")
+    rng = np.random.default_rng(5)
+    dl = rng.standard_normal((len(temps), e.nirr, e.nb))
+    g = e.adjoint_gamma(dl)
+    ref = _oracle_grad(pre, temps, dl)
+    assert g.shape == ref.shape
+    assert _nrel(g, ref) < 1e-10
+    # Euler: <m, dL/dm> = 2 L
+    lw = np.stack([e.linewidths(i)[0] for i in range(len(temps))])
+    L = np.sum(dl * lw)
+    assert np.isclose(np.sum(g * e.fc3["m"]), 2.0 * L, rtol=1e-10)
+
+
+def test_adjoint_kappa_rta_matches_oracle(inputs_dir, oracle_setup):
+    from oracle import adjoint as A, scattering as S
+    pre = oracle_setup("rocksalt", (4, 4, 4))
+    p, fc2, fc3, ops, mesh, disp = pre
+    e = _engine(inputs_dir, "rocksalt", (4, 4, 4))
+    e.scattering_rates([300.0], precision="fp64")
+    G = np.zeros((1, 3, 3))
+    G[0, 0, 0] = 1.0
+    G[0, 0, 1] = 0.25
+    g = e.adjoint_kappa_rta(G)
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
+    cot = A.kappa_rta_cotangent(p, mesh, disp, ops, sr, 300.0, G[0])
+    ref = A.mwm_to_m(p, fc3, A.gamma_adjoint(p, fc3, mesh, disp, ops, 300.0, A.raw_cotangent(disp, cot)))
+    assert _nrel(g, ref) < 1e-9
+
+
+@pytest.fixture(scope="module")
+def rocksalt16(inputs_dir):
+    e = _engine(inputs_dir, "rocksalt_adjoint", (16, 16, 16))
+    e.scattering_rates([300.0], precision="fp64")
+    return e
+
+
+def _forward(e, m, prec="fp64"):
+    fc3 = dict(e.fc3)
+    fc3["m"] = m
+    e.set_fc3(fc3)
+    e.scattering_rates([300.0], precision=prec)
+    lw = e.linewidths(0)[0]
+    ks, _, _ = e.kappa(maxiter=0, offdiag=False)
+    return lw, ks[0]
+
+
+def test_full_size_adjoint_against_central_differences(rocksalt16):
+    e = rocksalt16
+    m0 = e.fc3["m"].copy()
+    lw0, k0 = _forward(e, m0)
+    rng = np.random.default_rng(17)
+    dl = rng.standard_normal(lw0.shape)
+    g_lw = e.adjoint_gamma(dl[None])
+    G = np.zeros((1, 3, 3))
+    G[0, 0, 0] = 1.0
+    g_k = e.adjoint_kappa_rta(G)
+    print(f"[adjoint] rocksalt 16^3 fp64: {e.timings()['adjoint']:.3f} s", flush=True)
+    L0 = np.sum(dl * lw0)
+    assert np.isclose(np.sum(g_lw * m0), 2.0 * L0, rtol=1e-10)
+    d = rng.standard_normal(m0.shape) * m0          # step 1e-4 relative, entry by entry
+    h = 1e-4
+    lwp, kp = _forward(e, m0 + h * d)
+    lwm, km = _forward(e, m0 - h * d)
+    _forward(e, m0)
+    fd_l = (np.sum(dl * lwp) - np.sum(dl * lwm)) / (2 * h)
+    fd_k = (kp[0, 0] - km[0, 0]) / (2 * h)
+    an_l, an_k = np.sum(g_lw * d), np.sum(g_k * d)
+    print(f"[adjoint] fd/an linewidths {fd_l:.12e} {an_l:.12e}  kappa_xx {fd_k:.12e} {an_k:.12e}", flush=True)
+    assert abs(fd_l - an_l) < 1e-8 * abs(an_l)
+    assert abs(fd_k - an_k) < 1e-5 * abs(an_k)
+
+
+def test_full_size_fp32_adjoint_close_to_fp64(rocksalt16):
+    e = rocksalt16
+    _forward(e, e.fc3["m"].copy())
+    G = np.zeros((1, 3, 3))
+    G[0, 0, 0] = 1.0
+    g64 = e.adjoint_kappa_rta(G, precision="fp64")
+    g32 = e.adjoint_kappa_rta(G, precision="fp32")
+    err = _nrel(g32, g64)
+    print(f"[adjoint] fp32 vs fp64 gradient: {err:.3e}", flush=True)
+    assert err < 1e-4
+
+
+def test_autograd_functions_route_to_the_adjoint(inputs_dir):
+    import torch
+    from tdep_amd import autograd as AG
+    e = _engine(inputs_dir, "rocksalt", (4, 4, 4))
+    m = torch.tensor(e.fc3["m"], dtype=torch.float64, requires_grad=True)
+    w = torch.from_numpy(np.random.default_rng(2).standard_normal((1, e.nirr, e.nb)))
+    lw = AG.linewidths(e, m, [300.0])
+    L = (w * lw).sum()
+    # another forward in between: the backward must restore the first one's state
+    m2 = (m.detach() * 1.01).requires_grad_(True)
+    AG.linewidths(e, m2, [500.0])
+    L.backward()
+    assert np.array_equal(e.fc3["m"], m.detach().numpy()) and list(e.temperatures) == [300.0]
+    ref = e.adjoint_gamma(w.numpy())
+    assert _nrel(m.grad.numpy(), ref) < 1e-12
+    assert np.isclose(float((m.grad * m).sum()), 2.0 * float(L), rtol=1e-10)
+    # kappa through autograd, including a composed linear parametrisation m = B x
+    x = torch.ones(2, dtype=torch.float64, requires_grad=True)
+    m0 = torch.tensor(e.fc3["m"])
+    B = torch.stack([m0 * 0.6, m0 * 0.4], dim=-1)
+    k = AG.kappa_rta(e, B @ x, [300.0])
+    k[0, 0, 0].backward()
+    gm = e.adjoint_kappa_rta(np.eye(3)[None] * np.array([1, 0, 0])[None, :, None])
+    assert np.allclose(x.grad.numpy(), [np.sum(gm * 0.6 * m0.numpy()), np.sum(gm * 0.4 * m0.numpy())], rtol=1e-10)
