@@ -41,6 +41,30 @@ def build(verbose=False, jobs=8):
     return LIB
 
 
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+HOST = os.path.join(HERE, "tc_host")
+
+
+def build_fortran_host(verbose=False):
+    """The Fortran host program (tdep_amd/fortran/tc_host.f90) linked against libtc3.so;
+    skipped with a message when no Fortran compiler is present."""
+    if not os.path.exists(FC):
+        sys.stderr.write(f"{FC} not found: the Fortran host is not built\n")
+        return None
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    src = [os.path.join(HERE, "fortran", f) for f in ("tc3_api.f90", "tc_host.f90")]
+    tmp = HOST + ".tmp"
+    cmd = [FC, "-O2", "-module-dir", objdir, "-o", tmp] + src + ["-L" + HERE, "-ltc3", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode:
+        raise RuntimeError("Fortran host build failed:\n" + r.stdout)
+    os.replace(tmp, HOST)
+    return HOST
+
+
 def _wait(item):
     name, p = item
     out, _ = p.communicate()
@@ -54,3 +78,4 @@ def _wait(item):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran_host(verbose="-v" in sys.argv))

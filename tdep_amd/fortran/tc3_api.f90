@@ -52,6 +52,13 @@ module tc3_api
             real(c_double), intent(in) :: lv2(3, *), lv3(3, *), m(3, 3, 3, *)
         end function
 
+        integer(c_int) function tc3_set_isotopes(ctx, disorder, enable) bind(C, name="tc3_set_isotopes")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: disorder(*)
+            integer(c_int), value :: enable
+        end function
+
         integer(c_int) function tc3_set_qmesh(ctx, dims, timereversal) bind(C, name="tc3_set_qmesh")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
@@ -126,6 +133,31 @@ module tc3_api
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: t(*)
             integer(c_int), value :: n
+        end function
+
+        integer(c_int) function tc3_write_grid_hdf5(ctx, iT, path) bind(C, name="tc3_write_grid_hdf5")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: iT
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+
+        integer(c_int) function tc3_adjoint_gamma(ctx, dL_dlinewidth, precision, dL_dfc3) &
+                bind(C, name="tc3_adjoint_gamma")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: dL_dlinewidth(*)
+            integer(c_int), value :: precision
+            real(c_double), intent(out) :: dL_dfc3(3, 3, 3, *)
+        end function
+
+        integer(c_int) function tc3_adjoint_kappa_rta(ctx, dL_dkappa, precision, dL_dfc3) &
+                bind(C, name="tc3_adjoint_kappa_rta")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: dL_dkappa(3, 3, *)
+            integer(c_int), value :: precision
+            real(c_double), intent(out) :: dL_dfc3(3, 3, 3, *)
         end function
     end interface
 

@@ -111,3 +111,36 @@ def test_host_writes_both_outfiles(inputs_dir, tmp_path):
     # collective is the iterative kappa the grid file holds per mode
     assert rows.shape[0] == 4
     assert np.allclose(np.diag(kgrid), rows[0][:3] + rows[1][:3], rtol=1e-6)
+
+
+def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
+    """The Fortran host (tdep_amd/tc_host, iso_c_binding to libtc3) run in a directory of
+    infiles writes the same outfile.thermal_conductivity numbers (both temperatures) and the
+    same grid file as the Python host on the same inputs."""
+    import shutil
+    from tdep_amd import host
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdep_amd", "tc_host")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (no Fortran compiler)")
+    print("[case] Fortran host rocksalt 4x4x4", flush=True)
+    src = inputs_dir("rocksalt")
+    df, dp = tmp_path / "fortran", tmp_path / "python"
+    for d in (df, dp):
+        d.mkdir()
+        for f in os.listdir(src):
+            shutil.copy(os.path.join(src, f), d / f)
+    args = ["-qg", "4", "4", "4", "--temperature", "100", "300", "--readiso"]
+    r = subprocess.run([exe] + args, cwd=df, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "BTE iterations" in r.stdout
+    assert host.main(args + ["--directory", str(dp)]) == 0
+    for name in ("outfile.thermal_conductivity", "outfile.thermal_conductivity.T300"):
+        tf, tp = open(df / name).read(), open(dp / name).read()
+        assert [l for l in tf.splitlines() if l.startswith("#")] == [l for l in tp.splitlines() if l.startswith("#")]
+        a, b = np.loadtxt(df / name), np.loadtxt(dp / name)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14 * np.abs(b).max())
+    for name in ("outfile.thermal_conductivity_grid.hdf5", "outfile.thermal_conductivity_grid.T300.hdf5"):
+        hf, hp = _read(str(df / name)), _read(str(dp / name))
+        assert hf["attrs"] == hp["attrs"]
+        for k in ("frequencies", "linewidths", "thermal_conductivity"):
+            assert np.allclose(hf["data"][k]["values"], hp["data"][k]["values"], rtol=1e-12, atol=0)

@@ -99,3 +99,32 @@ def test_outfile_format_matches_reference_fixture():
         assert ln == " " + "".join(" " + _fe(x) for x in nums)
     t = float(ref[1].split()[-1])
     assert ref[1] == "# Temperature:" + " " * 8 + _fe(t, 20, 12)
+
+
+@pytest.mark.parametrize("name,grid", [("rocksalt", (8, 8, 8)), ("sl16", (10, 10, 4))])
+def test_fortran_host_reads_inputs_like_the_python_readers(inputs_dir, name, grid):
+    """The Fortran host's readers (tdep_amd/fortran/tc_host.f90) give the same mesh and
+    triplet counts as the Python readers through the same library (host-only context), and
+    the same input checksums."""
+    import subprocess
+    from tdep_amd.engine import Engine
+    from tdep_amd import io as tio
+    exe = os.path.join(ROOT, "tdep_amd", "tc_host")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (no Fortran compiler)")
+    d = inputs_dir(name)
+    r = subprocess.run([exe, "-qg"] + [str(g) for g in grid] + ["--device", "-1", "--inputs-only"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    e = Engine.from_directory(d, grid, device=-1)
+    assert [int(x) for x in out["mesh"]] == [e.nsym, e.nfull, e.nirr, e.ntriplets, e.nrows]
+    p, fc2, fc3 = tio.read_directory(d)
+    f = lambda k: np.array([float(x) for x in out[k][1:]])
+    assert int(out["structure"][0]) == p.na
+    assert np.allclose(f("structure"), [np.abs(p.latvec).sum(), p.rfrac.sum(), p.mass.sum()], rtol=1e-13)
+    assert int(out["fc2"][0]) == len(fc2["a1"]) and int(out["fc3"][0]) == len(fc3["a1"])
+    assert np.allclose(f("fc2"), [np.abs(fc2["m"]).sum(), fc2["m"][:, 0, 1].sum(), np.abs(fc2["lv2"]).sum()],
+                       rtol=1e-12, atol=1e-16)
+    assert np.allclose(f("fc3"), [np.abs(fc3["m"]).sum(), fc3["m"][:, 0, 1, 2].sum(), np.abs(fc3["lv2"]).sum(),
+                                  np.abs(fc3["lv3"]).sum()], rtol=1e-12, atol=1e-16)
