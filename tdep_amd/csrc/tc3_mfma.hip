@@ -386,7 +386,7 @@ template <int PREC, int NB, int NTY>
 size_t psi_lds() {
     typedef typename Stage<PREC>::T S;
     constexpr int NW = NB / 16, NMW = MaskWords<NB>::n, KS = KSteps<PREC, NB>::n;
-    size_t s = sizeof(S) * (2 * NTY * NB + NB + NW * kChunk * 16 * NTY);
+    size_t s = sizeof(S) * (2 * NTY * NB + 2 * NB + NW * kChunk * 16 * NTY) + sizeof(float) * NB;
     s = (s + 15) & ~(size_t)15;
     s += (PREC == 2) ? (size_t)KS * kPlanes * 64 * 16 : 0;    // E1 fragments
     s += sizeof(double) * (NB + 16 + 16 * NTY) + sizeof(float) * NB + sizeof(int) * 16 +
@@ -419,8 +419,11 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     S *n2h = (S *)smem;                    // [NTY][NB]     n2 + 1/2
     S *n3h = n2h + NTY * NB;               // [NTY][NB]     n3 + 1/2
     S *sg2s = n3h + NTY * NB;              // [NB]          sigma^2 of q2 modes
-    S *Xp = sg2s + NB;                     // [NW][kChunk][16][NTY] per-wave partial sums over b3
-    unsigned char *p8 = smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + NB + NW * kChunk * 16 * NTY));
+    S *sg3s = sg2s + NB;                   // [NB]          sigma^2 of q3 modes
+    S *Xp = sg3s + NB;                     // [NW][kChunk][16][NTY] per-wave partial sums over b3
+    float *w3f = (float *)(Xp + NW * kChunk * 16 * NTY);   // [NB] omega of q3 modes
+    unsigned char *p8 = smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + 2 * NB + NW * kChunk * 16 * NTY) +
+                                       sizeof(float) * NB);
     v8bf *e1s = (v8bf *)p8;                // [KS][6 planes][64 lanes] (BF16X3)
     if constexpr (PREC == 2) p8 += (size_t)KS * kPlanes * 64 * 16;
     double *w2s = (double *)p8;            // [NB]
@@ -440,6 +443,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         w2s[b] = om2[b];
         w2f[b] = (float)om2[b];
         sg2s[b] = (S)a.sigsq[(size_t)i2 * NB + b];
+        sg3s[b] = (S)a.sigsq[(size_t)i3 * NB + b];
+        w3f[b] = (float)om3[b];
     }
     if (threadIdx.x < 16) {
         const int b1 = 16 * bb + threadIdx.x;
@@ -491,17 +496,14 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     const int b3w = 16 * w;                         // first b3 of this wave
     const int sh3 = (b3w & 31);                     // bit of b3w in its mask word
     unsigned ok13 = 0;
-    S sg3[4], bm[4];
-    float x3[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int b3 = b3w + out_row<PREC>(lane, r);
         const double w3 = om3[b3];
-        x3[r] = (float)w3;
-        sg3[r] = (S)a.sigsq[(size_t)i3 * NB + b3];
         if (row1 >= 0 && !(w3 < kTol) && !(fabs(w1 - w3) < kTol)) ok13 |= 1u << (sh3 + out_row<PREC>(lane, r));
-        bm[r] = (c16 < nT) ? n3h[c16 * NB + b3] : (S)0;
     }
+    // loop-invariant per-lane values of b3 are read from LDS in the loop (register budget)
+    const int tcol = (c16 < nT) ? c16 : 0;
     // W operand (A[m = (b2, b3)][k = ia]), prefetched one tile ahead by buffer loads
     const T *Wt = (const T *)W_ + (size_t)tl * 2 * NB * NB2;
     const __amdgpu_buffer_rsrc_t wr = rsrc(Wt, sizeof(T) * NB * NB2);
@@ -539,8 +541,60 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     for (int r = 0; r < 4; ++r) yM[r] = 0;
     const S sc = (S)(a.psi_scale * a.perm[t]);
     const S sfix2 = (S)(a.sigma_fixed * a.sigma_fixed);
-    load_w(0);
-    for (int b2 = 0; b2 < NB; ++b2) {
+    // Tiles (16 b1 x 16 b3 at one b2) whose every element lies beyond the Gaussians' 8 sigma
+    // cut in all four terms contribute exactly zero (lo_gauss returns 0 there): a wave skips
+    // them.  Interval bounds over the tile's w1, w3 and largest sigma, with a margin far above
+    // the rounding of the element-wise test, decide per b2; the decision is wave-uniform.
+    constexpr int NLW = (NB + 63) / 64;
+    unsigned long long live[NLW];
+    {
+        double w1lo = 1e300, w1hi = -1e300, s1m = 0, w3lo = 1e300, w3hi = -1e300, s3m = 0;
+        for (int m = 0; m < 16; ++m) {
+            w1lo = fmin(w1lo, w1s[m]);
+            w1hi = fmax(w1hi, w1s[m]);
+            s1m = fmax(s1m, a.sigsq[(size_t)q1 * NB + 16 * bb + m]);
+            w3lo = fmin(w3lo, om3[b3w + m]);
+            w3hi = fmax(w3hi, om3[b3w + m]);
+            s3m = fmax(s3m, a.sigsq[(size_t)i3 * NB + b3w + m]);
+        }
+#pragma unroll
+        for (int wd = 0; wd < NLW; ++wd) {
+            const int b2 = 64 * wd + lane;
+            bool lv = false;
+            if (b2 < NB) {
+                const double w2 = w2s[b2];
+                const double smax = (a.integrationtype == 2) ? sqrt(s1m + a.sigsq[(size_t)i2 * NB + b2] + s3m)
+                                                             : a.sigma_fixed;
+                const double L = 8.0 * smax * (1.0 + 1e-4) + 1e-6 * (fabs(w1hi) + fabs(w2) + fabs(w3hi));
+                auto hit = [&](double lo, double hi) { return lo <= L && hi >= -L; };
+                lv = hit(w1lo + w2 - w3hi, w1hi + w2 - w3lo) || hit(w1lo - w2 + w3lo, w1hi - w2 + w3hi) ||
+                     hit(w1lo - w2 - w3hi, w1hi - w2 - w3lo) || hit(w1lo + w2 + w3lo, w1hi + w2 + w3hi);
+            }
+            const unsigned long long bl = __ballot(lv);
+            live[wd] = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(bl >> 32)) << 32) |
+                       (unsigned)__builtin_amdgcn_readfirstlane((unsigned)bl);
+        }
+    }
+    auto next_live = [&](int from) -> int {
+#pragma unroll
+        for (int wd = 0; wd < NLW; ++wd) {
+            if (from >= 64 * (wd + 1)) continue;
+            const int sh = from > 64 * wd ? from - 64 * wd : 0;
+            const unsigned long long m = live[wd] >> sh;
+            if (m) return 64 * wd + sh + __builtin_ctzll(m);
+        }
+        return NB;
+    };
+    int nxt = next_live(0);
+    if (nxt < NB) load_w(nxt);
+    for (int cb = 0; cb < NB; cb += kChunk) {
+    // this wave's partial sums of the chunk start at zero; live b2 overwrite their slots
+    for (int x = lane; x < kChunk * 16 * NTY; x += 64) Xp[w * kChunk * 16 * NTY + x] = (S)0;
+    unsigned cm = (unsigned)((live[cb >> 6] >> (cb & 63)) & ((1ull << kChunk) - 1));
+    while (cm) {
+        const int cs = __builtin_ctz(cm);
+        cm &= cm - 1;
+        const int b2 = cb + cs;
         Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
         if constexpr (PREC == 2) {
             Split ar[KS], ai[KS];
@@ -563,8 +617,9 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                 mma3(ci, ar[ks], bi);
                 mma3(ci, ai[ks], br);
             }
-            // next tile's W is in flight during the epilogue
-            if (b2 + 1 < NB) load_w(b2 + 1);
+            // next live tile's W is in flight during the epilogue
+            nxt = next_live(b2 + 1);
+            if (nxt < NB) load_w(nxt);
         } else {
             T ar[KS], ai[KS];
 #pragma unroll
@@ -572,7 +627,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                 ar[ks] = rr[ks][0];
                 ai[ks] = ri[ks][0];
             }
-            if (b2 + 1 < NB) load_w(b2 + 1);
+            nxt = next_live(b2 + 1);
+            if (nxt < NB) load_w(nxt);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const int k = 4 * ks + g;
@@ -595,12 +651,13 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
             if ((okm >> (sh3 + out_row<PREC>(lane, r))) & 1u) {
                 const S re = (S)cr[r], im = (S)ci[r];
                 const S psisq = sc * (re * re + im * im);
-                const S sig2 = (a.integrationtype == 2) ? sg12 + sg3[r] : sfix2;
+                const int b3 = b3w + out_row<PREC>(lane, r);
+                const S sig2 = (a.integrationtype == 2) ? sg12 + sg3s[b3] : sfix2;
                 S d0, d1;
                 if constexpr (PREC == 0)
-                    delta_pair(w1, w2s[b2], om3[b3w + out_row<PREC>(lane, r)], sig2, d0, d1);
+                    delta_pair(w1, w2s[b2], om3[b3], sig2, d0, d1);
                 else
-                    delta_pair(x1, w2f[b2], x3[r], sig2, d0, d1);
+                    delta_pair(x1, w2f[b2], w3f[b3], sig2, d0, d1);
                 const S A = psisq * d0, B = psisq * d1;
                 P[r] = A + B;
                 M[r] = B - A;
@@ -611,10 +668,10 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         Acc xa = {0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            mma1(xa, M[r], bm[r]);
+            const S bm = (c16 < nT) ? n3h[tcol * NB + b3w + out_row<PREC>(lane, r)] : (S)0;
+            mma1(xa, M[r], bm);
             mma1(xa, P[r], bp);
         }
-        const int cs = b2 % kChunk;
         if (c16 < nT) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Xp[((w * kChunk + cs) * 16 + out_row<PREC>(lane, r)) * NTY + c16] = (S)xa[r];
@@ -630,12 +687,12 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) yM[r] += M[r];
-        if (cs == kChunk - 1) {
+    }
+        {
             // sums over b3 of the chunk's b2: linewidth partials and the q2' entries of the
             // scattering matrix (:126-133).  Every target (row, q', band) belongs to one
             // triplet and, inside it, to one equivalent pair, so values are stored.
             __syncthreads();
-            const int cb = b2 - cs;
             for (int x = threadIdx.x; x < 16 * nT; x += blockDim.x) {
                 const int m = x % 16, T_ = x / 16;
                 S v[kChunk];
@@ -772,7 +829,7 @@ size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * (kp + 8) : es * 2 * nb * nb;
     uw += 2 * es * 9 * (size_t)maxblk + 4 * (size_t)maxblk;
     uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
-    size_t ps = es * (2 * nty * nb + nb + nw * kChunk * 16 * nty);
+    size_t ps = es * (2 * nty * nb + 2 * nb + nw * kChunk * 16 * nty) + 4 * nb;
     ps = ((ps + 15) & ~(size_t)15);
     if (precision == TC3_PREC_BF16X3) ps += ks * kPlanes * 64 * 16;
     ps += 8 * (nb + 16 + 16 * nty) + 4 * nb + 4 * 16 + 4 * (nb + 16) * nmw;
