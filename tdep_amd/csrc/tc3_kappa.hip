@@ -214,10 +214,15 @@ struct BteState {
     double tol;
 };
 
+// All temperatures iterate together: blockIdx.z is the temperature, every per-temperature
+// array is strided by it, and a temperature whose state says done is skipped.
 __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full, const int *__restrict__ op_of_full,
-                          const double *__restrict__ opm, const double *__restrict__ Fn, double *__restrict__ Fbb,
-                          const BteState *__restrict__ st) {
-    if (st->done) return;
+                          const double *__restrict__ opm, const double *__restrict__ Fn_, double *__restrict__ Fbb_,
+                          const BteState *__restrict__ st_, long sF, long sFbb) {
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const double *Fn = Fn_ + T * sF;
+    double *Fbb = Fbb_ + T * sFbb;
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (j >= (long)nfull * nb) return;
     const int q = (int)(j / nb), b = (int)(j % nb);
@@ -232,12 +237,20 @@ __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full
 // Workgroup per (kRows rows, column slice): every Fbb element fetched serves kRows rows,
 // and the grid holds enough workgroups to keep HBM busy.  Partial sums per slice are
 // added in slice order by k_gemv_sum, so the result does not depend on scheduling.
-constexpr int kRows = 4, kSlices = 8;
+#ifndef TC3_BTE_ROWS
+#define TC3_BTE_ROWS 4
+#endif
+constexpr int kRows = TC3_BTE_ROWS, kSlices = 8;
 template <typename XT>
-__global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const XT *__restrict__ xi,
-                                                    const double *__restrict__ Fbb, double *__restrict__ part,
-                                                    const BteState *__restrict__ st) {
-    if (st->done) return;
+__global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const XT *__restrict__ xi_,
+                                                    const double *__restrict__ Fbb_, double *__restrict__ part_,
+                                                    const BteState *__restrict__ st_, size_t sXi, long sFbb,
+                                                    long sPart) {
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const XT *xi = xi_ + T * sXi;
+    const double *Fbb = Fbb_ + T * sFbb;
+    double *part = part_ + T * sPart;
     __shared__ double red[3 * kRows][kBlock / 64];
     const int r0 = blockIdx.x * kRows, sl = blockIdx.y;
     const int nr = min(kRows, nrows - r0);
@@ -275,9 +288,12 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const 
     }
 }
 
-__global__ void k_gemv_sum(int n3, const double *__restrict__ part, double *__restrict__ buf,
-                           const BteState *__restrict__ st) {
-    if (st->done) return;
+__global__ void k_gemv_sum(int n3, const double *__restrict__ part_, double *__restrict__ buf_,
+                           const BteState *__restrict__ st_, long sPart, long sBuf) {
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const double *part = part_ + T * sPart;
+    double *buf = buf_ + T * sBuf;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n3) return;
     double v = 0;
@@ -288,12 +304,16 @@ __global__ void k_gemv_sum(int n3, const double *__restrict__ part, double *__re
 // Fnb = -buf/qs; degeneracy average then +F0, band by band in order (kappa.f90:415-448);
 // convergence partial and mixing (kappa.f90:456-468).  Workgroup per irreducible point;
 // the ordered degeneracy pass runs on one thread over LDS copies.
-__global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf,
-                             const double *__restrict__ qs, const double *__restrict__ F0,
+__global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf_,
+                             const double *__restrict__ qs_, const double *__restrict__ F0_,
                              const int *__restrict__ dg_off, const int *__restrict__ dg,
-                             double *__restrict__ Fn, double *__restrict__ g0part, const BteState *__restrict__ st) {
-    if (st->done) return;
-    const double mix = st->mix;
+                             double *__restrict__ Fn_, double *__restrict__ g0part_, const BteState *__restrict__ st_,
+                             long sBuf, long sQs, long sF, long sG) {
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const double mix = st_[T].mix;
+    const double *buf = buf_ + T * sBuf, *qs = qs_ + T * sQs, *F0 = F0_ + T * sF;
+    double *Fn = Fn_ + T * sF, *g0part = g0part_ + T * sG;
     const int q = blockIdx.x;
     __shared__ double Fnb[128 * 3], F0s[128 * 3], g1s[128];
     __shared__ int dgo[129], dgl[128 * 8];
@@ -346,12 +366,14 @@ __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const doubl
 // chop): workgroup per slice of modes, each writing its 9 partial sums; the host adds
 // the slices in order.  cvw[q][b] = cv(omega) * weight_q / volume (0 below lo_freqtol).
 constexpr int kKapSlices = 32;
-__global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const double *__restrict__ cvw,
-                                                   const double *__restrict__ Fn, const double *__restrict__ vel,
-                                                   const double *__restrict__ opm, double *__restrict__ klog,
-                                                   const BteState *__restrict__ st) {
-    if (st->done) return;
-    double *out = klog + (size_t)9 * gridDim.x * st->it;
+__global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const double *__restrict__ cvw_,
+                                                   const double *__restrict__ Fn_, const double *__restrict__ vel,
+                                                   const double *__restrict__ opm, double *__restrict__ klog_,
+                                                   const BteState *__restrict__ st_, long sCvw, long sF, long sLog) {
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const double *cvw = cvw_ + T * sCvw, *Fn = Fn_ + T * sF;
+    double *out = klog_ + T * sLog + (size_t)9 * gridDim.x * st_[T].it;
     __shared__ double red[9][256];
     double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int m0 = (int)((long)nmode * blockIdx.x / gridDim.x), m1 = (int)((long)nmode * (blockIdx.x + 1) / gridDim.x);
@@ -386,9 +408,12 @@ __global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const do
 // scfcheck = mean over modes of |Fn - Fnb|^2 / |Fn|^2 (partials per point added in order),
 // stop when the last three add up to less than tol; after iteration 15 the mixing factor
 // shrinks by 2% per iteration while above 0.5.
-__global__ void k_bte_check(int nirr, int nb, const double *__restrict__ g0part, double *__restrict__ scf,
-                            BteState *__restrict__ st) {
+__global__ void k_bte_check(int nirr, int nb, const double *__restrict__ g0part_, double *__restrict__ scf_,
+                            BteState *__restrict__ st_, long sG, long sScf) {
+    BteState *st = st_ + blockIdx.z;
     if (threadIdx.x != 0 || st->done) return;
+    const double *g0part = g0part_ + blockIdx.z * sG;
+    double *scf = scf_ + blockIdx.z * sScf;
     double s = 0;
     for (int q = 0; q < nirr; ++q) s += g0part[q];
     const int it = st->it + 1;   // 1-based number of the iteration just done
@@ -488,25 +513,28 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
         if (dgoff[(size_t)q * nb + nb] - dgoff[(size_t)q * nb] > 1024)
             return c->fail(1, "degenerate sets of one q-point exceed the BTE kernel's capacity");
     }
-    TRY(dalloc((void **)&d_Fn, sizeof(double) * qb * 3));
-    TRY(dalloc((void **)&d_F0, sizeof(double) * qb * 3));
-    TRY(dalloc((void **)&d_qs, sizeof(double) * qb));
-    TRY(dalloc((void **)&d_Fbb, sizeof(double) * (size_t)nfull * nb * 3));
-    TRY(dalloc((void **)&d_buf, sizeof(double) * (size_t)nrows * 3));
+    const int mi = std::max(maxiter, 1);
+    const long sF = (long)qb * 3, sQ = (long)qb, sFbb = (long)nfull * nb * 3, sBuf = 3l * std::max(nrows, 1),
+               sPart = sBuf * kSlices, sG = nirr, sScf = mi, sLog = 9l * kKapSlices * mi;
+    TRY(dalloc((void **)&d_Fn, sizeof(double) * sF * nT));
+    TRY(dalloc((void **)&d_F0, sizeof(double) * sF * nT));
+    TRY(dalloc((void **)&d_qs, sizeof(double) * sQ * nT));
+    TRY(dalloc((void **)&d_Fbb, sizeof(double) * sFbb * nT));
+    TRY(dalloc((void **)&d_buf, sizeof(double) * sBuf * nT));
     TRY(dalloc((void **)&d_opm, sizeof(double) * 9 * nsym));
-    TRY(dalloc((void **)&d_g0, sizeof(double) * nirr));
+    TRY(dalloc((void **)&d_g0, sizeof(double) * sG * nT));
     TRY(dalloc((void **)&d_opf, sizeof(int) * nfull));
     TRY(dalloc((void **)&d_rowof, sizeof(int) * qb));
     TRY(dalloc((void **)&d_dgoff, sizeof(int) * (qb + 1)));
     TRY(dalloc((void **)&d_dg, sizeof(int) * dg.size()));
-    TRY(dalloc((void **)&d_cvw, sizeof(double) * qb));
+    TRY(dalloc((void **)&d_cvw, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
-    TRY(dalloc((void **)&d_klog, sizeof(double) * 9 * kKapSlices * (size_t)std::max(maxiter, 1)));
+    TRY(dalloc((void **)&d_klog, sizeof(double) * sLog * nT));
     double *d_part, *d_scf;
     BteState *d_st;
-    TRY(dalloc((void **)&d_part, sizeof(double) * 3 * (size_t)std::max(nrows, 1) * kSlices));
-    TRY(dalloc((void **)&d_scf, sizeof(double) * (size_t)std::max(maxiter, 1)));
-    TRY(dalloc((void **)&d_st, sizeof(BteState)));
+    TRY(dalloc((void **)&d_part, sizeof(double) * sPart * nT));
+    TRY(dalloc((void **)&d_scf, sizeof(double) * sScf * nT));
+    TRY(dalloc((void **)&d_st, sizeof(BteState) * nT));
     TRY(hipMemcpyAsync(d_vel, c->vel_irr.data(), sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
@@ -514,14 +542,18 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     TRY(hipMemcpyAsync(d_dgoff, dgoff.data(), sizeof(int) * (qb + 1), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_dg, dg.data(), sizeof(int) * dg.size(), hipMemcpyHostToDevice, st));
     double t_bte = 0, t_od = 0;
-    std::vector<double> g0(nirr);
+    // per temperature on the host: SMA kappa, off-diagonal term, the first log record, the
+    // weights of the kappa log and the initial iteration state
+    std::vector<BteState> hs(nT);
+    std::vector<double> cvw(qb * nT, 0.0);
     for (int iT = 0; iT < nT; ++iT) {
         const double T = c->temps[iT];
         const double *F0 = &c->F0[(size_t)iT * qb * 3];
         double *Fn = &c->Fn[(size_t)iT * qb * 3];
-        double *ks = &c->kappa_sma[9 * iT], *ki = &c->kappa_iter[9 * iT], *ko = &c->kappa_od[9 * iT];
+        double *ks = &c->kappa_sma[9 * iT], *ko = &c->kappa_od[9 * iT];
         std::memcpy(Fn, F0, sizeof(double) * qb * 3);
         get_kappa(c, F0, T, c->classical, ks, nullptr);
+        hs[iT] = {0, 1, maxiter, 0, 0.95, tol};
         if (T < kTemperatureTol) {
             std::fill(ks, ks + 9, 0.0);
             continue;
@@ -531,96 +563,106 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             kappa_offdiag(c, iT, T, c->classical, ko);
             t_od += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
         }
-        const auto a = std::chrono::steady_clock::now();
-        int it_done = 0;
-        if (maxiter > 0) {
-            std::vector<double> cvw(qb, 0.0);
-            for (int q = 0; q < nirr; ++q)
-                for (int b = 0; b < nb; ++b) {
-                    const double om = c->om_irr[(size_t)q * nb + b];
-                    if (om < kFreqTol) continue;
-                    const double cv = c->classical ? kKbHartree : cv_host(T, om);
-                    cvw[(size_t)q * nb + b] = cv * c->mesh.weight_irr[q] / c->cr.volume;
-                }
-            TRY(hipMemcpyAsync(d_cvw, cvw.data(), sizeof(double) * qb, hipMemcpyHostToDevice, st));
-            TRY(hipMemcpyAsync(d_Fn, Fn, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
-            TRY(hipMemcpyAsync(d_F0, F0, sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
-            TRY(hipMemcpyAsync(d_qs, &c->qs[(size_t)iT * qb], sizeof(double) * qb, hipMemcpyHostToDevice, st));
-            double k0[9];
-            get_kappa(c, Fn, T, c->classical, k0, nullptr);
-            auto &log = c->kappa_log[iT];
-            log.insert(log.end(), {k0[0], k0[4], k0[8], k0[3], k0[6], k0[7], 0.0});
-            const char *xi = (const char *)c->d_xi + (size_t)c->xi_es * iT * c->xi_row * nrows;
-            BteState hs = {0, 0, maxiter, 0, 0.95, tol};
-            TRY(hipMemcpyAsync(d_st, &hs, sizeof(BteState), hipMemcpyHostToDevice, st));
-            // one iteration: fold out, Xi.F, update, kappa log, convergence test
-            auto iteration = [&]() {
-                const long nj = (long)nfull * nb;
-                hipLaunchKernelGGL(k_foldout, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, st, nfull, nb,
-                                   c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb, d_st);
-                if (nrows > 0) {
-                    if (c->xi_es == 8)
-                        hipLaunchKernelGGL(k_xi_gemv<double>, dim3((nrows + kRows - 1) / kRows, kSlices), dim3(kBlock),
-                                           0, st, nrows, (long)c->xi_row, (const double *)xi, d_Fbb, d_part, d_st);
-                    else
-                        hipLaunchKernelGGL(k_xi_gemv<float>, dim3((nrows + kRows - 1) / kRows, kSlices), dim3(kBlock),
-                                           0, st, nrows, (long)c->xi_row, (const float *)xi, d_Fbb, d_part, d_st);
-                    hipLaunchKernelGGL(k_gemv_sum, dim3((3 * nrows + 255) / 256), dim3(256), 0, st, 3 * nrows, d_part,
-                                       d_buf, d_st);
-                }
-                hipLaunchKernelGGL(k_bte_update, dim3(nirr), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0, d_dgoff,
-                                   d_dg, d_Fn, d_g0, d_st);
-                hipLaunchKernelGGL(k_kappa_sum, dim3(kKapSlices), dim3(256), 0, st, (int)qb, nsym, d_cvw, d_Fn, d_vel,
-                                   d_opm, d_klog, d_st);
-                hipLaunchKernelGGL(k_bte_check, dim3(1), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st);
-            };
-            // chunks of kChunkIt iterations captured once as a graph and replayed until the
-            // device reports convergence; direct launches if capture is not available
-            constexpr int kChunkIt = 8;
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t gexec = nullptr;
-            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                for (int k = 0; k < kChunkIt; ++k) iteration();
-                if (hipStreamEndCapture(st, &graph) != hipSuccess ||
-                    hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
-                    if (graph) (void)hipGraphDestroy(graph);
-                    graph = nullptr;
-                    gexec = nullptr;
-                }
+        if (maxiter <= 0) continue;
+        hs[iT].done = 0;
+        for (int q = 0; q < nirr; ++q)
+            for (int b = 0; b < nb; ++b) {
+                const double om = c->om_irr[(size_t)q * nb + b];
+                if (om < kFreqTol) continue;
+                const double cv = c->classical ? kKbHartree : cv_host(T, om);
+                cvw[(size_t)iT * qb + (size_t)q * nb + b] = cv * c->mesh.weight_irr[q] / c->cr.volume;
             }
-            (void)hipGetLastError();
-            for (int done = 0; !done;) {
-                if (gexec) TRY(hipGraphLaunch(gexec, st));
+        double k0[9];
+        get_kappa(c, Fn, T, c->classical, k0, nullptr);
+        c->kappa_log[iT].insert(c->kappa_log[iT].end(), {k0[0], k0[4], k0[8], k0[3], k0[6], k0[7], 0.0});
+    }
+    const auto a = std::chrono::steady_clock::now();
+    bool any = false;
+    for (int iT = 0; iT < nT; ++iT) any = any || !hs[iT].done;
+    if (any) {
+        TRY(hipMemcpyAsync(d_cvw, cvw.data(), sizeof(double) * qb * nT, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_Fn, c->Fn.data(), sizeof(double) * qb * 3 * nT, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_F0, c->F0.data(), sizeof(double) * qb * 3 * nT, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_qs, c->qs.data(), sizeof(double) * qb * nT, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_st, hs.data(), sizeof(BteState) * nT, hipMemcpyHostToDevice, st));
+        const size_t sXi = c->xi_row * (size_t)nrows;
+        // one iteration for every temperature still running: fold out, Xi.F, update, kappa
+        // log, convergence test (kappa.f90:395-489)
+        auto iteration = [&]() {
+            const long nj = (long)nfull * nb;
+            hipLaunchKernelGGL(k_foldout, dim3((unsigned)((nj + 255) / 256), 1, nT), dim3(256), 0, st, nfull, nb,
+                               c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb, d_st, sF, sFbb);
+            if (nrows > 0) {
+                const dim3 gg((nrows + kRows - 1) / kRows, kSlices, nT);
+                if (c->xi_es == 8)
+                    hipLaunchKernelGGL(k_xi_gemv<double>, gg, dim3(kBlock), 0, st, nrows, (long)c->xi_row,
+                                       (const double *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
                 else
-                    for (int k = 0; k < kChunkIt; ++k) iteration();
-                TRY(hipGetLastError());
-                TRY(hipMemcpyAsync(&hs, d_st, sizeof(BteState), hipMemcpyDeviceToHost, st));
-                TRY(hipStreamSynchronize(st));
-                done = hs.done;
+                    hipLaunchKernelGGL(k_xi_gemv<float>, gg, dim3(kBlock), 0, st, nrows, (long)c->xi_row,
+                                       (const float *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
+                hipLaunchKernelGGL(k_gemv_sum, dim3((3 * nrows + 255) / 256, 1, nT), dim3(256), 0, st, 3 * nrows,
+                                   d_part, d_buf, d_st, sPart, sBuf);
             }
-            if (gexec) (void)hipGraphExecDestroy(gexec);
-            if (graph) (void)hipGraphDestroy(graph);
-            it_done = hs.it;
-            std::vector<double> scf((size_t)std::max(it_done, 1));
-            if (it_done > 0)
-                TRY(hipMemcpyAsync(scf.data(), d_scf, sizeof(double) * it_done, hipMemcpyDeviceToHost, st));
-            // final Fn and the per-iteration kappa log (kappa.f90:473-474)
-            std::vector<double> klog(9 * kKapSlices * (size_t)it_done);
-            TRY(hipMemcpyAsync(Fn, d_Fn, sizeof(double) * qb * 3, hipMemcpyDeviceToHost, st));
-            if (it_done > 0)
-                TRY(hipMemcpyAsync(klog.data(), d_klog, sizeof(double) * klog.size(), hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL(k_bte_update, dim3(nirr, 1, nT), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0,
+                               d_dgoff, d_dg, d_Fn, d_g0, d_st, sBuf, sQ, sF, sG);
+            hipLaunchKernelGGL(k_kappa_sum, dim3(kKapSlices, 1, nT), dim3(256), 0, st, (int)qb, nsym, d_cvw, d_Fn,
+                               d_vel, d_opm, d_klog, d_st, sQ, sF, sLog);
+            hipLaunchKernelGGL(k_bte_check, dim3(1, 1, nT), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st, sG, sScf);
+        };
+        // chunks of kChunkIt iterations captured once as a graph and replayed until the
+        // device reports every temperature done; direct launches if capture is not available
+        constexpr int kChunkIt = 8;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            for (int k = 0; k < kChunkIt; ++k) iteration();
+            if (hipStreamEndCapture(st, &graph) != hipSuccess ||
+                hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                graph = nullptr;
+                gexec = nullptr;
+            }
+        }
+        (void)hipGetLastError();
+        for (bool done = false; !done;) {
+            if (gexec) TRY(hipGraphLaunch(gexec, st));
+            else
+                for (int k = 0; k < kChunkIt; ++k) iteration();
+            TRY(hipGetLastError());
+            TRY(hipMemcpyAsync(hs.data(), d_st, sizeof(BteState) * nT, hipMemcpyDeviceToHost, st));
             TRY(hipStreamSynchronize(st));
+            done = true;
+            for (int iT = 0; iT < nT; ++iT) done = done && hs[iT].done;
+        }
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        // final Fn and the per-iteration kappa log of every temperature (kappa.f90:473-474)
+        std::vector<double> scf(sScf * nT), klog(sLog * nT), Fall(qb * 3 * nT);
+        TRY(hipMemcpyAsync(scf.data(), d_scf, sizeof(double) * scf.size(), hipMemcpyDeviceToHost, st));
+        TRY(hipMemcpyAsync(klog.data(), d_klog, sizeof(double) * klog.size(), hipMemcpyDeviceToHost, st));
+        TRY(hipMemcpyAsync(Fall.data(), d_Fn, sizeof(double) * Fall.size(), hipMemcpyDeviceToHost, st));
+        TRY(hipStreamSynchronize(st));
+        for (int iT = 0; iT < nT; ++iT) {
+            if (c->temps[iT] < kTemperatureTol || maxiter <= 0) continue;
+            std::memcpy(&c->Fn[(size_t)iT * qb * 3], &Fall[(size_t)iT * qb * 3], sizeof(double) * qb * 3);
+            const int it_done = hs[iT].it;
+            c->niter[iT] = it_done;
+            auto &log = c->kappa_log[iT];
             for (int it = 0; it < it_done; ++it) {
                 double kk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 for (int sl = 0; sl < kKapSlices; ++sl)
-                    for (int i = 0; i < 9; ++i) kk[i] += klog[(9 * kKapSlices) * (size_t)it + 9 * sl + i];
+                    for (int i = 0; i < 9; ++i) kk[i] += klog[iT * sLog + (9 * kKapSlices) * (size_t)it + 9 * sl + i];
                 chop_kappa(kk);
-                log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf[it]});
+                log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf[iT * sScf + it]});
             }
         }
-        c->niter[iT] = it_done;
-        t_bte += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-        get_kappa(c, Fn, T, c->classical, ki, &c->kappa_mode[(size_t)iT * qb * 9]);
+    }
+    t_bte += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    for (int iT = 0; iT < nT; ++iT) {
+        const double T = c->temps[iT];
+        if (T < kTemperatureTol) continue;
+        double *ks = &c->kappa_sma[9 * iT], *ki = &c->kappa_iter[9 * iT], *ko = &c->kappa_od[9 * iT];
+        get_kappa(c, &c->Fn[(size_t)iT * qb * 3], T, c->classical, ki, &c->kappa_mode[(size_t)iT * qb * 9]);
         symmetrize_kappa(c, ki);
         symmetrize_kappa(c, ko);
         symmetrize_kappa(c, ks);
