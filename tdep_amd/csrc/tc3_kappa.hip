@@ -230,10 +230,12 @@ __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full
     const double *m = opm + 9 * ((iop > 0 ? iop : -iop) - 1);
     const double *F = Fn + ((size_t)k * nb + b) * 3;
     const double s = iop > 0 ? 1.0 : -1.0;
-    for (int i = 0; i < 3; ++i) Fbb[j * 3 + i] = s * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]);
+    const long ncol = (long)nfull * nb;
+    for (int i = 0; i < 3; ++i) Fbb[i * ncol + j] = s * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]);
 }
 
-// buf[row][0..2] = sum_j Xi[row][j] Fbb[j][0..2]  (lo_gemv x3, kappa.f90:411-413).
+// buf[row][0..2] = sum_j Xi[row][j] Fbb[0..2][j]  (lo_gemv x3, kappa.f90:411-413); the
+// folded-out F is planar, [component][column].
 // Workgroup per (kRows rows, column slice): every Fbb element fetched serves kRows rows,
 // and the grid holds enough workgroups to keep HBM busy.  Partial sums per slice are
 // added in slice order by k_gemv_sum, so the result does not depend on scheduling.
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const 
 #pragma unroll
     for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
     for (long j = c0 + threadIdx.x; j < c1; j += blockDim.x) {
-        const double f0 = Fbb[j * 3], f1 = Fbb[j * 3 + 1], f2 = Fbb[j * 3 + 2];
+        const double f0 = Fbb[j], f1 = Fbb[ncol + j], f2 = Fbb[2 * ncol + j];
 #pragma unroll
         for (int r = 0; r < kRows; ++r) {
             if (r < nr) {
@@ -268,6 +270,63 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const 
                 s[r][0] += v * f0;
                 s[r][1] += v * f1;
                 s[r][2] += v * f2;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double v = s[r][i];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[3 * r + i][wave] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < 3 * nr) {
+        double v = 0;
+        for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
+        part[((size_t)sl * nrows + r0) * 3 + threadIdx.x] = v;
+    }
+}
+
+// The same product with 16 bytes of Xi per row and thread (VW = 4 FP32 or 2 FP64 columns)
+// and the matching VW values of each F component; needs ncol % VW == 0 (rows stay aligned).
+template <typename XT, int VW>
+__global__ void __launch_bounds__(kBlock) k_xi_gemv_v(int nrows, long ncol, const XT *__restrict__ xi_,
+                                                      const double *__restrict__ Fbb_, double *__restrict__ part_,
+                                                      const BteState *__restrict__ st_, size_t sXi, long sFbb,
+                                                      long sPart) {
+    typedef XT XV __attribute__((ext_vector_type(VW)));
+    typedef double DV __attribute__((ext_vector_type(VW)));
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const XT *xi = xi_ + T * sXi;
+    const double *Fbb = Fbb_ + T * sFbb;
+    double *part = part_ + T * sPart;
+    __shared__ double red[3 * kRows][kBlock / 64];
+    const int r0 = blockIdx.x * kRows, sl = blockIdx.y;
+    const int nr = min(kRows, nrows - r0);
+    const long nv = ncol / VW;
+    const long v0 = nv * sl / kSlices, v1 = nv * (sl + 1) / kSlices;
+    const XT *x = xi + (size_t)r0 * ncol;
+    double s[kRows][3];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
+    for (long v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+        const long j = v * VW;
+        const DV f0 = *(const DV *)(Fbb + j), f1 = *(const DV *)(Fbb + ncol + j), f2 = *(const DV *)(Fbb + 2 * ncol + j);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (r < nr) {
+                const XV xv = *(const XV *)(x + (size_t)r * ncol + j);
+#pragma unroll
+                for (int k = 0; k < VW; ++k) {
+                    const double e = (double)xv[k];
+                    s[r][0] += e * f0[k];
+                    s[r][1] += e * f1[k];
+                    s[r][2] += e * f2[k];
+                }
             }
         }
     }
@@ -594,11 +653,18 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
                                c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb, d_st, sF, sFbb);
             if (nrows > 0) {
                 const dim3 gg((nrows + kRows - 1) / kRows, kSlices, nT);
-                if (c->xi_es == 8)
-                    hipLaunchKernelGGL(k_xi_gemv<double>, gg, dim3(kBlock), 0, st, nrows, (long)c->xi_row,
+                const long ncol = (long)c->xi_row;
+                if (c->xi_es == 8 && ncol % 2 == 0)
+                    hipLaunchKernelGGL((k_xi_gemv_v<double, 2>), gg, dim3(kBlock), 0, st, nrows, ncol,
                                        (const double *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
+                else if (c->xi_es == 8)
+                    hipLaunchKernelGGL(k_xi_gemv<double>, gg, dim3(kBlock), 0, st, nrows, ncol,
+                                       (const double *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
+                else if (ncol % 4 == 0)
+                    hipLaunchKernelGGL((k_xi_gemv_v<float, 4>), gg, dim3(kBlock), 0, st, nrows, ncol,
+                                       (const float *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
                 else
-                    hipLaunchKernelGGL(k_xi_gemv<float>, gg, dim3(kBlock), 0, st, nrows, (long)c->xi_row,
+                    hipLaunchKernelGGL(k_xi_gemv<float>, gg, dim3(kBlock), 0, st, nrows, ncol,
                                        (const float *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
                 hipLaunchKernelGGL(k_gemv_sum, dim3((3 * nrows + 255) / 256, 1, nT), dim3(256), 0, st, 3 * nrows,
                                    d_part, d_buf, d_st, sPart, sBuf);
