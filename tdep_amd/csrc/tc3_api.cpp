@@ -27,6 +27,7 @@ int device_error(hipError_t e, const char *what, const char *file, int line) {
 }  // namespace tc3
 
 tc3_ctx::~tc3_ctx() {
+    if (bte_exec) hipGraphExecDestroy(bte_exec);
     hipFree(d_om_irr);
     hipFree(d_om_full);
     hipFree(d_vel_irr);

@@ -91,6 +91,9 @@ struct tc3_ctx {
     long gen_model = 0;
     long scatter_tables_gen = -1;
     long adjoint_tables_gen = -1;
+    // instantiated BTE iteration graph and the arguments it was captured with
+    hipGraphExec_t bte_exec = nullptr;
+    std::vector<long> bte_key;
     hipError_t pool_get(const std::string &name, size_t bytes, void **p, bool *fresh = nullptr) {
         DevBuf &b = pool[name];
         if (fresh) *fresh = false;

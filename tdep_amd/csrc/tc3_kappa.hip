@@ -553,10 +553,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     };
     hipError_t e = hipSuccess;
 #define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
+    // device buffers of the solve live in the context's pool, so that repeated calls reuse
+    // them (and the captured iteration graph below)
+    int nbuf = 0;
     auto dalloc = [&](void **p, size_t n) -> hipError_t {
-        hipError_t r = hipMalloc(p, std::max<size_t>(n, 16));
-        if (r == hipSuccess) tmp.push_back(*p);
-        return r;
+        return c->pool_get("bte" + std::to_string(nbuf++), std::max<size_t>(n, 16), p);
     };
     double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_buf, *d_opm, *d_g0, *d_cvw, *d_vel, *d_klog;
     int *d_opf, *d_rowof, *d_dgoff, *d_dg;
@@ -676,20 +677,34 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             hipLaunchKernelGGL(k_bte_check, dim3(1, 1, nT), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st, sG, sScf);
         };
         // chunks of kChunkIt iterations captured once as a graph and replayed until the
-        // device reports every temperature done; direct launches if capture is not available
+        // device reports every temperature done; direct launches if capture is not available.
+        // The instantiated graph is kept in the context while every argument it captured
+        // (buffers, sizes, strides) stays the same.
         constexpr int kChunkIt = 8;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t gexec = nullptr;
-        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const std::vector<long> key = {(long)(uintptr_t)c->d_xi, (long)c->xi_es, (long)c->xi_row, nrows, nfull, nb,
+                                       nirr, nT, nsym, (long)(uintptr_t)c->d_irr_of_full, (long)(uintptr_t)d_Fn,
+                                       (long)(uintptr_t)d_F0, (long)(uintptr_t)d_qs, (long)(uintptr_t)d_Fbb,
+                                       (long)(uintptr_t)d_buf, (long)(uintptr_t)d_opm, (long)(uintptr_t)d_g0,
+                                       (long)(uintptr_t)d_opf, (long)(uintptr_t)d_rowof, (long)(uintptr_t)d_dgoff,
+                                       (long)(uintptr_t)d_dg, (long)(uintptr_t)d_cvw, (long)(uintptr_t)d_vel,
+                                       (long)(uintptr_t)d_klog, (long)(uintptr_t)d_part, (long)(uintptr_t)d_scf,
+                                       (long)(uintptr_t)d_st, sF, sFbb, sBuf, sPart, sG, sScf, sLog};
+        if (c->bte_exec && c->bte_key != key) {
+            (void)hipGraphExecDestroy(c->bte_exec);
+            c->bte_exec = nullptr;
+        }
+        if (!c->bte_exec && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            hipGraph_t graph = nullptr;
             for (int k = 0; k < kChunkIt; ++k) iteration();
-            if (hipStreamEndCapture(st, &graph) != hipSuccess ||
-                hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
-                if (graph) (void)hipGraphDestroy(graph);
-                graph = nullptr;
-                gexec = nullptr;
-            }
+            if (hipStreamEndCapture(st, &graph) == hipSuccess &&
+                hipGraphInstantiate(&c->bte_exec, graph, nullptr, nullptr, 0) == hipSuccess)
+                c->bte_key = key;
+            else
+                c->bte_exec = nullptr;
+            if (graph) (void)hipGraphDestroy(graph);
         }
         (void)hipGetLastError();
+        hipGraphExec_t gexec = c->bte_exec;
         for (bool done = false; !done;) {
             if (gexec) TRY(hipGraphLaunch(gexec, st));
             else
@@ -700,8 +715,6 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             done = true;
             for (int iT = 0; iT < nT; ++iT) done = done && hs[iT].done;
         }
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
         // final Fn and the per-iteration kappa log of every temperature (kappa.f90:473-474)
         std::vector<double> scf(sScf * nT), klog(sLog * nT), Fall(qb * 3 * nT);
         TRY(hipMemcpyAsync(scf.data(), d_scf, sizeof(double) * scf.size(), hipMemcpyDeviceToHost, st));
