@@ -290,6 +290,12 @@ int tc3_harmonic(tc3_ctx *ctx, double *omega_min) {
 
 int tc3_get_harmonic(const tc3_ctx *ctx, int full, double *omega, double *vel, double *egv) {
     if (!ctx || !ctx->have_harmonic) return 5;
+    if (full) {
+        // full-grid results are copied from the device on first request
+        tc3_ctx *mc = const_cast<tc3_ctx *>(ctx);
+        const int r = guarded(mc, [&]() -> int { return ensure_full_host(mc); });
+        if (r) return r;
+    }
     const int nb = ctx->nb;
     const int nq = full ? ctx->mesh.nfull : ctx->mesh.nirr;
     const auto &om = full ? ctx->om_full : ctx->om_irr;
@@ -500,6 +506,9 @@ int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *l
 
 int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path) {
     if (!ctx || !ctx->have_kappa || iT < 0 || iT >= ctx->nT || !path) return 5;
+    tc3_ctx *mc = const_cast<tc3_ctx *>(ctx);
+    const int rf = guarded(mc, [&]() -> int { return ensure_full_host(mc); });
+    if (rf) return rf;
     std::string err;
     int r = write_grid_hdf5(ctx, iT, path, err);
     if (r) const_cast<tc3_ctx *>(ctx)->err = err;

@@ -40,6 +40,8 @@ struct tc3_ctx {
     // harmonic results (host copies); egv stored [q][component][mode]
     std::vector<double> om_irr, vel_irr, om_full, vel_full;
     std::vector<tc3::cd> egv_irr, egv_full;
+    bool full_host_valid = false;             // om/vel/egv_full hold the device results
+    int harm_dims[3] = {-1, -1, -1};          // nb, nirr, nfull of the device harmonic arrays
     std::vector<std::vector<std::vector<int>>> degen;  // [q_irr][band] -> degenerate bands
     double omega_min = 0;
 
@@ -124,6 +126,7 @@ extern thread_local std::string g_device_error;
 
 // tc3_harmonic.hip
 int run_harmonic(tc3_ctx *c);
+int ensure_full_host(tc3_ctx *c);
 // tc3_scatter.hip
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma);
 // tc3_kappa.hip

@@ -92,8 +92,10 @@ __global__ void k_dynmat(int na, const double *__restrict__ qv, const int *__res
 // Parallel cyclic Jacobi for one Hermitian matrix per workgroup.  Round-robin pairing
 // (circle method) gives nb/2 disjoint (p,q) rotations per round; all rotations of a
 // round are applied to the columns, then to the rows, then to V.
+template <bool USE_LDS>
 __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, double *__restrict__ lam,
-                         int maxsweep, int *__restrict__ status, int use_lds) {
+                         int maxsweep, int *__restrict__ status) {
+    constexpr bool use_lds = USE_LDS;
     const int q = blockIdx.x;
     // A and V live in LDS when they fit (use_lds), else in global memory
     extern __shared__ cd jsm[];
@@ -110,30 +112,32 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
     for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) V[i] = {(i / nb) == (i % nb) ? 1.0 : 0.0, 0.0};
     __syncthreads();
     int sweep = 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     for (; sweep < maxsweep; ++sweep) {
-        // convergence test: off-diagonal vs diagonal Frobenius norm
+        // convergence test: off-diagonal vs diagonal Frobenius norm (wave sums, then LDS)
         double off = 0, dia = 0;
         for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) {
             const int r = i / nb, c = i % nb;
             if (r == c) dia += cabs2(A[i]);
             else off += cabs2(A[i]);
         }
-        red[threadIdx.x] = off;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o, 64);
+            dia += __shfl_xor(dia, o, 64);
         }
-        const double offt = red[0];
-        __syncthreads();
-        red[threadIdx.x] = dia;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
+        if (lane == 0) {
+            red[wv] = off;
+            red[nwv + wv] = dia;
         }
-        const double diat = red[0];
-        if (threadIdx.x == 0) done = (offt <= 1e-28 * diat) || offt == 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double offt = 0, diat = 0;
+            for (int w = 0; w < nwv; ++w) {
+                offt += red[w];
+                diat += red[nwv + w];
+            }
+            done = (offt <= 1e-28 * diat) || offt == 0.0;
+        }
         __syncthreads();
         if (done) break;
         for (int r = 0; r < mpad - 1; ++r) {
@@ -162,36 +166,57 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
                 sp[k] = p; sq[k] = qq; sc[k] = c; ss[k] = s; sphx[k] = ex; sphy[k] = ey;
             }
             __syncthreads();
-            // columns: A <- A U, V <- V U with U = diag(1, e^{-i phi}) [[c, s], [-s, c]];
-            // wave w takes the pairs k = w, w + 4, ..., its lanes the rows
-            const int lane = threadIdx.x & 63, nwv = blockDim.x >> 6;
-            for (int k = threadIdx.x >> 6; k < half; k += nwv)
-            for (int row = lane; row < nb; row += 64) {
+            // A <- U^H A U with U = diag(1, e^{-i phi}) [[c, s], [-s, c]] per pair: the rows
+            // of pair P and the columns of pair Q form a 2x2 block that only the rotations of
+            // P and Q touch; each block gets its column rotation, then its row rotation.
+            for (int it = threadIdx.x; it < half * half; it += blockDim.x) {
+                const int kP = it / half, kQ = it % half;
+                const int p = sp[kP], qq = sq[kP], p2 = sp[kQ], q2 = sq[kQ];
+                const bool rowP = qq >= 0 && ss[kP] != 0.0, colQ = q2 >= 0 && ss[kQ] != 0.0;
+                if (!rowP && !colQ) continue;
+                const bool hq = qq >= 0, hq2 = q2 >= 0;
+                const cd zero = {0.0, 0.0};
+                cd x00 = A[(size_t)p * nb + p2], x01 = zero, x10 = zero, x11 = zero;
+                if (hq2) x01 = A[(size_t)p * nb + q2];
+                if (hq) x10 = A[(size_t)qq * nb + p2];
+                if (hq && hq2) x11 = A[(size_t)qq * nb + q2];
+                if (colQ) {
+                    const double c = sc[kQ], s = ss[kQ];
+                    const cd em = {sphx[kQ], -sphy[kQ]};  // e^{-i phi}
+                    cd ap = x00, emaq = cmul(em, x01);
+                    x00 = csub(cscale(ap, c), cscale(emaq, s));
+                    x01 = cadd(cscale(ap, s), cscale(emaq, c));
+                    ap = x10;
+                    emaq = cmul(em, x11);
+                    x10 = csub(cscale(ap, c), cscale(emaq, s));
+                    x11 = cadd(cscale(ap, s), cscale(emaq, c));
+                }
+                if (rowP) {
+                    const double c = sc[kP], s = ss[kP];
+                    const cd ep = {sphx[kP], sphy[kP]};  // e^{i phi}
+                    cd ap = x00, epaq = cmul(ep, x10);
+                    x00 = csub(cscale(ap, c), cscale(epaq, s));
+                    x10 = cadd(cscale(ap, s), cscale(epaq, c));
+                    ap = x01;
+                    epaq = cmul(ep, x11);
+                    x01 = csub(cscale(ap, c), cscale(epaq, s));
+                    x11 = cadd(cscale(ap, s), cscale(epaq, c));
+                }
+                A[(size_t)p * nb + p2] = x00;
+                if (hq2) A[(size_t)p * nb + q2] = x01;
+                if (hq) A[(size_t)qq * nb + p2] = x10;
+                if (hq && hq2) A[(size_t)qq * nb + q2] = x11;
+            }
+            // V <- V U
+            for (int it = threadIdx.x; it < nb * half; it += blockDim.x) {
+                const int row = it / half, k = it % half;
                 const int p = sp[k], qq = sq[k];
                 if (qq < 0 || ss[k] == 0.0) continue;
                 const double c = sc[k], s = ss[k];
-                const cd em = {sphx[k], -sphy[k]};  // e^{-i phi}
-                cd ap = A[(size_t)row * nb + p], aq = A[(size_t)row * nb + qq];
-                cd emaq = cmul(em, aq);
-                A[(size_t)row * nb + p] = csub(cscale(ap, c), cscale(emaq, s));
-                A[(size_t)row * nb + qq] = cadd(cscale(ap, s), cscale(emaq, c));
-                cd vp = V[(size_t)row * nb + p], vq = V[(size_t)row * nb + qq];
-                cd emvq = cmul(em, vq);
+                const cd em = {sphx[k], -sphy[k]};
+                const cd vp = V[(size_t)row * nb + p], emvq = cmul(em, V[(size_t)row * nb + qq]);
                 V[(size_t)row * nb + p] = csub(cscale(vp, c), cscale(emvq, s));
                 V[(size_t)row * nb + qq] = cadd(cscale(vp, s), cscale(emvq, c));
-            }
-            __syncthreads();
-            // rows: A <- U^H A
-            for (int k = threadIdx.x >> 6; k < half; k += nwv)
-            for (int col = lane; col < nb; col += 64) {
-                const int p = sp[k], qq = sq[k];
-                if (qq < 0 || ss[k] == 0.0) continue;
-                const double c = sc[k], s = ss[k];
-                const cd ep = {sphx[k], sphy[k]};  // e^{i phi}
-                cd ap = A[(size_t)p * nb + col], aq = A[(size_t)qq * nb + col];
-                cd epaq = cmul(ep, aq);
-                A[(size_t)p * nb + col] = csub(cscale(ap, c), cscale(epaq, s));
-                A[(size_t)qq * nb + col] = cadd(cscale(ap, s), cscale(epaq, c));
             }
             __syncthreads();
         }
@@ -245,14 +270,19 @@ __device__ void small_eigh(int k, cd *M, cd *W, double *ev, int *ord) {
 
 // Per irreducible point: Gamma zeroing, sort by signed frequency, canonical gauge of the
 // degenerate subspaces, phase fix, chop, group velocities.
+template <bool USE_LDS>
 __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__restrict__ Vin,
                        const cd *__restrict__ gD, const int *__restrict__ at_gamma,
                        double *__restrict__ omega, cd *__restrict__ egv, double *__restrict__ vel,
-                       cd *__restrict__ scratch, double veltol, int use_lds) {
+                       cd *__restrict__ scratch, double veltol) {
+    constexpr bool use_lds = USE_LDS;
     const int q = blockIdx.x;
     __shared__ double lam[128], om[128], hf[3][128];
     __shared__ int ord[128], grp[129];
     __shared__ int ngrp;
+    // work arrays of the serial (thread 0) sections
+    __shared__ int s_idx[128], s_so[128], s_sub[128], s_first[128];
+    __shared__ double s_tmp[128], s_ev[128];
     // eigenvectors in LDS when they fit (use_lds), else in the output array
     extern __shared__ cd psm[];
     const cd *V = Vin + (size_t)q * nb * nb;
@@ -263,7 +293,7 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
         for (int i = 0; i < nb; ++i) lam[i] = lam_in[(size_t)q * nb + i];
         if (at_gamma[q]) {
             // three smallest |lambda| (stable) set to zero
-            int idx[128];
+            int *idx = s_idx;
             for (int i = 0; i < nb; ++i) idx[i] = i;
             for (int i = 1; i < nb; ++i) {
                 int o = idx[i], j = i - 1;
@@ -289,7 +319,7 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
         E[i] = V[(size_t)r * nb + ord[c]];
     }
     if (threadIdx.x == 0) {
-        double tmp[128];
+        double *tmp = s_tmp;
         for (int i = 0; i < nb; ++i) tmp[i] = lam[ord[i]];
         for (int i = 0; i < nb; ++i) {
             lam[i] = tmp[i];
@@ -303,6 +333,17 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
         grp[++n] = nb;
         ngrp = n;
     }
+    // fixed weights of the gauge probe and of the phase fix, one per component row
+    __shared__ double prw[128];
+    __shared__ cd phw[128];
+    for (int r = threadIdx.x; r < nb; r += blockDim.x) {
+        const double kk = (double)(r + 1);
+        prw[r] = fmod(kk * kGold, 1.0);
+        const double amp = 1.0 + fmod(kk * kPlastic, 1.0);
+        double sn, cs;
+        sincos(0.9 * kk, &sn, &cs);
+        phw[r] = {amp * cs, amp * sn};
+    }
     __syncthreads();
     // canonical gauge inside degenerate groups (serial; groups are small)
     if (threadIdx.x == 0) {
@@ -311,15 +352,13 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
             if (k < 2) continue;
             cd *M = scr;
             cd *W = scr + nb * nb;
-            double ev[128];
-            int so[128];
+            double *ev = s_ev;
+            int *so = s_so;
             for (int a = 0; a < k; ++a)
                 for (int b = 0; b < k; ++b) {
                     cd s = {0, 0};
-                    for (int r = 0; r < nb; ++r) {
-                        const double pr = fmod((r + 1) * kGold, 1.0);
-                        s = cadd(s, cscale(cmulc(E[(size_t)r * nb + i0 + a], E[(size_t)r * nb + i0 + b]), pr));
-                    }
+                    for (int r = 0; r < nb; ++r)
+                        s = cadd(s, cscale(cmulc(E[(size_t)r * nb + i0 + a], E[(size_t)r * nb + i0 + b]), prw[r]));
                     M[a * k + b] = s;
                 }
             for (int a = 0; a < k; ++a)
@@ -345,13 +384,7 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
     // phase fix and chop
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         cd s = {0, 0};
-        for (int r = 0; r < nb; ++r) {
-            const double kk = (double)(r + 1);
-            const double amp = 1.0 + fmod(kk * kPlastic, 1.0);
-            double sn, cs;
-            sincos(0.9 * kk, &sn, &cs);
-            s = cadd(s, cmul({amp * cs, amp * sn}, E[(size_t)r * nb + b]));
-        }
+        for (int r = 0; r < nb; ++r) s = cadd(s, cmul(phw[r], E[(size_t)r * nb + b]));
         const double a = sqrt(cabs2(s));
         const cd ph = {s.x / a, -s.y / a};
         for (int r = 0; r < nb; ++r) {
@@ -401,9 +434,9 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
     __syncthreads();
     if (threadIdx.x == 0) {
         // return_degenerate_subspaces grouping (…_dynamicalmatrix.f90:911-980)
-        int sub[128];
+        int *sub = s_sub;
         int nsub = 0;
-        int first[128];
+        int *first = s_first;
         for (int i = 0; i < nb; ++i) {
             int s = -1;
             for (int j = 0; j < i && s < 0; ++j)
@@ -515,44 +548,45 @@ int run_harmonic(tc3_ctx *c) {
     cd *d_D = nullptr, *d_gD = nullptr, *d_V = nullptr, *d_scr = nullptr;
     const size_t nn = (size_t)nb * nb;
     const int npair = (int)c->p_a1.size();
-    auto cleanup = [&]() {
-        hipFree(d_qv); hipFree(d_lv); hipFree(d_pm); hipFree(d_ism); hipFree(d_lam);
-        hipFree(d_opm); hipFree(d_v0); hipFree(d_qirr); hipFree(d_blk); hipFree(d_stat);
-        hipFree(d_atg); hipFree(d_src); hipFree(d_opf); hipFree(d_D); hipFree(d_gD);
-        hipFree(d_V); hipFree(d_scr);
-    };
+    auto cleanup = [&]() {};
     hipError_t e = hipSuccess;
 #define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
-    TRY(hipMalloc(&d_qv, sizeof(double) * 3 * nirr));
-    TRY(hipMalloc(&d_qirr, sizeof(double) * 3 * nirr));
-    TRY(hipMalloc(&d_lv, sizeof(double) * 3 * std::max(npair, 1)));
-    TRY(hipMalloc(&d_pm, sizeof(double) * 9 * std::max(npair, 1)));
-    TRY(hipMalloc(&d_ism, sizeof(double) * na));
-    TRY(hipMalloc(&d_blk, sizeof(int) * (na * na + 1)));
-    TRY(hipMalloc(&d_D, sizeof(cd) * nn * nirr));
-    TRY(hipMalloc(&d_gD, sizeof(cd) * 3 * nn * nirr));
-    TRY(hipMalloc(&d_V, sizeof(cd) * nn * nirr));
-    TRY(hipMalloc(&d_lam, sizeof(double) * nb * nirr));
-    TRY(hipMalloc(&d_stat, sizeof(int) * nirr));
-    TRY(hipMalloc(&d_atg, sizeof(int) * nirr));
-    TRY(hipMalloc(&d_scr, sizeof(cd) * (2 * nn + nb) * nirr));
-    TRY(hipMalloc(&d_opm, sizeof(double) * 9 * nsym));
-    TRY(hipMalloc(&d_v0, sizeof(double) * 3 * nsym * na));
-    TRY(hipMalloc(&d_src, sizeof(int) * nsym * na));
-    TRY(hipMalloc(&d_opf, sizeof(int) * nfull));
-    // persistent outputs
-    hipFree(c->d_om_irr); hipFree(c->d_om_full); hipFree(c->d_vel_irr); hipFree(c->d_vel_full);
-    hipFree(c->d_egv_irr); hipFree(c->d_egv_full); hipFree(c->d_irr_of_full);
-    c->d_om_irr = c->d_om_full = c->d_vel_irr = c->d_vel_full = nullptr;
-    c->d_egv_irr = c->d_egv_full = nullptr;
-    c->d_irr_of_full = nullptr;
-    TRY(hipMalloc(&c->d_om_irr, sizeof(double) * nb * nirr));
-    TRY(hipMalloc(&c->d_vel_irr, sizeof(double) * 3 * nb * nirr));
-    TRY(hipMalloc(&c->d_egv_irr, sizeof(cd) * nn * nirr));
-    TRY(hipMalloc(&c->d_om_full, sizeof(double) * nb * nfull));
-    TRY(hipMalloc(&c->d_vel_full, sizeof(double) * 3 * nb * nfull));
-    TRY(hipMalloc(&c->d_egv_full, sizeof(cd) * nn * nfull));
-    TRY(hipMalloc(&c->d_irr_of_full, sizeof(int) * nfull));
+    // work buffers from the context's pool (kept across calls)
+    auto work = [&](const char *name, void **p, size_t bytes) { return c->pool_get(name, std::max<size_t>(bytes, 16), p); };
+    TRY(work("h_qv", (void **)&d_qv, sizeof(double) * 3 * nirr));
+    TRY(work("h_qirr", (void **)&d_qirr, sizeof(double) * 3 * nirr));
+    TRY(work("h_lv", (void **)&d_lv, sizeof(double) * 3 * std::max(npair, 1)));
+    TRY(work("h_pm", (void **)&d_pm, sizeof(double) * 9 * std::max(npair, 1)));
+    TRY(work("h_ism", (void **)&d_ism, sizeof(double) * na));
+    TRY(work("h_blk", (void **)&d_blk, sizeof(int) * (na * na + 1)));
+    TRY(work("h_D", (void **)&d_D, sizeof(cd) * nn * nirr));
+    TRY(work("h_gD", (void **)&d_gD, sizeof(cd) * 3 * nn * nirr));
+    TRY(work("h_V", (void **)&d_V, sizeof(cd) * nn * nirr));
+    TRY(work("h_lam", (void **)&d_lam, sizeof(double) * nb * nirr));
+    TRY(work("h_stat", (void **)&d_stat, sizeof(int) * nirr));
+    TRY(work("h_atg", (void **)&d_atg, sizeof(int) * nirr));
+    TRY(work("h_scr", (void **)&d_scr, sizeof(cd) * (2 * nn + nb) * nirr));
+    TRY(work("h_opm", (void **)&d_opm, sizeof(double) * 9 * nsym));
+    TRY(work("h_v0", (void **)&d_v0, sizeof(double) * 3 * nsym * na));
+    TRY(work("h_src", (void **)&d_src, sizeof(int) * nsym * na));
+    TRY(work("h_opf", (void **)&d_opf, sizeof(int) * nfull));
+    // persistent outputs, reallocated only when their sizes change
+    if (c->harm_dims[0] != nb || c->harm_dims[1] != nirr || c->harm_dims[2] != nfull) {
+        hipFree(c->d_om_irr); hipFree(c->d_om_full); hipFree(c->d_vel_irr); hipFree(c->d_vel_full);
+        hipFree(c->d_egv_irr); hipFree(c->d_egv_full); hipFree(c->d_irr_of_full);
+        c->d_om_irr = c->d_om_full = c->d_vel_irr = c->d_vel_full = nullptr;
+        c->d_egv_irr = c->d_egv_full = nullptr;
+        c->d_irr_of_full = nullptr;
+        c->harm_dims[0] = c->harm_dims[1] = c->harm_dims[2] = -1;
+        TRY(hipMalloc(&c->d_om_irr, sizeof(double) * nb * nirr));
+        TRY(hipMalloc(&c->d_vel_irr, sizeof(double) * 3 * nb * nirr));
+        TRY(hipMalloc(&c->d_egv_irr, sizeof(cd) * nn * nirr));
+        TRY(hipMalloc(&c->d_om_full, sizeof(double) * nb * nfull));
+        TRY(hipMalloc(&c->d_vel_full, sizeof(double) * 3 * nb * nfull));
+        TRY(hipMalloc(&c->d_egv_full, sizeof(cd) * nn * nfull));
+        TRY(hipMalloc(&c->d_irr_of_full, sizeof(int) * nfull));
+        c->harm_dims[0] = nb; c->harm_dims[1] = nirr; c->harm_dims[2] = nfull;
+    }
     TRY(hipMemcpyAsync(d_qv, qv.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_qirr, qirr.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
     if (npair) {
@@ -573,14 +607,22 @@ int run_harmonic(tc3_ctx *c) {
     // matrices in LDS when they fit next to the kernels' static arrays
     const size_t jac_lds = 2 * sizeof(cd) * (size_t)nb * nb, post_lds = 2 * sizeof(cd) * (size_t)nb * nb;
     const int jac_use = jac_lds <= 144 * 1024, post_use = post_lds <= 96 * 1024;
-    if (jac_use) TRY(hipFuncSetAttribute((const void *)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jac_lds));
-    if (post_use) TRY(hipFuncSetAttribute((const void *)k_post, hipFuncAttributeMaxDynamicSharedMemorySize, (int)post_lds));
-    hipLaunchKernelGGL(k_jacobi, dim3(nirr), dim3(kBlock), jac_use ? jac_lds : 0, st, nb, d_D, d_V, d_lam, 100, d_stat,
-                       jac_use);
+    if (jac_use) {
+        TRY(hipFuncSetAttribute((const void *)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jac_lds));
+        hipLaunchKernelGGL(k_jacobi<true>, dim3(nirr), dim3(kBlock), jac_lds, st, nb, d_D, d_V, d_lam, 100, d_stat);
+    } else {
+        hipLaunchKernelGGL(k_jacobi<false>, dim3(nirr), dim3(kBlock), 0, st, nb, d_D, d_V, d_lam, 100, d_stat);
+    }
     TRY(hipGetLastError());
     const double veltol = 1e-9 / kGroupvelAuToMs;
-    hipLaunchKernelGGL(k_post, dim3(nirr), dim3(kBlock), post_use ? post_lds : 0, st, nb, d_lam, d_V, d_gD, d_atg,
-                       c->d_om_irr, c->d_egv_irr, c->d_vel_irr, d_scr, veltol, post_use);
+    if (post_use) {
+        TRY(hipFuncSetAttribute((const void *)k_post<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)post_lds));
+        hipLaunchKernelGGL(k_post<true>, dim3(nirr), dim3(kBlock), post_lds, st, nb, d_lam, d_V, d_gD, d_atg,
+                           c->d_om_irr, c->d_egv_irr, c->d_vel_irr, d_scr, veltol);
+    } else {
+        hipLaunchKernelGGL(k_post<false>, dim3(nirr), dim3(kBlock), 0, st, nb, d_lam, d_V, d_gD, d_atg, c->d_om_irr,
+                           c->d_egv_irr, c->d_vel_irr, d_scr, veltol);
+    }
     TRY(hipGetLastError());
     hipLaunchKernelGGL(k_unfold, dim3(nfull), dim3(kBlock), 0, st, nb, na, c->d_irr_of_full, d_opf, d_opm,
                        d_src, d_v0, d_qirr, c->d_egv_irr, c->d_vel_irr, c->d_om_irr, c->d_egv_full,
@@ -591,18 +633,13 @@ int run_harmonic(tc3_ctx *c) {
     c->om_irr.resize((size_t)nb * nirr);
     c->vel_irr.resize((size_t)3 * nb * nirr);
     c->egv_irr.resize(nn * nirr);
-    c->om_full.resize((size_t)nb * nfull);
-    c->vel_full.resize((size_t)3 * nb * nfull);
-    c->egv_full.resize(nn * nfull);
     TRY(hipMemcpyAsync(c->om_irr.data(), c->d_om_irr, sizeof(double) * nb * nirr, hipMemcpyDeviceToHost, st));
     TRY(hipMemcpyAsync(c->vel_irr.data(), c->d_vel_irr, sizeof(double) * 3 * nb * nirr, hipMemcpyDeviceToHost, st));
     TRY(hipMemcpyAsync(c->egv_irr.data(), c->d_egv_irr, sizeof(cd) * nn * nirr, hipMemcpyDeviceToHost, st));
-    TRY(hipMemcpyAsync(c->om_full.data(), c->d_om_full, sizeof(double) * nb * nfull, hipMemcpyDeviceToHost, st));
-    TRY(hipMemcpyAsync(c->vel_full.data(), c->d_vel_full, sizeof(double) * 3 * nb * nfull, hipMemcpyDeviceToHost, st));
-    TRY(hipMemcpyAsync(c->egv_full.data(), c->d_egv_full, sizeof(cd) * nn * nfull, hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
 #undef TRY
-    cleanup();
+    // full-grid results stay on the device; host copies are made on first use
+    c->full_host_valid = false;
     for (int k = 0; k < nirr; ++k)
         if (stat[k]) return c->fail(2, "Jacobi eigen-solver did not converge");
     // degeneracy table (type_phonon_dispersions.f90:1190-1214)
@@ -626,6 +663,27 @@ int run_harmonic(tc3_ctx *c) {
                 c->row_b.push_back(b);
             }
     c->nrows = (int)c->row_q.size();
+    return 0;
+}
+
+// Host copies of the full-grid frequencies, velocities and eigenvectors (on demand).
+int ensure_full_host(tc3_ctx *c) {
+    if (c->full_host_valid) return 0;
+    const int nb = c->nb, nfull = c->mesh.nfull;
+    const size_t nn = (size_t)nb * nb;
+    c->om_full.resize((size_t)nb * nfull);
+    c->vel_full.resize((size_t)3 * nb * nfull);
+    c->egv_full.resize(nn * nfull);
+    hipError_t e = hipMemcpyAsync(c->om_full.data(), c->d_om_full, sizeof(double) * nb * nfull,
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->vel_full.data(), c->d_vel_full, sizeof(double) * 3 * nb * nfull, hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->egv_full.data(), c->d_egv_full, sizeof(cd) * nn * nfull, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return device_error(e, "full-grid harmonic results to the host", __FILE__, __LINE__);
+    c->full_host_valid = true;
     return 0;
 }
 
