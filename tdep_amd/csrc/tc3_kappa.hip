@@ -39,10 +39,30 @@ void chop_kappa(double *k) {
     for (int i = 0; i < 9; ++i) k[i] = chop_scalar(k[i], s * 1e-6);
 }
 
-// get_kappa: kappa (column-major 3x3) and optional per-mode tensors.
+// The operation average of get_kappa, (1/N) sum_R (R F)_i (R v)_j = (1/N) sum_R (R X R^T)_ij
+// with X = F v^T, is a fixed linear map on 3x3 matrices: S[3 j + i][3 b + a] =
+// (1/N) sum_R R_ia R_jb (column-major indices, like the kappa arrays).
+void op_average_map(const tc3_ctx *c, double *S) {
+    const int nsym = (int)c->ops.size();
+    std::fill(S, S + 81, 0.0);
+    for (int o = 0; o < nsym; ++o) {
+        const double *m = c->ops[o].m;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i)
+                for (int b = 0; b < 3; ++b)
+                    for (int a = 0; a < 3; ++a) S[9 * (3 * j + i) + 3 * b + a] += m[3 * i + a] * m[3 * j + b];
+    }
+    for (int x = 0; x < 81; ++x) S[x] /= nsym;
+}
+
+// get_kappa (kappa.f90:28-75): kappa (column-major 3x3) and optional per-mode tensors.
+// Without per-mode output the operation average is applied once to the weighted sum of
+// F v^T (it is linear); with it, per mode.
 void get_kappa(const tc3_ctx *c, const double *Fn, double T, bool classical, double *kappa, double *kmode) {
-    const int nb = c->nb, nirr = c->mesh.nirr, nsym = (int)c->ops.size();
-    double k[9] = {0};
+    const int nb = c->nb, nirr = c->mesh.nirr;
+    double S[81];
+    op_average_map(c, S);
+    double X[9] = {0}, k[9] = {0};
     for (int q = 0; q < nirr; ++q)
         for (int b = 0; b < nb; ++b) {
             const double om = c->om_irr[(size_t)q * nb + b];
@@ -51,22 +71,27 @@ void get_kappa(const tc3_ctx *c, const double *Fn, double T, bool classical, dou
             const double cv = classical ? kKbHartree : cv_host(T, om);
             const double *F = Fn + ((size_t)q * nb + b) * 3;
             const double *v = &c->vel_irr[((size_t)q * nb + b) * 3];
-            double v2[9] = {0};
-            for (int o = 0; o < nsym; ++o) {
-                const double *m = c->ops[o].m;
-                double v0[3], v1[3];
-                for (int i = 0; i < 3; ++i) {
-                    v0[i] = m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2];
-                    v1[i] = m[3 * i] * v[0] + m[3 * i + 1] * v[1] + m[3 * i + 2] * v[2];
+            const double f = cv / c->cr.volume;
+            if (kmode) {
+                double *km = kmode + ((size_t)q * nb + b) * 9;
+                for (int x = 0; x < 9; ++x) {
+                    double sx = 0;
+                    for (int bb = 0; bb < 3; ++bb)
+                        for (int a = 0; a < 3; ++a) sx += S[9 * x + 3 * bb + a] * F[a] * v[bb];
+                    km[x] = f * sx;
+                    k[x] += km[x] * c->mesh.weight_irr[q];
                 }
-                for (int j = 0; j < 3; ++j)
-                    for (int i = 0; i < 3; ++i) v2[3 * j + i] += v0[i] * v1[j] / nsym;
+            } else {
+                const double w = f * c->mesh.weight_irr[q];
+                for (int bb = 0; bb < 3; ++bb)
+                    for (int a = 0; a < 3; ++a) X[3 * bb + a] += w * F[a] * v[bb];
             }
-            for (int i = 0; i < 9; ++i) {
-                const double buf = cv * v2[i] / c->cr.volume;
-                if (kmode) kmode[((size_t)q * nb + b) * 9 + i] = buf;
-                k[i] += buf * c->mesh.weight_irr[q];
-            }
+        }
+    if (!kmode)
+        for (int x = 0; x < 9; ++x) {
+            double sx = 0;
+            for (int y = 0; y < 9; ++y) sx += S[9 * x + y] * X[y];
+            k[x] = sx;
         }
     chop_kappa(k);
     std::memcpy(kappa, k, sizeof(k));
@@ -632,9 +657,8 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
                 const double cv = c->classical ? kKbHartree : cv_host(T, om);
                 cvw[(size_t)iT * qb + (size_t)q * nb + b] = cv * c->mesh.weight_irr[q] / c->cr.volume;
             }
-        double k0[9];
-        get_kappa(c, Fn, T, c->classical, k0, nullptr);
-        c->kappa_log[iT].insert(c->kappa_log[iT].end(), {k0[0], k0[4], k0[8], k0[3], k0[6], k0[7], 0.0});
+        // the first record is the kappa of F0 (kappa.f90:377), i.e. the SMA value
+        c->kappa_log[iT].insert(c->kappa_log[iT].end(), {ks[0], ks[4], ks[8], ks[3], ks[6], ks[7], 0.0});
     }
     const auto a = std::chrono::steady_clock::now();
     bool any = false;
