@@ -93,6 +93,10 @@ struct tc3_ctx {
     long gen_model = 0;
     long scatter_tables_gen = -1;
     long adjoint_tables_gen = -1;
+    // autotuned triplets per launch of the MFMA contraction, by shape, and the probes timed
+    std::map<std::string, long> tuned_batch;
+    struct TuneProbe { std::string key; double batch, ms; };
+    std::vector<TuneProbe> tune_log;
     // instantiated BTE iteration graph and the arguments it was captured with
     hipGraphExec_t bte_exec = nullptr;
     std::vector<long> bte_key;

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "tc3_chain.hpp"
 #include "tc3_ctx.hpp"
@@ -392,15 +393,14 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         TRY(table("blist", (void **)&d_blist, blist.data(), sizeof(int4) * blist.size()));
         TRY(table("a1boff", (void **)&d_a1boff, a1_boff.data(), sizeof(int) * a1_boff.size()));
         TRY(table("a12boff", (void **)&d_a12boff, a12_boff.data(), sizeof(int) * a12_boff.size()));
-        // W of one triplet: 2 * nb^3 reals; batches of ~192 MB keep it in the last-level cache
+        // W of one triplet: 2 * nb^3 reals.  Triplets per launch pair: at most 8 GB of W;
+        // TC3_BATCH_MB forces a size, otherwise the autotuner below picks it.
         const size_t es = mfma_elem_size(precision);
         const size_t wtrip = 2 * (size_t)nb3 * es;
-        long mb = 192l << 20;
-        if (const char *env = std::getenv("TC3_BATCH_MB")) mb = std::max(1l, std::atol(env)) << 20;
-        long mbatch = std::max<long>(64, mb / (long)wtrip);
-        mbatch = std::min<long>(mbatch, ntrip);
-        mbatch = std::min<long>(mbatch, (long)((size_t)8 << 30) / (long)wtrip);
-        mbatch = std::max<long>(mbatch, 1);
+        const long bmax = std::max<long>(1, std::min<long>(ntrip, (long)((size_t)8 << 30) / (long)wtrip));
+        long mbatch = bmax;
+        const char *env_mb = std::getenv("TC3_BATCH_MB");
+        if (env_mb) mbatch = std::max<long>(1, std::min<long>(bmax, (std::max(1l, std::atol(env_mb)) << 20) / (long)wtrip));
         MfmaPlan mp;
         mp.na = na; mp.nb = nb; mp.nirr = nirr; mp.nfull = nfull; mp.maxa1 = std::max(maxa1, 1);
         mp.maxblk = std::max(maxblk, 1);
@@ -417,7 +417,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             TRY(c->pool_get("m_Eti", 2 * mfma_split_elems(nb) * nirr, &mp.Eti));
             TRY(c->pool_get("m_Etf", 2 * mfma_split_elems(nb) * nfull, &mp.Etf));
         }
-        TRY(c->pool_get("m_work", wtrip * mbatch, &mp.work));
+        TRY(c->pool_get("m_work", wtrip * bmax, &mp.work));
         mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off;
         mp.blist = d_blist; mp.a1_boff = d_a1boff; mp.a12_boff = d_a12boff;
         mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
@@ -428,6 +428,49 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
         me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
         me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
+        // Autotuner: on the first pass of a shape (band count, precision, temperature block,
+        // triplet count), time one launch pair for each candidate batch (the largest, then
+        // halving, at least 64) over the first triplets and keep the fastest per triplet.
+        // The probes compute real results for those triplets; the pass below rewrites them.
+        if (!env_mb) {
+            const std::string key = std::to_string(nb) + "/" + std::to_string(precision) + "/" +
+                                    std::to_string(mp.nty) + "/" + std::to_string(ntrip);
+            auto it = c->tuned_batch.find(key);
+            if (it != c->tuned_batch.end()) {
+                mbatch = std::min<long>(it->second, bmax);
+            } else {
+                std::vector<long> cand;
+                for (long b = bmax; cand.size() < 4 && (b >= 64 || cand.empty()); b /= 2) cand.push_back(b);
+                hipEvent_t p0, p1;
+                TRY(hipEventCreate(&p0));
+                TRY(hipEventCreate(&p1));
+                double best = 1e300;
+                MfmaPlan pp = mp;
+                pp.ev = nullptr;
+                pp.epi.T0 = 0;
+                pp.epi.nT = std::min(mp.nty, nT);
+                pp.epi.be = d_be;
+                pp.epi.xi = c->d_xi;
+                for (long b : cand) {
+                    pp.ntrip = b;
+                    pp.batch = b;
+                    TRY(hipEventRecord(p0, st));
+                    int r = run_mfma_chain(pp, precision, st);
+                    if (r) { hipEventDestroy(p0); hipEventDestroy(p1); cleanup(); return r; }
+                    TRY(hipEventRecord(p1, st));
+                    TRY(hipEventSynchronize(p1));
+                    float ms = 0;
+                    hipEventElapsedTime(&ms, p0, p1);
+                    const double per = ms / (double)b;
+                    c->tune_log.push_back({key, (double)b, (double)ms});
+                    if (per < best) { best = per; mbatch = b; }
+                }
+                hipEventDestroy(p0);
+                hipEventDestroy(p1);
+                c->tuned_batch[key] = mbatch;
+            }
+        }
+        mp.batch = mbatch;
         const long nbatch = (ntrip + mbatch - 1) / mbatch;
         const int ntchunk = (nT + mp.nty - 1) / mp.nty;
         kev.resize(3 * nbatch * ntchunk);
