@@ -128,6 +128,193 @@ __global__ void k_epilogue(EpiArgs a, const P *__restrict__ psi, int psi_complex
     }
 }
 
+// Small cells (nb <= 6): one persistent kernel fed by a work queue.  Every wave takes the
+// next triplet index from a device counter and does the whole chain for it in LDS: phases,
+// Phi (pretransform_phi3, scattering_threephonon.f90:275-317), the three contractions into
+// Psi (:88-113), the smeared-delta weights of every temperature (get_dirac :219-271) and
+// the linewidth partial and scattering-matrix entries the triplet owns (:115-135) -- the
+// same arithmetic as k_phi + k_cgemm + k_epilogue, without batch buffers in HBM.  Triplets
+// finish in any order; each writes only its own outputs, so results do not depend on it.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, bool SPLIT>
+__device__ inline void cmac(T &rx, T &ry, cplx<T> a, cplx<T> b) {
+    if constexpr (SPLIT) {
+        float arh, arl, aih, ail, brh, brl, bih, bil;
+        split_bf16(a.x, arh, arl);
+        split_bf16(a.y, aih, ail);
+        split_bf16(b.x, brh, brl);
+        split_bf16(b.y, bih, bil);
+        rx += mul3(arh, arl, brh, brl) - mul3(aih, ail, bih, bil);
+        ry += mul3(arh, arl, bih, bil) + mul3(aih, ail, brh, brl);
+    } else {
+        rx += a.x * b.x - a.y * b.y;
+        ry += a.x * b.y + a.y * b.x;
+    }
+}
+
+// LDS bytes per wave of k_small.
+template <int NB, typename T>
+size_t small_wave_lds(int nfc3) {
+    constexpr int NB2 = NB * NB, NB3 = NB2 * NB;
+    size_t b = sizeof(cd) * (size_t)std::max(nfc3, 1);
+    b += sizeof(cplx<T>) * (3 * (size_t)NB3 + 3 * (size_t)NB2);
+    b += sizeof(double) * ((size_t)NB3 + 2 * (size_t)NB2);
+    return (b + 15) & ~(size_t)15;
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+template <int NB, typename T, bool SPLIT>
+__global__ void __launch_bounds__(256) k_small(EpiArgs a, int ntrip, int *__restrict__ queue, size_t wave_lds,
+                                               const cplx<T> *__restrict__ E1all, const cplx<T> *__restrict__ Efall,
+                                               const double *__restrict__ qcart, const int *__restrict__ blk_off,
+                                               const double *__restrict__ lv2, const double *__restrict__ lv3,
+                                               const double *__restrict__ mwm, int nfc3) {
+    constexpr int NA = NB / 3, NB2 = NB * NB, NB3 = NB2 * NB;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned char *base = smem + (size_t)w * wave_lds;
+    cd *ph = (cd *)base;
+    cplx<T> *phi = (cplx<T> *)(base + sizeof(cd) * (size_t)std::max(nfc3, 1));
+    cplx<T> *t1 = phi + NB3, *t2 = t1 + NB3;
+    cplx<T> *e1 = t2 + NB3, *e2 = e1 + NB2, *e3 = e2 + NB2;
+    double *fs = (double *)(e3 + NB2);
+    double *s2 = fs + NB3, *s3 = s2 + NB2;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(queue, 1);
+        t = __shfl(t, 0, 64);
+        if (t >= ntrip) break;
+        const int q1 = a.tq1[t], q2 = a.tq2[t], q3 = a.tq3[t];
+        for (int e = lane; e < NB2; e += 64) {
+            e1[e] = E1all[(size_t)q1 * NB2 + e];
+            e2[e] = Efall[(size_t)q2 * NB2 + e];
+            e3[e] = Efall[(size_t)q3 * NB2 + e];
+        }
+        const double *Q2 = qcart + 3 * q2, *Q3 = qcart + 3 * q3;
+        for (int f = lane; f < nfc3; f += 64) {
+            const double arg = -kTwoPi * (Q2[0] * lv2[3 * f] + Q2[1] * lv2[3 * f + 1] + Q2[2] * lv2[3 * f + 2] +
+                                          Q3[0] * lv3[3 * f] + Q3[1] * lv3[3 * f + 1] + Q3[2] * lv3[3 * f + 2]);
+            double sn, cs;
+            sincos(arg, &sn, &cs);
+            ph[f] = {cs, sn};
+        }
+        wave_sync();
+        // Phi[ia][ib][ic]
+        for (int e = lane; e < NB3; e += 64) {
+            const int ia = e / NB2, ib = (e / NB) % NB, ic = e % NB;
+            const int blk = ((ia / 3) * NA + ib / 3) * NA + ic / 3;
+            const int comp = 9 * (ia % 3) + 3 * (ib % 3) + ic % 3;
+            double ax = 0, ay = 0;
+            for (int f = blk_off[blk]; f < blk_off[blk + 1]; ++f) {
+                const double m = mwm[27 * f + comp];
+                const cd p = ph[f];
+                ax += m * p.x;
+                ay += m * p.y;
+            }
+            phi[e] = {(T)ax, (T)ay};
+        }
+        wave_sync();
+        // t1[b1][ib][ic] = sum_ia E1[ia][b1] Phi[ia][ib][ic]
+        for (int e = lane; e < NB3; e += 64) {
+            const int b1 = e / NB2, r = e % NB2;
+            T rx = 0, ry = 0;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) cmac<T, SPLIT>(rx, ry, e1[k * NB + b1], phi[k * NB2 + r]);
+            t1[e] = {rx, ry};
+        }
+        wave_sync();
+        // t2[b1][b2][ic] = sum_ib E2[ib][b2] t1[b1][ib][ic]
+        for (int e = lane; e < NB3; e += 64) {
+            const int b1 = e / NB2, b2 = (e / NB) % NB, ic = e % NB;
+            T rx = 0, ry = 0;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) cmac<T, SPLIT>(rx, ry, e2[k * NB + b2], t1[b1 * NB2 + k * NB + ic]);
+            t2[e] = {rx, ry};
+        }
+        wave_sync();
+        // Psi[b1][b2][b3] = sum_ic t2[b1][b2][ic] E3[ic][b3], into the Phi buffer
+        for (int e = lane; e < NB3; e += 64) {
+            const int r = e / NB, b3 = e % NB;
+            T rx = 0, ry = 0;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) cmac<T, SPLIT>(rx, ry, t2[r * NB + k], e3[k * NB + b3]);
+            phi[e] = {rx, ry};
+        }
+        wave_sync();
+        const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
+        const double perm = a.perm[t];
+        const int e0 = a.eq_off[t], e1q = a.eq_off[t + 1];
+        for (int T_ = 0; T_ < a.nT; ++T_) {
+            const double *be = a.be + (size_t)T_ * a.nirr * NB;
+            for (int e = lane; e < NB3; e += 64) {
+                const int b1 = e / NB2, b2 = (e / NB) % NB, b3 = e % NB;
+                const double w1 = a.om_irr[q1 * NB + b1];
+                const double w2 = a.om_full[(size_t)q2 * NB + b2], w3 = a.om_full[(size_t)q3 * NB + b3];
+                double f = 0.0;
+                if (a.row_of[q1 * NB + b1] >= 0 && !(w2 < 1e-9) && !(fabs(w1 - w2) < 1e-9) && !(w3 < 1e-9) &&
+                    !(fabs(w1 - w3) < 1e-9) && !(fabs(w2 - w3) < 1e-9)) {
+                    const double re = (double)phi[e].x, im = (double)phi[e].y;
+                    const double psisq = a.psi_scale * (re * re + im * im);
+                    const double sig = (a.integrationtype == 2)
+                                           ? sqrt(a.sigsq[q1 * NB + b1] + a.sigsq[i2 * NB + b2] + a.sigsq[i3 * NB + b3])
+                                           : a.sigma_fixed;
+                    const double d0 = gauss(w1, -w2 + w3, sig) - gauss(w1, w2 - w3, sig);
+                    const double d1 = gauss(w1, w2 + w3, sig) - gauss(w1, -w2 - w3, sig);
+                    const double n2 = be[i2 * NB + b2], n3 = be[i3 * NB + b3];
+                    f = perm * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1);
+                }
+                fs[e] = f;
+            }
+            wave_sync();
+            for (int x = lane; x < NB2; x += 64) {
+                const int b1 = x / NB, b = x % NB;
+                double u = 0, v = 0;
+                for (int k = 0; k < NB; ++k) {
+                    u += fs[b1 * NB2 + b * NB + k];
+                    v += fs[b1 * NB2 + k * NB + b];
+                }
+                s2[x] = u;
+                s3[x] = v;
+            }
+            wave_sync();
+            for (int b1 = lane; b1 < NB; b1 += 64) {
+                double tot = 0;
+                for (int b = 0; b < NB; ++b) tot += s2[b1 * NB + b];
+                a.gpart[((size_t)t * a.nT + T_) * NB + b1] =
+                    (a.row_of[q1 * NB + b1] >= 0) ? tot * (double)(e1q - e0) : 0.0;
+            }
+            // every (row, q', band) target belongs to one triplet and one equivalent pair:
+            // store the values (q2' == q3' receives both terms)
+            T *xiT = (T *)a.xi + (size_t)T_ * a.xi_T;
+            for (int e = e0; e < e1q; ++e) {
+                const int x2 = a.eq_q2[e], x3 = a.eq_q3[e];
+                for (int x = lane; x < NB2; x += 64) {
+                    const int b1 = x / NB, b = x % NB;
+                    const int row = a.row_of[q1 * NB + b1];
+                    if (row < 0) continue;
+                    const double w1 = a.om_irr[q1 * NB + b1];
+                    T *xr = xiT + (size_t)row * a.xi_row;
+                    const double v2 = 2.0 * s2[x] * a.om_full[(size_t)q2 * NB + b] / w1;
+                    const double v3 = 2.0 * s3[x] * a.om_full[(size_t)q3 * NB + b] / w1;
+                    if (x2 == x3) {
+                        xr[(size_t)x2 * NB + b] = (T)(v2 + v3);
+                    } else {
+                        xr[(size_t)x2 * NB + b] = (T)v2;
+                        xr[(size_t)x3 * NB + b] = (T)v3;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
 // Linewidth of every row: deterministic sum over the triplets of its q1.
 __global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_q, const int *__restrict__ row_b,
                         const int *__restrict__ q1_off, const double *__restrict__ gpart, double *__restrict__ lw) {
@@ -330,10 +517,23 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     const size_t per_trip = 4 * (size_t)nb3 * elem;
     long batch = std::max<long>(1, std::min<long>(ntrip, (long)((size_t)2 << 30) / (long)per_trip));
     batch = std::min<long>(batch, 1 << 20);
+    // small cells go to the persistent kernel when its per-wave LDS fits (4 waves per
+    // workgroup while at least two workgroups fit a CU, fewer otherwise)
+    size_t small_lds = 0;
+    int small_waves = 0;
+    if (!use_mfma && (nb == 3 || nb == 6) && std::getenv("TC3_NO_SMALL") == nullptr) {
+        small_lds = (nb == 3) ? ((precision == TC3_PREC_FP64) ? small_wave_lds<3, double>(c->nfc3)
+                                                              : small_wave_lds<3, float>(c->nfc3))
+                              : ((precision == TC3_PREC_FP64) ? small_wave_lds<6, double>(c->nfc3)
+                                                              : small_wave_lds<6, float>(c->nfc3));
+        for (int nw = 4; nw >= 1 && small_waves == 0; --nw)
+            if (small_lds * nw <= 80 * 1024 || (nw == 1 && small_lds <= 160 * 1024)) small_waves = nw;
+    }
+    const bool use_small = small_waves > 0;
     void *d_work = nullptr;
     void *d_E1 = nullptr, *d_Ef = nullptr;
     if (!use_mfma) {
-        TRY(c->pool_get("g_work", per_trip * batch, &d_work));
+        if (!use_small) TRY(c->pool_get("g_work", per_trip * batch, &d_work));
         TRY(c->pool_get("g_E1", elem * (size_t)nirr * nb * nb, &d_E1));
         TRY(c->pool_get("g_Ef", elem * (size_t)nfull * nb * nb, &d_Ef));
         if (precision == TC3_PREC_FP64) {
@@ -488,6 +688,41 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             if (r) { cleanup(); return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r; }
         }
         c->batch = mbatch;
+    } else if (use_small) {
+        // persistent work-queue kernel for small cells
+        TRY(hipEventRecord(ev0, st));
+        kev.resize(3);
+        for (auto &x : kev) TRY(hipEventCreate(&x));
+        int *d_queue;
+        TRY(c->pool_get("small_queue", sizeof(int), (void **)&d_queue));
+        int ncu = 0;
+        TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        const int nwg = std::max(1, ncu) * 8;
+        TRY(hipEventRecord(kev[0], st));
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_queue);
+        TRY(hipEventRecord(kev[1], st));
+        const dim3 g(nwg), b(64 * small_waves);
+        const size_t shm = small_lds * small_waves;
+        auto launch = [&](auto kern, auto E1p, auto Efp) -> hipError_t {
+            hipError_t r = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (r != hipSuccess) return r;
+            hipLaunchKernelGGL(kern, g, b, shm, st, ea, (int)ntrip, d_queue, small_lds, E1p, Efp, d_qcart, d_blk3,
+                               d_lv2, d_lv3, d_mwm, c->nfc3);
+            return hipGetLastError();
+        };
+        using C64 = const cplx<double> *;
+        using C32 = const cplx<float> *;
+        if (nb == 3) {
+            if (precision == TC3_PREC_FP64) TRY(launch(k_small<3, double, false>, (C64)d_E1, (C64)d_Ef));
+            else if (precision == TC3_PREC_FP32) TRY(launch(k_small<3, float, false>, (C32)d_E1, (C32)d_Ef));
+            else TRY(launch(k_small<3, float, true>, (C32)d_E1, (C32)d_Ef));
+        } else {
+            if (precision == TC3_PREC_FP64) TRY(launch(k_small<6, double, false>, (C64)d_E1, (C64)d_Ef));
+            else if (precision == TC3_PREC_FP32) TRY(launch(k_small<6, float, false>, (C32)d_E1, (C32)d_Ef));
+            else TRY(launch(k_small<6, float, true>, (C32)d_E1, (C32)d_Ef));
+        }
+        TRY(hipEventRecord(kev[2], st));
+        c->batch = ntrip;
     } else {
     TRY(hipEventRecord(ev0, st));
     kev.resize(3 * ((ntrip + batch - 1) / batch));
