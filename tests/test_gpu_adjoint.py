@@ -150,3 +150,39 @@ def test_autograd_functions_route_to_the_adjoint(inputs_dir):
     k[0, 0, 0].backward()
     gm = e.adjoint_kappa_rta(np.eye(3)[None] * np.array([1, 0, 0])[None, :, None])
     assert np.allclose(x.grad.numpy(), [np.sum(gm * 0.6 * m0.numpy()), np.sum(gm * 0.4 * m0.numpy())], rtol=1e-10)
+
+
+def test_gradient_with_respect_to_symmetry_adapted_coefficients(inputs_dir):
+    """m = B x with B the symmetry-adapted basis (space group, permutations, acoustic sum
+    rule; tdep_amd/irreducible.py): dL/dx through autograd equals central differences in
+    x (step 1e-4 relative), and <x, dL/dx> = 2 L for the linewidths (quadratic in x)."""
+    import torch
+    from tdep_amd import autograd as AG, irreducible as IR
+    e = _engine(inputs_dir, "rocksalt", (4, 4, 4))
+    B = IR.engine_basis(e, asr=True)
+    x0 = B.T @ e.fc3["m"].reshape(-1)
+    Bt = torch.from_numpy(B)
+    w = torch.from_numpy(np.random.default_rng(4).standard_normal((1, e.nirr, e.nb)))
+
+    def lw_of(x):
+        return (w * AG.linewidths(e, (Bt @ x).reshape(-1, 3, 3, 3), [300.0])).sum()
+
+    def kxx_of(x):
+        return AG.kappa_rta(e, (Bt @ x).reshape(-1, 3, 3, 3), [300.0])[0, 0, 0]
+
+    for f, tol in ((lw_of, 1e-8), (kxx_of, 1e-5)):
+        x = torch.tensor(x0, requires_grad=True)
+        L = f(x)
+        L.backward()
+        g = x.grad.numpy().copy()
+        if f is lw_of:
+            assert np.isclose(np.dot(g, x0), 2.0 * float(L), rtol=1e-10)
+        for c in np.argsort(-np.abs(x0))[:2]:
+            h = 1e-4 * abs(x0[c])
+            xp, xm = x0.copy(), x0.copy()
+            xp[c] += h
+            xm[c] -= h
+            with torch.no_grad():
+                fd = (float(f(torch.from_numpy(xp))) - float(f(torch.from_numpy(xm)))) / (2 * h)
+            print(f"[adjoint] coefficient {c}: fd {fd:.10e} adjoint {g[c]:.10e}", flush=True)
+            assert abs(fd - g[c]) <= tol * abs(g[c])
