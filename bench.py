@@ -144,7 +144,7 @@ def main():
         t_mx = scatter_pass(eng, temps, args.precision, args.steps, args.warmup)
         ks_mx = kernel_stats(eng)
         tk = eng.timings()
-        log(f"[bench] {args.precision}: {t_mx * 1e3:.2f} ms/step  kernels {ks_mx}")
+        log(f"[bench] {args.precision}: {t_mx * 1e3:.2f} ms/step  kernels {ks_mx}  probes {eng.tuning()}")
         if args.quick:
             return
         # FP64 GPU form of the same pass
@@ -231,6 +231,8 @@ def main():
             "kappa_xx_W_mK_300K": {"mixed": float(kit_mx[2][0, 0] * KAPPA_AU_TO_SI),
                                    "fp64": float(kit_64[2][0, 0] * KAPPA_AU_TO_SI)},
             "roofline": roof,
+            "autotune": {"chosen_batch": batch,
+                         "probes_ms": [[b, ms_] for b, ms_ in eng.tuning()]},
         }
         if not args.no_cpu_baseline:
             try:

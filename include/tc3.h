@@ -177,6 +177,12 @@ int tc3_adjoint_gamma(tc3_ctx *ctx, const double *dL_dlinewidth, int precision, 
  * identity.  Needs tc3_scatter first (not tc3_kappa). */
 int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, double *dL_dfc3);
 
+/* Probes of the contraction's batch autotuner since the context was created: up to maxn
+ * records of (triplets per launch, best probe time in ms) in the order they ran; *n
+ * receives the number of records available.  The chosen batch of the last tc3_scatter is
+ * reported by tc3_get_kernel_stats.  No counterpart in the reference (a tuning aid). */
+int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int *n);
+
 /* Write outfile.thermal_conductivity_grid.hdf5 for temperature iT (layout of
  * lo_phonon_dispersions%write_to_hdf5, type_phonon_dispersions.f90:316-641, after the
  * Fn*omega/T prefactor of main.f90:323-328). */

@@ -504,6 +504,17 @@ int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *l
     return 0;
 }
 
+int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int *n) {
+    if (!ctx || maxn < 0) return 5;
+    const int m = (int)ctx->tune_log.size();
+    if (n) *n = m;
+    for (int i = 0; i < m && i < maxn; ++i) {
+        if (batch) batch[i] = ctx->tune_log[i].batch;
+        if (ms) ms[i] = ctx->tune_log[i].ms;
+    }
+    return 0;
+}
+
 int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path) {
     if (!ctx || !ctx->have_kappa || iT < 0 || iT >= ctx->nT || !path) return 5;
     tc3_ctx *mc = const_cast<tc3_ctx *>(ctx);

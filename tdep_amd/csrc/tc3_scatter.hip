@@ -629,8 +629,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
         me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
         // Autotuner: on the first pass of a shape (band count, precision, temperature block,
-        // triplet count), time one launch pair for each candidate batch (the largest, then
-        // halving, at least 64) over the first triplets and keep the fastest per triplet.
+        // triplet count), time each candidate batch (the largest, then halving, at least 64)
+        // on the same first triplets (as many as the largest batch) and keep the fastest.
         // The probes compute real results for those triplets; the pass below rewrites them.
         if (!env_mb) {
             const std::string key = std::to_string(nb) + "/" + std::to_string(precision) + "/" +
@@ -651,19 +651,32 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
                 pp.epi.nT = std::min(mp.nty, nT);
                 pp.epi.be = d_be;
                 pp.epi.xi = c->d_xi;
-                for (long b : cand) {
-                    pp.ntrip = b;
-                    pp.batch = b;
-                    TRY(hipEventRecord(p0, st));
+                // one untimed launch pair first: code loading and cold caches are not the
+                // candidates' cost
+                pp.ntrip = std::min<long>(ntrip, 64);
+                pp.batch = pp.ntrip;
+                {
                     int r = run_mfma_chain(pp, precision, st);
                     if (r) { hipEventDestroy(p0); hipEventDestroy(p1); cleanup(); return r; }
-                    TRY(hipEventRecord(p1, st));
-                    TRY(hipEventSynchronize(p1));
-                    float ms = 0;
-                    hipEventElapsedTime(&ms, p0, p1);
-                    const double per = ms / (double)b;
-                    c->tune_log.push_back({key, (double)b, (double)ms});
-                    if (per < best) { best = per; mbatch = b; }
+                }
+                for (long b : cand) {
+                    // every candidate runs the same triplets (work per triplet varies with q1);
+                    // best of two timings
+                    pp.ntrip = bmax;
+                    pp.batch = b;
+                    float msb = 1e30f;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        TRY(hipEventRecord(p0, st));
+                        int r = run_mfma_chain(pp, precision, st);
+                        if (r) { hipEventDestroy(p0); hipEventDestroy(p1); cleanup(); return r; }
+                        TRY(hipEventRecord(p1, st));
+                        TRY(hipEventSynchronize(p1));
+                        float ms = 0;
+                        hipEventElapsedTime(&ms, p0, p1);
+                        msb = std::min(msb, ms);
+                    }
+                    c->tune_log.push_back({key, (double)b, (double)msb});
+                    if (msb < best) { best = msb; mbatch = b; }
                 }
                 hipEventDestroy(p0);
                 hipEventDestroy(p1);

@@ -228,6 +228,15 @@ class Engine:
         return dict(harmonic=t[0], scatter=t[1], kappa=t[2], bte=t[3], offdiag=t[4], scatter_kernels=t[5],
                     adjoint=t[6], adjoint_kernels=t[7])
 
+    def tuning(self):
+        """Batch autotuner probes so far: list of (triplets per launch, best time ms)."""
+        n = ctypes.c_int()
+        self._check(self.lib.tc3_get_tuning(self.ctx, 0, None, None, ctypes.byref(n)))
+        b = np.zeros(max(1, n.value))
+        ms = np.zeros(max(1, n.value))
+        self._check(self.lib.tc3_get_tuning(self.ctx, n.value, dptr(b), dptr(ms), ctypes.byref(n)))
+        return [(int(b[i]), float(ms[i])) for i in range(n.value)]
+
     def write_grid_hdf5(self, path, iT=0):
         self._check(self.lib.tc3_write_grid_hdf5(self.ctx, iT, path.encode()))
 
