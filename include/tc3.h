@@ -151,8 +151,10 @@ int tc3_get_timings(const tc3_ctx *ctx, double *t, int n);
 
 /* Device time (ms, HIP events on the context's stream) and launch count of one kernel
  * class in the last tc3_scatter / tc3_kappa call: 0 Phi + first two contractions,
- * 1 last contraction + smeared-delta epilogue, 2 scattering-matrix symmetrisation,
- * 3 BTE scattering-matrix product.  Also the triplet batch of the last scatter call.
+ * 1 last contraction + smeared-delta epilogue, 2 scattering-matrix assembly and
+ * symmetrisation, 3 BTE scattering-matrix product, 4 the same as 2 when the rows were
+ * assembled from compact per-triplet sums (no launch counted when the contraction kernels
+ * stored the entries directly).  Also the triplet batch of the last scatter call.
  * No counterpart in the reference (its timers are host wall-clock buckets). */
 int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches,
                          int64_t *batch);

@@ -497,7 +497,7 @@ int tc3_get_timings(const tc3_ctx *ctx, double *t, int n) {
 }
 
 int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches, int64_t *batch) {
-    if (!ctx || which < 0 || which > 3) return 5;
+    if (!ctx || which < 0 || which > 4) return 5;
     if (total_ms) *total_ms = ctx->kstat_ms[which];
     if (launches) *launches = ctx->kstat_n[which];
     if (batch) *batch = ctx->batch;

@@ -70,8 +70,8 @@ struct tc3_ctx {
     double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // per-kernel-class device time of the last calls (HIP events): 0 Phi+U+W kernel,
     // 1 Psi + smeared-delta epilogue kernel, 2 scattering-matrix symmetrisation, 3 BTE Xi.F
-    double kstat_ms[4] = {0, 0, 0, 0};
-    int kstat_n[4] = {0, 0, 0, 0};
+    double kstat_ms[5] = {0, 0, 0, 0, 0};
+    int kstat_n[5] = {0, 0, 0, 0, 0};
     long batch = 0;
 
     // device buffers
@@ -95,6 +95,10 @@ struct tc3_ctx {
     long adjoint_tables_gen = -1;
     // autotuned triplets per launch of the MFMA contraction, by shape, and the probes timed
     std::map<std::string, long> tuned_batch;
+    // owner of every scattering-matrix entry of the rows of each irreducible q1 (k_xi_asm)
+    std::vector<int> xi_src;
+    bool xi_src_ok = false;
+    long xi_src_gen = -1;
     struct TuneProbe { std::string key; double batch, ms; };
     std::vector<TuneProbe> tune_log;
     // instantiated BTE iteration graph and the arguments it was captured with

@@ -166,12 +166,23 @@ struct UwArgs {
 
 template <typename T> struct C2 { T x, y; };
 
+// U on MFMA (dense Phi_i . E3 in split BF16) for nb = 48 in the BF16X3 mode; on the
+// vector ALUs over the nonzero IFC blocks otherwise.
+template <int PREC, int NB> struct UwMfmaU { static constexpr bool on = (PREC == 2 && NB == 48); };
+
+// LDS of k_uw: the U region (BF16X3: split planes [4][NB][RS]; else planar [2][NB][NB]),
+// which also holds the staged IFC entries of a1 and, with U on MFMA, Phi_i; the blocks of
+// a1 (int4); the Phi blocks of a1 [block][i][j][k]; the phases of the entries of a1.
+template <int PREC, int NB>
+__host__ __device__ constexpr size_t uw_region() {
+    typedef typename Traits<PREC>::T T;
+    constexpr int RS = kpad(NB) + 8;
+    return (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB * NB;
+}
 template <int PREC, int NB>
 size_t uw_lds(int maxblk, int maxa1) {
     typedef typename Traits<PREC>::T T;
-    constexpr int RS = kpad(NB) + 8;
-    size_t s = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB * NB;   // U
-    s += sizeof(C2<T>) * 9 * (size_t)maxblk + sizeof(int) * (size_t)maxblk;
+    size_t s = uw_region<PREC, NB>() + sizeof(int4) * (size_t)maxblk + sizeof(C2<T>) * 27 * (size_t)maxblk;
     s = (s + 15) & ~(size_t)15;
     return s + sizeof(cd) * (size_t)std::max(maxa1, 1);
 }
@@ -192,37 +203,64 @@ __device__ inline T bload_t(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     else return bload(r, voff, soff);
 }
 
-// Workgroup per (triplet, atom a1), one wave per 16 values of b2.  For i = 0, 1, 2
-// (ia = 3 a1 + i) the workgroup forms U[ib][b3] = sum_(a3,k) Phi[ia][ib][3 a3 + k]
-// E3[3 a3 + k][b3] from the nonzero IFC blocks of a1 on the vector ALUs (four b3 per
-// thread), then each wave forms W[b2][b3] = sum_ib E2[ib][b2] U[ib][b3] for its b2 on
-// MFMA with the E2 fragments held in registers for the whole workgroup.
+// -x for a split operand: both halves change sign.
+__device__ inline v8bf negbf(v8bf x) {
+    v4u u = __builtin_bit_cast(v4u, x);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(v8bf, u);
+}
+
+// Four FP32 values split into bf16 hi and lo (as split8), packed two per word.
+__device__ inline void split4(const v4f x, uint2 &h, uint2 &l) {
+    unsigned hu[2], lu[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const v2f v = {x[2 * p], x[2 * p + 1]};
+        const unsigned u = __builtin_bit_cast(unsigned, __builtin_convertvector(v, v2bf));
+        const v2f hf = {__builtin_bit_cast(float, u << 16), __builtin_bit_cast(float, u & 0xffff0000u)};
+        hu[p] = u;
+        lu[p] = __builtin_bit_cast(unsigned, __builtin_convertvector(v - hf, v2bf));
+    }
+    h = make_uint2(hu[0], hu[1]);
+    l = make_uint2(lu[0], lu[1]);
+}
+
+// Workgroup per (triplet, atom a1), one wave per 16 values of b2.
+//   Phi blocks: the IFC entries of a1 and their phases exp(-2 pi i (q2.R2 + q3.R3)) are
+//   staged in LDS, and every nonzero block (a2, a3) of a1 is summed once for all 27 (i, j,
+//   k) (pretransform_phi3, scattering_threephonon.f90:275-317).
+//   For i = 0, 1, 2 (ia = 3 a1 + i):
+//     U[ib][b3] = sum_ic Phi[ia][ib][ic] E3[ic][b3]: BF16X3 with nb = 48 on MFMA, Phi_i
+//     written densely into LDS (zero outside the blocks), each wave one tile of 16 b3 with
+//     its E3 fragments in registers; otherwise on the vector ALUs over the nonzero blocks
+//     (four b3 per thread);
+//     W[b2][b3] = sum_ib E2[ib][b2] U[ib][b3] on MFMA, each wave its 16 b2 with the E2
+//     fragments in registers for the whole workgroup; W[t][ia][b2][b3] is stored.
 template <int PREC, int NB>
 __global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
     constexpr int NA = NB / 3, NB2 = NB * NB, NT = NB / 16;
     constexpr int KP = kpad(NB), RS = KP + 8, KS = KSteps<PREC, NB>::n;
+    constexpr bool MU = UwMfmaU<PREC, NB>::on;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tl = blockIdx.x / NA, a1 = blockIdx.x % NA;
     const int t = u.t0 + tl;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
-    // LDS: U (BF16X3: split planes re hi/lo, im hi/lo, [b3][ib] with row stride RS; else
-    // planar [re/im][ib][b3]), Phi blocks of one i [block][j][k], a3 of each block, phases
-    constexpr size_t ub = (PREC == 2) ? sizeof(__bf16) * 4 * NB * RS : sizeof(T) * 2 * NB2;
+    constexpr size_t ub = uw_region<PREC, NB>();
     __bf16 *ut = (__bf16 *)smem;
     T *us = (T *)smem;
-    C2<T> *pb = (C2<T> *)(smem + ub);
-    int *pa3 = (int *)(pb + 9 * u.maxblk);
-    cd *ph = (cd *)(smem + align16((size_t)((unsigned char *)(pa3 + u.maxblk) - smem)));
+    T *mst = (T *)smem;                    // staged IFC entries of a1 (before the i loop)
+    int4 *pbl = (int4 *)(smem + ub);
+    C2<T> *pb = (C2<T> *)(pbl + u.maxblk);
+    cd *ph = (cd *)(smem + align16((size_t)((unsigned char *)(pb + 27 * u.maxblk) - smem)));
     const int q2 = u.tq2[t], q3 = u.tq3[t];
-    if constexpr (PREC == 2) {
-        // zero U including the K padding read by the MFMA fragments
-        uint4 *z = (uint4 *)ut;
-        for (int x = threadIdx.x; x < 4 * NB * RS / 8; x += blockDim.x) z[x] = make_uint4(0, 0, 0, 0);
-    }
     const double *Q2 = u.qcart + 3 * q2, *Q3 = u.qcart + 3 * q3;
-    const int f0 = u.a1_off[a1], f1 = u.a1_off[a1 + 1];
+    const int f0 = u.a1_off[a1], f1 = u.a1_off[a1 + 1], nf = f1 - f0;
+    const int bo0 = u.a1_boff[a1], nblk = u.a1_boff[a1 + 1] - bo0;
+    const bool staged = (size_t)27 * nf * sizeof(T) <= ub;
+    if (staged)
+        for (int x = threadIdx.x; x < 27 * nf; x += blockDim.x) mst[x] = (T)u.mwm[27 * (size_t)f0 + x];
     for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
         const double arg = -kTwoPi * (Q2[0] * u.lv2[3 * f] + Q2[1] * u.lv2[3 * f + 1] + Q2[2] * u.lv2[3 * f + 2] +
                                       Q3[0] * u.lv3[3 * f] + Q3[1] * u.lv3[3 * f + 1] + Q3[2] * u.lv3[3 * f + 2]);
@@ -230,9 +268,11 @@ __global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
         sincos(arg, &s, &c);
         ph[f - f0] = {c, s};
     }
+    for (int bl = threadIdx.x; bl < nblk; bl += blockDim.x) pbl[bl] = u.blist[bo0 + bl];
     // E2 operand (A[m = b2][k = ib] = E2[ib][b2]) of this wave's b2 tile, for all i
     const int m0 = 16 * w;
     Split ea_r[PREC == 2 ? KS : 1], ea_i[PREC == 2 ? KS : 1], ea_n[PREC == 2 ? KS : 1];
+    Split eb_r[MU ? KS : 1], eb_i[MU ? KS : 1];
     T fa_r[PREC == 2 ? 1 : KS], fa_i[PREC == 2 ? 1 : KS];
     if constexpr (PREC == 2) {
         const size_t ps = (size_t)NB * KP;
@@ -246,6 +286,17 @@ __global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
             ea_n[ks].h = ld8(pa + 4 * ps + 32 * ks);
             ea_n[ks].l = ld8(pa + 5 * ps + 32 * ks);
         }
+        if constexpr (MU) {
+            // E3 operand (B[k = ic][n = b3] = E3[ic][b3]) of this wave's b3 tile 16 w
+            const __bf16 *pe = u.Et + (size_t)q3 * kPlanes * ps + (size_t)(m0 + c16) * KP + 8 * g;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                eb_r[ks].h = ld8(pe + 32 * ks);
+                eb_r[ks].l = ld8(pe + ps + 32 * ks);
+                eb_i[ks].h = ld8(pe + 2 * ps + 32 * ks);
+                eb_i[ks].l = ld8(pe + 3 * ps + 32 * ks);
+            }
+        }
     } else {
         const T *E2r = (const T *)u.Ef_re + (size_t)q2 * NB2 + m0 + c16;
         const T *E2i = (const T *)u.Ef_im + (size_t)q2 * NB2 + m0 + c16;
@@ -256,55 +307,120 @@ __global__ void __launch_bounds__(4 * NB) k_uw(UwArgs u) {
         }
     }
     __syncthreads();
-    const int bo0 = u.a1_boff[a1], nblk = u.a1_boff[a1 + 1] - bo0;
-    for (int bl = threadIdx.x; bl < nblk; bl += blockDim.x) pa3[bl] = u.blist[bo0 + bl].y;
+    // Phi blocks of a1, all (i, j, k): pb[block][9 i + 3 j + k]
+    for (int it = threadIdx.x; it < 27 * nblk; it += blockDim.x) {
+        const int bl = it / 27, ijk = it % 27;
+        const int4 b = pbl[bl];
+        double ax = 0, ay = 0;
+        for (int f = b.z; f < b.w; ++f) {
+            const double m = staged ? (double)mst[27 * (f - f0) + ijk] : u.mwm[27 * (size_t)f + ijk];
+            const cd p = ph[f - f0];
+            ax += m * p.x;
+            ay += m * p.y;
+        }
+        pb[it] = {(T)ax, (T)ay};
+    }
+    __syncthreads();
+    if constexpr (PREC == 2) {
+        // zero the region including the K padding read by the MFMA fragments (the staged
+        // entries are no longer needed)
+        uint4 *z = (uint4 *)ut;
+        for (int x = threadIdx.x; x < 4 * NB * RS / 8; x += blockDim.x) z[x] = make_uint4(0, 0, 0, 0);
+        __syncthreads();
+    }
     const T *E3r = (const T *)u.Ef_re + (size_t)q3 * NB2, *E3i = (const T *)u.Ef_im + (size_t)q3 * NB2;
     T *Wt = (T *)u.W + (size_t)tl * 2 * NB * NB2;
     for (int i = 0; i < 3; ++i) {
-        // Phi blocks of (a1, i): [block][j][k]
-        for (int it = threadIdx.x; it < 9 * nblk; it += blockDim.x) {
-            const int bl = it / 9, jk = it % 9;
-            const int4 b = u.blist[bo0 + bl];
-            double ax = 0, ay = 0;
-            for (int f = b.z; f < b.w; ++f) {
-                const double m = u.mwm[27 * f + 9 * i + jk];
-                const cd p = ph[f - f0];
-                ax += m * p.x;
-                ay += m * p.y;
+        if constexpr (MU) {
+            // Phi_i[ib][ic] in split planes (re hi, re lo, im hi, im lo), zero outside the blocks
+            if (i > 0) {
+                for (int x = threadIdx.x; x < 4 * NB * (NB / 8); x += blockDim.x) {
+                    const int row = x / (NB / 8), c8 = x % (NB / 8);
+                    *(uint4 *)(ut + (size_t)row * RS + 8 * c8) = make_uint4(0, 0, 0, 0);
+                }
+                __syncthreads();
             }
-            pb[it] = {(T)ax, (T)ay};
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < NB2 / 4; e += blockDim.x) {
-            const int ib = e / (NB / 4), b3 = 4 * (e % (NB / 4)), a2 = ib / 3, j = ib % 3;
-            const int s0 = u.a12_boff[a1 * NA + a2] - bo0, s1 = u.a12_boff[a1 * NA + a2 + 1] - bo0;
-            T ur[4] = {0, 0, 0, 0}, ui[4] = {0, 0, 0, 0};
-            for (int bl = s0; bl < s1; ++bl) {
-                const int a3 = pa3[bl];
-                const C2<T> *pp = pb + bl * 9 + 3 * j;
+            for (int it = threadIdx.x; it < 9 * nblk; it += blockDim.x) {
+                const int bl = it / 9, jk = it % 9;
+                const int4 b = pbl[bl];
+                const C2<T> v = pb[27 * bl + 9 * i + jk];
+                const int ib = 3 * b.x + jk / 3, ic = 3 * b.y + jk % 3;
+                const __bf16 rh = (__bf16)v.x, ih = (__bf16)v.y;
+                __bf16 *o = ut + (size_t)ib * RS + ic;
+                o[0] = rh;
+                o[(size_t)NB * RS] = (__bf16)(v.x - (float)rh);
+                o[(size_t)2 * NB * RS] = ih;
+                o[(size_t)3 * NB * RS] = (__bf16)(v.y - (float)ih);
+            }
+            __syncthreads();
+            // U[ib][b3] for this wave's b3 tile, every ib tile
+            v4f ur[NT], ui[NT];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const C2<T> p = pp[k];
-                    const T *er = E3r + (3 * a3 + k) * NB + b3, *ei = E3i + (3 * a3 + k) * NB + b3;
+            for (int mt = 0; mt < NT; ++mt) {
+                ur[mt] = {0, 0, 0, 0};
+                ui[mt] = {0, 0, 0, 0};
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        ur[v] += p.x * er[v] - p.y * ei[v];
-                        ui[v] += p.x * ei[v] + p.y * er[v];
-                    }
+                for (int ks = 0; ks < KS; ++ks) {
+                    const __bf16 *pa = ut + (size_t)(16 * mt + c16) * RS + 32 * ks + 8 * g;
+                    Split ar, ai, an;
+                    ar.h = ld8(pa);
+                    ar.l = ld8(pa + (size_t)NB * RS);
+                    ai.h = ld8(pa + (size_t)2 * NB * RS);
+                    ai.l = ld8(pa + (size_t)3 * NB * RS);
+                    an.h = negbf(ai.h);
+                    an.l = negbf(ai.l);
+                    mma3(ur[mt], ar, eb_r[ks]);
+                    mma3(ur[mt], an, eb_i[ks]);
+                    mma3(ui[mt], ar, eb_i[ks]);
+                    mma3(ui[mt], ai, eb_r[ks]);
                 }
             }
+            __syncthreads();
+            // U into the same region as the W operand: [plane][b3][ib]; this lane holds
+            // ib = 16 mt + 4 g + r, b3 = 16 w + c16
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                if constexpr (PREC == 2) {
-                    const __bf16 rh = (__bf16)ur[v], ih = (__bf16)ui[v];
-                    __bf16 *o = ut + (size_t)(b3 + v) * RS + ib;
-                    o[0] = rh;
-                    o[(size_t)NB * RS] = (__bf16)(ur[v] - (float)rh);
-                    o[(size_t)2 * NB * RS] = ih;
-                    o[(size_t)3 * NB * RS] = (__bf16)(ui[v] - (float)ih);
-                } else {
-                    us[ib * NB + b3 + v] = ur[v];
-                    us[NB2 + ib * NB + b3 + v] = ui[v];
+            for (int mt = 0; mt < NT; ++mt) {
+                uint2 rh, rl, ih, il;
+                split4(ur[mt], rh, rl);
+                split4(ui[mt], ih, il);
+                __bf16 *o = ut + (size_t)(m0 + c16) * RS + 16 * mt + 4 * g;
+                *(uint2 *)o = rh;
+                *(uint2 *)(o + (size_t)NB * RS) = rl;
+                *(uint2 *)(o + (size_t)2 * NB * RS) = ih;
+                *(uint2 *)(o + (size_t)3 * NB * RS) = il;
+            }
+        } else {
+            for (int e = threadIdx.x; e < NB2 / 4; e += blockDim.x) {
+                const int ib = e / (NB / 4), b3 = 4 * (e % (NB / 4)), a2 = ib / 3, j = ib % 3;
+                const int s0 = u.a12_boff[a1 * NA + a2] - bo0, s1 = u.a12_boff[a1 * NA + a2 + 1] - bo0;
+                T ur[4] = {0, 0, 0, 0}, ui[4] = {0, 0, 0, 0};
+                for (int bl = s0; bl < s1; ++bl) {
+                    const int a3 = pbl[bl].y;
+                    const C2<T> *pp = pb + 27 * bl + 9 * i + 3 * j;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const C2<T> p = pp[k];
+                        const T *er = E3r + (3 * a3 + k) * NB + b3, *ei = E3i + (3 * a3 + k) * NB + b3;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            ur[v] += p.x * er[v] - p.y * ei[v];
+                            ui[v] += p.x * ei[v] + p.y * er[v];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if constexpr (PREC == 2) {
+                        const __bf16 rh = (__bf16)ur[v], ih = (__bf16)ui[v];
+                        __bf16 *o = ut + (size_t)(b3 + v) * RS + ib;
+                        o[0] = rh;
+                        o[(size_t)NB * RS] = (__bf16)(ur[v] - (float)rh);
+                        o[(size_t)2 * NB * RS] = ih;
+                        o[(size_t)3 * NB * RS] = (__bf16)(ui[v] - (float)ih);
+                    } else {
+                        us[ib * NB + b3 + v] = ur[v];
+                        us[NB2 + ib * NB + b3 + v] = ui[v];
+                    }
                 }
             }
         }
@@ -708,6 +824,12 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                 gam[T_ * 16 + m] += gs;
                 const int r = rows[m];
                 if (r < 0) continue;
+                if (a.comp) {
+                    S *cp = (S *)a.comp + (size_t)T_ * a.comp_T + ((size_t)t * NB + 16 * bb + m) * 2 * NB + cb;
+#pragma unroll
+                    for (int c = 0; c < kChunk; ++c) cp[c] = v[c];
+                    continue;
+                }
                 const double iw1 = 2.0 / w1s[m];
                 for (int e = e0; e < e1; ++e) {
                     XT *xr = xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)a.eq_q2[e] * NB + cb;
@@ -725,7 +847,18 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     }
     // q3' entries (:134) from this lane's registers; where q3' = q2' the entry already
     // holds the q2' term written above by this workgroup, and the q3' term is added.
-    if (row1 >= 0) {
+    if (row1 >= 0 && a.comp) {
+        S *cp = (S *)a.comp + ((size_t)t * NB + 16 * bb + c16) * 2 * NB + NB + b3w;
+#pragma unroll
+        for (int T_ = 0; T_ < NTY; ++T_) {
+            if (T_ >= nT) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int b3 = b3w + out_row<PREC>(lane, r);
+                cp[(size_t)T_ * a.comp_T + out_row<PREC>(lane, r)] = y[T_][r] + n3h[T_ * NB + b3] * yM[r];
+            }
+        }
+    } else if (row1 >= 0) {
         const double iw1 = 2.0 / w1;
         for (int e = e0; e < e1; ++e) {
             const int x2 = a.eq_q2[e], x3q = a.eq_q3[e];
@@ -827,7 +960,7 @@ size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     const size_t kp = kpad(nb), nw = nb / 16, nmw = (nb + 31) / 32;
     const size_t ks = (precision == TC3_PREC_BF16X3) ? (nb + 31) / 32 : nb / 4;
     size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * (kp + 8) : es * 2 * nb * nb;
-    uw += 2 * es * 9 * (size_t)maxblk + 4 * (size_t)maxblk;
+    uw += 2 * es * 27 * (size_t)maxblk + 16 * (size_t)maxblk;
     uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
     size_t ps = es * (2 * nty * nb + 2 * nb + nw * kChunk * 16 * nty) + 4 * nb;
     ps = ((ps + 15) & ~(size_t)15);

@@ -16,6 +16,12 @@ struct MfmaEpiArgs {
     void *xi;                // first temperature of the chunk; FP64 entries in FP64 mode, else FP32
     double *gpart;
     size_t xi_row, xi_T;
+    // Compact per-triplet sums instead of scattering-matrix entries, when not null:
+    // comp[T][t][b1][0][b2] = sum_b3 f_T (the q2' entries before their frequency factor) and
+    // comp[T][t][b1][1][b3] = sum_b2 f_T (the q3' entries), in the epilogue type; first
+    // temperature of the chunk, comp_T elements per temperature.
+    void *comp;
+    size_t comp_T;
 };
 
 struct MfmaPlan {

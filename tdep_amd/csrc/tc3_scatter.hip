@@ -435,6 +435,228 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
     }
 }
 
+// The same orbit averaging without isotopes, for nb <= 256, over chunks of whole orbits:
+// the members of a chunk (orbits [chk[c], chk[c + 1]) of q1) and their entries are staged
+// in LDS with independent loads, every (orbit, band) is averaged there in the member order
+// of k_xi_sym, and the chunk is written back.  Each thread keeps one band; the workgroup
+// has nb * (256 / nb) threads.  Workgroup per (row, T).
+#ifndef TC3_XI_DEPTH
+#define TC3_XI_DEPTH 16
+#endif
+constexpr int kXiDepth = TC3_XI_DEPTH;
+
+template <typename XT>
+__global__ void __launch_bounds__(kBlock) k_xi_sym_lds(int nrows, int nb, const int *__restrict__ row_q,
+                                                       const int *__restrict__ chk_off, const int *__restrict__ chk,
+                                                       const int *__restrict__ orb_off,
+                                                       const int *__restrict__ members, XT *__restrict__ xi,
+                                                       size_t xi_row, size_t xi_T, int capm) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT *val = (XT *)smem;
+    int *mem = (int *)(val + (size_t)capm * nb);
+    int *ooff = mem + capm;                  // member offsets of the chunk's orbits
+    const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
+    const int q1 = row_q[r];
+    XT *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
+    const int b = threadIdx.x % nb, p0 = threadIdx.x / nb, ps = blockDim.x / nb;
+    for (int c = chk_off[q1]; c + 1 < chk_off[q1 + 1]; ++c) {
+        const int oa = chk[c], ob = chk[c + 1];
+        const int pa = orb_off[oa], n = orb_off[ob] - pa;
+        for (int p = threadIdx.x; p < n; p += blockDim.x) mem[p] = members[pa + p];
+        for (int o = threadIdx.x; o <= ob - oa; o += blockDim.x) ooff[o] = orb_off[oa + o] - pa;
+        __syncthreads();
+        // kXiDepth independent loads in flight per thread before their LDS stores
+        for (int p = p0; p < n; p += kXiDepth * ps) {
+            XT v[kXiDepth];
+#pragma unroll
+            for (int k = 0; k < kXiDepth; ++k) {
+                const int pk = p + k * ps;
+                v[k] = (pk < n) ? x[(size_t)mem[pk] * nb + b] : (XT)0;
+            }
+#pragma unroll
+            for (int k = 0; k < kXiDepth; ++k) {
+                const int pk = p + k * ps;
+                if (pk < n) val[(size_t)pk * nb + b] = v[k];
+            }
+        }
+        __syncthreads();
+        for (int o = oa + p0; o < ob; o += ps) {
+            const int m0 = ooff[o - oa], m1 = ooff[o - oa + 1];
+            double sp = 0, sa = 0;
+            int cnt = 0;
+            for (int m = m0; m < m1; ++m) {
+                const double v = (double)val[(size_t)m * nb + b];
+                sa += v;
+                if (v > 0.0) { sp += v; ++cnt; }
+            }
+            double sum = sa;
+            if (cnt > 0) {
+                const double avg = sp / cnt;
+                sum = 0;
+                for (int m = m0; m < m1; ++m) sum += avg;
+            }
+            const XT out = (XT)(sum / (m1 - m0));
+            for (int m = m0; m < m1; ++m) val[(size_t)m * nb + b] = out;
+        }
+        __syncthreads();
+        for (int p = p0; p < n; p += kXiDepth * ps) {
+#pragma unroll
+            for (int k = 0; k < kXiDepth; ++k) {
+                const int pk = p + k * ps;
+                if (pk < n) x[(size_t)mem[pk] * nb + b] = val[(size_t)pk * nb + b];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+#ifndef TC3_ASM_DEPTH
+#define TC3_ASM_DEPTH 8
+#endif
+constexpr int kAsmDepth = TC3_ASM_DEPTH;
+
+// Scattering-matrix rows from the compact per-triplet sums of the MFMA kernels, then (AVG)
+// the orbit averaging of k_xi_sym_lds.  msrc, in the member order of the orbits, holds for
+// every q' the owner 4 t + mode (the triplet t of q1 whose equivalent pairs own the entries
+// (q', b) of the rows of q1: mode 1 the q2' sums, mode 2 the q3' sums, mode 3 both, q2' =
+// q3' within one pair; 0 none, entry zero) and the frequency offsets nb q of the owning
+// role and of q3': (q', owner, offset, offset q3').  The
+// entries are formed with the arithmetic of the direct stores of k_psi (frequency of the
+// mode times 2 / omega_1, rounding to the entry type, the q3' term added to the rounded
+// q2' term), so the rows are the same bits.  Workgroup per (row, T), one band per thread.
+template <typename XT, typename S, bool AVG>
+__global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int *__restrict__ row_q,
+                                                   const int *__restrict__ row_b, const double *__restrict__ om_irr,
+                                                   const double *__restrict__ om_full, const int4 *__restrict__ msrc,
+                                                   const S *__restrict__ comp, size_t comp_T,
+                                                   const int *__restrict__ chk_off, const int *__restrict__ chk,
+                                                   const int *__restrict__ orb_off, XT *__restrict__ xi,
+                                                   size_t xi_row, size_t xi_T, int capm) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT *val = (XT *)smem;
+    int *mem = (int *)(val + (size_t)capm * nb);
+    int *ooff = mem + capm;
+    int *srcs = ooff + capm + 1;
+    int *w2o = srcs + capm, *w3o = w2o + capm;
+    const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
+    const int q1 = row_q[r], b1 = row_b[r];
+    const double iw1 = 2.0 / om_irr[(size_t)q1 * nb + b1];
+    XT *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
+    const S *cT = comp + (size_t)T * comp_T;
+    const int b = threadIdx.x % nb, p0 = threadIdx.x / nb, ps = blockDim.x / nb;
+    for (int c = chk_off[q1]; c + 1 < chk_off[q1 + 1]; ++c) {
+        const int oa = chk[c], ob = chk[c + 1];
+        const int pa = orb_off[oa], n = orb_off[ob] - pa;
+        for (int p = threadIdx.x; p < n; p += blockDim.x) {
+            const int4 m = msrc[pa + p];
+            mem[p] = m.x;
+            srcs[p] = m.y;
+            w2o[p] = m.z;
+            w3o[p] = m.w;
+        }
+        for (int o = threadIdx.x; o <= ob - oa; o += blockDim.x) ooff[o] = orb_off[oa + o] - pa;
+        __syncthreads();
+        // loads are unconditional (an entry without owner reads triplet 0 and is masked) so
+        // that kAsmDepth of them are in flight per thread
+        for (int p = p0; p < n; p += kAsmDepth * ps) {
+            S v[kAsmDepth];
+            double f[kAsmDepth];
+#pragma unroll
+            for (int k = 0; k < kAsmDepth; ++k) {
+                const int pk = min(p + k * ps, n - 1);
+                const int sm = srcs[pk];
+                v[k] = cT[((size_t)(sm >> 2) * nb + b1) * 2 * nb + ((sm & 3) == 2 ? nb : 0) + b];
+                f[k] = om_full[(size_t)w2o[pk] + b];
+            }
+#pragma unroll
+            for (int k = 0; k < kAsmDepth; ++k) {
+                const int pk = p + k * ps;
+                if (pk >= n) continue;
+                const int sm = srcs[pk];
+                XT e = (sm & 3) ? (XT)((double)v[k] * f[k] * iw1) : (XT)0;
+                if ((sm & 3) == 3) {
+                    // q2' = q3' within one pair: the q3' term added to the rounded q2' term
+                    const S v3 = cT[((size_t)(sm >> 2) * nb + b1) * 2 * nb + nb + b];
+                    e = (XT)((double)e + (double)v3 * om_full[(size_t)w3o[pk] + b] * iw1);
+                }
+                val[(size_t)pk * nb + b] = e;
+            }
+        }
+        __syncthreads();
+        if constexpr (AVG) {
+            for (int o = oa + p0; o < ob; o += ps) {
+                const int m0 = ooff[o - oa], m1 = ooff[o - oa + 1];
+                double sp = 0, sa = 0;
+                int cnt = 0;
+                for (int m = m0; m < m1; ++m) {
+                    const double v = (double)val[(size_t)m * nb + b];
+                    sa += v;
+                    if (v > 0.0) { sp += v; ++cnt; }
+                }
+                double sum = sa;
+                if (cnt > 0) {
+                    const double avg = sp / cnt;
+                    sum = 0;
+                    for (int m = m0; m < m1; ++m) sum += avg;
+                }
+                const XT out = (XT)(sum / (m1 - m0));
+                for (int m = m0; m < m1; ++m) val[(size_t)m * nb + b] = out;
+            }
+            __syncthreads();
+        }
+        for (int p = p0; p < n; p += kAsmDepth * ps) {
+#pragma unroll
+            for (int k = 0; k < kAsmDepth; ++k) {
+                const int pk = p + k * ps;
+                if (pk < n) x[(size_t)mem[pk] * nb + b] = val[(size_t)pk * nb + b];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Owner table of k_xi_asm in the member order of the orbits of every irreducible q1, from
+// the triplets' equivalent pairs in the order of the direct stores of k_psi (all q2'
+// entries of a triplet, then its q3' entries: a store, or an addition where q2' = q3' in the
+// pair): four ints per member position (see k_xi_asm).  False if an entry would be owned by
+// two triplets or take a form the table cannot express.
+bool xi_source_map(const tc3::TripletSet &tr, const tc3::OrbitSet &orb, int nirr, int nfull, int nb,
+                   std::vector<int> &msrc) {
+    std::vector<int> s(nfull);
+    msrc.assign(4 * orb.members.size(), 0);
+    for (int q1 = 0; q1 < nirr; ++q1) {
+        std::fill(s.begin(), s.end(), 0);
+        for (int t = tr.q1_off[q1]; t < tr.q1_off[q1 + 1]; ++t) {
+            for (int e = tr.eq_off[t]; e < tr.eq_off[t + 1]; ++e) {
+                int &v = s[tr.eq_q2[e]];
+                if ((v & 3) && (v >> 2) != t) return false;
+                v = 4 * t + 1;
+            }
+            for (int e = tr.eq_off[t]; e < tr.eq_off[t + 1]; ++e) {
+                const int q3 = tr.eq_q3[e];
+                int &v = s[q3];
+                if ((v & 3) && (v >> 2) != t) return false;
+                if (tr.eq_q2[e] == q3) {
+                    if ((v & 3) != 1) return false;
+                    v = 4 * t + 3;
+                } else {
+                    v = 4 * t + 2;
+                }
+            }
+        }
+        const int pa = orb.orb_off[orb.q1_off[q1]], pb = orb.orb_off[orb.q1_off[q1 + 1]];
+        for (int p = pa; p < pb; ++p) {
+            const int q = orb.members[p], sm = s[q], t = sm >> 2;
+            int *m = msrc.data() + 4 * (size_t)p;
+            m[0] = q;
+            m[1] = sm;
+            m[2] = ((sm & 3) == 2 ? tr.q3[t] : tr.q2[t]) * nb;
+            m[3] = tr.q3[t] * nb;
+        }
+    }
+    return true;
+}
+
 }  // namespace
 
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
@@ -477,6 +699,28 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(table("orboff", (void **)&d_orboff, c->orb.orb_off.data(), sizeof(int) * c->orb.orb_off.size()));
     TRY(table("orbq1", (void **)&d_orbq1, c->orb.q1_off.data(), sizeof(int) * c->orb.q1_off.size()));
     TRY(table("members", (void **)&d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size()));
+    // chunks of whole orbits of every q1 for k_xi_sym_lds: at most capm members (8192
+    // entries, or the largest orbit)
+    int capm = 0;
+    std::vector<int> chk_off(nirr + 1, 0), chk;
+    {
+        int maxorb = 1;
+        for (size_t o = 0; o + 1 < c->orb.orb_off.size(); ++o)
+            maxorb = std::max(maxorb, c->orb.orb_off[o + 1] - c->orb.orb_off[o]);
+        capm = std::max(maxorb, 8192 / std::max(nb, 1));
+        for (int q = 0; q < nirr; ++q) {
+            chk_off[q] = (int)chk.size();
+            const int oa = c->orb.q1_off[q], ob = c->orb.q1_off[q + 1];
+            chk.push_back(oa);
+            for (int o = oa; o < ob; ++o)
+                if (c->orb.orb_off[o + 1] - c->orb.orb_off[chk.back()] > capm) chk.push_back(o);
+            if (chk.back() != ob) chk.push_back(ob);
+        }
+        chk_off[nirr] = (int)chk.size();
+    }
+    int *d_chkoff, *d_chk;
+    TRY(table("xchkoff", (void **)&d_chkoff, chk_off.data(), sizeof(int) * chk_off.size()));
+    TRY(table("xchk", (void **)&d_chk, chk.data(), sizeof(int) * chk.size()));
     TRY(table("a1off", (void **)&d_a1off, c->a1_off.data(), sizeof(int) * (na + 1)));
     TRY(table("blk3", (void **)&d_blk3, c->blk3_off.data(), sizeof(int) * ((size_t)na * na * na + 1)));
     TRY(table("qcart", (void **)&d_qcart, c->mesh.qcart.data(), sizeof(double) * 3 * nfull));
@@ -563,6 +807,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     ea.xi = c->d_xi; ea.gpart = d_gpart; ea.xi_row = c->xi_row; ea.xi_T = xi_T; ea.nirr = nirr;
     const size_t epi_shmem = sizeof(double) * ((size_t)nb * nb + 2 * nb);
 
+    int *d_xisrc = nullptr;
+    void *d_comp = nullptr;
     hipEvent_t ev0, ev1;
     TRY(hipEventCreate(&ev0));
     TRY(hipEventCreate(&ev1));
@@ -628,6 +874,19 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
         me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
         me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
+        // compact per-triplet sums, assembled into the rows by k_xi_asm, when every entry
+        // has one owner (TC3_XI_DIRECT: direct stores)
+        me.comp = nullptr;
+        me.comp_T = (size_t)ntrip * nb * 2 * nb;
+        if (c->xi_src_gen != c->gen_model) {
+            c->xi_src_ok = xi_source_map(c->trip, c->orb, nirr, nfull, nb, c->xi_src);
+            c->xi_src_gen = c->gen_model;
+        }
+        if (c->xi_src_ok && std::getenv("TC3_XI_DIRECT") == nullptr) {
+            TRY(table("xisrc", (void **)&d_xisrc, c->xi_src.data(), sizeof(int4) * (c->xi_src.size() / 4)));
+            TRY(c->pool_get("xi_comp", (size_t)c->xi_es * nT * me.comp_T, &me.comp));
+            d_comp = me.comp;
+        }
         // Autotuner: on the first pass of a shape (band count, precision, temperature block,
         // triplet count), time each candidate batch (the largest, then halving, at least 64)
         // on the same first triplets (as many as the largest batch) and keep the fastest.
@@ -696,6 +955,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             me.nT = std::min(mp.nty, nT - T0);
             me.be = d_be + (size_t)T0 * nirr * nb;
             me.xi = (char *)c->d_xi + (size_t)c->xi_es * T0 * xi_T;
+            if (d_comp) me.comp = (char *)d_comp + (size_t)c->xi_es * T0 * me.comp_T;
             mp.ev = kev.data() + 3 * nbatch * tc;
             int r = run_mfma_chain(mp, precision, st);
             if (r) { cleanup(); return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r; }
@@ -804,11 +1064,42 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         iso.na = na; iso.integrationtype = integrationtype;
     }
     {
-        const dim3 g(nrows * nT), b(kBlock);
-        if (c->xi_es == 8) {
+        const dim3 g(nrows * nT), b(kBlock), bl(nb <= kBlock ? nb * (kBlock / nb) : kBlock);
+        const size_t shl = (size_t)capm * nb * c->xi_es + sizeof(int) * (2 * (size_t)capm + 1);
+        const size_t sha = (size_t)capm * nb * c->xi_es + sizeof(int) * (5 * (size_t)capm + 1);
+        if (d_comp && (nb > kBlock || sha > 96 * 1024)) { cleanup(); return c->fail(5, "scattering-matrix assembly exceeds the LDS"); }
+        if (d_comp) {
+            // rows from the compact sums; with isotopes the averaging below follows
+            const size_t cT = (size_t)ntrip * nb * 2 * nb;
+            if (c->xi_es == 8) {
+                auto kf = use_iso ? k_xi_asm<double, double, false> : k_xi_asm<double, double, true>;
+                TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
+                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
+                                   (const int4 *)d_xisrc, (const double *)d_comp, cT, d_chkoff, d_chk, d_orboff,
+                                   (double *)c->d_xi, c->xi_row, xi_T, capm);
+            } else {
+                auto kf = use_iso ? k_xi_asm<float, float, false> : k_xi_asm<float, float, true>;
+                TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
+                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
+                                   (const int4 *)d_xisrc, (const float *)d_comp, cT, d_chkoff, d_chk, d_orboff,
+                                   (float *)c->d_xi, c->xi_row, xi_T, capm);
+            }
+            TRY(hipGetLastError());
+        }
+        const bool use_lds = nb <= kBlock && shl <= 80 * 1024 && std::getenv("TC3_XI_GLOBAL") == nullptr;
+        if (use_lds) {
+            const void *kf = (c->xi_es == 8) ? (const void *)k_xi_sym_lds<double> : (const void *)k_xi_sym_lds<float>;
+            TRY(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shl));
+        }
+        if (d_comp && !use_iso) {
+            // averaged by k_xi_asm
+        } else if (c->xi_es == 8) {
             if (use_iso)
                 hipLaunchKernelGGL((k_xi_sym<double, true>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
                                    d_members, (double *)c->d_xi, c->xi_row, xi_T, iso);
+            else if (use_lds)
+                hipLaunchKernelGGL((k_xi_sym_lds<double>), g, bl, shl, st, nrows, nb, d_rowq, d_chkoff, d_chk,
+                                   d_orboff, d_members, (double *)c->d_xi, c->xi_row, xi_T, capm);
             else
                 hipLaunchKernelGGL((k_xi_sym<double, false>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
                                    d_members, (double *)c->d_xi, c->xi_row, xi_T, iso);
@@ -816,6 +1107,9 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             if (use_iso)
                 hipLaunchKernelGGL((k_xi_sym<float, true>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
                                    d_members, (float *)c->d_xi, c->xi_row, xi_T, iso);
+            else if (use_lds)
+                hipLaunchKernelGGL((k_xi_sym_lds<float>), g, bl, shl, st, nrows, nb, d_rowq, d_chkoff, d_chk,
+                                   d_orboff, d_members, (float *)c->d_xi, c->xi_row, xi_T, capm);
             else
                 hipLaunchKernelGGL((k_xi_sym<float, false>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
                                    d_members, (float *)c->d_xi, c->xi_row, xi_T, iso);
@@ -848,6 +1142,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         float m3 = 0;
         hipEventElapsedTime(&m3, evs0, evs1);
         c->kstat_ms[2] += m3; c->kstat_n[2] += 1;
+        if (d_comp) { c->kstat_ms[4] += m3; c->kstat_n[4] += 1; }
     }
     hipEventDestroy(evs0);
     hipEventDestroy(evs1);
