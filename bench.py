@@ -52,7 +52,7 @@ def profile_traffic(kernel):
 def kernel_stats(eng):
     import ctypes
     out = {}
-    for w, name in ((0, "phi_u_w"), (1, "psi_epilogue"), (2, "xi_symmetrise")):
+    for w, name in ((0, "phi_u_w"), (1, "psi_epilogue"), (2, "xi_symmetrise"), (4, "xi_assembled")):
         ms = ctypes.c_double()
         n = ctypes.c_int()
         b = ctypes.c_int64()
@@ -177,14 +177,20 @@ def main():
             dom, fl, ms, n = "k_psi (Psi=E1^T.W + smeared-delta epilogue)", f_psi, ms_psi, n_psi
         # kernel statistics cover the last tc3_scatter call, i.e. one step.  Algorithmic
         # bytes per launch: W (2 nb^3 reals per triplet, FP32 in the mixed modes) written by
-        # k_uw and read once by k_psi, plus the scattering-matrix entries k_psi stores
-        # (FP32 in the mixed modes); tables and eigenvectors are negligible.  The roofline
-        # reported is the one whose minimum time (algorithmic work at peak) is larger.
+        # k_uw and read once by k_psi, plus what k_psi stores: the compact per-triplet sums
+        # (2 nb per (b1, T)) when the rows are assembled afterwards, else the
+        # scattering-matrix entries (FP32 in the mixed modes); tables and eigenvectors are
+        # negligible.  The roofline reported is the one whose minimum time (algorithmic
+        # work at peak) is larger.
         avg_ms = ms / max(1, n)
         es = 8 if args.precision == "fp64" else 4
         w_bytes = 2 * nb ** 3 * es * batch
-        xi_bytes = len(temps) * eng.nrows * eng.nfull * nb * es / max(1, n_psi)
-        by = w_bytes if dom.startswith("k_uw") else w_bytes + xi_bytes
+        assembled = ks_mx["xi_assembled"][1] > 0
+        if assembled:
+            out_bytes = len(temps) * eng.ntriplets * nb * 2 * nb * es / max(1, n_psi)
+        else:
+            out_bytes = len(temps) * eng.nrows * eng.nfull * nb * es / max(1, n_psi)
+        by = w_bytes if dom.startswith("k_uw") else w_bytes + out_bytes
         fl_launch = fl / max(1, n)
         issue = 3.0 if args.precision == "bf16x3" else 1.0
         peak = PEAK_TFLOPS[args.precision]
@@ -206,7 +212,7 @@ def main():
                     unit=main["unit"], frac=main["frac"], traffic=traffic, traffic_source=tsrc, kernel=dom,
                     launches_per_step=n, avg_launch_ms=avg_ms, triplets_per_launch=batch,
                     flop_per_triplet=fpt_uw if dom.startswith("k_uw") else 8.0 * nb ** 4,
-                    ifc_block_fraction=frac_blk, hbm=hbm_view, mfma=mfma_view)
+                    ifc_block_fraction=frac_blk, rows_assembled=assembled, hbm=hbm_view, mfma=mfma_view)
         if traffic:
             roof["measured_hbm_GBps"] = traffic / (avg_ms * 1e-3) / 1e9
         res = {

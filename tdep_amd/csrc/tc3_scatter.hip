@@ -444,6 +444,10 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
 #define TC3_XI_DEPTH 16
 #endif
 constexpr int kXiDepth = TC3_XI_DEPTH;
+#ifndef TC3_XI_CHUNK
+#define TC3_XI_CHUNK 4096
+#endif
+constexpr int kXiChunk = TC3_XI_CHUNK;   // entries per orbit chunk staged in LDS
 
 template <typename XT>
 __global__ void __launch_bounds__(kBlock) k_xi_sym_lds(int nrows, int nb, const int *__restrict__ row_q,
@@ -524,8 +528,28 @@ constexpr int kAsmDepth = TC3_ASM_DEPTH;
 // entries are formed with the arithmetic of the direct stores of k_psi (frequency of the
 // mode times 2 / omega_1, rounding to the entry type, the q3' term added to the rounded
 // q2' term), so the rows are the same bits.  Workgroup per (row, T), one band per thread.
+// Raw buffer access with 32-bit byte offsets from a uniform base (gfx9 descriptor word 3).
+typedef unsigned xv2u __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t xrsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ inline T xload(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <typename T>
+__device__ inline void xstore(T v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(xv2u, v), r, (int)off, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+
 template <typename XT, typename S, bool AVG>
-__global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int *__restrict__ row_q,
+__global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull, const int *__restrict__ row_q,
                                                    const int *__restrict__ row_b, const double *__restrict__ om_irr,
                                                    const double *__restrict__ om_full, const int4 *__restrict__ msrc,
                                                    const S *__restrict__ comp, size_t comp_T,
@@ -534,16 +558,20 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int 
                                                    size_t xi_row, size_t xi_T, int capm) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT *val = (XT *)smem;
-    int *mem = (int *)(val + (size_t)capm * nb);
+    int *mem = (int *)(val + capm * nb);
     int *ooff = mem + capm;
     int *srcs = ooff + capm + 1;
     int *w2o = srcs + capm, *w3o = w2o + capm;
     const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
     const int q1 = row_q[r], b1 = row_b[r];
     const double iw1 = 2.0 / om_irr[(size_t)q1 * nb + b1];
-    XT *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
-    const S *cT = comp + (size_t)T * comp_T;
+    // byte offsets below 2^32 (checked by the caller)
+    const __amdgpu_buffer_rsrc_t rx = xrsrc(xi + (size_t)T * xi_T + (size_t)r * xi_row, (unsigned)(xi_row * sizeof(XT)));
+    const __amdgpu_buffer_rsrc_t rc = xrsrc(comp + (size_t)T * comp_T, (unsigned)(comp_T * sizeof(S)));
+    const __amdgpu_buffer_rsrc_t ro = xrsrc(om_full, (unsigned)((size_t)nfull * nb * sizeof(double)));
     const int b = threadIdx.x % nb, p0 = threadIdx.x / nb, ps = blockDim.x / nb;
+    const unsigned tstride = (unsigned)(2 * nb * nb * sizeof(S));
+    const unsigned cbase = (unsigned)((2 * nb * b1 + b) * sizeof(S)), c3 = (unsigned)(nb * sizeof(S));
     for (int c = chk_off[q1]; c + 1 < chk_off[q1 + 1]; ++c) {
         const int oa = chk[c], ob = chk[c + 1];
         const int pa = orb_off[oa], n = orb_off[ob] - pa;
@@ -565,8 +593,8 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int 
             for (int k = 0; k < kAsmDepth; ++k) {
                 const int pk = min(p + k * ps, n - 1);
                 const int sm = srcs[pk];
-                v[k] = cT[((size_t)(sm >> 2) * nb + b1) * 2 * nb + ((sm & 3) == 2 ? nb : 0) + b];
-                f[k] = om_full[(size_t)w2o[pk] + b];
+                v[k] = xload<S>(rc, (unsigned)(sm >> 2) * tstride + cbase + ((sm & 3) == 2 ? c3 : 0u));
+                f[k] = xload<double>(ro, (unsigned)(w2o[pk] + b) * 8u);
             }
 #pragma unroll
             for (int k = 0; k < kAsmDepth; ++k) {
@@ -576,10 +604,11 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int 
                 XT e = (sm & 3) ? (XT)((double)v[k] * f[k] * iw1) : (XT)0;
                 if ((sm & 3) == 3) {
                     // q2' = q3' within one pair: the q3' term added to the rounded q2' term
-                    const S v3 = cT[((size_t)(sm >> 2) * nb + b1) * 2 * nb + nb + b];
-                    e = (XT)((double)e + (double)v3 * om_full[(size_t)w3o[pk] + b] * iw1);
+                    const S v3 = xload<S>(rc, (unsigned)(sm >> 2) * tstride + cbase + c3);
+                    const double f3 = xload<double>(ro, (unsigned)(w3o[pk] + b) * 8u);
+                    e = (XT)((double)e + (double)v3 * f3 * iw1);
                 }
-                val[(size_t)pk * nb + b] = e;
+                val[pk * nb + b] = e;
             }
         }
         __syncthreads();
@@ -589,7 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int 
                 double sp = 0, sa = 0;
                 int cnt = 0;
                 for (int m = m0; m < m1; ++m) {
-                    const double v = (double)val[(size_t)m * nb + b];
+                    const double v = (double)val[m * nb + b];
                     sa += v;
                     if (v > 0.0) { sp += v; ++cnt; }
                 }
@@ -600,17 +629,11 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, const int 
                     for (int m = m0; m < m1; ++m) sum += avg;
                 }
                 const XT out = (XT)(sum / (m1 - m0));
-                for (int m = m0; m < m1; ++m) val[(size_t)m * nb + b] = out;
+                for (int m = m0; m < m1; ++m) val[m * nb + b] = out;
             }
             __syncthreads();
         }
-        for (int p = p0; p < n; p += kAsmDepth * ps) {
-#pragma unroll
-            for (int k = 0; k < kAsmDepth; ++k) {
-                const int pk = p + k * ps;
-                if (pk < n) x[(size_t)mem[pk] * nb + b] = val[(size_t)pk * nb + b];
-            }
-        }
+        for (int p = p0; p < n; p += ps) xstore<XT>(val[p * nb + b], rx, (unsigned)(mem[p] * nb + b) * (unsigned)sizeof(XT));
         __syncthreads();
     }
 }
@@ -699,15 +722,15 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(table("orboff", (void **)&d_orboff, c->orb.orb_off.data(), sizeof(int) * c->orb.orb_off.size()));
     TRY(table("orbq1", (void **)&d_orbq1, c->orb.q1_off.data(), sizeof(int) * c->orb.q1_off.size()));
     TRY(table("members", (void **)&d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size()));
-    // chunks of whole orbits of every q1 for k_xi_sym_lds: at most capm members (8192
-    // entries, or the largest orbit)
+    // chunks of whole orbits of every q1 for k_xi_sym_lds and k_xi_asm: at most capm
+    // members (kXiChunk entries, or the largest orbit)
     int capm = 0;
     std::vector<int> chk_off(nirr + 1, 0), chk;
     {
         int maxorb = 1;
         for (size_t o = 0; o + 1 < c->orb.orb_off.size(); ++o)
             maxorb = std::max(maxorb, c->orb.orb_off[o + 1] - c->orb.orb_off[o]);
-        capm = std::max(maxorb, 8192 / std::max(nb, 1));
+        capm = std::max(maxorb, kXiChunk / std::max(nb, 1));
         for (int q = 0; q < nirr; ++q) {
             chk_off[q] = (int)chk.size();
             const int oa = c->orb.q1_off[q], ob = c->orb.q1_off[q + 1];
@@ -882,7 +905,9 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             c->xi_src_ok = xi_source_map(c->trip, c->orb, nirr, nfull, nb, c->xi_src);
             c->xi_src_gen = c->gen_model;
         }
-        if (c->xi_src_ok && std::getenv("TC3_XI_DIRECT") == nullptr) {
+        // k_xi_asm addresses one temperature's sums and one row with 32-bit byte offsets
+        const bool fits32 = (double)c->xi_es * me.comp_T < 4.0e9 && (double)c->xi_es * c->xi_row < 4.0e9;
+        if (c->xi_src_ok && fits32 && std::getenv("TC3_XI_DIRECT") == nullptr) {
             TRY(table("xisrc", (void **)&d_xisrc, c->xi_src.data(), sizeof(int4) * (c->xi_src.size() / 4)));
             TRY(c->pool_get("xi_comp", (size_t)c->xi_es * nT * me.comp_T, &me.comp));
             d_comp = me.comp;
@@ -1074,13 +1099,13 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             if (c->xi_es == 8) {
                 auto kf = use_iso ? k_xi_asm<double, double, false> : k_xi_asm<double, double, true>;
                 TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
-                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
+                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, nfull, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
                                    (const int4 *)d_xisrc, (const double *)d_comp, cT, d_chkoff, d_chk, d_orboff,
                                    (double *)c->d_xi, c->xi_row, xi_T, capm);
             } else {
                 auto kf = use_iso ? k_xi_asm<float, float, false> : k_xi_asm<float, float, true>;
                 TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
-                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
+                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, nfull, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
                                    (const int4 *)d_xisrc, (const float *)d_comp, cT, d_chkoff, d_chk, d_orboff,
                                    (float *)c->d_xi, c->xi_row, xi_T, capm);
             }
