@@ -657,6 +657,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     for (int r = 0; r < 4; ++r) yM[r] = 0;
     const S sc = (S)(a.psi_scale * a.perm[t]);
     const S sfix2 = (S)(a.sigma_fixed * a.sigma_fixed);
+    const bool itype2 = a.integrationtype == 2;
     // Tiles (16 b1 x 16 b3 at one b2) whose every element lies beyond the Gaussians' 8 sigma
     // cut in all four terms contribute exactly zero (lo_gauss returns 0 there): a wave skips
     // them.  Interval bounds over the tile's w1, w3 and largest sigma, with a margin far above
@@ -755,29 +756,42 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                 mma1(ci, ai[ks], br);
             }
         }
-        // smeared energy conservation for the 4 elements (b1, b2, b3) of this lane
+        // smeared energy conservation for the 4 elements (b1, b2, b3) of this lane; the pair
+        // masks select the results (a masked element's value is computed and discarded)
         const unsigned ok2 = (v12w[b2 >> 5] >> (b2 & 31)) & 1u;
         const unsigned okm = ok2 ? (ok13 & v23[b2 * NMW + (b3w >> 5)]) : 0u;
         const S sg12 = sg1 + sg2s[b2];
+        // frequencies and sigma^2 of this lane's b3 (4 consecutive values outside FP64)
+        S w3v[4], s3v[4];
+        if constexpr (PREC == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int b3 = b3w + out_row<PREC>(lane, r);
+                w3v[r] = om3[b3];
+                s3v[r] = sg3s[b3];
+            }
+        } else {
+            const int b30 = b3w + out_row<PREC>(lane, 0);
+            const float4 wv = *(const float4 *)(w3f + b30), sv = *(const float4 *)(sg3s + b30);
+            w3v[0] = wv.x; w3v[1] = wv.y; w3v[2] = wv.z; w3v[3] = wv.w;
+            s3v[0] = sv.x; s3v[1] = sv.y; s3v[2] = sv.z; s3v[3] = sv.w;
+        }
+        const S x2 = (PREC == 0) ? (S)w2s[b2] : (S)w2f[b2];
         S P[4], M[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            P[r] = 0;
-            M[r] = 0;
-            if ((okm >> (sh3 + out_row<PREC>(lane, r))) & 1u) {
-                const S re = (S)cr[r], im = (S)ci[r];
-                const S psisq = sc * (re * re + im * im);
-                const int b3 = b3w + out_row<PREC>(lane, r);
-                const S sig2 = (a.integrationtype == 2) ? sg12 + sg3s[b3] : sfix2;
-                S d0, d1;
-                if constexpr (PREC == 0)
-                    delta_pair(w1, w2s[b2], om3[b3], sig2, d0, d1);
-                else
-                    delta_pair(x1, w2f[b2], w3f[b3], sig2, d0, d1);
-                const S A = psisq * d0, B = psisq * d1;
-                P[r] = A + B;
-                M[r] = B - A;
-            }
+            const S re = (S)cr[r], im = (S)ci[r];
+            const S psisq = sc * (re * re + im * im);
+            const S sig2 = itype2 ? sg12 + s3v[r] : sfix2;
+            S d0, d1;
+            if constexpr (PREC == 0)
+                delta_pair(w1, x2, w3v[r], sig2, d0, d1);
+            else
+                delta_pair(x1, x2, w3v[r], sig2, d0, d1);
+            const S A = psisq * d0, B = psisq * d1;
+            const bool on = (okm >> (sh3 + out_row<PREC>(lane, r))) & 1u;
+            P[r] = on ? A + B : (S)0;
+            M[r] = on ? B - A : (S)0;
         }
         // sum over this wave's b3 of f_T, temperature T in output column c16
         const S bp = (c16 < nT) ? n2h[c16 * NB + b2] : (S)0;
