@@ -28,6 +28,8 @@ int device_error(hipError_t e, const char *what, const char *file, int line) {
 
 tc3_ctx::~tc3_ctx() {
     if (bte_exec) hipGraphExecDestroy(bte_exec);
+    for (hipEvent_t e : ev_pool) hipEventDestroy(e);
+    if (pin) hipHostFree(pin);
     hipFree(d_om_irr);
     hipFree(d_om_full);
     hipFree(d_vel_irr);

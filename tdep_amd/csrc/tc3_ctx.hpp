@@ -99,6 +99,23 @@ struct tc3_ctx {
     std::vector<int> xi_src;
     bool xi_src_ok = false;
     long xi_src_gen = -1;
+    // orbit chunks of the scattering-matrix kernels (per model) and pooled HIP events
+    std::vector<int> xchk_off, xchk;
+    int xcapm = 0;
+    long xchk_gen = -1;
+    std::vector<hipEvent_t> ev_pool;
+    // pinned host staging buffer of the per-call transfers (grown as needed)
+    void *pin = nullptr;
+    size_t pin_bytes = 0;
+    hipError_t pinned(size_t bytes) {
+        if (bytes <= pin_bytes) return hipSuccess;
+        if (pin) hipHostFree(pin);
+        pin = nullptr;
+        pin_bytes = 0;
+        const hipError_t r = hipHostMalloc(&pin, bytes, hipHostMallocDefault);
+        if (r == hipSuccess) pin_bytes = bytes;
+        return r;
+    }
     struct TuneProbe { std::string key; double batch, ms; };
     std::vector<TuneProbe> tune_log;
     // instantiated BTE iteration graph and the arguments it was captured with

@@ -724,9 +724,11 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(table("members", (void **)&d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size()));
     // chunks of whole orbits of every q1 for k_xi_sym_lds and k_xi_asm: at most capm
     // members (kXiChunk entries, or the largest orbit)
-    int capm = 0;
-    std::vector<int> chk_off(nirr + 1, 0), chk;
-    {
+    if (c->xchk_gen != c->gen_model) {
+        std::vector<int> &chk_off = c->xchk_off, &chk = c->xchk;
+        int &capm = c->xcapm;
+        chk_off.assign(nirr + 1, 0);
+        chk.clear();
         int maxorb = 1;
         for (size_t o = 0; o + 1 < c->orb.orb_off.size(); ++o)
             maxorb = std::max(maxorb, c->orb.orb_off[o + 1] - c->orb.orb_off[o]);
@@ -740,7 +742,10 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             if (chk.back() != ob) chk.push_back(ob);
         }
         chk_off[nirr] = (int)chk.size();
+        c->xchk_gen = c->gen_model;
     }
+    const int capm = c->xcapm;
+    const std::vector<int> &chk_off = c->xchk_off, &chk = c->xchk;
     int *d_chkoff, *d_chk;
     TRY(table("xchkoff", (void **)&d_chkoff, chk_off.data(), sizeof(int) * chk_off.size()));
     TRY(table("xchk", (void **)&d_chk, chk.data(), sizeof(int) * chk.size()));
@@ -755,8 +760,15 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(c->pool_get("sigsq", sizeof(double) * nirr * nb, (void **)&d_sigsq));
     TRY(c->pool_get("gpart", sizeof(double) * ntrip * nT * nb, (void **)&d_gpart));
     TRY(c->pool_get("lw", sizeof(double) * nrows * nT, (void **)&d_lw));
-    TRY(hipMemcpyAsync(d_be, c->be.data(), sizeof(double) * nT * nirr * nb, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_sigsq, c->sigsq.data(), sizeof(double) * nirr * nb, hipMemcpyHostToDevice, st));
+    // per-call transfers go through the context's pinned staging buffer: occupations and
+    // widths up, linewidths (and the isotope term) down
+    const size_t n_be = (size_t)nT * nirr * nb, n_sg = (size_t)nirr * nb, n_lw = (size_t)nrows * nT;
+    TRY(c->pinned(sizeof(double) * (n_be + n_sg + n_lw + nrows)));
+    double *h_be = (double *)c->pin, *h_sg = h_be + n_be, *h_lw = h_sg + n_sg, *h_iso = h_lw + n_lw;
+    std::memcpy(h_be, c->be.data(), sizeof(double) * n_be);
+    std::memcpy(h_sg, c->sigsq.data(), sizeof(double) * n_sg);
+    TRY(hipMemcpyAsync(d_be, h_be, sizeof(double) * n_be, hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_sigsq, h_sg, sizeof(double) * n_sg, hipMemcpyHostToDevice, st));
     // scattering matrix: every entry that any triplet owns is rewritten by every pass, so
     // the buffer is zeroed only when it is new or the model changed
     c->xi_row = (size_t)nfull * nb;
@@ -832,9 +844,23 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
 
     int *d_xisrc = nullptr;
     void *d_comp = nullptr;
-    hipEvent_t ev0, ev1;
-    TRY(hipEventCreate(&ev0));
-    TRY(hipEventCreate(&ev1));
+    // events come from the context's pool (created once, kept until the context ends)
+    auto pooled_events = [&](size_t n) -> hipError_t {
+        while (c->ev_pool.size() < n) {
+            hipEvent_t e;
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            c->ev_pool.push_back(e);
+        }
+        return hipSuccess;
+    };
+    auto take_events = [&](std::vector<hipEvent_t> &v, size_t n) -> hipError_t {
+        const hipError_t r = pooled_events(4 + n);
+        if (r == hipSuccess) v.assign(c->ev_pool.begin() + 4, c->ev_pool.begin() + 4 + n);
+        return r;
+    };
+    TRY(pooled_events(4));
+    hipEvent_t ev0 = c->ev_pool[0], ev1 = c->ev_pool[1];
     std::vector<hipEvent_t> kev;
     for (double &x : c->kstat_ms) x = 0;
     for (int &x : c->kstat_n) x = 0;
@@ -970,8 +996,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         mp.batch = mbatch;
         const long nbatch = (ntrip + mbatch - 1) / mbatch;
         const int ntchunk = (nT + mp.nty - 1) / mp.nty;
-        kev.resize(3 * nbatch * ntchunk);
-        for (auto &x : kev) TRY(hipEventCreate(&x));
+        TRY(take_events(kev, 3 * nbatch * ntchunk));
         TRY(hipEventRecord(ev0, st));
         // temperatures in blocks of mp.nty; each block recomputes the contraction
         for (int tc = 0; tc < ntchunk; ++tc) {
@@ -989,8 +1014,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     } else if (use_small) {
         // persistent work-queue kernel for small cells
         TRY(hipEventRecord(ev0, st));
-        kev.resize(3);
-        for (auto &x : kev) TRY(hipEventCreate(&x));
+        TRY(take_events(kev, 3));
         int *d_queue;
         TRY(c->pool_get("small_queue", sizeof(int), (void **)&d_queue));
         int ncu = 0;
@@ -1023,8 +1047,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         c->batch = ntrip;
     } else {
     TRY(hipEventRecord(ev0, st));
-    kev.resize(3 * ((ntrip + batch - 1) / batch));
-    for (auto &x : kev) TRY(hipEventCreate(&x));
+    TRY(take_events(kev, 3 * ((ntrip + batch - 1) / batch)));
     for (long t0 = 0; t0 < ntrip; t0 += batch) {
         const int nt = (int)std::min<long>(batch, ntrip - t0);
         hipEvent_t *ev = &kev[3 * (t0 / batch)];
@@ -1064,9 +1087,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     }
     c->batch = batch;
     }
-    hipEvent_t evs0, evs1;
-    TRY(hipEventCreate(&evs0));
-    TRY(hipEventCreate(&evs1));
+    hipEvent_t evs0 = c->ev_pool[2], evs1 = c->ev_pool[3];
     hipLaunchKernelGGL(k_gamma, dim3((nrows * nT + 255) / 256), dim3(256), 0, st, nrows, nT, nb, d_rowq, d_rowb,
                        d_q1off, d_gpart, d_lw);
     TRY(hipGetLastError());
@@ -1143,18 +1164,18 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs1, st));
     TRY(hipEventRecord(ev1, st));
-    std::vector<double> lw((size_t)nrows * nT), lw_iso(use_iso ? nrows : 0);
-    TRY(hipMemcpyAsync(lw.data(), d_lw, sizeof(double) * nrows * nT, hipMemcpyDeviceToHost, st));
-    if (use_iso) TRY(hipMemcpyAsync(lw_iso.data(), iso.gam, sizeof(double) * nrows, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(h_lw, d_lw, sizeof(double) * n_lw, hipMemcpyDeviceToHost, st));
+    if (use_iso) TRY(hipMemcpyAsync(h_iso, iso.gam, sizeof(double) * nrows, hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
+    std::vector<double> lw(h_lw, h_lw + n_lw), lw_iso(use_iso ? nrows : 0);
+    if (use_iso) std::copy(h_iso, h_iso + nrows, lw_iso.begin());
     // isotope linewidth first, three-phonon added (scattering.f90:226-236)
     if (use_iso)
         for (int T = 0; T < nT; ++T)
             for (int r = 0; r < nrows; ++r) lw[(size_t)T * nrows + r] = lw_iso[r] + lw[(size_t)T * nrows + r];
     float ms = 0;
     hipEventElapsedTime(&ms, ev0, ev1);
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
+
     for (size_t b = 0; b + 2 < kev.size(); b += 3) {
         float m1 = 0, m2 = 0;
         hipEventElapsedTime(&m1, kev[b], kev[b + 1]);
@@ -1162,15 +1183,14 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         c->kstat_ms[0] += m1; c->kstat_n[0] += 1;
         c->kstat_ms[1] += m2; c->kstat_n[1] += 1;
     }
-    for (auto &x : kev) hipEventDestroy(x);
+
     {
         float m3 = 0;
         hipEventElapsedTime(&m3, evs0, evs1);
         c->kstat_ms[2] += m3; c->kstat_n[2] += 1;
         if (d_comp) { c->kstat_ms[4] += m3; c->kstat_n[4] += 1; }
     }
-    hipEventDestroy(evs0);
-    hipEventDestroy(evs1);
+
 #undef TRY
     cleanup();
     c->timings[5] = ms * 1e-3;
