@@ -35,15 +35,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profile_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary (two rocprofv3
-    --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel), or None."""
+def profile_traffic(kernel, triplets):
+    """HBM bytes of one launch of `kernel` over `triplets` triplets, from the committed
+    counter summary (HBM bytes per triplet of two rocprofv3 --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy kernel), or None."""
     path = os.path.join(ROOT, "profiles", "scatter_sl16_bf16x3_counters.md")
     try:
         for line in open(path):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 5 and cells[0] == kernel:
-                return float(cells[4]), os.path.relpath(path, ROOT)
+            if len(cells) == 4 and cells[0] == kernel:
+                return float(cells[3]) * triplets, os.path.relpath(path, ROOT)
     except OSError:
         pass
     return None, None
@@ -206,7 +207,7 @@ def main():
                         min_ms_at_peak=t_hbm * 1e3)
         traffic, tsrc = (None, None)
         if args.precision == "bf16x3":
-            traffic, tsrc = profile_traffic("k_uw" if dom.startswith("k_uw") else "k_psi")
+            traffic, tsrc = profile_traffic("k_uw" if dom.startswith("k_uw") else "k_psi", batch)
         main = hbm_view if t_hbm >= t_mfma else mfma_view
         roof = dict(bound="hbm" if t_hbm >= t_mfma else "mfma", achieved=main["achieved"], peak=main["peak"],
                     unit=main["unit"], frac=main["frac"], traffic=traffic, traffic_source=tsrc, kernel=dom,
