@@ -73,6 +73,13 @@ int tc3_set_fc3(tc3_ctx *ctx, int ntrip, const int *a1, const int *a2, const int
  * tc3_set_structure).  The term is zero when every g is zero. */
 int tc3_set_isotopes(tc3_ctx *ctx, const double *disorder, int enable);
 
+/* Boundary scattering: a mean-free-path limit L (Bohr; the reference's --max_mfp in metres,
+ * converted at options.f90:218) adds |v|/L to the linewidth of every mode whose group velocity
+ * exceeds 1e-10 a.u., after the isotope and three-phonon terms and before the degeneracy
+ * average (scattering.f90:245-250).  L <= 0 switches it off (the default).  Replaces the
+ * opts%mfp_max input of lo_scattering_rates%generate. */
+int tc3_set_boundary(tc3_ctx *ctx, double mfp_max);
+
 /* Space group, Gamma-centred FFT q-mesh, irreducible points, little groups and the
  * irreducible triplet lists: replaces uc%classify('wedge') (type_crystalstructure_symmetry.f90:7-91),
  * lo_generate_qmesh (type_qpointmesh.f90:452-657, type_qpointmesh_gridgeneration.f90:8-545)

@@ -33,14 +33,16 @@ def setup(indir, qgrid, timereversal=True):
 
 
 def run(indir, qgrid, temperature, maxiter=200, tol=1e-5, classical=False, offdiag=True,
-        integrationtype=2, sigma=1.0, pre=None):
+        integrationtype=2, sigma=1.0, pre=None, max_mfp=None):
+    """max_mfp: boundary scattering length in metres (--max_mfp), None for none."""
     p, fc2, fc3, ops, mesh, disp = pre if pre is not None else setup(indir, qgrid)
     if disp.omega_min < -U.FREQTOL:
         raise RuntimeError("FOUND UNSTABLE MODES")
     r = Result()
     r.p, r.fc2, r.fc3, r.ops, r.mesh, r.disp = p, fc2, fc3, ops, mesh, disp
     sr = S.scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=classical,
-                            integrationtype=integrationtype, sigma=sigma)
+                            integrationtype=integrationtype, sigma=sigma,
+                            mfp_max=None if max_mfp is None else max_mfp * U.M_TO_BOHR)
     r.sr = sr
     r.kappa_sma = K.get_kappa(p, mesh, disp, ops, sr.F0, temperature, classical)
     r.kappa_offdiag = (K.get_kappa_offdiag(p, fc2, mesh, disp, ops, sr.linewidth,

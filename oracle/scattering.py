@@ -181,10 +181,11 @@ def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0
 
 
 def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
-                     integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None):
+                     integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None, mfp_max=None):
     """lo_scattering_rates%generate (scattering.f90:58-358): three-phonon scattering with
     qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q), plus
-    isotope scattering (:229-232) when per-atom mass-disorder parameters are given."""
+    isotope scattering (:229-232) when per-atom mass-disorder parameters are given, plus
+    boundary scattering |v| / mfp_max (Bohr, :245-250) when mfp_max > 0."""
     nb = disp.nb
     nirr, nfull = mesh.nirr, mesh.nfull
     sr = ScatteringRates()
@@ -271,6 +272,12 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
             gi, xi = isotope_row(mesh, disp, sigsq, q1, b1, isotope_g, integrationtype, sr.sigma_fixed)
             gamma_row[i] += gi
             Xi[i] += xi.reshape(-1)
+    if mfp_max is not None and mfp_max > 0.0:
+        for i, (q1, b1) in enumerate(rows):
+            v = disp.vel_irr[q1, b1]
+            vn = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+            if vn > U.GROUPVELTOL:
+                gamma_row[i] += vn / mfp_max
     # reduce + degeneracy average + F0 (:260-292)
     lw = np.zeros((nirr, nb))
     for i, (q, b) in enumerate(rows):

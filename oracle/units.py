@@ -41,6 +41,7 @@ KAPPA_AU_TO_SI = HARTREE_TO_JOULE / TIME_AU_TO_S / BOHR_TO_M
 TOL = 1e-5
 SQTOL = TOL ** 2
 FREQTOL = TOL * 1e-4
+GROUPVELTOL = TOL * 1e-5      # lo_phonongroupveltol
 TEMPERATURETOL = 1e-3
 DEGENVECTOR = np.array([1.0, 1.33, 1.45624623]) * 1e-8
 

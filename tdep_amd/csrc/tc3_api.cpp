@@ -220,6 +220,15 @@ int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal) {
     });
 }
 
+int tc3_set_boundary(tc3_ctx *ctx, double mfp_max) {
+    return guarded(ctx, [&]() -> int {
+        if (!(mfp_max == mfp_max)) return ctx->fail(3, "boundary length is not a number");
+        ctx->mfp_max = mfp_max;
+        ctx->have_scatter = false;
+        return 0;
+    });
+}
+
 int tc3_set_isotopes(tc3_ctx *ctx, const double *disorder, int enable) {
     return guarded(ctx, [&]() -> int {
         if (!ctx->have_structure) return ctx->fail(5, "set the structure first");

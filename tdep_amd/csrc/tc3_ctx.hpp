@@ -59,6 +59,7 @@ struct tc3_ctx {
     std::vector<double> be, sigsq;            // [T][q][b], [q][b]
     std::vector<double> iso_g;                // mass-disorder parameter per atom
     bool iso_on = false;                      // isotope scattering requested
+    double mfp_max = -1.0;                    // boundary scattering length (Bohr), off if <= 0
     std::vector<double> linewidth, qs, F0, mfp, smfp;  // [T][q][b](x3)
 
     // kappa results

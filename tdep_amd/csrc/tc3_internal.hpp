@@ -14,6 +14,7 @@ constexpr double kTwoPi = 6.283185307179586;
 constexpr double kTol = 1e-5;
 constexpr double kSqTol = kTol * kTol;
 constexpr double kFreqTol = kTol * 1e-4;
+constexpr double kGroupVelTol = kTol * 1e-5;   // lo_phonongroupveltol
 constexpr double kTemperatureTol = 1e-3;
 constexpr double kHartreeToEv = 27.21138602;
 constexpr double kEvToHartree = 1.0 / kHartreeToEv;

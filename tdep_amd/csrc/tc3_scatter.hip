@@ -1169,10 +1169,18 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(hipStreamSynchronize(st));
     std::vector<double> lw(h_lw, h_lw + n_lw), lw_iso(use_iso ? nrows : 0);
     if (use_iso) std::copy(h_iso, h_iso + nrows, lw_iso.begin());
-    // isotope linewidth first, three-phonon added (scattering.f90:226-236)
+    // isotope linewidth first, three-phonon added (scattering.f90:226-236), then boundary
+    // scattering |v| / L for group velocities above lo_phonongroupveltol (:245-250)
     if (use_iso)
         for (int T = 0; T < nT; ++T)
             for (int r = 0; r < nrows; ++r) lw[(size_t)T * nrows + r] = lw_iso[r] + lw[(size_t)T * nrows + r];
+    if (c->mfp_max > 0.0)
+        for (int r = 0; r < nrows; ++r) {
+            const double *v = &c->vel_irr[((size_t)c->row_q[r] * nb + c->row_b[r]) * 3];
+            const double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            if (vn > kGroupVelTol)
+                for (int T = 0; T < nT; ++T) lw[(size_t)T * nrows + r] += vn / c->mfp_max;
+        }
     float ms = 0;
     hipEventElapsedTime(&ms, ev0, ev1);
 

@@ -23,6 +23,7 @@ from .lib import Tc3Error, dptr, iptr, load
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 KAPPA_AU_TO_SI = (27.21138602 * 1.6021766208e-19) / (
     (6.582119514e-16 * 1.6021766208e-19) / (27.21138602 * 1.6021766208e-19)) / 0.52917721067e-10
+M_TO_BOHR = 1.0 / 0.52917721067e-10   # lo_m_to_Bohr (konstanter.f90:45-46)
 
 
 def _fe(x, w=24, d=12):
@@ -69,6 +70,12 @@ class Engine:
         eng.set_fc3(fc3)
         eng.set_qmesh(qgrid, timereversal)
         return eng
+
+    def set_boundary(self, max_mfp):
+        """Boundary scattering |v| / L with L = max_mfp in metres (the reference's --max_mfp,
+        scattering.f90:245-250); None or a value <= 0 switches it off."""
+        L = -1.0 if max_mfp is None or max_mfp <= 0 else float(max_mfp) * M_TO_BOHR
+        self._check(self.lib.tc3_set_boundary(self.ctx, L))
 
     def set_isotopes(self, disorder, enable=True):
         """Isotope scattering with per-atom mass-disorder parameters (scattering_isotope.f90);

@@ -59,6 +59,12 @@ module tc3_api
             integer(c_int), value :: enable
         end function
 
+        integer(c_int) function tc3_set_boundary(ctx, mfp_max) bind(C, name="tc3_set_boundary")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: mfp_max
+        end function
+
         integer(c_int) function tc3_set_qmesh(ctx, dims, timereversal) bind(C, name="tc3_set_qmesh")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

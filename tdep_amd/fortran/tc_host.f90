@@ -15,6 +15,7 @@
 !   --temperature T [T ...]                    temperatures in K (300)
 !   -it 1|2 | --integrationtype 1|2            fixed or adaptive Gaussian (2)
 !   --sigma s                                  fixed smearing in THz, integrationtype 1 (1.0)
+!   --max_mfp L                                boundary scattering, mean free path limit in m (off)
 !   --iterative_tolerance x                    BTE tolerance (1e-5)
 !   --iterative_maxsteps n                     BTE iterations (200)
 !   --classical, --noisotope, --notr, --readiso (masses always come from infile.isotopes)
@@ -408,7 +409,7 @@ program tc_host
     real(c_double), parameter :: time_au_to_s = (6.582119514e-16_c_double*1.6021766208e-19_c_double)/hartree_to_joule
     real(c_double), parameter :: kappa_au_to_si = hartree_to_joule/time_au_to_s/0.52917721067e-10_c_double
     integer(c_int) :: qg(3), integrationtype, maxsteps, prec, nT
-    real(c_double) :: sigma, tol
+    real(c_double) :: sigma, tol, max_mfp
     real(c_double), allocatable :: temps(:)
     logical :: classical, noisotope, notr, offdiag, inputs_only
     integer(c_int) :: device
@@ -432,6 +433,9 @@ program tc_host
     call check(tc3_create(ctx, device), 'tc3_create')
     call check(tc3_set_structure(ctx, int(na, c_int), latvec, rfrac, species, mass), 'tc3_set_structure')
     call check(tc3_set_isotopes(ctx, disorder, merge(0_c_int, 1_c_int, noisotope)), 'tc3_set_isotopes')
+    ! options.f90:218: metres to Bohr (lo_m_to_Bohr)
+    call check(tc3_set_boundary(ctx, merge(max_mfp/0.52917721067e-10_c_double, -1.0_c_double, max_mfp > 0)), &
+               'tc3_set_boundary')
     call check(tc3_set_fc2(ctx, int(npair, c_int), p1, p2, lv, m2), 'tc3_set_fc2')
     call check(tc3_set_fc3(ctx, int(nfc3, c_int), t1, t2, t3, lv2, lv3, m3), 'tc3_set_fc3')
     call check(tc3_set_qmesh(ctx, qg, merge(0_c_int, 1_c_int, notr)), 'tc3_set_qmesh')
@@ -502,6 +506,7 @@ contains
         qg = [26, 26, 26]
         integrationtype = 2
         sigma = 1.0_c_double
+        max_mfp = -1.0_c_double
         tol = 1.0e-5_c_double
         maxsteps = 200
         prec = TC3_PREC_FP64
@@ -541,6 +546,10 @@ contains
             case ('--sigma')
                 call get_command_argument(i + 1, val)
                 read (val, *) sigma
+                i = i + 1
+            case ('--max_mfp')
+                call get_command_argument(i + 1, val)
+                read (val, *) max_mfp
                 i = i + 1
             case ('--iterative_tolerance')
                 call get_command_argument(i + 1, val)

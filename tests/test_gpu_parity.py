@@ -139,3 +139,30 @@ def test_isotope_scattering_matches_oracle(inputs_dir, oracle_setup, name, grid)
     e.scattering_rates([T], precision="fp64")
     sr0 = S.scattering_rates(p, fc3, mesh, disp, ops, T)
     assert _rel(e.linewidths(0)[0], sr0.linewidth) < 1e-9
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (6, 6, 6)), ("sl16", (4, 4, 2))])
+def test_boundary_scattering_matches_oracle(inputs_dir, oracle_setup, name, grid):
+    """Boundary scattering |v| / L (scattering.f90:245-250, --max_mfp): FP64 linewidths and
+    iterative kappa against the oracle with L = 20 nm (comparable to the three-phonon term),
+    and switching it off restores the three-phonon result."""
+    from oracle import kappa as K
+    from oracle import scattering as S
+    from oracle import units as U
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
+    T, length = 300.0, 20e-9
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, T, mfp_max=length * U.M_TO_BOHR)
+    e = _engine(inputs_dir, name, grid)
+    e.set_boundary(length)
+    e.scattering_rates([T], precision="fp64")
+    lw, F0, _ = e.linewidths(0)
+    assert _rel(lw, sr.linewidth) < 1e-9
+    ks = e.kappa(offdiag=False)
+    Fn, _, _ = K.iterative_solution(p, mesh, disp, ops, sr, T)
+    kit = K.symmetrize_kappa(K.get_kappa(p, mesh, disp, ops, Fn, T), ops)
+    assert _rel(ks[1][0], kit) < 1e-8
+    e.set_boundary(None)
+    e.scattering_rates([T], precision="fp64")
+    sr0 = S.scattering_rates(p, fc3, mesh, disp, ops, T)
+    assert _rel(e.linewidths(0)[0], sr0.linewidth) < 1e-9
+    assert e.kappa(offdiag=False)[1][0][0, 0] > ks[1][0][0, 0]

@@ -386,3 +386,28 @@ def test_isotope_rates_linear_and_detailed_balance(oracle_setup):
             r21 = x2[f1, b1] * disp.om_irr[q2, b2] / disp.om_irr[q1, b1]
             worst = max(worst, abs(r12 - r21) / max(abs(r12), 1e-300))
     assert worst < 1e-12
+
+
+def test_boundary_term_is_velocity_over_length_before_degeneracy_average(oracle_setup):
+    """scattering.f90:245-250 then :264-276: each mode gains |v| / L (when |v| exceeds
+    lo_phonongroupveltol) before the degenerate modes are averaged, so the change of the
+    linewidth is the degeneracy average of |v| / L, and F0 = v / (2 Gamma) follows."""
+    from oracle import scattering as S
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (3, 3, 3))
+    length = 1e-8 * U.M_TO_BOHR
+    sr0 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
+    sr1 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, mfp_max=length)
+    nb = disp.nb
+    for q in range(mesh.nirr):
+        for b in range(nb):
+            if disp.om_irr[q, b] < U.FREQTOL:
+                assert sr1.linewidth[q, b] == 0.0
+                continue
+            dm = disp.degen_irr[q][b]
+            vn = np.linalg.norm(disp.vel_irr[q, dm], axis=1)
+            add = np.where(vn > U.GROUPVELTOL, vn / length, 0.0).mean()
+            assert abs(sr1.linewidth[q, b] - sr0.linewidth[q, b] - add) <= 1e-12 * sr1.linewidth[q, b]
+            np.testing.assert_allclose(sr1.F0[q, b], disp.vel_irr[q, b] / (2.0 * sr1.linewidth[q, b]), rtol=1e-13)
+    # a huge length leaves the three-phonon result unchanged to rounding
+    sr2 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, mfp_max=1e30)
+    np.testing.assert_allclose(sr2.linewidth, sr0.linewidth, rtol=1e-14, atol=0)
