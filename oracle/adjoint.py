@@ -24,7 +24,7 @@ import numpy as np
 
 from . import units as U
 from .scattering import (THREEPHONON_PREFACTOR, irreducible_triplets, occupations_and_smearing,
-                         pretransform_phi3, psi_chain, rotation_table)
+                         pretransform_phi3, psi_chain, rotation_table, velocity_difference_widths)
 
 
 def raw_cotangent(disp, dL_dlw):
@@ -48,8 +48,9 @@ def gamma_adjoint(p, fc3, mesh, disp, ops, temperature, g_raw, classical=False,
     temperature; the loop of scattering.scattering_rates run backwards."""
     nb = disp.nb
     nfull = mesh.nfull
-    be, sigsq, _ = occupations_and_smearing(p, mesh, disp, temperature, classical)
+    be, sigsq, sigavg = occupations_and_smearing(p, mesh, disp, temperature, classical)
     sigma_fixed = sigma * U.FREQ_THZ_TO_HARTREE
+    thresh, reclat6 = np.max(sigavg) * 1e-4, p.reclat / np.array(mesh.dims)[None, :]
     rot = rotation_table(p, mesh, ops)
     mcw = 1.0 / nfull
     grad = np.zeros_like(fc3["mwm"])
@@ -73,6 +74,9 @@ def gamma_adjoint(p, fc3, mesh, disp, ops, temperature, g_raw, classical=False,
             if integrationtype == 2:
                 sig = np.sqrt(sigsq[q1][:, None, None] + sigsq[i2][None, :, None]
                               + sigsq[i3][None, None, :])
+            elif integrationtype == 6:
+                sig = velocity_difference_widths(disp.vel_irr[q1], disp.vel_full[q2], disp.vel_full[q3],
+                                                 reclat6, thresh)
             else:
                 sig = np.full((nb, nb, nb), sigma_fixed)
             d0 = U.gauss(w1, -w2 + w3, sig) - U.gauss(w1, w2 - w3, sig)

@@ -140,6 +140,19 @@ def occupations_and_smearing(p, mesh, disp, temperature, classical=False):
     return be, sigsq, sigavg
 
 
+def velocity_difference_widths(v1, v2, v3, reclat, thresh):
+    """get_dirac integration type 6 (scattering_threephonon.f90:251-261): for the group
+    velocities v1 [n1, 3], v2 [n2, 3], v3 [n3, 3] the widths [n1, n2, n3], the sum over the
+    three pairs of sqrt(max_c ((v_a - v_b) reclat)_c^2 / 2) / 3, plus thresh; reclat is the
+    reciprocal basis with column j divided by the mesh dimension j (scattering.f90:138-140)."""
+    def pw(a, b):
+        return np.sqrt(np.max(((a - b) @ reclat) ** 2, axis=-1) * 0.5)
+    s = pw(v1[:, None, :], v2[None, :, :])[:, :, None] / 3.0
+    s = s + pw(v1[:, None, :], v3[None, :, :])[:, None, :] / 3.0
+    s = s + pw(v2[:, None, :], v3[None, :, :])[None, :, :] / 3.0
+    return s + thresh
+
+
 ISOTOPE_PREFACTOR = U.PI / 4.0  # scattering.f90:29
 
 
@@ -153,9 +166,10 @@ def isotope_strength(e1, e2, g):
     return f0 @ np.asarray(g, float)
 
 
-def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0):
+def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0, reclat=None, thresh=0.0):
     """compute_isotope_scattering (scattering_isotope.f90:2-70) for one row (q1, b1):
-    returns the linewidth term and the row of scattering-matrix terms [q2, b2]."""
+    returns the linewidth term and the row of scattering-matrix terms [q2, b2].  Integration
+    type 6 (:52-54) needs reclat (columns divided by the mesh dimensions) and thresh."""
     nb, nfull = disp.nb, mesh.nfull
     om1 = disp.om_irr[q1, b1]
     e1 = disp.egv_irr[q1][:, b1]
@@ -171,6 +185,8 @@ def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0
                 continue
             if integrationtype == 2:
                 sig = np.sqrt(sigsq[q1, b1] + sigsq[i2, b2])
+            elif integrationtype == 6:
+                sig = np.sqrt(np.max((disp.vel_full[q2, b2] @ reclat) ** 2) * 0.5) + thresh
             else:
                 sig = sigma_fixed
             psisq = st[b2] * ISOTOPE_PREFACTOR / nfull
@@ -193,6 +209,7 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
     sr.be, sr.sigsq = be, sigsq
     sr.sigma_fixed = sigma * U.FREQ_THZ_TO_HARTREE
     sr.thresh_sigma = np.max(sigavg) * 1e-4
+    reclat6 = p.reclat / np.array(mesh.dims)[None, :]
     rows = [(q, b) for q in range(nirr) for b in range(nb) if disp.om_irr[q, b] >= U.FREQTOL]
     sr.rows = rows
     row_of = {r: i for i, r in enumerate(rows)}
@@ -230,7 +247,8 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
             elif integrationtype == 1:
                 sig = np.full((nb, nb, nb), sr.sigma_fixed)
             else:
-                raise NotImplementedError("integrationtype 6")
+                sig = velocity_difference_widths(disp.vel_irr[q1], disp.vel_full[q2], disp.vel_full[q3],
+                                                 reclat6, sr.thresh_sigma)
             d0 = U.gauss(w1, -w2 + w3, sig) - U.gauss(w1, w2 - w3, sig)
             d1 = U.gauss(w1, w2 + w3, sig) - U.gauss(w1, -w2 - w3, sig)
             f = perm * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1)
@@ -269,7 +287,8 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
         for i, (q1, b1) in enumerate(rows):
             if rows_only is not None and i not in rows_only:
                 continue
-            gi, xi = isotope_row(mesh, disp, sigsq, q1, b1, isotope_g, integrationtype, sr.sigma_fixed)
+            gi, xi = isotope_row(mesh, disp, sigsq, q1, b1, isotope_g, integrationtype, sr.sigma_fixed,
+                                 reclat6, sr.thresh_sigma)
             gamma_row[i] += gi
             Xi[i] += xi.reshape(-1)
     if mfp_max is not None and mfp_max > 0.0:

@@ -33,6 +33,8 @@ struct AdjArgs {
     const int *tq1, *tq2, *tq3, *eq_off, *irr_of_full, *row_of;
     const double *perm, *om_irr, *om_full, *be, *sigsq;
     const double *g;                // [T][row] cotangent of the raw linewidths
+    const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
+    double thresh6;
 };
 
 // Z[tl,b1](b2,b3) = H conj(Psi) in place; H as in the file comment, zero where the forward
@@ -60,9 +62,12 @@ __global__ void k_adj_weight(AdjArgs a, cplx<T> *__restrict__ psi) {
         double h = 0.0;
         if (!(w2 < 1e-9) && !(fabs(w1 - w2) < 1e-9) && !(w3 < 1e-9) && !(fabs(w1 - w3) < 1e-9) &&
             !(fabs(w2 - w3) < 1e-9)) {
-            const double sig = (a.integrationtype == 2)
-                                   ? sqrt(sg1 + a.sigsq[i2 * nb + b2] + a.sigsq[i3 * nb + b3])
-                                   : a.sigma_fixed;
+            const double sig =
+                (a.integrationtype == 2) ? sqrt(sg1 + a.sigsq[i2 * nb + b2] + a.sigsq[i3 * nb + b3])
+                : (a.integrationtype == 6)
+                    ? sigma_type6(a.u_irr + ((size_t)q1 * nb + b1) * 3, a.u_full + ((size_t)q2 * nb + b2) * 3,
+                                  a.u_full + ((size_t)q3 * nb + b3) * 3, a.thresh6)
+                    : a.sigma_fixed;
             const double d0 = gauss(w1, -w2 + w3, sig) - gauss(w1, w2 - w3, sig);
             const double d1 = gauss(w1, w2 + w3, sig) - gauss(w1, -w2 - w3, sig);
             for (int T_ = 0; T_ < a.nT; ++T_) {
@@ -275,6 +280,16 @@ int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<do
     AdjArgs aa;
     aa.nb = nb; aa.nT = nT; aa.t0 = 0; aa.nirr = nirr; aa.nrows = nrows; aa.integrationtype = c->integrationtype;
     aa.sigma_fixed = c->sigma * kFreqThzToHartree;
+    // reduced group velocities of integration type 6, left in the pool by the scattering pass
+    aa.u_irr = aa.u_full = nullptr;
+    aa.thresh6 = c->thresh6;
+    if (c->integrationtype == 6) {
+        double *ui, *uf;
+        TRY(c->pool_get("u_irr", sizeof(double) * 3 * nirr * nb, (void **)&ui));
+        TRY(c->pool_get("u_full", sizeof(double) * 3 * nfull * nb, (void **)&uf));
+        aa.u_irr = ui;
+        aa.u_full = uf;
+    }
     aa.psi_scale = (kPi / 16.0) / (double)nfull;
     aa.tq1 = d_tq1; aa.tq2 = d_tq2; aa.tq3 = d_tq3; aa.eq_off = d_eqoff; aa.irr_of_full = c->d_irr_of_full;
     aa.row_of = d_rowof; aa.perm = d_perm; aa.om_irr = c->d_om_irr; aa.om_full = c->d_om_full; aa.be = d_be;

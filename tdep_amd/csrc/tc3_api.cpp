@@ -330,8 +330,8 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
     return guarded(ctx, [&]() -> int {
         if (!ctx->have_harmonic || !ctx->have_fc3) return ctx->fail(5, "run tc3_harmonic and set fc3 first");
         if (nT <= 0 || !temperatures) return ctx->fail(5, "need at least one temperature");
-        if (integrationtype != 1 && integrationtype != 2)
-            return ctx->fail(5, "integrationtype must be 1 or 2");
+        if (integrationtype != 1 && integrationtype != 2 && integrationtype != 6)
+            return ctx->fail(5, "integrationtype must be 1, 2 or 6");
         if (precision < 0 || precision > 2) return ctx->fail(5, "unknown precision mode");
         if (ctx->omega_min < -kFreqTol) return ctx->fail(3, "FOUND UNSTABLE MODES");
         const int nb = ctx->nb, nirr = ctx->mesh.nirr;
@@ -365,6 +365,7 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
             }
         const double base = *std::max_element(sigavg.begin(), sigavg.end()) * 1e-4;
         for (double &s : ctx->sigsq) s += base;
+        ctx->thresh6 = base;
         int r = run_scatter(ctx, precision, integrationtype, sigma);
         if (r) return r;
         ctx->have_scatter = true;

@@ -507,7 +507,8 @@ size_t psi_lds() {
     s += (PREC == 2) ? (size_t)KS * kPlanes * 64 * 16 : 0;    // E1 fragments
     s += sizeof(double) * (NB + 16 + 16 * NTY) + sizeof(float) * NB + sizeof(int) * 16 +
          sizeof(unsigned) * (size_t)(NB + 16) * NMW;
-    return s;
+    s = (s + 15) & ~(size_t)15;
+    return s + sizeof(S) * 3 * (16 + 2 * NB);    // reduced velocities (integration type 6)
 }
 
 template <int PREC, int NB, int NTY>
@@ -549,6 +550,17 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     int *rows = (int *)(w2f + NB);         // [16]
     unsigned *v23 = (unsigned *)(rows + 16);   // [NB][NMW]  |w2 - w3| >= tol, by b3
     unsigned *v12 = v23 + NB * NMW;            // [16][NMW]  b2 allowed for b1, by b2
+    // [16 b1, NB b2, NB b3][3] reduced group velocities (integration type 6)
+    S *u6 = (S *)(smem + align16((size_t)((unsigned char *)(v12 + 16 * NMW) - smem)));
+    const bool itype6 = a.integrationtype == 6;
+    if (itype6)
+        for (int x = threadIdx.x; x < 3 * (16 + 2 * NB); x += blockDim.x) {
+            const int m = x / 3, cc = x % 3;
+            const double v = (m < 16)        ? a.u_irr[((size_t)q1 * NB + 16 * bb + m) * 3 + cc]
+                             : (m < 16 + NB) ? a.u_full[((size_t)q2 * NB + m - 16) * 3 + cc]
+                                             : a.u_full[((size_t)q3 * NB + m - 16 - NB) * 3 + cc];
+            u6[x] = (S)v;
+        }
     const double *om2 = a.om_full + (size_t)q2 * NB, *om3 = a.om_full + (size_t)q3 * NB;
     for (int x = threadIdx.x; x < nT * NB; x += blockDim.x) {
         const int T_ = x / NB, b = x % NB;
@@ -665,7 +677,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     constexpr int NLW = (NB + 63) / 64;
     unsigned long long live[NLW];
     {
-        double w1lo = 1e300, w1hi = -1e300, s1m = 0, w3lo = 1e300, w3hi = -1e300, s3m = 0;
+        double w1lo = 1e300, w1hi = -1e300, s1m = 0, w3lo = 1e300, w3hi = -1e300, s3m = 0, u1m = 0, u3m = 0;
         for (int m = 0; m < 16; ++m) {
             w1lo = fmin(w1lo, w1s[m]);
             w1hi = fmax(w1hi, w1s[m]);
@@ -673,6 +685,11 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
             w3lo = fmin(w3lo, om3[b3w + m]);
             w3hi = fmax(w3hi, om3[b3w + m]);
             s3m = fmax(s3m, a.sigsq[(size_t)i3 * NB + b3w + m]);
+            if (itype6)
+                for (int cc = 0; cc < 3; ++cc) {
+                    u1m = fmax(u1m, fabs(a.u_irr[((size_t)q1 * NB + 16 * bb + m) * 3 + cc]));
+                    u3m = fmax(u3m, fabs(a.u_full[((size_t)q3 * NB + b3w + m) * 3 + cc]));
+                }
         }
 #pragma unroll
         for (int wd = 0; wd < NLW; ++wd) {
@@ -680,8 +697,13 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
             bool lv = false;
             if (b2 < NB) {
                 const double w2 = w2s[b2];
+                // type 6: each pair width is at most sqrt(1/2) (|u_a|_inf + |u_b|_inf)
+                double u2m = 0;
+                if (itype6)
+                    for (int cc = 0; cc < 3; ++cc) u2m = fmax(u2m, fabs(a.u_full[((size_t)q2 * NB + b2) * 3 + cc]));
                 const double smax = (a.integrationtype == 2) ? sqrt(s1m + a.sigsq[(size_t)i2 * NB + b2] + s3m)
-                                                             : a.sigma_fixed;
+                                    : itype6 ? 0.7071067811865476 * (2.0 / 3.0) * (u1m + u2m + u3m) + a.thresh6
+                                             : a.sigma_fixed;
                 const double L = 8.0 * smax * (1.0 + 1e-4) + 1e-6 * (fabs(w1hi) + fabs(w2) + fabs(w3hi));
                 auto hit = [&](double lo, double hi) { return lo <= L && hi >= -L; };
                 lv = hit(w1lo + w2 - w3hi, w1hi + w2 - w3lo) || hit(w1lo - w2 + w3lo, w1hi - w2 + w3hi) ||
@@ -782,7 +804,12 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         for (int r = 0; r < 4; ++r) {
             const S re = (S)cr[r], im = (S)ci[r];
             const S psisq = sc * (re * re + im * im);
-            const S sig2 = itype2 ? sg12 + s3v[r] : sfix2;
+            S sig2 = itype2 ? sg12 + s3v[r] : sfix2;
+            if (itype6) {
+                const S s6 = sigma_type6<S>(u6 + 3 * c16, u6 + 3 * (16 + b2), u6 + 3 * (16 + NB + b3w + out_row<PREC>(lane, r)),
+                                            (S)a.thresh6);
+                sig2 = s6 * s6;
+            }
             S d0, d1;
             if constexpr (PREC == 0)
                 delta_pair(w1, x2, w3v[r], sig2, d0, d1);
@@ -980,6 +1007,7 @@ size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     ps = ((ps + 15) & ~(size_t)15);
     if (precision == TC3_PREC_BF16X3) ps += ks * kPlanes * 64 * 16;
     ps += 8 * (nb + 16 + 16 * nty) + 4 * nb + 4 * 16 + 4 * (nb + 16) * nmw;
+    ps = ((ps + 15) & ~(size_t)15) + es * 3 * (16 + 2 * (size_t)nb);
     return std::max(uw, ps);
 }
 

@@ -22,6 +22,8 @@ struct MfmaEpiArgs {
     // temperature of the chunk, comp_T elements per temperature.
     void *comp;
     size_t comp_T;
+    const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
+    double thresh6;                 // and the baseline width
 };
 
 struct MfmaPlan {

@@ -38,7 +38,19 @@ struct EpiArgs {
     double *gpart;   // [t][T][nb]
     size_t xi_row, xi_T;
     int nirr;
+    const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
+    double thresh6;                 // and the baseline width
 };
+
+// Reduced group velocities u_j = sum_i v_i reclat_ij / dims_j of every mode (the operand of
+// the integration type 6 widths, scattering.f90:138-140).
+struct Mat3 { double m[9]; };
+__global__ void k_reduced_vel(long nmode, const double *__restrict__ vel, Mat3 r, double *__restrict__ u) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nmode; i += (long)gridDim.x * blockDim.x) {
+        const double *v = vel + 3 * i;
+        for (int j = 0; j < 3; ++j) u[3 * i + j] = v[0] * r.m[j] + v[1] * r.m[3 + j] + v[2] * r.m[6 + j];
+    }
+}
 
 // |Psi|^2 -> smeared-delta weighted rates for every temperature; linewidth partials and
 // scattering-matrix rows.  Workgroup per (triplet, b1).  psi_ld: stride of b1 rows.
@@ -83,6 +95,9 @@ __global__ void k_epilogue(EpiArgs a, const P *__restrict__ psi, int psi_complex
                 double sig;
                 if (a.integrationtype == 2)
                     sig = sqrt(sg1 + a.sigsq[i2 * nb + b2] + a.sigsq[i3 * nb + b3]);
+                else if (a.integrationtype == 6)
+                    sig = sigma_type6(a.u_irr + ((size_t)q1 * nb + b1) * 3, a.u_full + ((size_t)q2 * nb + b2) * 3,
+                                      a.u_full + ((size_t)q3 * nb + b3) * 3, a.thresh6);
                 else
                     sig = a.sigma_fixed;
                 const double d0 = gauss(w1, -w2 + w3, sig) - gauss(w1, w2 - w3, sig);
@@ -261,9 +276,13 @@ __global__ void __launch_bounds__(256) k_small(EpiArgs a, int ntrip, int *__rest
                     !(fabs(w1 - w3) < 1e-9) && !(fabs(w2 - w3) < 1e-9)) {
                     const double re = (double)phi[e].x, im = (double)phi[e].y;
                     const double psisq = a.psi_scale * (re * re + im * im);
-                    const double sig = (a.integrationtype == 2)
-                                           ? sqrt(a.sigsq[q1 * NB + b1] + a.sigsq[i2 * NB + b2] + a.sigsq[i3 * NB + b3])
-                                           : a.sigma_fixed;
+                    const double sig =
+                        (a.integrationtype == 2)
+                            ? sqrt(a.sigsq[q1 * NB + b1] + a.sigsq[i2 * NB + b2] + a.sigsq[i3 * NB + b3])
+                        : (a.integrationtype == 6)
+                            ? sigma_type6(a.u_irr + ((size_t)q1 * NB + b1) * 3, a.u_full + ((size_t)q2 * NB + b2) * 3,
+                                          a.u_full + ((size_t)q3 * NB + b3) * 3, a.thresh6)
+                            : a.sigma_fixed;
                     const double d0 = gauss(w1, -w2 + w3, sig) - gauss(w1, w2 - w3, sig);
                     const double d1 = gauss(w1, w2 + w3, sig) - gauss(w1, -w2 - w3, sig);
                     const double n2 = be[i2 * NB + b2], n3 = be[i3 * NB + b3];
@@ -336,6 +355,8 @@ struct IsoArgs {
     const int *row_b, *irr_of_full;
     double sigma_fixed, pref;      // pi/4 / N_q
     int na, integrationtype;
+    const double *u_full;          // integration type 6: reduced group velocities, and baseline
+    double thresh6;
     double *gam;                   // [row] linewidth term (written by the T = 0 workgroup)
 };
 
@@ -411,9 +432,10 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
                         }
                         st += f0 * f0 * gsh[a];
                     }
-                    const double sig = (iso.integrationtype == 2)
-                                           ? sqrt(sg1 + iso.sigsq[(size_t)iso.irr_of_full[q2] * nb + b])
-                                           : iso.sigma_fixed;
+                    const double sig =
+                        (iso.integrationtype == 2) ? sqrt(sg1 + iso.sigsq[(size_t)iso.irr_of_full[q2] * nb + b])
+                        : (iso.integrationtype == 6) ? sigma_type6_iso(iso.u_full + ((size_t)q2 * nb + b) * 3, iso.thresh6)
+                                                     : iso.sigma_fixed;
                     const double f = st * iso.pref * om1 * om2 * gauss_iso(om1, om2, sig);
                     gsum += f;
                     v += f * om2 / om1;
@@ -832,8 +854,21 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     for (int a = 0; a < na; ++a) maxa1 = std::max(maxa1, c->a1_off[a + 1] - c->a1_off[a]);
     const size_t phi_shmem = sizeof(cd) * std::max(maxa1, 1);
 
+    // integration type 6: reduced group velocities of the irreducible and full meshes
+    double *d_uirr = nullptr, *d_ufull = nullptr;
+    if (integrationtype == 6) {
+        Mat3 r;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) r.m[3 * i + j] = c->cr.rec[3 * i + j] / c->mesh.dims[j];
+        TRY(c->pool_get("u_irr", sizeof(double) * 3 * nirr * nb, (void **)&d_uirr));
+        TRY(c->pool_get("u_full", sizeof(double) * 3 * nfull * nb, (void **)&d_ufull));
+        hipLaunchKernelGGL(k_reduced_vel, dim3(256), dim3(256), 0, st, (long)nirr * nb, c->d_vel_irr, r, d_uirr);
+        hipLaunchKernelGGL(k_reduced_vel, dim3(1024), dim3(256), 0, st, (long)nfull * nb, c->d_vel_full, r, d_ufull);
+        TRY(hipGetLastError());
+    }
     EpiArgs ea;
     ea.nb = nb; ea.nT = nT; ea.nfull = nfull; ea.integrationtype = integrationtype;
+    ea.u_irr = d_uirr; ea.u_full = d_ufull; ea.thresh6 = c->thresh6;
     ea.sigma_fixed = sigma * kFreqThzToHartree;
     ea.psi_scale = (kPi / 16.0) / (double)nfull;
     ea.tq1 = d_tq1; ea.tq2 = d_tq2; ea.tq3 = d_tq3; ea.eq_off = d_eqoff; ea.eq_q2 = d_eq2; ea.eq_q3 = d_eq3;
@@ -921,6 +956,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         me.sigma_fixed = ea.sigma_fixed; me.psi_scale = ea.psi_scale;
         me.tq1 = d_tq1; me.tq2 = d_tq2; me.tq3 = d_tq3; me.eq_off = d_eqoff; me.eq_q2 = d_eq2; me.eq_q3 = d_eq3;
         me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
+        me.u_irr = d_uirr; me.u_full = d_ufull; me.thresh6 = c->thresh6;
         me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
         me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
         // compact per-triplet sums, assembled into the rows by k_xi_asm, when every entry
@@ -1108,6 +1144,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         iso.row_b = d_rowb; iso.irr_of_full = c->d_irr_of_full;
         iso.sigma_fixed = ea.sigma_fixed; iso.pref = (kPi / 4.0) / (double)nfull;
         iso.na = na; iso.integrationtype = integrationtype;
+        iso.u_full = d_ufull; iso.thresh6 = c->thresh6;
     }
     {
         const dim3 g(nrows * nT), b(kBlock), bl(nb <= kBlock ? nb * (kBlock / nb) : kBlock);

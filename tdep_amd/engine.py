@@ -162,6 +162,9 @@ class Engine:
     # ----------------------------------------------------------- scattering
     def scattering_rates(self, temperatures, precision="fp64", integrationtype=2, sigma=1.0,
                          classical=False):
+        """Three-phonon (plus isotope and boundary) scattering for all temperatures at once.
+        integrationtype: 1 fixed Gaussian of width sigma (THz), 2 adaptive, 6 adaptive from
+        group-velocity differences (scattering_threephonon.f90:242-262)."""
         T = np.ascontiguousarray(np.atleast_1d(temperatures), dtype=np.float64)
         self.temperatures = T
         self.classical = bool(classical)

@@ -13,7 +13,7 @@
 ! Options (options.f90:44-237, the subset of this path):
 !   -qg n1 n2 n3 | --qpoint_grid n1 n2 n3     q-mesh (26 26 26)
 !   --temperature T [T ...]                    temperatures in K (300)
-!   -it 1|2 | --integrationtype 1|2            fixed or adaptive Gaussian (2)
+!   -it 1|2|6 | --integrationtype 1|2|6        fixed, adaptive or velocity-difference Gaussian (2)
 !   --sigma s                                  fixed smearing in THz, integrationtype 1 (1.0)
 !   --max_mfp L                                boundary scattering, mean free path limit in m (off)
 !   --iterative_tolerance x                    BTE tolerance (1e-5)

@@ -186,3 +186,19 @@ def test_gradient_with_respect_to_symmetry_adapted_coefficients(inputs_dir):
                 fd = (float(f(torch.from_numpy(xp))) - float(f(torch.from_numpy(xm)))) / (2 * h)
             print(f"[adjoint] coefficient {c}: fd {fd:.10e} adjoint {g[c]:.10e}", flush=True)
             assert abs(fd - g[c]) <= tol * abs(g[c])
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (4, 4, 4)), ("sl16", (4, 4, 2))])
+def test_adjoint_gamma_integration_type6_matches_oracle(inputs_dir, oracle_setup, name, grid):
+    """The adjoint with the velocity-difference widths of integration type 6."""
+    from oracle import adjoint as A
+    p, fc2, fc3, ops, mesh, disp = pre = oracle_setup(name, grid)
+    e = _engine(inputs_dir, name, grid)
+    T = 300.0
+    e.scattering_rates([T], precision="fp64", integrationtype=6)
+    rng = np.random.default_rng(9)
+    dl = rng.standard_normal((1, e.nirr, e.nb))
+    g = e.adjoint_gamma(dl)
+    ref = A.mwm_to_m(p, fc3, A.gamma_adjoint(p, fc3, mesh, disp, ops, T, A.raw_cotangent(disp, dl[0]),
+                                             integrationtype=6))
+    assert _nrel(g, ref) < 1e-10

@@ -411,3 +411,26 @@ def test_boundary_term_is_velocity_over_length_before_degeneracy_average(oracle_
     # a huge length leaves the three-phonon result unchanged to rounding
     sr2 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, mfp_max=1e30)
     np.testing.assert_allclose(sr2.linewidth, sr0.linewidth, rtol=1e-14, atol=0)
+
+
+def test_velocity_difference_widths_pinned():
+    """Integration type 6 widths (scattering_threephonon.f90:251-261): symmetric in the three
+    modes, exactly the baseline for equal velocities, the closed form for one moving mode,
+    and v reclat taken as sum_i v_i reclat_ij (matmul(v, reclat))."""
+    rng = np.random.default_rng(3)
+    R = rng.normal(size=(3, 3))
+    v1, v2, v3 = rng.normal(size=(4, 3)), rng.normal(size=(5, 3)), rng.normal(size=(6, 3))
+    w = S.velocity_difference_widths(v1, v2, v3, R, 0.01)
+    np.testing.assert_allclose(S.velocity_difference_widths(v2, v1, v3, R, 0.01), w.transpose(1, 0, 2), rtol=1e-14)
+    np.testing.assert_allclose(S.velocity_difference_widths(v1, v3, v2, R, 0.01), w.transpose(0, 2, 1), rtol=1e-14)
+    np.testing.assert_allclose(S.velocity_difference_widths(v3, v2, v1, R, 0.01), w.transpose(2, 1, 0), rtol=1e-14)
+    same = np.array([[0.3, -0.2, 0.7]])
+    assert S.velocity_difference_widths(same, same, same, R, 0.01)[0, 0, 0] == 0.01
+    x = 0.37
+    one = S.velocity_difference_widths(np.array([[x, -2 * x, 0.5 * x]]), np.zeros((1, 3)), np.zeros((1, 3)),
+                                       np.eye(3), 0.0)
+    assert abs(one[0, 0, 0] - 2.0 * np.sqrt(2.0) * x / 3.0) < 1e-15
+    # orientation: with R = [[1, 2, 0], [0, 1, 0], [0, 0, 1]], v = (1, 0, 0) maps to (1, 2, 0)
+    Rn = np.array([[1.0, 2.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    o = S.velocity_difference_widths(np.array([[1.0, 0.0, 0.0]]), np.zeros((1, 3)), np.zeros((1, 3)), Rn, 0.0)
+    assert abs(o[0, 0, 0] - 2.0 * np.sqrt(0.5 * 4.0) / 3.0) < 1e-15
