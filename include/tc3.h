@@ -118,9 +118,10 @@ int tc3_get_harmonic(const tc3_ctx *ctx, int full, double *omega, double *vel, d
  * replaces lo_scattering_rates%generate (scattering.f90:58-358) with
  * compute_threephonon_scattering (scattering_threephonon.f90:2-217) for
  * qg3ph = qgrid, plus isotope scattering when tc3_set_isotopes enabled it
- * (compute_isotope_scattering, scattering_isotope.f90:2-70); no boundary term.  integrationtype 1 (fixed sigma
- * in THz) or 2 (adaptive); precision one of TC3_PREC_*.  The scattering matrix stays
- * on the device. */
+ * (compute_isotope_scattering, scattering_isotope.f90:2-70) and boundary scattering when
+ * tc3_set_boundary set a length (:245-250).  integrationtype 1 (fixed sigma in THz), 2
+ * (adaptive) or 6 (adaptive from group-velocity differences, scattering_threephonon.f90:
+ * 251-261); precision one of TC3_PREC_*.  The scattering matrix stays on the device. */
 int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
                 int integrationtype, double sigma, int classical);
 
