@@ -80,6 +80,15 @@ int tc3_set_isotopes(tc3_ctx *ctx, const double *disorder, int enable);
  * opts%mfp_max input of lo_scattering_rates%generate. */
 int tc3_set_boundary(tc3_ctx *ctx, double mfp_max);
 
+/* Monte-Carlo integration grid of the three-phonon term (-qg3ph, --seed; options.f90:116,
+ * 226-230; lo_montecarlo_grid, lo_fftgrid_helper.f90:30-99): for every row (q1, b1) a
+ * stratified random draw of prod(qg3ph) points of the q-mesh from the reference's 64-bit
+ * Mersenne twister seeded as lo_mersennetwister%init(0, 1/seed) (one MPI rank), each draw
+ * weighted 1/prod(qg3ph).  qg3ph NULL, or every entry outside 1..qgrid, means the full
+ * mesh; a Monte-Carlo grid needs seed > 0 (the reference's wall-clock seed is not
+ * reproducible).  Set after tc3_set_qmesh. */
+int tc3_set_montecarlo(tc3_ctx *ctx, const int *qg3ph, int seed);
+
 /* Space group, Gamma-centred FFT q-mesh, irreducible points, little groups and the
  * irreducible triplet lists: replaces uc%classify('wedge') (type_crystalstructure_symmetry.f90:7-91),
  * lo_generate_qmesh (type_qpointmesh.f90:452-657, type_qpointmesh_gridgeneration.f90:8-545)

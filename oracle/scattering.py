@@ -197,11 +197,15 @@ def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0
 
 
 def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
-                     integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None, mfp_max=None):
+                     integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None, mfp_max=None,
+                     qg3ph=None, seed=None):
     """lo_scattering_rates%generate (scattering.f90:58-358): three-phonon scattering with
     qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q), plus
     isotope scattering (:229-232) when per-atom mass-disorder parameters are given, plus
-    boundary scattering |v| / mfp_max (Bohr, :245-250) when mfp_max > 0."""
+    boundary scattering |v| / mfp_max (Bohr, :245-250) when mfp_max > 0.  With qg3ph smaller
+    than the mesh, every row (q1, b1) visits the triplets of its own Monte-Carlo q2 draw
+    (compute_threephonon_scattering :69-78, montecarlo.row_grids), each draw weighted
+    1 / prod(qg3ph)."""
     nb = disp.nb
     nirr, nfull = mesh.nirr, mesh.nfull
     sr = ScatteringRates()
@@ -219,6 +223,13 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
     rot = rotation_table(p, mesh, ops)
     sr.rot = rot
     mcw = 1.0 / nfull
+    grids = None
+    if qg3ph is not None:
+        from . import montecarlo as MC
+        mcd = MC.mc_dims_of(list(mesh.dims), list(qg3ph))
+        if mcd != list(mesh.dims):
+            grids, npts = MC.row_grids(list(mesh.dims), rows, mcd, seed)
+            mcw = 1.0 / npts
     ntrip = 0
     for q1 in range(nirr):
         myb = [b for b in range(nb) if (q1, b) in row_of]
@@ -232,6 +243,12 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
         trips = irreducible_triplets(mesh, rot, q1)
         ntrip += len(trips)
         for (q2, q3, perm, red) in trips:
+            if grids is not None:
+                # draw count of q2 for every row of q1 (zero: the row does not visit it)
+                cnt = np.array([np.count_nonzero(grids[(q1, b)] == q2) if (q1, b) in grids else 0
+                                for b in range(nb)], dtype=float)
+                if not np.any(cnt[myb]):
+                    continue
             om2 = disp.om_full[q2]; om3 = disp.om_full[q3]
             E2 = disp.egv_full[q2] / np.sqrt(np.where(om2 > 0, om2, 1.0))[None, :]
             E3 = disp.egv_full[q3] / np.sqrt(np.where(om3 > 0, om3, 1.0))[None, :]
@@ -252,6 +269,8 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
             d0 = U.gauss(w1, -w2 + w3, sig) - U.gauss(w1, w2 - w3, sig)
             d1 = U.gauss(w1, w2 + w3, sig) - U.gauss(w1, -w2 - w3, sig)
             f = perm * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1)
+            if grids is not None:
+                f = f * cnt[:, None, None]
             mask = ((w2 >= U.FREQTOL) & (np.abs(w1 - w2) >= U.FREQTOL)
                     & (w3 >= U.FREQTOL) & (np.abs(w1 - w3) >= U.FREQTOL)
                     & (np.abs(w2 - w3) >= U.FREQTOL))

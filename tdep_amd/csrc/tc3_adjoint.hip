@@ -35,6 +35,7 @@ struct AdjArgs {
     const double *g;                // [T][row] cotangent of the raw linewidths
     const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
     double thresh6;
+    const float *tw;                // Monte-Carlo draw counts [t][b1], or null (all 1)
 };
 
 // Z[tl,b1](b2,b3) = H conj(Psi) in place; H as in the file comment, zero where the forward
@@ -55,7 +56,8 @@ __global__ void k_adj_weight(AdjArgs a, cplx<T> *__restrict__ psi) {
     const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
     const double w1 = a.om_irr[q1 * nb + b1];
     const double sg1 = a.sigsq[q1 * nb + b1];
-    const double scale = a.perm[t] * a.psi_scale * (double)(a.eq_off[t + 1] - a.eq_off[t]);
+    const double scale = a.perm[t] * a.psi_scale * (double)(a.eq_off[t + 1] - a.eq_off[t]) *
+                         (a.tw ? (double)a.tw[(size_t)t * nb + b1] : 1.0);
     for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
         const int b2 = e / nb, b3 = e % nb;
         const double w2 = a.om_full[(size_t)q2 * nb + b2], w3 = a.om_full[(size_t)q3 * nb + b3];
@@ -290,7 +292,13 @@ int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<do
         aa.u_irr = ui;
         aa.u_full = uf;
     }
-    aa.psi_scale = (kPi / 16.0) / (double)nfull;
+    aa.psi_scale = (kPi / 16.0) / (double)(c->trip_w.empty() ? (long)nfull : c->mc_npoints);
+    aa.tw = nullptr;
+    if (!c->trip_w.empty()) {
+        float *tw;
+        TRY(table("tw", (void **)&tw, c->trip_w.data(), sizeof(float) * c->trip_w.size(), upload));
+        aa.tw = tw;
+    }
     aa.tq1 = d_tq1; aa.tq2 = d_tq2; aa.tq3 = d_tq3; aa.eq_off = d_eqoff; aa.irr_of_full = c->d_irr_of_full;
     aa.row_of = d_rowof; aa.perm = d_perm; aa.om_irr = c->d_om_irr; aa.om_full = c->d_om_full; aa.be = d_be;
     aa.sigsq = d_sigsq; aa.g = d_g;

@@ -209,13 +209,40 @@ int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal) {
         if (s.code) return ctx->fail(s.code, s.msg);
         s = build_mesh(ctx->cr, ctx->ops, dims, timereversal != 0, ctx->mesh);
         if (s.code) return ctx->fail(s.code, s.msg);
-        s = build_triplets(ctx->mesh, ctx->trip);
+        s = build_triplets(ctx->mesh, ctx->trip_full);
         if (s.code) return ctx->fail(s.code, s.msg);
+        ctx->trip = ctx->trip_full;
+        ctx->trip_w.clear();
+        ctx->mc_gen = -1;
         s = build_orbits(ctx->mesh, ctx->orb);
         if (s.code) return ctx->fail(s.code, s.msg);
         ctx->have_mesh = true;
         ++ctx->gen_model;
         ctx->have_harmonic = ctx->have_scatter = false;
+        return 0;
+    });
+}
+
+int tc3_set_montecarlo(tc3_ctx *ctx, const int *qg3ph, int seed) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->have_mesh) return ctx->fail(5, "set the q-mesh first");
+        bool on = qg3ph != nullptr;
+        if (on) {
+            on = false;
+            for (int i = 0; i < 3; ++i) {
+                // options.f90:226-230: outside 1..qgrid means the full grid
+                const int d = (qg3ph[i] < 1 || qg3ph[i] > ctx->mesh.dims[i]) ? ctx->mesh.dims[i] : qg3ph[i];
+                ctx->mc_dims[i] = d;
+                on = on || d < ctx->mesh.dims[i];
+            }
+        }
+        if (on && seed <= 0)
+            return ctx->fail(5, "Monte-Carlo grids need a positive seed (a wall-clock seed is not reproducible)");
+        ctx->mc_on = on;
+        ctx->mc_seed = seed;
+        ctx->mc_gen = -1;
+        ++ctx->gen_model;
+        ctx->have_scatter = false;
         return 0;
     });
 }
@@ -366,6 +393,18 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
         const double base = *std::max_element(sigavg.begin(), sigavg.end()) * 1e-4;
         for (double &s : ctx->sigsq) s += base;
         ctx->thresh6 = base;
+        // triplets of the pass: the Monte-Carlo subset with its draw counts, or all
+        if (ctx->mc_on) {
+            if (ctx->mc_gen != ctx->gen_model) {
+                Status ms = montecarlo_triplets(ctx->mesh, ctx->trip_full, ctx->om_irr, nb, ctx->mc_dims, ctx->mc_seed,
+                                                ctx->trip, ctx->trip_w, ctx->mc_npoints);
+                if (ms.code) return ctx->fail(ms.code, ms.msg);
+                ctx->mc_gen = ctx->gen_model;
+            }
+        } else if (!ctx->trip_w.empty()) {
+            ctx->trip = ctx->trip_full;
+            ctx->trip_w.clear();
+        }
         int r = run_scatter(ctx, precision, integrationtype, sigma);
         if (r) return r;
         ctx->have_scatter = true;

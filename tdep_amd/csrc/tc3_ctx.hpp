@@ -34,7 +34,14 @@ struct tc3_ctx {
     std::vector<int> t_src;                   // caller's index of each sorted triplet
 
     tc3::Mesh mesh;
-    tc3::TripletSet trip;
+    tc3::TripletSet trip;        // triplets a pass visits (the full list, or the Monte-Carlo subset)
+    tc3::TripletSet trip_full;   // every irreducible triplet of the mesh
+    // Monte-Carlo q2 grids (-qg3ph, --seed): draw counts per (triplet, b1) of the subset
+    int mc_dims[3] = {0, 0, 0};
+    int mc_seed = 0;
+    bool mc_on = false;
+    long mc_gen = -1, mc_npoints = 0;
+    std::vector<float> trip_w;
     tc3::OrbitSet orb;
 
     // harmonic results (host copies); egv stored [q][component][mode]

@@ -510,3 +510,127 @@ Status build_orbits(const Mesh &m, OrbitSet &o) {
 }
 
 }  // namespace tc3
+
+namespace tc3 {
+
+void MersenneTwister64::init_int(uint64_t seed) {
+    mt[0] = seed;
+    for (int i = 1; i < 312; ++i) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+    mti = 312;
+}
+
+void MersenneTwister64::init_tdep(int iseed, double rseed) {
+    // lo_randomnumbers.f90:216-227
+    double f0 = rseed - std::floor(rseed) - 0.5;
+    if (iseed > 0) f0 = std::fmod(f0 + std::log((double)iseed), 1.0) - 0.5;
+    else if (iseed < 0) f0 = std::fmod(f0 - std::log((double)-iseed), 1.0) - 0.5;
+    const int64_t s = (int64_t)std::round(f0 * (double)(1 << 30));
+    init_int((uint64_t)s);
+}
+
+uint64_t MersenneTwister64::next() {
+    constexpr int NN = 312, MM = 156;
+    constexpr uint64_t MATRIX_A = 0xB5026F5AA96619E9ULL, UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL;
+    if (mti >= NN) {
+        auto mag = [](uint64_t x) { return (x & 1ULL) ? MATRIX_A : 0ULL; };
+        int i = 0;
+        for (; i < NN - MM; ++i) {
+            const uint64_t x = (mt[i] & UM) | (mt[i + 1] & LM);
+            mt[i] = mt[i + MM] ^ (x >> 1) ^ mag(x);
+        }
+        for (; i < NN - 1; ++i) {
+            const uint64_t x = (mt[i] & UM) | (mt[i + 1] & LM);
+            mt[i] = mt[i + (MM - NN)] ^ (x >> 1) ^ mag(x);
+        }
+        const uint64_t x = (mt[NN - 1] & UM) | (mt[0] & LM);
+        mt[NN - 1] = mt[MM - 1] ^ (x >> 1) ^ mag(x);
+        mti = 0;
+    }
+    uint64_t x = mt[mti++];
+    x ^= (x >> 29) & 0x5555555555555555ULL;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+    x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+    x ^= (x >> 43);
+    return x;
+}
+
+double MersenneTwister64::real1() { return (double)(next() >> 11) * (1.0 / 9007199254740991.0); }
+
+Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vector<double> &om_irr, int nb,
+                           const int *mc_dims_in, int seed, TripletSet &sub, std::vector<float> &w, long &npoints) {
+    Status st;
+    int mcd[3];
+    double ratio[3];
+    for (int i = 0; i < 3; ++i) {
+        mcd[i] = std::max(1, std::min(mc_dims_in[i], m.dims[i]));
+        ratio[i] = (double)m.dims[i] / (double)mcd[i];
+    }
+    npoints = (long)mcd[0] * mcd[1] * mcd[2];
+    // scattering.f90:107-115, one MPI rank: integer seed 0, real seed 1 / seed
+    MersenneTwister64 rng;
+    rng.init_tdep(0, 1.0 / (double)seed);
+    // mc_point_to_full (lo_fftgrid_helper.f90:56-73), 0-based point indices
+    auto point_to_full = [&](long imc) {
+        const int k = (int)(imc % mcd[2]), j = (int)((imc / mcd[2]) % mcd[1]), i = (int)(imc / ((long)mcd[1] * mcd[2]));
+        const int gm[3] = {i, j, k};
+        int gf[3];
+        for (int d = 0; d < 3; ++d) {
+            const double r = rng.real1();
+            // 1-based ceiling((g - 1) ratio + r ratio); a draw of exactly 0 (probability 2^-53)
+            // is kept on its own cell
+            const int g = (int)std::ceil((double)gm[d] * ratio[d] + r * ratio[d]);
+            gf[d] = std::max(1, std::min(g, m.dims[d])) - 1;
+        }
+        return ((long)gf[0] * m.dims[1] + gf[1]) * m.dims[2] + gf[2];
+    };
+    const long np = m.nfull;
+    std::vector<long> tof(np);
+    std::vector<float> wfull;
+    wfull.assign(full.q1.size() * (size_t)nb, 0.0f);
+    std::vector<long> grid(npoints);
+    for (int q1 = 0; q1 < m.nirr; ++q1) {
+        std::fill(tof.begin(), tof.end(), -1);
+        for (int t = full.q1_off[q1]; t < full.q1_off[q1 + 1]; ++t) tof[full.q2[t]] = t;
+        for (int b1 = 0; b1 < nb; ++b1) {
+            if (om_irr[(size_t)q1 * nb + b1] < kFreqTol) continue;
+            // generate_grid (:75-99): no point equal to the previous one
+            grid[0] = point_to_full(0);
+            long prev = grid[0], test = grid[0];
+            for (long qi = 1; qi < npoints; ++qi) {
+                while (test == prev) test = point_to_full(qi);
+                grid[qi] = test;
+                prev = test;
+            }
+            for (long qi = 0; qi < npoints; ++qi) {
+                const long t = tof[grid[qi]];
+                if (t >= 0) wfull[(size_t)t * nb + b1] += 1.0f;
+            }
+        }
+    }
+    // triplets drawn for any row, in the order of the full list
+    sub = TripletSet();
+    sub.eq_off.push_back(0);
+    sub.q1_off.push_back(0);
+    w.clear();
+    for (int q1 = 0; q1 < m.nirr; ++q1) {
+        for (int t = full.q1_off[q1]; t < full.q1_off[q1 + 1]; ++t) {
+            bool any = false;
+            for (int b = 0; b < nb && !any; ++b) any = wfull[(size_t)t * nb + b] > 0.0f;
+            if (!any) continue;
+            sub.q1.push_back(full.q1[t]);
+            sub.q2.push_back(full.q2[t]);
+            sub.q3.push_back(full.q3[t]);
+            sub.perm.push_back(full.perm[t]);
+            for (int e = full.eq_off[t]; e < full.eq_off[t + 1]; ++e) {
+                sub.eq_q2.push_back(full.eq_q2[e]);
+                sub.eq_q3.push_back(full.eq_q3[e]);
+            }
+            sub.eq_off.push_back((int)sub.eq_q2.size());
+            w.insert(w.end(), wfull.begin() + (size_t)t * nb, wfull.begin() + (size_t)(t + 1) * nb);
+        }
+        sub.q1_off.push_back((int)sub.q1.size());
+    }
+    return st;
+}
+
+}  // namespace tc3

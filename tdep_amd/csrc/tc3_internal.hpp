@@ -96,6 +96,26 @@ Status build_mesh(const Crystal &c, const std::vector<SymOp> &ops, const int *di
                   bool timereversal, Mesh &m);
 int third_grid_index(int i1, int i2, const int *dims);
 Status build_triplets(const Mesh &m, TripletSet &t);
+
+// 64-bit Mersenne twister (MT19937-64, Matsumoto and Nishimura) with the seeding of
+// lo_mersennetwister%init (lo_randomnumbers.f90:204-236): a real seed and an integer seed
+// (the MPI rank) give the integer that starts init_genrand64.
+struct MersenneTwister64 {
+    uint64_t mt[312];
+    int mti = 313;
+    void init_int(uint64_t seed);
+    void init_tdep(int iseed, double rseed);
+    uint64_t next();
+    double real1();   // [0, 1]: (x >> 11) / (2^53 - 1), genrand64_real1
+};
+
+// Monte-Carlo q2 grids (lo_montecarlo_grid, lo_fftgrid_helper.f90:30-99): for every row (q1,
+// b1) with omega >= freqtol, in (q1, b1) order, one stratified draw of prod(mc_dims) points
+// of the full grid; the triplets of `full` whose first point is drawn form `sub`, and
+// w[t * nb + b1] counts how often triplet t is drawn for row (q1, b1).  npoints = the
+// number of points per draw (the weight of a draw is 1 / npoints).
+Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vector<double> &om_irr, int nb,
+                           const int *mc_dims, int seed, TripletSet &sub, std::vector<float> &w, long &npoints);
 Status build_orbits(const Mesh &m, OrbitSet &o);
 
 }  // namespace tc3

@@ -667,7 +667,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         for (int r = 0; r < 4; ++r) y[T_][r] = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) yM[r] = 0;
-    const S sc = (S)(a.psi_scale * a.perm[t]);
+    // this lane's b1: prefactor, permutation factor and Monte-Carlo draw count
+    const S sc = (S)(a.psi_scale * a.perm[t] * (a.tw ? (double)a.tw[(size_t)t * NB + 16 * bb + c16] : 1.0));
     const S sfix2 = (S)(a.sigma_fixed * a.sigma_fixed);
     const bool itype2 = a.integrationtype == 2;
     // Tiles (16 b1 x 16 b3 at one b2) whose every element lies beyond the Gaussians' 8 sigma

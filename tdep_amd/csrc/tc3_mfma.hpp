@@ -24,6 +24,7 @@ struct MfmaEpiArgs {
     size_t comp_T;
     const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
     double thresh6;                 // and the baseline width
+    const float *tw;                // Monte-Carlo draw counts [t][b1], or null (all 1)
 };
 
 struct MfmaPlan {

@@ -40,6 +40,7 @@ struct EpiArgs {
     int nirr;
     const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
     double thresh6;                 // and the baseline width
+    const float *tw;                // Monte-Carlo draw counts [t][b1], or null (all 1)
 };
 
 // Reduced group velocities u_j = sum_i v_i reclat_ij / dims_j of every mode (the operand of
@@ -71,7 +72,7 @@ __global__ void k_epilogue(EpiArgs a, const P *__restrict__ psi, int psi_complex
     }
     const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
     const double w1 = a.om_irr[q1 * nb + b1];
-    const double perm = a.perm[t];
+    const double perm = a.perm[t] * (a.tw ? (double)a.tw[(size_t)t * nb + b1] : 1.0);
     const double sg1 = a.sigsq[q1 * nb + b1];
     const int e0 = a.eq_off[t], e1 = a.eq_off[t + 1];
     const size_t base = ((size_t)tl * nb + b1) * nb * nb;
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(256) k_small(EpiArgs a, int ntrip, int *__rest
                     const double d0 = gauss(w1, -w2 + w3, sig) - gauss(w1, w2 - w3, sig);
                     const double d1 = gauss(w1, w2 + w3, sig) - gauss(w1, -w2 - w3, sig);
                     const double n2 = be[i2 * NB + b2], n3 = be[i3 * NB + b3];
-                    f = perm * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1);
+                    f = perm * (a.tw ? (double)a.tw[(size_t)t * NB + b1] : 1.0) * psisq * ((n2 - n3) * d0 + (n2 + n3 + 1.0) * d1);
                 }
                 fs[e] = f;
             }
@@ -870,7 +871,11 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     ea.nb = nb; ea.nT = nT; ea.nfull = nfull; ea.integrationtype = integrationtype;
     ea.u_irr = d_uirr; ea.u_full = d_ufull; ea.thresh6 = c->thresh6;
     ea.sigma_fixed = sigma * kFreqThzToHartree;
-    ea.psi_scale = (kPi / 16.0) / (double)nfull;
+    // weight of a draw: 1 / N_q, or 1 / npoints of the Monte-Carlo grid (mcg%weight)
+    ea.psi_scale = (kPi / 16.0) / (double)(c->trip_w.empty() ? (long)nfull : c->mc_npoints);
+    ea.tw = nullptr;
+    if (!c->trip_w.empty())
+        TRY(table("tw", (void **)&ea.tw, c->trip_w.data(), sizeof(float) * c->trip_w.size()));
     ea.tq1 = d_tq1; ea.tq2 = d_tq2; ea.tq3 = d_tq3; ea.eq_off = d_eqoff; ea.eq_q2 = d_eq2; ea.eq_q3 = d_eq3;
     ea.irr_of_full = c->d_irr_of_full; ea.row_of = d_rowof;
     ea.perm = d_perm; ea.om_irr = c->d_om_irr; ea.om_full = c->d_om_full; ea.be = d_be; ea.sigsq = d_sigsq;
@@ -957,6 +962,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         me.tq1 = d_tq1; me.tq2 = d_tq2; me.tq3 = d_tq3; me.eq_off = d_eqoff; me.eq_q2 = d_eq2; me.eq_q3 = d_eq3;
         me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
         me.u_irr = d_uirr; me.u_full = d_ufull; me.thresh6 = c->thresh6;
+        me.tw = ea.tw;
         me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
         me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
         // compact per-triplet sums, assembled into the rows by k_xi_asm, when every entry

@@ -71,6 +71,15 @@ class Engine:
         eng.set_qmesh(qgrid, timereversal)
         return eng
 
+    def set_montecarlo(self, qg3ph, seed):
+        """Monte-Carlo q2 grid of the three-phonon term (-qg3ph, --seed): a stratified draw of
+        prod(qg3ph) points per row; None (or qg3ph = qgrid) for the full mesh."""
+        if qg3ph is None:
+            self._check(self.lib.tc3_set_montecarlo(self.ctx, None, int(seed or 0)))
+        else:
+            q = np.ascontiguousarray(qg3ph, dtype=np.int32)
+            self._check(self.lib.tc3_set_montecarlo(self.ctx, iptr(q), int(seed)))
+
     def set_boundary(self, max_mfp):
         """Boundary scattering |v| / L with L = max_mfp in metres (the reference's --max_mfp,
         scattering.f90:245-250); None or a value <= 0 switches it off."""

@@ -59,6 +59,13 @@ module tc3_api
             integer(c_int), value :: enable
         end function
 
+        integer(c_int) function tc3_set_montecarlo(ctx, qg3ph, seed) bind(C, name="tc3_set_montecarlo")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(in) :: qg3ph(3)
+            integer(c_int), value :: seed
+        end function
+
         integer(c_int) function tc3_set_boundary(ctx, mfp_max) bind(C, name="tc3_set_boundary")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx

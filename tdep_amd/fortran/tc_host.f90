@@ -12,6 +12,8 @@
 !
 ! Options (options.f90:44-237, the subset of this path):
 !   -qg n1 n2 n3 | --qpoint_grid n1 n2 n3     q-mesh (26 26 26)
+!   -qg3ph n1 n2 n3 | --qpoint_grid3ph ...     Monte-Carlo q2 grid of the three-phonon term (mesh)
+!   --seed s                                   seed of the Monte-Carlo draws (needed with -qg3ph)
 !   --temperature T [T ...]                    temperatures in K (300)
 !   -it 1|2|6 | --integrationtype 1|2|6        fixed, adaptive or velocity-difference Gaussian (2)
 !   --sigma s                                  fixed smearing in THz, integrationtype 1 (1.0)
@@ -408,7 +410,7 @@ program tc_host
     real(c_double), parameter :: hartree_to_joule = 27.21138602_c_double*1.6021766208e-19_c_double
     real(c_double), parameter :: time_au_to_s = (6.582119514e-16_c_double*1.6021766208e-19_c_double)/hartree_to_joule
     real(c_double), parameter :: kappa_au_to_si = hartree_to_joule/time_au_to_s/0.52917721067e-10_c_double
-    integer(c_int) :: qg(3), integrationtype, maxsteps, prec, nT
+    integer(c_int) :: qg(3), qg3(3), integrationtype, maxsteps, prec, nT, seed
     real(c_double) :: sigma, tol, max_mfp
     real(c_double), allocatable :: temps(:)
     logical :: classical, noisotope, notr, offdiag, inputs_only
@@ -439,6 +441,7 @@ program tc_host
     call check(tc3_set_fc2(ctx, int(npair, c_int), p1, p2, lv, m2), 'tc3_set_fc2')
     call check(tc3_set_fc3(ctx, int(nfc3, c_int), t1, t2, t3, lv2, lv3, m3), 'tc3_set_fc3')
     call check(tc3_set_qmesh(ctx, qg, merge(0_c_int, 1_c_int, notr)), 'tc3_set_qmesh')
+    call check(tc3_set_montecarlo(ctx, qg3, seed), 'tc3_set_montecarlo')
     if (inputs_only) then
         call report_inputs()
     else
@@ -504,6 +507,8 @@ contains
         integer :: i, n, k
         real(c_double) :: tbuf(1024)
         qg = [26, 26, 26]
+        qg3 = [-1, -1, -1]
+        seed = 0
         integrationtype = 2
         sigma = 1.0_c_double
         max_mfp = -1.0_c_double
@@ -539,6 +544,16 @@ contains
                     i = i + 1
                 end do
                 if (nT == 0) call fatal(5, '--temperature needs at least one value')
+            case ('-qg3ph', '--qpoint_grid3ph')
+                do k = 1, 3
+                    call get_command_argument(i + k, val)
+                    read (val, *) qg3(k)
+                end do
+                i = i + 3
+            case ('--seed')
+                call get_command_argument(i + 1, val)
+                read (val, *) seed
+                i = i + 1
             case ('-it', '--integrationtype')
                 call get_command_argument(i + 1, val)
                 read (val, *) integrationtype

@@ -198,3 +198,29 @@ def test_integration_type6_matches_oracle(inputs_dir, oracle_setup, name, grid):
         assert _rel(e.linewidths(0)[0], sri.linewidth) < 1e-9
         assert _rel(e.scattering_matrix(0)[0], sri.Xi) < 1e-9
         e.set_isotopes(g, enable=False)
+
+
+@pytest.mark.parametrize("name,grid,qg3", [("fcc_al", (6, 6, 6), (3, 3, 3)), ("rocksalt", (4, 4, 4), (2, 2, 2)),
+                                           ("sl16", (4, 4, 2), (2, 2, 1))])
+def test_montecarlo_grid_matches_oracle(inputs_dir, oracle_setup, name, grid, qg3):
+    """Monte-Carlo q2 grids (-qg3ph, --seed 7): the host's 64-bit Mersenne twister and
+    stratified draws give the oracle's per-row triplets and weights, so FP64 linewidths and
+    scattering matrix agree to 1e-9 on the small-cell and MFMA paths; mixed kappa within 1e-4;
+    switching the grid off restores the full-mesh rates."""
+    from oracle import scattering as S
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
+    T = 300.0
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, T, qg3ph=qg3, seed=7)
+    e = _engine(inputs_dir, name, grid)
+    e.set_montecarlo(qg3, 7)
+    e.scattering_rates([T], precision="fp64")
+    lw, F0, _ = e.linewidths(0)
+    assert _rel(lw, sr.linewidth) < 1e-9
+    assert _rel(e.scattering_matrix(0)[0], sr.Xi) < 1e-9
+    k64 = e.kappa(offdiag=False)
+    e.scattering_rates([T], precision="bf16x3" if e.nb in (48, 96) else "fp32")
+    assert _rel(e.kappa(offdiag=False)[1], k64[1]) < 1e-4
+    e.set_montecarlo(None, 0)
+    e.scattering_rates([T], precision="fp64")
+    sr0 = S.scattering_rates(p, fc3, mesh, disp, ops, T)
+    assert _rel(e.linewidths(0)[0], sr0.linewidth) < 1e-9

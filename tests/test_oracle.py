@@ -434,3 +434,83 @@ def test_velocity_difference_widths_pinned():
     Rn = np.array([[1.0, 2.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
     o = S.velocity_difference_widths(np.array([[1.0, 0.0, 0.0]]), np.zeros((1, 3)), np.zeros((1, 3)), Rn, 0.0)
     assert abs(o[0, 0, 0] - 2.0 * np.sqrt(0.5 * 4.0) / 3.0) < 1e-15
+
+
+def test_mersenne_twister_matches_the_standard_library(tmp_path):
+    """The oracle's MT19937-64 (lo_randomnumbers.f90) against std::mt19937_64, whose seeding
+    is the same init_genrand64 recurrence: 2000 outputs for the default seed and for the
+    integers TDEP's seeding makes of --seed 1 and 7; the C++ standard's required value of the
+    10000th output of the default seed; genrand64_real1 on [0, 1]."""
+    import shutil
+    import subprocess
+    from oracle import montecarlo as MC
+    r = MC.MersenneTwister64()
+    r.init_int(5489)
+    v = [r.next() for _ in range(10000)]
+    assert v[9999] == 9981545732273789042
+    seeds = {}
+    for s in (1, 7):
+        t = MC.MersenneTwister64()
+        t.init_tdep(0, 1.0 / s)
+        seeds[s] = t.mt[0]
+    assert seeds[1] == (-536870912) & ((1 << 64) - 1)
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no C++ compiler")
+    src = tmp_path / "mt.cpp"
+    src.write_text("#include <random>\n#include <cstdio>\n#include <cstdlib>\nint main(int c, char **v) {\n"
+                   "  std::mt19937_64 g(std::strtoull(v[1], 0, 10));\n"
+                   "  for (int i = 0; i < 2000; ++i) std::printf(\"%llu\\n\", (unsigned long long)g());\n}\n")
+    exe = tmp_path / "mt"
+    subprocess.run([gxx, "-O1", "-o", str(exe), str(src)], check=True)
+    for s in (5489, seeds[1], seeds[7]):
+        out = [int(x) for x in subprocess.run([str(exe), str(s)], check=True, capture_output=True,
+                                               text=True).stdout.split()]
+        t = MC.MersenneTwister64()
+        t.init_int(s)
+        assert [t.next() for _ in range(2000)] == out
+    t = MC.MersenneTwister64()
+    t.init_tdep(0, 1.0)
+    x = [t.real1() for _ in range(1000)]
+    assert min(x) >= 0.0 and max(x) <= 1.0
+
+
+def test_montecarlo_grids_are_stratified(oracle_setup):
+    """lo_montecarlo_grid: point i of a draw lies in cell i of the coarse grid, consecutive
+    points differ, and with the coarse grid equal to the mesh each draw is the mesh in order."""
+    from oracle import montecarlo as MC
+    dims = [6, 6, 4]
+    mcd = [3, 2, 2]
+    rows = [(0, b) for b in range(5)]
+    grids, npts = MC.row_grids(dims, rows, mcd, seed=11)
+    assert npts == 12
+    ratio = [dims[i] // mcd[i] for i in range(3)]
+    for g in grids.values():
+        for imc, q in enumerate(g):
+            gm = (imc // (mcd[1] * mcd[2]), (imc // mcd[2]) % mcd[1], imc % mcd[2])
+            gf = (q // (dims[1] * dims[2]), (q // dims[2]) % dims[1], q % dims[2])
+            assert all(gm[d] * ratio[d] <= gf[d] < (gm[d] + 1) * ratio[d] for d in range(3))
+        assert np.all(np.diff(g) != 0)
+    assert len({tuple(g) for g in grids.values()}) == len(rows)       # a fresh draw per row
+    full, n = MC.row_grids(dims, rows, dims, seed=3)
+    for g in full.values():
+        assert np.array_equal(g, np.arange(n))
+
+
+def test_montecarlo_on_the_full_mesh_reproduces_the_deterministic_rates(oracle_setup):
+    """With a 'Monte-Carlo' grid equal to the mesh every row visits every triplet once with
+    weight 1/N_q: the rates equal the full-grid ones bit for bit; a coarser grid gives
+    different rates of the same magnitude in total (a sparse draw can miss every triplet
+    of a row, which then has no three-phonon rate)."""
+    from oracle import montecarlo as MC
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("fcc_al", (4, 4, 4))
+    sr0 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
+    rows = sr0.rows
+    grids, n = MC.row_grids(list(mesh.dims), rows, list(mesh.dims), seed=5)
+    assert all(np.array_equal(g, np.arange(n)) for g in grids.values())
+    # the forced Monte-Carlo path with the full grid
+    sr1 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, qg3ph=(4, 4, 4), seed=5)
+    np.testing.assert_array_equal(sr1.linewidth, sr0.linewidth)
+    sr2 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, qg3ph=(2, 2, 2), seed=5)
+    assert np.all(sr2.linewidth >= 0) and not np.allclose(sr2.linewidth, sr0.linewidth)
+    assert 0.3 < sr2.linewidth.sum() / sr0.linewidth.sum() < 3.0
