@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <cstdlib>
 
 #include "tc3_ctx.hpp"
 
@@ -114,82 +115,281 @@ void symmetrize_kappa(const tc3_ctx *c, double *k) {
     chop_kappa(k);
 }
 
-// get_kappa_offdiag with the coherence velocities formed as avg_op[egw^H G egw]
-// (entry for entry the product the reference forms through its Kronecker matrix).
-void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, double *kod) {
-    const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr;
-    double k[9] = {0};
-    std::vector<zc> G(3 * (size_t)nb * nb), egw((size_t)nb * nb), tmp((size_t)nb * nb), V(3 * (size_t)nb * nb);
+// get_kappa_offdiag (kappa.f90:77-306) with the coherence velocities formed as
+// avg_op[egw^H G egw] (entry for entry the product the reference forms through its
+// Kronecker matrix).  The velocities are temperature independent: V_z(q) = sum over the
+// little group of q of egw^H G_z egw, with G_z the z-derivative of the dynamical matrix
+// without masses, G[(a2, j)][(a1, i)] = sum_pairs i R_z exp(i q.R) phi_ij, and egw the
+// rotated eigenvectors scaled by 1 / sqrt(m 2 omega).  od_velocity_host forms one q on the
+// host; k_od_velocity forms them on the device.
+void od_velocity_host(const tc3_ctx *c, int iq, zc *V) {
+    const int na = c->cr.na, nb = c->nb;
+    std::vector<zc> G(3 * (size_t)nb * nb), egw((size_t)nb * nb), tmp((size_t)nb * nb);
     std::vector<double> mw(nb);
     for (int a = 0; a < na; ++a)
         for (int i = 0; i < 3; ++i) mw[3 * a + i] = c->cr.invsqrtmass[a];
-    const double vtol = 1e-10 / (kGroupvelAuToMs / 1000.0);
-    for (int iq = 0; iq < nirr; ++iq) {
-        const double *q = &c->mesh.qcart[3 * c->mesh.irr_full_index[iq]];
-        double qv[3];
-        for (int i = 0; i < 3; ++i) qv[i] = chop_scalar(q[i] * kTwoPi, 1e-13);
-        // gradient of the dynamical matrix without masses, indexed G[al][(a2,j)][(a1,i)]
-        std::fill(G.begin(), G.end(), zc(0, 0));
-        for (size_t p = 0; p < c->p_a1.size(); ++p) {
-            const int a1 = c->p_a1[p], a2 = c->p_a2[p];
-            const double *R = &c->p_lv[3 * p];
-            const double qr = R[0] * qv[0] + R[1] * qv[1] + R[2] * qv[2];
-            const zc e(std::cos(qr), std::sin(qr));
+    const double *q = &c->mesh.qcart[3 * c->mesh.irr_full_index[iq]];
+    double qv[3];
+    for (int i = 0; i < 3; ++i) qv[i] = chop_scalar(q[i] * kTwoPi, 1e-13);
+    for (size_t p = 0; p < c->p_a1.size(); ++p) {
+        const int a1 = c->p_a1[p], a2 = c->p_a2[p];
+        const double *R = &c->p_lv[3 * p];
+        const double qr = R[0] * qv[0] + R[1] * qv[1] + R[2] * qv[2];
+        const zc e(std::cos(qr), std::sin(qr));
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double m = c->p_m[9 * p + 3 * i + j];
+                for (int al = 0; al < 3; ++al)
+                    G[(size_t)al * nb * nb + (size_t)(3 * a2 + j) * nb + (3 * a1 + i)] += zc(0, R[al]) * e * m;
+            }
+    }
+    const double *om = &c->om_irr[(size_t)iq * nb];
+    std::fill(V, V + 3 * (size_t)nb * nb, zc(0, 0));
+    for (int o : c->mesh.littlegroup[iq]) {
+        const SymOp &op = c->ops[o];
+        // egw = Gamma(R) egv, scaled by invsqrtmass / sqrt(2 omega)
+        for (int a1 = 0; a1 < na; ++a1) {
+            int a2 = 0;
+            for (int x = 0; x < na; ++x)
+                if (op.fmap[x] == a1) a2 = x;
+            double d[3], v0[3];
+            for (int i = 0; i < 3; ++i) d[i] = c->cr.rcart[3 * a1 + i] - op.tr[i];
             for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) {
-                    const double m = c->p_m[9 * p + 3 * i + j];
-                    for (int al = 0; al < 3; ++al)
-                        G[(size_t)al * nb * nb + (size_t)(3 * a2 + j) * nb + (3 * a1 + i)] += zc(0, R[al]) * e * m;
+                v0[i] = op.m[i] * d[0] + op.m[3 + i] * d[1] + op.m[6 + i] * d[2] - c->cr.rcart[3 * a2 + i];
+            const double ph = -(v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2]) * kTwoPi;
+            const zc e(std::cos(ph), std::sin(ph));
+            for (int al = 0; al < 3; ++al)
+                for (int b = 0; b < nb; ++b) {
+                    zc s(0, 0);
+                    for (int be = 0; be < 3; ++be) {
+                        const cd x = c->egv_irr[((size_t)iq * nb + 3 * a2 + be) * nb + b];
+                        s += op.m[3 * al + be] * zc(x.x, x.y);
+                    }
+                    s *= e;
+                    const double sc = (om[b] > kFreqTol) ? mw[3 * a1 + al] / std::sqrt(om[b] * 2.0) : 0.0;
+                    egw[(size_t)(3 * a1 + al) * nb + b] = s * sc;
                 }
         }
-        const double *om = &c->om_irr[(size_t)iq * nb];
-        const auto &lg = c->mesh.littlegroup[iq];
-        std::fill(V.begin(), V.end(), zc(0, 0));
-        for (int o : lg) {
-            const SymOp &op = c->ops[o];
-            // egw = Gamma(R) egv, scaled by invsqrtmass / sqrt(2 omega)
-            for (int a1 = 0; a1 < na; ++a1) {
-                int a2 = 0;
-                for (int x = 0; x < na; ++x)
-                    if (op.fmap[x] == a1) a2 = x;
-                double d[3], v0[3];
-                for (int i = 0; i < 3; ++i) d[i] = c->cr.rcart[3 * a1 + i] - op.tr[i];
-                for (int i = 0; i < 3; ++i)
-                    v0[i] = op.m[i] * d[0] + op.m[3 + i] * d[1] + op.m[6 + i] * d[2] - c->cr.rcart[3 * a2 + i];
-                const double ph = -(v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2]) * kTwoPi;
-                const zc e(std::cos(ph), std::sin(ph));
-                for (int al = 0; al < 3; ++al)
-                    for (int b = 0; b < nb; ++b) {
-                        zc s(0, 0);
-                        for (int be = 0; be < 3; ++be) {
-                            const cd x = c->egv_irr[((size_t)iq * nb + 3 * a2 + be) * nb + b];
-                            s += op.m[3 * al + be] * zc(x.x, x.y);
-                        }
-                        s *= e;
-                        const double sc = (om[b] > kFreqTol) ? mw[3 * a1 + al] / std::sqrt(om[b] * 2.0) : 0.0;
-                        egw[(size_t)(3 * a1 + al) * nb + b] = s * sc;
-                    }
-            }
-            for (int z = 0; z < 3; ++z) {
-                // tmp = G egw ; V += egw^H tmp
-                const zc *Gz = &G[(size_t)z * nb * nb];
-                for (int r = 0; r < nb; ++r)
-                    for (int b = 0; b < nb; ++b) {
-                        zc s(0, 0);
-                        for (int x = 0; x < nb; ++x) s += Gz[(size_t)r * nb + x] * egw[(size_t)x * nb + b];
-                        tmp[(size_t)r * nb + b] = s;
-                    }
-                for (int i = 0; i < nb; ++i)
-                    for (int j = 0; j < nb; ++j) {
-                        zc s(0, 0);
-                        for (int r = 0; r < nb; ++r) s += std::conj(egw[(size_t)r * nb + i]) * tmp[(size_t)r * nb + j];
-                        V[(size_t)z * nb * nb + (size_t)i * nb + j] += s;
-                    }
-            }
+        for (int z = 0; z < 3; ++z) {
+            // tmp = G egw ; V += egw^H tmp
+            const zc *Gz = &G[(size_t)z * nb * nb];
+            for (int r = 0; r < nb; ++r)
+                for (int b = 0; b < nb; ++b) {
+                    zc s(0, 0);
+                    for (int x = 0; x < nb; ++x) s += Gz[(size_t)r * nb + x] * egw[(size_t)x * nb + b];
+                    tmp[(size_t)r * nb + b] = s;
+                }
+            for (int i = 0; i < nb; ++i)
+                for (int j = 0; j < nb; ++j) {
+                    zc s(0, 0);
+                    for (int r = 0; r < nb; ++r) s += std::conj(egw[(size_t)r * nb + i]) * tmp[(size_t)r * nb + j];
+                    V[(size_t)z * nb * nb + (size_t)i * nb + j] += s;
+                }
         }
-        const double pref = c->mesh.weight_irr[iq] / c->cr.volume;
-        const auto &so = c->mesh.star_ops[iq];
+    }
+}
+
+struct OdArgs {
+    int nb, na;
+    const double *qv, *qirr;        // [nirr][3]: 2 pi q (chopped) and q of the irreducible points
+    const cd *egv;                  // [q][component][mode]
+    const double *om;               // [q][mode]
+    const int *blk2;                // pairs of (a1, a2): [blk2[a1 na + a2], blk2[a1 na + a2 + 1])
+    const double *plv, *pm;         // 3 / 9 per pair
+    const double *opm, *optr;       // 9 / 3 per operation
+    const int *invf;                // [op][a1]: the atom a2 with fmap[a2] = a1
+    const int *lg_off, *lg;         // little group of every irreducible point
+    const double *rcart, *mw;       // [na][3], [nb]
+    cd *V;                          // [q][3][nb][nb]
+};
+
+__device__ inline cd zmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+constexpr int kOdSlots = 9;   // nb^2 / 256 entries of V per thread (nb <= 48)
+
+// Workgroup per (irreducible q, direction z): G_z, egw and G_z egw in LDS (3 nb^2 complex),
+// the sums over the little group in registers, in the order of od_velocity_host.
+__global__ void __launch_bounds__(256) k_od_velocity(OdArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nb = a.nb, na = a.na, nb2 = nb * nb;
+    const int iq = blockIdx.x / 3, z = blockIdx.x % 3;
+    cd *G = (cd *)smem, *E = G + nb2, *Tm = E + nb2;
+    const double *qv = a.qv + 3 * iq, *q = a.qirr + 3 * iq;
+    for (int x = threadIdx.x; x < nb2; x += blockDim.x) {
+        const int r = x / nb, col = x % nb;
+        const int a2 = r / 3, j = r % 3, a1 = col / 3, i = col % 3;
+        cd g = {0, 0};
+        for (int p = a.blk2[a1 * na + a2]; p < a.blk2[a1 * na + a2 + 1]; ++p) {
+            const double *R = a.plv + 3 * p;
+            const double qr = R[0] * qv[0] + R[1] * qv[1] + R[2] * qv[2];
+            const cd e = {cos(qr), sin(qr)};
+            const cd t = zmul({0.0, R[z]}, e);
+            const double m = a.pm[9 * p + 3 * i + j];
+            g.x += t.x * m;
+            g.y += t.y * m;
+        }
+        G[x] = g;
+    }
+    cd acc[kOdSlots];
+    for (int s = 0; s < kOdSlots; ++s) acc[s] = {0, 0};
+    for (int l = a.lg_off[iq]; l < a.lg_off[iq + 1]; ++l) {
+        const int o = a.lg[l];
+        const double *m = a.opm + 9 * o, *tr = a.optr + 3 * o;
+        for (int x = threadIdx.x; x < nb2; x += blockDim.x) {
+            const int row = x / nb, b = x % nb, a1 = row / 3, al = row % 3;
+            const int a2 = a.invf[o * na + a1];
+            double d[3], v0[3];
+            for (int i = 0; i < 3; ++i) d[i] = a.rcart[3 * a1 + i] - tr[i];
+            for (int i = 0; i < 3; ++i) v0[i] = m[i] * d[0] + m[3 + i] * d[1] + m[6 + i] * d[2] - a.rcart[3 * a2 + i];
+            const double ph = -(v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2]) * kTwoPi;
+            cd s = {0, 0};
+            for (int be = 0; be < 3; ++be) {
+                const cd e = a.egv[((size_t)iq * nb + 3 * a2 + be) * nb + b];
+                s.x += m[3 * al + be] * e.x;
+                s.y += m[3 * al + be] * e.y;
+            }
+            s = zmul(s, {cos(ph), sin(ph)});
+            const double w = a.om[(size_t)iq * nb + b];
+            const double sc = (w > kFreqTol) ? a.mw[row] / sqrt(w * 2.0) : 0.0;
+            E[x] = {s.x * sc, s.y * sc};
+        }
+        __syncthreads();
+        for (int x = threadIdx.x; x < nb2; x += blockDim.x) {
+            const int r = x / nb, b = x % nb;
+            cd s = {0, 0};
+            for (int k = 0; k < nb; ++k) {
+                const cd t = zmul(G[r * nb + k], E[k * nb + b]);
+                s.x += t.x;
+                s.y += t.y;
+            }
+            Tm[x] = s;
+        }
+        __syncthreads();
+        for (int sl = 0; sl < kOdSlots; ++sl) {
+            const int x = threadIdx.x + sl * blockDim.x;
+            if (x >= nb2) break;
+            const int i = x / nb, j = x % nb;
+            cd s = {0, 0};
+            for (int r = 0; r < nb; ++r) {
+                const cd e = E[r * nb + i];
+                const cd t = zmul({e.x, -e.y}, Tm[r * nb + j]);
+                s.x += t.x;
+                s.y += t.y;
+            }
+            acc[sl].x += s.x;
+            acc[sl].y += s.y;
+        }
+        __syncthreads();
+    }
+    for (int sl = 0; sl < kOdSlots; ++sl) {
+        const int x = threadIdx.x + sl * blockDim.x;
+        if (x < nb2) a.V[((size_t)iq * 3 + z) * nb2 + x] = acc[sl];
+    }
+}
+
+// The coherence velocities of every mode pair, vel[q][j][k][z] = Re V_z(q)[j][k] / |lg(q)|,
+// below 1e-10 km/s set to zero (kappa.f90:203-213).
+void od_real_velocities(const tc3_ctx *c, const std::vector<zc> &V, std::vector<double> &vel) {
+    const int nb = c->nb, nirr = c->mesh.nirr;
+    const size_t nb2 = (size_t)nb * nb;
+    const double vtol = 1e-10 / (kGroupvelAuToMs / 1000.0);
+    vel.assign((size_t)nirr * nb2 * 3, 0.0);
+    for (int iq = 0; iq < nirr; ++iq) {
+        const double ng = (double)c->mesh.littlegroup[iq].size();
+        for (int z = 0; z < 3; ++z)
+            for (size_t jk = 0; jk < nb2; ++jk) {
+                const double re = (V[((size_t)iq * 3 + z) * nb2 + jk] / ng).real();
+                vel[((size_t)iq * nb2 + jk) * 3 + z] = std::fabs(re) < vtol ? 0.0 : re;
+            }
+    }
+}
+
+// V of every irreducible point: on the device when 3 nb^2 complex values fit the LDS and nb^2
+// <= 256 kOdSlots, else (or with TC3_OD_HOST set) on the host.
+int od_velocities(tc3_ctx *c, std::vector<double> &vel) {
+    const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr, nsym = (int)c->ops.size();
+    const size_t nb2 = (size_t)nb * nb;
+    std::vector<zc> V((size_t)nirr * 3 * nb2, zc(0, 0));
+    const size_t lds = 3 * nb2 * sizeof(cd);
+    if (nb2 > 256 * (size_t)kOdSlots || lds > 150 * 1024 || std::getenv("TC3_OD_HOST") != nullptr) {
+        for (int iq = 0; iq < nirr; ++iq) od_velocity_host(c, iq, V.data() + (size_t)iq * 3 * nb2);
+        od_real_velocities(c, V, vel);
+        return 0;
+    }
+    hipStream_t st = c->stream;
+    hipError_t e;
+#define TRY(x) do { e = (x); if (e != hipSuccess) return device_error(e, #x, __FILE__, __LINE__); } while (0)
+    std::vector<double> qv(3 * (size_t)nirr), qirr(3 * (size_t)nirr), opm(9 * (size_t)nsym), optr(3 * (size_t)nsym), mw(nb);
+    std::vector<int> invf((size_t)nsym * na), lg_off(nirr + 1, 0), lg;
+    for (int iq = 0; iq < nirr; ++iq) {
+        const double *q = &c->mesh.qcart[3 * c->mesh.irr_full_index[iq]];
+        for (int i = 0; i < 3; ++i) {
+            qirr[3 * iq + i] = q[i];
+            qv[3 * iq + i] = chop_scalar(q[i] * kTwoPi, 1e-13);
+        }
+        lg_off[iq] = (int)lg.size();
+        lg.insert(lg.end(), c->mesh.littlegroup[iq].begin(), c->mesh.littlegroup[iq].end());
+    }
+    lg_off[nirr] = (int)lg.size();
+    for (int o = 0; o < nsym; ++o) {
+        std::memcpy(&opm[9 * o], c->ops[o].m, sizeof(double) * 9);
+        std::memcpy(&optr[3 * o], c->ops[o].tr, sizeof(double) * 3);
+        for (int a1 = 0; a1 < na; ++a1) {
+            int a2 = 0;
+            for (int x = 0; x < na; ++x)
+                if (c->ops[o].fmap[x] == a1) a2 = x;
+            invf[(size_t)o * na + a1] = a2;
+        }
+    }
+    for (int a = 0; a < na; ++a)
+        for (int i = 0; i < 3; ++i) mw[3 * a + i] = c->cr.invsqrtmass[a];
+    auto up = [&](const char *name, const void *src, size_t bytes, void **dst) -> hipError_t {
+        hipError_t r = c->pool_get(name, std::max<size_t>(bytes, 16), dst);
+        if (r == hipSuccess && bytes) r = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, st);
+        return r;
+    };
+    OdArgs a;
+    a.nb = nb;
+    a.na = na;
+    void *p;
+    TRY(up("od_qv", qv.data(), sizeof(double) * qv.size(), &p)); a.qv = (const double *)p;
+    TRY(up("od_qirr", qirr.data(), sizeof(double) * qirr.size(), &p)); a.qirr = (const double *)p;
+    TRY(up("od_blk2", c->blk2_off.data(), sizeof(int) * c->blk2_off.size(), &p)); a.blk2 = (const int *)p;
+    TRY(up("od_plv", c->p_lv.data(), sizeof(double) * c->p_lv.size(), &p)); a.plv = (const double *)p;
+    TRY(up("od_pm", c->p_m.data(), sizeof(double) * c->p_m.size(), &p)); a.pm = (const double *)p;
+    TRY(up("od_opm", opm.data(), sizeof(double) * opm.size(), &p)); a.opm = (const double *)p;
+    TRY(up("od_optr", optr.data(), sizeof(double) * optr.size(), &p)); a.optr = (const double *)p;
+    TRY(up("od_invf", invf.data(), sizeof(int) * invf.size(), &p)); a.invf = (const int *)p;
+    TRY(up("od_lgoff", lg_off.data(), sizeof(int) * lg_off.size(), &p)); a.lg_off = (const int *)p;
+    TRY(up("od_lg", lg.data(), sizeof(int) * lg.size(), &p)); a.lg = (const int *)p;
+    TRY(up("od_rcart", c->cr.rcart.data(), sizeof(double) * 3 * na, &p)); a.rcart = (const double *)p;
+    TRY(up("od_mw", mw.data(), sizeof(double) * nb, &p)); a.mw = (const double *)p;
+    a.egv = c->d_egv_irr;
+    a.om = c->d_om_irr;
+    TRY(c->pool_get("od_V", sizeof(cd) * V.size(), &p));
+    a.V = (cd *)p;
+    TRY(hipFuncSetAttribute((const void *)k_od_velocity, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_od_velocity, dim3(3 * nirr), dim3(256), lds, st, a);
+    TRY(hipGetLastError());
+    TRY(hipMemcpyAsync(V.data(), a.V, sizeof(cd) * V.size(), hipMemcpyDeviceToHost, st));
+    TRY(hipStreamSynchronize(st));
+#undef TRY
+    od_real_velocities(c, V, vel);
+    return 0;
+}
+
+// The temperature-dependent part of get_kappa_offdiag (kappa.f90:220-287) for the velocities
+// vel of od_velocities: the sum over the star of q of (m v)(m v)^T is linear in v v^T, so the
+// weighted v v^T of all mode pairs of a q is summed first and rotated through the star once.
+void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, const double *vel, double *kod) {
+    const int nb = c->nb, nirr = c->mesh.nirr;
+    double k[9] = {0};
+    std::vector<double> cv(nb);
+    for (int iq = 0; iq < nirr; ++iq) {
+        const double *om = &c->om_irr[(size_t)iq * nb];
         const double *lw = &c->linewidth[(size_t)iT * nirr * nb + (size_t)iq * nb];
+        for (int j = 0; j < nb; ++j) cv[j] = (classical || om[j] < kFreqTol) ? 0.0 : cv_host(T, om[j]);
+        double M[9] = {0};
         for (int j = 0; j < nb; ++j) {
             const double om1 = om[j];
             if (om1 < kFreqTol) continue;
@@ -201,25 +401,25 @@ void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, double *k
                 if (om2 < kFreqTol) continue;
                 const double t2 = lw[kk];
                 if (t2 < kFreqTol) continue;
-                double vel[3];
-                for (int z = 0; z < 3; ++z) {
-                    zc x = V[(size_t)z * nb * nb + (size_t)j * nb + kk] / (double)lg.size();
-                    double re = x.real();
-                    if (std::fabs(re) < vtol) re = 0.0;
-                    vel[z] = re;
-                }
-                double vsq[9] = {0};
-                for (int s : so) {
-                    const double *m = c->ops[std::abs(s) - 1].m;
-                    double v1[3];
-                    for (int i = 0; i < 3; ++i) v1[i] = m[3 * i] * vel[0] + m[3 * i + 1] * vel[1] + m[3 * i + 2] * vel[2];
-                    for (int b = 0; b < 3; ++b)
-                        for (int a = 0; a < 3; ++a) vsq[3 * b + a] += v1[a] * v1[b];
-                }
-                const double f0 = classical ? kKbHartree : 0.5 * (cv_host(T, om1) + cv_host(T, om2));
+                const double *v = &vel[(((size_t)iq * nb + j) * nb + kk) * 3];
+                const double f0 = classical ? kKbHartree : 0.5 * (cv[j] + cv[kk]);
                 const double tau = (t1 + t2) / ((t1 + t2) * (t1 + t2) + (om1 - om2) * (om1 - om2));
-                for (int i = 0; i < 9; ++i) k[i] += vsq[i] / so.size() * tau * f0 * pref;
+                const double w = tau * f0;
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y) M[3 * x + y] += w * v[x] * v[y];
             }
+        }
+        const auto &so = c->mesh.star_ops[iq];
+        const double pref = c->mesh.weight_irr[iq] / c->cr.volume / (double)so.size();
+        for (int s : so) {
+            const double *m = c->ops[std::abs(s) - 1].m;
+            double mM[9];   // m M
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y)
+                    mM[3 * x + y] = m[3 * x] * M[y] + m[3 * x + 1] * M[3 + y] + m[3 * x + 2] * M[6 + y];
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y)
+                    k[3 * x + y] += pref * (mM[3 * x] * m[3 * y] + mM[3 * x + 1] * m[3 * y + 1] + mM[3 * x + 2] * m[3 * y + 2]);
         }
     }
     chop_kappa(k);
@@ -627,6 +827,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     TRY(hipMemcpyAsync(d_dgoff, dgoff.data(), sizeof(int) * (qb + 1), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_dg, dg.data(), sizeof(int) * dg.size(), hipMemcpyHostToDevice, st));
     double t_bte = 0, t_od = 0;
+    std::vector<double> od_V;   // coherence velocities, formed on the first temperature that needs them
     // per temperature on the host: SMA kappa, off-diagonal term, the first log record, the
     // weights of the kappa log and the initial iteration state
     std::vector<BteState> hs(nT);
@@ -645,7 +846,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
         }
         if (offdiag) {
             const auto a = std::chrono::steady_clock::now();
-            kappa_offdiag(c, iT, T, c->classical, ko);
+            if (od_V.empty()) {
+                const int r = od_velocities(c, od_V);
+                if (r) return r;
+            }
+            kappa_offdiag(c, iT, T, c->classical, od_V.data(), ko);
             t_od += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
         }
         if (maxiter <= 0) continue;

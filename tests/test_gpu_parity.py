@@ -57,6 +57,19 @@ def test_fp64_scattering_and_kappa_match_golden(inputs_dir, name, grid):
         assert e.niter[iT] == int(g[f"niter_{iT}"])
 
 
+@pytest.mark.parametrize("name,grid", [("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2))])
+def test_offdiag_device_velocities_match_host(inputs_dir, monkeypatch, name, grid):
+    """The coherence velocities formed on the device give the off-diagonal kappa of the host
+    loop (the path taken for more than 48 modes) to summation order."""
+    e = _engine(inputs_dir, name, grid)
+    e.scattering_rates([100.0, 300.0], precision="fp64")
+    _, _, kd = e.kappa()
+    monkeypatch.setenv("TC3_OD_HOST", "1")
+    _, _, kh = e.kappa()
+    assert np.abs(kh).max() > 0
+    assert _rel(kd, kh) < 1e-11
+
+
 @pytest.mark.parametrize("name,grid,T", [("rocksalt", (4, 4, 4), 300.0), ("sl16", (4, 4, 2), 300.0)])
 def test_fp64_scattering_matrix_matches_oracle(inputs_dir, oracle_setup, name, grid, T):
     from oracle import scattering as S
