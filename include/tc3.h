@@ -207,6 +207,22 @@ int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int 
  * Fn*omega/T prefactor of main.f90:323-328). */
 int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path);
 
+/* Write outfile.cumulative_thermal_conductivity.hdf5 for temperature iT after tc3_kappa:
+ * phonon DOS (lo_phonon_dos%generate, type_phonon_dos.f90:110-297), cumulative kappa
+ * against mean free path (get_cumulative_kappa, cumulative_kappa.f90:58-414), kappa with
+ * boundary scattering (get_boundary_kappa, :416-508) and spectral kappa
+ * (type_phonon_dos_integrations.f90:428-809), laid out as lo_cumulative_kappa%write_to_hdf5
+ * (cumulative_kappa.f90:580-777) without its angular-momentum datasets.  mfppts, freqpts,
+ * dosintegrationtype (1 Gaussian, 2 adaptive Gaussian, 3 tetrahedron), sigma, dossigma and
+ * unit ("thz", "mev", "icm") are thermal_conductivity's --mfppts, --freqpts,
+ * --dosintegrationtype, --sigma, --dossigma and --unit (options.f90:86-147);
+ * atomic_species is the space-separated element list of the file attribute.  Returns 5 for
+ * invalid arguments or an unknown unit, 6 when the HDF5 library cannot be loaded or the
+ * file cannot be written. */
+int tc3_write_cumulative_hdf5(const tc3_ctx *ctx, int iT, const char *path, int mfppts, int freqpts,
+                              int dosintegrationtype, double sigma, double dossigma, const char *unit,
+                              const char *atomic_species);
+
 #ifdef __cplusplus
 }
 #endif

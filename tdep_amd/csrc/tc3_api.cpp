@@ -12,6 +12,7 @@
 
 #include "../../include/tc3.h"
 #include "tc3_ctx.hpp"
+#include "tc3_cumulative.hpp"
 
 using namespace tc3;
 
@@ -574,6 +575,30 @@ int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path) {
     std::string err;
     int r = write_grid_hdf5(ctx, iT, path, err);
     if (r) const_cast<tc3_ctx *>(ctx)->err = err;
+    return r;
+}
+
+int tc3_write_cumulative_hdf5(const tc3_ctx *ctx, int iT, const char *path, int mfppts, int freqpts,
+                              int dosintegrationtype, double sigma, double dossigma, const char *unit,
+                              const char *atomic_species) {
+    if (!ctx || !ctx->have_kappa || iT < 0 || iT >= ctx->nT || !path || mfppts < 2 || freqpts < 2 ||
+        dosintegrationtype < 1 || dosintegrationtype > 3 || !(sigma > 0) || !(dossigma > 0) || ctx->Fn.empty())
+        return 5;
+    tc3_ctx *mc = const_cast<tc3_ctx *>(ctx);
+    const int rf = guarded(mc, [&]() -> int { return ensure_full_host(mc); });
+    if (rf) return rf;
+    CumulativeOpts o;
+    o.mfppts = mfppts;
+    o.freqpts = freqpts;
+    o.dosint = dosintegrationtype;
+    o.sigma = sigma;
+    o.dossigma = dossigma;
+    o.unit = unit ? unit : "thz";
+    std::string err;
+    const int r = guarded(mc, [&]() -> int {
+        return write_cumulative_hdf5(ctx, iT, path, o, atomic_species ? atomic_species : "", err);
+    });
+    if (r && !err.empty()) mc->err = err;
     return r;
 }
 

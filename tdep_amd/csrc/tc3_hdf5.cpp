@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "tc3_ctx.hpp"
+#include "tc3_cumulative.hpp"
 
 namespace tc3 {
 
@@ -250,6 +251,101 @@ int write_grid_hdf5(const tc3_ctx *c, int iT, const char *path, std::string &err
     ok = ok && h.num_attr(f, "temperature", *h.native_double, &T);
     for (size_t i = 0; i < d.size(); ++i) d[i] = cv_of(T, c->om_full[i]) * kHartreeToJoule;
     ok = ok && h.data(f, "harmonic_heat_capacity", {Q, B}, d, "J/K", "q-vector,mode");
+    ok = (h.fclose(f) >= 0) && ok;
+    if (!ok) {
+        err = std::string("HDF5 write failed for ") + path;
+        return 6;
+    }
+    return 0;
+}
+
+// outfile.cumulative_thermal_conductivity.hdf5 (lo_cumulative_kappa%write_to_hdf5,
+// cumulative_kappa.f90:580-777, as main.f90:298-340 fills it) for temperature iT.  Fortran
+// arrays d(p, .., i, j) appear in C order (j, i, .., p); the mean-free-path and boundary
+// tensors are expanded from Voigt order (xx yy zz yz xz xy), the spectral ones from the
+// nine components in row order.  The angular-momentum datasets of the reference's file are
+// not written.
+int write_cumulative_hdf5(const tc3_ctx *c, int iT, const char *path, const CumulativeOpts &o,
+                          const std::string &species, std::string &err) {
+    double uf;
+    const char *omstr, *dosstr;
+    if (o.unit == "thz") {
+        uf = kFreqHartreeToThz; omstr = "THz"; dosstr = "States/THz";
+    } else if (o.unit == "mev") {
+        uf = 1000.0 * kHartreeToEv; omstr = "meV"; dosstr = "States/meV";
+    } else if (o.unit == "icm") {
+        uf = 1.0 / kHbarHartree / kTwoPi / 299792458.0 / 100.0; omstr = "cm^-1"; dosstr = "States/cm^-1";
+    } else {
+        err = "Unknown unit";
+        return 5;
+    }
+    std::vector<int> tc;
+    std::vector<double> tw;
+    if (o.dosint == 3) irreducible_tetrahedra(c, tc, tw);
+    PhononDos d;
+    phonon_dos(c, o, tc, tw, d);
+    CumulativeResult r;
+    cumulative_kappa(c, iT, o, r);
+    boundary_kappa(c, iT, o, r);
+    spectral_kappa(c, iT, o, d, tc, tw, r);
+
+    H5 h;
+    if (!h.load(err)) return 6;
+    const hid_t f = h.fcreate(path, kAccTrunc, kDefault, kDefault);
+    if (f < 0) {
+        err = std::string("cannot create ") + path;
+        return 6;
+    }
+    const int na = c->cr.na, nb = c->nb, n = o.freqpts, np = o.mfppts;
+    const hsize_t N = (hsize_t)n, P = (hsize_t)np, B = (hsize_t)nb, A = (hsize_t)na;
+    bool ok = h.str_attr(f, "atomic_species", species);
+    // the reference stores the word itself (cumulative_kappa.f90:624)
+    ok = ok && h.str_attr(f, "temperature", "temperature");
+    std::vector<double> v(n);
+    for (int i = 0; i < n; ++i) v[i] = d.omega[i] * uf;
+    ok = ok && h.data(f, "frequencies", {N}, v, omstr, "frequency");
+    for (int i = 0; i < n; ++i) v[i] = d.dos[i] / uf;
+    ok = ok && h.data(f, "dos", {N}, v, dosstr, "frequency");
+    v.resize(np);
+    for (int i = 0; i < np; ++i) v[i] = r.mfaxis[i] * kBohrToM;
+    ok = ok && h.data(f, "mean_free_path_axis", {P}, v, "m", "mfp");
+    static const int voigt[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+    // [j][i][m][p] of a [p][m][6] Voigt array
+    auto expand6 = [&](const std::vector<double> &src, int np_, int nm) {
+        std::vector<double> out((size_t)9 * nm * np_);
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i)
+                for (int m = 0; m < nm; ++m)
+                    for (int p = 0; p < np_; ++p)
+                        out[(((size_t)j * 3 + i) * nm + m) * np_ + p] =
+                            src[((size_t)p * nm + m) * 6 + voigt[i][j]] * kKappaAuToSi;
+        return out;
+    };
+    auto expand9 = [&](const std::vector<double> &src, int nx, int nm) {
+        std::vector<double> out((size_t)9 * nm * nx);
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i)
+                for (int m = 0; m < nm; ++m)
+                    for (int x = 0; x < nx; ++x)
+                        out[(((size_t)j * 3 + i) * nm + m) * nx + x] = src[((size_t)x * nm + m) * 9 + 3 * i + j] * kKappaAuToSi / uf;
+        return out;
+    };
+    ok = ok && h.data(f, "cumulative_kappa_vs_mean_free_path", {3, 3, P}, expand6(r.mf, np, 1), "W/m/K", "xyz,xyz,mfp");
+    ok = ok && h.data(f, "cumulative_kappa_vs_mean_free_path_per_mode", {3, 3, B, P}, expand6(r.mf_band, np, nb), "W/m/K",
+                      "xyz,xyz,mode,mfp");
+    ok = ok && h.data(f, "cumulative_kappa_vs_mean_free_path_per_atom", {3, 3, A, P}, expand6(r.mf_atom, np, na), "W/m/K",
+                      "xyz,xyz,atom,mfp");
+    ok = ok && h.data(f, "spectral_kappa_vs_frequency", {3, 3, N}, expand9(r.spec, n, 1), "W/m/K", "xyz,xyz,frequency");
+    ok = ok && h.data(f, "spectral_kappa_vs_frequency_per_mode", {3, 3, B, N}, expand9(r.spec_band, n, nb), "W/m/K",
+                      "xyz,xyz,mode,frequency");
+    ok = ok && h.data(f, "spectral_kappa_vs_frequency_per_atom", {3, 3, A, N}, expand9(r.spec_atom, n, na), "W/m/K",
+                      "xyz,xyz,atom,frequency");
+    for (int i = 0; i < np; ++i) v[i] = r.baxis[i] * kBohrToM;
+    ok = ok && h.data(f, "boundary_scattering_lengths", {P}, v, "m", "domainsize");
+    std::vector<double> bt((size_t)np * 6);
+    for (int p = 0; p < np; ++p)
+        for (int k = 0; k < 6; ++k) bt[(size_t)p * 6 + k] = r.bkappa[(size_t)k * np + p];
+    ok = ok && h.data(f, "boundary_scattering_kappa", {3, 3, P}, expand6(bt, np, 1), "W/m/K", "xyz,xyz,domainsize");
     ok = (h.fclose(f) >= 0) && ok;
     if (!ok) {
         err = std::string("HDF5 write failed for ") + path;

@@ -259,6 +259,20 @@ class Engine:
     def write_grid_hdf5(self, path, iT=0):
         self._check(self.lib.tc3_write_grid_hdf5(self.ctx, iT, path.encode()))
 
+    def write_cumulative_hdf5(self, path, iT=0, mfppts=1000, freqpts=1000, dosintegrationtype=3, sigma=1.0,
+                              dossigma=1.0, unit="thz"):
+        """outfile.cumulative_thermal_conductivity.hdf5 (main.f90:298-340): DOS, cumulative
+        kappa against mean free path, kappa with boundary scattering and spectral kappa of
+        temperature iT, with thermal_conductivity's --mfppts, --freqpts,
+        --dosintegrationtype, --sigma, --dossigma and --unit."""
+        species = []
+        for s in getattr(self.structure, "symbols", []):
+            if s not in species:
+                species.append(s)
+        self._check(self.lib.tc3_write_cumulative_hdf5(self.ctx, iT, path.encode(), int(mfppts), int(freqpts),
+                                                        int(dosintegrationtype), float(sigma), float(dossigma),
+                                                        unit.encode(), " ".join(species).encode()))
+
     def outfile_text(self, iT=0):
         """outfile.thermal_conductivity (main.f90:248-294)."""
         def six(m):

@@ -155,6 +155,17 @@ module tc3_api
             character(kind=c_char), intent(in) :: path(*)
         end function
 
+        integer(c_int) function tc3_write_cumulative_hdf5(ctx, iT, path, mfppts, freqpts, dosintegrationtype, &
+                                                          sigma, dossigma, unit, atomic_species) &
+                bind(C, name="tc3_write_cumulative_hdf5")
+            import :: c_int, c_ptr, c_char, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: iT, mfppts, freqpts, dosintegrationtype
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: sigma, dossigma
+            character(kind=c_char), intent(in) :: unit(*), atomic_species(*)
+        end function
+
         integer(c_int) function tc3_adjoint_gamma(ctx, dL_dlinewidth, precision, dL_dfc3) &
                 bind(C, name="tc3_adjoint_gamma")
             import :: c_int, c_ptr, c_double

@@ -6,8 +6,9 @@
 ! type_crystalstructure_io.f90:9-350, :921-959, type_forceconstant_secondorder_io.f90:109-247,
 ! type_forceconstant_thirdorder.f90:243-311), hands everything numeric to libtc3 through the
 ! iso_c_binding interface (module tc3_api), and writes outfile.thermal_conductivity
-! (main.f90:246-296) and outfile.thermal_conductivity_grid.hdf5 (main.f90:323-328) for every
-! requested temperature (files of temperatures after the first get a ".T<temperature>"
+! (main.f90:246-296), outfile.thermal_conductivity_grid.hdf5 (main.f90:323-328) and
+! outfile.cumulative_thermal_conductivity.hdf5 (main.f90:298-340) for every requested
+! temperature (files of temperatures after the first get a ".T<temperature>"
 ! suffix: all temperatures share one scattering pass).
 !
 ! Options (options.f90:44-237, the subset of this path):
@@ -16,8 +17,13 @@
 !   --seed s                                   seed of the Monte-Carlo draws (needed with -qg3ph)
 !   --temperature T [T ...]                    temperatures in K (300)
 !   -it 1|2|6 | --integrationtype 1|2|6        fixed, adaptive or velocity-difference Gaussian (2)
-!   --sigma s                                  fixed smearing in THz, integrationtype 1 (1.0)
+!   --sigma s                                  fixed smearing in THz, integrationtype 1; also
+!                                              scales the cumulative kappa's smoothing (1.0)
 !   --max_mfp L                                boundary scattering, mean free path limit in m (off)
+!   --mfppts n, --freqpts n                    points of the mean-free-path / frequency axes (1000)
+!   --dosintegrationtype 1|2|3                 Gaussian, adaptive Gaussian, tetrahedron spectra (3)
+!   --dossigma s                               smearing scale of the spectra (1.0)
+!   --unit thz|mev|icm                         frequency unit of the spectra (thz)
 !   --iterative_tolerance x                    BTE tolerance (1e-5)
 !   --iterative_maxsteps n                     BTE iterations (200)
 !   --classical, --noisotope, --notr, --readiso (masses always come from infile.isotopes)
@@ -154,12 +160,13 @@ contains
 
     ! POSCAR-style crystal structure.  latvec(:, i) is lattice vector i in Bohr, rfrac(:, a)
     ! fractional, species(a) 0-based id by first appearance of the element.
-    subroutine read_ucposcar(path, na, latvec, rfrac, species)
+    subroutine read_ucposcar(path, na, latvec, rfrac, species, elist)
         character(len=*), intent(in) :: path
         integer, intent(out) :: na
         real(c_double), intent(out) :: latvec(3, 3)
         real(c_double), allocatable, intent(out) :: rfrac(:, :)
         integer(c_int), allocatable, intent(out) :: species(:)
+        character(len=:), allocatable, intent(out) :: elist   ! elements in order, space separated
         character(len=1024) :: line
         character(len=8), allocatable :: sym(:), usym(:)
         integer, allocatable :: counts(:), z(:)
@@ -221,6 +228,10 @@ contains
                 j = nu
             end if
             species(a) = j - 1
+        end do
+        elist = trim(usym(1))
+        do i = 2, nu
+            elist = elist//' '//trim(usym(i))
         end do
         latvec = chop(m*a_to_bohr, sqtol)
     end subroutine
@@ -410,8 +421,10 @@ program tc_host
     real(c_double), parameter :: hartree_to_joule = 27.21138602_c_double*1.6021766208e-19_c_double
     real(c_double), parameter :: time_au_to_s = (6.582119514e-16_c_double*1.6021766208e-19_c_double)/hartree_to_joule
     real(c_double), parameter :: kappa_au_to_si = hartree_to_joule/time_au_to_s/0.52917721067e-10_c_double
-    integer(c_int) :: qg(3), qg3(3), integrationtype, maxsteps, prec, nT, seed
-    real(c_double) :: sigma, tol, max_mfp
+    integer(c_int) :: qg(3), qg3(3), integrationtype, maxsteps, prec, nT, seed, mfppts, freqpts, dosint
+    real(c_double) :: sigma, tol, max_mfp, dossigma
+    character(len=3) :: unit
+    character(len=:), allocatable :: elist
     real(c_double), allocatable :: temps(:)
     logical :: classical, noisotope, notr, offdiag, inputs_only
     integer(c_int) :: device
@@ -426,7 +439,7 @@ program tc_host
     character(len=:), allocatable :: sfx
 
     call parse_options()
-    call read_ucposcar('infile.ucposcar', na, latvec, rfrac, species)
+    call read_ucposcar('infile.ucposcar', na, latvec, rfrac, species, elist)
     allocate (mass(na), disorder(na))
     call read_isotopes('infile.isotopes', na, mass, disorder)
     call read_fc2('infile.forceconstant', na, latvec, npair, p1, p2, lv, m2)
@@ -473,6 +486,10 @@ contains
             call check(tc3_write_grid_hdf5(ctx, int(iT - 1, c_int), &
                                            'outfile.thermal_conductivity_grid'//sfx//'.hdf5'//c_null_char), &
                        'tc3_write_grid_hdf5')
+            call check(tc3_write_cumulative_hdf5(ctx, int(iT - 1, c_int), &
+                                                 'outfile.cumulative_thermal_conductivity'//sfx//'.hdf5'//c_null_char, &
+                                                 mfppts, freqpts, dosint, sigma, dossigma, trim(unit)//c_null_char, &
+                                                 elist//c_null_char), 'tc3_write_cumulative_hdf5')
         end do
         st = tc3_get_timings(ctx, times, 8_c_int)
         write (*, '(A,F0.3,A,F0.3,A,F0.3,A)', advance='no') 'harmonic ', times(1), ' s, scattering ', times(2), &
@@ -512,6 +529,11 @@ contains
         integrationtype = 2
         sigma = 1.0_c_double
         max_mfp = -1.0_c_double
+        mfppts = 1000
+        freqpts = 1000
+        dosint = 3
+        dossigma = 1.0_c_double
+        unit = 'thz'
         tol = 1.0e-5_c_double
         maxsteps = 200
         prec = TC3_PREC_FP64
@@ -565,6 +587,26 @@ contains
             case ('--max_mfp')
                 call get_command_argument(i + 1, val)
                 read (val, *) max_mfp
+                i = i + 1
+            case ('--mfppts')
+                call get_command_argument(i + 1, val)
+                read (val, *) mfppts
+                i = i + 1
+            case ('--freqpts')
+                call get_command_argument(i + 1, val)
+                read (val, *) freqpts
+                i = i + 1
+            case ('--dosintegrationtype')
+                call get_command_argument(i + 1, val)
+                read (val, *) dosint
+                i = i + 1
+            case ('--dossigma')
+                call get_command_argument(i + 1, val)
+                read (val, *) dossigma
+                i = i + 1
+            case ('--unit')
+                call get_command_argument(i + 1, val)
+                unit = trim(val)
                 i = i + 1
             case ('--iterative_tolerance')
                 call get_command_argument(i + 1, val)

@@ -115,9 +115,10 @@ def test_host_writes_both_outfiles(inputs_dir, tmp_path):
 
 def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
     """The Fortran host (tdep_amd/tc_host, iso_c_binding to libtc3) run in a directory of
-    infiles writes the same outfile.thermal_conductivity numbers (both temperatures) and the
-    same grid file as the Python host on the same inputs (to 1e-12: the two readers round
-    unit conversions in a different order, so inputs differ in the last bit)."""
+    infiles writes the same outfile.thermal_conductivity numbers (both temperatures), the
+    same grid file and the same cumulative file as the Python host on the same inputs (to
+    1e-12, the cumulative spectra to 1e-9: the two readers round unit conversions in a
+    different order, so inputs differ in the last bit)."""
     import shutil
     from tdep_amd import host
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdep_amd", "tc_host")
@@ -146,3 +147,12 @@ def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
         for k in ("frequencies", "linewidths", "thermal_conductivity"):
             a, b = np.array(hf["data"][k]["values"]), np.array(hp["data"][k]["values"])
             assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+    # the cumulative file: the same element list, axes and spectra
+    for name in ("outfile.cumulative_thermal_conductivity.hdf5", "outfile.cumulative_thermal_conductivity.T300.hdf5"):
+        hf, hp = _read(str(df / name)), _read(str(dp / name))
+        assert hf["attrs"] == hp["attrs"] and hp["attrs"]["atomic_species"].split() == ["Na", "Cl"]
+        assert sorted(hf["data"]) == sorted(hp["data"])
+        for k in ("frequencies", "dos", "cumulative_kappa_vs_mean_free_path", "spectral_kappa_vs_frequency",
+                  "boundary_scattering_kappa"):
+            a, b = np.array(hf["data"][k]["values"]), np.array(hp["data"][k]["values"])
+            assert np.allclose(a, b, rtol=1e-9, atol=1e-9 * np.abs(b).max())

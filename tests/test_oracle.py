@@ -514,3 +514,135 @@ def test_montecarlo_on_the_full_mesh_reproduces_the_deterministic_rates(oracle_s
     sr2 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, qg3ph=(2, 2, 2), seed=5)
     assert np.all(sr2.linewidth >= 0) and not np.allclose(sr2.linewidth, sr0.linewidth)
     assert 0.3 < sr2.linewidth.sum() / sr0.linewidth.sum() < 3.0
+
+
+# --------------------------------------------------------------------- cumulative / spectral
+def _lt_dos(e, z):
+    """Linear-tetrahedron density of states (Lehmann and Taut) of one tetrahedron of unit
+    weight at z, textbook closed form per interval."""
+    e1, e2, e3, e4 = np.sort(e)
+    if z <= e1 or z >= e4:
+        return 0.0
+    if z < e2:
+        return 3 * (z - e1) ** 2 / ((e2 - e1) * (e3 - e1) * (e4 - e1))
+    if z < e3:
+        return (3 * (e2 - e1) + 6 * (z - e2) - 3 * (e3 - e1 + e4 - e2) * (z - e2) ** 2 / ((e3 - e2) * (e4 - e2))) / (
+            (e3 - e1) * (e4 - e1))
+    return 3 * (e4 - z) ** 2 / ((e4 - e1) * (e4 - e2) * (e4 - e3))
+
+
+def test_lv_weights_sum_to_the_linear_tetrahedron_dos_and_match_brute_force():
+    from oracle import cumulative as C
+    rng = np.random.default_rng(3)
+    e = np.array([0.31, 0.05, 0.77, 0.52])
+    for z in np.linspace(0.0, 0.8, 33):
+        w = C.lv_tetrahedron_weights(e, z, U.FREQTOL, 0.01)
+        assert abs(w.sum() - _lt_dos(e, z)) < 1e-12 * max(1.0, _lt_dos(e, z))
+    # each corner: w_i(z) = < lambda_i delta(z - sum_j lambda_j e_j) > over the tetrahedron
+    lam = rng.dirichlet(np.ones(4), size=4_000_000)
+    en = lam @ e
+    h = 0.01
+    for z in (0.2, 0.4, 0.6):
+        sel = np.abs(en - z) < h / 2
+        bf = lam[sel].sum(axis=0) / len(lam) / h
+        w = C.lv_tetrahedron_weights(e, z, U.FREQTOL, 0.01)
+        assert np.abs(w - bf).max() < 0.02 * w.sum()
+    # each corner carries a quarter of the states
+    zs = np.linspace(0.0, 0.8, 40001)
+    W = np.array([C.lv_tetrahedron_weights(e, z, U.FREQTOL, 0.01) for z in zs])
+    assert np.abs(np.trapezoid(W, zs, axis=0) - 0.25).max() < 1e-6
+    # degenerate corners: four Gaussians of a quarter each
+    w = C.lv_tetrahedron_weights(np.full(4, 0.5) + [0, 1e-12, 0, 0], 0.5 + 1e-13, U.FREQTOL, 0.02)
+    assert abs(w.sum() - U.gauss(0.0, 0.0, 0.02)) < 1e-6 * w.sum()
+
+
+def test_box_tetrahedra_tile_the_box_and_irreducible_ones_cover_the_mesh(oracle_setup):
+    from oracle import cumulative as C
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("fcc_al", (6, 6, 6))
+    dims = [int(x) for x in mesh.dims]
+    off, tets = C.chop_box(p.reclat, dims, C.cell_center((1, 2, 3), dims))
+    corner = np.array([p.reclat @ (np.array(o) / dims) for o in off])
+    vols = [abs(np.linalg.det(np.array([corner[t[k]] - corner[t[0]] for k in (1, 2, 3)]))) / 6 for t in tets]
+    box = abs(np.linalg.det(p.reclat / np.array(dims)[None, :]))
+    assert abs(sum(vols) - box) < 1e-12 * box and max(vols) - min(vols) < 1e-12 * box
+    # a random point of the box lies in exactly one tetrahedron
+    rng = np.random.default_rng(0)
+    for x in rng.random((200, 3)):
+        r = p.reclat @ (x / dims)
+        inside = 0
+        for t in tets:
+            A = np.array([corner[t[k]] - corner[t[0]] for k in (1, 2, 3)]).T
+            lam = np.linalg.solve(A, r - corner[t[0]])
+            inside += bool(np.all(lam >= -1e-12) and lam.sum() <= 1 + 1e-12)
+        assert inside == 1
+    corners, weights = C.irreducible_tetrahedra(p, ops, mesh)
+    assert weights.sum() == 6 * mesh.nfull and len(weights) < 6 * mesh.nfull
+    # the irreducible tetrahedra integrate the DOS as all boxes cut directly do (the shortest
+    # diagonal of the FCC mesh box is unique, so every box is cut the same way)
+    allc = []
+    for g in mesh.gridind:
+        o2, t2 = C.chop_box(p.reclat, dims, C.cell_center(g, dims))
+        idx = [mesh.gi2ind([(g[d] + o[d]) % dims[d] + 1 for d in range(3)]) for o in o2]
+        allc += [[idx[c] for c in t] for t in t2]
+    d1 = C.phonon_dos(p, ops, mesh, disp, 150, 3, 1.0, tets=(corners, weights))
+    d2 = C.phonon_dos(p, ops, mesh, disp, 150, 3, 1.0, tets=(np.array(allc), np.ones(len(allc))))
+    assert np.abs(d1.dos - d2.dos).max() < 1e-9 * d1.dos.max()
+    assert abs(C.trapezoid(d1.omega, d1.dos) - 3) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def fcc_run(inputs_dir):
+    from oracle import run as R
+    d = inputs_dir("fcc_al")
+    return R.run(d, (6, 6, 6), 300.0, pre=R.setup(d, (6, 6, 6)))
+
+
+def test_cumulative_total_equals_kappa_iter_and_boundary_limit(fcc_run):
+    from oracle import cumulative as C
+    r = fcc_run
+    mfp = C.scalar_mfp(r.disp, r.sr.linewidth)
+    cu = C.cumulative_kappa(r.p, r.ops, r.mesh, r.disp, r.Fn, mfp, 300.0, npts=30)
+    ki = np.diag(r.kappa_iter)
+    # the full-grid sum of rotated F (x) v with atom projections adding to one is the
+    # star-averaged kappa of get_kappa
+    assert np.abs(cu.kappa_total[:3] - ki).max() < 1e-10 * ki.max()
+    assert np.abs(cu.mf_kappa[-1, :3] - ki).max() < 1e-10 * ki.max()
+    assert np.all(np.diff(cu.mf_kappa[:, 0]) >= 0)
+    # per-band parts add up to the total where it exceeds kappatol (1e-5 W/mK), and are zero
+    # below it
+    ktol = 1e-5 / U.KAPPA_AU_TO_SI
+    assert np.abs(cu.mf_kappa_band[:, :, 0].sum(axis=1) - cu.mf_kappa[:, 0]).max() < max(1e-12 * ki.max(), ktol)
+    # step-function limit: the smoothed curve at the median mean free path holds about the
+    # conductivity of the modes below it
+    ax, bk = C.boundary_kappa(r.p, r.ops, r.mesh, r.disp, r.Fn, r.sr.linewidth, mfp, r.kappa_offdiag, 300.0, npts=30)
+    kod = np.diag(r.kappa_offdiag)
+    assert np.abs(bk[:3, -1] - (ki + kod)).max() < 2e-6 * ki.max()
+    assert np.all(np.diff(bk[0]) >= 0) and bk[0, 0] - kod[0] < 0.1 * ki[0]
+    # Matthiessen's rule for one length: kappa(L) = kod + sum c w (F v) Gamma / (Gamma + |v|/L)
+    L = ax[10]
+    ref = np.zeros(3)
+    for q in range(r.mesh.nirr):
+        for b in range(r.disp.nb):
+            om = r.disp.om_irr[q, b]
+            if om < U.FREQTOL:
+                continue
+            v = r.disp.vel_irr[q, b]
+            g = r.sr.linewidth[q, b]
+            f1 = g / (g + np.linalg.norm(v) / L)
+            ref += U.harmonic_oscillator_cv(300.0, om) * r.mesh.weight_irr[q] * f1 * (r.Fn[q, b] @ v) / 3 / r.p.volume
+    assert np.abs(bk[:3, 10] - (ref + kod)).max() < 1e-8 * ki.max()
+
+
+def test_spectral_kappa_integrates_to_the_total(fcc_run):
+    from oracle import cumulative as C
+    r = fcc_run
+    for it in (3, 1, 2):
+        dos = C.phonon_dos(r.p, r.ops, r.mesh, r.disp, 150, it, 1.0)
+        assert abs(C.trapezoid(dos.omega, dos.dos) - 3) < 1e-12
+        sp, sb, sa, kt = C.spectral_kappa(r.p, r.ops, r.mesh, r.disp, r.Fn, dos, 300.0)
+        ki = r.kappa_iter[0, 0]
+        assert abs(kt[0] - ki) < 1e-10 * ki and abs(C.trapezoid(dos.omega, sp[:, 0]) - ki) < 1e-10 * ki
+        assert np.abs(sb[:, :, 0].sum(axis=1) - sp[:, 0]).max() < 1e-12 * sp[:, 0].max()
+        if it == 3:
+            # tetrahedra put nothing above the highest frequency but the smoothing tail
+            assert sp[dos.omega > r.disp.om_irr.max() * 1.15, 0].max() < 1e-3 * sp[:, 0].max()
