@@ -212,7 +212,8 @@ int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path);
  * against mean free path (get_cumulative_kappa, cumulative_kappa.f90:58-414), kappa with
  * boundary scattering (get_boundary_kappa, :416-508) and spectral kappa
  * (type_phonon_dos_integrations.f90:428-809), laid out as lo_cumulative_kappa%write_to_hdf5
- * (cumulative_kappa.f90:580-777) without its angular-momentum datasets.  mfppts, freqpts,
+ * (cumulative_kappa.f90:580-777), with the angular-momentum tensor and its spectrum
+ * (type_phonon_dispersions.f90:868-971, type_phonon_dos_integrations.f90:8-425).  mfppts, freqpts,
  * dosintegrationtype (1 Gaussian, 2 adaptive Gaussian, 3 tetrahedron), sigma, dossigma and
  * unit ("thz", "mev", "icm") are thermal_conductivity's --mfppts, --freqpts,
  * --dosintegrationtype, --sigma, --dossigma and --unit (options.f90:86-147);

@@ -12,12 +12,18 @@ Restates, for one temperature:
   the irreducible tetrahedra of the FFT mesh type_qpointmesh_gridgeneration.f90:700-1260,
       1342-1470
   lo_LV_tetrahedron_weights                  type_qpointmesh_integrationweights.f90:148-262
+  phonon_angular_momentum_matrix             type_phonon_dispersions.f90:868-971
+  spectral_angular_momentum                  type_phonon_dos_integrations.f90:8-425
 and the helpers they call (gottochblandat_calculus.f90).  Inputs are the dispersions, the
 final F vectors (before the omega/T prefactor of main.f90:323-328), the linewidths and the
 symmetrised off-diagonal kappa, all in atomic units.  Two places where the reference reads a
 variable before it is ever set are given a value here, and the same value in the engine: the
 boundary-scattering rate of cumulative_kappa.f90:464-467 starts at zero, and the full-grid
 per-mode kappa of modes below lo_freqtol (cumulative_kappa.f90:538-553) is zero.
+
+spectral_angular_momentum: parity unpinned.  The tensor vanishes by symmetry in every test
+crystal, so the spectrum's normalisation is a ratio of rounding noise and no closed form,
+invariant or brute force checks it beyond what it shares with spectral_kappa.
 """
 import math
 
@@ -753,3 +759,183 @@ def spectral_kappa(p, ops, mesh, disp, Fn, dos, temperature, classical=False, si
                 sb[i, :, dr] = 0.0
                 sa[i, :, dr] = 0.0
     return spec, sb, sa, ktot
+
+
+# ---------------------------------------------------------------- angular momentum
+
+def _small_group(p, ops, qc):
+    """lo_get_small_group_of_qpoint (type_qpointmesh.f90:660-700), no time reversal."""
+    out = []
+    for i, o in enumerate(ops):
+        v1 = p.inv_reclat @ (qc - o.m @ qc)
+        v1 = v1 - np.round(v1)
+        if v1 @ v1 < U.SQTOL:
+            out.append(i)
+    return out
+
+
+def _mode_angular_momentum(egv, na, mode):
+    """sum over atoms of e^H M_a e, M_a the spin-1 generators of cumulative_kappa's
+    dispersions routine: 2 Im(conj(e_y) e_z), 2 Im(conj(e_z) e_x), 2 Im(conj(e_x) e_y)."""
+    cv = np.zeros(3, dtype=np.complex128)
+    M = np.zeros((3, 3, 3), dtype=np.complex128)
+    M[0, 1, 2], M[0, 2, 1] = 1, -1
+    M[1, 0, 2], M[1, 2, 0] = -1, 1
+    M[2, 0, 1], M[2, 1, 0] = 1, -1
+    M = -M * 1j
+    for k in range(na):
+        c1 = egv[3 * k:3 * k + 3, mode]
+        for a in range(3):
+            cv[a] += np.vdot(c1, M[a] @ c1)
+    return cv.real
+
+
+def angular_momentum_matrix(p, ops, mesh, disp, linewidth, temperature):
+    """phonon_angular_momentum_matrix (type_phonon_dispersions.f90:868-971), with linewidths."""
+    alpha = np.zeros((3, 3))
+    for i in range(mesh.nfull):
+        lg = _small_group(p, ops, mesh.qcart[i])
+        k = int(mesh.irr_of_full[i])
+        for j in range(disp.nb):
+            om = disp.om_full[i, j]
+            if om < U.FREQTOL:
+                continue
+            v1 = _mode_angular_momentum(disp.egv_full[i], p.na, j)
+            w1 = disp.vel_full[i, j]
+            v0 = np.zeros(3)
+            w0 = np.zeros(3)
+            for o in lg:
+                v0 = v0 + ops[o].m @ v1
+                w0 = w0 + ops[o].m @ w1
+            v0, w0 = v0 / len(lg), w0 / len(lg)
+            f0 = float(U.harmonic_oscillator_cv(temperature, om)) / om
+            f0 = f0 * (1.0 / mesh.nfull)
+            f0 = f0 / (2.0 * linewidth[k, j])
+            alpha = alpha + np.outer(v0, w0) * f0
+    alpha = alpha / p.volume
+    return U.chop(alpha, np.linalg.norm(alpha) * 1e-10)
+
+
+def spectral_angular_momentum(p, ops, mesh, disp, linewidth, dos, temperature, sigma=1.0, tets=None):
+    """spectral_angular_momentum (type_phonon_dos_integrations.f90:8-425), as written: the
+    mode's angular momentum (not its little-group average) times c_v/omega times twice the
+    linewidth, outer the averaged velocity.  Returns (spec [n, 9], band [n, nb, 9],
+    atom [n, na, 9], total [9]), components in the order of the outer product's rows."""
+    na, nb, nirr, nf = p.na, disp.nb, mesh.nirr, mesh.nfull
+    tol = 1e-30
+    n, om = dos.n, dos.omega
+    kf = np.zeros((9, nirr, nb))
+    for iq in range(nf):
+        ii = int(mesh.irr_of_full[iq])
+        lg = _small_group(p, ops, mesh.qcart[iq])
+        for b in range(nb):
+            w = disp.om_full[iq, b]
+            if w < U.FREQTOL:
+                continue
+            cv0 = _mode_angular_momentum(disp.egv_full[iq], na, b)
+            w0 = np.zeros(3)
+            for o in lg:
+                w0 = w0 + ops[o].m @ disp.vel_full[iq, b]
+            w0 = w0 / len(lg)
+            f0 = float(U.harmonic_oscillator_cv(temperature, w))
+            f0 = f0 / w
+            f0 = f0 * (1.0 / nf)
+            f0 = f0 * (2.0 * linewidth[ii, b])
+            kf[:, ii, b] += np.outer(cv0 * f0, w0).reshape(-1)
+    flat = np.zeros((nf, nb, 9))
+    for iq in range(nf):
+        ii = int(mesh.irr_of_full[iq])
+        flat[iq] = (kf[:, ii, :] * (1.0 / mesh.weight_irr[ii])).T
+    total = np.array([flat[:, :, i].sum() / nf for i in range(9)])
+    total = U.chop(total, np.linalg.norm(total) * 1e-9)
+    sb = np.zeros((n, nb, 9))
+    sa = np.zeros((n, na, 9))
+    ds = dos.default_smearing
+    invf = n / om.max()
+    if dos.integrationtype in (1, 2):
+        scaled = p.reclat * 2 * math.pi / np.array(mesh.dims, dtype=np.float64)[None, :]
+        for dr in range(9):
+            if abs(total[dr]) < tol:
+                continue
+            for iq in range(nirr):
+                wq = mesh.weight_irr[iq]
+                fi = int(mesh.irr_full_index[iq])
+                for b in range(nb):
+                    f1 = disp.om_irr[iq, b]
+                    if f1 < dos.omega_min * 0.5:
+                        continue
+                    if dos.integrationtype == 1:
+                        sg = ds[b] * sigma
+                    else:
+                        g = np.abs(disp.vel_irr[iq, b])
+                        s = 0.0
+                        for i in range(3):
+                            s += float(g @ scaled[:, i]) ** 2
+                        sg = math.sqrt(s * 0.083333333333) * sigma
+                        sg = min(sg, sigma * ds[b] * 4.0)
+                        sg = max(sg, sigma * ds[b] * 0.25)
+                    proj = site_projection(disp.egv_irr[iq], na, b)
+                    qa = flat[fi, b, dr]
+                    ii = max(math.floor((f1 - 4 * sg) * invf), 1)
+                    jj = min(math.floor((f1 + 4 * sg) * invf), n)
+                    for x in range(ii, jj + 1):
+                        f0 = gauss(f1, om[x - 1], sg) * wq
+                        sb[x - 1, b, dr] += qa * f0
+                        sa[x - 1, :, dr] += qa * proj * f0
+    else:
+        corners, weights = tets if tets is not None else irreducible_tetrahedra(p, ops, mesh)
+        tps = [[np.array([site_projection(disp.egv_full[c], na, b) for c in corners[it]])
+                for it in range(len(weights))] for b in range(nb)]
+        for dr in range(9):
+            for b in range(nb):
+                sg = ds[b]
+                for it in range(len(weights)):
+                    cs = corners[it]
+                    tete = np.array([disp.om_full[c, b] for c in cs])
+                    ta = np.array([flat[c, b, dr] for c in cs])
+                    ii = max(math.floor(tete.min() * invf), 1)
+                    jj = min(math.ceil(tete.max() * invf), n)
+                    for x in range(ii, jj + 1):
+                        w = lv_tetrahedron_weights(tete, om[x - 1], U.FREQTOL, sg) * weights[it]
+                        sb[x - 1, b, dr] += np.sum(w * ta)
+                        sa[x - 1, :, dr] += (w * ta) @ tps[b][it]
+    ker = np.array([gauss(0.0, i * 1.0, 1.5) for i in range(-6, 7)])
+    ker = ker / ker.sum()
+    spec = np.zeros((n, 9))
+    deg, mem_ = atom_classes(ops, na)
+
+    def smooth(y):
+        ysm = np.zeros(n + 12)
+        for i in range(n):
+            for j in range(-6, 7):
+                ysm[i + 6 + j] += ker[j + 6] * y[i]
+        return ysm[6:6 + n].copy()
+
+    for dr in range(9):
+        if abs(total[dr]) < tol:
+            continue
+        for a in range(na):
+            sa[:, a, dr] = smooth(sa[:, a, dr])
+        for b in range(nb):
+            sb[:, b, dr] = smooth(sb[:, b, dr])
+        for i in range(na):
+            ysm = np.zeros(n)
+            for _ in range(deg[i]):
+                ysm = ysm + sa[:, i, dr] / (1.0 * deg[i])
+            for j in range(deg[i]):
+                sa[:, mem_[i][j], dr] = ysm
+        spec[:, dr] = 0.0
+        for b in range(nb):
+            spec[:, dr] = spec[:, dr] + sb[:, b, dr]
+        f0 = trapezoid(om, spec[:, dr])
+        spec[:, dr] = spec[:, dr] * total[dr] / f0 if abs(f0) > tol else 0.0
+        for i in range(n):
+            f0, f1, f2 = sa[i, :, dr].sum(), sb[i, :, dr].sum(), spec[i, dr]
+            if abs(f2) > tol:
+                sa[i, :, dr] = sa[i, :, dr] * f2 / f0 if abs(f0) > tol else 0.0
+                sb[i, :, dr] = sb[i, :, dr] * f2 / f1 if abs(f1) > tol else 0.0
+            else:
+                spec[i, dr] = 0.0
+                sb[i, :, dr] = 0.0
+                sa[i, :, dr] = 0.0
+    return spec, sb, sa, total

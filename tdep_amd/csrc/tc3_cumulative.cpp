@@ -7,6 +7,8 @@
 //                          :927-1021)
 //   irreducible tetrahedra type_qpointmesh_gridgeneration.f90:700-1260, 1342-1470
 //   LV weights             type_qpointmesh_integrationweights.f90:148-262
+//   angular momentum       type_phonon_dispersions.f90:868-971,
+//                          type_phonon_dos_integrations.f90:8-425
 // Host FP64: the work is O(tetrahedra x modes x frequency points) once per output file.  Two
 // values the reference reads before it sets them are fixed here: the boundary rate of a mode
 // without group velocity before any mode had one (cumulative_kappa.f90:464-467) starts at
@@ -937,6 +939,272 @@ void spectral_kappa(const tc3_ctx *c, int iT, const CumulativeOpts &o, const Pho
     r.spec_band = std::move(sb);
     r.spec_atom = std::move(sa);
     std::memcpy(r.spec_total, ktot, sizeof(ktot));
+}
+
+namespace {
+
+// lo_get_small_group_of_qpoint (type_qpointmesh.f90:660-700), no time reversal
+std::vector<int> small_group(const tc3_ctx *c, const double *q) {
+    std::vector<int> out;
+    for (int o = 0; o < (int)c->ops.size(); ++o) {
+        const double *m = c->ops[o].m;
+        double d[3], f[3], s = 0.0;
+        for (int i = 0; i < 3; ++i) d[i] = q[i] - (m[3 * i] * q[0] + m[3 * i + 1] * q[1] + m[3 * i + 2] * q[2]);
+        for (int i = 0; i < 3; ++i) {
+            f[i] = c->cr.invrec[3 * i] * d[0] + c->cr.invrec[3 * i + 1] * d[1] + c->cr.invrec[3 * i + 2] * d[2];
+            f[i] -= std::nearbyint(f[i]);
+            s += f[i] * f[i];
+        }
+        if (s < kSqTol) out.push_back(o);
+    }
+    return out;
+}
+
+// sum over atoms of e^H M_a e with M_a = -i (spin-1 generators) of
+// phonon_angular_momentum_matrix (type_phonon_dispersions.f90:886-897):
+// 2 Im(conj(e_y) e_z), 2 Im(conj(e_z) e_x), 2 Im(conj(e_x) e_y)
+void mode_angular_momentum(const cd *egv, int nb, int na, int mode, double *l) {
+    l[0] = l[1] = l[2] = 0.0;
+    for (int k = 0; k < na; ++k) {
+        const cd x = egv[(size_t)(3 * k) * nb + mode], y = egv[(size_t)(3 * k + 1) * nb + mode],
+                 z = egv[(size_t)(3 * k + 2) * nb + mode];
+        auto im_conj_mul = [](cd a, cd b) { return a.x * b.y - a.y * b.x; };   // Im(conj(a) b)
+        l[0] += 2.0 * im_conj_mul(y, z);
+        l[1] += 2.0 * im_conj_mul(z, x);
+        l[2] += 2.0 * im_conj_mul(x, y);
+    }
+}
+
+}  // namespace
+
+// phonon_angular_momentum_matrix (type_phonon_dispersions.f90:868-971) with the linewidths of
+// temperature iT; alpha row-major (alpha[3a + b] = sum l_a v_b ...)
+void angular_momentum_matrix(const tc3_ctx *c, int iT, double *alpha) {
+    const int na = c->cr.na, nb = c->nb, nf = c->mesh.nfull, nirr = c->mesh.nirr;
+    const double T = c->temps[iT];
+    const double *lw = &c->linewidth[(size_t)iT * nirr * nb];
+    for (int i = 0; i < 9; ++i) alpha[i] = 0.0;
+    for (int i = 0; i < nf; ++i) {
+        const std::vector<int> lg = small_group(c, &c->mesh.qcart[3 * (size_t)i]);
+        const int k = c->mesh.irr_of_full[i];
+        for (int j = 0; j < nb; ++j) {
+            const double om = c->om_full[(size_t)i * nb + j];
+            if (om < kFreqTol) continue;
+            double v1[3], v0[3] = {0, 0, 0}, w0[3] = {0, 0, 0};
+            mode_angular_momentum(&c->egv_full[(size_t)i * nb * nb], nb, na, j, v1);
+            const double *w1 = &c->vel_full[((size_t)i * nb + j) * 3];
+            for (int o : lg) {
+                const double *m = c->ops[o].m;
+                for (int a = 0; a < 3; ++a) {
+                    v0[a] = v0[a] + (m[3 * a] * v1[0] + m[3 * a + 1] * v1[1] + m[3 * a + 2] * v1[2]);
+                    w0[a] = w0[a] + (m[3 * a] * w1[0] + m[3 * a + 1] * w1[1] + m[3 * a + 2] * w1[2]);
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                v0[a] /= (double)lg.size();
+                w0[a] /= (double)lg.size();
+            }
+            double f0 = cv_of(T, om, false) / om;
+            f0 = f0 * (1.0 / nf);
+            f0 = f0 / (2.0 * lw[(size_t)k * nb + j]);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) alpha[3 * a + b] = alpha[3 * a + b] + v0[a] * w0[b] * f0;
+        }
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        alpha[i] /= c->cr.volume;
+        nrm += alpha[i] * alpha[i];
+    }
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < 9; ++i) alpha[i] = chop_scalar(alpha[i], nrm * 1e-10);
+}
+
+// spectral_angular_momentum (type_phonon_dos_integrations.f90:8-425) as written: the mode's
+// angular momentum (not its little-group average) times c_v / omega times twice the
+// linewidth, outer the little-group-averaged velocity
+void spectral_angular_momentum(const tc3_ctx *c, int iT, const CumulativeOpts &o, const PhononDos &d,
+                               const std::vector<int> &tc, const std::vector<double> &tw, CumulativeResult &r) {
+    const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr, nf = c->mesh.nfull, n = o.freqpts;
+    const double T = c->temps[iT];
+    const double tol = 1e-30;
+    const double *lw = &c->linewidth[(size_t)iT * nirr * nb];
+    std::vector<double> kf((size_t)nirr * nb * 9, 0.0);
+    for (int iq = 0; iq < nf; ++iq) {
+        const int ii = c->mesh.irr_of_full[iq];
+        const std::vector<int> lg = small_group(c, &c->mesh.qcart[3 * (size_t)iq]);
+        for (int b = 0; b < nb; ++b) {
+            const double w = c->om_full[(size_t)iq * nb + b];
+            if (w < kFreqTol) continue;
+            double l[3], w0[3] = {0, 0, 0};
+            mode_angular_momentum(&c->egv_full[(size_t)iq * nb * nb], nb, na, b, l);
+            const double *w1 = &c->vel_full[((size_t)iq * nb + b) * 3];
+            for (int op : lg) {
+                const double *m = c->ops[op].m;
+                for (int a = 0; a < 3; ++a) w0[a] = w0[a] + (m[3 * a] * w1[0] + m[3 * a + 1] * w1[1] + m[3 * a + 2] * w1[2]);
+            }
+            for (double &x : w0) x /= (double)lg.size();
+            double f0 = cv_of(T, w, false);
+            f0 = f0 / w;
+            f0 = f0 * (1.0 / nf);
+            f0 = f0 * (2.0 * lw[(size_t)ii * nb + b]);
+            double *p = &kf[((size_t)ii * nb + b) * 9];
+            for (int a = 0; a < 3; ++a)
+                for (int e = 0; e < 3; ++e) p[3 * a + e] += (l[a] * f0) * w0[e];
+        }
+    }
+    std::vector<double> flat((size_t)nf * nb * 9);
+    for (int iq = 0; iq < nf; ++iq) {
+        const int ii = c->mesh.irr_of_full[iq];
+        const double f0 = 1.0 / c->mesh.weight_irr[ii];
+        for (int b = 0; b < nb; ++b)
+            for (int i = 0; i < 9; ++i) flat[((size_t)iq * nb + b) * 9 + i] = kf[((size_t)ii * nb + b) * 9 + i] * f0;
+    }
+    double total[9], nrm = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        double s = 0.0;
+        for (size_t x = i; x < flat.size(); x += 9) s += flat[x];
+        total[i] = s / nf;
+        nrm += total[i] * total[i];
+    }
+    nrm = std::sqrt(nrm);
+    for (double &v : total) v = chop_scalar(v, nrm * 1e-9);
+    std::vector<double> sb((size_t)n * nb * 9, 0.0), sa((size_t)n * na * 9, 0.0);
+    const double invf = n / *std::max_element(d.omega.begin(), d.omega.end());
+    if (o.dosint == 1 || o.dosint == 2) {
+        double scaled[9];
+        for (int x = 0; x < 3; ++x)
+            for (int i = 0; i < 3; ++i) scaled[3 * x + i] = c->cr.rec[3 * x + i] * kTwoPi / c->mesh.dims[i];
+        for (int dr = 0; dr < 9; ++dr) {
+            if (std::fabs(total[dr]) < tol) continue;
+            for (int iq = 0; iq < nirr; ++iq) {
+                const double wq = c->mesh.weight_irr[iq];
+                const int fi = c->mesh.irr_full_index[iq];
+                for (int b = 0; b < nb; ++b) {
+                    const double f1 = c->om_irr[(size_t)iq * nb + b];
+                    if (f1 < d.omin * 0.5) continue;
+                    double sg;
+                    if (o.dosint == 1) {
+                        sg = d.ds[b] * o.dossigma;
+                    } else {
+                        const double *v = &c->vel_irr[((size_t)iq * nb + b) * 3];
+                        const double g[3] = {std::fabs(v[0]), std::fabs(v[1]), std::fabs(v[2])};
+                        double s = 0.0;
+                        for (int i = 0; i < 3; ++i) {
+                            const double dp = g[0] * scaled[i] + g[1] * scaled[3 + i] + g[2] * scaled[6 + i];
+                            s += dp * dp;
+                        }
+                        sg = std::sqrt(s * 0.083333333333) * o.dossigma;
+                        sg = std::min(sg, o.dossigma * d.ds[b] * 4.0);
+                        sg = std::max(sg, o.dossigma * d.ds[b] * 0.25);
+                    }
+                    const double qa = flat[((size_t)fi * nb + b) * 9 + dr];
+                    const int ii = std::max((int)std::floor((f1 - 4 * sg) * invf), 1);
+                    const int jj = std::min((int)std::floor((f1 + 4 * sg) * invf), n);
+                    for (int x = ii; x <= jj; ++x) {
+                        const double f0 = gauss(f1, d.omega[x - 1], sg) * wq;
+                        sb[((size_t)(x - 1) * nb + b) * 9 + dr] += qa * f0;
+                        for (int a = 0; a < na; ++a)
+                            sa[((size_t)(x - 1) * na + a) * 9 + dr] +=
+                                qa * site_projection(&c->egv_irr[(size_t)iq * nb * nb], nb, a, b) * f0;
+                    }
+                }
+            }
+        }
+    } else {
+        const size_t ntet = tw.size();
+        std::vector<double> tp(ntet * nb * 4 * na);
+        for (size_t it = 0; it < ntet; ++it)
+            for (int b = 0; b < nb; ++b)
+                for (int x = 0; x < 4; ++x)
+                    for (int a = 0; a < na; ++a)
+                        tp[((it * nb + b) * 4 + x) * na + a] = site_projection(&c->egv_full[(size_t)tc[4 * it + x] * nb * nb], nb, a, b);
+        for (int dr = 0; dr < 9; ++dr)
+            for (int b = 0; b < nb; ++b) {
+                const double sg = d.ds[b];
+                for (size_t it = 0; it < ntet; ++it) {
+                    double tete[4], ta[4];
+                    for (int x = 0; x < 4; ++x) {
+                        const int cq = tc[4 * it + x];
+                        tete[x] = c->om_full[(size_t)cq * nb + b];
+                        ta[x] = flat[((size_t)cq * nb + b) * 9 + dr];
+                    }
+                    const int ii = std::max((int)std::floor(*std::min_element(tete, tete + 4) * invf), 1);
+                    const int jj = std::min((int)std::ceil(*std::max_element(tete, tete + 4) * invf), n);
+                    const double *p = &tp[(it * nb + b) * 4 * na];
+                    for (int x = ii; x <= jj; ++x) {
+                        double w[4];
+                        lv_tetrahedron_weights(tete, d.omega[x - 1], kFreqTol, sg, w);
+                        double s = 0.0;
+                        for (int k = 0; k < 4; ++k) {
+                            w[k] *= tw[it];
+                            s += w[k] * ta[k];
+                        }
+                        sb[((size_t)(x - 1) * nb + b) * 9 + dr] += s;
+                        for (int a = 0; a < na; ++a) {
+                            double s1 = 0.0;
+                            for (int k = 0; k < 4; ++k) s1 += w[k] * ta[k] * p[k * na + a];
+                            sa[((size_t)(x - 1) * na + a) * 9 + dr] += s1;
+                        }
+                    }
+                }
+            }
+    }
+    double ker[13], ks = 0.0;
+    for (int i = -6; i <= 6; ++i) ks += (ker[i + 6] = gauss(0.0, i * 1.0, 1.5));
+    for (double &v : ker) v /= ks;
+    r.am_spec.assign((size_t)n * 9, 0.0);
+    std::vector<int> deg;
+    std::vector<std::vector<int>> mem;
+    atom_classes(c, deg, mem);
+    std::vector<double> ysm(n + 12), ys(n);
+    auto smooth = [&](std::vector<double> &v, int ncol, int col, int dr) {
+        std::fill(ysm.begin(), ysm.end(), 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int j = -6; j <= 6; ++j) ysm[i + 6 + j] += ker[j + 6] * v[((size_t)i * ncol + col) * 9 + dr];
+        for (int i = 0; i < n; ++i) v[((size_t)i * ncol + col) * 9 + dr] = ysm[i + 6];
+    };
+    for (int dr = 0; dr < 9; ++dr) {
+        if (std::fabs(total[dr]) < tol) continue;
+        for (int a = 0; a < na; ++a) smooth(sa, na, a, dr);
+        for (int b = 0; b < nb; ++b) smooth(sb, nb, b, dr);
+        for (int i = 0; i < na; ++i) {
+            std::fill(ys.begin(), ys.end(), 0.0);
+            for (int j = 0; j < deg[i]; ++j)
+                for (int x = 0; x < n; ++x) ys[x] = ys[x] + sa[((size_t)x * na + i) * 9 + dr] / (1.0 * deg[i]);
+            for (int j = 0; j < deg[i]; ++j)
+                for (int x = 0; x < n; ++x) sa[((size_t)x * na + mem[i][j]) * 9 + dr] = ys[x];
+        }
+        for (int x = 0; x < n; ++x) {
+            double s = 0.0;
+            for (int b = 0; b < nb; ++b) s = s + sb[((size_t)x * nb + b) * 9 + dr];
+            r.am_spec[(size_t)x * 9 + dr] = s;
+        }
+        const double f0 = trapezoid(d.omega, &r.am_spec[dr], 9);
+        for (int x = 0; x < n; ++x)
+            r.am_spec[(size_t)x * 9 + dr] = (std::fabs(f0) > tol) ? r.am_spec[(size_t)x * 9 + dr] * total[dr] / f0 : 0.0;
+        for (int x = 0; x < n; ++x) {
+            double f0a = 0.0, f1b = 0.0;
+            for (int a = 0; a < na; ++a) f0a += sa[((size_t)x * na + a) * 9 + dr];
+            for (int b = 0; b < nb; ++b) f1b += sb[((size_t)x * nb + b) * 9 + dr];
+            const double f2 = r.am_spec[(size_t)x * 9 + dr];
+            if (std::fabs(f2) > tol) {
+                for (int a = 0; a < na; ++a) {
+                    double &v = sa[((size_t)x * na + a) * 9 + dr];
+                    v = (std::fabs(f0a) > tol) ? v * f2 / f0a : 0.0;
+                }
+                for (int b = 0; b < nb; ++b) {
+                    double &v = sb[((size_t)x * nb + b) * 9 + dr];
+                    v = (std::fabs(f1b) > tol) ? v * f2 / f1b : 0.0;
+                }
+            } else {
+                r.am_spec[(size_t)x * 9 + dr] = 0.0;
+                for (int a = 0; a < na; ++a) sa[((size_t)x * na + a) * 9 + dr] = 0.0;
+                for (int b = 0; b < nb; ++b) sb[((size_t)x * nb + b) * 9 + dr] = 0.0;
+            }
+        }
+    }
+    std::memcpy(r.am_total, total, sizeof(total));
 }
 
 }  // namespace tc3

@@ -30,6 +30,9 @@ struct CumulativeResult {
     std::vector<double> baxis, bkappa;            // [np], [6][np]
     double spec_total[9];
     std::vector<double> spec, spec_band, spec_atom;     // [n][9], [n][nb][9], [n][na][9]
+    double alpha[9];                              // angular-momentum tensor, row-major
+    double am_total[9];
+    std::vector<double> am_spec;                  // [n][9] spectral angular momentum
 };
 
 void lv_tetrahedron_weights(const double *ein, double z, double tol, double sigma, double *w);
@@ -40,6 +43,9 @@ void cumulative_kappa(const tc3_ctx *c, int iT, const CumulativeOpts &o, Cumulat
 void boundary_kappa(const tc3_ctx *c, int iT, const CumulativeOpts &o, CumulativeResult &r);
 void spectral_kappa(const tc3_ctx *c, int iT, const CumulativeOpts &o, const PhononDos &d, const std::vector<int> &tc,
                     const std::vector<double> &tw, CumulativeResult &r);
+void angular_momentum_matrix(const tc3_ctx *c, int iT, double *alpha);
+void spectral_angular_momentum(const tc3_ctx *c, int iT, const CumulativeOpts &o, const PhononDos &d,
+                               const std::vector<int> &tc, const std::vector<double> &tw, CumulativeResult &r);
 int write_cumulative_hdf5(const tc3_ctx *c, int iT, const char *path, const CumulativeOpts &o,
                           const std::string &species, std::string &err);
 

@@ -263,8 +263,9 @@ int write_grid_hdf5(const tc3_ctx *c, int iT, const char *path, std::string &err
 // cumulative_kappa.f90:580-777, as main.f90:298-340 fills it) for temperature iT.  Fortran
 // arrays d(p, .., i, j) appear in C order (j, i, .., p); the mean-free-path and boundary
 // tensors are expanded from Voigt order (xx yy zz yz xz xy), the spectral ones from the
-// nine components in row order.  The angular-momentum datasets of the reference's file are
-// not written.
+// nine components in row order; the angular-momentum spectrum from its nine components in
+// column order (the reshape of cumulative_kappa.f90:762) and the tensor as stored, in
+// atomic units (:770-771).
 int write_cumulative_hdf5(const tc3_ctx *c, int iT, const char *path, const CumulativeOpts &o,
                           const std::string &species, std::string &err) {
     double uf;
@@ -288,6 +289,8 @@ int write_cumulative_hdf5(const tc3_ctx *c, int iT, const char *path, const Cumu
     cumulative_kappa(c, iT, o, r);
     boundary_kappa(c, iT, o, r);
     spectral_kappa(c, iT, o, d, tc, tw, r);
+    angular_momentum_matrix(c, iT, r.alpha);
+    spectral_angular_momentum(c, iT, o, d, tc, tw, r);
 
     H5 h;
     if (!h.load(err)) return 6;
@@ -346,6 +349,16 @@ int write_cumulative_hdf5(const tc3_ctx *c, int iT, const char *path, const Cumu
     for (int p = 0; p < np; ++p)
         for (int k = 0; k < 6; ++k) bt[(size_t)p * 6 + k] = r.bkappa[(size_t)k * np + p];
     ok = ok && h.data(f, "boundary_scattering_kappa", {3, 3, P}, expand6(bt, np, 1), "W/m/K", "xyz,xyz,domainsize");
+    std::vector<double> am((size_t)9 * n);
+    for (int cc = 0; cc < 3; ++cc)
+        for (int rr = 0; rr < 3; ++rr)
+            for (int x = 0; x < n; ++x) am[((size_t)cc * 3 + rr) * n + x] = r.am_spec[(size_t)x * 9 + 3 * cc + rr] / uf / kBohrToM;
+    ok = ok && h.data(f, "generating_angular_momentum_tensor_vs_frequency", {3, 3, N}, am, "hbar/m/K",
+                      "xyz,xyz,frequency");
+    std::vector<double> al(9);
+    for (int cc = 0; cc < 3; ++cc)
+        for (int rr = 0; rr < 3; ++rr) al[3 * cc + rr] = r.alpha[3 * rr + cc];
+    ok = ok && h.data(f, "generating_angular_momentum_tensor", {3, 3}, al, "hbar/m/K", "xyz,xyz");
     ok = (h.fclose(f) >= 0) && ok;
     if (!ok) {
         err = std::string("HDF5 write failed for ") + path;

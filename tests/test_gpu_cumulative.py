@@ -9,6 +9,9 @@ the default sizes (1000-point axes) the file is checked through properties that 
 depend on the axes: the DOS integrates to 3N, the cumulative and spectral kappa end at /
 integrate to the total, per-mode and per-atom parts add up to the total, and kappa with
 boundary scattering rises with the boundary length towards kappa_iter + kappa_offdiag.
+The angular-momentum tensor vanishes by inversion symmetry in the test crystals: it is
+compared with the oracle's against its no-cancellation size, and its spectrum (a ratio of
+rounding noise here) only for shape and finiteness.
 The file is read back with h5py from the image's conda interpreter.
 """
 import json
@@ -60,7 +63,7 @@ def _engine_state(inputs_dir, name, grid, T):
     ks, ki, ko = e.kappa()
     d = _Disp()
     d.om_irr, d.vel_irr, d.egv_irr = e.harmonic()
-    d.om_full, _, d.egv_full = e.harmonic(full=True)
+    d.om_full, d.vel_full, d.egv_full = e.harmonic(full=True)
     d.nb = e.nb
     Fn = e.modes(0)[0]
     lw = e.linewidths(0)[0]
@@ -118,6 +121,22 @@ def test_cumulative_file_matches_oracle(inputs_dir, oracle_setup, tmp_path, name
     assert _rel(h["spectral_kappa_vs_frequency_per_mode"], specb * k / uf) < 1e-9
     speca = np.array([[sa[:, :, 3 * i + j].T for i in range(3)] for j in range(3)])
     assert _rel(h["spectral_kappa_vs_frequency_per_atom"], speca * k / uf) < 1e-9
+    # angular momentum: zero by inversion symmetry in these crystals, so the tensor is
+    # rounding noise; it matches the oracle's against the size it would have if every mode
+    # were circularly polarised, sum |v| c_v / omega / (2 Gamma) / (N V)
+    alpha = C.angular_momentum_matrix(p, ops, mesh, d, lw, T)
+    scale = 0.0
+    for q in range(mesh.nfull):
+        for b in range(d.nb):
+            w = d.om_full[q, b]
+            if w >= U.FREQTOL:
+                scale += np.linalg.norm(d.vel_full[q, b]) * float(U.harmonic_oscillator_cv(T, w)) / w / (
+                    2 * lw[mesh.irr_of_full[q], b]) / mesh.nfull / p.volume
+    assert np.abs(h["generating_angular_momentum_tensor"] - alpha.T).max() < 1e-12 * scale
+    assert np.abs(alpha).max() < 1e-9 * scale
+    am = h["generating_angular_momentum_tensor_vs_frequency"]
+    assert am.shape == (3, 3, nfreq) and np.all(np.isfinite(am))
+    assert raw["generating_angular_momentum_tensor"]["unit"] == "hbar/m/K"
 
 
 def test_cumulative_file_properties_at_default_sizes(inputs_dir, tmp_path):

@@ -646,3 +646,36 @@ def test_spectral_kappa_integrates_to_the_total(fcc_run):
         if it == 3:
             # tetrahedra put nothing above the highest frequency but the smoothing tail
             assert sp[dos.omega > r.disp.om_irr.max() * 1.15, 0].max() < 1e-3 * sp[:, 0].max()
+
+
+def test_mode_angular_momentum_of_circular_and_linear_polarisation():
+    from oracle import cumulative as C
+    e = np.zeros((6, 3), dtype=np.complex128)
+    e[0:3, 0] = np.array([1, 1j, 0]) / np.sqrt(2)      # atom 1 circular in xy, atom 2 at rest
+    e[3:6, 1] = np.array([0, 1, -1j]) / np.sqrt(2)     # atom 2 circular in yz, other sense
+    e[0:3, 2] = np.array([1, 2, 3]) / np.sqrt(14)      # linear
+    assert np.allclose(C._mode_angular_momentum(e, 2, 0), [0, 0, 1])
+    assert np.allclose(C._mode_angular_momentum(e, 2, 1), [-1, 0, 0])
+    assert np.allclose(C._mode_angular_momentum(e, 2, 2), [0, 0, 0])
+
+
+def test_angular_momentum_tensor_vanishes_with_inversion(oracle_setup):
+    """16-atom superlattice 4x4x2: complex eigenvectors with |l| of order one, yet l(q) = l(-q)
+    and v(q) = -v(-q) cancel the tensor to rounding."""
+    from oracle import cumulative as C
+    from oracle import run as R
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("sl16", (4, 4, 2))
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0)
+    a = C.angular_momentum_matrix(p, ops, mesh, disp, sr.linewidth, 300.0)
+    scale, lmax = 0.0, 0.0
+    for q in range(mesh.nfull):
+        k = mesh.irr_of_full[q]
+        for b in range(disp.nb):
+            w = disp.om_full[q, b]
+            if w < U.FREQTOL:
+                continue
+            l = C._mode_angular_momentum(disp.egv_full[q], p.na, b)
+            lmax = max(lmax, np.abs(l).max())
+            scale += np.abs(l).sum() * np.abs(disp.vel_full[q, b]).sum() * float(
+                U.harmonic_oscillator_cv(300.0, w)) / w / (2 * sr.linewidth[k, b]) / mesh.nfull / p.volume
+    assert lmax > 0.1 and np.abs(a).max() < 1e-12 * scale
