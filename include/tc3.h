@@ -202,6 +202,18 @@ int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, 
  * reported by tc3_get_kernel_stats.  No counterpart in the reference (a tuning aid). */
 int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int *n);
 
+/* Fourth-order IFCs: replaces lo_forceconstant_fourthorder%readfromfile
+ * (type_forceconstant_fourthorder.f90:259-340) and switches four-phonon scattering on in the
+ * following tc3_scatter calls (--fourthorder; nquart = 0 switches it off).  lv2, lv3, lv4:
+ * 3*nquart Cartesian lattice vectors (Bohr); m: 81*nquart tensors (Ha/Bohr^4),
+ * m[81*t + 27*l + 9*k + 3*j + i] = Phi_ijkl; the mass weighting (:296) is applied inside.
+ * compute_fourphonon_scattering (scattering_fourphonon.f90:1-328) then adds its linewidth
+ * and scattering-matrix terms with the full q-grid for the quartets (-qg4ph equal to the
+ * mesh), FP64 whatever the contraction precision; cells of up to 2 atoms (6 bands), and not
+ * together with tc3_set_montecarlo (tc3_scatter returns 5). */
+int tc3_set_fc4(tc3_ctx *ctx, int nquart, const int *a1, const int *a2, const int *a3, const int *a4,
+                const double *lv2, const double *lv3, const double *lv4, const double *m);
+
 /* Write outfile.thermal_conductivity_grid.hdf5 for temperature iT (layout of
  * lo_phonon_dispersions%write_to_hdf5, type_phonon_dispersions.f90:316-641, after the
  * Fn*omega/T prefactor of main.f90:323-328). */

@@ -168,6 +168,38 @@ def read_fc3(p, path):
     return fc
 
 
+def read_fc4(p, path):
+    """lo_forceconstant_fourthorder%readfromfile (type_forceconstant_fourthorder.f90:259-340).
+    Flat quartet arrays with Cartesian lattice vectors (Bohr) and the mass-weighted tensor
+    mwm[t, i, j, k, l] (Ha/Bohr^4/emu^2)."""
+    lines = [ln.split() for ln in open(path).read().split("\n") if ln.strip()]
+    pos = 0
+    na = int(lines[pos][0]); pos += 1
+    pos += 1  # cutoff
+    out = {k: [] for k in ("a1", "a2", "a3", "a4", "lv2", "lv3", "lv4", "m")}
+    for a in range(na):
+        n = int(lines[pos][0]); pos += 1
+        for _ in range(n):
+            ii = [int(lines[pos + k][0]) - 1 for k in range(4)]
+            lv = [[float(x) for x in lines[pos + 4 + k][:3]] for k in range(4)]
+            pos += 8
+            m = np.zeros((3, 3, 3, 3))
+            for i in range(3):
+                for j in range(3):
+                    for k in range(3):
+                        m[i, j, k, :] = [float(x) for x in lines[pos][:3]]
+                        pos += 1
+            out["a1"].append(a); out["a2"].append(ii[1]); out["a3"].append(ii[2]); out["a4"].append(ii[3])
+            out["lv2"].append(lv[1]); out["lv3"].append(lv[2]); out["lv4"].append(lv[3])
+            out["m"].append(m * U.FC4_EVA_TO_HARTREEBOHR)
+    fc = {k: np.array(v) for k, v in out.items()}
+    ism = p.invsqrtmass
+    fc["mwm"] = fc["m"] * (ism[fc["a1"]] * ism[fc["a2"]] * ism[fc["a3"]] * ism[fc["a4"]])[:, None, None, None, None]
+    for k in ("lv2", "lv3", "lv4"):
+        fc[k] = U.chop(fc[k] @ p.latvec.T, U.SQTOL)
+    return fc
+
+
 # ----------------------------------------------------------------------------
 # Space group
 # ----------------------------------------------------------------------------

@@ -198,14 +198,16 @@ def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0
 
 def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
                      integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None, mfp_max=None,
-                     qg3ph=None, seed=None):
+                     qg3ph=None, seed=None, fc4=None):
     """lo_scattering_rates%generate (scattering.f90:58-358): three-phonon scattering with
     qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q), plus
     isotope scattering (:229-232) when per-atom mass-disorder parameters are given, plus
     boundary scattering |v| / mfp_max (Bohr, :245-250) when mfp_max > 0.  With qg3ph smaller
     than the mesh, every row (q1, b1) visits the triplets of its own Monte-Carlo q2 draw
     (compute_threephonon_scattering :69-78, montecarlo.row_grids), each draw weighted
-    1 / prod(qg3ph)."""
+    1 / prod(qg3ph).  With fourth-order force constants fc4 (lattice.read_fc4) the
+    four-phonon term of compute_fourphonon_scattering is added to every row (qg4ph = qgrid,
+    oracle/fourphonon.py)."""
     nb = disp.nb
     nirr, nfull = mesh.nirr, mesh.nfull
     sr = ScatteringRates()
@@ -301,6 +303,14 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
                     od[b1, x, upd] = avg[upd]
         for b1 in myb:
             Xi[row_of[(q1, b1)]] += od[b1].reshape(-1)
+        if fc4 is not None:
+            from . import fourphonon as F4
+            g4, od4 = F4.row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sr.sigma_fixed,
+                                   sr.thresh_sigma, rot)
+            od4 = F4.symmetrize_od(mesh, rot, q1, od4)
+            for b1 in myb:
+                gamma_row[row_of[(q1, b1)]] += g4[b1]
+                Xi[row_of[(q1, b1)]] += od4[b1].reshape(-1)
     sr.ntriplets = ntrip
     if isotope_g is not None:
         for i, (q1, b1) in enumerate(rows):

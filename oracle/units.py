@@ -32,6 +32,7 @@ EMU_TO_AMU = 1.0 / AMU_TO_EMU
 TIME_AU_TO_S = HBAR_JOULE / HARTREE_TO_JOULE
 FC2_EVA_TO_HARTREEBOHR = EV_TO_HARTREE / (A_TO_BOHR ** 2)
 FC3_EVA_TO_HARTREEBOHR = EV_TO_HARTREE / (A_TO_BOHR ** 3)
+FC4_EVA_TO_HARTREEBOHR = EV_TO_HARTREE / (A_TO_BOHR ** 4)
 FREQ_HARTREE_TO_HZ = 1.0 / HBAR_HARTREE
 FREQ_HARTREE_TO_THZ = 1e-12 / TWOPI / HBAR_HARTREE
 FREQ_THZ_TO_HARTREE = 1.0 / FREQ_HARTREE_TO_THZ

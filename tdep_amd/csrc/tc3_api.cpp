@@ -578,6 +578,50 @@ int tc3_write_grid_hdf5(const tc3_ctx *ctx, int iT, const char *path) {
     return r;
 }
 
+int tc3_set_fc4(tc3_ctx *ctx, int nquart, const int *a1, const int *a2, const int *a3, const int *a4,
+                const double *lv2, const double *lv3, const double *lv4, const double *m) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->have_structure) return ctx->fail(5, "set the structure first");
+        if (nquart < 0) return ctx->fail(5, "negative quartet count");
+        const int na = ctx->cr.na;
+        for (int t = 0; t < nquart; ++t)
+            if (a1[t] < 0 || a1[t] >= na || a2[t] < 0 || a2[t] >= na || a3[t] < 0 || a3[t] >= na || a4[t] < 0 ||
+                a4[t] >= na)
+                return ctx->fail(1, "quartet atom index out of range");
+        std::vector<int> idx(nquart);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            return std::make_tuple(a1[x], a2[x], a3[x], a4[x]) < std::make_tuple(a1[y], a2[y], a3[y], a4[y]);
+        });
+        const size_t nb4 = (size_t)na * na * na * na;
+        ctx->nfc4 = nquart;
+        ctx->blk4_off.assign(nb4 + 1, 0);
+        ctx->f4_lv.resize(9 * (size_t)nquart);
+        ctx->f4_mwm.resize(81 * (size_t)nquart);
+        const auto &ism = ctx->cr.invsqrtmass;
+        for (int n = 0; n < nquart; ++n) {
+            const int t = idx[n];
+            for (int i = 0; i < 3; ++i) {
+                ctx->f4_lv[9 * (size_t)n + i] = lv2[3 * t + i];
+                ctx->f4_lv[9 * (size_t)n + 3 + i] = lv3[3 * t + i];
+                ctx->f4_lv[9 * (size_t)n + 6 + i] = lv4[3 * t + i];
+            }
+            // type_forceconstant_fourthorder.f90:296
+            const double w = ism[a1[t]] * ism[a2[t]] * ism[a3[t]] * ism[a4[t]];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k)
+                        for (int l = 0; l < 3; ++l)
+                            ctx->f4_mwm[81 * (size_t)n + 27 * i + 9 * j + 3 * k + l] =
+                                m[81 * (size_t)t + 27 * l + 9 * k + 3 * j + i] * w;
+            ctx->blk4_off[(((size_t)a1[t] * na + a2[t]) * na + a3[t]) * na + a4[t] + 1]++;
+        }
+        for (size_t b = 0; b < nb4; ++b) ctx->blk4_off[b + 1] += ctx->blk4_off[b];
+        ++ctx->gen_model;
+        return 0;
+    });
+}
+
 int tc3_write_cumulative_hdf5(const tc3_ctx *ctx, int iT, const char *path, int mfppts, int freqpts,
                               int dosintegrationtype, double sigma, double dossigma, const char *unit,
                               const char *atomic_species) {

@@ -31,6 +31,16 @@ struct tc3_ctx {
     int nfc3 = 0;
     std::vector<int> t_a1, t_a2, t_a3, blk3_off, a1_off;
     std::vector<double> t_lv2, t_lv3, t_mwm;  // 3 / 3 / 27 per triplet (mwm [i][j][k])
+
+    // fourth order (four-phonon scattering when nfc4 > 0), sorted by (a1, a2, a3, a4);
+    // blk4_off over na^4 blocks; lattice vectors 9 per quartet (lv2, lv3, lv4), mwm 81 [i][j][k][l]
+    int nfc4 = 0;
+    std::vector<int> blk4_off;
+    std::vector<double> f4_lv, f4_mwm;
+    // irreducible quartets of every irreducible q1 (rebuilt when gen_model changes):
+    // 5 ints each (q1, q2, q3, q4, multiplicity class), images 3 ints each from img_off
+    std::vector<int> quart, quart_img_off, quart_img;
+    long quart_gen = -1;
     std::vector<int> t_src;                   // caller's index of each sorted triplet
 
     tc3::Mesh mesh;

@@ -1,0 +1,339 @@
+// Four-phonon scattering: compute_fourphonon_scattering (scattering_fourphonon.f90:1-328)
+// with qg4ph = qgrid (the Monte-Carlo grids are the identity, weight 1/N_q).  For every
+// irreducible q1 the quartets q2 <= q3 <= q4 = -q1-q2-q3 that quartet_is_irreducible
+// (:330-456) keeps, with their images under the little group of q1; per quartet one
+// workgroup forms pretransform_phi4 (:459-506) and Psi = sum phi4 e1 e2 e3 e4 (e / sqrt(w),
+// unconjugated) in four contraction stages in LDS, FP64, and the four occupation and
+// smeared-delta channels with the permutation multiplicities (:96-175) for every
+// temperature, summed over the other three bands per leg.  The host adds the terms of every
+// image to the row linewidths and to the off-diagonal scattering-matrix terms and averages
+// those over the little group (:143-205).  Bands up to 6 (phi4 of one quartet, 6^4 complex
+// values, sits in LDS twice).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tc3_chain.hpp"
+#include "tc3_ctx.hpp"
+#include "tc3_fourphonon.hpp"
+
+namespace tc3 {
+
+namespace {
+
+constexpr int kF4MaxNb = 6, kF4MaxN4 = 1296, kF4Block = 256;
+
+struct F4Args {
+    int nb, na, nT, nirr;
+    const int *fq;                      // [t][5] q1 (irreducible), q2, q3, q4 (full), multiplicity class
+    const int *blk4;                    // [na^4 + 1]
+    const double *lv, *mwm;             // [p][9], [p][81]
+    const double *qcart;                // [full q][3]
+    const cd *egv_irr, *egv_full;       // [q][component][mode]
+    const double *om_irr, *om_full;
+    const double *be, *sigsq;           // [T][nirr][nb], [nirr][nb]
+    const int *irr_of_full, *row_of;
+    const double *u_irr, *u_full;       // reduced group velocities (integration type 6)
+    int itype;
+    double sigma_fixed, thresh, pref;   // pref = pi/96 / N_q^2
+    double *g, *s;                      // [t][T][nb], [t][T][3][nb][nb]
+};
+
+__device__ inline double pair_w6(const double *ua, const double *ub) {
+    double m = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double d = ua[k] - ub[k];
+        m = fmax(m, d * d);
+    }
+    return sqrt(m * 0.5) / 6.0;
+}
+
+__global__ void __launch_bounds__(kF4Block) k_fourphonon(F4Args a) {
+    __shared__ cd P[kF4MaxN4], Q[kF4MaxN4], E[4][kF4MaxNb * kF4MaxNb];
+    __shared__ double ps[kF4MaxN4];
+    double *const fb = (double *)Q;   // per-temperature terms, once Psi is formed
+    const int t = blockIdx.x, nb = a.nb, na = a.na, nb2 = nb * nb, nb3 = nb2 * nb, n4 = nb3 * nb;
+    const int q[4] = {a.fq[5 * t], a.fq[5 * t + 1], a.fq[5 * t + 2], a.fq[5 * t + 3]};
+    const int mc = a.fq[5 * t + 4];
+    for (int x = threadIdx.x; x < 4 * nb2; x += blockDim.x) {
+        const int k = x / nb2, r = x % nb2, comp = r / nb, mode = r % nb;
+        const cd e = (k == 0) ? a.egv_irr[((size_t)q[0] * nb + comp) * nb + mode] : a.egv_full[((size_t)q[k] * nb + comp) * nb + mode];
+        const double w = (k == 0) ? a.om_irr[(size_t)q[0] * nb + mode] : a.om_full[(size_t)q[k] * nb + mode];
+        const double sc = (w >= kFreqTol) ? 1.0 / sqrt(w) : 0.0;
+        E[k][r] = {e.x * sc, e.y * sc};
+    }
+    // pretransform_phi4: phases exp(-2 pi i (q2.R2 + q3.R3 + q4.R4)), q Cartesian without 2 pi
+    const double *v2 = a.qcart + 3 * (size_t)q[1], *v3 = a.qcart + 3 * (size_t)q[2], *v4 = a.qcart + 3 * (size_t)q[3];
+    for (int x = threadIdx.x; x < n4; x += blockDim.x) {
+        const int ia = x / nb3, ib = (x / nb2) % nb, ic = (x / nb) % nb, id = x % nb;
+        const int blk = (((ia / 3) * na + ib / 3) * na + ic / 3) * na + id / 3;
+        const int cmp = 27 * (ia % 3) + 9 * (ib % 3) + 3 * (ic % 3) + id % 3;
+        double re = 0.0, im = 0.0;
+        for (int p = a.blk4[blk]; p < a.blk4[blk + 1]; ++p) {
+            const double *l = a.lv + 9 * (size_t)p;
+            const double iqr = -(v2[0] * l[0] + v2[1] * l[1] + v2[2] * l[2] + v3[0] * l[3] + v3[1] * l[4] +
+                                 v3[2] * l[5] + v4[0] * l[6] + v4[1] * l[7] + v4[2] * l[8]) * kTwoPi;
+            const double m = a.mwm[81 * (size_t)p + cmp];
+            re += m * cos(iqr);
+            im += m * sin(iqr);
+        }
+        P[x] = {re, im};
+    }
+    __syncthreads();
+    // Psi[b1][b2][b3][b4]: contract id, ic, ib, ia in turn (P -> Q -> P -> Q -> P)
+    auto stage = [&](const cd *src, cd *dst, const cd *e, int lvl) {
+        // lvl 3: dst[ia,ib,ic,b4] = sum_id src[ia,ib,ic,id] e[id][b4]
+        // lvl 2: dst[ia,ib,b3,b4] = sum_ic src[ia,ib,ic,b4] e[ic][b3]
+        // lvl 1: dst[ia,b2,b3,b4] = sum_ib src[ia,ib,b3,b4] e[ib][b2]
+        // lvl 0: dst[b1,b2,b3,b4] = sum_ia src[ia,b2,b3,b4] e[ia][b1]
+        int st = 1;
+        for (int k = lvl; k < 3; ++k) st *= nb;
+        for (int x = threadIdx.x; x < n4; x += blockDim.x) {
+            const int bo = (x / st) % nb, base = x - bo * st;
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < nb; ++k) {
+                const cd u = src[base + k * st], v = e[k * nb + bo];
+                re += u.x * v.x - u.y * v.y;
+                im += u.x * v.y + u.y * v.x;
+            }
+            dst[x] = {re, im};
+        }
+        __syncthreads();
+    };
+    stage(P, Q, E[3], 3);
+    stage(Q, P, E[2], 2);
+    stage(P, Q, E[1], 1);
+    stage(Q, P, E[0], 0);
+    for (int x = threadIdx.x; x < n4; x += blockDim.x) ps[x] = a.pref * (P[x].x * P[x].x + P[x].y * P[x].y);
+    const double m0 = (mc == 0) ? 1.0 : (mc == 1 ? 3.0 : 6.0), m1 = (mc == 2) ? 2.0 : 1.0;
+    const int i2 = a.irr_of_full[q[1]], i3 = a.irr_of_full[q[2]], i4 = a.irr_of_full[q[3]];
+    __syncthreads();
+    for (int T = 0; T < a.nT; ++T) {
+        for (int x = threadIdx.x; x < n4; x += blockDim.x) {
+            const int b1 = x / nb3, b2 = (x / nb2) % nb, b3 = (x / nb) % nb, b4 = x % nb;
+            const double w1 = a.om_irr[(size_t)q[0] * nb + b1], w2 = a.om_full[(size_t)q[1] * nb + b2],
+                         w3 = a.om_full[(size_t)q[2] * nb + b3], w4 = a.om_full[(size_t)q[3] * nb + b4];
+            double f = 0.0;
+            if (a.row_of[(size_t)q[0] * nb + b1] >= 0 && w2 >= kFreqTol && w3 >= kFreqTol && w4 >= kFreqTol) {
+                const double n2 = a.be[((size_t)T * a.nirr + i2) * nb + b2], n3 = a.be[((size_t)T * a.nirr + i3) * nb + b3],
+                             n4 = a.be[((size_t)T * a.nirr + i4) * nb + b4];
+                const double n2p = n2 + 1.0, n3p = n3 + 1.0, n4p = n4 + 1.0;
+                const double plf0 = n2p * n3p * n4p - n2 * n3 * n4;
+                const double plf1 = 3.0 * n2 * n3p * n4p - n2p * n3 * n4;
+                const double plf2 = 3.0 * n3 * n2p * n4p - n3p * n2 * n4;
+                const double plf3 = 3.0 * n4 * n3p * n2p - n4p * n3 * n2;
+                double sg;
+                if (a.itype == 1) {
+                    sg = a.sigma_fixed;
+                } else if (a.itype == 2) {
+                    sg = sqrt(a.sigsq[(size_t)q[0] * nb + b1] + a.sigsq[(size_t)i2 * nb + b2] +
+                              a.sigsq[(size_t)i3 * nb + b3] + a.sigsq[(size_t)i4 * nb + b4]);
+                } else {
+                    const double *u1 = a.u_irr + ((size_t)q[0] * nb + b1) * 3, *u2 = a.u_full + ((size_t)q[1] * nb + b2) * 3,
+                                 *u3 = a.u_full + ((size_t)q[2] * nb + b3) * 3, *u4 = a.u_full + ((size_t)q[3] * nb + b4) * 3;
+                    sg = 0.0;
+                    sg += pair_w6(u1, u2);
+                    sg += pair_w6(u1, u3);
+                    sg += pair_w6(u1, u4);
+                    sg += pair_w6(u2, u3);
+                    sg += pair_w6(u2, u4);
+                    sg += pair_w6(u3, u4);
+                    sg += a.thresh;
+                }
+                const double d0 = gauss(w1, w2 + w3 + w4, sg) - gauss(w1, -w2 - w3 - w4, sg);
+                const double d1 = gauss(w1, -w2 + w3 + w4, sg) - gauss(w1, w2 - w3 - w4, sg);
+                const double d2 = gauss(w1, w2 - w3 + w4, sg) - gauss(w1, -w2 + w3 - w4, sg);
+                const double d3 = gauss(w1, w2 + w3 - w4, sg) - gauss(w1, -w2 - w3 + w4, sg);
+                const double p = ps[x];
+                f = m0 * p * plf0 * d0 + m1 * p * plf1 * d1 + m1 * p * plf2 * d2 + m1 * p * plf3 * d3;
+            }
+            fb[x] = f;
+        }
+        __syncthreads();
+        double *g = a.g + ((size_t)t * a.nT + T) * nb;
+        double *s = a.s + ((size_t)t * a.nT + T) * 3 * nb2;
+        for (int y = threadIdx.x; y < nb; y += blockDim.x) {
+            double tot = 0.0;
+            for (int z = 0; z < nb3; ++z) tot += fb[y * nb3 + z];
+            g[y] = tot;
+        }
+        for (int y = threadIdx.x; y < nb2; y += blockDim.x) {
+            const int b1 = y / nb, b = y % nb;
+            double s2 = 0.0, s3 = 0.0, s4 = 0.0;
+            for (int u = 0; u < nb; ++u)
+                for (int v = 0; v < nb; ++v) {
+                    s2 += fb[((b1 * nb + b) * nb + u) * nb + v];
+                    s3 += fb[((b1 * nb + u) * nb + b) * nb + v];
+                    s4 += fb[((b1 * nb + u) * nb + v) * nb + b];
+                }
+            s[y] = s2;
+            s[nb2 + y] = s3;
+            s[2 * nb2 + y] = s4;
+        }
+        __syncthreads();
+    }
+}
+
+int fourth_grid_index(int i1, int i2, int i3, const int *dims) {
+    const int n2 = dims[1], n3 = dims[2];
+    int g[3] = {0, 0, 0};
+    for (int i : {i1, i2, i3}) {
+        g[0] += i / (n2 * n3);
+        g[1] += (i / n3) % n2;
+        g[2] += i % n3;
+    }
+    for (int l = 0; l < 3; ++l) g[l] = ((-g[l]) % dims[l] + dims[l]) % dims[l];
+    return (g[0] * n2 + g[1]) * n3 + g[2];
+}
+
+// quartet_is_irreducible (scattering_fourphonon.f90:330-456) for every irreducible q1
+void build_quartets(tc3_ctx *c) {
+    const Mesh &m = c->mesh;
+    const int np = m.nfull;
+    c->quart.clear();
+    c->quart_img_off.assign(1, 0);
+    c->quart_img.clear();
+    std::vector<std::array<int, 3>> nq, ns;
+    for (int q1 = 0; q1 < m.nirr; ++q1) {
+        const std::vector<int> &lg = m.littlegroup[q1];
+        const int q1f = m.irr_full_index[q1];
+        for (int q2 = 0; q2 < np; ++q2)
+            for (int q3 = q2; q3 < np; ++q3) {
+                const int q4 = fourth_grid_index(q1f, q2, q3, m.dims);
+                if (q4 < q3) continue;
+                nq.assign(lg.size(), {q2, q3, q4});
+                ns.assign(lg.size(), {q2, q3, q4});
+                bool isred = false;
+                for (size_t j = 0; j < lg.size(); ++j) {
+                    const int o = lg[j];
+                    const int a = m.rot[(size_t)o * np + q2], b = m.rot[(size_t)o * np + q3], cc = m.rot[(size_t)o * np + q4];
+                    if (a < 0 || b < 0 || cc < 0) continue;
+                    std::array<int, 3> s = {a, b, cc};
+                    std::sort(s.begin(), s.end());
+                    nq[j] = {a, b, cc};
+                    ns[j] = s;
+                    if (s[0] > q2 || s[1] > q3) isred = true;
+                }
+                if (isred) continue;
+                int mc = 2;
+                if (q2 == q3 && q3 == q4) mc = 0;
+                else if (q3 == q4 || q2 == q4 || q2 == q3) mc = 1;
+                c->quart.insert(c->quart.end(), {q1, q2, q3, q4, mc});
+                for (size_t j = 0; j < lg.size(); ++j) {
+                    bool later = false;
+                    for (size_t k = j + 1; k < lg.size(); ++k) later = later || (ns[j] == ns[k]);
+                    if (!later) c->quart_img.insert(c->quart_img.end(), {nq[j][0], nq[j][1], nq[j][2]});
+                }
+                c->quart_img_off.push_back((int)c->quart_img.size() / 3);
+            }
+    }
+    c->quart_gen = c->gen_model;
+}
+
+}  // namespace
+
+int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, std::vector<double> &od4) {
+    const int nb = c->nb, na = c->cr.na, nT = in.nT, nirr = c->mesh.nirr, np = c->mesh.nfull, nrows = (int)c->row_q.size();
+    if (nb > kF4MaxNb) return c->fail(5, "four-phonon scattering handles at most 6 bands");
+    if (c->quart_gen != c->gen_model) build_quartets(c);
+    const int nqt = (int)c->quart.size() / 5;
+    g4.assign((size_t)nT * nrows, 0.0);
+    od4.assign((size_t)nT * nrows * np * nb, 0.0);
+    if (nqt == 0) return 0;
+    hipStream_t st = c->stream;
+    hipError_t e;
+#define TRY(x) do { e = (x); if (e != hipSuccess) return device_error(e, #x, __FILE__, __LINE__); } while (0)
+    auto up = [&](const char *name, const void *src, size_t bytes, void **dst) -> hipError_t {
+        hipError_t r = c->pool_get(name, std::max<size_t>(bytes, 16), dst);
+        if (r == hipSuccess && bytes) r = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, st);
+        return r;
+    };
+    F4Args a{};
+    a.nb = nb; a.na = na; a.nT = nT; a.nirr = nirr;
+    void *p;
+    TRY(up("f4_q", c->quart.data(), sizeof(int) * c->quart.size(), &p)); a.fq = (const int *)p;
+    TRY(up("f4_blk", c->blk4_off.data(), sizeof(int) * c->blk4_off.size(), &p)); a.blk4 = (const int *)p;
+    TRY(up("f4_lv", c->f4_lv.data(), sizeof(double) * c->f4_lv.size(), &p)); a.lv = (const double *)p;
+    TRY(up("f4_mwm", c->f4_mwm.data(), sizeof(double) * c->f4_mwm.size(), &p)); a.mwm = (const double *)p;
+    TRY(up("f4_qc", c->mesh.qcart.data(), sizeof(double) * c->mesh.qcart.size(), &p)); a.qcart = (const double *)p;
+    a.egv_irr = c->d_egv_irr; a.egv_full = c->d_egv_full; a.om_irr = c->d_om_irr; a.om_full = c->d_om_full;
+    a.be = in.d_be; a.sigsq = in.d_sigsq; a.irr_of_full = c->d_irr_of_full; a.row_of = in.d_rowof;
+    a.u_irr = in.d_uirr; a.u_full = in.d_ufull;
+    a.itype = in.integrationtype; a.sigma_fixed = in.sigma_fixed; a.thresh = in.thresh;
+    a.pref = (kPi / 96.0) / ((double)np * (double)np);
+    TRY(c->pool_get("f4_g", sizeof(double) * (size_t)nqt * nT * nb, &p)); a.g = (double *)p;
+    TRY(c->pool_get("f4_s", sizeof(double) * (size_t)nqt * nT * 3 * nb * nb, &p)); a.s = (double *)p;
+    hipLaunchKernelGGL(k_fourphonon, dim3(nqt), dim3(kF4Block), 0, st, a);
+    TRY(hipGetLastError());
+    std::vector<double> hg((size_t)nqt * nT * nb), hs((size_t)nqt * nT * 3 * nb * nb);
+    TRY(hipMemcpyAsync(hg.data(), a.g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(hs.data(), a.s, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st));
+    TRY(hipStreamSynchronize(st));
+#undef TRY
+    // the terms of every image (scattering_fourphonon.f90:176-187)
+    std::vector<int> row_of((size_t)nirr * nb, -1);
+    for (int r = 0; r < nrows; ++r) row_of[(size_t)c->row_q[r] * nb + c->row_b[r]] = r;
+    const size_t ncol = (size_t)np * nb;
+    for (int t = 0; t < nqt; ++t) {
+        const int q1 = c->quart[5 * t], qq[3] = {c->quart[5 * t + 1], c->quart[5 * t + 2], c->quart[5 * t + 3]};
+        for (int T = 0; T < nT; ++T) {
+            const double *gt = &hg[((size_t)t * nT + T) * nb];
+            const double *stt = &hs[((size_t)t * nT + T) * 3 * nb * nb];
+            for (int b1 = 0; b1 < nb; ++b1) {
+                const int r = row_of[(size_t)q1 * nb + b1];
+                if (r < 0) continue;
+                const double w1 = c->om_irr[(size_t)q1 * nb + b1];
+                double *od = &od4[((size_t)T * nrows + r) * ncol];
+                for (int i = c->quart_img_off[t]; i < c->quart_img_off[t + 1]; ++i) {
+                    g4[(size_t)T * nrows + r] += gt[b1];
+                    for (int leg = 0; leg < 3; ++leg) {
+                        const int qp = c->quart_img[3 * i + leg];
+                        // frequency of the quartet's own point (the host holds those of the
+                        // irreducible points; a full-grid point has its irreducible point's)
+                        const double *wq = &c->om_irr[(size_t)c->mesh.irr_of_full[qq[leg]] * nb];
+                        for (int b = 0; b < nb; ++b)
+                            od[(size_t)qp * nb + b] += 2.0 * stt[(size_t)leg * nb * nb + b1 * nb + b] * wq[b] / w1;
+                    }
+                }
+            }
+        }
+    }
+    // symmetrize_and_distribute (scattering_fourphonon.f90:143-205)
+    std::vector<double> buf(nb);
+    std::vector<int> nn(nb), imgs;
+    for (int T = 0; T < nT; ++T)
+        for (int r = 0; r < nrows; ++r) {
+            const int q1 = c->row_q[r];
+            const std::vector<int> &lg = c->mesh.littlegroup[q1];
+            double *od = &od4[((size_t)T * nrows + r) * ncol];
+            for (int q2 = 0; q2 < np; ++q2) {
+                imgs.clear();
+                bool skip = false;
+                for (int o : lg) {
+                    const int x = c->mesh.rot[(size_t)o * np + q2];
+                    if (x < 0) continue;
+                    if (x < q2) { skip = true; break; }
+                    imgs.push_back(x);
+                }
+                if (skip) continue;
+                std::fill(buf.begin(), buf.end(), 0.0);
+                std::fill(nn.begin(), nn.end(), 0);
+                for (int x : imgs)
+                    for (int b = 0; b < nb; ++b)
+                        if (od[(size_t)x * nb + b] > 0.0) {
+                            ++nn[b];
+                            buf[b] += od[(size_t)x * nb + b];
+                        }
+                for (int x : imgs)
+                    for (int b = 0; b < nb; ++b)
+                        if (nn[b]) od[(size_t)x * nb + b] = buf[b] / (double)nn[b];
+            }
+        }
+    return 0;
+}
+
+}  // namespace tc3

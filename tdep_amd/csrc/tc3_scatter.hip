@@ -23,6 +23,7 @@
 
 #include "tc3_chain.hpp"
 #include "tc3_ctx.hpp"
+#include "tc3_fourphonon.hpp"
 #include "tc3_mfma.hpp"
 
 namespace tc3 {
@@ -46,6 +47,18 @@ struct EpiArgs {
 // Reduced group velocities u_j = sum_i v_i reclat_ij / dims_j of every mode (the operand of
 // the integration type 6 widths, scattering.f90:138-140).
 struct Mat3 { double m[9]; };
+// Xi[T][row][c] += od[T][row][c] (four-phonon off-diagonal terms, FP64 sums)
+template <typename XT>
+__global__ void k_add_rows(XT *__restrict__ xi, const double *__restrict__ od, int nT, int nrows, size_t ncol,
+                           size_t xi_row, size_t xi_T) {
+    const size_t n = (size_t)nT * nrows * ncol;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t cc = i % ncol, r = (i / ncol) % nrows, T = i / (ncol * nrows);
+        XT &x = xi[T * xi_T + r * xi_row + cc];
+        x = (XT)((double)x + od[i]);
+    }
+}
+
 __global__ void k_reduced_vel(long nmode, const double *__restrict__ vel, Mat3 r, double *__restrict__ u) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nmode; i += (long)gridDim.x * blockDim.x) {
         const double *v = vel + 3 * i;
@@ -1129,6 +1142,34 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     }
     c->batch = batch;
     }
+    // four-phonon terms (scattering.f90:239-242): off-diagonal terms into the rows before the
+    // symmetrisation below, linewidth terms after the isotope term
+    std::vector<double> g4;
+    if (c->nfc4 > 0) {
+        if (use_mfma) { cleanup(); return c->fail(5, "four-phonon scattering handles at most 6 bands"); }
+        if (!c->trip_w.empty()) {
+            cleanup();
+            return c->fail(5, "four-phonon scattering with a Monte-Carlo three-phonon grid is not supported");
+        }
+        F4Inputs fi;
+        fi.nT = nT; fi.integrationtype = integrationtype; fi.sigma_fixed = ea.sigma_fixed; fi.thresh = c->thresh6;
+        fi.d_be = d_be; fi.d_sigsq = d_sigsq; fi.d_uirr = d_uirr; fi.d_ufull = d_ufull; fi.d_rowof = d_rowof;
+        std::vector<double> od4;
+        const int r4 = fourphonon_terms(c, fi, g4, od4);
+        if (r4) { cleanup(); return r4; }
+        double *d_od4;
+        TRY(c->pool_get("f4_od", sizeof(double) * std::max<size_t>(od4.size(), 1), (void **)&d_od4));
+        TRY(hipMemcpyAsync(d_od4, od4.data(), sizeof(double) * od4.size(), hipMemcpyHostToDevice, st));
+        const size_t ncol = (size_t)nfull * nb;
+        if (c->xi_es == 8)
+            hipLaunchKernelGGL(k_add_rows<double>, dim3(1024), dim3(256), 0, st, (double *)c->d_xi, (const double *)d_od4,
+                               nT, nrows, ncol, c->xi_row, xi_T);
+        else
+            hipLaunchKernelGGL(k_add_rows<float>, dim3(1024), dim3(256), 0, st, (float *)c->d_xi, (const double *)d_od4,
+                               nT, nrows, ncol, c->xi_row, xi_T);
+        TRY(hipGetLastError());
+        TRY(hipStreamSynchronize(st));
+    }
     hipEvent_t evs0 = c->ev_pool[2], evs1 = c->ev_pool[3];
     hipLaunchKernelGGL(k_gamma, dim3((nrows * nT + 255) / 256), dim3(256), 0, st, nrows, nT, nb, d_rowq, d_rowb,
                        d_q1off, d_gpart, d_lw);
@@ -1217,6 +1258,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     if (use_iso)
         for (int T = 0; T < nT; ++T)
             for (int r = 0; r < nrows; ++r) lw[(size_t)T * nrows + r] = lw_iso[r] + lw[(size_t)T * nrows + r];
+    if (!g4.empty())
+        for (size_t i = 0; i < g4.size(); ++i) lw[i] += g4[i];
     if (c->mfp_max > 0.0)
         for (int r = 0; r < nrows; ++r) {
             const double *v = &c->vel_irr[((size_t)c->row_q[r] * nb + c->row_b[r]) * 3];

@@ -14,6 +14,7 @@ own names:
 Errors from the library raise Tc3Error with the reference's exit code.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -114,6 +115,24 @@ class Engine:
         self._check(self.lib.tc3_set_fc3(self.ctx, len(fc3["a1"]), iptr(fc3["a1"]), iptr(fc3["a2"]),
                                          iptr(fc3["a3"]), dptr(np.ascontiguousarray(fc3["lv2"].reshape(-1))),
                                          dptr(np.ascontiguousarray(fc3["lv3"].reshape(-1))), dptr(m)))
+
+    def set_fc4(self, fc4):
+        """Fourth-order IFCs (io.read_fc4): four-phonon scattering in the following
+        scattering_rates calls (--fourthorder, full q-grid for the quartets); None switches
+        it off."""
+        if fc4 is None:
+            z = np.zeros(1, np.int32)
+            self._check(self.lib.tc3_set_fc4(self.ctx, 0, iptr(z), iptr(z), iptr(z), iptr(z), None, None, None, None))
+            return
+        m = np.ascontiguousarray(np.transpose(fc4["m"], (0, 4, 3, 2, 1)).reshape(-1))   # m[81t+27l+9k+3j+i]
+        flat = lambda k: dptr(np.ascontiguousarray(fc4[k].reshape(-1)))
+        self._check(self.lib.tc3_set_fc4(self.ctx, len(fc4["a1"]), iptr(fc4["a1"]), iptr(fc4["a2"]), iptr(fc4["a3"]),
+                                         iptr(fc4["a4"]), flat("lv2"), flat("lv3"), flat("lv4"), dptr(m)))
+
+    def set_fourthorder(self, indir):
+        """Read indir/infile.forceconstant_fourthorder and switch four-phonon scattering on."""
+        from tdep_amd import io as tio
+        self.set_fc4(tio.read_fc4(self.structure, os.path.join(indir, "infile.forceconstant_fourthorder")))
 
     def set_qmesh(self, qgrid, timereversal=True):
         d = np.ascontiguousarray(qgrid, dtype=np.int32)

@@ -35,6 +35,8 @@ def parse(argv=None):
     ap.add_argument("-qg3ph", "--qpoint_grid3ph", type=int, nargs=3, default=[-1, -1, -1],
                     help="Monte-Carlo q-point grid of the three-phonon term")
     ap.add_argument("--seed", type=int, default=-1, help="seed of the Monte-Carlo grids")
+    ap.add_argument("--fourthorder", action="store_true",
+                    help="add four-phonon scattering (reads infile.forceconstant_fourthorder; full q-grid)")
     ap.add_argument("--mfppts", type=int, default=1000, help="points of the mean-free-path axes")
     ap.add_argument("--freqpts", type=int, default=1000, help="points of the frequency axis")
     ap.add_argument("--dosintegrationtype", type=int, default=3, choices=[1, 2, 3],
@@ -63,6 +65,8 @@ def main(argv=None):
     eng.set_boundary(a.max_mfp)
     if max(a.qpoint_grid3ph) > 0:
         eng.set_montecarlo(tuple(a.qpoint_grid3ph), a.seed)
+    if a.fourthorder:
+        eng.set_fourthorder(a.directory)
     eng.dispersions()
     eng.scattering_rates(a.temperature, precision=a.precision, integrationtype=a.integrationtype,
                          sigma=a.sigma, classical=a.classical)

@@ -40,6 +40,13 @@ class Morse:
         d3 = D * (-8 * a ** 3 * e2 + 2 * a ** 3 * e1)
         return d1, d2, d3
 
+    def derivs4(self, r, D=None):
+        """Fourth derivative of phi at r (eV/A^4)."""
+        D = self.D if D is None else D
+        a, r0 = self.alpha, self.r0
+        e1 = np.exp(-a * (r - r0))
+        return D * (16 * a ** 4 * e1 * e1 - 2 * a ** 4 * e1)
+
 
 def pair_tensors(R, d1, d2, d3):
     """Second (H_ij) and third (T_ijk) Cartesian derivatives of phi(|R|)."""
@@ -53,6 +60,23 @@ def pair_tensors(R, d1, d2, d3):
            + np.einsum("jk,i->ijk", I, u))
     T = d3 * uuu + c * (sym - 3 * uuu)
     return H, T
+
+
+def pair_tensor4(R, d1, d2, d3, d4):
+    """Fourth Cartesian derivative of phi(|R|) from its radial derivatives d1..d4: with
+    A_1 = phi'/r and A_(n+1) = (1/r) dA_n/dr,
+    Q_abcd = (d_ab d_cd + 2 perms) A_2 + (d_ab R_c R_d + 5 perms) A_3 + R_a R_b R_c R_d A_4."""
+    r = np.linalg.norm(R)
+    A2 = d2 / r ** 2 - d1 / r ** 3
+    A3 = d3 / r ** 3 - 3 * d2 / r ** 4 + 3 * d1 / r ** 5
+    A4 = d4 / r ** 4 - 6 * d3 / r ** 5 + 15 * d2 / r ** 6 - 15 * d1 / r ** 7
+    I = np.eye(3)
+    dd = np.einsum("ab,cd->abcd", I, I)
+    dd = dd + dd.transpose(0, 2, 1, 3) + dd.transpose(0, 3, 2, 1)
+    dRR = np.einsum("ab,c,d->abcd", I, R, R)
+    six = (dRR + dRR.transpose(0, 2, 1, 3) + dRR.transpose(0, 2, 3, 1) + dRR.transpose(2, 0, 1, 3)
+           + dRR.transpose(2, 0, 3, 1) + dRR.transpose(2, 3, 0, 1))
+    return dd * A2 + six * A3 + np.einsum("a,b,c,d->abcd", R, R, R, R) * A4
 
 
 class Crystal:
@@ -121,8 +145,9 @@ def _neighbours(cr, cutoff):
     return out
 
 
-def morse_ifcs(cr, potential, cutoff, seed=0, jitter=0.01):
-    """Second- and third-order IFC lists (eV/A^2, eV/A^3) for a pair potential.
+def morse_ifcs(cr, potential, cutoff, seed=0, jitter=0.01, fourth=False):
+    """Second- and third-order IFC lists (eV/A^2, eV/A^3) for a pair potential, and with
+    fourth=True the fourth-order list (eV/A^4) as a third value.
     potential(sym1, sym2) -> Morse."""
     nb = _neighbours(cr, cutoff)
     rng = np.random.default_rng(seed)
@@ -137,6 +162,8 @@ def morse_ifcs(cr, potential, cutoff, seed=0, jitter=0.01):
     fc3 = [[] for _ in range(cr.na)]
     self2 = [np.zeros((3, 3)) for _ in range(cr.na)]
     self3 = [np.zeros((3, 3, 3)) for _ in range(cr.na)]
+    fc4 = [[] for _ in range(cr.na)]
+    self4 = [np.zeros((3, 3, 3, 3)) for _ in range(cr.na)]
     z3 = np.zeros(3)
     for (a1, a2, L, R, d), key in zip(nb, keys):
         pot = potential(cr.symbols[a1], cr.symbols[a2])
@@ -149,10 +176,44 @@ def morse_ifcs(cr, potential, cutoff, seed=0, jitter=0.01):
         fc3[a1].append((a1, a1, a2, z3, z3, L, T))
         fc3[a1].append((a1, a2, a1, z3, L, z3, T))
         self3[a1] -= T
+        if fourth:
+            # clusters of a1 and b, a1 first: (-1)^(number of a1) Q
+            Q = pair_tensor4(R, d1, d2, d3, pot.derivs4(d, pot.D * shells[key]))
+            for s2 in (0, 1):
+                for s3 in (0, 1):
+                    for s4 in (0, 1):
+                        if s2 + s3 + s4 == 0:
+                            continue
+                        n1 = 4 - (s2 + s3 + s4)
+                        at = [a2 if s else a1 for s in (s2, s3, s4)]
+                        lv = [L if s else z3 for s in (s2, s3, s4)]
+                        fc4[a1].append((a1, at[0], at[1], at[2], z3, lv[0], lv[1], lv[2], Q * (-1.0) ** n1))
+            self4[a1] += Q
     for a in range(cr.na):
         fc2[a].insert(0, (a, z3, self2[a]))
         fc3[a].insert(0, (a, a, a, z3, z3, z3, self3[a]))
+        fc4[a].insert(0, (a, a, a, a, z3, z3, z3, z3, self4[a]))
+    if fourth:
+        return fc2, fc3, fc4
     return fc2, fc3
+
+
+def write_fourthorder(cr, fc4, cutoff, path):
+    """infile.forceconstant_fourthorder (type_forceconstant_fourthorder.f90:259-299)."""
+    with open(os.path.join(path, "infile.forceconstant_fourthorder"), "w") as f:
+        f.write(f"{cr.na:11d}               How many atoms per unit cell\n")
+        f.write(f"{cutoff:21.15f}     Realspace cutoff (A)\n")
+        for a in range(cr.na):
+            f.write(f"{len(fc4[a]):11d}               How many quartets are atom {a + 1:3d} part of\n")
+            for n, (i1, i2, i3, i4, l1, l2, l3, l4, m) in enumerate(fc4[a]):
+                for k, ii in enumerate((i1, i2, i3, i4)):
+                    f.write(f"{ii + 1:11d}                atom {k + 1} in quartet {n + 1}\n")
+                for L in (l1, l2, l3, l4):
+                    f.write(" ".join(f"{x:25.16e}" for x in L) + "\n")
+                for ii in range(3):
+                    for jj in range(3):
+                        for kk in range(3):
+                            f.write(" ".join(f"{x:25.16e}" for x in m[ii, jj, kk]) + "\n")
 
 
 def write_inputs(cr, fc2, fc3, cutoff, path):
@@ -244,9 +305,15 @@ def make_crystal(name, seed=0):
     raise KeyError(name)
 
 
-def make_config(name, path, seed=0):
-    """Write the four input files of a workload family into `path`; returns its dict."""
+def make_config(name, path, seed=0, fourth=False):
+    """Write the four input files of a workload family into `path` (and with fourth=True
+    infile.forceconstant_fourthorder); returns its dict."""
     cr, pot, cutoff = make_crystal(name, seed)
-    fc2, fc3 = morse_ifcs(cr, pot, cutoff, seed=seed)
+    if fourth:
+        fc2, fc3, fc4 = morse_ifcs(cr, pot, cutoff, seed=seed, fourth=True)
+    else:
+        fc2, fc3 = morse_ifcs(cr, pot, cutoff, seed=seed)
     write_inputs(cr, fc2, fc3, cutoff, path)
+    if fourth:
+        write_fourthorder(cr, fc4, cutoff, path)
     return dict(CONFIGS[name], name=name, natoms=cr.na)

@@ -22,6 +22,7 @@ EV_TO_HARTREE = 1.0 / HARTREE_TO_EV
 AMU_TO_EMU = 1.660539040e-27 / 9.10938356e-31
 FC2_UNIT = EV_TO_HARTREE / A_TO_BOHR ** 2
 FC3_UNIT = EV_TO_HARTREE / A_TO_BOHR ** 3
+FC4_UNIT = EV_TO_HARTREE / (A_TO_BOHR ** 4)   # lo_forceconstant_4th_eVA_to_HartreeBohr
 SQTOL = 1e-10
 
 _ELEMENTS = (
@@ -178,6 +179,31 @@ def read_fc3(p, path):
     fc["lv2"] = np.ascontiguousarray(chop(np.array(out["lv2"]) @ p.latvec.T, SQTOL))
     fc["lv3"] = np.ascontiguousarray(chop(np.array(out["lv3"]) @ p.latvec.T, SQTOL))
     fc["m"] = np.array(out["m"]) * FC3_UNIT          # [n, i, j, k]
+    return fc
+
+
+def read_fc4(p, path):
+    """infile.forceconstant_fourthorder (type_forceconstant_fourthorder.f90:259-340):
+    quartet arrays with Cartesian lattice vectors (Bohr) and m[n, i, j, k, l] (Ha/Bohr^4)."""
+    rec = _records(path)
+    na = int(rec[0][0])
+    pos = 2
+    out = {k: [] for k in ("a1", "a2", "a3", "a4", "lv2", "lv3", "lv4", "m")}
+    for a1 in range(na):
+        n = int(rec[pos][0]); pos += 1
+        for _ in range(n):
+            out["a1"].append(a1)
+            for k, key in ((1, "a2"), (2, "a3"), (3, "a4")):
+                out[key].append(int(rec[pos + k][0]) - 1)
+            for k, key in ((5, "lv2"), (6, "lv3"), (7, "lv4")):
+                out[key].append([float(x) for x in rec[pos + k][:3]])
+            m = np.array([[float(x) for x in rec[pos + 8 + r][:3]] for r in range(27)]).reshape(3, 3, 3, 3)
+            out["m"].append(m)
+            pos += 35
+    fc = {k: np.array(out[k], np.int32) for k in ("a1", "a2", "a3", "a4")}
+    for k in ("lv2", "lv3", "lv4"):
+        fc[k] = np.ascontiguousarray(chop(np.array(out[k]).reshape(-1, 3) @ p.latvec.T, SQTOL))
+    fc["m"] = np.array(out["m"]).reshape(-1, 3, 3, 3, 3) * FC4_UNIT   # [n, i, j, k, l]
     return fc
 
 

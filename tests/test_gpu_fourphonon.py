@@ -1,0 +1,46 @@
+"""Four-phonon scattering (--fourthorder) on the GPU against the oracle (oracle/fourphonon.py),
+through the C ABI: linewidths and the symmetrised scattering matrix, FP64, within 1e-9 of
+their largest value (1e-8 with velocity-difference widths); the four-phonon term changes the three-phonon linewidths by more than
+that."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name,grid,it", [("fcc_al", (4, 4, 4), 2), ("fcc_al", (4, 4, 4), 1),
+                                          ("fcc_al", (4, 4, 4), 6), ("rocksalt", (3, 3, 3), 2)])
+def test_fourphonon_matches_oracle(tmp_path, name, grid, it):
+    from oracle import lattice as L, run as R, scattering as S
+    from tdep_amd import inputs
+    from tdep_amd.engine import Engine
+    print(f"[case] fourphonon {name} {grid} it={it}", flush=True)
+    d = str(tmp_path / name)
+    inputs.make_config(name, d, fourth=True)
+    temps = [100.0, 300.0]
+    e = Engine.from_directory(d, grid, device=0)
+    e.dispersions()
+    e.scattering_rates(temps, precision="fp64", integrationtype=it)
+    lw3 = [e.linewidths(i)[0].copy() for i in range(len(temps))]
+    e.set_fourthorder(d)
+    e.scattering_rates(temps, precision="fp64", integrationtype=it)
+    p, fc2, fc3, ops, mesh, disp = R.setup(d, grid)
+    fc4 = L.read_fc4(p, os.path.join(d, "infile.forceconstant_fourthorder"))
+    # integration type 6 takes its widths from group-velocity differences, which the engine
+    # and the oracle agree on to 1e-9; a Gaussian out to its 8-sigma cut scales a relative
+    # width change by up to 63, hence 1e-8 there
+    tol = 1e-8 if it == 6 else 1e-9
+    for iT, T in enumerate(temps):
+        sr = S.scattering_rates(p, fc3, mesh, disp, ops, T, integrationtype=it, fc4=fc4)
+        lw = e.linewidths(iT)[0]
+        assert _rel(lw, sr.linewidth) < tol
+        assert _rel(lw, lw3[iT]) > 1e-6
+        xi, rq = e.scattering_matrix(iT)
+        assert [tuple(x) for x in rq] == list(sr.rows)
+        assert _rel(xi, sr.Xi) < tol

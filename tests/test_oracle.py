@@ -679,3 +679,74 @@ def test_angular_momentum_tensor_vanishes_with_inversion(oracle_setup):
             scale += np.abs(l).sum() * np.abs(disp.vel_full[q, b]).sum() * float(
                 U.harmonic_oscillator_cv(300.0, w)) / w / (2 * sr.linewidth[k, b]) / mesh.nfull / p.volume
     assert lmax > 0.1 and np.abs(a).max() < 1e-12 * scale
+
+
+# --------------------------------------------------------------------- four-phonon
+@pytest.fixture(scope="module")
+def fcc4(tmp_path_factory):
+    from oracle import lattice as L2, run as R
+    from tdep_amd import inputs
+    d = str(tmp_path_factory.mktemp("fcc4"))
+    inputs.make_config("fcc_al", d, fourth=True)
+    p, fc2, fc3, ops, mesh, disp = R.setup(d, (3, 3, 3))
+    fc4 = L2.read_fc4(p, d + "/infile.forceconstant_fourthorder")
+    return p, fc2, fc3, fc4, ops, mesh, disp
+
+
+def test_morse_fourth_derivative_is_the_gradient_of_the_third():
+    from tdep_amd import inputs as I
+    m = I.MORSE_AL
+    R = np.array([1.3, 2.1, -0.7])
+
+    def T(R):
+        return I.pair_tensors(R, *m.derivs(np.linalg.norm(R)))[1]
+    d = np.linalg.norm(R)
+    Q = I.pair_tensor4(R, *m.derivs(d), m.derivs4(d))
+    h = 1e-5
+    fd = np.stack([(T(R + h * e) - T(R - h * e)) / (2 * h) for e in np.eye(3)], axis=-1)
+    assert np.abs(fd - Q).max() < 1e-8 * np.abs(Q).max()
+
+
+def test_fourphonon_psi_symmetric_and_reduction_equals_brute_force(fcc4):
+    from oracle import fourphonon as F4
+    p, fc2, fc3, fc4, ops, mesh, disp = fcc4
+    be, sigsq, _ = S.occupations_and_smearing(p, mesh, disp, 300.0)
+    rot = S.rotation_table(p, mesh, ops)
+    ident = [i for i, o in enumerate(ops) if np.allclose(o.m, np.eye(3)) and np.allclose(o.ftr % 1.0, 0)][0]
+    for q1 in (1, 2):
+        lg = mesh.littlegroup_irr[q1]
+        mesh.littlegroup_irr[q1] = np.array([ident])
+        try:
+            g, _ = F4.row_terms(p, fc4, mesh, disp, be, sigsq, q1, 2, 0.0, 0.0, rot)
+        finally:
+            mesh.littlegroup_irr[q1] = lg
+        gb, _ = F4.row_terms(p, fc4, mesh, disp, be, sigsq, q1, 2, 0.0, 0.0, rot, reduce=False)
+        assert np.abs(g).max() > 0 and np.abs(g - gb).max() < 1e-10 * np.abs(gb).max()
+
+
+def test_quartet_rule_keeps_the_componentwise_dominant_member_of_each_orbit(fcc4):
+    """quartet_is_irreducible keeps a quartet only when no image under the little group of q1
+    has a larger smallest or a larger middle index: of each orbit exactly the member that
+    dominates all others in both, and none when no member does."""
+    from oracle import fourphonon as F4
+    p, fc2, fc3, fc4, ops, mesh, disp = fcc4
+    rot = S.rotation_table(p, mesh, ops)
+    dims = [int(x) for x in mesh.dims]
+    for q1 in range(mesh.nirr):
+        q1f = int(mesh.irr_full_index[q1])
+        quartets = set()
+        for q2 in range(mesh.nfull):
+            for q3 in range(q2, mesh.nfull):
+                q4 = F4.fourth_grid_index(q1f, q2, q3, dims)
+                if q4 >= q3:
+                    quartets.add((q2, q3, q4))
+        lg = mesh.littlegroup_irr[q1]
+        seen = set()
+        for qt in sorted(quartets):
+            if qt in seen:
+                continue
+            orbit = {tuple(sorted(int(rot[o, q]) for q in qt)) for o in lg}
+            seen |= orbit
+            dom = [m for m in orbit if all(m[0] >= x[0] and m[1] >= x[1] for x in orbit)]
+            kept = [m for m in orbit if not F4.quartet_images(mesh, rot, q1, *m)[0]]
+            assert kept == dom
