@@ -25,7 +25,7 @@ namespace tc3 {
 
 namespace {
 
-constexpr int kF4MaxNb = 6, kF4MaxN4 = 1296, kF4Block = 256;
+constexpr int kF4MaxNb = 6, kF4MaxN4 = 1296, kF4Block = 256, kF4Chunk = 512;
 
 struct F4Args {
     int nb, na, nT, nirr;
@@ -55,6 +55,7 @@ __device__ inline double pair_w6(const double *ua, const double *ub) {
 __global__ void __launch_bounds__(kF4Block) k_fourphonon(F4Args a) {
     __shared__ cd P[kF4MaxN4], Q[kF4MaxN4], E[4][kF4MaxNb * kF4MaxNb];
     __shared__ double ps[kF4MaxN4];
+    __shared__ cd ph[kF4Chunk];
     double *const fb = (double *)Q;   // per-temperature terms, once Psi is formed
     const int t = blockIdx.x, nb = a.nb, na = a.na, nb2 = nb * nb, nb3 = nb2 * nb, n4 = nb3 * nb;
     const int q[4] = {a.fq[5 * t], a.fq[5 * t + 1], a.fq[5 * t + 2], a.fq[5 * t + 3]};
@@ -66,22 +67,35 @@ __global__ void __launch_bounds__(kF4Block) k_fourphonon(F4Args a) {
         const double sc = (w >= kFreqTol) ? 1.0 / sqrt(w) : 0.0;
         E[k][r] = {e.x * sc, e.y * sc};
     }
-    // pretransform_phi4: phases exp(-2 pi i (q2.R2 + q3.R3 + q4.R4)), q Cartesian without 2 pi
+    // pretransform_phi4: phases exp(-2 pi i (q2.R2 + q3.R3 + q4.R4)), q Cartesian without 2 pi,
+    // once per IFC entry into an LDS table, in chunks of kF4Chunk entries
     const double *v2 = a.qcart + 3 * (size_t)q[1], *v3 = a.qcart + 3 * (size_t)q[2], *v4 = a.qcart + 3 * (size_t)q[3];
-    for (int x = threadIdx.x; x < n4; x += blockDim.x) {
-        const int ia = x / nb3, ib = (x / nb2) % nb, ic = (x / nb) % nb, id = x % nb;
-        const int blk = (((ia / 3) * na + ib / 3) * na + ic / 3) * na + id / 3;
-        const int cmp = 27 * (ia % 3) + 9 * (ib % 3) + 3 * (ic % 3) + id % 3;
-        double re = 0.0, im = 0.0;
-        for (int p = a.blk4[blk]; p < a.blk4[blk + 1]; ++p) {
+    const int nent = a.blk4[na * na * na * na];
+    for (int x = threadIdx.x; x < n4; x += blockDim.x) P[x] = {0.0, 0.0};
+    for (int c0 = 0; c0 < nent; c0 += kF4Chunk) {
+        const int c1 = min(nent, c0 + kF4Chunk);
+        __syncthreads();
+        for (int p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
             const double *l = a.lv + 9 * (size_t)p;
             const double iqr = -(v2[0] * l[0] + v2[1] * l[1] + v2[2] * l[2] + v3[0] * l[3] + v3[1] * l[4] +
                                  v3[2] * l[5] + v4[0] * l[6] + v4[1] * l[7] + v4[2] * l[8]) * kTwoPi;
-            const double m = a.mwm[81 * (size_t)p + cmp];
-            re += m * cos(iqr);
-            im += m * sin(iqr);
+            ph[p - c0] = {cos(iqr), sin(iqr)};
         }
-        P[x] = {re, im};
+        __syncthreads();
+        for (int x = threadIdx.x; x < n4; x += blockDim.x) {
+            const int ia = x / nb3, ib = (x / nb2) % nb, ic = (x / nb) % nb, id = x % nb;
+            const int blk = (((ia / 3) * na + ib / 3) * na + ic / 3) * na + id / 3;
+            const int pa = max(a.blk4[blk], c0), pb = min(a.blk4[blk + 1], c1);
+            if (pa >= pb) continue;
+            const int cmp = 27 * (ia % 3) + 9 * (ib % 3) + 3 * (ic % 3) + id % 3;
+            double re = P[x].x, im = P[x].y;
+            for (int p = pa; p < pb; ++p) {
+                const double m = a.mwm[81 * (size_t)p + cmp];
+                re += m * ph[p - c0].x;
+                im += m * ph[p - c0].y;
+            }
+            P[x] = {re, im};
+        }
     }
     __syncthreads();
     // Psi[b1][b2][b3][b4]: contract id, ic, ib, ia in turn (P -> Q -> P -> Q -> P)

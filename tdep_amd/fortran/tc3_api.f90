@@ -52,6 +52,14 @@ module tc3_api
             real(c_double), intent(in) :: lv2(3, *), lv3(3, *), m(3, 3, 3, *)
         end function
 
+        integer(c_int) function tc3_set_fc4(ctx, nquart, a1, a2, a3, a4, lv2, lv3, lv4, m) bind(C, name="tc3_set_fc4")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nquart
+            integer(c_int), intent(in) :: a1(*), a2(*), a3(*), a4(*)
+            real(c_double), intent(in) :: lv2(3, *), lv3(3, *), lv4(3, *), m(3, 3, 3, 3, *)
+        end function
+
         integer(c_int) function tc3_set_isotopes(ctx, disorder, enable) bind(C, name="tc3_set_isotopes")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx

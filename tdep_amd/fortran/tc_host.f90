@@ -29,6 +29,8 @@
 !   --classical, --noisotope, --notr, --readiso (masses always come from infile.isotopes)
 !   --precision fp64|fp32|bf16x3               precision of the |Psi|^2 contraction (fp64)
 !   --no-offdiag                               skip the off-diagonal (coherence) term
+!   --fourthorder                              add four-phonon scattering from
+!                                              infile.forceconstant_fourthorder (full q-grid)
 !   --device n                                 HIP device (0)
 !   --inputs-only                              read the inputs, build the mesh on the host and
 !                                              print sizes and input checksums; no device work
@@ -36,7 +38,7 @@ module tc_host_io
     use iso_c_binding
     implicit none
     private
-    public :: fatal, read_ucposcar, read_isotopes, read_fc2, read_fc3, write_outfile, temperature_suffix
+    public :: fatal, read_ucposcar, read_isotopes, read_fc2, read_fc3, read_fc4, write_outfile, temperature_suffix
 
     ! konstanter.f90:36-91
     real(c_double), parameter :: bohr_to_a = 0.52917721067_c_double
@@ -314,6 +316,69 @@ contains
 
     ! Third-order force constants: triplet list, Cartesian lattice vectors of atoms 2 and 3
     ! (Bohr), m(:, :, :, t) in Ha/Bohr^3 unweighted; atoms 0-based.
+    ! type_forceconstant_fourthorder.f90:259-299: per atom a count, then per quartet four atom
+    ! indices, four fractional lattice vectors and 27 rows of the tensor (eV/A^4)
+    subroutine read_fc4(path, na, latvec, nq, a1, a2, a3, a4, lv2, lv3, lv4, m)
+        character(len=*), intent(in) :: path
+        integer, intent(in) :: na
+        real(c_double), intent(in) :: latvec(3, 3)
+        integer, intent(out) :: nq
+        integer(c_int), allocatable, intent(out) :: a1(:), a2(:), a3(:), a4(:)
+        real(c_double), allocatable, intent(out) :: lv2(:, :), lv3(:, :), lv4(:, :), m(:, :, :, :, :)
+        integer :: u, n0, a, k, t, i, j, l, idum, ncount
+        real(c_double) :: rc, v(3)
+        integer, allocatable :: nper(:)
+        call open_input(u, path)
+        read (u, *) n0
+        if (n0 /= na) call fatal(1, 'different number of atoms in "'//path//'" and the structure')
+        read (u, *) rc
+        allocate (nper(na))
+        do a = 1, na
+            read (u, *) nper(a)
+            do k = 1, nper(a)
+                do j = 1, 35
+                    read (u, *)
+                end do
+            end do
+        end do
+        rewind (u)
+        read (u, *) n0
+        read (u, *) rc
+        nq = sum(nper)
+        allocate (a1(nq), a2(nq), a3(nq), a4(nq), lv2(3, nq), lv3(3, nq), lv4(3, nq), m(3, 3, 3, 3, nq))
+        t = 0
+        do a = 1, na
+            read (u, *) ncount
+            do k = 1, ncount
+                t = t + 1
+                a1(t) = a - 1
+                read (u, *) idum
+                read (u, *) a2(t)
+                read (u, *) a3(t)
+                read (u, *) a4(t)
+                a2(t) = a2(t) - 1
+                a3(t) = a3(t) - 1
+                a4(t) = a4(t) - 1
+                read (u, *) v
+                read (u, *) v
+                lv2(:, t) = chop(matmul(latvec, v), sqtol)
+                read (u, *) v
+                lv3(:, t) = chop(matmul(latvec, v), sqtol)
+                read (u, *) v
+                lv4(:, t) = chop(matmul(latvec, v), sqtol)
+                do i = 1, 3
+                    do j = 1, 3
+                        do l = 1, 3
+                            read (u, *) m(i, j, l, :, t)
+                        end do
+                    end do
+                end do
+            end do
+        end do
+        close (u)
+        m = m*ev_to_hartree/a_to_bohr**4
+    end subroutine
+
     subroutine read_fc3(path, na, latvec, nt, a1, a2, a3, lv2, lv3, m)
         character(len=*), intent(in) :: path
         integer, intent(in) :: na
@@ -426,14 +491,16 @@ program tc_host
     character(len=3) :: unit
     character(len=:), allocatable :: elist
     real(c_double), allocatable :: temps(:)
-    logical :: classical, noisotope, notr, offdiag, inputs_only
+    logical :: classical, noisotope, notr, offdiag, inputs_only, fourthorder
     integer(c_int) :: device
     type(c_ptr) :: ctx
     integer :: na, npair, nfc3, iT, st
     real(c_double) :: latvec(3, 3), wmin, times(8)
     real(c_double), allocatable :: rfrac(:, :), mass(:), disorder(:), lv(:, :), m2(:, :, :), lv2(:, :), lv3(:, :), &
                                    m3(:, :, :, :)
-    integer(c_int), allocatable :: species(:), p1(:), p2(:), t1(:), t2(:), t3(:)
+    integer(c_int), allocatable :: species(:), p1(:), p2(:), t1(:), t2(:), t3(:), f1(:), f2(:), f3(:), f4(:)
+    real(c_double), allocatable :: fl2(:, :), fl3(:, :), fl4(:, :), m4(:, :, :, :, :)
+    integer :: nfc4
     real(c_double), allocatable, target :: ksma(:, :, :), kiter(:, :, :), kod(:, :, :)
     integer(c_int), allocatable, target :: nit(:)
     character(len=:), allocatable :: sfx
@@ -453,6 +520,10 @@ program tc_host
                'tc3_set_boundary')
     call check(tc3_set_fc2(ctx, int(npair, c_int), p1, p2, lv, m2), 'tc3_set_fc2')
     call check(tc3_set_fc3(ctx, int(nfc3, c_int), t1, t2, t3, lv2, lv3, m3), 'tc3_set_fc3')
+    if (fourthorder) then
+        call read_fc4('infile.forceconstant_fourthorder', na, latvec, nfc4, f1, f2, f3, f4, fl2, fl3, fl4, m4)
+        call check(tc3_set_fc4(ctx, int(nfc4, c_int), f1, f2, f3, f4, fl2, fl3, fl4, m4), 'tc3_set_fc4')
+    end if
     call check(tc3_set_qmesh(ctx, qg, merge(0_c_int, 1_c_int, notr)), 'tc3_set_qmesh')
     call check(tc3_set_montecarlo(ctx, qg3, seed), 'tc3_set_montecarlo')
     if (inputs_only) then
@@ -542,6 +613,7 @@ contains
         notr = .false.
         offdiag = .true.
         inputs_only = .false.
+        fourthorder = .false.
         device = 0
         nT = 1
         tbuf(1) = 300.0_c_double
@@ -630,6 +702,7 @@ contains
             case ('--notr'); notr = .true.
             case ('--readiso')
             case ('--no-offdiag'); offdiag = .false.
+            case ('--fourthorder'); fourthorder = .true.
             case ('--inputs-only'); inputs_only = .true.
             case ('--device')
                 call get_command_argument(i + 1, val)

@@ -44,3 +44,26 @@ def test_fourphonon_matches_oracle(tmp_path, name, grid, it):
         xi, rq = e.scattering_matrix(iT)
         assert [tuple(x) for x in rq] == list(sr.rows)
         assert _rel(xi, sr.Xi) < tol
+
+
+def test_hosts_with_fourthorder(tmp_path):
+    """Both hosts run with --fourthorder on the same infiles write the same
+    outfile.thermal_conductivity; the four-phonon term lowers kappa."""
+    import subprocess
+    from tdep_amd import host, inputs
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdep_amd", "tc_host")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (no Fortran compiler)")
+    print("[case] hosts --fourthorder fcc_al 4x4x4", flush=True)
+    dirs = {k: tmp_path / k for k in ("f", "p", "p3")}
+    for d in dirs.values():
+        inputs.make_config("fcc_al", str(d), fourth=True)
+    args = ["-qg", "4", "4", "4", "--temperature", "300", "--readiso"]
+    r = subprocess.run([exe] + args + ["--fourthorder"], cwd=dirs["f"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert host.main(args + ["--fourthorder", "--directory", str(dirs["p"])]) == 0
+    assert host.main(args + ["--directory", str(dirs["p3"])]) == 0
+    a, b, c = (np.loadtxt(dirs[k] / "outfile.thermal_conductivity") for k in ("f", "p", "p3"))
+    assert np.allclose(a, b, rtol=1e-10, atol=1e-14 * np.abs(b).max())
+    # total kappa (last row): xx
+    assert b[-1][0] < c[-1][0] * (1 - 1e-6)
