@@ -209,10 +209,18 @@ int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int 
  * m[81*t + 27*l + 9*k + 3*j + i] = Phi_ijkl; the mass weighting (:296) is applied inside.
  * compute_fourphonon_scattering (scattering_fourphonon.f90:1-328) then adds its linewidth
  * and scattering-matrix terms with the full q-grid for the quartets (-qg4ph equal to the
- * mesh), FP64 whatever the contraction precision; cells of up to 2 atoms (6 bands), and not
- * together with tc3_set_montecarlo (tc3_scatter returns 5). */
+ * mesh unless tc3_set_montecarlo4 draws them), FP64 whatever the contraction precision;
+ * cells of up to 2 atoms (6 bands, else tc3_scatter returns 5). */
 int tc3_set_fc4(tc3_ctx *ctx, int nquart, const int *a1, const int *a2, const int *a3, const int *a4,
                 const double *lv2, const double *lv3, const double *lv4, const double *m);
+
+/* Monte-Carlo grids of the four-phonon term: replaces opts%qg4ph and mcg4%initialize
+ * (options.f90:226-236, scattering.f90:122-124).  qg4ph: 3 grid dimensions (NULL, or a
+ * value outside 1..qgrid, for the full mesh); the seed is tc3_set_montecarlo's.  Every row
+ * (q1, b1) then draws its three-phonon grid and two four-phonon grids in turn from one
+ * generator (compute_threephonon_scattering :69-70, compute_fourphonon_scattering :84-85),
+ * visits the (q2, q3) pairs of its two grids and weights them 1 / prod(qg4ph)^2. */
+int tc3_set_montecarlo4(tc3_ctx *ctx, const int *qg4ph);
 
 /* Write outfile.thermal_conductivity_grid.hdf5 for temperature iT (layout of
  * lo_phonon_dispersions%write_to_hdf5, type_phonon_dispersions.f90:316-641, after the

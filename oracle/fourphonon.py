@@ -88,23 +88,30 @@ def widths6(v1, v2, v3, v4, reclat, thresh):
     return s + thresh
 
 
-def row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sigma_fixed, thresh, rot, reduce=True):
+def row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sigma_fixed, thresh, rot, reduce=True,
+              pairs=None, npoints=None):
     """Linewidth contributions g[b1] and off-diagonal terms od[b1, q, b] of every row of q1
     (before the little-group average).  reduce=False sums every ordered (q2, q3) pair with
     channels 0 and 1 only (the brute-force form: channels 2 and 3 relabel channel 1), and
     channel 1 three times where q2 = q3 = q4 (the reference weights each of the three
-    channels once there, :114-119)."""
+    channels once there, :114-119).  pairs: the (q2, q3) draws of a Monte-Carlo row in loop
+    order (every pair visited as drawn, duplicates included) and npoints the points of one
+    grid; the terms then carry the weight 1 / npoints^2."""
     nb, nfull, dims = disp.nb, mesh.nfull, [int(x) for x in mesh.dims]
     om1 = disp.om_irr[q1]
     E = lambda e, w: e / np.sqrt(np.where(w > 0, w, 1.0))[None, :]
     E1 = E(disp.egv_irr[q1], om1)
-    w = 1.0 / nfull
+    w = 1.0 / (nfull if npoints is None else npoints)
     q1f = int(mesh.irr_full_index[q1])
     reclat6 = p.reclat / np.array(dims)[None, :]
     g = np.zeros(nb)
     od = np.zeros((nb, nfull, nb))
-    for q2 in range(nfull):
-        for q3 in range(q2 if reduce else 0, nfull):
+    if pairs is None:
+        pairs = [(q2, q3) for q2 in range(nfull) for q3 in range(q2 if reduce else 0, nfull)]
+    for (q2, q3) in pairs:
+        if reduce and q3 < q2:
+            continue
+        if True:
             q4 = fourth_grid_index(q1f, q2, q3, dims)
             if reduce:
                 if q4 < q3:
@@ -159,10 +166,11 @@ def row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sigma_fixed, t
     return g, od
 
 
-def symmetrize_od(mesh, rot, q1, od):
-    """symmetrize_and_distribute (scattering_fourphonon.f90:143-205) for every b1 of q1."""
+def symmetrize_od(mesh, rot, q1, od, grid1=None):
+    """symmetrize_and_distribute (scattering_fourphonon.f90:143-205) for every b1 of q1, over
+    the points of the first Monte-Carlo grid in drawn order (all points on the full mesh)."""
     lg = mesh.littlegroup_irr[q1]
-    for q2 in range(mesh.nfull):
+    for q2 in (range(mesh.nfull) if grid1 is None else grid1):
         imgs = rot[lg, q2]
         imgs = imgs[imgs >= 0]
         if np.any(imgs < q2):

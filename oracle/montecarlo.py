@@ -92,3 +92,39 @@ def row_grids(dims, rows, mc_dims, seed):
             prev = test
         out[r] = np.array(grid, dtype=np.int64)
     return out, npoints
+
+
+def row_grids_4ph(dims, rows, mc3, mc4, seed):
+    """The grids of every row (q1, b1) when four-phonon scattering is on: per row the
+    three-phonon q2 grid (drawn even on the full mesh, compute_threephonon_scattering
+    :69-70), then the two four-phonon grids (scattering_fourphonon.f90:84-85), all from one
+    generator in row order.  Returns {row: (g3, g4a, g4b)}, npoints3, npoints4."""
+    rng = MersenneTwister64()
+    rng.init_tdep(0, 1.0 / float(seed))
+
+    def draw(mcd):
+        ratio = [dims[i] / mcd[i] for i in range(3)]
+        npts = mcd[0] * mcd[1] * mcd[2]
+
+        def point_to_full(imc):
+            gm = (imc // (mcd[1] * mcd[2]), (imc // mcd[2]) % mcd[1], imc % mcd[2])
+            gf = []
+            for d in range(3):
+                g = math.ceil(gm[d] * ratio[d] + rng.real1() * ratio[d])
+                gf.append(min(max(g, 1), dims[d]) - 1)
+            return (gf[0] * dims[1] + gf[1]) * dims[2] + gf[2]
+
+        grid = [point_to_full(0)]
+        prev = test = grid[0]
+        for qi in range(1, npts):
+            while test == prev:
+                test = point_to_full(qi)
+            grid.append(test)
+            prev = test
+        return np.array(grid, dtype=np.int64)
+
+    out = {}
+    for r in rows:
+        g3 = draw(mc3)
+        out[r] = (g3, draw(mc4), draw(mc4))
+    return out, mc3[0] * mc3[1] * mc3[2], mc4[0] * mc4[1] * mc4[2]

@@ -198,7 +198,7 @@ def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0
 
 def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
                      integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None, mfp_max=None,
-                     qg3ph=None, seed=None, fc4=None):
+                     qg3ph=None, seed=None, fc4=None, qg4ph=None):
     """lo_scattering_rates%generate (scattering.f90:58-358): three-phonon scattering with
     qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q), plus
     isotope scattering (:229-232) when per-atom mass-disorder parameters are given, plus
@@ -226,12 +226,20 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
     sr.rot = rot
     mcw = 1.0 / nfull
     grids = None
-    if qg3ph is not None:
-        from . import montecarlo as MC
-        mcd = MC.mc_dims_of(list(mesh.dims), list(qg3ph))
-        if mcd != list(mesh.dims):
-            grids, npts = MC.row_grids(list(mesh.dims), rows, mcd, seed)
-            mcw = 1.0 / npts
+    grids4 = None
+    from . import montecarlo as MC
+    mc3 = MC.mc_dims_of(list(mesh.dims), list(qg3ph)) if qg3ph is not None else list(mesh.dims)
+    mc4 = MC.mc_dims_of(list(mesh.dims), list(qg4ph)) if qg4ph is not None else list(mesh.dims)
+    if fc4 is not None and (mc3 != list(mesh.dims) or mc4 != list(mesh.dims)):
+        # with four-phonon scattering every row draws its three grids from one generator
+        allg, npts, npts4 = MC.row_grids_4ph(list(mesh.dims), rows, mc3, mc4, seed)
+        grids = {r: g[0] for r, g in allg.items()}
+        mcw = 1.0 / npts
+        if mc4 != list(mesh.dims):
+            grids4 = {r: (g[1], g[2], npts4) for r, g in allg.items()}
+    elif mc3 != list(mesh.dims):
+        grids, npts = MC.row_grids(list(mesh.dims), rows, mc3, seed)
+        mcw = 1.0 / npts
     ntrip = 0
     for q1 in range(nirr):
         myb = [b for b in range(nb) if (q1, b) in row_of]
@@ -305,12 +313,22 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
             Xi[row_of[(q1, b1)]] += od[b1].reshape(-1)
         if fc4 is not None:
             from . import fourphonon as F4
-            g4, od4 = F4.row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sr.sigma_fixed,
-                                   sr.thresh_sigma, rot)
-            od4 = F4.symmetrize_od(mesh, rot, q1, od4)
-            for b1 in myb:
-                gamma_row[row_of[(q1, b1)]] += g4[b1]
-                Xi[row_of[(q1, b1)]] += od4[b1].reshape(-1)
+            if grids4 is None:
+                g4, od4 = F4.row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sr.sigma_fixed,
+                                       sr.thresh_sigma, rot)
+                od4 = F4.symmetrize_od(mesh, rot, q1, od4)
+                for b1 in myb:
+                    gamma_row[row_of[(q1, b1)]] += g4[b1]
+                    Xi[row_of[(q1, b1)]] += od4[b1].reshape(-1)
+            else:
+                for b1 in myb:
+                    ga, gb, n4 = grids4[(q1, b1)]
+                    pairs = [(int(x), int(y)) for x in ga for y in gb]
+                    g4, od4 = F4.row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sr.sigma_fixed,
+                                           sr.thresh_sigma, rot, pairs=pairs, npoints=n4)
+                    od4 = F4.symmetrize_od(mesh, rot, q1, od4, grid1=[int(x) for x in ga])
+                    gamma_row[row_of[(q1, b1)]] += g4[b1]
+                    Xi[row_of[(q1, b1)]] += od4[b1].reshape(-1)
     sr.ntriplets = ntrip
     if isotope_g is not None:
         for i, (q1, b1) in enumerate(rows):

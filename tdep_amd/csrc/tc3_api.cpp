@@ -224,6 +224,23 @@ int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal) {
     });
 }
 
+int tc3_set_montecarlo4(tc3_ctx *ctx, const int *qg4ph) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->have_mesh) return ctx->fail(5, "set the q-mesh first");
+        bool on = false;
+        for (int i = 0; i < 3; ++i) {
+            const int d = (!qg4ph || qg4ph[i] < 1 || qg4ph[i] > ctx->mesh.dims[i]) ? ctx->mesh.dims[i] : qg4ph[i];
+            ctx->mc4_dims[i] = d;
+            on = on || d < ctx->mesh.dims[i];
+        }
+        ctx->mc4_on = on;
+        ctx->mc_gen = -1;
+        ++ctx->gen_model;
+        ctx->have_scatter = false;
+        return 0;
+    });
+}
+
 int tc3_set_montecarlo(tc3_ctx *ctx, const int *qg3ph, int seed) {
     return guarded(ctx, [&]() -> int {
         if (!ctx->have_mesh) return ctx->fail(5, "set the q-mesh first");
@@ -395,10 +412,18 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
         for (double &s : ctx->sigsq) s += base;
         ctx->thresh6 = base;
         // triplets of the pass: the Monte-Carlo subset with its draw counts, or all
-        if (ctx->mc_on) {
+        // with four-phonon scattering every row draws its three-phonon grid and then its two
+        // four-phonon grids from the one generator (full-mesh grids included)
+        const bool f4 = ctx->nfc4 > 0;
+        if (ctx->mc_on || (f4 && ctx->mc4_on)) {
+            if (ctx->mc_seed <= 0)
+                return ctx->fail(5, "Monte-Carlo grids need a positive seed (a wall-clock seed is not reproducible)");
             if (ctx->mc_gen != ctx->gen_model) {
-                Status ms = montecarlo_triplets(ctx->mesh, ctx->trip_full, ctx->om_irr, nb, ctx->mc_dims, ctx->mc_seed,
-                                                ctx->trip, ctx->trip_w, ctx->mc_npoints);
+                int d3[3];
+                for (int i = 0; i < 3; ++i) d3[i] = ctx->mc_on ? ctx->mc_dims[i] : ctx->mesh.dims[i];
+                Status ms = montecarlo_triplets(ctx->mesh, ctx->trip_full, ctx->om_irr, nb, d3, ctx->mc_seed, ctx->trip,
+                                                ctx->trip_w, ctx->mc_npoints, f4 ? ctx->mc4_dims : nullptr, &ctx->g4,
+                                                &ctx->mc4_npoints);
                 if (ms.code) return ctx->fail(ms.code, ms.msg);
                 ctx->mc_gen = ctx->gen_model;
             }

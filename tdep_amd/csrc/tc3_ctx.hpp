@@ -52,6 +52,12 @@ struct tc3_ctx {
     bool mc_on = false;
     long mc_gen = -1, mc_npoints = 0;
     std::vector<float> trip_w;
+    // Monte-Carlo grids of the four-phonon term (-qg4ph): per row q1 * nb + b1 the two drawn
+    // grids, 2 * mc4_npoints full-grid indices
+    int mc4_dims[3] = {0, 0, 0};
+    bool mc4_on = false;
+    long mc4_npoints = 0;
+    std::vector<long> g4;
     tc3::OrbitSet orb;
 
     // harmonic results (host copies); egv stored [q][component][mode]

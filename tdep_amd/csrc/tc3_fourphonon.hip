@@ -39,7 +39,8 @@ struct F4Args {
     const int *irr_of_full, *row_of;
     const double *u_irr, *u_full;       // reduced group velocities (integration type 6)
     int itype;
-    double sigma_fixed, thresh, pref;   // pref = pi/96 / N_q^2
+    double sigma_fixed, thresh, pref;   // pref = pi/96 / N^2 (N points of one grid)
+    const float *tw;                    // [t][nb] draw counts of Monte-Carlo rows, or null
     double *g, *s;                      // [t][T][nb], [t][T][3][nb][nb]
 };
 
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(kF4Block) k_fourphonon(F4Args a) {
                 const double d1 = gauss(w1, -w2 + w3 + w4, sg) - gauss(w1, w2 - w3 - w4, sg);
                 const double d2 = gauss(w1, w2 - w3 + w4, sg) - gauss(w1, -w2 + w3 - w4, sg);
                 const double d3 = gauss(w1, w2 + w3 - w4, sg) - gauss(w1, -w2 - w3 + w4, sg);
-                const double p = ps[x];
+                const double p = ps[x] * (a.tw ? (double)a.tw[(size_t)t * nb + b1] : 1.0);
                 f = m0 * p * plf0 * d0 + m1 * p * plf1 * d1 + m1 * p * plf2 * d2 + m1 * p * plf3 * d3;
             }
             fb[x] = f;
@@ -204,48 +205,96 @@ int fourth_grid_index(int i1, int i2, int i3, const int *dims) {
     return (g[0] * n2 + g[1]) * n3 + g[2];
 }
 
-// quartet_is_irreducible (scattering_fourphonon.f90:330-456) for every irreducible q1
+// quartet_is_irreducible (scattering_fourphonon.f90:330-456): false for a reducible quartet,
+// else its images (permutations removed) appended to img
+bool quartet_images(const Mesh &m, int q1, int q2, int q3, int q4, std::vector<int> &img) {
+    const int np = m.nfull;
+    const std::vector<int> &lg = m.littlegroup[q1];
+    std::vector<std::array<int, 3>> nq(lg.size(), {q2, q3, q4}), ns(lg.size(), {q2, q3, q4});
+    for (size_t j = 0; j < lg.size(); ++j) {
+        const int o = lg[j];
+        const int a = m.rot[(size_t)o * np + q2], b = m.rot[(size_t)o * np + q3], cc = m.rot[(size_t)o * np + q4];
+        if (a < 0 || b < 0 || cc < 0) continue;
+        std::array<int, 3> s = {a, b, cc};
+        std::sort(s.begin(), s.end());
+        nq[j] = {a, b, cc};
+        ns[j] = s;
+        if (s[0] > q2 || s[1] > q3) return false;
+    }
+    for (size_t j = 0; j < lg.size(); ++j) {
+        bool later = false;
+        for (size_t k = j + 1; k < lg.size(); ++k) later = later || (ns[j] == ns[k]);
+        if (!later) img.insert(img.end(), {nq[j][0], nq[j][1], nq[j][2]});
+    }
+    return true;
+}
+
+int multiplicity_class(int q2, int q3, int q4) {
+    if (q2 == q3 && q3 == q4) return 0;
+    if (q3 == q4 || q2 == q4 || q2 == q3) return 1;
+    return 2;
+}
+
+// the irreducible quartets of every irreducible q1 on the full mesh
 void build_quartets(tc3_ctx *c) {
     const Mesh &m = c->mesh;
     const int np = m.nfull;
     c->quart.clear();
     c->quart_img_off.assign(1, 0);
     c->quart_img.clear();
-    std::vector<std::array<int, 3>> nq, ns;
     for (int q1 = 0; q1 < m.nirr; ++q1) {
-        const std::vector<int> &lg = m.littlegroup[q1];
         const int q1f = m.irr_full_index[q1];
         for (int q2 = 0; q2 < np; ++q2)
             for (int q3 = q2; q3 < np; ++q3) {
                 const int q4 = fourth_grid_index(q1f, q2, q3, m.dims);
                 if (q4 < q3) continue;
-                nq.assign(lg.size(), {q2, q3, q4});
-                ns.assign(lg.size(), {q2, q3, q4});
-                bool isred = false;
-                for (size_t j = 0; j < lg.size(); ++j) {
-                    const int o = lg[j];
-                    const int a = m.rot[(size_t)o * np + q2], b = m.rot[(size_t)o * np + q3], cc = m.rot[(size_t)o * np + q4];
-                    if (a < 0 || b < 0 || cc < 0) continue;
-                    std::array<int, 3> s = {a, b, cc};
-                    std::sort(s.begin(), s.end());
-                    nq[j] = {a, b, cc};
-                    ns[j] = s;
-                    if (s[0] > q2 || s[1] > q3) isred = true;
-                }
-                if (isred) continue;
-                int mc = 2;
-                if (q2 == q3 && q3 == q4) mc = 0;
-                else if (q3 == q4 || q2 == q4 || q2 == q3) mc = 1;
-                c->quart.insert(c->quart.end(), {q1, q2, q3, q4, mc});
-                for (size_t j = 0; j < lg.size(); ++j) {
-                    bool later = false;
-                    for (size_t k = j + 1; k < lg.size(); ++k) later = later || (ns[j] == ns[k]);
-                    if (!later) c->quart_img.insert(c->quart_img.end(), {nq[j][0], nq[j][1], nq[j][2]});
-                }
+                if (!quartet_images(m, q1, q2, q3, q4, c->quart_img)) continue;
+                c->quart.insert(c->quart.end(), {q1, q2, q3, q4, multiplicity_class(q2, q3, q4)});
                 c->quart_img_off.push_back((int)c->quart_img.size() / 3);
             }
     }
     c->quart_gen = c->gen_model;
+}
+
+// the quartets visited by the Monte-Carlo rows (the (q2, q3) pairs of each row's two grids,
+// compute_fourphonon_scattering :87-95), with the draw count of every row in tw [t][nb]
+void build_quartets_mc(const tc3_ctx *c, std::vector<int> &quart, std::vector<int> &img_off, std::vector<int> &img,
+                       std::vector<float> &tw) {
+    const Mesh &m = c->mesh;
+    const int np = m.nfull, nb = c->nb;
+    const long n4 = c->mc4_npoints;
+    quart.clear();
+    img_off.assign(1, 0);
+    img.clear();
+    tw.clear();
+    std::vector<int> slot((size_t)np * np);
+    for (int q1 = 0; q1 < m.nirr; ++q1) {
+        std::fill(slot.begin(), slot.end(), -2);   // -2 unseen, -1 reducible, else quartet
+        const int q1f = m.irr_full_index[q1];
+        for (int b1 = 0; b1 < nb; ++b1) {
+            if (c->row_of[(size_t)q1 * nb + b1] < 0) continue;
+            const long *ga = &c->g4[((size_t)q1 * nb + b1) * 2 * n4], *gb = ga + n4;
+            for (long qi = 0; qi < n4; ++qi)
+                for (long qj = 0; qj < n4; ++qj) {
+                    const int q2 = (int)ga[qi], q3 = (int)gb[qj];
+                    if (q3 < q2) continue;
+                    const int q4 = fourth_grid_index(q1f, q2, q3, m.dims);
+                    if (q4 < q3) continue;
+                    int &s = slot[(size_t)q2 * np + q3];
+                    if (s == -2) {
+                        if (quartet_images(m, q1, q2, q3, q4, img)) {
+                            s = (int)quart.size() / 5;
+                            quart.insert(quart.end(), {q1, q2, q3, q4, multiplicity_class(q2, q3, q4)});
+                            img_off.push_back((int)img.size() / 3);
+                            tw.resize(tw.size() + nb, 0.0f);
+                        } else {
+                            s = -1;
+                        }
+                    }
+                    if (s >= 0) tw[(size_t)s * nb + b1] += 1.0f;
+                }
+        }
+    }
 }
 
 }  // namespace
@@ -253,8 +302,14 @@ void build_quartets(tc3_ctx *c) {
 int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, std::vector<double> &od4) {
     const int nb = c->nb, na = c->cr.na, nT = in.nT, nirr = c->mesh.nirr, np = c->mesh.nfull, nrows = (int)c->row_q.size();
     if (nb > kF4MaxNb) return c->fail(5, "four-phonon scattering handles at most 6 bands");
-    if (c->quart_gen != c->gen_model) build_quartets(c);
-    const int nqt = (int)c->quart.size() / 5;
+    const bool mc = c->mc4_on;
+    std::vector<int> mq, mio, mim;
+    std::vector<float> mtw;
+    if (mc) build_quartets_mc(c, mq, mio, mim, mtw);
+    else if (c->quart_gen != c->gen_model) build_quartets(c);
+    const std::vector<int> &quart = mc ? mq : c->quart, &qoff = mc ? mio : c->quart_img_off,
+                           &qimg = mc ? mim : c->quart_img;
+    const int nqt = (int)quart.size() / 5;
     g4.assign((size_t)nT * nrows, 0.0);
     od4.assign((size_t)nT * nrows * np * nb, 0.0);
     if (nqt == 0) return 0;
@@ -269,7 +324,12 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
     F4Args a{};
     a.nb = nb; a.na = na; a.nT = nT; a.nirr = nirr;
     void *p;
-    TRY(up("f4_q", c->quart.data(), sizeof(int) * c->quart.size(), &p)); a.fq = (const int *)p;
+    TRY(up("f4_q", quart.data(), sizeof(int) * quart.size(), &p)); a.fq = (const int *)p;
+    a.tw = nullptr;
+    if (mc) {
+        TRY(up("f4_tw", mtw.data(), sizeof(float) * mtw.size(), &p));
+        a.tw = (const float *)p;
+    }
     TRY(up("f4_blk", c->blk4_off.data(), sizeof(int) * c->blk4_off.size(), &p)); a.blk4 = (const int *)p;
     TRY(up("f4_lv", c->f4_lv.data(), sizeof(double) * c->f4_lv.size(), &p)); a.lv = (const double *)p;
     TRY(up("f4_mwm", c->f4_mwm.data(), sizeof(double) * c->f4_mwm.size(), &p)); a.mwm = (const double *)p;
@@ -278,7 +338,8 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
     a.be = in.d_be; a.sigsq = in.d_sigsq; a.irr_of_full = c->d_irr_of_full; a.row_of = in.d_rowof;
     a.u_irr = in.d_uirr; a.u_full = in.d_ufull;
     a.itype = in.integrationtype; a.sigma_fixed = in.sigma_fixed; a.thresh = in.thresh;
-    a.pref = (kPi / 96.0) / ((double)np * (double)np);
+    const double npg = mc ? (double)c->mc4_npoints : (double)np;   // points of one grid (mcg%weight)
+    a.pref = (kPi / 96.0) / (npg * npg);
     TRY(c->pool_get("f4_g", sizeof(double) * (size_t)nqt * nT * nb, &p)); a.g = (double *)p;
     TRY(c->pool_get("f4_s", sizeof(double) * (size_t)nqt * nT * 3 * nb * nb, &p)); a.s = (double *)p;
     hipLaunchKernelGGL(k_fourphonon, dim3(nqt), dim3(kF4Block), 0, st, a);
@@ -293,7 +354,7 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
     for (int r = 0; r < nrows; ++r) row_of[(size_t)c->row_q[r] * nb + c->row_b[r]] = r;
     const size_t ncol = (size_t)np * nb;
     for (int t = 0; t < nqt; ++t) {
-        const int q1 = c->quart[5 * t], qq[3] = {c->quart[5 * t + 1], c->quart[5 * t + 2], c->quart[5 * t + 3]};
+        const int q1 = quart[5 * t], qq[3] = {quart[5 * t + 1], quart[5 * t + 2], quart[5 * t + 3]};
         for (int T = 0; T < nT; ++T) {
             const double *gt = &hg[((size_t)t * nT + T) * nb];
             const double *stt = &hs[((size_t)t * nT + T) * 3 * nb * nb];
@@ -302,10 +363,10 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
                 if (r < 0) continue;
                 const double w1 = c->om_irr[(size_t)q1 * nb + b1];
                 double *od = &od4[((size_t)T * nrows + r) * ncol];
-                for (int i = c->quart_img_off[t]; i < c->quart_img_off[t + 1]; ++i) {
+                for (int i = qoff[t]; i < qoff[t + 1]; ++i) {
                     g4[(size_t)T * nrows + r] += gt[b1];
                     for (int leg = 0; leg < 3; ++leg) {
-                        const int qp = c->quart_img[3 * i + leg];
+                        const int qp = qimg[3 * i + leg];
                         // frequency of the quartet's own point (the host holds those of the
                         // irreducible points; a full-grid point has its irreducible point's)
                         const double *wq = &c->om_irr[(size_t)c->mesh.irr_of_full[qq[leg]] * nb];
@@ -316,15 +377,19 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
             }
         }
     }
-    // symmetrize_and_distribute (scattering_fourphonon.f90:143-205)
+    // symmetrize_and_distribute (scattering_fourphonon.f90:143-205), over the row's first
+    // grid in drawn order (every point on the full mesh)
     std::vector<double> buf(nb);
     std::vector<int> nn(nb), imgs;
+    const long nlist = mc ? c->mc4_npoints : np;
     for (int T = 0; T < nT; ++T)
         for (int r = 0; r < nrows; ++r) {
             const int q1 = c->row_q[r];
             const std::vector<int> &lg = c->mesh.littlegroup[q1];
             double *od = &od4[((size_t)T * nrows + r) * ncol];
-            for (int q2 = 0; q2 < np; ++q2) {
+            const long *ga = mc ? &c->g4[((size_t)q1 * nb + c->row_b[r]) * 2 * c->mc4_npoints] : nullptr;
+            for (long qi = 0; qi < nlist; ++qi) {
+                const int q2 = mc ? (int)ga[qi] : (int)qi;
                 imgs.clear();
                 bool skip = false;
                 for (int o : lg) {

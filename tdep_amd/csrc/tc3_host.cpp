@@ -557,31 +557,50 @@ uint64_t MersenneTwister64::next() {
 double MersenneTwister64::real1() { return (double)(next() >> 11) * (1.0 / 9007199254740991.0); }
 
 Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vector<double> &om_irr, int nb,
-                           const int *mc_dims_in, int seed, TripletSet &sub, std::vector<float> &w, long &npoints) {
+                           const int *mc_dims_in, int seed, TripletSet &sub, std::vector<float> &w, long &npoints,
+                           const int *mc4_dims, std::vector<long> *g4, long *npoints4) {
     Status st;
-    int mcd[3];
-    double ratio[3];
+    int mcd[3], mcd4[3];
+    double ratio[3], ratio4[3];
     for (int i = 0; i < 3; ++i) {
         mcd[i] = std::max(1, std::min(mc_dims_in[i], m.dims[i]));
         ratio[i] = (double)m.dims[i] / (double)mcd[i];
+        mcd4[i] = mc4_dims ? std::max(1, std::min(mc4_dims[i], m.dims[i])) : 1;
+        ratio4[i] = (double)m.dims[i] / (double)mcd4[i];
     }
     npoints = (long)mcd[0] * mcd[1] * mcd[2];
+    const long np4 = (long)mcd4[0] * mcd4[1] * mcd4[2];
+    if (mc4_dims) {
+        *npoints4 = np4;
+        g4->assign((size_t)m.nirr * nb * 2 * np4, 0);
+    }
     // scattering.f90:107-115, one MPI rank: integer seed 0, real seed 1 / seed
     MersenneTwister64 rng;
     rng.init_tdep(0, 1.0 / (double)seed);
     // mc_point_to_full (lo_fftgrid_helper.f90:56-73), 0-based point indices
-    auto point_to_full = [&](long imc) {
-        const int k = (int)(imc % mcd[2]), j = (int)((imc / mcd[2]) % mcd[1]), i = (int)(imc / ((long)mcd[1] * mcd[2]));
+    auto point_to_full_d = [&](long imc, const int *md, const double *rt) {
+        const int k = (int)(imc % md[2]), j = (int)((imc / md[2]) % md[1]), i = (int)(imc / ((long)md[1] * md[2]));
         const int gm[3] = {i, j, k};
         int gf[3];
         for (int d = 0; d < 3; ++d) {
             const double r = rng.real1();
             // 1-based ceiling((g - 1) ratio + r ratio); a draw of exactly 0 (probability 2^-53)
             // is kept on its own cell
-            const int g = (int)std::ceil((double)gm[d] * ratio[d] + r * ratio[d]);
+            const int g = (int)std::ceil((double)gm[d] * rt[d] + r * rt[d]);
             gf[d] = std::max(1, std::min(g, m.dims[d])) - 1;
         }
         return ((long)gf[0] * m.dims[1] + gf[1]) * m.dims[2] + gf[2];
+    };
+    auto point_to_full = [&](long imc) { return point_to_full_d(imc, mcd, ratio); };
+    // generate_grid (lo_fftgrid_helper.f90:75-99) of the four-phonon grids
+    auto draw4 = [&](long *out) {
+        out[0] = point_to_full_d(0, mcd4, ratio4);
+        long prev = out[0], test = out[0];
+        for (long qi = 1; qi < np4; ++qi) {
+            while (test == prev) test = point_to_full_d(qi, mcd4, ratio4);
+            out[qi] = test;
+            prev = test;
+        }
     };
     const long np = m.nfull;
     std::vector<long> tof(np);
@@ -604,6 +623,12 @@ Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vec
             for (long qi = 0; qi < npoints; ++qi) {
                 const long t = tof[grid[qi]];
                 if (t >= 0) wfull[(size_t)t * nb + b1] += 1.0f;
+            }
+            // four-phonon grids of the row (scattering_fourphonon.f90:84-85)
+            if (mc4_dims) {
+                long *o = &(*g4)[((size_t)q1 * nb + b1) * 2 * np4];
+                draw4(o);
+                draw4(o + np4);
             }
         }
     }

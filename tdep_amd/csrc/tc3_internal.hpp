@@ -115,7 +115,8 @@ struct MersenneTwister64 {
 // w[t * nb + b1] counts how often triplet t is drawn for row (q1, b1).  npoints = the
 // number of points per draw (the weight of a draw is 1 / npoints).
 Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vector<double> &om_irr, int nb,
-                           const int *mc_dims, int seed, TripletSet &sub, std::vector<float> &w, long &npoints);
+                           const int *mc_dims, int seed, TripletSet &sub, std::vector<float> &w, long &npoints,
+                           const int *mc4_dims = nullptr, std::vector<long> *g4 = nullptr, long *npoints4 = nullptr);
 Status build_orbits(const Mesh &m, OrbitSet &o);
 
 }  // namespace tc3

@@ -47,18 +47,6 @@ struct EpiArgs {
 // Reduced group velocities u_j = sum_i v_i reclat_ij / dims_j of every mode (the operand of
 // the integration type 6 widths, scattering.f90:138-140).
 struct Mat3 { double m[9]; };
-// Xi[T][row][c] += od[T][row][c] (four-phonon off-diagonal terms, FP64 sums)
-template <typename XT>
-__global__ void k_add_rows(XT *__restrict__ xi, const double *__restrict__ od, int nT, int nrows, size_t ncol,
-                           size_t xi_row, size_t xi_T) {
-    const size_t n = (size_t)nT * nrows * ncol;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const size_t cc = i % ncol, r = (i / ncol) % nrows, T = i / (ncol * nrows);
-        XT &x = xi[T * xi_T + r * xi_row + cc];
-        x = (XT)((double)x + od[i]);
-    }
-}
-
 __global__ void k_reduced_vel(long nmode, const double *__restrict__ vel, Mat3 r, double *__restrict__ u) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nmode; i += (long)gridDim.x * blockDim.x) {
         const double *v = vel + 3 * i;
@@ -392,8 +380,14 @@ __device__ inline double gauss_iso(double x, double mu, double sigma) {
 template <typename XT, bool ISO>
 __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const int *__restrict__ q1_off,
                          const int *__restrict__ orb_off, const int *__restrict__ members,
-                         XT *__restrict__ xi, size_t xi_row, size_t xi_T, IsoArgs iso) {
+                         XT *__restrict__ xi, size_t xi_row, size_t xi_T, IsoArgs iso,
+                         const double *__restrict__ add4 = nullptr, size_t add_row = 0) {
+    // add4: four-phonon terms [T][row][q * nb + b], already averaged over the little group of
+    // q1 and added, like the isotope term, after the positive mean of the three-phonon terms
+    // and before the plain mean (compute_fourphonon_scattering adds them to Xi after
+    // compute_threephonon_scattering has averaged its own, scattering.f90:233-242)
     const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
+    const double *a4 = add4 ? add4 + ((size_t)T * nrows + r) * add_row : nullptr;
     const int q1 = row_q[r];
     XT *x = xi + (size_t)T * xi_T + (size_t)r * xi_row;
     const int o0 = q1_off[q1], o1 = q1_off[q1 + 1];
@@ -423,7 +417,14 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
         }
         double s = sa;
         if constexpr (!ISO) {
-            if (n > 0) {
+            if (a4) {
+                const double avg = (n > 0) ? sp / n : 0.0;
+                s = 0;
+                for (int m = m0; m < m1; ++m) {
+                    const int q2 = members[m];
+                    s += ((n > 0) ? avg : (double)x[(size_t)q2 * nb + b]) + a4[(size_t)q2 * nb + b];
+                }
+            } else if (n > 0) {
                 const double avg = sp / n;
                 s = 0;
                 for (int m = m0; m < m1; ++m) s += avg;
@@ -434,6 +435,7 @@ __global__ void k_xi_sym(int nrows, int nb, const int *__restrict__ row_q, const
             for (int m = m0; m < m1; ++m) {
                 const int q2 = members[m];
                 double v = (n > 0) ? avg : (double)x[(size_t)q2 * nb + b];
+                if (a4) v += a4[(size_t)q2 * nb + b];
                 const double om2 = iso.om_full[(size_t)q2 * nb + b];
                 if (!(om2 < kFreqTol)) {
                     double st = 0;
@@ -1145,29 +1147,17 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     // four-phonon terms (scattering.f90:239-242): off-diagonal terms into the rows before the
     // symmetrisation below, linewidth terms after the isotope term
     std::vector<double> g4;
+    double *d_od4 = nullptr;   // four-phonon terms for the symmetrisation below
     if (c->nfc4 > 0) {
         if (use_mfma) { cleanup(); return c->fail(5, "four-phonon scattering handles at most 6 bands"); }
-        if (!c->trip_w.empty()) {
-            cleanup();
-            return c->fail(5, "four-phonon scattering with a Monte-Carlo three-phonon grid is not supported");
-        }
         F4Inputs fi;
         fi.nT = nT; fi.integrationtype = integrationtype; fi.sigma_fixed = ea.sigma_fixed; fi.thresh = c->thresh6;
         fi.d_be = d_be; fi.d_sigsq = d_sigsq; fi.d_uirr = d_uirr; fi.d_ufull = d_ufull; fi.d_rowof = d_rowof;
         std::vector<double> od4;
         const int r4 = fourphonon_terms(c, fi, g4, od4);
         if (r4) { cleanup(); return r4; }
-        double *d_od4;
         TRY(c->pool_get("f4_od", sizeof(double) * std::max<size_t>(od4.size(), 1), (void **)&d_od4));
         TRY(hipMemcpyAsync(d_od4, od4.data(), sizeof(double) * od4.size(), hipMemcpyHostToDevice, st));
-        const size_t ncol = (size_t)nfull * nb;
-        if (c->xi_es == 8)
-            hipLaunchKernelGGL(k_add_rows<double>, dim3(1024), dim3(256), 0, st, (double *)c->d_xi, (const double *)d_od4,
-                               nT, nrows, ncol, c->xi_row, xi_T);
-        else
-            hipLaunchKernelGGL(k_add_rows<float>, dim3(1024), dim3(256), 0, st, (float *)c->d_xi, (const double *)d_od4,
-                               nT, nrows, ncol, c->xi_row, xi_T);
-        TRY(hipGetLastError());
         TRY(hipStreamSynchronize(st));
     }
     hipEvent_t evs0 = c->ev_pool[2], evs1 = c->ev_pool[3];
@@ -1199,16 +1189,17 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         const size_t sha = (size_t)capm * nb * c->xi_es + sizeof(int) * (5 * (size_t)capm + 1);
         if (d_comp && (nb > kBlock || sha > 96 * 1024)) { cleanup(); return c->fail(5, "scattering-matrix assembly exceeds the LDS"); }
         if (d_comp) {
-            // rows from the compact sums; with isotopes the averaging below follows
+            // rows from the compact sums; with isotope or four-phonon terms the averaging below
+            // follows
             const size_t cT = (size_t)ntrip * nb * 2 * nb;
             if (c->xi_es == 8) {
-                auto kf = use_iso ? k_xi_asm<double, double, false> : k_xi_asm<double, double, true>;
+                auto kf = (use_iso || d_od4) ? k_xi_asm<double, double, false> : k_xi_asm<double, double, true>;
                 TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
                 hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, nfull, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
                                    (const int4 *)d_xisrc, (const double *)d_comp, cT, d_chkoff, d_chk, d_orboff,
                                    (double *)c->d_xi, c->xi_row, xi_T, capm);
             } else {
-                auto kf = use_iso ? k_xi_asm<float, float, false> : k_xi_asm<float, float, true>;
+                auto kf = (use_iso || d_od4) ? k_xi_asm<float, float, false> : k_xi_asm<float, float, true>;
                 TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
                 hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, nfull, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
                                    (const int4 *)d_xisrc, (const float *)d_comp, cT, d_chkoff, d_chk, d_orboff,
@@ -1216,33 +1207,37 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             }
             TRY(hipGetLastError());
         }
-        const bool use_lds = nb <= kBlock && shl <= 80 * 1024 && std::getenv("TC3_XI_GLOBAL") == nullptr;
+        const bool use_lds = nb <= kBlock && shl <= 80 * 1024 && std::getenv("TC3_XI_GLOBAL") == nullptr && !d_od4;
         if (use_lds) {
             const void *kf = (c->xi_es == 8) ? (const void *)k_xi_sym_lds<double> : (const void *)k_xi_sym_lds<float>;
             TRY(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shl));
         }
-        if (d_comp && !use_iso) {
+        if (d_comp && !use_iso && !d_od4) {
             // averaged by k_xi_asm
         } else if (c->xi_es == 8) {
             if (use_iso)
                 hipLaunchKernelGGL((k_xi_sym<double, true>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
-                                   d_members, (double *)c->d_xi, c->xi_row, xi_T, iso);
+                                   d_members, (double *)c->d_xi, c->xi_row, xi_T, iso, (const double *)d_od4,
+                                   (size_t)nfull * nb);
             else if (use_lds)
                 hipLaunchKernelGGL((k_xi_sym_lds<double>), g, bl, shl, st, nrows, nb, d_rowq, d_chkoff, d_chk,
                                    d_orboff, d_members, (double *)c->d_xi, c->xi_row, xi_T, capm);
             else
                 hipLaunchKernelGGL((k_xi_sym<double, false>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
-                                   d_members, (double *)c->d_xi, c->xi_row, xi_T, iso);
+                                   d_members, (double *)c->d_xi, c->xi_row, xi_T, iso, (const double *)d_od4,
+                                   (size_t)nfull * nb);
         } else {
             if (use_iso)
                 hipLaunchKernelGGL((k_xi_sym<float, true>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
-                                   d_members, (float *)c->d_xi, c->xi_row, xi_T, iso);
+                                   d_members, (float *)c->d_xi, c->xi_row, xi_T, iso, (const double *)d_od4,
+                                   (size_t)nfull * nb);
             else if (use_lds)
                 hipLaunchKernelGGL((k_xi_sym_lds<float>), g, bl, shl, st, nrows, nb, d_rowq, d_chkoff, d_chk,
                                    d_orboff, d_members, (float *)c->d_xi, c->xi_row, xi_T, capm);
             else
                 hipLaunchKernelGGL((k_xi_sym<float, false>), g, b, 0, st, nrows, nb, d_rowq, d_orbq1, d_orboff,
-                                   d_members, (float *)c->d_xi, c->xi_row, xi_T, iso);
+                                   d_members, (float *)c->d_xi, c->xi_row, xi_T, iso, (const double *)d_od4,
+                                   (size_t)nfull * nb);
         }
     }
     TRY(hipGetLastError());

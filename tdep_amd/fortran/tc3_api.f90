@@ -52,6 +52,12 @@ module tc3_api
             real(c_double), intent(in) :: lv2(3, *), lv3(3, *), m(3, 3, 3, *)
         end function
 
+        integer(c_int) function tc3_set_montecarlo4(ctx, qg4ph) bind(C, name="tc3_set_montecarlo4")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(in) :: qg4ph(3)
+        end function
+
         integer(c_int) function tc3_set_fc4(ctx, nquart, a1, a2, a3, a4, lv2, lv3, lv4, m) bind(C, name="tc3_set_fc4")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx

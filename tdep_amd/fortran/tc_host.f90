@@ -14,7 +14,8 @@
 ! Options (options.f90:44-237, the subset of this path):
 !   -qg n1 n2 n3 | --qpoint_grid n1 n2 n3     q-mesh (26 26 26)
 !   -qg3ph n1 n2 n3 | --qpoint_grid3ph ...     Monte-Carlo q2 grid of the three-phonon term (mesh)
-!   --seed s                                   seed of the Monte-Carlo draws (needed with -qg3ph)
+!   -qg4ph n1 n2 n3 | --qpoint_grid4ph ...     Monte-Carlo grids of the four-phonon term (mesh)
+!   --seed s                                   seed of the Monte-Carlo draws (needed with -qg3ph, -qg4ph)
 !   --temperature T [T ...]                    temperatures in K (300)
 !   -it 1|2|6 | --integrationtype 1|2|6        fixed, adaptive or velocity-difference Gaussian (2)
 !   --sigma s                                  fixed smearing in THz, integrationtype 1; also
@@ -486,7 +487,7 @@ program tc_host
     real(c_double), parameter :: hartree_to_joule = 27.21138602_c_double*1.6021766208e-19_c_double
     real(c_double), parameter :: time_au_to_s = (6.582119514e-16_c_double*1.6021766208e-19_c_double)/hartree_to_joule
     real(c_double), parameter :: kappa_au_to_si = hartree_to_joule/time_au_to_s/0.52917721067e-10_c_double
-    integer(c_int) :: qg(3), qg3(3), integrationtype, maxsteps, prec, nT, seed, mfppts, freqpts, dosint
+    integer(c_int) :: qg(3), qg3(3), qg4(3), integrationtype, maxsteps, prec, nT, seed, mfppts, freqpts, dosint
     real(c_double) :: sigma, tol, max_mfp, dossigma
     character(len=3) :: unit
     character(len=:), allocatable :: elist
@@ -526,6 +527,7 @@ program tc_host
     end if
     call check(tc3_set_qmesh(ctx, qg, merge(0_c_int, 1_c_int, notr)), 'tc3_set_qmesh')
     call check(tc3_set_montecarlo(ctx, qg3, seed), 'tc3_set_montecarlo')
+    call check(tc3_set_montecarlo4(ctx, qg4), 'tc3_set_montecarlo4')
     if (inputs_only) then
         call report_inputs()
     else
@@ -596,6 +598,7 @@ contains
         real(c_double) :: tbuf(1024)
         qg = [26, 26, 26]
         qg3 = [-1, -1, -1]
+        qg4 = [-1, -1, -1]
         seed = 0
         integrationtype = 2
         sigma = 1.0_c_double
@@ -642,6 +645,12 @@ contains
                 do k = 1, 3
                     call get_command_argument(i + k, val)
                     read (val, *) qg3(k)
+                end do
+                i = i + 3
+            case ('-qg4ph', '--qpoint_grid4ph')
+                do k = 1, 3
+                    call get_command_argument(i + k, val)
+                    read (val, *) qg4(k)
                 end do
                 i = i + 3
             case ('--seed')

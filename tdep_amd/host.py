@@ -35,6 +35,8 @@ def parse(argv=None):
     ap.add_argument("-qg3ph", "--qpoint_grid3ph", type=int, nargs=3, default=[-1, -1, -1],
                     help="Monte-Carlo q-point grid of the three-phonon term")
     ap.add_argument("--seed", type=int, default=-1, help="seed of the Monte-Carlo grids")
+    ap.add_argument("-qg4ph", "--qpoint_grid4ph", type=int, nargs=3, default=[-1, -1, -1],
+                    help="Monte-Carlo q-point grid of the four-phonon term")
     ap.add_argument("--fourthorder", action="store_true",
                     help="add four-phonon scattering (reads infile.forceconstant_fourthorder; full q-grid)")
     ap.add_argument("--mfppts", type=int, default=1000, help="points of the mean-free-path axes")
@@ -63,8 +65,10 @@ def main(argv=None):
     eng = Engine.from_directory(a.directory, tuple(a.qpoint_grid), device=a.device, timereversal=not a.notr)
     eng.set_isotopes(eng.structure.mass_disorder, enable=not a.noisotope)
     eng.set_boundary(a.max_mfp)
-    if max(a.qpoint_grid3ph) > 0:
-        eng.set_montecarlo(tuple(a.qpoint_grid3ph), a.seed)
+    if max(a.qpoint_grid3ph) > 0 or max(a.qpoint_grid4ph) > 0:
+        eng.set_montecarlo(tuple(a.qpoint_grid3ph) if max(a.qpoint_grid3ph) > 0 else None, a.seed)
+    if max(a.qpoint_grid4ph) > 0:
+        eng.set_montecarlo4(tuple(a.qpoint_grid4ph))
     if a.fourthorder:
         eng.set_fourthorder(a.directory)
     eng.dispersions()

@@ -41,6 +41,7 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_int64)]),
     "tc3_write_grid_hdf5": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]),
     "tc3_set_fc4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8),
+    "tc3_set_montecarlo4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "tc3_write_cumulative_hdf5": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_char_p, ctypes.c_char_p]),

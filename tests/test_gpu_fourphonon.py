@@ -67,3 +67,29 @@ def test_hosts_with_fourthorder(tmp_path):
     assert np.allclose(a, b, rtol=1e-10, atol=1e-14 * np.abs(b).max())
     # total kappa (last row): xx
     assert b[-1][0] < c[-1][0] * (1 - 1e-6)
+
+
+@pytest.mark.parametrize("name,grid,qg3,qg4", [("fcc_al", (4, 4, 4), None, (2, 2, 2)),
+                                               ("rocksalt", (3, 3, 3), (2, 2, 2), (2, 2, 2))])
+def test_fourphonon_montecarlo_matches_oracle(tmp_path, name, grid, qg3, qg4):
+    """-qg4ph (and -qg3ph) with --seed 7: every row draws its three-phonon grid and two
+    four-phonon grids in turn; linewidths and scattering matrix as the oracle's."""
+    from oracle import lattice as L, run as R, scattering as S
+    from tdep_amd import inputs
+    from tdep_amd.engine import Engine
+    print(f"[case] fourphonon Monte-Carlo {name} {grid} qg3ph={qg3} qg4ph={qg4}", flush=True)
+    d = str(tmp_path / name)
+    inputs.make_config(name, d, fourth=True)
+    e = Engine.from_directory(d, grid, device=0)
+    e.set_montecarlo(qg3, 7)
+    e.set_montecarlo4(qg4)
+    e.set_fourthorder(d)
+    e.dispersions()
+    e.scattering_rates([300.0], precision="fp64")
+    p, fc2, fc3, ops, mesh, disp = R.setup(d, grid)
+    fc4 = L.read_fc4(p, os.path.join(d, "infile.forceconstant_fourthorder"))
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, fc4=fc4, qg3ph=qg3, qg4ph=qg4, seed=7)
+    assert _rel(e.linewidths(0)[0], sr.linewidth) < 1e-9
+    xi, rq = e.scattering_matrix(0)
+    assert [tuple(x) for x in rq] == list(sr.rows)
+    assert _rel(xi, sr.Xi) < 1e-9

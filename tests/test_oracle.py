@@ -750,3 +750,20 @@ def test_quartet_rule_keeps_the_componentwise_dominant_member_of_each_orbit(fcc4
             dom = [m for m in orbit if all(m[0] >= x[0] and m[1] >= x[1] for x in orbit)]
             kept = [m for m in orbit if not F4.quartet_images(mesh, rot, q1, *m)[0]]
             assert kept == dom
+
+
+def test_fourphonon_grids_follow_the_three_phonon_grid_of_each_row():
+    """One generator: row r draws its three-phonon grid, then two four-phonon grids; the first
+    row's three-phonon grid is the one the three-phonon-only sequence draws, later rows differ,
+    and every grid is a stratified draw with no point repeating the one before it."""
+    from oracle import montecarlo as MC
+    dims, rows = [6, 6, 6], [(0, 1), (0, 2), (1, 0)]
+    allg, n3, n4 = MC.row_grids_4ph(dims, rows, [3, 3, 3], [2, 2, 2], 7)
+    only3, _ = MC.row_grids(dims, rows, [3, 3, 3], 7)
+    assert np.array_equal(allg[rows[0]][0], only3[rows[0]]) and not np.array_equal(allg[rows[1]][0], only3[rows[1]])
+    for g3, ga, gb in allg.values():
+        for g, mcd in ((g3, [3, 3, 3]), (ga, [2, 2, 2]), (gb, [2, 2, 2])):
+            assert len(g) == mcd[0] * mcd[1] * mcd[2] and np.all(g[1:] != g[:-1])
+            gi = np.stack([g // 36, (g // 6) % 6, g % 6], axis=1)
+            cells = [tuple(x) for x in (gi * np.array(mcd)) // 6]
+            assert len(set(cells)) == len(g)
