@@ -111,58 +111,57 @@ def row_terms(p, fc4, mesh, disp, be, sigsq, q1, integrationtype, sigma_fixed, t
     for (q2, q3) in pairs:
         if reduce and q3 < q2:
             continue
-        if True:
-            q4 = fourth_grid_index(q1f, q2, q3, dims)
-            if reduce:
-                if q4 < q3:
-                    continue
-                red, imgs = quartet_images(mesh, rot, q1, q2, q3, q4)
-                if red:
-                    continue
-                if q2 == q3 == q4:
-                    mult = (1.0, 1.0, 1.0, 1.0)
-                elif q3 == q4 or q2 == q4 or q2 == q3:
-                    mult = (3.0, 1.0, 1.0, 1.0)
-                else:
-                    mult = (6.0, 2.0, 2.0, 2.0)
+        q4 = fourth_grid_index(q1f, q2, q3, dims)
+        if reduce:
+            if q4 < q3:
+                continue
+            red, imgs = quartet_images(mesh, rot, q1, q2, q3, q4)
+            if red:
+                continue
+            if q2 == q3 == q4:
+                mult = (1.0, 1.0, 1.0, 1.0)
+            elif q3 == q4 or q2 == q4 or q2 == q3:
+                mult = (3.0, 1.0, 1.0, 1.0)
             else:
-                imgs = [(q2, q3, q4)]
-                mult = (1.0, 3.0, 0.0, 0.0) if q2 == q3 == q4 else (1.0, 1.0, 0.0, 0.0)
-            oms = [disp.om_full[q] for q in (q2, q3, q4)]
-            Es = [E(disp.egv_full[q], o) for q, o in zip((q2, q3, q4), oms)]
-            ptf = pretransform_phi4(fc4, mesh.qcart[q2], mesh.qcart[q3], mesh.qcart[q4], nb)
-            psi = np.einsum("abcd,ai,bj,ck,dl->ijkl", ptf, E1, Es[0], Es[1], Es[2], optimize=True)
-            psisq = FOURPHONON_PREFACTOR * np.abs(psi) ** 2 * w * w
-            n = [be[mesh.irr_of_full[q]] for q in (q2, q3, q4)]
-            n2, n3, n4 = n[0][None, :, None, None], n[1][None, None, :, None], n[2][None, None, None, :]
-            n2p, n3p, n4p = n2 + 1.0, n3 + 1.0, n4 + 1.0
-            plf0 = n2p * n3p * n4p - n2 * n3 * n4
-            plf1 = 3.0 * n2 * n3p * n4p - n2p * n3 * n4
-            plf2 = 3.0 * n3 * n2p * n4p - n3p * n2 * n4
-            plf3 = 3.0 * n4 * n3p * n2p - n4p * n3 * n2
-            w1 = om1[:, None, None, None]
-            w2, w3, w4 = oms[0][None, :, None, None], oms[1][None, None, :, None], oms[2][None, None, None, :]
-            if integrationtype == 2:
-                sg = [sigsq[q1]] + [sigsq[mesh.irr_of_full[q]] for q in (q2, q3, q4)]
-                sig = np.sqrt(sg[0][:, None, None, None] + sg[1][None, :, None, None] + sg[2][None, None, :, None]
-                              + sg[3][None, None, None, :])
-            elif integrationtype == 1:
-                sig = np.full((nb,) * 4, sigma_fixed)
-            else:
-                sig = widths6(disp.vel_irr[q1], disp.vel_full[q2], disp.vel_full[q3], disp.vel_full[q4],
-                              reclat6, thresh)
-            d0, d1, d2, d3 = dirac4(w1, w2, w3, w4, sig)
-            f = psisq * (mult[0] * plf0 * d0 + mult[1] * plf1 * d1 + mult[2] * plf2 * d2 + mult[3] * plf3 * d3)
-            mask = (w2 >= U.FREQTOL) & (w3 >= U.FREQTOL) & (w4 >= U.FREQTOL) & (w1 >= U.FREQTOL)
-            f = np.where(mask, f, 0.0)
-            tot = f.sum(axis=(1, 2, 3))
-            s2, s3, s4 = f.sum(axis=(2, 3)), f.sum(axis=(1, 3)), f.sum(axis=(1, 2))
-            inv1 = np.where(om1 > 0, 1.0 / np.where(om1 > 0, om1, 1.0), 0.0)
-            for (a, b, c) in imgs:
-                g += tot
-                od[:, a, :] += 2.0 * s2 * oms[0][None, :] * inv1[:, None]
-                od[:, b, :] += 2.0 * s3 * oms[1][None, :] * inv1[:, None]
-                od[:, c, :] += 2.0 * s4 * oms[2][None, :] * inv1[:, None]
+                mult = (6.0, 2.0, 2.0, 2.0)
+        else:
+            imgs = [(q2, q3, q4)]
+            mult = (1.0, 3.0, 0.0, 0.0) if q2 == q3 == q4 else (1.0, 1.0, 0.0, 0.0)
+        oms = [disp.om_full[q] for q in (q2, q3, q4)]
+        Es = [E(disp.egv_full[q], o) for q, o in zip((q2, q3, q4), oms)]
+        ptf = pretransform_phi4(fc4, mesh.qcart[q2], mesh.qcart[q3], mesh.qcart[q4], nb)
+        psi = np.einsum("abcd,ai,bj,ck,dl->ijkl", ptf, E1, Es[0], Es[1], Es[2], optimize=True)
+        psisq = FOURPHONON_PREFACTOR * np.abs(psi) ** 2 * w * w
+        n = [be[mesh.irr_of_full[q]] for q in (q2, q3, q4)]
+        n2, n3, n4 = n[0][None, :, None, None], n[1][None, None, :, None], n[2][None, None, None, :]
+        n2p, n3p, n4p = n2 + 1.0, n3 + 1.0, n4 + 1.0
+        plf0 = n2p * n3p * n4p - n2 * n3 * n4
+        plf1 = 3.0 * n2 * n3p * n4p - n2p * n3 * n4
+        plf2 = 3.0 * n3 * n2p * n4p - n3p * n2 * n4
+        plf3 = 3.0 * n4 * n3p * n2p - n4p * n3 * n2
+        w1 = om1[:, None, None, None]
+        w2, w3, w4 = oms[0][None, :, None, None], oms[1][None, None, :, None], oms[2][None, None, None, :]
+        if integrationtype == 2:
+            sg = [sigsq[q1]] + [sigsq[mesh.irr_of_full[q]] for q in (q2, q3, q4)]
+            sig = np.sqrt(sg[0][:, None, None, None] + sg[1][None, :, None, None] + sg[2][None, None, :, None]
+                          + sg[3][None, None, None, :])
+        elif integrationtype == 1:
+            sig = np.full((nb,) * 4, sigma_fixed)
+        else:
+            sig = widths6(disp.vel_irr[q1], disp.vel_full[q2], disp.vel_full[q3], disp.vel_full[q4],
+                          reclat6, thresh)
+        d0, d1, d2, d3 = dirac4(w1, w2, w3, w4, sig)
+        f = psisq * (mult[0] * plf0 * d0 + mult[1] * plf1 * d1 + mult[2] * plf2 * d2 + mult[3] * plf3 * d3)
+        mask = (w2 >= U.FREQTOL) & (w3 >= U.FREQTOL) & (w4 >= U.FREQTOL) & (w1 >= U.FREQTOL)
+        f = np.where(mask, f, 0.0)
+        tot = f.sum(axis=(1, 2, 3))
+        s2, s3, s4 = f.sum(axis=(2, 3)), f.sum(axis=(1, 3)), f.sum(axis=(1, 2))
+        inv1 = np.where(om1 > 0, 1.0 / np.where(om1 > 0, om1, 1.0), 0.0)
+        for (a, b, c) in imgs:
+            g += tot
+            od[:, a, :] += 2.0 * s2 * oms[0][None, :] * inv1[:, None]
+            od[:, b, :] += 2.0 * s3 * oms[1][None, :] * inv1[:, None]
+            od[:, c, :] += 2.0 * s4 * oms[2][None, :] * inv1[:, None]
     return g, od
 
 
