@@ -141,7 +141,7 @@ def test_autograd_functions_route_to_the_adjoint(inputs_dir):
     assert np.array_equal(e.fc3["m"], m.detach().numpy()) and list(e.temperatures) == [300.0]
     ref = e.adjoint_gamma(w.numpy())
     assert _nrel(m.grad.numpy(), ref) < 1e-12
-    assert np.isclose(float((m.grad * m).sum()), 2.0 * float(L), rtol=1e-10)
+    assert np.isclose(float((m.grad * m.detach()).sum()), 2.0 * float(L.detach()), rtol=1e-10)
     # kappa through autograd, including a composed linear parametrisation m = B x
     x = torch.ones(2, dtype=torch.float64, requires_grad=True)
     m0 = torch.tensor(e.fc3["m"])
@@ -176,7 +176,7 @@ def test_gradient_with_respect_to_symmetry_adapted_coefficients(inputs_dir):
         L.backward()
         g = x.grad.numpy().copy()
         if f is lw_of:
-            assert np.isclose(np.dot(g, x0), 2.0 * float(L), rtol=1e-10)
+            assert np.isclose(np.dot(g, x0), 2.0 * float(L.detach()), rtol=1e-10)
         for c in np.argsort(-np.abs(x0))[:2]:
             h = 1e-4 * abs(x0[c])
             xp, xm = x0.copy(), x0.copy()
