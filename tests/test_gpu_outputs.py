@@ -113,6 +113,37 @@ def test_host_writes_both_outfiles(inputs_dir, tmp_path):
     assert np.allclose(np.diag(kgrid), rows[0][:3] + rows[1][:3], rtol=1e-6)
 
 
+def test_hosts_with_the_reference_tests_flags(tmp_path):
+    """The reference test's command line (tests/thermal_conductivity/Makefile:1-6:
+    -qg 5 5 5 --fourthorder -qg4ph 3 3 3 --seed 42, natural isotopes) on a two-atom
+    rocksalt cell with Morse IFCs (the reference test's MgO is polar): the consistency
+    check of test_thermal_conductivity.py:25-40 holds with four-phonon and Monte-Carlo
+    terms, both hosts write the same outfile.thermal_conductivity, and the four-phonon term
+    lowers kappa."""
+    from tdep_amd import host, inputs
+    print("[case] reference test flags rocksalt 5x5x5 --fourthorder -qg4ph 3 3 3 --seed 42", flush=True)
+    dirs = {k: tmp_path / k for k in ("p", "f", "p3")}
+    for d in dirs.values():
+        inputs.make_config("rocksalt", str(d), fourth=True)
+    base = ["-qg", "5", "5", "5", "--temperature", "300", "--readiso"]
+    args = base + ["--fourthorder", "-qg4ph", "3", "3", "3", "--seed", "42"]
+    assert host.main(args + ["--directory", str(dirs["p"])]) == 0
+    assert host.main(base + ["--directory", str(dirs["p3"])]) == 0
+    rows = np.loadtxt(dirs["p"] / "outfile.thermal_conductivity")
+    rows3 = np.loadtxt(dirs["p3"] / "outfile.thermal_conductivity")
+    assert rows.shape == (4, 6) and np.all(np.isfinite(rows))
+    h = _read(str(dirs["p"] / "outfile.thermal_conductivity_grid.hdf5"))
+    kgrid = np.array(h["data"]["thermal_conductivity"]["values"]).sum(axis=(0, 1)) / 125
+    assert np.allclose(np.diag(kgrid), rows[0][:3] + rows[1][:3], rtol=1e-6)
+    assert 0 < rows[-1][0] < rows3[-1][0] * (1 - 1e-6)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdep_amd", "tc_host")
+    if os.path.exists(exe):
+        r = subprocess.run([exe] + args, cwd=dirs["f"], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        rf = np.loadtxt(dirs["f"] / "outfile.thermal_conductivity")
+        assert np.allclose(rf, rows, rtol=1e-10, atol=1e-14 * np.abs(rows).max())
+
+
 def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
     """The Fortran host (tdep_amd/tc_host, iso_c_binding to libtc3) run in a directory of
     infiles writes the same outfile.thermal_conductivity numbers (both temperatures), the
