@@ -591,7 +591,8 @@ __global__ void k_gemv_sum(int n3, const double *__restrict__ part_, double *__r
 __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf_,
                              const double *__restrict__ qs_, const double *__restrict__ F0_,
                              const int *__restrict__ dg_off, const int *__restrict__ dg,
-                             double *__restrict__ Fn_, double *__restrict__ g0part_, const BteState *__restrict__ st_,
+                             const int *__restrict__ fzero, double *__restrict__ Fn_, double *__restrict__ g0part_,
+                             const BteState *__restrict__ st_,
                              long sBuf, long sQs, long sF, long sG) {
     const int T = blockIdx.z;
     if (st_[T].done) return;
@@ -627,8 +628,17 @@ __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const doubl
         }
     }
     __syncthreads();
+    // fzero[q]: the little group of q fixes no vector (its rotations sum to zero), so F(q) = R F(q)
+    // for every R in it forces F(q) = 0; what the sum over columns leaves there is rounding,
+    // which single-precision scattering matrices raise above the |F|^2 > 1e-10 test below
+    const bool fz = fzero[q] != 0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         double *F = Fn + ((size_t)q * nb + b) * 3;
+        if (fz) {
+            F[0] = F[1] = F[2] = 0.0;
+            g1s[b] = 0.0;
+            continue;
+        }
         double g1 = 0, g2 = 0, f[3];
         for (int i = 0; i < 3; ++i) {
             f[i] = F[i];
@@ -812,6 +822,39 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     TRY(dalloc((void **)&d_rowof, sizeof(int) * qb));
     TRY(dalloc((void **)&d_dgoff, sizeof(int) * (qb + 1)));
     TRY(dalloc((void **)&d_dg, sizeof(int) * dg.size()));
+    int *d_fz;
+    {
+        // points where symmetry fixes no vector, so that F vanishes: F(q) = R F(q) for R q = q
+        // and, with time reversal (F(-q) = -F(q)), F(q) = -R F(q) for R q = -q; F = 0 where the
+        // signed sum of these R is zero
+        std::vector<int> fzero(nirr, 0);
+        auto matvec3 = [](const double *m, const double *x, double *y) {
+            for (int i = 0; i < 3; ++i) y[i] = m[3 * i] * x[0] + m[3 * i + 1] * x[1] + m[3 * i + 2] * x[2];
+        };
+        for (int q = 0; q < nirr; ++q) {
+            const double *qc = &c->mesh.qcart[3 * (size_t)c->mesh.irr_full_index[q]];
+            double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mx = 0;
+            int cnt = 0;
+            for (int o = 0; o < nsym; ++o) {
+                double rq[3];
+                matvec3(c->ops[o].m, qc, rq);
+                for (int sgn = 1; sgn >= (c->mesh.timereversal ? -1 : 1); sgn -= 2) {
+                    double d[3], f[3], s = 0;
+                    for (int i = 0; i < 3; ++i) d[i] = qc[i] - sgn * rq[i];
+                    matvec3(c->cr.invrec, d, f);
+                    for (int i = 0; i < 3; ++i) { f[i] -= std::nearbyint(f[i]); s += f[i] * f[i]; }
+                    if (s < kSqTol) {
+                        for (int k = 0; k < 9; ++k) P[k] += sgn * c->ops[o].m[k];
+                        ++cnt;
+                    }
+                }
+            }
+            for (int k = 0; k < 9; ++k) mx = std::max(mx, std::fabs(P[k]));
+            fzero[q] = (cnt > 0 && mx < 1e-8 * (double)cnt) ? 1 : 0;
+        }
+        TRY(dalloc((void **)&d_fz, sizeof(int) * std::max(nirr, 1)));
+        TRY(hipMemcpy(d_fz, fzero.data(), sizeof(int) * nirr, hipMemcpyHostToDevice));
+    }
     TRY(dalloc((void **)&d_cvw, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
     TRY(dalloc((void **)&d_klog, sizeof(double) * sLog * nT));
@@ -900,7 +943,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
                                    d_part, d_buf, d_st, sPart, sBuf);
             }
             hipLaunchKernelGGL(k_bte_update, dim3(nirr, 1, nT), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0,
-                               d_dgoff, d_dg, d_Fn, d_g0, d_st, sBuf, sQ, sF, sG);
+                               d_dgoff, d_dg, d_fz, d_Fn, d_g0, d_st, sBuf, sQ, sF, sG);
             hipLaunchKernelGGL(k_kappa_sum, dim3(kKapSlices, 1, nT), dim3(256), 0, st, (int)qb, nsym, d_cvw, d_Fn,
                                d_vel, d_opm, d_klog, d_st, sQ, sF, sLog);
             hipLaunchKernelGGL(k_bte_check, dim3(1, 1, nT), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st, sG, sScf);
