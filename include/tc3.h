@@ -51,6 +51,15 @@ const char *tc3_last_error(const tc3_ctx *ctx);
 int tc3_set_structure(tc3_ctx *ctx, int na, const double *latvec, const double *rfrac,
                       const int *species, const double *mass);
 
+/* Natural isotope composition of an element: replaces lo_isotope_distribution%naturaldistribution
+ * (type_crystalstructure.f90:1521-2466), which gives every atom its masses when infile.isotopes
+ * is not read (--readiso off, the reference's default; type_crystalstructure.f90:650-652).
+ * symbol: element symbol ("Si"); *n receives the number of isotopes; conc and mass (amu)
+ * receive the abundances and masses when maxn >= *n.  The caller normalises and averages them
+ * as for infile.isotopes (type_crystalstructure_io.f90:953-957).  Returns 5 for an unknown
+ * symbol, 1 when maxn is too small for non-NULL arrays.  Needs no context. */
+int tc3_natural_isotopes(const char *symbol, int maxn, double *conc, double *mass, int *n);
+
 /* Second-order IFCs: replaces lo_forceconstant_secondorder%readfromfile
  * (type_forceconstant_secondorder_io.f90:109-247), non-polar.  a1, a2: 0-based atoms;
  * lv2: 3*npair Cartesian lattice vectors (Bohr); m: 9*npair tensors (Ha/Bohr^2),
@@ -92,8 +101,24 @@ int tc3_set_montecarlo(tc3_ctx *ctx, const int *qg3ph, int seed);
 /* Space group, Gamma-centred FFT q-mesh, irreducible points, little groups and the
  * irreducible triplet lists: replaces uc%classify('wedge') (type_crystalstructure_symmetry.f90:7-91),
  * lo_generate_qmesh (type_qpointmesh.f90:452-657, type_qpointmesh_gridgeneration.f90:8-545)
- * and triplet_is_irreducible (scattering_threephonon.f90:319-448).  Host-side. */
+ * and triplet_is_irreducible (scattering_threephonon.f90:319-448).  Host-side.  Resets the
+ * three- and four-phonon Monte-Carlo grids to the full mesh (the defaults of -qg3ph and
+ * -qg4ph, options.f90:226-236). */
 int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal);
+
+/* tc3_set_qmesh with the reference's hidden --nosym (options.f90:157, 206; lo_generate_qmesh
+ * nosym, type_qpointmesh_gridgeneration.f90:54-58): qpsymmetry = 0 reduces the mesh by the
+ * identity and time reversal only (every point its own little group), while the thermal
+ * conductivity is still averaged over the crystal's space group (kappa.f90:28-75, 308-324).
+ * tc3_set_qmesh is qpsymmetry = 1. */
+int tc3_set_qmesh_sym(tc3_ctx *ctx, const int *dims, int timereversal, int qpsymmetry);
+
+/* Three-phonon term on (enable = 1, the default) or off (the reference's --nothirdorder,
+ * options.f90:77-80, 187-188; scattering.f90:119-121, 234-238): without it tc3_scatter forms
+ * the linewidths and scattering matrix from the isotope, four-phonon and boundary terms only
+ * (and fails with status 5 when none is on), no three-phonon Monte-Carlo grid is drawn, and
+ * the adjoints return status 5. */
+int tc3_set_thirdorder(tc3_ctx *ctx, int enable);
 
 /* Sizes of what tc3_set_qmesh built. */
 int tc3_mesh_info(const tc3_ctx *ctx, int *nsym, int *nfull, int *nirr, int64_t *ntriplets,

@@ -95,12 +95,21 @@ def iterative_solution(p, mesh, disp, ops, sr, temperature, niter=200, tol=1e-5,
     return Fn, scf, hist
 
 
-def get_kappa_offdiag(p, fc2, mesh, disp, ops, linewidth, temperature, classical=False):
+def get_kappa_offdiag(p, fc2, mesh, disp, ops, linewidth, temperature, classical=False,
+                      include_diagonal=False, terms=False):
     """get_kappa_offdiag (kappa.f90:77-306).  The reference forms a (nb^2 x nb^2)
     Kronecker matrix and multiplies it by the flattened gradient; that product
-    equals, entry for entry, avg_op[ egw^H G egw ], which is what is formed here."""
+    equals, entry for entry, avg_op[ egw^H G egw ], which is what is formed here.
+
+    The reference sums over mode pairs j != k (:246-285).  include_diagonal=True adds the
+    j = k terms, where the coherence velocity is the group velocity, the Lorentzian is
+    1/(2 Gamma) and the heat-capacity mean is c_v, so that each such term is the mode's
+    single-mode-approximation kappa: a check on the velocities, the Lorentzian, the c_v
+    mean and the prefactor.  terms=True also returns every pair's weighted contribution
+    [q_irr, j, k, 3, 3] (before the chop)."""
     nb = disp.nb
     kod = np.zeros((3, 3))
+    tq = np.zeros((mesh.nirr, nb, nb, 3, 3)) if terms else None
     ism = p.invsqrtmass
     mw = np.repeat(ism, 3)
     for iq in range(mesh.nirr):
@@ -140,7 +149,7 @@ def get_kappa_offdiag(p, fc2, mesh, disp, ops, linewidth, temperature, classical
             if t1 < U.FREQTOL:
                 continue
             for k in range(nb):
-                if j == k:
+                if j == k and not include_diagonal:
                     continue
                 om2 = om[k]
                 if om2 < U.FREQTOL:
@@ -155,4 +164,9 @@ def get_kappa_offdiag(p, fc2, mesh, disp, ops, linewidth, temperature, classical
                                 + float(U.harmonic_oscillator_cv(temperature, om2)))
                 tau = (t1 + t2) / ((t1 + t2) ** 2 + (om1 - om2) ** 2)
                 kod += velsq[j, k] * tau * f0 * pref
-    return U.chop(kod, np.abs(kod).sum() * 1e-6)
+                if terms:
+                    tq[iq, j, k] = velsq[j, k] * tau * f0 * pref
+    kod = U.chop(kod, np.abs(kod).sum() * 1e-6)
+    if terms:
+        return kod, tq
+    return kod

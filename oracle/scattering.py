@@ -198,7 +198,7 @@ def isotope_row(mesh, disp, sigsq, q1, b1, g, integrationtype=2, sigma_fixed=0.0
 
 def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
                      integrationtype=2, sigma=1.0, rows_only=None, isotope_g=None, mfp_max=None,
-                     qg3ph=None, seed=None, fc4=None, qg4ph=None):
+                     qg3ph=None, seed=None, fc4=None, qg4ph=None, thirdorder=True):
     """lo_scattering_rates%generate (scattering.f90:58-358): three-phonon scattering with
     qg3ph = qgrid (the Monte-Carlo grid is then the identity with weight 1/N_q), plus
     isotope scattering (:229-232) when per-atom mass-disorder parameters are given, plus
@@ -207,7 +207,10 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
     (compute_threephonon_scattering :69-78, montecarlo.row_grids), each draw weighted
     1 / prod(qg3ph).  With fourth-order force constants fc4 (lattice.read_fc4) the
     four-phonon term of compute_fourphonon_scattering is added to every row (qg4ph = qgrid,
-    oracle/fourphonon.py)."""
+    oracle/fourphonon.py).  thirdorder=False leaves the three-phonon term out (--nothirdorder,
+    scattering.f90:234-238; full q-grids only here)."""
+    if not thirdorder and (qg3ph is not None or qg4ph is not None):
+        raise NotImplementedError("the oracle leaves the three-phonon term out on full q-grids only")
     nb = disp.nb
     nirr, nfull = mesh.nirr, mesh.nfull
     sr = ScatteringRates()
@@ -250,7 +253,7 @@ def scattering_rates(p, fc3, mesh, disp, ops, temperature, classical=False,
         om1 = disp.om_irr[q1]
         E1 = disp.egv_irr[q1] / np.sqrt(np.where(om1 > 0, om1, 1.0))[None, :]
         od = np.zeros((nb, nfull, nb))   # [b1, q, b]
-        trips = irreducible_triplets(mesh, rot, q1)
+        trips = irreducible_triplets(mesh, rot, q1) if thirdorder else []
         ntrip += len(trips)
         for (q2, q3, perm, red) in trips:
             if grids is not None:

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtc3.so")
-SOURCES = ["tc3_host.cpp", "tc3_api.cpp", "tc3_hdf5.cpp", "tc3_cumulative.cpp", "tc3_harmonic.hip", "tc3_scatter.hip",
+SOURCES = ["tc3_host.cpp", "tc3_api.cpp", "tc3_isotopes.cpp", "tc3_hdf5.cpp", "tc3_cumulative.cpp", "tc3_harmonic.hip", "tc3_scatter.hip",
            "tc3_kappa.hip", "tc3_mfma.hip", "tc3_adjoint.hip", "tc3_fourphonon.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 VALU instructions.

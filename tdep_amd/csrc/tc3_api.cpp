@@ -204,17 +204,30 @@ int tc3_set_fc3(tc3_ctx *ctx, int ntrip, const int *a1, const int *a2, const int
 }
 
 int tc3_set_qmesh(tc3_ctx *ctx, const int *dims, int timereversal) {
+    return tc3_set_qmesh_sym(ctx, dims, timereversal, 1);
+}
+
+int tc3_set_qmesh_sym(tc3_ctx *ctx, const int *dims, int timereversal, int qpsymmetry) {
     return guarded(ctx, [&]() -> int {
         if (!ctx->have_structure) return ctx->fail(5, "set the structure first");
+        if (!dims) return ctx->fail(5, "q-grid dimensions missing");
         Status s = spacegroup(ctx->cr, ctx->ops);
         if (s.code) return ctx->fail(s.code, s.msg);
-        s = build_mesh(ctx->cr, ctx->ops, dims, timereversal != 0, ctx->mesh);
+        // without q-point symmetry the mesh is reduced by the identity (the first operation)
+        // and time reversal only (build_grid with nosymmetry, type_qpointmesh_gridgeneration.f90:
+        // 54-58); the crystal keeps its space group for the kappa averages
+        const std::vector<SymOp> mops(ctx->ops.begin(), ctx->ops.begin() + (qpsymmetry ? ctx->ops.size() : 1));
+        s = build_mesh(ctx->cr, mops, dims, timereversal != 0, ctx->mesh);
         if (s.code) return ctx->fail(s.code, s.msg);
         s = build_triplets(ctx->mesh, ctx->trip_full);
         if (s.code) return ctx->fail(s.code, s.msg);
         ctx->trip = ctx->trip_full;
         ctx->trip_w.clear();
         ctx->mc_gen = -1;
+        // the Monte-Carlo grids default to the full mesh (options.f90:226-236): a three- or
+        // four-phonon grid left unset draws every point
+        for (int i = 0; i < 3; ++i) ctx->mc_dims[i] = ctx->mc4_dims[i] = ctx->mesh.dims[i];
+        ctx->mc_on = ctx->mc4_on = false;
         s = build_orbits(ctx->mesh, ctx->orb);
         if (s.code) return ctx->fail(s.code, s.msg);
         ctx->have_mesh = true;
@@ -260,6 +273,15 @@ int tc3_set_montecarlo(tc3_ctx *ctx, const int *qg3ph, int seed) {
         ctx->mc_seed = seed;
         ctx->mc_gen = -1;
         ++ctx->gen_model;
+        ctx->have_scatter = false;
+        return 0;
+    });
+}
+
+int tc3_set_thirdorder(tc3_ctx *ctx, int enable) {
+    return guarded(ctx, [&]() -> int {
+        ctx->thirdorder = enable != 0;
+        ctx->mc_gen = -1;
         ctx->have_scatter = false;
         return 0;
     });
@@ -415,7 +437,9 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
         // with four-phonon scattering every row draws its three-phonon grid and then its two
         // four-phonon grids from the one generator (full-mesh grids included)
         const bool f4 = ctx->nfc4 > 0;
-        if (ctx->mc_on || (f4 && ctx->mc4_on)) {
+        if (!ctx->thirdorder && !f4 && !ctx->iso_on && !(ctx->mfp_max > 0.0))
+            return ctx->fail(5, "no scattering term is switched on");
+        if ((ctx->mc_on && ctx->thirdorder) || (f4 && ctx->mc4_on)) {
             if (ctx->mc_seed <= 0)
                 return ctx->fail(5, "Monte-Carlo grids need a positive seed (a wall-clock seed is not reproducible)");
             if (ctx->mc_gen != ctx->gen_model) {
@@ -423,7 +447,7 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
                 for (int i = 0; i < 3; ++i) d3[i] = ctx->mc_on ? ctx->mc_dims[i] : ctx->mesh.dims[i];
                 Status ms = montecarlo_triplets(ctx->mesh, ctx->trip_full, ctx->om_irr, nb, d3, ctx->mc_seed, ctx->trip,
                                                 ctx->trip_w, ctx->mc_npoints, f4 ? ctx->mc4_dims : nullptr, &ctx->g4,
-                                                &ctx->mc4_npoints);
+                                                &ctx->mc4_npoints, ctx->thirdorder);
                 if (ms.code) return ctx->fail(ms.code, ms.msg);
                 ctx->mc_gen = ctx->gen_model;
             }
@@ -549,6 +573,7 @@ static int adjoint_from_linewidths(tc3_ctx *ctx, const double *dlw, int precisio
 int tc3_adjoint_gamma(tc3_ctx *ctx, const double *dL_dlinewidth, int precision, double *dL_dfc3) {
     return guarded(ctx, [&]() -> int {
         if (!ctx->have_scatter) return ctx->fail(5, "run tc3_scatter first");
+        if (!ctx->thirdorder) return ctx->fail(5, "the three-phonon term is switched off");
         if (!dL_dlinewidth || !dL_dfc3) return ctx->fail(5, "null argument");
         if (precision < 0 || precision > 2) return ctx->fail(5, "unknown precision mode");
         return adjoint_from_linewidths(ctx, dL_dlinewidth, precision, dL_dfc3);
@@ -558,6 +583,7 @@ int tc3_adjoint_gamma(tc3_ctx *ctx, const double *dL_dlinewidth, int precision, 
 int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, double *dL_dfc3) {
     return guarded(ctx, [&]() -> int {
         if (!ctx->have_scatter) return ctx->fail(5, "run tc3_scatter first");
+        if (!ctx->thirdorder) return ctx->fail(5, "the three-phonon term is switched off");
         if (!dL_dkappa || !dL_dfc3) return ctx->fail(5, "null argument");
         if (precision < 0 || precision > 2) return ctx->fail(5, "unknown precision mode");
         const size_t qb = (size_t)ctx->mesh.nirr * ctx->nb;

@@ -83,6 +83,7 @@ struct tc3_ctx {
     std::vector<double> iso_g;                // mass-disorder parameter per atom
     bool iso_on = false;                      // isotope scattering requested
     double mfp_max = -1.0;                    // boundary scattering length (Bohr), off if <= 0
+    bool thirdorder = true;                   // three-phonon term on (--nothirdorder switches it off)
     double thresh6 = 0.0;                     // baseline width of integration type 6 (scattering.f90:165-167)
     std::vector<double> linewidth, qs, F0, mfp, smfp;  // [T][q][b](x3)
 

@@ -299,6 +299,7 @@ Status build_mesh(const Crystal &c, const std::vector<SymOp> &ops, const int *di
     const int nsym = (int)ops.size();
     m.nfull = np;
     m.timereversal = timereversal;
+    m.nops = nsym;
     std::vector<double> pts(3 * np);
     for (int i = 0, l = 0; i < n1; ++i)
         for (int j = 0; j < n2; ++j)
@@ -558,14 +559,16 @@ double MersenneTwister64::real1() { return (double)(next() >> 11) * (1.0 / 90071
 
 Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vector<double> &om_irr, int nb,
                            const int *mc_dims_in, int seed, TripletSet &sub, std::vector<float> &w, long &npoints,
-                           const int *mc4_dims, std::vector<long> *g4, long *npoints4) {
+                           const int *mc4_dims, std::vector<long> *g4, long *npoints4, bool draw3) {
     Status st;
     int mcd[3], mcd4[3];
     double ratio[3], ratio4[3];
     for (int i = 0; i < 3; ++i) {
-        mcd[i] = std::max(1, std::min(mc_dims_in[i], m.dims[i]));
+        // a grid dimension outside 1..qgrid means the full mesh (options.f90:226-236)
+        auto dim = [&](int d) { return (d < 1 || d > m.dims[i]) ? m.dims[i] : d; };
+        mcd[i] = dim(mc_dims_in[i]);
         ratio[i] = (double)m.dims[i] / (double)mcd[i];
-        mcd4[i] = mc4_dims ? std::max(1, std::min(mc4_dims[i], m.dims[i])) : 1;
+        mcd4[i] = mc4_dims ? dim(mc4_dims[i]) : 1;
         ratio4[i] = (double)m.dims[i] / (double)mcd4[i];
     }
     npoints = (long)mcd[0] * mcd[1] * mcd[2];
@@ -612,17 +615,20 @@ Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vec
         for (int t = full.q1_off[q1]; t < full.q1_off[q1 + 1]; ++t) tof[full.q2[t]] = t;
         for (int b1 = 0; b1 < nb; ++b1) {
             if (om_irr[(size_t)q1 * nb + b1] < kFreqTol) continue;
-            // generate_grid (:75-99): no point equal to the previous one
-            grid[0] = point_to_full(0);
-            long prev = grid[0], test = grid[0];
-            for (long qi = 1; qi < npoints; ++qi) {
-                while (test == prev) test = point_to_full(qi);
-                grid[qi] = test;
-                prev = test;
-            }
-            for (long qi = 0; qi < npoints; ++qi) {
-                const long t = tof[grid[qi]];
-                if (t >= 0) wfull[(size_t)t * nb + b1] += 1.0f;
+            // generate_grid (:75-99): no point equal to the previous one; without the
+            // three-phonon term (--nothirdorder) its grid is not drawn (scattering.f90:119-121, 234)
+            if (draw3) {
+                grid[0] = point_to_full(0);
+                long prev = grid[0], test = grid[0];
+                for (long qi = 1; qi < npoints; ++qi) {
+                    while (test == prev) test = point_to_full(qi);
+                    grid[qi] = test;
+                    prev = test;
+                }
+                for (long qi = 0; qi < npoints; ++qi) {
+                    const long t = tof[grid[qi]];
+                    if (t >= 0) wfull[(size_t)t * nb + b1] += 1.0f;
+                }
             }
             // four-phonon grids of the row (scattering_fourphonon.f90:84-85)
             if (mc4_dims) {

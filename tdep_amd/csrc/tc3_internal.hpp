@@ -61,6 +61,7 @@ struct Mesh {
     int dims[3] = {0, 0, 0};
     int nfull = 0, nirr = 0;
     bool timereversal = true;
+    int nops = 0;                      // operations the mesh was reduced with (1 with --nosym)
     std::vector<int> irr_full_index;   // nirr
     std::vector<int> irr_of_full;      // nfull
     std::vector<int> op_of_full;       // nfull, 1-based, negative = time reversal
@@ -113,10 +114,12 @@ struct MersenneTwister64 {
 // b1) with omega >= freqtol, in (q1, b1) order, one stratified draw of prod(mc_dims) points
 // of the full grid; the triplets of `full` whose first point is drawn form `sub`, and
 // w[t * nb + b1] counts how often triplet t is drawn for row (q1, b1).  npoints = the
-// number of points per draw (the weight of a draw is 1 / npoints).
+// number of points per draw (the weight of a draw is 1 / npoints).  With draw3 false (no
+// three-phonon term) only the four-phonon grids are drawn and `sub` is empty.
 Status montecarlo_triplets(const Mesh &m, const TripletSet &full, const std::vector<double> &om_irr, int nb,
                            const int *mc_dims, int seed, TripletSet &sub, std::vector<float> &w, long &npoints,
-                           const int *mc4_dims = nullptr, std::vector<long> *g4 = nullptr, long *npoints4 = nullptr);
+                           const int *mc4_dims = nullptr, std::vector<long> *g4 = nullptr, long *npoints4 = nullptr,
+                           bool draw3 = true);
 Status build_orbits(const Mesh &m, OrbitSet &o);
 
 }  // namespace tc3

@@ -825,8 +825,13 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     int *d_fz;
     {
         // points where symmetry fixes no vector, so that F vanishes: F(q) = R F(q) for R q = q
-        // and, with time reversal (F(-q) = -F(q)), F(q) = -R F(q) for R q = -q; F = 0 where the
-        // signed sum of these R is zero
+        // (the scattering-matrix rows are averaged over these R, so the reference's iteration
+        // keeps F(q) in their fixed space up to rounding) and, with time reversal (F(-q) =
+        // -F(q)), F(q) = -R F(q) for R q = -q; F = 0 where the signed sum of these R is zero.
+        // Time reversal holds for the scattering matrix only when every q2 is visited: a
+        // Monte-Carlo grid draws q2 and -q2 independently, the reference keeps the F of
+        // Monte-Carlo-noise size there, and so only the proper operations count then.
+        const bool tr_fix = c->mesh.timereversal && !c->mc_on && !(c->nfc4 > 0 && c->mc4_on);
         std::vector<int> fzero(nirr, 0);
         auto matvec3 = [](const double *m, const double *x, double *y) {
             for (int i = 0; i < 3; ++i) y[i] = m[3 * i] * x[0] + m[3 * i + 1] * x[1] + m[3 * i + 2] * x[2];
@@ -835,10 +840,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             const double *qc = &c->mesh.qcart[3 * (size_t)c->mesh.irr_full_index[q]];
             double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mx = 0;
             int cnt = 0;
-            for (int o = 0; o < nsym; ++o) {
+            // only the operations the mesh (and so the row symmetrisation) was built with
+            for (int o = 0; o < c->mesh.nops; ++o) {
                 double rq[3];
                 matvec3(c->ops[o].m, qc, rq);
-                for (int sgn = 1; sgn >= (c->mesh.timereversal ? -1 : 1); sgn -= 2) {
+                for (int sgn = 1; sgn >= (tr_fix ? -1 : 1); sgn -= 2) {
                     double d[3], f[3], s = 0;
                     for (int i = 0; i < 3; ++i) d[i] = qc[i] - sgn * rq[i];
                     matvec3(c->cr.invrec, d, f);

@@ -796,7 +796,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     // per call: occupations, smearing, partial sums
     TRY(c->pool_get("be", sizeof(double) * nT * nirr * nb, (void **)&d_be));
     TRY(c->pool_get("sigsq", sizeof(double) * nirr * nb, (void **)&d_sigsq));
-    TRY(c->pool_get("gpart", sizeof(double) * ntrip * nT * nb, (void **)&d_gpart));
+    TRY(c->pool_get("gpart", sizeof(double) * std::max<long>(ntrip, 1) * nT * nb, (void **)&d_gpart));
     TRY(c->pool_get("lw", sizeof(double) * nrows * nT, (void **)&d_lw));
     // per-call transfers go through the context's pinned staging buffer: occupations and
     // widths up, linewidths (and the isotope term) down
@@ -919,7 +919,14 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     std::vector<hipEvent_t> kev;
     for (double &x : c->kstat_ms) x = 0;
     for (int &x : c->kstat_n) x = 0;
-    if (use_mfma) {
+    if (!c->thirdorder) {
+        // no three-phonon term (--nothirdorder, scattering.f90:234-238): zero partial sums and
+        // scattering matrix, to which the isotope, four-phonon and boundary terms are added
+        TRY(hipEventRecord(ev0, st));
+        TRY(hipMemsetAsync(d_gpart, 0, sizeof(double) * std::max<long>(ntrip, 1) * nT * nb, st));
+        TRY(hipMemsetAsync(c->d_xi, 0, (size_t)c->xi_es * xi_T * nT, st));
+        c->batch = 0;
+    } else if (use_mfma) {
         // nonzero (a2, a3) IFC blocks of every atom a1, in (a1, a2, a3) order
         std::vector<int4> blist;
         std::vector<int> a1_boff(na + 1, 0), a12_boff((size_t)na * na + 1, 0);

@@ -63,14 +63,20 @@ class Engine:
 
     # ---------------------------------------------------------------- inputs
     @classmethod
-    def from_directory(cls, indir, qgrid, device=0, timereversal=True):
+    def from_directory(cls, indir, qgrid, device=0, timereversal=True, readiso=True, qpsymmetry=True):
+        """readiso: masses from infile.isotopes (--readiso), else from the natural isotope
+        distributions (the reference's default); qpsymmetry=False is the reference's --nosym."""
         eng = cls(device)
-        p, fc2, fc3 = tio.read_directory(indir)
+        p, fc2, fc3 = tio.read_directory(indir, readiso=readiso)
         eng.set_structure(p)
         eng.set_fc2(fc2)
         eng.set_fc3(fc3)
-        eng.set_qmesh(qgrid, timereversal)
+        eng.set_qmesh(qgrid, timereversal, qpsymmetry)
         return eng
+
+    def set_thirdorder(self, enable=True):
+        """Three-phonon term on or off (enable=False is the reference's --nothirdorder)."""
+        self._check(self.lib.tc3_set_thirdorder(self.ctx, int(bool(enable))))
 
     def set_montecarlo(self, qg3ph, seed):
         """Monte-Carlo q2 grid of the three-phonon term (-qg3ph, --seed): a stratified draw of
@@ -144,9 +150,9 @@ class Engine:
         from tdep_amd import io as tio
         self.set_fc4(tio.read_fc4(self.structure, os.path.join(indir, "infile.forceconstant_fourthorder")))
 
-    def set_qmesh(self, qgrid, timereversal=True):
+    def set_qmesh(self, qgrid, timereversal=True, qpsymmetry=True):
         d = np.ascontiguousarray(qgrid, dtype=np.int32)
-        self._check(self.lib.tc3_set_qmesh(self.ctx, iptr(d), int(bool(timereversal))))
+        self._check(self.lib.tc3_set_qmesh_sym(self.ctx, iptr(d), int(bool(timereversal)), int(bool(qpsymmetry))))
         self.qgrid = tuple(int(x) for x in qgrid)
         self.mesh_info()
 

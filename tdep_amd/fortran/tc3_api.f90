@@ -36,6 +36,29 @@ module tc3_api
             integer(c_int), intent(in) :: species(*)
         end function
 
+        integer(c_int) function tc3_natural_isotopes(symbol, maxn, conc, mass, n) &
+                bind(C, name="tc3_natural_isotopes")
+            import :: c_int, c_char, c_double
+            character(kind=c_char), intent(in) :: symbol(*)
+            integer(c_int), value :: maxn
+            real(c_double), intent(out) :: conc(*), mass(*)
+            integer(c_int), intent(out) :: n
+        end function
+
+        integer(c_int) function tc3_set_qmesh_sym(ctx, dims, timereversal, qpsymmetry) &
+                bind(C, name="tc3_set_qmesh_sym")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(in) :: dims(3)
+            integer(c_int), value :: timereversal, qpsymmetry
+        end function
+
+        integer(c_int) function tc3_set_thirdorder(ctx, enable) bind(C, name="tc3_set_thirdorder")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: enable
+        end function
+
         integer(c_int) function tc3_set_fc2(ctx, npair, a1, a2, lv2, m) bind(C, name="tc3_set_fc2")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx

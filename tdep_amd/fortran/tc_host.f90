@@ -1,8 +1,8 @@
 ! Fortran host of the three-phonon thermal-conductivity path on one MI355X.
 !
 ! Takes the place of src/thermal_conductivity/main.f90 for the part of the program built
-! here: reads infile.ucposcar, infile.isotopes, infile.forceconstant and
-! infile.forceconstant_thirdorder from the working directory (formats of
+! here: reads infile.ucposcar, infile.forceconstant, infile.forceconstant_thirdorder and, with
+! --readiso, infile.isotopes from the working directory (formats of
 ! type_crystalstructure_io.f90:9-350, :921-959, type_forceconstant_secondorder_io.f90:109-247,
 ! type_forceconstant_thirdorder.f90:243-311), hands everything numeric to libtc3 through the
 ! iso_c_binding interface (module tc3_api), and writes outfile.thermal_conductivity
@@ -27,7 +27,10 @@
 !   --unit thz|mev|icm                         frequency unit of the spectra (thz)
 !   --iterative_tolerance x                    BTE tolerance (1e-5)
 !   --iterative_maxsteps n                     BTE iterations (200)
-!   --classical, --noisotope, --notr, --readiso (masses always come from infile.isotopes)
+!   --classical, --noisotope, --notr
+!   --readiso                                  masses from infile.isotopes (else natural isotopes)
+!   --nothirdorder                             no three-phonon term (isotope, four-phonon, boundary only)
+!   --nosym                                    mesh reduced by the identity and time reversal only
 !   --precision fp64|fp32|bf16x3               precision of the |Psi|^2 contraction (fp64)
 !   --no-offdiag                               skip the off-diagonal (coherence) term
 !   --fourthorder                              add four-phonon scattering from
@@ -37,9 +40,10 @@
 !                                              print sizes and input checksums; no device work
 module tc_host_io
     use iso_c_binding
+    use tc3_api, only: tc3_natural_isotopes
     implicit none
     private
-    public :: fatal, read_ucposcar, read_isotopes, read_fc2, read_fc3, read_fc4, write_outfile, temperature_suffix
+    public :: fatal, read_ucposcar, read_isotopes, natural_masses, read_fc2, read_fc3, read_fc4, write_outfile, temperature_suffix
 
     ! konstanter.f90:36-91
     real(c_double), parameter :: bohr_to_a = 0.52917721067_c_double
@@ -254,13 +258,71 @@ contains
             do j = 1, n
                 read (u, *) c(j), mm(j)
             end do
-            mm = mm*amu_to_emu
-            c = c/sum(c)
-            mass(a) = sum(c*mm)
-            disorder(a) = sum(c*(mm - mass(a))**2)/mass(a)**2
+            call distribution(c, mm, mass(a), disorder(a))
             deallocate (c, mm)
         end do
         close (u)
+    end subroutine
+
+    ! Masses and mass-disorder parameters from the natural isotope distribution of every atom's
+    ! element (naturaldistribution, type_crystalstructure.f90:1521-2466, through
+    ! tc3_natural_isotopes), normalised and averaged as read_isotopes does.  elist holds the
+    ! species symbols separated by blanks, species(a) the 0-based species of atom a.
+    subroutine natural_masses(na, species, elist, mass, disorder)
+        integer, intent(in) :: na
+        integer(c_int), intent(in) :: species(na)
+        character(len=*), intent(in) :: elist
+        real(c_double), intent(out) :: mass(na), disorder(na)
+        character(len=8) :: sym(na)
+        real(c_double) :: c(16), mm(16)
+        integer(c_int) :: n
+        integer :: a, ns, i0, i1
+        ns = 0
+        i0 = 1
+        do while (i0 <= len_trim(elist))
+            if (elist(i0:i0) == ' ') then
+                i0 = i0 + 1
+                cycle
+            end if
+            i1 = i0
+            do while (i1 < len_trim(elist))
+                if (elist(i1 + 1:i1 + 1) == ' ') exit
+                i1 = i1 + 1
+            end do
+            ns = ns + 1
+            sym(ns) = elist(i0:i1)
+            i0 = i1 + 1
+        end do
+        do a = 1, na
+            if (tc3_natural_isotopes(trim(sym(species(a) + 1))//c_null_char, 16_c_int, c, mm, n) /= 0) &
+                call fatal(5, 'no isotope distributions available for '//trim(sym(species(a) + 1))// &
+                           ', are you sure it is a stable element?')
+            call distribution(c(1:n), mm(1:n), mass(a), disorder(a))
+        end do
+    end subroutine
+
+    ! Mean mass (emu) and g = sum_i c_i (m_i - m)^2 / m^2 of abundances c and masses mm (amu),
+    ! abundances normalised first (type_crystalstructure_io.f90:953-957).
+    subroutine distribution(c0, mm0, mass, disorder)
+        real(c_double), intent(in) :: c0(:), mm0(:)
+        real(c_double), intent(out) :: mass, disorder
+        real(c_double) :: c(size(c0)), mm(size(c0)), tot
+        integer :: j
+        mm = mm0*amu_to_emu
+        tot = 0.0_c_double
+        do j = 1, size(c0)
+            tot = tot + c0(j)
+        end do
+        c = c0/tot
+        mass = 0.0_c_double
+        do j = 1, size(c0)
+            mass = mass + c(j)*mm(j)
+        end do
+        disorder = 0.0_c_double
+        do j = 1, size(c0)
+            disorder = disorder + c(j)*(mm(j) - mass)**2
+        end do
+        disorder = disorder/mass**2
     end subroutine
 
     ! Second-order force constants: pair list with Cartesian lattice vectors (Bohr) and
@@ -492,7 +554,7 @@ program tc_host
     character(len=3) :: unit
     character(len=:), allocatable :: elist
     real(c_double), allocatable :: temps(:)
-    logical :: classical, noisotope, notr, offdiag, inputs_only, fourthorder
+    logical :: classical, noisotope, notr, offdiag, inputs_only, fourthorder, readiso, nothirdorder, nosym
     integer(c_int) :: device
     type(c_ptr) :: ctx
     integer :: na, npair, nfc3, iT, st
@@ -509,13 +571,20 @@ program tc_host
     call parse_options()
     call read_ucposcar('infile.ucposcar', na, latvec, rfrac, species, elist)
     allocate (mass(na), disorder(na))
-    call read_isotopes('infile.isotopes', na, mass, disorder)
+    ! masses from infile.isotopes with --readiso, else natural isotopes (main.f90:98-104,
+    ! type_crystalstructure.f90:650-652)
+    if (readiso) then
+        call read_isotopes('infile.isotopes', na, mass, disorder)
+    else
+        call natural_masses(na, species, elist, mass, disorder)
+    end if
     call read_fc2('infile.forceconstant', na, latvec, npair, p1, p2, lv, m2)
     call read_fc3('infile.forceconstant_thirdorder', na, latvec, nfc3, t1, t2, t3, lv2, lv3, m3)
 
     call check(tc3_create(ctx, device), 'tc3_create')
     call check(tc3_set_structure(ctx, int(na, c_int), latvec, rfrac, species, mass), 'tc3_set_structure')
     call check(tc3_set_isotopes(ctx, disorder, merge(0_c_int, 1_c_int, noisotope)), 'tc3_set_isotopes')
+    call check(tc3_set_thirdorder(ctx, merge(0_c_int, 1_c_int, nothirdorder)), 'tc3_set_thirdorder')
     ! options.f90:218: metres to Bohr (lo_m_to_Bohr)
     call check(tc3_set_boundary(ctx, merge(max_mfp/0.52917721067e-10_c_double, -1.0_c_double, max_mfp > 0)), &
                'tc3_set_boundary')
@@ -525,7 +594,8 @@ program tc_host
         call read_fc4('infile.forceconstant_fourthorder', na, latvec, nfc4, f1, f2, f3, f4, fl2, fl3, fl4, m4)
         call check(tc3_set_fc4(ctx, int(nfc4, c_int), f1, f2, f3, f4, fl2, fl3, fl4, m4), 'tc3_set_fc4')
     end if
-    call check(tc3_set_qmesh(ctx, qg, merge(0_c_int, 1_c_int, notr)), 'tc3_set_qmesh')
+    call check(tc3_set_qmesh_sym(ctx, qg, merge(0_c_int, 1_c_int, notr), merge(0_c_int, 1_c_int, nosym)), &
+               'tc3_set_qmesh_sym')
     call check(tc3_set_montecarlo(ctx, qg3, seed), 'tc3_set_montecarlo')
     call check(tc3_set_montecarlo4(ctx, qg4), 'tc3_set_montecarlo4')
     if (inputs_only) then
@@ -614,6 +684,9 @@ contains
         classical = .false.
         noisotope = .false.
         notr = .false.
+        readiso = .false.
+        nothirdorder = .false.
+        nosym = .false.
         offdiag = .true.
         inputs_only = .false.
         fourthorder = .false.
@@ -709,7 +782,9 @@ contains
             case ('--classical'); classical = .true.
             case ('--noisotope'); noisotope = .true.
             case ('--notr'); notr = .true.
-            case ('--readiso')
+            case ('--readiso'); readiso = .true.
+            case ('--nothirdorder'); nothirdorder = .true.
+            case ('--nosym'); nosym = .true.
             case ('--no-offdiag'); offdiag = .false.
             case ('--fourthorder'); fourthorder = .true.
             case ('--inputs-only'); inputs_only = .true.
@@ -723,5 +798,10 @@ contains
             i = i + 1
         end do
         temps = tbuf(1:nT)
+        ! options.f90:220-223
+        if (any(qg4 > 0) .and. .not. fourthorder) then
+            write (*, *) 'You have to enable fourthorder to use a fourth order Monte-Carlo grid, stopping calculation.'
+            stop
+        end if
     end subroutine
 end program tc_host

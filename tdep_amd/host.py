@@ -48,7 +48,10 @@ def parse(argv=None):
     ap.add_argument("--iterative_tolerance", type=float, default=1e-5)
     ap.add_argument("--iterative_maxsteps", type=int, default=200)
     ap.add_argument("--classical", action="store_true")
-    ap.add_argument("--readiso", action="store_true", help="read infile.isotopes (required: masses come from it)")
+    ap.add_argument("--readiso", action="store_true",
+                    help="read the isotope distribution from infile.isotopes (default: natural isotopes)")
+    ap.add_argument("--nothirdorder", action="store_true", help="do not consider three-phonon scattering")
+    ap.add_argument("--nosym", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--noisotope", action="store_true", help="switch off isotope scattering")
     ap.add_argument("--notr", action="store_true", help="do not use time-reversal symmetry")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"],
@@ -62,8 +65,14 @@ def parse(argv=None):
 def main(argv=None):
     a = parse(argv)
     from tdep_amd.engine import Engine
-    eng = Engine.from_directory(a.directory, tuple(a.qpoint_grid), device=a.device, timereversal=not a.notr)
+    if max(a.qpoint_grid4ph) > 0 and not a.fourthorder:
+        # options.f90:220-223
+        print("You have to enable fourthorder to use a fourth order Monte-Carlo grid, stopping calculation.")
+        return 0
+    eng = Engine.from_directory(a.directory, tuple(a.qpoint_grid), device=a.device, timereversal=not a.notr,
+                                readiso=a.readiso, qpsymmetry=not a.nosym)
     eng.set_isotopes(eng.structure.mass_disorder, enable=not a.noisotope)
+    eng.set_thirdorder(not a.nothirdorder)
     eng.set_boundary(a.max_mfp)
     if max(a.qpoint_grid3ph) > 0 or max(a.qpoint_grid4ph) > 0:
         eng.set_montecarlo(tuple(a.qpoint_grid3ph) if max(a.qpoint_grid3ph) > 0 else None, a.seed)

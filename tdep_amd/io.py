@@ -118,20 +118,63 @@ def read_ucposcar(path):
     return p
 
 
+def _distribution(conc, mass_amu):
+    """Mean mass (emu) and mass-disorder parameter g = sum_i c_i (m_i - m)^2 / m^2 of an
+    isotope distribution, abundances normalised first (type_crystalstructure_io.f90:953-957,
+    type_crystalstructure.f90:1505-1519); sums run in order, as Fortran's sum does."""
+    mm = [float(x) * AMU_TO_EMU for x in mass_amu]
+    tot = 0.0
+    for c in conc:
+        tot += float(c)
+    c = [float(x) / tot for x in conc]
+    mean = 0.0
+    for ci, mi in zip(c, mm):
+        mean += ci * mi
+    f = 0.0
+    for ci, mi in zip(c, mm):
+        f += ci * (mi - mean) ** 2
+    return mean, f / mean ** 2
+
+
 def read_isotopes(p, path):
+    """infile.isotopes: per atom n, then n lines of abundance and mass (amu)."""
     tok = open(path).read().split()
     pos = 0
     p.mass = np.zeros(p.na)
     p.mass_disorder = np.zeros(p.na)
     for a in range(p.na):
         n = int(tok[pos]); pos += 1
-        conc = np.array([float(tok[pos + 2 * j]) for j in range(n)])
-        mm = np.array([float(tok[pos + 2 * j + 1]) for j in range(n)]) * AMU_TO_EMU
+        conc = [float(tok[pos + 2 * j]) for j in range(n)]
+        mm = [float(tok[pos + 2 * j + 1]) for j in range(n)]
         pos += 2 * n
-        conc = conc / conc.sum()
-        mean = (conc * mm).sum()
-        p.mass[a] = mean
-        p.mass_disorder[a] = (conc * (mm - mean) ** 2).sum() / mean ** 2
+        p.mass[a], p.mass_disorder[a] = _distribution(conc, mm)
+    return p
+
+
+def natural_isotopes(symbol):
+    """(abundances, masses in amu) of an element's natural isotope distribution, from the
+    library's table (tc3_natural_isotopes; naturaldistribution, type_crystalstructure.f90:1521-2466)."""
+    import ctypes
+    from .lib import dptr, load
+    lib = load()
+    n = ctypes.c_int(0)
+    conc = np.zeros(16)
+    mass = np.zeros(16)
+    r = lib.tc3_natural_isotopes(symbol.encode(), 16, dptr(conc), dptr(mass), ctypes.byref(n))
+    if r:
+        raise ValueError(f"no isotope distributions available for {symbol}, are you sure it is a stable element?")
+    return conc[:n.value].copy(), mass[:n.value].copy()
+
+
+def natural_masses(p):
+    """Masses and mass-disorder parameters of every atom from the natural isotope
+    distribution of its element (type_crystalstructure.f90:650-652, the default without
+    --readiso)."""
+    p.mass = np.zeros(p.na)
+    p.mass_disorder = np.zeros(p.na)
+    for a in range(p.na):
+        conc, mm = natural_isotopes(p.symbols[a])
+        p.mass[a], p.mass_disorder[a] = _distribution(conc, mm)
     return p
 
 
@@ -208,12 +251,14 @@ def read_fc4(p, path):
 
 
 def read_directory(indir, readiso=True):
+    """Structure and force constants of a run directory; masses from infile.isotopes with
+    readiso (--readiso), else from the natural isotope distributions (the reference's default,
+    type_crystalstructure.f90:650-652, readisotopefromfile only with --readiso, main.f90:100-104)."""
     p = read_ucposcar(os.path.join(indir, "infile.ucposcar"))
     if readiso:
         read_isotopes(p, os.path.join(indir, "infile.isotopes"))
     else:
-        raise NotImplementedError("natural-isotope masses are not tabulated here; "
-                                  "provide infile.isotopes (--readiso)")
+        natural_masses(p)
     fc2 = read_fc2(p, os.path.join(indir, "infile.forceconstant"))
     fc3 = read_fc3(p, os.path.join(indir, "infile.forceconstant_thirdorder"))
     return p, fc2, fc3

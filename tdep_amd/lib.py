@@ -21,7 +21,10 @@ SIGNATURES = {
     "tc3_set_fc3": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
                                    _c_dbl_p, _c_dbl_p]),
     "tc3_set_isotopes": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int]),
+    "tc3_natural_isotopes": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_int_p]),
     "tc3_set_qmesh": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, ctypes.c_int]),
+    "tc3_set_qmesh_sym": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, ctypes.c_int, ctypes.c_int]),
+    "tc3_set_thirdorder": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tc3_mesh_info": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p,
                                      ctypes.POINTER(ctypes.c_int64), _c_int_p]),
     "tc3_get_mesh": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p]),

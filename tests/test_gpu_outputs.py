@@ -115,17 +115,17 @@ def test_host_writes_both_outfiles(inputs_dir, tmp_path):
 
 def test_hosts_with_the_reference_tests_flags(tmp_path):
     """The reference test's command line (tests/thermal_conductivity/Makefile:1-6:
-    -qg 5 5 5 --fourthorder -qg4ph 3 3 3 --seed 42, natural isotopes) on a two-atom
-    rocksalt cell with Morse IFCs (the reference test's MgO is polar): the consistency
-    check of test_thermal_conductivity.py:25-40 holds with four-phonon and Monte-Carlo
-    terms, both hosts write the same outfile.thermal_conductivity, and the four-phonon term
-    lowers kappa."""
+    -qg 5 5 5 --fourthorder -qg4ph 3 3 3 --seed 42) on a two-atom rocksalt cell with Morse
+    IFCs (the reference test's MgO is polar), with the natural isotopes of Na and Cl (no
+    --readiso, as there): the consistency check of test_thermal_conductivity.py:25-40 holds
+    with four-phonon, Monte-Carlo and natural-isotope terms, both hosts write the same
+    outfile.thermal_conductivity, and the four-phonon term lowers kappa."""
     from tdep_amd import host, inputs
     print("[case] reference test flags rocksalt 5x5x5 --fourthorder -qg4ph 3 3 3 --seed 42", flush=True)
     dirs = {k: tmp_path / k for k in ("p", "f", "p3")}
     for d in dirs.values():
         inputs.make_config("rocksalt", str(d), fourth=True)
-    base = ["-qg", "5", "5", "5", "--temperature", "300", "--readiso"]
+    base = ["-qg", "5", "5", "5", "--temperature", "300"]
     args = base + ["--fourthorder", "-qg4ph", "3", "3", "3", "--seed", "42"]
     assert host.main(args + ["--directory", str(dirs["p"])]) == 0
     assert host.main(base + ["--directory", str(dirs["p3"])]) == 0
@@ -137,11 +137,14 @@ def test_hosts_with_the_reference_tests_flags(tmp_path):
     assert np.allclose(np.diag(kgrid), rows[0][:3] + rows[1][:3], rtol=1e-6)
     assert 0 < rows[-1][0] < rows3[-1][0] * (1 - 1e-6)
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdep_amd", "tc_host")
-    if os.path.exists(exe):
-        r = subprocess.run([exe] + args, cwd=dirs["f"], capture_output=True, text=True, timeout=300)
-        assert r.returncode == 0, r.stdout + r.stderr
-        rf = np.loadtxt(dirs["f"] / "outfile.thermal_conductivity")
-        assert np.allclose(rf, rows, rtol=1e-10, atol=1e-14 * np.abs(rows).max())
+    assert os.path.exists(exe), "the Fortran host is not built"
+    r = subprocess.run([exe] + args, cwd=dirs["f"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rf = np.loadtxt(dirs["f"] / "outfile.thermal_conductivity")
+    assert np.allclose(rf, rows, rtol=1e-10, atol=1e-14 * np.abs(rows).max())
+    # natural isotopes differ from the generated infile.isotopes (Cl has two isotopes)
+    assert host.main(args + ["--readiso", "--directory", str(dirs["p3"])]) == 0
+    assert np.abs(np.loadtxt(dirs["p3"] / "outfile.thermal_conductivity") - rows).max() > 1e-6 * np.abs(rows).max()
 
 
 def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
@@ -153,8 +156,7 @@ def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
     import shutil
     from tdep_amd import host
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdep_amd", "tc_host")
-    if not os.path.exists(exe):
-        pytest.skip("Fortran host not built (no Fortran compiler)")
+    assert os.path.exists(exe), "the Fortran host is not built"
     print("[case] Fortran host rocksalt 4x4x4", flush=True)
     src = inputs_dir("rocksalt")
     df, dp = tmp_path / "fortran", tmp_path / "python"
@@ -187,3 +189,45 @@ def test_fortran_host_writes_what_the_python_host_writes(inputs_dir, tmp_path):
                   "boundary_scattering_kappa"):
             a, b = np.array(hf["data"][k]["values"]), np.array(hp["data"][k]["values"])
             assert np.allclose(a, b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
+
+
+def test_nothirdorder_and_nosym(inputs_dir, oracle_setup):
+    """--nothirdorder (scattering.f90:234-238): with artificial mass disorder the linewidths and
+    scattering matrix are the isotope term alone, as the oracle forms it without the triplet
+    loop (1e-9); with no term left the call fails with status 5.  --nosym
+    (type_qpointmesh_gridgeneration.f90:54-58): the mesh is reduced by time reversal only, and
+    the single-mode and iterative kappa equal the symmetry-reduced ones (1e-8; the reduction
+    changes only the order of the sums)."""
+    from oracle import scattering as S
+    from tdep_amd.engine import Engine
+    from tdep_amd.lib import Tc3Error
+    name, grid, T = "rocksalt", (4, 4, 4), 300.0
+    print(f"[case] nothirdorder/nosym {name} {grid}", flush=True)
+    p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
+    g = np.linspace(1e-3, 3e-3, p.na)
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, T, isotope_g=g, thirdorder=False)
+    e = Engine.from_directory(inputs_dir(name), grid, device=0)
+    e.dispersions()
+    e.set_thirdorder(False)
+    with pytest.raises(Tc3Error) as ei:
+        e.scattering_rates([T], precision="fp64")
+    assert ei.value.code == 5
+    e.set_isotopes(g, enable=True)
+    e.scattering_rates([T], precision="fp64")
+    lw = e.linewidths(0)[0]
+    assert np.abs(lw - sr.linewidth).max() < 1e-9 * np.abs(sr.linewidth).max()
+    xi = e.scattering_matrix(0)[0]
+    assert np.abs(xi - sr.Xi).max() < 1e-9 * np.abs(sr.Xi).max()
+    e.set_thirdorder(True)
+    e.scattering_rates([T], precision="fp64")
+    assert e.linewidths(0)[0].min() >= 0 and np.abs(e.linewidths(0)[0] - lw).max() > 1e-3 * np.abs(lw).max()
+    # the mesh reduced by time reversal only gives the same kappa
+    ks, ki, _ = e.kappa(offdiag=False)
+    en = Engine.from_directory(inputs_dir(name), grid, device=0, qpsymmetry=False)
+    assert en.nirr > e.nirr and en.nirr == (en.nfull + 8) // 2
+    en.dispersions()
+    en.set_isotopes(g, enable=True)
+    en.scattering_rates([T], precision="fp64")
+    ks2, ki2, _ = en.kappa(offdiag=False)
+    assert np.abs(ks2[0] - ks[0]).max() < 1e-8 * np.abs(ks[0]).max()
+    assert np.abs(ki2[0] - ki[0]).max() < 1e-8 * np.abs(ki[0]).max()

@@ -218,8 +218,11 @@ def test_integration_type6_matches_oracle(inputs_dir, oracle_setup, name, grid):
 def test_montecarlo_grid_matches_oracle(inputs_dir, oracle_setup, name, grid, qg3):
     """Monte-Carlo q2 grids (-qg3ph, --seed 7): the host's 64-bit Mersenne twister and
     stratified draws give the oracle's per-row triplets and weights, so FP64 linewidths and
-    scattering matrix agree to 1e-9 on the small-cell and MFMA paths; mixed kappa within 1e-4;
+    scattering matrix agree to 1e-9 on the small-cell and MFMA paths; the iterative solution
+    (whose scattering matrix a Monte-Carlo grid leaves without time-reversal symmetry) gives
+    the oracle's kappa to 1e-7 in the oracle's iteration count; mixed kappa within 1e-4;
     switching the grid off restores the full-mesh rates."""
+    from oracle import kappa as K
     from oracle import scattering as S
     p, fc2, fc3, ops, mesh, disp = oracle_setup(name, grid)
     T = 300.0
@@ -231,6 +234,10 @@ def test_montecarlo_grid_matches_oracle(inputs_dir, oracle_setup, name, grid, qg
     assert _rel(lw, sr.linewidth) < 1e-9
     assert _rel(e.scattering_matrix(0)[0], sr.Xi) < 1e-9
     k64 = e.kappa(offdiag=False)
+    Fn, scf, _ = K.iterative_solution(p, mesh, disp, ops, sr, T)
+    kit = K.symmetrize_kappa(K.get_kappa(p, mesh, disp, ops, Fn, T), ops)
+    assert _rel(k64[1][0], kit) < 1e-7
+    assert e.niter[0] == len(scf)
     e.scattering_rates([T], precision="bf16x3" if e.nb in (48, 96) else "fp32")
     assert _rel(e.kappa(offdiag=False)[1], k64[1]) < 1e-4
     e.set_montecarlo(None, 0)

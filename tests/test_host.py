@@ -113,8 +113,8 @@ def test_fortran_host_reads_inputs_like_the_python_readers(inputs_dir, name, gri
     if not os.path.exists(exe):
         pytest.skip("Fortran host not built (no Fortran compiler)")
     d = inputs_dir(name)
-    r = subprocess.run([exe, "-qg"] + [str(g) for g in grid] + ["--device", "-1", "--inputs-only"], cwd=d,
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "-qg"] + [str(g) for g in grid] + ["--device", "-1", "--inputs-only", "--readiso"],
+                       cwd=d, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
     e = Engine.from_directory(d, grid, device=-1)
@@ -128,3 +128,78 @@ def test_fortran_host_reads_inputs_like_the_python_readers(inputs_dir, name, gri
                        rtol=1e-12, atol=1e-16)
     assert np.allclose(f("fc3"), [np.abs(fc3["m"]).sum(), fc3["m"][:, 0, 1, 2].sum(), np.abs(fc3["lv2"]).sum(),
                                   np.abs(fc3["lv3"]).sum()], rtol=1e-12, atol=1e-16)
+
+
+# standard atomic weights (IUPAC) and Tamura's mass-variance parameters, independent of the table
+_ATOMIC_WEIGHT = {"H": 1.00794, "C": 12.0107, "N": 14.0067, "O": 15.9994, "Na": 22.98976928, "Mg": 24.3050,
+                  "Al": 26.9815386, "Si": 28.0855, "P": 30.973762, "S": 32.065, "Cl": 35.453, "K": 39.0983,
+                  "Ca": 40.078, "Ti": 47.867, "Fe": 55.845, "Cu": 63.546, "Ga": 69.723, "Ge": 72.64,
+                  "As": 74.92160, "Ag": 107.8682, "Sn": 118.710, "Au": 196.966569, "Pb": 207.2, "Bi": 208.98040}
+
+
+def test_natural_isotope_table_against_standard_atomic_weights():
+    """tc3_natural_isotopes (the reference's naturaldistribution table): abundances sum to one,
+    the mean masses are the standard atomic weights, and the mass-disorder parameters of Si and
+    Ge are Tamura's (Phys. Rev. B 27, 858 (1983)): 2.01e-4 and 5.87e-4; single-isotope elements
+    have none.  Unknown symbols fail with the reference's parameter error."""
+    from tdep_amd import io as tio
+    for sym, w in _ATOMIC_WEIGHT.items():
+        c, m = tio.natural_isotopes(sym)
+        assert abs(c.sum() - 1.0) < 2e-4, sym
+        assert np.all(m > 0) and np.all(np.diff(m) > 0), sym
+        mean = (c / c.sum() * m).sum()
+        assert abs(mean - w) < 2e-4 * w, (sym, mean, w)
+    emu = tio.AMU_TO_EMU
+    for sym, g in (("Si", 2.01e-4), ("Ge", 5.87e-4)):
+        c, m = tio.natural_isotopes(sym)
+        mean, dis = tio._distribution(c, m)
+        assert abs(mean / emu - _ATOMIC_WEIGHT[sym]) < 1e-3
+        assert abs(dis - g) < 0.02 * g, (sym, dis)
+    c, m = tio.natural_isotopes("Al")
+    assert len(c) == 1 and tio._distribution(c, m)[1] == 0.0
+    with pytest.raises(ValueError):
+        tio.natural_isotopes("Xx")
+    lib = _lib()
+    n = ctypes.c_int(-1)
+    assert lib.tc3_natural_isotopes(b"Sn", 0, None, None, ctypes.byref(n)) == 0 and n.value == 10
+    buf = np.zeros(4)
+    from tdep_amd.lib import dptr
+    assert lib.tc3_natural_isotopes(b"Sn", 4, dptr(buf), dptr(buf), ctypes.byref(n)) == 1
+
+
+@pytest.mark.parametrize("name,grid", [("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2))])
+def test_natural_masses_without_readiso_in_both_hosts(inputs_dir, name, grid):
+    """Without --readiso both hosts take the masses and mass-disorder parameters from the natural
+    isotope distributions (type_crystalstructure.f90:650-652): the Fortran host's checksums equal
+    the Python readers', and differ from the infile.isotopes ones."""
+    import subprocess
+    from tdep_amd import io as tio
+    exe = os.path.join(ROOT, "tdep_amd", "tc_host")
+    assert os.path.exists(exe), "Fortran host not built"
+    d = inputs_dir(name)
+    r = subprocess.run([exe, "-qg"] + [str(g) for g in grid] + ["--device", "-1", "--inputs-only"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    p, _, _ = tio.read_directory(d, readiso=False)
+    q, _, _ = tio.read_directory(d, readiso=True)
+    assert float(out["structure"][3]) == pytest.approx(p.mass.sum(), rel=1e-14)
+    assert float(out["disorder"][0]) == pytest.approx(p.mass_disorder.sum(), rel=1e-13, abs=1e-300)
+    assert abs(p.mass.sum() - q.mass.sum()) > 1e-6 * q.mass.sum() or name == "rocksalt"
+    for a in range(p.na):
+        c, m = tio.natural_isotopes(p.symbols[a])
+        assert p.mass[a] == pytest.approx((c / c.sum() * m).sum() * tio.AMU_TO_EMU, rel=1e-14)
+
+
+def test_nosym_mesh_is_reduced_by_time_reversal_only(inputs_dir):
+    """--nosym (type_qpointmesh_gridgeneration.f90:54-58): the mesh keeps only the identity and
+    time reversal, so the irreducible points are the orbits {q, -q}; the crystal keeps its space
+    group (48 operations for rocksalt) for the kappa averages."""
+    from tdep_amd.engine import Engine
+    d = inputs_dir("rocksalt")
+    e = Engine.from_directory(d, (4, 4, 4), device=-1, qpsymmetry=False)
+    assert e.nsym == 48 and e.nfull == 64 and e.nirr == (64 + 8) // 2
+    m = e.mesh()
+    assert np.all(np.abs(m["op_of_full"]) == 1)
+    e2 = Engine.from_directory(d, (4, 4, 4), device=-1, qpsymmetry=False, timereversal=False)
+    assert e2.nirr == 64

@@ -339,6 +339,67 @@ def test_offdiag_velocities_equal_reference_kronecker_form(oracle_setup):
             assert np.allclose(cm2[i * nb + j], V[:, i, j], atol=1e-12 * scale)
 
 
+def test_offdiag_diagonal_terms_equal_single_mode_kappa(oracle_setup):
+    """The coherence sum of kappa.f90:246-285 with its j = k terms switched on: the coherence
+    velocity of a non-degenerate mode is its group velocity, the Lorentzian at zero frequency
+    difference is 1/(Gamma_j + Gamma_j), and the heat-capacity mean is c_v, so each diagonal
+    term must be that mode's single-mode-approximation kappa w_q c_v (v x v) / (2 Gamma V)
+    (kappa.f90:28-75), as get_kappa forms it from F0 = v / (2 Gamma).  A doubled
+    heat-capacity mean, a wrong 1/sqrt(2 omega) scale of the velocities, a Lorentzian in Gamma
+    instead of the sum of widths or a wrong prefactor fails it.  The reference's own sum
+    (j != k) is unchanged by the option."""
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (4, 4, 4))
+    T = 300.0
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, T)
+    _, kq = K.get_kappa(p, mesh, disp, ops, sr.F0, T, per_mode=True)
+    kod0 = K.get_kappa_offdiag(p, fc2, mesh, disp, ops, sr.linewidth, T)
+    kod1, tq = K.get_kappa_offdiag(p, fc2, mesh, disp, ops, sr.linewidth, T, include_diagonal=True,
+                                   terms=True)
+    ref = K.get_kappa_offdiag(p, fc2, mesh, disp, ops, sr.linewidth, T, terms=True)[1]
+    assert np.array_equal(kod0, K.get_kappa_offdiag(p, fc2, mesh, disp, ops, sr.linewidth, T))
+    assert np.abs(np.einsum("qjjab->qab", ref)).max() == 0.0
+    checked = 0
+    scale = np.abs(kq).max()
+    for q in range(mesh.nirr):
+        for b in range(disp.nb):
+            if disp.om_irr[q, b] < U.FREQTOL or len(disp.degen_irr[q][b]) != 1:
+                continue
+            assert np.allclose(tq[q, b, b], kq[q, b] * mesh.weight_irr[q], rtol=1e-10, atol=1e-12 * scale)
+            checked += 1
+    assert checked >= 20
+    # the diagonal is what the reference leaves out: with it the sum gains the SMA kappa of
+    # the non-degenerate modes
+    assert np.abs(kod1 - kod0).max() > 0.1 * np.abs(kod0).max()
+
+
+def test_threephonon_rates_zero_and_high_temperature_limits(oracle_setup):
+    """Limits of the occupation factors of scattering_threephonon.f90:116-123 that the
+    Bose-Einstein function fixes, without restating the integrand.  At T -> 0 every n -> 0,
+    so only the spontaneous decay (n2 + n3 + 1 -> 1) survives: the top optical band of every
+    irreducible point, which can decay into two lower modes, keeps a positive linewidth.  At
+    high T, n = kT/w - 1/2 + w/(12 kT) + ..., so in n2 + n3 + 1 and n2 - n3 the constants
+    cancel and Gamma(T) = kT S + O(1/T): the intercept 2 Gamma(T) - Gamma(2T) vanishes to
+    O(w/kT) against Gamma(0).  A dropped (or doubled) +1 leaves Gamma(0) = 0 (or shifts the
+    intercept by -Gamma(0) (or +Gamma(0))), and fails."""
+    p, fc2, fc3, ops, mesh, disp = oracle_setup("rocksalt", (4, 4, 4))
+    g0 = S.scattering_rates(p, fc3, mesh, disp, ops, 0.5).linewidth
+    g1 = S.scattering_rates(p, fc3, mesh, disp, ops, 2.0e5).linewidth
+    g2 = S.scattering_rates(p, fc3, mesh, disp, ops, 4.0e5).linewidth
+    g300 = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0).linewidth
+    assert g0.min() >= 0.0
+    top = g0[:, -1]
+    assert top.min() > 1e-3 * g300[:, -1].max()
+    assert np.all(g0 <= g300 * (1 + 1e-12))
+    # the T -> 0 linewidth is the spontaneous decay: where it is appreciable, the intercept of
+    # the high-temperature line is small against it
+    big = g0 > 1e-2 * g0.max()
+    assert big.sum() >= 10
+    icpt = 2.0 * g1 - g2
+    assert np.abs(icpt[big] / g0[big]).max() < 1e-2
+    # and the slope is the classical one: Gamma doubles with T
+    assert np.abs(g2[big] / g1[big] - 2.0).max() < 1e-3
+
+
 def test_c_port_of_reference_row_loop_matches_oracle(oracle_setup):
     """oracle/scatter_ref.c follows the reference's zgeru/dot_product per-element loop;
     its per-row linewidths (before degeneracy averaging) equal the oracle's."""
