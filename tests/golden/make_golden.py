@@ -17,12 +17,18 @@ from oracle import run as R  # noqa: E402
 from tdep_amd import inputs  # noqa: E402
 
 CASES = [("fcc_al", (6, 6, 6), (300.0,)), ("rocksalt", (4, 4, 4), (100.0, 300.0)),
-         ("sl16", (4, 4, 2), (300.0,))]
+         ("sl16", (4, 4, 2), (300.0,)),
+         # the monatomic FCC workload at its full 12x12x12 mesh, and the 32-atom superlattice
+         # (96 bands) on a small in-plane mesh
+         ("fcc_al", (12, 12, 12), (300.0,)), ("sl32", (4, 4, 1), (300.0,))]
 
 
-def main():
+def main(only=None):
+    """only: fixture names (e.g. "sl32_2x2x2") to write; all when empty."""
     out = os.path.join(ROOT, "tests", "golden")
     for name, grid, temps in CASES:
+        if only and f"{name}_{'x'.join(map(str, grid))}" not in only:
+            continue
         with tempfile.TemporaryDirectory() as d:
             inputs.make_config(name, d)
             pre = R.setup(d, grid)
@@ -41,4 +47,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
