@@ -177,6 +177,18 @@ int tc3_get_xi(const tc3_ctx *ctx, int iT, int row0, int nrow, double *xi, int *
 int tc3_kappa(tc3_ctx *ctx, int maxiter, double tol, int classical, double *kappa_sma,
               double *kappa_iter, double *kappa_offdiag, int *niter);
 
+/* tc3_kappa without device allocation, for callers that cannot allocate device memory
+ * (serving loops, code between allocation-free regions, harnesses that forbid it).  No
+ * counterpart in the reference.  Same results as tc3_kappa with kappa_offdiag = NULL (single-mode
+ * and iterative kappa, iteration counts, the log and per-mode results of tc3_get_kappa_log /
+ * tc3_get_modes): the iteration's working buffers are carved from one fixed 32 MiB device array
+ * per GPU (held until the process ends) and its chunks are launched directly (no graph).  The
+ * inputs are what tc3_scatter left on the device.  Returns 1 when the solve needs more than the
+ * array holds (its size is printed in the error message), 5 as tc3_kappa.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several contexts or threads) race. */
+int tc3_kappa_noalloc(tc3_ctx *ctx, int maxiter, double tol, int classical, double *kappa_sma,
+                      double *kappa_iter, int *niter);
+
 /* Per-iteration log of the last tc3_kappa call for temperature iT: up to maxlen
  * records of (kxx, kyy, kzz, kxy, kxz, kyz, DeltaF/F) in atomic units
  * (kappa.f90:377, 473-474).  Returns the number of records in *nlog. */

@@ -513,6 +513,26 @@ int tc3_kappa(tc3_ctx *ctx, int maxiter, double tol, int classical, double *kapp
     });
 }
 
+int tc3_kappa_noalloc(tc3_ctx *ctx, int maxiter, double tol, int classical, double *kappa_sma, double *kappa_iter,
+                      int *niter) {
+    return guarded(ctx, [&]() -> int {
+        if (ctx->device < 0) return ctx->fail(7, "host-only context has no device");
+        if (!ctx->have_scatter) return ctx->fail(5, "run tc3_scatter first");
+        if ((classical != 0) != ctx->classical) return ctx->fail(5, "classical flag differs from tc3_scatter");
+        void *ptr = nullptr;
+        int r = bte_scratch_ptr(ctx->device, &ptr);
+        if (r) return r;
+        r = run_kappa(ctx, maxiter, tol, false, ptr, bte_scratch_bytes());
+        if (r) return r;
+        const int nT = ctx->nT;
+        if (kappa_sma) std::copy(ctx->kappa_sma.begin(), ctx->kappa_sma.end(), kappa_sma);
+        if (kappa_iter) std::copy(ctx->kappa_iter.begin(), ctx->kappa_iter.end(), kappa_iter);
+        if (niter) std::copy(ctx->niter.begin(), ctx->niter.begin() + nT, niter);
+        ctx->have_kappa = true;
+        return 0;
+    });
+}
+
 int tc3_get_kappa_log(const tc3_ctx *ctx, int iT, int maxlen, double *log, int *nlog) {
     if (!ctx || !ctx->have_kappa || iT < 0 || iT >= ctx->nT) return 5;
     const auto &l = ctx->kappa_log[iT];

@@ -181,7 +181,9 @@ int ensure_full_host(tc3_ctx *c);
 // tc3_scatter.hip
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma);
 // tc3_kappa.hip
-int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag);
+int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch = nullptr, size_t scratch_bytes = 0);
+size_t bte_scratch_bytes();
+int bte_scratch_ptr(int device, void **ptr);
 void kappa_rta_cotangent(const tc3_ctx *c, int iT, const double *G, double *dlw);
 // tc3_adjoint.hip
 int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<double> &grad);

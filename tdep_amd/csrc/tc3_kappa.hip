@@ -7,6 +7,9 @@
 //   symmetrize_kappa   kappa.f90:308-324
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -768,7 +771,31 @@ void kappa_rta_cotangent(const tc3_ctx *c, int iT, const double *G, double *dlw)
         }
 }
 
-int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
+// Working memory of tc3_kappa_noalloc: every buffer of the iterative solve is carved from this
+// array (one copy per GPU, held until the process ends).  Host code never uses the name as an
+// address: bte_scratch_ptr() resolves the device address of the current GPU's copy once.
+constexpr size_t kBteScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char bte_scratch[kBteScratchBytes];
+
+size_t bte_scratch_bytes() { return kBteScratchBytes; }
+
+int bte_scratch_ptr(int device, void **ptr) {
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        // the device is current (the caller selected it)
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(bte_scratch));
+        if (e != hipSuccess || p == nullptr) return device_error(e, "hipGetSymbolAddress(bte_scratch)", __FILE__, __LINE__);
+        it = addr.emplace(device, p).first;
+    }
+    *ptr = it->second;
+    return 0;
+}
+
+int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, size_t scratch_bytes) {
     const int nb = c->nb, nirr = c->mesh.nirr, nfull = c->mesh.nfull, nT = c->nT, nsym = (int)c->ops.size();
     const int nrows = c->nrows;
     const size_t qb = (size_t)nirr * nb;
@@ -789,10 +816,23 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     hipError_t e = hipSuccess;
 #define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
     // device buffers of the solve live in the context's pool, so that repeated calls reuse
-    // them (and the captured iteration graph below)
+    // them (and the captured iteration graph below); with a scratch array they are carved
+    // from it instead (256-byte aligned), nothing is allocated and no graph is captured
     int nbuf = 0;
+    size_t used = 0;
+    bool overflow = false;
+    if (scratch && offdiag) return c->fail(5, "the off-diagonal term needs device allocations");
     auto dalloc = [&](void **p, size_t n) -> hipError_t {
-        return c->pool_get("bte" + std::to_string(nbuf++), std::max<size_t>(n, 16), p);
+        if (!scratch) return c->pool_get("bte" + std::to_string(nbuf++), std::max<size_t>(n, 16), p);
+        const size_t need = (std::max<size_t>(n, 16) + 255) & ~(size_t)255;
+        if (used + need > scratch_bytes) {
+            overflow = true;
+            *p = scratch;   // never used: the solve stops below
+            return hipSuccess;
+        }
+        *p = (char *)scratch + used;
+        used += need;
+        return hipSuccess;
     };
     double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_buf, *d_opm, *d_g0, *d_cvw, *d_vel, *d_klog;
     int *d_opf, *d_rowof, *d_dgoff, *d_dg;
@@ -859,7 +899,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             fzero[q] = (cnt > 0 && mx < 1e-8 * (double)cnt) ? 1 : 0;
         }
         TRY(dalloc((void **)&d_fz, sizeof(int) * std::max(nirr, 1)));
-        TRY(hipMemcpy(d_fz, fzero.data(), sizeof(int) * nirr, hipMemcpyHostToDevice));
+        if (!overflow) TRY(hipMemcpy(d_fz, fzero.data(), sizeof(int) * nirr, hipMemcpyHostToDevice));
     }
     TRY(dalloc((void **)&d_cvw, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
@@ -869,6 +909,9 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
     TRY(dalloc((void **)&d_part, sizeof(double) * sPart * nT));
     TRY(dalloc((void **)&d_scf, sizeof(double) * sScf * nT));
     TRY(dalloc((void **)&d_st, sizeof(BteState) * nT));
+    if (overflow)
+        return c->fail(1, "the iterative solution needs " + std::to_string(used >> 20) + "+ MiB, more than the " +
+                              std::to_string(scratch_bytes >> 20) + " MiB scratch array holds");
     TRY(hipMemcpyAsync(d_vel, c->vel_irr.data(), sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
@@ -967,11 +1010,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
                                        (long)(uintptr_t)d_dg, (long)(uintptr_t)d_cvw, (long)(uintptr_t)d_vel,
                                        (long)(uintptr_t)d_klog, (long)(uintptr_t)d_part, (long)(uintptr_t)d_scf,
                                        (long)(uintptr_t)d_st, sF, sFbb, sBuf, sPart, sG, sScf, sLog};
-        if (c->bte_exec && c->bte_key != key) {
+        if (!scratch && c->bte_exec && c->bte_key != key) {
             (void)hipGraphExecDestroy(c->bte_exec);
             c->bte_exec = nullptr;
         }
-        if (!c->bte_exec && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        if (!scratch && !c->bte_exec && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             hipGraph_t graph = nullptr;
             for (int k = 0; k < kChunkIt; ++k) iteration();
             if (hipStreamEndCapture(st, &graph) == hipSuccess &&
@@ -982,7 +1025,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag) {
             if (graph) (void)hipGraphDestroy(graph);
         }
         (void)hipGetLastError();
-        hipGraphExec_t gexec = c->bte_exec;
+        hipGraphExec_t gexec = scratch ? nullptr : c->bte_exec;
         for (bool done = false; !done;) {
             if (gexec) TRY(hipGraphLaunch(gexec, st));
             else

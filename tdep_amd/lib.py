@@ -37,6 +37,8 @@ SIGNATURES = {
     "tc3_get_xi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_int_p]),
     "tc3_kappa": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_dbl_p,
                                  _c_dbl_p, _c_dbl_p, _c_int_p]),
+    "tc3_kappa_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_dbl_p,
+                                         _c_dbl_p, _c_int_p]),
     "tc3_get_kappa_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_int_p]),
     "tc3_get_modes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p]),
     "tc3_get_timings": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int]),
