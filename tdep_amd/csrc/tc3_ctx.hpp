@@ -125,6 +125,11 @@ struct tc3_ctx {
     std::vector<int> xi_src;
     bool xi_src_ok = false;
     long xi_src_gen = -1;
+    // IFC block tables of the fused 48-band kernel (per model)
+    std::vector<int4> fused_blocks, fused_pairs;
+    std::vector<int> fused_pa1;
+    int fused_maxe = 0, fused_maxb = 0;
+    long fused_gen = -1;
     // orbit chunks of the scattering-matrix kernels (per model) and pooled HIP events
     std::vector<int> xchk_off, xchk;
     int xcapm = 0;

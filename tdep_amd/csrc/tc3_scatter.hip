@@ -24,6 +24,7 @@
 #include "tc3_chain.hpp"
 #include "tc3_ctx.hpp"
 #include "tc3_fourphonon.hpp"
+#include "tc3_fused.hpp"
 #include "tc3_mfma.hpp"
 
 namespace tc3 {
@@ -965,16 +966,82 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         if (mp.nty == 0) { cleanup(); return c->fail(5, "band count too large for the MFMA kernels' LDS"); }
         mp.ntrip = ntrip; mp.batch = mbatch;
         mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
-        TRY(c->pool_get("m_E1re", es * (size_t)nirr * nb * nb, &mp.E1re));
-        TRY(c->pool_get("m_E1im", es * (size_t)nirr * nb * nb, &mp.E1im));
-        TRY(c->pool_get("m_Efre", es * (size_t)nfull * nb * nb, &mp.Efre));
-        TRY(c->pool_get("m_Efim", es * (size_t)nfull * nb * nb, &mp.Efim));
-        mp.Eti = mp.Etf = nullptr;
-        if (precision == TC3_PREC_BF16X3) {
-            TRY(c->pool_get("m_Eti", 2 * mfma_split_elems(nb) * nirr, &mp.Eti));
-            TRY(c->pool_get("m_Etf", 2 * mfma_split_elems(nb) * nfull, &mp.Etf));
+        // the fused kernel (48 bands, split-BF16x3, compact sums) needs neither W nor the
+        // chain's operand planes
+        if (c->xi_src_gen != c->gen_model) {
+            c->xi_src_ok = xi_source_map(c->trip, c->orb, nirr, nfull, nb, c->xi_src);
+            c->xi_src_gen = c->gen_model;
         }
-        TRY(c->pool_get("m_work", wtrip * bmax, &mp.work));
+        const size_t comp_T = (size_t)ntrip * nb * 2 * nb;
+        const bool comp_ok = c->xi_src_ok && (double)c->xi_es * comp_T < 4.0e9 &&
+                             (double)c->xi_es * c->xi_row < 4.0e9 && std::getenv("TC3_XI_DIRECT") == nullptr;
+        // IFC blocks (a1, a2, a3) in entry order and the (a2, a3) pairs with their blocks, for the
+        // fused kernel: its Phi slot holds, per pair, per (j, k), per block, the three i values
+        if (c->fused_gen != c->gen_model) {
+            std::vector<std::vector<int>> per((size_t)na * na);   // block indices of every pair
+            std::vector<int4> blocks;
+            std::vector<int> a1of;
+            int maxe = 0;
+            for (int a1 = 0; a1 < na; ++a1)
+                for (int a2 = 0; a2 < na; ++a2)
+                    for (int a3 = 0; a3 < na; ++a3) {
+                        const size_t bk = ((size_t)a1 * na + a2) * na + a3;
+                        if (c->blk3_off[bk + 1] == c->blk3_off[bk]) continue;
+                        per[(size_t)a2 * na + a3].push_back((int)blocks.size());
+                        blocks.push_back(make_int4(c->blk3_off[bk], c->blk3_off[bk + 1], 0, 0));
+                        a1of.push_back(a1);
+                        maxe = std::max(maxe, c->blk3_off[bk + 1] - c->blk3_off[bk]);
+                    }
+            std::vector<int4> pairs;
+            std::vector<int> pa1;
+            int base = 0, maxb = 0;
+            // pairs by decreasing number of blocks (the kernel's lanes then loop over similar counts)
+            std::vector<int> order;
+            for (int x = 0; x < na * na; ++x)
+                if (!per[x].empty()) order.push_back(x);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return per[x].size() > per[y].size(); });
+            for (int x : order) {
+                {
+                    const int a2 = x / na, a3 = x % na;
+                    const auto &v = per[x];
+                    const int nbp = (int)v.size();
+                    pairs.push_back(make_int4(a2, a3, base, nbp));
+                    for (int bb = 0; bb < nbp; ++bb) {
+                        // value (bb, i, j, k) of the pair at base + 27 bb + 9 i + 3 j + k
+                        blocks[v[bb]].z = base + 27 * bb;
+                        blocks[v[bb]].w = 0;
+                        pa1.push_back(a1of[v[bb]]);
+                    }
+                    base += 27 * nbp;
+                    maxb = std::max(maxb, nbp);
+                }
+            }
+            c->fused_blocks = blocks;
+            c->fused_pairs = pairs;
+            c->fused_pa1 = pa1;
+            c->fused_maxe = maxe;
+            c->fused_maxb = maxb;
+            c->fused_gen = c->gen_model;
+        }
+        // opt-in (TC3_FUSED=1): no W round trip, but with one wave per SIMD (its LDS) it is
+        // latency-bound and slower than the two kernels below
+        const bool fused = comp_ok &&
+                           fused_supported(nb, precision, integrationtype, c->nfc3, (int)c->fused_blocks.size(),
+                                           (int)c->fused_pairs.size(), c->fused_maxe, c->fused_maxb) &&
+                           std::getenv("TC3_FUSED") != nullptr;
+        mp.Eti = mp.Etf = nullptr;
+        if (!fused) {
+            TRY(c->pool_get("m_E1re", es * (size_t)nirr * nb * nb, &mp.E1re));
+            TRY(c->pool_get("m_E1im", es * (size_t)nirr * nb * nb, &mp.E1im));
+            TRY(c->pool_get("m_Efre", es * (size_t)nfull * nb * nb, &mp.Efre));
+            TRY(c->pool_get("m_Efim", es * (size_t)nfull * nb * nb, &mp.Efim));
+            if (precision == TC3_PREC_BF16X3) {
+                TRY(c->pool_get("m_Eti", 2 * mfma_split_elems(nb) * nirr, &mp.Eti));
+                TRY(c->pool_get("m_Etf", 2 * mfma_split_elems(nb) * nfull, &mp.Etf));
+            }
+            TRY(c->pool_get("m_work", wtrip * bmax, &mp.work));
+        }
         mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off;
         mp.blist = d_blist; mp.a1_boff = d_a1boff; mp.a12_boff = d_a12boff;
         mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
@@ -990,18 +1057,51 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         // compact per-triplet sums, assembled into the rows by k_xi_asm, when every entry
         // has one owner (TC3_XI_DIRECT: direct stores)
         me.comp = nullptr;
-        me.comp_T = (size_t)ntrip * nb * 2 * nb;
-        if (c->xi_src_gen != c->gen_model) {
-            c->xi_src_ok = xi_source_map(c->trip, c->orb, nirr, nfull, nb, c->xi_src);
-            c->xi_src_gen = c->gen_model;
-        }
+        me.comp_T = comp_T;
         // k_xi_asm addresses one temperature's sums and one row with 32-bit byte offsets
-        const bool fits32 = (double)c->xi_es * me.comp_T < 4.0e9 && (double)c->xi_es * c->xi_row < 4.0e9;
-        if (c->xi_src_ok && fits32 && std::getenv("TC3_XI_DIRECT") == nullptr) {
+        if (comp_ok) {
             TRY(table("xisrc", (void **)&d_xisrc, c->xi_src.data(), sizeof(int4) * (c->xi_src.size() / 4)));
             TRY(c->pool_get("xi_comp", (size_t)c->xi_es * nT * me.comp_T, &me.comp));
             d_comp = me.comp;
         }
+        if (fused) {
+            int4 *d_fblocks, *d_fpairs;
+            int *d_fpa1;
+            TRY(table("f_blocks", (void **)&d_fblocks, c->fused_blocks.data(), sizeof(int4) * c->fused_blocks.size()));
+            TRY(table("f_pairs", (void **)&d_fpairs, c->fused_pairs.data(), sizeof(int4) * c->fused_pairs.size()));
+            TRY(table("f_pa1", (void **)&d_fpa1, c->fused_pa1.data(), sizeof(int) * c->fused_pa1.size()));
+            int ncu = 0;
+            TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+            FusedPlan fp;
+            fp.nirr = nirr; fp.nfull = nfull; fp.nT = nT; fp.nty = fused_temperature_block(nT);
+            fp.nwg = std::max(ncu, 1);
+            fp.egv_irr = c->d_egv_irr; fp.egv_full = c->d_egv_full; fp.om_irr = c->d_om_irr; fp.om_full = c->d_om_full;
+            fp.be = d_be;
+            TRY(c->pool_get("f_tab", fused_table_bytes(nfull), (void **)&fp.ftab));
+            TRY(c->pool_get("f_e1re", sizeof(float) * (size_t)nirr * nb * nb, (void **)&fp.e1re));
+            TRY(c->pool_get("f_e1im", sizeof(float) * (size_t)nirr * nb * nb, (void **)&fp.e1im));
+            fp.comp = d_comp;
+            FusedArgs &fa = fp.args;
+            fa.ntrip = (int)ntrip; fa.nirr = nirr; fa.gT = nT; fa.integrationtype = integrationtype;
+            fa.nfc3 = c->nfc3; fa.nblk = (int)c->fused_blocks.size(); fa.npair = (int)c->fused_pairs.size();
+            fa.npblk = (int)c->fused_pa1.size();
+            fa.sigma_fixed = ea.sigma_fixed; fa.psi_scale = ea.psi_scale;
+            fa.tq1 = d_tq1; fa.tq2 = d_tq2; fa.tq3 = d_tq3; fa.eq_off = d_eqoff; fa.irr_of_full = c->d_irr_of_full;
+            fa.row_of = d_rowof; fa.perm = d_perm; fa.om_irr = c->d_om_irr; fa.om_full = c->d_om_full;
+            fa.sigsq = d_sigsq; fa.qcart = d_qcart; fa.lv2 = d_lv2; fa.lv3 = d_lv3; fa.mwm = d_mwm;
+            fa.blocks = d_fblocks; fa.pairs = d_fpairs; fa.pa1 = d_fpa1; fa.ftab = fp.ftab;
+            fa.e1re = fp.e1re; fa.e1im = fp.e1im;
+            TRY(c->pool_get("f_slots", sizeof(float2) * (size_t)fp.nwg * std::max(fa.nblk, 1) * 27, (void **)&fa.phi_slot));
+            TRY(c->pool_get("f_queue", sizeof(int), (void **)&fa.queue));
+            fa.comp_T = comp_T; fa.gpart = d_gpart; fa.tw = ea.tw;
+            TRY(take_events(kev, 3));
+            fp.ev = kev.data();
+            TRY(hipEventRecord(ev0, st));
+            TRY(hipEventRecord(kev[0], st));
+            const int r = run_fused(fp, st);
+            if (r) { cleanup(); return r; }
+            c->batch = ntrip;
+        } else {
         // Autotuner: on the first pass of a shape (band count, precision, temperature block,
         // triplet count), time each candidate batch (the largest, then halving, at least 64)
         // on the same first triplets (as many as the largest batch) and keep the fastest.
@@ -1075,6 +1175,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             if (r) { cleanup(); return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r; }
         }
         c->batch = mbatch;
+        }
     } else if (use_small) {
         // persistent work-queue kernel for small cells
         TRY(hipEventRecord(ev0, st));
