@@ -26,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}   # dense MFMA peaks of the arithmetic used
 PEAK_HBM_GBPS = 8000.0
 WORKLOAD = "sl16"
 
@@ -156,66 +156,88 @@ def main():
         # graph instantiation on first use are process start-up, not part of the solve)
         kappa_wall(eng, temps, args.precision)
         kw_mx, ksma_mx, kit_mx, st_mx = kappa_wall(eng, temps, args.precision)
+        lw_mx = np.array([eng.linewidths(i)[0] for i in range(len(temps))])
         kappa_wall(eng, temps, "fp64")
         kw_64, ksma_64, kit_64, st_64 = kappa_wall(eng, temps, "fp64")
+        lw_64 = np.array([eng.linewidths(i)[0] for i in range(len(temps))])
         err = float(np.max(np.abs(kit_mx - kit_64)) / np.max(np.abs(kit_64)))
         err_sma = float(np.max(np.abs(ksma_mx - ksma_64)) / np.max(np.abs(ksma_64)))
+        big = lw_64 > 1e-3 * lw_64.max(axis=(1, 2), keepdims=True)
+        lw_rel = np.abs(lw_mx[big] / lw_64[big] - 1)
+        lw_err = dict(max=float(lw_rel.max()), median=float(np.median(lw_rel)),
+                      modes="linewidths above 1e-3 of the largest, all temperatures")
+        # end to end: a fresh context from the input files to outfile.thermal_conductivity and
+        # the grid file (autotuner probes, graph instantiation and uploads included)
+        e2e = {}
+        for prec in (args.precision, "fp64"):
+            t0 = time.perf_counter()
+            e2 = Engine.from_directory(d, grid, device=0)
+            e2.dispersions()
+            e2.scattering_rates(temps, precision=prec)
+            e2.kappa()
+            with open(os.path.join(d, "outfile.thermal_conductivity"), "w") as fh:
+                fh.write(e2.outfile_text(0))
+            e2.write_grid_hdf5(os.path.join(d, "outfile.thermal_conductivity_grid.hdf5"), 0)
+            e2e[prec if prec == "fp64" else "mixed"] = time.perf_counter() - t0
+            del e2
         log(f"[bench] kappa wall {args.precision} {kw_mx:.3f}s fp64 {kw_64:.3f}s rel err {err:.2e}")
-        # roofline of the dominant kernel (algorithmic complex-GEMM FLOPs, 8 per complex MAC)
-        # k_uw: W = E2^T U is a dense nb^4 complex GEMM per triplet; U = Phi E3 is formed only
-        # on the nonzero (a1, a2, a3) IFC blocks, i.e. nb^4 complex MACs times their fraction
-        na = nb // 3
-        blk = len(set(zip(eng.fc3["a1"].tolist(), eng.fc3["a2"].tolist(), eng.fc3["a3"].tolist())))
-        frac_blk = blk / float(na ** 3)
-        fpt_uw = 8.0 * nb ** 4 * (1.0 + frac_blk)
-        f_uw = fpt_uw * eng.ntriplets
-        f_psi = 8.0 * nb ** 4 * eng.ntriplets       # one nb^4 complex GEMM per triplet
+        # Roofline (MFMA-bound contraction): the |Psi|^2 chain is three complex contractions of
+        # nb^4 MACs per visited triplet, 24 nb^4 real FLOP algorithmically (8 per complex MAC),
+        # whatever the kernels issue: k_uw forms U = Phi E3 and W = E2^T U (16 nb^4), k_psi
+        # Psi = E1^T W (8 nb^4) and the smeared-delta epilogue.  Peak: BF16 dense MFMA (the
+        # split-BF16x3 products issue three BF16 MFMAs each, so 1/3 is the attainable ceiling).
+        # The kernel statistics cover the last tc3_scatter call, i.e. one step.
         ms_uw, n_uw, batch = ks_mx["phi_u_w"]
         ms_psi, n_psi, _ = ks_mx["psi_epilogue"]
-        if ms_uw >= ms_psi:
-            dom, fl, ms, n = "k_uw (Phi build + U=Phi.E3 + W=E2^T.U)", f_uw, ms_uw, n_uw
-        else:
-            dom, fl, ms, n = "k_psi (Psi=E1^T.W + smeared-delta epilogue)", f_psi, ms_psi, n_psi
-        # kernel statistics cover the last tc3_scatter call, i.e. one step.  Algorithmic
-        # bytes per launch: W (2 nb^3 reals per triplet, FP32 in the mixed modes) written by
-        # k_uw and read once by k_psi, plus what k_psi stores: the compact per-triplet sums
-        # (2 nb per (b1, T)) when the rows are assembled afterwards, else the
-        # scattering-matrix entries (FP32 in the mixed modes); tables and eigenvectors are
-        # negligible.  The roofline reported is the one whose minimum time (algorithmic
-        # work at peak) is larger.
-        avg_ms = ms / max(1, n)
-        es = 8 if args.precision == "fp64" else 4
-        w_bytes = 2 * nb ** 3 * es * batch
-        assembled = ks_mx["xi_assembled"][1] > 0
-        if assembled:
-            out_bytes = len(temps) * eng.ntriplets * nb * 2 * nb * es / max(1, n_psi)
-        else:
-            out_bytes = len(temps) * eng.nrows * eng.nfull * nb * es / max(1, n_psi)
-        by = w_bytes if dom.startswith("k_uw") else w_bytes + out_bytes
-        fl_launch = fl / max(1, n)
-        issue = 3.0 if args.precision == "bf16x3" else 1.0
+        ms_asm, n_asm, _ = ks_mx["xi_assembled"]
         peak = PEAK_TFLOPS[args.precision]
-        t_mfma = fl_launch * issue / (peak * 1e12)
-        t_hbm = by / (PEAK_HBM_GBPS * 1e9)
-        mfma_view = dict(achieved=fl_launch / (avg_ms * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
-                         frac=fl_launch / (avg_ms * 1e-3) / 1e12 / peak,
-                         flop_per_launch=fl_launch, issued_min_ms_at_peak=t_mfma * 1e3)
+        per_trip = {"k_uw": 16.0 * nb ** 4, "k_psi": 8.0 * nb ** 4}
+        kern = {}
+        for name, ms, n in (("k_uw", ms_uw, n_uw), ("k_psi", ms_psi, n_psi)):
+            avg = ms / max(1, n)
+            fl = per_trip[name] * batch
+            kern[name] = dict(avg_launch_ms=avg, launches=n, triplets_per_launch=batch, flop_per_launch=fl,
+                              achieved_TFLOPs=fl / (avg * 1e-3) / 1e12, frac=fl / (avg * 1e-3) / 1e12 / peak)
+        t_con = (ms_uw + ms_psi) * 1e-3
+        fl_con = 24.0 * nb ** 4 * eng.ntriplets
+        contraction = dict(flop_per_step=fl_con, kernel_ms_per_step=t_con * 1e3,
+                           achieved_TFLOPs=fl_con / t_con / 1e12, frac=fl_con / t_con / 1e12 / peak,
+                           peak_TFLOPs=peak, unit="TFLOP/s")
         if args.precision == "bf16x3":
-            mfma_view["issue_frac"] = 3 * mfma_view["frac"]
-        hbm_view = dict(achieved=by / (avg_ms * 1e-3) / 1e9, peak=PEAK_HBM_GBPS, unit="GB/s",
-                        frac=by / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, bytes_per_launch=by,
-                        min_ms_at_peak=t_hbm * 1e3)
+            contraction["frac_of_split_bf16x3_ceiling"] = 3 * contraction["frac"]
+        dom = "k_psi" if ms_psi >= ms_uw else "k_uw"
+        kd = kern[dom]
+        # measured HBM traffic per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 counters): the W
+        # intermediate (2 nb^3 reals per triplet) is not algorithmic work; its round trip is
+        # the waste against the output (the compact per-triplet sums)
         traffic, tsrc = (None, None)
         if args.precision == "bf16x3":
-            traffic, tsrc = profile_traffic("k_uw" if dom.startswith("k_uw") else "k_psi", batch)
-        main = hbm_view if t_hbm >= t_mfma else mfma_view
-        roof = dict(bound="hbm" if t_hbm >= t_mfma else "mfma", achieved=main["achieved"], peak=main["peak"],
-                    unit=main["unit"], frac=main["frac"], traffic=traffic, traffic_source=tsrc, kernel=dom,
-                    launches_per_step=n, avg_launch_ms=avg_ms, triplets_per_launch=batch,
-                    flop_per_triplet=fpt_uw if dom.startswith("k_uw") else 8.0 * nb ** 4,
-                    ifc_block_fraction=frac_blk, rows_assembled=assembled, hbm=hbm_view, mfma=mfma_view)
+            traffic, tsrc = profile_traffic(dom, batch)
+        es = 8 if args.precision == "fp64" else 4
+        out_trip = len(temps) * nb * 2 * nb * es
+        w_trip = 2 * nb ** 3 * es
+        roof = dict(bound="mfma", achieved=kd["achieved_TFLOPs"], peak=peak, unit="TFLOP/s", frac=kd["frac"],
+                    traffic=traffic, traffic_source=tsrc, kernel=dom, avg_launch_ms=kd["avg_launch_ms"],
+                    launches_per_step=kd["launches"], triplets_per_launch=batch,
+                    flop_per_triplet=per_trip[dom], kernels=kern, contraction=contraction,
+                    hbm_waste={"w_bytes_per_triplet": w_trip, "w_round_trip_bytes_per_step": 2 * w_trip * eng.ntriplets,
+                               "output_bytes_per_triplet": out_trip})
         if traffic:
-            roof["measured_hbm_GBps"] = traffic / (avg_ms * 1e-3) / 1e9
+            roof["traffic_per_triplet_vs_output"] = traffic / batch / out_trip
+        # HBM-bound stages: rows of the scattering matrix assembled from the compact sums
+        # (read once, rows written once) and the BTE (every iteration of every temperature
+        # streams its scattering matrix and the folded-out F)
+        xi_T = eng.nrows * eng.nfull * nb * es
+        if n_asm:
+            by = len(temps) * (eng.ntriplets * nb * 2 * nb * es + xi_T)
+            roof["k_xi_asm"] = dict(bytes=by, ms=ms_asm, GBps=by / (ms_asm * 1e-3) / 1e9,
+                                    frac=by / (ms_asm * 1e-3) / 1e9 / PEAK_HBM_GBPS)
+        it_mx = st_mx["bte_iterations"]
+        by_bte = sum(it_mx) * (xi_T + 3 * eng.nfull * nb * 8)
+        roof["bte"] = dict(bytes=by_bte, ms=st_mx["bte"] * 1e3, GBps=by_bte / st_mx["bte"] / 1e9,
+                           frac=by_bte / st_mx["bte"] / 1e9 / PEAK_HBM_GBPS,
+                           temperature_iterations=int(sum(it_mx)),
+                           note="whole BTE (fold-out, Xi.F, update, kappa log, checks) against the Xi + F bytes")
         res = {
             "metric": "3ph matrix elements/s + κ wall-time, mixed vs FP64 GPU @≤1e-4 rel κ err",
             "value": elements / t_mx,
@@ -235,6 +257,8 @@ def main():
             "kappa_wall_s": {"mixed": kw_mx, "fp64": kw_64, "speedup": kw_64 / kw_mx,
                              "stages_mixed": st_mx, "stages_fp64": st_64},
             "kappa_rel_err_vs_fp64": {"iterative": err, "sma": err_sma},
+            "linewidth_rel_err_vs_fp64": lw_err,
+            "end_to_end_s": e2e,
             "kappa_xx_W_mK_300K": {"mixed": float(kit_mx[2][0, 0] * KAPPA_AU_TO_SI),
                                    "fp64": float(kit_64[2][0, 0] * KAPPA_AU_TO_SI)},
             "roofline": roof,
