@@ -487,9 +487,13 @@ Status build_orbits(const Mesh &m, OrbitSet &o) {
     o = OrbitSet();
     o.q1_off.push_back(0);
     o.orb_off.push_back(0);
-    std::vector<int> imgs;
+    std::vector<int> imgs, multi, multi_len;
     for (int q1 = 0; q1 < m.nirr; ++q1) {
         const auto &lg = m.littlegroup[q1];
+        multi.clear();
+        multi_len.clear();
+        int ns = 0;
+        // orbits of one member first, then the others, each in the order of their first member
         for (int q2 = 0; q2 < np; ++q2) {
             imgs.clear();
             bool skip = false;
@@ -502,10 +506,23 @@ Status build_orbits(const Mesh &m, OrbitSet &o) {
             if (skip || imgs.empty()) continue;
             std::sort(imgs.begin(), imgs.end());
             imgs.erase(std::unique(imgs.begin(), imgs.end()), imgs.end());
-            for (int x : imgs) o.members.push_back(x);
+            if (imgs.size() == 1) {
+                o.members.push_back(imgs[0]);
+                o.orb_off.push_back((int)o.members.size());
+                ++ns;
+            } else {
+                multi.insert(multi.end(), imgs.begin(), imgs.end());
+                multi_len.push_back((int)imgs.size());
+            }
+        }
+        size_t p = 0;
+        for (int n : multi_len) {
+            o.members.insert(o.members.end(), multi.begin() + p, multi.begin() + p + n);
             o.orb_off.push_back((int)o.members.size());
+            p += n;
         }
         o.q1_off.push_back((int)o.orb_off.size() - 1);
+        o.nsingle.push_back(ns);
     }
     return st;
 }

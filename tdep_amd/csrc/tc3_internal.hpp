@@ -86,6 +86,7 @@ struct OrbitSet {
     std::vector<int> q1_off;         // nirr+1 offsets into orbit list
     std::vector<int> orb_off;        // norbit+1 offsets into members
     std::vector<int> members;
+    std::vector<int> nsingle;        // nirr: orbits of one member, the first of every q1
 };
 
 // Host-side preprocessing (tc3_host.cpp)

@@ -553,22 +553,18 @@ __global__ void __launch_bounds__(kBlock) k_xi_sym_lds(int nrows, int nb, const 
     }
 }
 
-#ifndef TC3_ASM_DEPTH
-#define TC3_ASM_DEPTH 8
-#endif
-constexpr int kAsmDepth = TC3_ASM_DEPTH;
-
 // Scattering-matrix rows from the compact per-triplet sums of the MFMA kernels, then (AVG)
 // the orbit averaging of k_xi_sym_lds.  msrc, in the member order of the orbits, holds for
 // every q' the owner 4 t + mode (the triplet t of q1 whose equivalent pairs own the entries
 // (q', b) of the rows of q1: mode 1 the q2' sums, mode 2 the q3' sums, mode 3 both, q2' =
 // q3' within one pair; 0 none, entry zero) and the frequency offsets nb q of the owning
-// role and of q3': (q', owner, offset, offset q3').  The
-// entries are formed with the arithmetic of the direct stores of k_psi (frequency of the
-// mode times 2 / omega_1, rounding to the entry type, the q3' term added to the rounded
-// q2' term), so the rows are the same bits.  Workgroup per (row, T), one band per thread.
+// role and of q3': (q', owner, offset, offset q3').  The entries are formed with the
+// arithmetic of the direct stores of k_psi (frequency of the mode times 2 / omega_1,
+// rounding to the entry type, the q3' term added to the rounded q2' term), so the rows are
+// the same bits.
 // Raw buffer access with 32-bit byte offsets from a uniform base (gfx9 descriptor word 3).
 typedef unsigned xv2u __attribute__((ext_vector_type(2)));
+typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
 __device__ inline __amdgpu_buffer_rsrc_t xrsrc(const void *p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
@@ -587,93 +583,189 @@ __device__ inline void xstore(T v, __amdgpu_buffer_rsrc_t r, unsigned off) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
 }
 
+// No LDS staging: three bands per thread (nb = 3 na, so na lanes cover a member's nb
+// entries), the orbit sums in registers.  Without AVG a lane group takes one member.  With
+// AVG the orbits of one member (ordered first among the orbits of q1; their average is
+// 0 + x) are taken the same way, and each orbit of several members by nb lanes, one band
+// each, which form the entries of every member in member order, accumulate the sums of the
+// averaging in that order and write the orbit's value to every member.
+template <typename T>
+__device__ inline void xload3(T (&v)[3], __amdgpu_buffer_rsrc_t r, unsigned off) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = xload<T>(r, off + (unsigned)sizeof(T) * j);
+}
+template <typename T>
+__device__ inline void xstore3(const T (&v)[3], __amdgpu_buffer_rsrc_t r, unsigned off) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) xstore<T>(v[j], r, off + (unsigned)sizeof(T) * j);
+}
+
+#ifndef TC3_ORB_DEPTH
+#define TC3_ORB_DEPTH 2
+#endif
+constexpr int kOrbDepth = TC3_ORB_DEPTH;
+#ifndef TC3_ORB_TEMPS
+#define TC3_ORB_TEMPS 4
+#endif
+constexpr int kOrbTemps = TC3_ORB_TEMPS;   // temperatures per workgroup
+
+// Workgroup per (row, block of kOrbTemps temperatures): the owner table and the frequencies
+// are read once for the block's temperatures, whose sums are independent loads in flight.
 template <typename XT, typename S, bool AVG>
-__global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull, const int *__restrict__ row_q,
+__global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull, int nT, const int *__restrict__ row_q,
                                                    const int *__restrict__ row_b, const double *__restrict__ om_irr,
                                                    const double *__restrict__ om_full, const int4 *__restrict__ msrc,
                                                    const S *__restrict__ comp, size_t comp_T,
-                                                   const int *__restrict__ chk_off, const int *__restrict__ chk,
-                                                   const int *__restrict__ orb_off, XT *__restrict__ xi,
-                                                   size_t xi_row, size_t xi_T, int capm) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    XT *val = (XT *)smem;
-    int *mem = (int *)(val + capm * nb);
-    int *ooff = mem + capm;
-    int *srcs = ooff + capm + 1;
-    int *w2o = srcs + capm, *w3o = w2o + capm;
-    const int r = blockIdx.x % nrows, T = blockIdx.x / nrows;
+                                                   const int *__restrict__ orb_q1, const int *__restrict__ orb_off,
+                                                   const int *__restrict__ orb_ns, XT *__restrict__ xi, size_t xi_row,
+                                                   size_t xi_T) {
+    constexpr int TB = kOrbTemps;
+    const int L = nb / 3;
+    const int gs = blockDim.x / L, g0 = threadIdx.x / L, b = 3 * (threadIdx.x % L);
+    if (g0 >= gs) return;
+    const int r = blockIdx.x % nrows, T0 = (blockIdx.x / nrows) * TB, nt = min(TB, nT - T0);
     const int q1 = row_q[r], b1 = row_b[r];
     const double iw1 = 2.0 / om_irr[(size_t)q1 * nb + b1];
     // byte offsets below 2^32 (checked by the caller)
-    const __amdgpu_buffer_rsrc_t rx = xrsrc(xi + (size_t)T * xi_T + (size_t)r * xi_row, (unsigned)(xi_row * sizeof(XT)));
-    const __amdgpu_buffer_rsrc_t rc = xrsrc(comp + (size_t)T * comp_T, (unsigned)(comp_T * sizeof(S)));
+    __amdgpu_buffer_rsrc_t rx[TB], rc[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+        const int Tt = min(T0 + t, nT - 1);
+        rx[t] = xrsrc(xi + (size_t)Tt * xi_T + (size_t)r * xi_row, (unsigned)(xi_row * sizeof(XT)));
+        rc[t] = xrsrc(comp + (size_t)Tt * comp_T, (unsigned)(comp_T * sizeof(S)));
+    }
     const __amdgpu_buffer_rsrc_t ro = xrsrc(om_full, (unsigned)((size_t)nfull * nb * sizeof(double)));
-    const int b = threadIdx.x % nb, p0 = threadIdx.x / nb, ps = blockDim.x / nb;
     const unsigned tstride = (unsigned)(2 * nb * nb * sizeof(S));
     const unsigned cbase = (unsigned)((2 * nb * b1 + b) * sizeof(S)), c3 = (unsigned)(nb * sizeof(S));
-    for (int c = chk_off[q1]; c + 1 < chk_off[q1 + 1]; ++c) {
-        const int oa = chk[c], ob = chk[c + 1];
-        const int pa = orb_off[oa], n = orb_off[ob] - pa;
-        for (int p = threadIdx.x; p < n; p += blockDim.x) {
-            const int4 m = msrc[pa + p];
-            mem[p] = m.x;
-            srcs[p] = m.y;
-            w2o[p] = m.z;
-            w3o[p] = m.w;
-        }
-        for (int o = threadIdx.x; o <= ob - oa; o += blockDim.x) ooff[o] = orb_off[oa + o] - pa;
-        __syncthreads();
-        // loads are unconditional (an entry without owner reads triplet 0 and is masked) so
-        // that kAsmDepth of them are in flight per thread
-        for (int p = p0; p < n; p += kAsmDepth * ps) {
-            S v[kAsmDepth];
-            double f[kAsmDepth];
+    // the three entries of member position m at the block's temperatures (zero without owner)
+    auto entries = [&](const int4 ms, XT (&e)[TB][3]) {
+        const int mode = ms.y & 3;
+        const unsigned tb = (unsigned)(ms.y >> 2) * tstride + cbase;
+        S v[TB][3];
+        double f[3];
 #pragma unroll
-            for (int k = 0; k < kAsmDepth; ++k) {
-                const int pk = min(p + k * ps, n - 1);
-                const int sm = srcs[pk];
-                v[k] = xload<S>(rc, (unsigned)(sm >> 2) * tstride + cbase + ((sm & 3) == 2 ? c3 : 0u));
-                f[k] = xload<double>(ro, (unsigned)(w2o[pk] + b) * 8u);
-            }
+        for (int t = 0; t < TB; ++t)
+            if (t < nt) xload3<S>(v[t], rc[t], tb + (mode == 2 ? c3 : 0u));
+        xload3<double>(f, ro, (unsigned)(ms.z + b) * 8u);
 #pragma unroll
-            for (int k = 0; k < kAsmDepth; ++k) {
-                const int pk = p + k * ps;
-                if (pk >= n) continue;
-                const int sm = srcs[pk];
-                XT e = (sm & 3) ? (XT)((double)v[k] * f[k] * iw1) : (XT)0;
-                if ((sm & 3) == 3) {
-                    // q2' = q3' within one pair: the q3' term added to the rounded q2' term
-                    const S v3 = xload<S>(rc, (unsigned)(sm >> 2) * tstride + cbase + c3);
-                    const double f3 = xload<double>(ro, (unsigned)(w3o[pk] + b) * 8u);
-                    e = (XT)((double)e + (double)v3 * f3 * iw1);
-                }
-                val[pk * nb + b] = e;
-            }
+        for (int t = 0; t < TB; ++t)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) e[t][j] = mode ? (XT)((double)v[t][j] * f[j] * iw1) : (XT)0;
+        if (mode == 3) {
+            // q2' = q3' within one pair: the q3' term added to the rounded q2' term
+            S v3[TB][3];
+            double f3[3];
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+                if (t < nt) xload3<S>(v3[t], rc[t], tb + c3);
+            xload3<double>(f3, ro, (unsigned)(ms.w + b) * 8u);
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) e[t][j] = (XT)((double)e[t][j] + (double)v3[t][j] * f3[j] * iw1);
         }
-        __syncthreads();
-        if constexpr (AVG) {
-            for (int o = oa + p0; o < ob; o += ps) {
-                const int m0 = ooff[o - oa], m1 = ooff[o - oa + 1];
-                double sp = 0, sa = 0;
-                int cnt = 0;
-                for (int m = m0; m < m1; ++m) {
-                    const double v = (double)val[m * nb + b];
-                    sa += v;
-                    if (v > 0.0) { sp += v; ++cnt; }
+    };
+    auto store = [&](const XT (&e)[TB][3], int mem) {
+#pragma unroll
+        for (int t = 0; t < TB; ++t)
+            if (t < nt) xstore3<XT>(e[t], rx[t], (unsigned)(mem * nb + b) * (unsigned)sizeof(XT));
+    };
+    // members one per lane group, kOrbDepth in flight; with AVG the orbits of one member (the
+    // first orbits of q1), whose average reduces to 0 + x
+    const int pa = orb_off[orb_q1[q1]], pb = AVG ? pa + orb_ns[q1] : orb_off[orb_q1[q1 + 1]];
+    for (int m = pa + g0; m < pb; m += kOrbDepth * gs) {
+        int4 ms[kOrbDepth];
+#pragma unroll
+        for (int k = 0; k < kOrbDepth; ++k)
+            if (m + k * gs < pb) ms[k] = msrc[m + k * gs];
+#pragma unroll
+        for (int k = 0; k < kOrbDepth; ++k) {
+            if (m + k * gs >= pb) break;
+            XT e[TB][3];
+            entries(ms[k], e);
+            if constexpr (AVG) {
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) e[t][j] = (XT)(0.0 + (double)e[t][j]);
+            }
+            store(e, ms[k].x);
+        }
+    }
+    if constexpr (AVG) {
+        // orbits of several members: nb lanes per orbit, one band each, the sums of the block's
+        // temperatures in member order
+        const int g1 = threadIdx.x / nb, gs1 = blockDim.x / nb, b0 = threadIdx.x % nb;
+        const unsigned cbase1 = (unsigned)((2 * nb * b1 + b0) * sizeof(S));
+        for (int o = orb_q1[q1] + orb_ns[q1] + g1; g1 < gs1 && o < orb_q1[q1 + 1]; o += gs1) {
+            const int m0 = orb_off[o], m1 = orb_off[o + 1];
+            double sa[TB] = {}, sp[TB] = {};
+            int cnt[TB] = {};
+            int mfirst[kOrbDepth];   // members of the first batch, for the stores
+            for (int m = m0; m < m1; m += kOrbDepth) {
+                int4 ms[kOrbDepth];
+#pragma unroll
+                for (int k = 0; k < kOrbDepth; ++k)
+                    if (m + k < m1) ms[k] = msrc[m + k];
+                S v[kOrbDepth][TB], v3[kOrbDepth][TB];
+                double f[kOrbDepth], f3[kOrbDepth];
+#pragma unroll
+                for (int k = 0; k < kOrbDepth; ++k) {
+                    if (m + k >= m1) break;
+                    const int mode = ms[k].y & 3;
+                    const unsigned tb = (unsigned)(ms[k].y >> 2) * tstride + cbase1;
+#pragma unroll
+                    for (int t = 0; t < TB; ++t)
+                        if (t < nt) v[k][t] = xload<S>(rc[t], tb + (mode == 2 ? c3 : 0u));
+                    f[k] = xload<double>(ro, (unsigned)(ms[k].z + b0) * 8u);
+                    if (mode == 3) {
+#pragma unroll
+                        for (int t = 0; t < TB; ++t)
+                            if (t < nt) v3[k][t] = xload<S>(rc[t], tb + c3);
+                        f3[k] = xload<double>(ro, (unsigned)(ms[k].w + b0) * 8u);
+                    }
                 }
-                double sum = sa;
-                if (cnt > 0) {
-                    const double avg = sp / cnt;
+#pragma unroll
+                for (int k = 0; k < kOrbDepth; ++k) {
+                    if (m + k >= m1) break;
+                    if (m == m0) mfirst[k] = ms[k].x;
+                    const int mode = ms[k].y & 3;
+#pragma unroll
+                    for (int t = 0; t < TB; ++t) {
+                        XT e = mode ? (XT)((double)v[k][t] * f[k] * iw1) : (XT)0;
+                        // q2' = q3' within one pair: the q3' term added to the rounded q2' term
+                        if (mode == 3) e = (XT)((double)e + (double)v3[k][t] * f3[k] * iw1);
+                        const double x = (double)e;
+                        sa[t] += x;
+                        if (x > 0.0) { sp[t] += x; ++cnt[t]; }
+                    }
+                }
+            }
+            XT out[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                double sum = sa[t];
+                if (cnt[t] > 0) {
+                    const double avg = sp[t] / cnt[t];
                     sum = 0;
                     for (int m = m0; m < m1; ++m) sum += avg;
                 }
-                const XT out = (XT)(sum / (m1 - m0));
-                for (int m = m0; m < m1; ++m) val[m * nb + b] = out;
+                out[t] = (XT)(sum / (m1 - m0));
             }
-            __syncthreads();
+            for (int m = m0; m < m1; m += kOrbDepth) {
+                int mm[kOrbDepth];
+#pragma unroll
+                for (int k = 0; k < kOrbDepth; ++k)
+                    if (m + k < m1) mm[k] = (m == m0) ? mfirst[k] : msrc[m + k].x;
+#pragma unroll
+                for (int k = 0; k < kOrbDepth; ++k) {
+                    if (m + k >= m1) break;
+#pragma unroll
+                    for (int t = 0; t < TB; ++t)
+                        if (t < nt) xstore<XT>(out[t], rx[t], (unsigned)(mm[k] * nb + b0) * (unsigned)sizeof(XT));
+                }
+            }
         }
-        for (int p = p0; p < n; p += ps) xstore<XT>(val[p * nb + b], rx, (unsigned)(mem[p] * nb + b) * (unsigned)sizeof(XT));
-        __syncthreads();
     }
 }
 
@@ -761,7 +853,9 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(table("orboff", (void **)&d_orboff, c->orb.orb_off.data(), sizeof(int) * c->orb.orb_off.size()));
     TRY(table("orbq1", (void **)&d_orbq1, c->orb.q1_off.data(), sizeof(int) * c->orb.q1_off.size()));
     TRY(table("members", (void **)&d_members, c->orb.members.data(), sizeof(int) * c->orb.members.size()));
-    // chunks of whole orbits of every q1 for k_xi_sym_lds and k_xi_asm: at most capm
+    int *d_orbns;
+    TRY(table("orbns", (void **)&d_orbns, c->orb.nsingle.data(), sizeof(int) * nirr));
+    // chunks of whole orbits of every q1 for k_xi_sym_lds: at most capm
     // members (kXiChunk entries, or the largest orbit)
     if (c->xchk_gen != c->gen_model) {
         std::vector<int> &chk_off = c->xchk_off, &chk = c->xchk;
@@ -1294,25 +1388,24 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     {
         const dim3 g(nrows * nT), b(kBlock), bl(nb <= kBlock ? nb * (kBlock / nb) : kBlock);
         const size_t shl = (size_t)capm * nb * c->xi_es + sizeof(int) * (2 * (size_t)capm + 1);
-        const size_t sha = (size_t)capm * nb * c->xi_es + sizeof(int) * (5 * (size_t)capm + 1);
-        if (d_comp && (nb > kBlock || sha > 96 * 1024)) { cleanup(); return c->fail(5, "scattering-matrix assembly exceeds the LDS"); }
+        if (d_comp && nb > kBlock) { cleanup(); return c->fail(5, "scattering-matrix assembly needs nb <= 256"); }
         if (d_comp) {
             // rows from the compact sums; with isotope or four-phonon terms the averaging below
             // follows
             const size_t cT = (size_t)ntrip * nb * 2 * nb;
-            if (c->xi_es == 8) {
-                auto kf = (use_iso || d_od4) ? k_xi_asm<double, double, false> : k_xi_asm<double, double, true>;
-                TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
-                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, nfull, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
-                                   (const int4 *)d_xisrc, (const double *)d_comp, cT, d_chkoff, d_chk, d_orboff,
-                                   (double *)c->d_xi, c->xi_row, xi_T, capm);
-            } else {
-                auto kf = (use_iso || d_od4) ? k_xi_asm<float, float, false> : k_xi_asm<float, float, true>;
-                TRY(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
-                hipLaunchKernelGGL(kf, g, bl, sha, st, nrows, nb, nfull, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
-                                   (const int4 *)d_xisrc, (const float *)d_comp, cT, d_chkoff, d_chk, d_orboff,
-                                   (float *)c->d_xi, c->xi_row, xi_T, capm);
-            }
+            const int L = nb / 3;
+            const dim3 bo(L * (kBlock / L)), go(nrows * ((nT + kOrbTemps - 1) / kOrbTemps));
+            const bool avg = !(use_iso || d_od4);
+            auto launch = [&](auto kf, auto *cp, auto *xp) {
+                hipLaunchKernelGGL(kf, go, bo, 0, st, nrows, nb, nfull, nT, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
+                                   (const int4 *)d_xisrc, cp, cT, d_orbq1, d_orboff, d_orbns, xp, c->xi_row, xi_T);
+            };
+            if (c->xi_es == 8)
+                launch(avg ? k_xi_asm<double, double, true> : k_xi_asm<double, double, false>, (const double *)d_comp,
+                       (double *)c->d_xi);
+            else
+                launch(avg ? k_xi_asm<float, float, true> : k_xi_asm<float, float, false>, (const float *)d_comp,
+                       (float *)c->d_xi);
             TRY(hipGetLastError());
         }
         const bool use_lds = nb <= kBlock && shl <= 80 * 1024 && std::getenv("TC3_XI_GLOBAL") == nullptr && !d_od4;
