@@ -444,13 +444,14 @@ struct BteState {
 
 // All temperatures iterate together: blockIdx.z is the temperature, every per-temperature
 // array is strided by it, and a temperature whose state says done is skipped.
+template <typename FT>
 __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full, const int *__restrict__ op_of_full,
-                          const double *__restrict__ opm, const double *__restrict__ Fn_, double *__restrict__ Fbb_,
+                          const double *__restrict__ opm, const double *__restrict__ Fn_, FT *__restrict__ Fbb_,
                           const BteState *__restrict__ st_, long sF, long sFbb) {
     const int T = blockIdx.z;
     if (st_[T].done) return;
     const double *Fn = Fn_ + T * sF;
-    double *Fbb = Fbb_ + T * sFbb;
+    FT *Fbb = Fbb_ + T * sFbb;
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (j >= (long)nfull * nb) return;
     const int q = (int)(j / nb), b = (int)(j % nb);
@@ -459,89 +460,49 @@ __global__ void k_foldout(int nfull, int nb, const int *__restrict__ irr_of_full
     const double *F = Fn + ((size_t)k * nb + b) * 3;
     const double s = iop > 0 ? 1.0 : -1.0;
     const long ncol = (long)nfull * nb;
-    for (int i = 0; i < 3; ++i) Fbb[i * ncol + j] = s * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]);
+    for (int i = 0; i < 3; ++i) Fbb[i * ncol + j] = (FT)(s * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]));
 }
 
-// buf[row][0..2] = sum_j Xi[row][j] Fbb[0..2][j]  (lo_gemv x3, kappa.f90:411-413); the
-// folded-out F is planar, [component][column].
-// Workgroup per (kRows rows, column slice): every Fbb element fetched serves kRows rows,
-// and the grid holds enough workgroups to keep HBM busy.  Partial sums per slice are
-// added in slice order by k_gemv_sum, so the result does not depend on scheduling.
+// part[slice][row][0..2] = sum over the slice's columns j of Xi[row][j] Fbb[0..2][j]
+// (lo_gemv x3, kappa.f90:411-413); the folded-out F is planar, [component][column], in FP32
+// next to FP32 Xi (each product of an FP32 Xi and an FP32 F is exact in FP64; the sums are
+// FP64) and in FP64 next to FP64 Xi.  The slices are added in slice order by k_bte_update, so
+// the result does not depend on scheduling.
 #ifndef TC3_BTE_ROWS
 #define TC3_BTE_ROWS 4
 #endif
-constexpr int kRows = TC3_BTE_ROWS, kSlices = 8;
-template <typename XT>
-__global__ void __launch_bounds__(kBlock) k_xi_gemv(int nrows, long ncol, const XT *__restrict__ xi_,
-                                                    const double *__restrict__ Fbb_, double *__restrict__ part_,
-                                                    const BteState *__restrict__ st_, size_t sXi, long sFbb,
-                                                    long sPart) {
-    const int T = blockIdx.z;
-    if (st_[T].done) return;
-    const XT *xi = xi_ + T * sXi;
-    const double *Fbb = Fbb_ + T * sFbb;
-    double *part = part_ + T * sPart;
-    __shared__ double red[3 * kRows][kBlock / 64];
-    const int r0 = blockIdx.x * kRows, sl = blockIdx.y;
-    const int nr = min(kRows, nrows - r0);
-    const long c0 = ncol * sl / kSlices, c1 = ncol * (sl + 1) / kSlices;
-    const XT *x = xi + (size_t)r0 * ncol;
-    double s[kRows][3];
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
-    for (long j = c0 + threadIdx.x; j < c1; j += blockDim.x) {
-        const double f0 = Fbb[j], f1 = Fbb[ncol + j], f2 = Fbb[2 * ncol + j];
-#pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            if (r < nr) {
-                const double v = (double)x[(size_t)r * ncol + j];
-                s[r][0] += v * f0;
-                s[r][1] += v * f1;
-                s[r][2] += v * f2;
-            }
-        }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int r = 0; r < kRows; ++r)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            double v = s[r][i];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) red[3 * r + i][wave] = v;
-        }
-    __syncthreads();
-    if (threadIdx.x < 3 * nr) {
-        double v = 0;
-        for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
-        part[((size_t)sl * nrows + r0) * 3 + threadIdx.x] = v;
-    }
-}
+#ifndef TC3_BTE_SLICES
+#define TC3_BTE_SLICES 8
+#endif
+constexpr int kRows = TC3_BTE_ROWS, kSlices = TC3_BTE_SLICES;
 
-// The same product with 16 bytes of Xi per row and thread (VW = 4 FP32 or 2 FP64 columns)
-// and the matching VW values of each F component; needs ncol % VW == 0 (rows stay aligned).
-template <typename XT, int VW>
-__global__ void __launch_bounds__(kBlock) k_xi_gemv_v(int nrows, long ncol, const XT *__restrict__ xi_,
-                                                      const double *__restrict__ Fbb_, double *__restrict__ part_,
+// Workgroup per (4 kRows rows, column slice): each wave takes kRows rows and all waves sweep
+// the same columns together, so every F value fetched from L2 serves the four waves (from
+// L1) and kRows rows each; VW columns per load (16 bytes of FP32 Xi), and the per-row sums
+// need only a wave reduction.
+template <typename XT, int VW, typename FT>
+__global__ void __launch_bounds__(kBlock) k_xi_gemv_w(int nrows, long ncol, const XT *__restrict__ xi_,
+                                                      const FT *__restrict__ Fbb_, double *__restrict__ part_,
                                                       const BteState *__restrict__ st_, size_t sXi, long sFbb,
                                                       long sPart) {
     typedef XT XV __attribute__((ext_vector_type(VW)));
-    typedef double DV __attribute__((ext_vector_type(VW)));
+    typedef FT DV __attribute__((ext_vector_type(VW)));
     const int T = blockIdx.z;
     if (st_[T].done) return;
     const XT *xi = xi_ + T * sXi;
-    const double *Fbb = Fbb_ + T * sFbb;
+    const FT *Fbb = Fbb_ + T * sFbb;
     double *part = part_ + T * sPart;
-    __shared__ double red[3 * kRows][kBlock / 64];
-    const int r0 = blockIdx.x * kRows, sl = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * (kBlock / 64) + wave) * kRows, sl = blockIdx.y;
     const int nr = min(kRows, nrows - r0);
+    if (nr <= 0) return;
     const long nv = ncol / VW;
     const long v0 = nv * sl / kSlices, v1 = nv * (sl + 1) / kSlices;
     const XT *x = xi + (size_t)r0 * ncol;
     double s[kRows][3];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
-    for (long v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+    for (long v = v0 + lane; v < v1; v += 64) {
         const long j = v * VW;
         const DV f0 = *(const DV *)(Fbb + j), f1 = *(const DV *)(Fbb + ncol + j), f2 = *(const DV *)(Fbb + 2 * ncol + j);
 #pragma unroll
@@ -551,57 +512,56 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv_v(int nrows, long ncol, cons
 #pragma unroll
                 for (int k = 0; k < VW; ++k) {
                     const double e = (double)xv[k];
-                    s[r][0] += e * f0[k];
-                    s[r][1] += e * f1[k];
-                    s[r][2] += e * f2[k];
+                    s[r][0] += e * (double)f0[k];
+                    s[r][1] += e * (double)f1[k];
+                    s[r][2] += e * (double)f2[k];
                 }
             }
         }
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < kRows; ++r)
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             double v = s[r][i];
             for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) red[3 * r + i][wave] = v;
+            s[r][i] = v;
         }
-    __syncthreads();
-    if (threadIdx.x < 3 * nr) {
+    if (lane < 3 * nr) {
         double v = 0;
-        for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
-        part[((size_t)sl * nrows + r0) * 3 + threadIdx.x] = v;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                if (lane == 3 * r + i) v = s[r][i];
+        part[((size_t)sl * nrows + r0) * 3 + lane] = v;
     }
-}
-
-__global__ void k_gemv_sum(int n3, const double *__restrict__ part_, double *__restrict__ buf_,
-                           const BteState *__restrict__ st_, long sPart, long sBuf) {
-    const int T = blockIdx.z;
-    if (st_[T].done) return;
-    const double *part = part_ + T * sPart;
-    double *buf = buf_ + T * sBuf;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n3) return;
-    double v = 0;
-    for (int sl = 0; sl < kSlices; ++sl) v += part[(size_t)sl * n3 + i];
-    buf[i] = v;
 }
 
 // Fnb = -buf/qs; degeneracy average then +F0, band by band in order (kappa.f90:415-448);
 // convergence partial and mixing (kappa.f90:456-468).  Workgroup per irreducible point;
-// the ordered degeneracy pass runs on one thread over LDS copies.
-__global__ void k_bte_update(int nb, const int *__restrict__ row_of, const double *__restrict__ buf_,
+// the ordered degeneracy pass runs over LDS copies: where the degenerate sets of q partition
+// its bands (dgpart), one thread per set takes its bands in order (the other sets' bands are
+// untouched by them, so this is the band-by-band pass), else one thread takes every band.
+// The products of the slices are added here in slice order (Xi.F of the rows of q), and the
+// new F of q is folded out to the full-grid points of its star for the next product:
+// Fbb[i][p nb + b] = s (R F(q, b))_i for p = R q (s = -1 for the operations with time reversal).
+template <typename FT>
+__global__ void k_bte_update(int nb, int nrows, const int *__restrict__ row_of, const double *__restrict__ part_,
                              const double *__restrict__ qs_, const double *__restrict__ F0_,
                              const int *__restrict__ dg_off, const int *__restrict__ dg,
-                             const int *__restrict__ fzero, double *__restrict__ Fn_, double *__restrict__ g0part_,
+                             const int *__restrict__ dgpart, const int *__restrict__ fzero,
+                             const int *__restrict__ star_off, const int *__restrict__ star,
+                             const int *__restrict__ op_of_full, const double *__restrict__ opm, long ncol,
+                             double *__restrict__ Fn_, FT *__restrict__ Fbb_, double *__restrict__ g0part_,
                              const BteState *__restrict__ st_,
-                             long sBuf, long sQs, long sF, long sG) {
+                             long sPart, long sQs, long sF, long sFbb, long sG) {
     const int T = blockIdx.z;
     if (st_[T].done) return;
     const double mix = st_[T].mix;
-    const double *buf = buf_ + T * sBuf, *qs = qs_ + T * sQs, *F0 = F0_ + T * sF;
+    const double *part = part_ + T * sPart, *qs = qs_ + T * sQs, *F0 = F0_ + T * sF;
     double *Fn = Fn_ + T * sF, *g0part = g0part_ + T * sG;
+    FT *Fbb = Fbb_ + T * sFbb;
     const int q = blockIdx.x;
     __shared__ double Fnb[128 * 3], F0s[128 * 3], g1s[128];
     __shared__ int dgo[129], dgl[128 * 8];
@@ -611,35 +571,47 @@ __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const doubl
         const int r = row_of[q * nb + b];
         const double s = qs[q * nb + b];
         for (int i = 0; i < 3; ++i) {
-            Fnb[3 * b + i] = (r < 0 || s < 1e-9) ? 0.0 : -buf[(size_t)r * 3 + i] / s;
+            double v = 0;
+            if (r >= 0)
+                for (int sl = 0; sl < kSlices; ++sl) v += part[((size_t)sl * nrows + r) * 3 + i];
+            Fnb[3 * b + i] = (r < 0 || s < 1e-9) ? 0.0 : -v / s;
             F0s[3 * b + i] = F0[((size_t)q * nb + b) * 3 + i];
         }
     }
     for (int b = threadIdx.x; b <= nb; b += blockDim.x) dgo[b] = dg_off[q * nb + b] - o_base;
     for (int x = threadIdx.x; x < ndg; x += blockDim.x) dgl[x] = dg[o_base + x];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int b1 = 0; b1 < nb; ++b1) {
-            const int o0 = dgo[b1], o1 = dgo[b1 + 1];
-            double v0[3] = {0, 0, 0};
-            for (int o = o0; o < o1; ++o)
-                for (int i = 0; i < 3; ++i) v0[i] += Fnb[3 * dgl[o] + i];
-            for (int i = 0; i < 3; ++i) v0[i] /= (double)(o1 - o0);
-            for (int o = o0; o < o1; ++o)
-                for (int i = 0; i < 3; ++i) Fnb[3 * dgl[o] + i] = v0[i];
-            for (int i = 0; i < 3; ++i) Fnb[3 * b1 + i] = F0s[3 * b1 + i] + Fnb[3 * b1 + i];
+    // the pass for band b1 with the set dgl[o0..o1)
+    auto step = [&](int b1, int o0, int o1) {
+        double v0[3] = {0, 0, 0};
+        for (int o = o0; o < o1; ++o)
+            for (int i = 0; i < 3; ++i) v0[i] += Fnb[3 * dgl[o] + i];
+        for (int i = 0; i < 3; ++i) v0[i] /= (double)(o1 - o0);
+        for (int o = o0; o < o1; ++o)
+            for (int i = 0; i < 3; ++i) Fnb[3 * dgl[o] + i] = v0[i];
+        for (int i = 0; i < 3; ++i) Fnb[3 * b1 + i] = F0s[3 * b1 + i] + Fnb[3 * b1 + i];
+    };
+    if (dgpart[q]) {
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int o0 = dgo[b], o1 = dgo[b + 1];
+            if (dgl[o0] != b) continue;   // the set's first band takes it
+            for (int o = o0; o < o1; ++o) step(dgl[o], o0, o1);
         }
+    } else if (threadIdx.x == 0) {
+        for (int b1 = 0; b1 < nb; ++b1) step(b1, dgo[b1], dgo[b1 + 1]);
     }
     __syncthreads();
     // fzero[q]: the little group of q fixes no vector (its rotations sum to zero), so F(q) = R F(q)
     // for every R in it forces F(q) = 0; what the sum over columns leaves there is rounding,
     // which single-precision scattering matrices raise above the |F|^2 > 1e-10 test below
     const bool fz = fzero[q] != 0;
+    __syncthreads();
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         double *F = Fn + ((size_t)q * nb + b) * 3;
         if (fz) {
             F[0] = F[1] = F[2] = 0.0;
             g1s[b] = 0.0;
+            F0s[3 * b] = F0s[3 * b + 1] = F0s[3 * b + 2] = 0.0;
             continue;
         }
         double g1 = 0, g2 = 0, f[3];
@@ -649,13 +621,26 @@ __global__ void k_bte_update(int nb, const int *__restrict__ row_of, const doubl
             g2 += f[i] * f[i];
         }
         g1s[b] = (g2 > 1e-10) ? g1 / g2 : 0.0;
-        for (int i = 0; i < 3; ++i) F[i] = f[i] * (1.0 - mix) + mix * Fnb[3 * b + i];
+        for (int i = 0; i < 3; ++i) {
+            F[i] = f[i] * (1.0 - mix) + mix * Fnb[3 * b + i];
+            F0s[3 * b + i] = F[i];   // the new F, for the fold-out below
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double g0 = 0;
         for (int b = 0; b < nb; ++b) g0 += g1s[b];
         g0part[q] = g0;
+    }
+    const int p0 = star_off[q], np = star_off[q + 1] - p0;
+    for (int x = threadIdx.x; x < np * nb; x += blockDim.x) {
+        const int p = star[p0 + x / nb], b = x % nb, iop = op_of_full[p];
+        const double *m = opm + 9 * ((iop > 0 ? iop : -iop) - 1);
+        const double sg = iop > 0 ? 1.0 : -1.0;
+        const double *F = F0s + 3 * b;
+        const long j = (long)p * nb + b;
+        for (int i = 0; i < 3; ++i)
+            Fbb[i * ncol + j] = (FT)(sg * (m[3 * i] * F[0] + m[3 * i + 1] * F[1] + m[3 * i + 2] * F[2]));
     }
 }
 
@@ -834,34 +819,50 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         used += need;
         return hipSuccess;
     };
-    double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_buf, *d_opm, *d_g0, *d_cvw, *d_vel, *d_klog;
-    int *d_opf, *d_rowof, *d_dgoff, *d_dg;
+    double *d_Fn, *d_F0, *d_qs, *d_Fbb, *d_opm, *d_g0, *d_cvw, *d_vel, *d_klog;
+    int *d_opf, *d_rowof, *d_dgoff, *d_dg, *d_dgp, *d_star_off, *d_star;
     std::vector<double> opm(9 * nsym);
     for (int o = 0; o < nsym; ++o) std::memcpy(&opm[9 * o], c->ops[o].m, sizeof(double) * 9);
-    std::vector<int> dgoff(qb + 1, 0), dg;
+    std::vector<int> dgoff(qb + 1, 0), dg, dgpart(std::max(nirr, 1), 1);
     for (int q = 0; q < nirr; ++q) {
         for (int b = 0; b < nb; ++b) {
             for (int x : c->degen[q][b]) dg.push_back(x);
             dgoff[(size_t)q * nb + b + 1] = (int)dg.size();
+            // the sets partition the bands of q when every member lists the same set (the lists
+            // are ascending); then k_bte_update takes the sets in parallel
+            const auto &set = c->degen[q][b];
+            if (set.empty() || std::find(set.begin(), set.end(), b) == set.end()) dgpart[q] = 0;
+            for (int x : set)
+                if (c->degen[q][x] != set) dgpart[q] = 0;
         }
         // k_bte_update holds one point's degenerate-set lists in LDS (1024 entries)
         if (dgoff[(size_t)q * nb + nb] - dgoff[(size_t)q * nb] > 1024)
             return c->fail(1, "degenerate sets of one q-point exceed the BTE kernel's capacity");
     }
     const int mi = std::max(maxiter, 1);
-    const long sF = (long)qb * 3, sQ = (long)qb, sFbb = (long)nfull * nb * 3, sBuf = 3l * std::max(nrows, 1),
-               sPart = sBuf * kSlices, sG = nirr, sScf = mi, sLog = 9l * kKapSlices * mi;
+    const long sF = (long)qb * 3, sQ = (long)qb, sFbb = (long)nfull * nb * 3,
+               sPart = 3l * std::max(nrows, 1) * kSlices, sG = nirr, sScf = mi, sLog = 9l * kKapSlices * mi;
+    // the full-grid points of every irreducible point's star, for the fold-out in k_bte_update
+    std::vector<int> star_off(nirr + 1, 0), star(std::max(nfull, 1));
+    for (int p = 0; p < nfull; ++p) ++star_off[c->mesh.irr_of_full[p] + 1];
+    for (int q = 0; q < nirr; ++q) star_off[q + 1] += star_off[q];
+    {
+        std::vector<int> fill(star_off.begin(), star_off.end() - 1);
+        for (int p = 0; p < nfull; ++p) star[fill[c->mesh.irr_of_full[p]]++] = p;
+    }
     TRY(dalloc((void **)&d_Fn, sizeof(double) * sF * nT));
     TRY(dalloc((void **)&d_F0, sizeof(double) * sF * nT));
     TRY(dalloc((void **)&d_qs, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_Fbb, sizeof(double) * sFbb * nT));
-    TRY(dalloc((void **)&d_buf, sizeof(double) * sBuf * nT));
+    TRY(dalloc((void **)&d_star_off, sizeof(int) * (nirr + 1)));
+    TRY(dalloc((void **)&d_star, sizeof(int) * star.size()));
     TRY(dalloc((void **)&d_opm, sizeof(double) * 9 * nsym));
     TRY(dalloc((void **)&d_g0, sizeof(double) * sG * nT));
     TRY(dalloc((void **)&d_opf, sizeof(int) * nfull));
     TRY(dalloc((void **)&d_rowof, sizeof(int) * qb));
     TRY(dalloc((void **)&d_dgoff, sizeof(int) * (qb + 1)));
     TRY(dalloc((void **)&d_dg, sizeof(int) * dg.size()));
+    TRY(dalloc((void **)&d_dgp, sizeof(int) * dgpart.size()));
     int *d_fz;
     {
         // points where symmetry fixes no vector, so that F vanishes: F(q) = R F(q) for R q = q
@@ -918,6 +919,9 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(hipMemcpyAsync(d_rowof, c->row_of.data(), sizeof(int) * qb, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_dgoff, dgoff.data(), sizeof(int) * (qb + 1), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_dg, dg.data(), sizeof(int) * dg.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_dgp, dgpart.data(), sizeof(int) * dgpart.size(), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_star_off, star_off.data(), sizeof(int) * (nirr + 1), hipMemcpyHostToDevice, st));
+    TRY(hipMemcpyAsync(d_star, star.data(), sizeof(int) * star.size(), hipMemcpyHostToDevice, st));
     double t_bte = 0, t_od = 0;
     std::vector<double> od_V;   // coherence velocities, formed on the first temperature that needs them
     // per temperature on the host: SMA kappa, off-diagonal term, the first log record, the
@@ -967,32 +971,43 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         TRY(hipMemcpyAsync(d_qs, c->qs.data(), sizeof(double) * qb * nT, hipMemcpyHostToDevice, st));
         TRY(hipMemcpyAsync(d_st, hs.data(), sizeof(BteState) * nT, hipMemcpyHostToDevice, st));
         const size_t sXi = c->xi_row * (size_t)nrows;
-        // one iteration for every temperature still running: fold out, Xi.F, update, kappa
+        const bool f32 = c->xi_es == 4;   // FP32 folded-out F next to FP32 Xi
+        const long nj = (long)nfull * nb;
+        // F0 folded out for the first product; each update folds out the F it makes
+        if (f32)
+            hipLaunchKernelGGL(k_foldout<float>, dim3((unsigned)((nj + 255) / 256), 1, nT), dim3(256), 0, st, nfull, nb,
+                               c->d_irr_of_full, d_opf, d_opm, d_Fn, (float *)d_Fbb, d_st, sF, sFbb);
+        else
+            hipLaunchKernelGGL(k_foldout<double>, dim3((unsigned)((nj + 255) / 256), 1, nT), dim3(256), 0, st, nfull,
+                               nb, c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb, d_st, sF, sFbb);
+        TRY(hipGetLastError());
+        // one iteration for every temperature still running: Xi.F, update and fold-out, kappa
         // log, convergence test (kappa.f90:395-489)
         auto iteration = [&]() {
-            const long nj = (long)nfull * nb;
-            hipLaunchKernelGGL(k_foldout, dim3((unsigned)((nj + 255) / 256), 1, nT), dim3(256), 0, st, nfull, nb,
-                               c->d_irr_of_full, d_opf, d_opm, d_Fn, d_Fbb, d_st, sF, sFbb);
             if (nrows > 0) {
-                const dim3 gg((nrows + kRows - 1) / kRows, kSlices, nT);
+                const dim3 gw((nrows + 4 * kRows - 1) / (4 * kRows), kSlices, nT);
                 const long ncol = (long)c->xi_row;
-                if (c->xi_es == 8 && ncol % 2 == 0)
-                    hipLaunchKernelGGL((k_xi_gemv_v<double, 2>), gg, dim3(kBlock), 0, st, nrows, ncol,
+                if (!f32 && ncol % 2 == 0)
+                    hipLaunchKernelGGL((k_xi_gemv_w<double, 2, double>), gw, dim3(kBlock), 0, st, nrows, ncol,
                                        (const double *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
-                else if (c->xi_es == 8)
-                    hipLaunchKernelGGL(k_xi_gemv<double>, gg, dim3(kBlock), 0, st, nrows, ncol,
+                else if (!f32)
+                    hipLaunchKernelGGL((k_xi_gemv_w<double, 1, double>), gw, dim3(kBlock), 0, st, nrows, ncol,
                                        (const double *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
                 else if (ncol % 4 == 0)
-                    hipLaunchKernelGGL((k_xi_gemv_v<float, 4>), gg, dim3(kBlock), 0, st, nrows, ncol,
-                                       (const float *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
+                    hipLaunchKernelGGL((k_xi_gemv_w<float, 4, float>), gw, dim3(kBlock), 0, st, nrows, ncol,
+                                       (const float *)c->d_xi, (const float *)d_Fbb, d_part, d_st, sXi, sFbb, sPart);
                 else
-                    hipLaunchKernelGGL(k_xi_gemv<float>, gg, dim3(kBlock), 0, st, nrows, ncol,
-                                       (const float *)c->d_xi, d_Fbb, d_part, d_st, sXi, sFbb, sPart);
-                hipLaunchKernelGGL(k_gemv_sum, dim3((3 * nrows + 255) / 256, 1, nT), dim3(256), 0, st, 3 * nrows,
-                                   d_part, d_buf, d_st, sPart, sBuf);
+                    hipLaunchKernelGGL((k_xi_gemv_w<float, 1, float>), gw, dim3(kBlock), 0, st, nrows, ncol,
+                                       (const float *)c->d_xi, (const float *)d_Fbb, d_part, d_st, sXi, sFbb, sPart);
             }
-            hipLaunchKernelGGL(k_bte_update, dim3(nirr, 1, nT), dim3(64), 0, st, nb, d_rowof, d_buf, d_qs, d_F0,
-                               d_dgoff, d_dg, d_fz, d_Fn, d_g0, d_st, sBuf, sQ, sF, sG);
+            if (f32)
+                hipLaunchKernelGGL(k_bte_update<float>, dim3(nirr, 1, nT), dim3(64), 0, st, nb, nrows, d_rowof, d_part,
+                                   d_qs, d_F0, d_dgoff, d_dg, d_dgp, d_fz, d_star_off, d_star, d_opf, d_opm, nj, d_Fn,
+                                   (float *)d_Fbb, d_g0, d_st, sPart, sQ, sF, sFbb, sG);
+            else
+                hipLaunchKernelGGL(k_bte_update<double>, dim3(nirr, 1, nT), dim3(64), 0, st, nb, nrows, d_rowof, d_part,
+                                   d_qs, d_F0, d_dgoff, d_dg, d_dgp, d_fz, d_star_off, d_star, d_opf, d_opm, nj, d_Fn,
+                                   d_Fbb, d_g0, d_st, sPart, sQ, sF, sFbb, sG);
             hipLaunchKernelGGL(k_kappa_sum, dim3(kKapSlices, 1, nT), dim3(256), 0, st, (int)qb, nsym, d_cvw, d_Fn,
                                d_vel, d_opm, d_klog, d_st, sQ, sF, sLog);
             hipLaunchKernelGGL(k_bte_check, dim3(1, 1, nT), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st, sG, sScf);
@@ -1005,11 +1020,12 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         const std::vector<long> key = {(long)(uintptr_t)c->d_xi, (long)c->xi_es, (long)c->xi_row, nrows, nfull, nb,
                                        nirr, nT, nsym, (long)(uintptr_t)c->d_irr_of_full, (long)(uintptr_t)d_Fn,
                                        (long)(uintptr_t)d_F0, (long)(uintptr_t)d_qs, (long)(uintptr_t)d_Fbb,
-                                       (long)(uintptr_t)d_buf, (long)(uintptr_t)d_opm, (long)(uintptr_t)d_g0,
-                                       (long)(uintptr_t)d_opf, (long)(uintptr_t)d_rowof, (long)(uintptr_t)d_dgoff,
+                                       (long)(uintptr_t)d_star_off, (long)(uintptr_t)d_star,
+                                       (long)(uintptr_t)d_opm, (long)(uintptr_t)d_g0,
+                                       (long)(uintptr_t)d_opf, (long)(uintptr_t)d_rowof, (long)(uintptr_t)d_dgoff, (long)(uintptr_t)d_dgp,
                                        (long)(uintptr_t)d_dg, (long)(uintptr_t)d_cvw, (long)(uintptr_t)d_vel,
                                        (long)(uintptr_t)d_klog, (long)(uintptr_t)d_part, (long)(uintptr_t)d_scf,
-                                       (long)(uintptr_t)d_st, sF, sFbb, sBuf, sPart, sG, sScf, sLog};
+                                       (long)(uintptr_t)d_st, sF, sFbb, sPart, sG, sScf, sLog};
         if (!scratch && c->bte_exec && c->bte_key != key) {
             (void)hipGraphExecDestroy(c->bte_exec);
             c->bte_exec = nullptr;
