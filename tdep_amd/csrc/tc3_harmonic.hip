@@ -92,8 +92,12 @@ __global__ void k_dynmat(int na, const double *__restrict__ qv, const int *__res
 // Parallel cyclic Jacobi for one Hermitian matrix per workgroup.  Round-robin pairing
 // (circle method) gives nb/2 disjoint (p,q) rotations per round; all rotations of a
 // round are applied to the columns, then to the rows, then to V.
+#ifndef TC3_JAC_BLOCK
+#define TC3_JAC_BLOCK 1024
+#endif
+constexpr int kJacBlock = TC3_JAC_BLOCK;   // threads per matrix
 template <bool USE_LDS>
-__global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, double *__restrict__ lam,
+__global__ void __launch_bounds__(kJacBlock) k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, double *__restrict__ lam,
                          int maxsweep, int *__restrict__ status) {
     constexpr bool use_lds = USE_LDS;
     const int q = blockIdx.x;
@@ -107,7 +111,7 @@ __global__ void k_jacobi(int nb, cd *__restrict__ Aall, cd *__restrict__ Vall, d
     const int half = mpad / 2;
     __shared__ int sp[128], sq[128];
     __shared__ double sc[128], ss[128], sphx[128], sphy[128];
-    __shared__ double red[kBlock];
+    __shared__ double red[2 * (kJacBlock / 64)];
     __shared__ int done;
     for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) V[i] = {(i / nb) == (i % nb) ? 1.0 : 0.0, 0.0};
     __syncthreads();
@@ -280,51 +284,49 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
     __shared__ double lam[128], om[128], hf[3][128];
     __shared__ int ord[128], grp[129];
     __shared__ int ngrp;
-    // work arrays of the serial (thread 0) sections
-    __shared__ int s_idx[128], s_so[128], s_sub[128], s_first[128];
-    __shared__ double s_tmp[128], s_ev[128];
+    // work arrays of the degenerate-group and subspace passes
+    __shared__ int s_so[128], s_first[128];
+    __shared__ double s_ev[128];
     // eigenvectors in LDS when they fit (use_lds), else in the output array
     extern __shared__ cd psm[];
     const cd *V = Vin + (size_t)q * nb * nb;
     cd *Eg = egv + (size_t)q * nb * nb;
     cd *E = use_lds ? psm : Eg;
     cd *scr = scratch + (size_t)q * (2 * nb * nb + nb);
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < nb; ++i) lam[i] = lam_in[(size_t)q * nb + i];
+    // sorts as ranks (ties by index, as a stable sort): the three smallest |lambda| at Gamma
+    // set to zero, then the modes ordered by signed frequency
+    __shared__ double lam0[128];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) lam0[i] = lam_in[(size_t)q * nb + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        double l = lam0[i];
         if (at_gamma[q]) {
-            // three smallest |lambda| (stable) set to zero
-            int *idx = s_idx;
-            for (int i = 0; i < nb; ++i) idx[i] = i;
-            for (int i = 1; i < nb; ++i) {
-                int o = idx[i], j = i - 1;
-                while (j >= 0 && fabs(lam[idx[j]]) > fabs(lam[o])) { idx[j + 1] = idx[j]; --j; }
-                idx[j + 1] = o;
-            }
-            for (int i = 0; i < 3 && i < nb; ++i) lam[idx[i]] = 0.0;
+            int rk = 0;
+            for (int j = 0; j < nb; ++j) rk += (fabs(lam0[j]) < fabs(l) || (fabs(lam0[j]) == fabs(l) && j < i)) ? 1 : 0;
+            if (rk < 3) l = 0.0;
         }
-        for (int i = 0; i < nb; ++i) {
-            const double l = lam[i];
-            om[i] = sqrt(fabs(l)) * (l >= 0 ? 1.0 : -1.0);
-            ord[i] = i;
-        }
-        for (int i = 1; i < nb; ++i) {
-            int o = ord[i], j = i - 1;
-            while (j >= 0 && om[ord[j]] > om[o]) { ord[j + 1] = ord[j]; --j; }
-            ord[j + 1] = o;
-        }
+        lam[i] = l;
+        om[i] = sqrt(fabs(l)) * (l >= 0 ? 1.0 : -1.0);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        int rk = 0;
+        for (int j = 0; j < nb; ++j) rk += (om[j] < om[i] || (om[j] == om[i] && j < i)) ? 1 : 0;
+        ord[rk] = i;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) {
         const int r = i / nb, c = i % nb;
         E[i] = V[(size_t)r * nb + ord[c]];
     }
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) lam0[i] = lam[ord[i]];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        lam[i] = lam0[i];
+        om[i] = sqrt(fabs(lam0[i])) * (lam0[i] >= 0 ? 1.0 : -1.0);
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double *tmp = s_tmp;
-        for (int i = 0; i < nb; ++i) tmp[i] = lam[ord[i]];
-        for (int i = 0; i < nb; ++i) {
-            lam[i] = tmp[i];
-            om[i] = sqrt(fabs(tmp[i])) * (tmp[i] >= 0 ? 1.0 : -1.0);
-        }
         // groups of consecutive modes closer than lo_freqtol
         int n = 0;
         grp[0] = 0;
@@ -345,39 +347,38 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
         phw[r] = {amp * cs, amp * sn};
     }
     __syncthreads();
-    // canonical gauge inside degenerate groups (serial; groups are small)
-    if (threadIdx.x == 0) {
-        for (int g = 0; g < ngrp; ++g) {
-            const int i0 = grp[g], k = grp[g + 1] - grp[g];
-            if (k < 2) continue;
-            cd *M = scr;
-            cd *W = scr + nb * nb;
-            double *ev = s_ev;
-            int *so = s_so;
-            for (int a = 0; a < k; ++a)
-                for (int b = 0; b < k; ++b) {
-                    cd s = {0, 0};
-                    for (int r = 0; r < nb; ++r)
-                        s = cadd(s, cscale(cmulc(E[(size_t)r * nb + i0 + a], E[(size_t)r * nb + i0 + b]), prw[r]));
-                    M[a * k + b] = s;
-                }
-            for (int a = 0; a < k; ++a)
-                for (int b = a + 1; b < k; ++b) {
-                    cd x = M[a * k + b], y = cconj(M[b * k + a]);
-                    M[a * k + b] = cscale(cadd(x, y), 0.5);
-                    M[b * k + a] = cconj(M[a * k + b]);
-                }
-            for (int a = 0; a < k; ++a) M[a * k + a].y = 0.0;
-            small_eigh(k, M, W, ev, so);
-            cd *tmp = scr + 2 * nb * nb;
-            for (int r = 0; r < nb; ++r) {
-                for (int b = 0; b < k; ++b) {
-                    cd s = {0, 0};
-                    for (int a = 0; a < k; ++a) s = cadd(s, cmul(E[(size_t)r * nb + i0 + a], W[a * k + so[b]]));
-                    tmp[b] = s;
-                }
-                for (int b = 0; b < k; ++b) E[(size_t)r * nb + i0 + b] = tmp[b];
+    // canonical gauge inside degenerate groups: one thread per group (the groups own disjoint
+    // columns of E and disjoint parts of the scratch)
+    for (int g = threadIdx.x; g < ngrp; g += blockDim.x) {
+        const int i0 = grp[g], k = grp[g + 1] - grp[g];
+        if (k < 2) continue;
+        cd *M = scr + (size_t)i0 * nb;
+        cd *W = scr + (size_t)nb * nb + (size_t)i0 * nb;
+        double *ev = s_ev + i0;
+        int *so = s_so + i0;
+        for (int a = 0; a < k; ++a)
+            for (int b = 0; b < k; ++b) {
+                cd s = {0, 0};
+                for (int r = 0; r < nb; ++r)
+                    s = cadd(s, cscale(cmulc(E[(size_t)r * nb + i0 + a], E[(size_t)r * nb + i0 + b]), prw[r]));
+                M[a * k + b] = s;
             }
+        for (int a = 0; a < k; ++a)
+            for (int b = a + 1; b < k; ++b) {
+                cd x = M[a * k + b], y = cconj(M[b * k + a]);
+                M[a * k + b] = cscale(cadd(x, y), 0.5);
+                M[b * k + a] = cconj(M[a * k + b]);
+            }
+        for (int a = 0; a < k; ++a) M[a * k + a].y = 0.0;
+        small_eigh(k, M, W, ev, so);
+        cd *tmp = scr + 2 * (size_t)nb * nb + i0;
+        for (int r = 0; r < nb; ++r) {
+            for (int b = 0; b < k; ++b) {
+                cd s = {0, 0};
+                for (int a = 0; a < k; ++a) s = cadd(s, cmul(E[(size_t)r * nb + i0 + a], W[a * k + so[b]]));
+                tmp[b] = s;
+            }
+            for (int b = 0; b < k; ++b) E[(size_t)r * nb + i0 + b] = tmp[b];
         }
     }
     __syncthreads();
@@ -432,34 +433,36 @@ __global__ void k_post(int nb, const double *__restrict__ lam_in, const cd *__re
     if (use_lds)
         for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) Eg[i] = E[i];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // return_degenerate_subspaces grouping (…_dynamicalmatrix.f90:911-980)
-        int *sub = s_sub;
-        int nsub = 0;
-        int *first = s_first;
-        for (int i = 0; i < nb; ++i) {
-            int s = -1;
-            for (int j = 0; j < i && s < 0; ++j)
-                if (fabs(om[j] - om[i]) < 1e-9) s = sub[j];
-            if (s < 0) { s = nsub++; first[s] = i; }
-            sub[i] = s;
-        }
-        for (int s = 0; s < nsub; ++s) {
-            double m0 = 0, m1 = 0, m2 = 0;
-            int cnt = 0;
-            for (int i = 0; i < nb; ++i)
-                if (sub[i] == s) { m0 += hf[0][i]; m1 += hf[1][i]; m2 += hf[2][i]; ++cnt; }
-            for (int i = 0; i < nb; ++i)
-                if (sub[i] == s) {
-                    double v[3] = {m0 / cnt, m1 / cnt, m2 / cnt};
-                    for (int al = 0; al < 3; ++al) {
-                        double x = (om[i] > 1e-9) ? v[al] / (2.0 * om[i]) : 0.0;
-                        vel[((size_t)q * nb + i) * 3 + al] = chop_dev(x, veltol);
-                    }
-                }
-        }
-        for (int i = 0; i < nb; ++i) omega[(size_t)q * nb + i] = om[i];
+    // return_degenerate_subspaces grouping (…_dynamicalmatrix.f90:911-980): mode i joins the
+    // subspace of the first earlier mode within lo_freqtol, else starts one (s_first: that
+    // earlier mode, then the subspace's first mode)
+    int *first = s_first;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        int j = 0;
+        while (j < i && !(fabs(om[j] - om[i]) < 1e-9)) ++j;
+        first[i] = j < i ? j : -1;
     }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nb; ++i) first[i] = first[i] < 0 ? i : first[first[i]];
+    __syncthreads();
+    // velocities averaged over each subspace, its modes summed in order
+    for (int s0 = threadIdx.x; s0 < nb; s0 += blockDim.x) {
+        if (first[s0] != s0) continue;
+        double m0 = 0, m1 = 0, m2 = 0;
+        int cnt = 0;
+        for (int i = s0; i < nb; ++i)
+            if (first[i] == s0) { m0 += hf[0][i]; m1 += hf[1][i]; m2 += hf[2][i]; ++cnt; }
+        for (int i = s0; i < nb; ++i)
+            if (first[i] == s0) {
+                double v[3] = {m0 / cnt, m1 / cnt, m2 / cnt};
+                for (int al = 0; al < 3; ++al) {
+                    double x = (om[i] > 1e-9) ? v[al] / (2.0 * om[i]) : 0.0;
+                    vel[((size_t)q * nb + i) * 3 + al] = chop_dev(x, veltol);
+                }
+            }
+    }
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) omega[(size_t)q * nb + i] = om[i];
 }
 
 // Full-grid eigenvectors e_full = Gamma(R) e_irr (conjugated under time reversal),
@@ -609,9 +612,9 @@ int run_harmonic(tc3_ctx *c) {
     const int jac_use = jac_lds <= 144 * 1024, post_use = post_lds <= 96 * 1024;
     if (jac_use) {
         TRY(hipFuncSetAttribute((const void *)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jac_lds));
-        hipLaunchKernelGGL(k_jacobi<true>, dim3(nirr), dim3(kBlock), jac_lds, st, nb, d_D, d_V, d_lam, 100, d_stat);
+        hipLaunchKernelGGL(k_jacobi<true>, dim3(nirr), dim3(kJacBlock), jac_lds, st, nb, d_D, d_V, d_lam, 100, d_stat);
     } else {
-        hipLaunchKernelGGL(k_jacobi<false>, dim3(nirr), dim3(kBlock), 0, st, nb, d_D, d_V, d_lam, 100, d_stat);
+        hipLaunchKernelGGL(k_jacobi<false>, dim3(nirr), dim3(kJacBlock), 0, st, nb, d_D, d_V, d_lam, 100, d_stat);
     }
     TRY(hipGetLastError());
     const double veltol = 1e-9 / kGroupvelAuToMs;
