@@ -690,6 +690,84 @@ __global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const do
 // scfcheck = mean over modes of |Fn - Fnb|^2 / |Fn|^2 (partials per point added in order),
 // stop when the last three add up to less than tol; after iteration 15 the mixing factor
 // shrinks by 2% per iteration while above 0.5.
+// lo_harmonic_oscillator_cv (gottochblandat_physics.f90:331-367), as cv_host
+__device__ double cv_dev(double T, double om) {
+    const double x = om / T / kKbHartree;
+    if (x > 200.0) return 0.0;
+    if (x > 21.0) return kKbHartree * x * x * exp(-x);
+    if (x < 2e-2) return kKbHartree * (1.0 - x * x / 12.0 + x * x * x * x / 240.0);
+    const double ex = exp(x);
+    return kKbHartree * x * x * ex / ((ex - 1) * (ex - 1));
+}
+
+// Heat capacity of every mode and temperature, and cv w_q / V, the weight of the kappa log
+// (zero below lo_freqtol or at T = 0).
+__global__ void k_cv(int qb, int nb, int classical, const double *__restrict__ temps, const double *__restrict__ om,
+                     const double *__restrict__ wq, double volume, double *__restrict__ cvm, double *__restrict__ cvw,
+                     long sQ) {
+    const int T = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= qb) return;
+    const double tk = temps[T];
+    double cv = 0, w = 0;
+    if (!(om[i] < kFreqTol) && !(tk < kTemperatureTol)) {
+        cv = classical ? kKbHartree : cv_dev(tk, om[i]);
+        w = cv * wq[i / nb] / volume;
+    }
+    cvm[T * sQ + i] = cv;
+    cvw[T * sQ + i] = w;
+}
+
+// get_kappa of the final F (kappa.f90:52-70) on the device: per mode km[x] = (cv / V) sum_ba
+// S[9 x + 3 b + a] F_a v_b (S the operation average of op_average_map), then per temperature
+// kappa[x] = sum over the modes in order of km[x] w_q.  Products and sums unfused, in the
+// order of the host restatement get_kappa, so the results are its bits.
+__global__ void k_kappa_mode(int qb, const double *__restrict__ S, const double *__restrict__ om,
+                             const double *__restrict__ cv_, double volume, const double *__restrict__ Fn_,
+                             const double *__restrict__ vel, double *__restrict__ km_, long sQ, long sF) {
+    const int T = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= qb) return;
+    double *km = km_ + (size_t)T * qb * 9 + (size_t)i * 9;
+    if (om[i] < kFreqTol) {
+        for (int x = 0; x < 9; ++x) km[x] = 0.0;
+        return;
+    }
+    const double f = __ddiv_rn(cv_[T * sQ + i], volume);
+    const double *F = Fn_ + T * sF + (size_t)i * 3, *v = vel + (size_t)i * 3;
+    for (int x = 0; x < 9; ++x) {
+        double sx = 0;
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) sx = __dadd_rn(sx, __dmul_rn(__dmul_rn(S[9 * x + 3 * b + a], F[a]), v[b]));
+        km[x] = __dmul_rn(f, sx);
+    }
+}
+
+// The sum in mode order: chunks of 256 modes staged in LDS (coalesced loads), then summed by
+// one thread per component.
+__global__ void __launch_bounds__(256) k_kappa_total(int nb, int qb, const double *__restrict__ om,
+                                                     const double *__restrict__ km_, const double *__restrict__ wq,
+                                                     double *__restrict__ kt) {
+    const int T = blockIdx.z, x = threadIdx.x;
+    const double *km = km_ + (size_t)T * qb * 9;
+    __shared__ double ck[256 * 9], cw[256];
+    double k = 0;
+    for (int i0 = 0; i0 < qb; i0 += 256) {
+        const int n = min(256, qb - i0);
+        for (int j = threadIdx.x; j < 9 * n; j += blockDim.x) ck[j] = km[(size_t)i0 * 9 + j];
+        if (threadIdx.x < n) {
+            const int i = i0 + threadIdx.x;
+            cw[threadIdx.x] = (om[i] < kFreqTol) ? -1.0 : wq[i / nb];
+        }
+        __syncthreads();
+        if (x < 9)
+            for (int j = 0; j < n; ++j)
+                if (cw[j] >= 0.0) k = __dadd_rn(k, __dmul_rn(ck[9 * j + x], cw[j]));
+        __syncthreads();
+    }
+    if (x < 9) kt[9 * T + x] = k;
+}
+
 __global__ void k_bte_check(int nirr, int nb, const double *__restrict__ g0part_, double *__restrict__ scf_,
                             BteState *__restrict__ st_, long sG, long sScf) {
     BteState *st = st_ + blockIdx.z;
@@ -854,6 +932,14 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(dalloc((void **)&d_F0, sizeof(double) * sF * nT));
     TRY(dalloc((void **)&d_qs, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_Fbb, sizeof(double) * sFbb * nT));
+    double *d_S, *d_cvm, *d_wq, *d_kmode, *d_kt, *d_ks, *d_temps;
+    TRY(dalloc((void **)&d_ks, sizeof(double) * 9 * nT));
+    TRY(dalloc((void **)&d_temps, sizeof(double) * nT));
+    TRY(dalloc((void **)&d_S, sizeof(double) * 81));
+    TRY(dalloc((void **)&d_cvm, sizeof(double) * sQ * nT));
+    TRY(dalloc((void **)&d_wq, sizeof(double) * std::max(nirr, 1)));
+    TRY(dalloc((void **)&d_kmode, sizeof(double) * qb * 9 * nT));
+    TRY(dalloc((void **)&d_kt, sizeof(double) * 9 * nT));
     TRY(dalloc((void **)&d_star_off, sizeof(int) * (nirr + 1)));
     TRY(dalloc((void **)&d_star, sizeof(int) * star.size()));
     TRY(dalloc((void **)&d_opm, sizeof(double) * 9 * nsym));
@@ -924,52 +1010,76 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(hipMemcpyAsync(d_star, star.data(), sizeof(int) * star.size(), hipMemcpyHostToDevice, st));
     double t_bte = 0, t_od = 0;
     std::vector<double> od_V;   // coherence velocities, formed on the first temperature that needs them
-    // per temperature on the host: SMA kappa, off-diagonal term, the first log record, the
-    // weights of the kappa log and the initial iteration state
+    // per temperature on the host: the off-diagonal term and the initial iteration state; the
+    // heat capacities, the SMA kappa (get_kappa of F0) and the kappa of the final F come from
+    // the device
     std::vector<BteState> hs(nT);
-    std::vector<double> cvw(qb * nT, 0.0);
     for (int iT = 0; iT < nT; ++iT) {
         const double T = c->temps[iT];
-        const double *F0 = &c->F0[(size_t)iT * qb * 3];
-        double *Fn = &c->Fn[(size_t)iT * qb * 3];
-        double *ks = &c->kappa_sma[9 * iT], *ko = &c->kappa_od[9 * iT];
-        std::memcpy(Fn, F0, sizeof(double) * qb * 3);
-        get_kappa(c, F0, T, c->classical, ks, nullptr);
         hs[iT] = {0, 1, maxiter, 0, 0.95, tol};
-        if (T < kTemperatureTol) {
-            std::fill(ks, ks + 9, 0.0);
-            continue;
-        }
+        if (T < kTemperatureTol) continue;
         if (offdiag) {
             const auto a = std::chrono::steady_clock::now();
             if (od_V.empty()) {
                 const int r = od_velocities(c, od_V);
                 if (r) return r;
             }
-            kappa_offdiag(c, iT, T, c->classical, od_V.data(), ko);
+            kappa_offdiag(c, iT, T, c->classical, od_V.data(), &c->kappa_od[9 * iT]);
             t_od += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
         }
         if (maxiter <= 0) continue;
         hs[iT].done = 0;
-        for (int q = 0; q < nirr; ++q)
-            for (int b = 0; b < nb; ++b) {
-                const double om = c->om_irr[(size_t)q * nb + b];
-                if (om < kFreqTol) continue;
-                const double cv = c->classical ? kKbHartree : cv_host(T, om);
-                cvw[(size_t)iT * qb + (size_t)q * nb + b] = cv * c->mesh.weight_irr[q] / c->cr.volume;
-            }
-        // the first record is the kappa of F0 (kappa.f90:377), i.e. the SMA value
-        c->kappa_log[iT].insert(c->kappa_log[iT].end(), {ks[0], ks[4], ks[8], ks[3], ks[6], ks[7], 0.0});
     }
     const auto a = std::chrono::steady_clock::now();
     bool any = false;
     for (int iT = 0; iT < nT; ++iT) any = any || !hs[iT].done;
+    // transfers staged through the context's pinned buffer (a host vector without device
+    // allocation in scratch mode), 256-byte slots; the downloads reuse it after the uploads,
+    // which the stream has finished by then
+    auto slot = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    const size_t b_qT = sizeof(double) * qb * nT, b_F = sizeof(double) * qb * 3 * nT;
+    const size_t b_up = slot(b_qT) + slot(b_F) + slot(sizeof(BteState) * nT) + slot(sizeof(double) * 81) +
+                        slot(sizeof(double) * std::max(nirr, 1)) + slot(sizeof(double) * nT);
+    const size_t b_down = slot(sizeof(double) * sScf * nT) + slot(sizeof(double) * sLog * nT) + slot(b_F) +
+                          2 * slot(sizeof(double) * 9 * nT) + slot(sizeof(double) * qb * 9 * nT);
+    std::vector<char> stage_host;
+    char *stage = nullptr;
+    if (scratch) {
+        stage_host.resize(std::max(b_up, b_down));
+        stage = stage_host.data();
+    } else {
+        TRY(c->pinned(std::max(b_up, b_down)));
+        stage = (char *)c->pin;
+    }
+    {
+        char *h = stage;
+        auto up = [&](void *dst, const void *src, size_t n) -> hipError_t {
+            std::memcpy(h, src, n);
+            const hipError_t r = hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, st);
+            h += slot(n);
+            return r;
+        };
+        double S[81];
+        op_average_map(c, S);
+        TRY(up(d_F0, c->F0.data(), b_F));
+        TRY(hipMemcpyAsync(d_Fn, d_F0, b_F, hipMemcpyDeviceToDevice, st));
+        if (any) {
+            TRY(up(d_qs, c->qs.data(), b_qT));
+            TRY(up(d_st, hs.data(), sizeof(BteState) * nT));
+        }
+        TRY(up(d_S, S, sizeof(S)));
+        TRY(up(d_wq, c->mesh.weight_irr.data(), sizeof(double) * nirr));
+        TRY(up(d_temps, c->temps.data(), sizeof(double) * nT));
+    }
+    const dim3 gm((unsigned)((qb + 255) / 256), 1, nT);
+    hipLaunchKernelGGL(k_cv, gm, dim3(256), 0, st, (int)qb, nb, (int)c->classical, d_temps, c->d_om_irr, d_wq,
+                       c->cr.volume, d_cvm, d_cvw, sQ);
+    // SMA kappa: get_kappa of F0 (kappa.f90:52-70); the per-mode values are overwritten below
+    hipLaunchKernelGGL(k_kappa_mode, gm, dim3(256), 0, st, (int)qb, d_S, c->d_om_irr, d_cvm, c->cr.volume, d_F0, d_vel,
+                       d_kmode, sQ, sF);
+    hipLaunchKernelGGL(k_kappa_total, dim3(1, 1, nT), dim3(256), 0, st, nb, (int)qb, c->d_om_irr, d_kmode, d_wq, d_ks);
+    TRY(hipGetLastError());
     if (any) {
-        TRY(hipMemcpyAsync(d_cvw, cvw.data(), sizeof(double) * qb * nT, hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_Fn, c->Fn.data(), sizeof(double) * qb * 3 * nT, hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_F0, c->F0.data(), sizeof(double) * qb * 3 * nT, hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_qs, c->qs.data(), sizeof(double) * qb * nT, hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_st, hs.data(), sizeof(BteState) * nT, hipMemcpyHostToDevice, st));
         const size_t sXi = c->xi_row * (size_t)nrows;
         const bool f32 = c->xi_es == 4;   // FP32 folded-out F next to FP32 Xi
         const long nj = (long)nfull * nb;
@@ -1052,37 +1162,57 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
             done = true;
             for (int iT = 0; iT < nT; ++iT) done = done && hs[iT].done;
         }
-        // final Fn and the per-iteration kappa log of every temperature (kappa.f90:473-474)
-        std::vector<double> scf(sScf * nT), klog(sLog * nT), Fall(qb * 3 * nT);
-        TRY(hipMemcpyAsync(scf.data(), d_scf, sizeof(double) * scf.size(), hipMemcpyDeviceToHost, st));
-        TRY(hipMemcpyAsync(klog.data(), d_klog, sizeof(double) * klog.size(), hipMemcpyDeviceToHost, st));
-        TRY(hipMemcpyAsync(Fall.data(), d_Fn, sizeof(double) * Fall.size(), hipMemcpyDeviceToHost, st));
+    }
+    // kappa of the final F, per mode and in total, on the device; then the final F, the kappa
+    // log (kappa.f90:473-474) and the kappa tensors to the host in one batch
+    hipLaunchKernelGGL(k_kappa_mode, dim3((unsigned)((qb + 255) / 256), 1, nT), dim3(256), 0, st, (int)qb, d_S,
+                       c->d_om_irr, d_cvm, c->cr.volume, d_Fn, d_vel, d_kmode, sQ, sF);
+    hipLaunchKernelGGL(k_kappa_total, dim3(1, 1, nT), dim3(256), 0, st, nb, (int)qb, c->d_om_irr, d_kmode, d_wq, d_kt);
+    TRY(hipGetLastError());
+    {
+        char *h = stage;
+        auto down = [&](const void *src, size_t n, double **dst) -> hipError_t {
+            *dst = (double *)h;
+            h += slot(n);
+            return hipMemcpyAsync(*dst, src, n, hipMemcpyDeviceToHost, st);
+        };
+        double *scf, *klog, *Fall, *ksd, *kt, *km;
+        TRY(down(d_scf, sizeof(double) * sScf * nT, &scf));
+        TRY(down(d_klog, sizeof(double) * sLog * nT, &klog));
+        TRY(down(d_Fn, b_F, &Fall));
+        TRY(down(d_ks, sizeof(double) * 9 * nT, &ksd));
+        TRY(down(d_kt, sizeof(double) * 9 * nT, &kt));
+        TRY(down(d_kmode, sizeof(double) * qb * 9 * nT, &km));
         TRY(hipStreamSynchronize(st));
+        std::memcpy(c->Fn.data(), Fall, b_F);
         for (int iT = 0; iT < nT; ++iT) {
-            if (c->temps[iT] < kTemperatureTol || maxiter <= 0) continue;
-            std::memcpy(&c->Fn[(size_t)iT * qb * 3], &Fall[(size_t)iT * qb * 3], sizeof(double) * qb * 3);
-            const int it_done = hs[iT].it;
-            c->niter[iT] = it_done;
-            auto &log = c->kappa_log[iT];
-            for (int it = 0; it < it_done; ++it) {
-                double kk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                for (int sl = 0; sl < kKapSlices; ++sl)
-                    for (int i = 0; i < 9; ++i) kk[i] += klog[iT * sLog + (9 * kKapSlices) * (size_t)it + 9 * sl + i];
-                chop_kappa(kk);
-                log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf[iT * sScf + it]});
+            if (c->temps[iT] < kTemperatureTol) continue;
+            std::memcpy(&c->kappa_mode[(size_t)iT * qb * 9], &km[(size_t)iT * qb * 9], sizeof(double) * qb * 9);
+            double *ks = &c->kappa_sma[9 * iT], *ki = &c->kappa_iter[9 * iT], *ko = &c->kappa_od[9 * iT];
+            std::memcpy(ks, &ksd[9 * iT], sizeof(double) * 9);
+            chop_kappa(ks);
+            std::memcpy(ki, &kt[9 * iT], sizeof(double) * 9);
+            chop_kappa(ki);
+            if (any && maxiter > 0) {
+                const int it_done = hs[iT].it;
+                c->niter[iT] = it_done;
+                auto &log = c->kappa_log[iT];
+                // the first record is the kappa of F0 (kappa.f90:377), i.e. the SMA value
+                log.insert(log.end(), {ks[0], ks[4], ks[8], ks[3], ks[6], ks[7], 0.0});
+                for (int it = 0; it < it_done; ++it) {
+                    double kk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int sl = 0; sl < kKapSlices; ++sl)
+                        for (int i = 0; i < 9; ++i) kk[i] += klog[iT * sLog + (9 * kKapSlices) * (size_t)it + 9 * sl + i];
+                    chop_kappa(kk);
+                    log.insert(log.end(), {kk[0], kk[4], kk[8], kk[3], kk[6], kk[7], scf[iT * sScf + it]});
+                }
             }
+            symmetrize_kappa(c, ki);
+            symmetrize_kappa(c, ko);
+            symmetrize_kappa(c, ks);
         }
     }
     t_bte += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-    for (int iT = 0; iT < nT; ++iT) {
-        const double T = c->temps[iT];
-        if (T < kTemperatureTol) continue;
-        double *ks = &c->kappa_sma[9 * iT], *ki = &c->kappa_iter[9 * iT], *ko = &c->kappa_od[9 * iT];
-        get_kappa(c, &c->Fn[(size_t)iT * qb * 3], T, c->classical, ki, &c->kappa_mode[(size_t)iT * qb * 9]);
-        symmetrize_kappa(c, ki);
-        symmetrize_kappa(c, ko);
-        symmetrize_kappa(c, ks);
-    }
 #undef TRY
     cleanup();
     c->timings[2] = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
