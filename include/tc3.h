@@ -233,6 +233,24 @@ int tc3_adjoint_gamma(tc3_ctx *ctx, const double *dL_dlinewidth, int precision, 
  * identity.  Needs tc3_scatter first (not tc3_kappa). */
 int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, double *dL_dfc3);
 
+/* Directional derivatives of the harmonic properties, the linewidths and the single-mode
+ * (RTA) thermal conductivity along a change of the force constants: dfc2 (9 per pair, the
+ * layout and order of tc3_set_fc2's m) and dfc3 (27 per triplet, as tc3_set_fc3's m), either
+ * null for no change.  Outputs (each may be null): dlinewidth (nT x nirr x nb, as
+ * tc3_get_scattering's linewidth), domega (nirr x nb), dvel (nirr x nb x 3), dkappa_sma (nT x 9,
+ * as tc3_kappa's kappa_sma, lo_chop taken as the identity).  Frequencies and eigenvectors change
+ * by first-order perturbation of the dynamical matrix (rotations inside degenerate subspaces
+ * left out: the degeneracy-averaged linewidths do not depend on them), |Psi|^2 and the smeared
+ * energy conservation by the chain rule, all in FP64 on the device.  Forward-mode
+ * differentiation of lo_scattering_rates%generate (scattering.f90:58-358), get_kappa
+ * (kappa.f90:28-75) and the dispersions (type_phonon_dispersions.f90); the reference has no
+ * derivatives, so this replaces nothing.  Gradients with respect to a few parameters of the
+ * force constants take one call per parameter.  Needs tc3_scatter first; three-phonon
+ * scattering only (status 5 with isotope disorder, boundary or four-phonon scattering, Monte-
+ * Carlo grids or integration type 6). */
+int tc3_jvp(tc3_ctx *ctx, const double *dfc2, const double *dfc3, double *dlinewidth, double *domega, double *dvel,
+            double *dkappa_sma);
+
 /* Probes of the contraction's batch autotuner since the context was created: up to maxn
  * records of (triplets per launch, best probe time in ms) in the order they ran; *n
  * receives the number of records available.  The chosen batch of the last tc3_scatter is

@@ -7,7 +7,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtc3.so")
 SOURCES = ["tc3_host.cpp", "tc3_api.cpp", "tc3_isotopes.cpp", "tc3_hdf5.cpp", "tc3_cumulative.cpp", "tc3_harmonic.hip", "tc3_scatter.hip",
-           "tc3_kappa.hip", "tc3_mfma.hip", "tc3_fused.hip", "tc3_adjoint.hip", "tc3_fourphonon.hip"]
+           "tc3_kappa.hip", "tc3_mfma.hip", "tc3_fused.hip", "tc3_adjoint.hip", "tc3_fourphonon.hip",
+           "tc3_jvp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 VALU instructions.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",

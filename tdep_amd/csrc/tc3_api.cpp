@@ -138,6 +138,7 @@ int tc3_set_fc2(tc3_ctx *ctx, int npair, const int *a1, const int *a2, const dou
             return std::make_pair(a1[x], a2[x]) < std::make_pair(a1[y], a2[y]);
         });
         ctx->p_a1.resize(npair); ctx->p_a2.resize(npair);
+        ctx->p_src = idx;
         ctx->p_lv.resize(3 * (size_t)npair); ctx->p_m.resize(9 * (size_t)npair);
         ctx->blk2_off.assign(na * na + 1, 0);
         for (int n = 0; n < npair; ++n) {
@@ -610,6 +611,28 @@ int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, 
         std::vector<double> dlw(qb * ctx->nT);
         for (int T = 0; T < ctx->nT; ++T) kappa_rta_cotangent(ctx, T, dL_dkappa + 9 * (size_t)T, &dlw[qb * T]);
         return adjoint_from_linewidths(ctx, dlw.data(), precision, dL_dfc3);
+    });
+}
+
+int tc3_jvp(tc3_ctx *ctx, const double *dfc2, const double *dfc3, double *dlinewidth, double *domega, double *dvel,
+            double *dkappa_sma) {
+    return guarded(ctx, [&]() -> int {
+        if (ctx->device < 0) return ctx->fail(7, "host-only context has no device");
+        if (!ctx->have_scatter) return ctx->fail(5, "run tc3_scatter first");
+        if (!ctx->thirdorder) return ctx->fail(5, "the three-phonon term is switched off");
+        bool iso = false;
+        for (double g : ctx->iso_g) iso = iso || (ctx->iso_on && g > 0.0);
+        if (iso || ctx->mfp_max > 0.0 || ctx->nfc4 > 0 || !ctx->trip_w.empty() || ctx->integrationtype == 6)
+            return ctx->fail(5, "derivatives cover three-phonon scattering on the full grid with integration type 1 "
+                                "or 2 (no isotope disorder, boundary or four-phonon scattering)");
+        std::vector<double> dlw, dom, dv, dks;
+        const int r = run_jvp(ctx, dfc2, dfc3, dlw, dom, dv, dks);
+        if (r) return r;
+        if (dlinewidth) std::copy(dlw.begin(), dlw.end(), dlinewidth);
+        if (domega) std::copy(dom.begin(), dom.end(), domega);
+        if (dvel) std::copy(dv.begin(), dv.end(), dvel);
+        if (dkappa_sma) std::copy(dks.begin(), dks.end(), dkappa_sma);
+        return 0;
     });
 }
 

@@ -24,7 +24,7 @@ struct tc3_ctx {
     int nb = 0;
 
     // second order, sorted by (a1, a2); blk2_off[(a1*na + a2)] .. [+1]
-    std::vector<int> p_a1, p_a2, blk2_off;
+    std::vector<int> p_a1, p_a2, blk2_off, p_src;  // p_src: the caller's index of each sorted pair
     std::vector<double> p_lv, p_m;  // 3 / 9 per pair (m row-major [i][j])
 
     // third order, sorted by (a1, a2, a3); blk3_off over na^3 blocks; a1_off over atoms
@@ -183,6 +183,14 @@ extern thread_local std::string g_device_error;
 // tc3_harmonic.hip
 int run_harmonic(tc3_ctx *c);
 int ensure_full_host(tc3_ctx *c);
+// device derivatives of the harmonic properties along a change of the second-order force
+// constants (pool buffers of the context)
+struct HarmonicJvp {
+    cd *de_irr = nullptr, *de_full = nullptr;       // [q][component][mode]
+    double *dom_irr = nullptr, *dom_full = nullptr; // [q][mode]
+    double *dv_irr = nullptr, *dv_full = nullptr;   // [q][mode][3]
+};
+int run_harmonic_jvp(tc3_ctx *c, const std::vector<double> &dpm, HarmonicJvp &out);
 // tc3_scatter.hip
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma);
 // tc3_kappa.hip
@@ -190,6 +198,10 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch =
 size_t bte_scratch_bytes();
 int bte_scratch_ptr(int device, void **ptr);
 void kappa_rta_cotangent(const tc3_ctx *c, int iT, const double *G, double *dlw);
+void symmetrize_kappa_tensor(const tc3_ctx *c, double *k);   // symmetrize_kappa, kappa.f90:308-324
+// tc3_jvp.hip
+int run_jvp(tc3_ctx *c, const double *dfc2, const double *dfc3, std::vector<double> &dlw, std::vector<double> &dom,
+            std::vector<double> &dvel, std::vector<double> &dks);
 // tc3_adjoint.hip
 int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<double> &grad);
 // tc3_hdf5.cpp

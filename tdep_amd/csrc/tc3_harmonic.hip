@@ -507,6 +507,108 @@ __global__ void k_unfold(int nb, int na, const int *__restrict__ irr_of_full,
     }
 }
 
+// First-order change of the harmonic properties of one irreducible point along a change dD of
+// its dynamical matrix (and dG of dD/dq), workgroup per point, working arrays in global
+// scratch (Y = dD E, then X = E^H dD E):
+//   d lambda_n = mean over the degenerate group of n of Re X_mm (the group stays degenerate
+//     under a symmetric change: E_S^H dD E_S is then a multiple of 1),
+//   de_n = sum over m outside the group of n of e_m X_mn / (lambda_n - lambda_m),
+//   d omega = d lambda / (2 omega), lambda = omega |omega|,
+//   dv = mean over the group of [Re e^H dG e + 2 Re e^H G de] / (2 omega) - v d omega / omega,
+// the derivative of the group-averaged Hellmann-Feynman velocity of k_post.  Rotations inside
+// a degenerate group and phases of single modes are left out: the linewidths averaged over
+// degenerate modes do not depend on them.  Modes below lo_freqtol get zero.
+__global__ void k_eig_jvp(int nb, const cd *__restrict__ egv, const double *__restrict__ om,
+                          const double *__restrict__ vel, const cd *__restrict__ gD, const cd *__restrict__ dD,
+                          const cd *__restrict__ dgD, cd *__restrict__ scr, cd *__restrict__ de,
+                          double *__restrict__ dom, double *__restrict__ dvel) {
+    const int q = blockIdx.x;
+    const size_t nn = (size_t)nb * nb;
+    const cd *E = egv + (size_t)q * nn;
+    const cd *A = dD + (size_t)q * nn;
+    cd *Y = scr + (size_t)q * 2 * nn, *X = Y + nn;
+    cd *dE = de + (size_t)q * nn;
+    const double *w = om + (size_t)q * nb;
+    __shared__ int gfirst[128], glast[128];
+    __shared__ double dl[128], dw[128];
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {
+        const int r = i / nb, b = i % nb;
+        cd s = {0, 0};
+        for (int c = 0; c < nb; ++c) s = cadd(s, cmul(A[(size_t)r * nb + c], E[(size_t)c * nb + b]));
+        Y[i] = s;
+    }
+    if (threadIdx.x == 0) {
+        // consecutive modes closer than lo_freqtol, as the gauge groups of k_post
+        int g0 = 0;
+        for (int i = 1; i <= nb; ++i)
+            if (i == nb || !(w[i] - w[i - 1] < 1e-9)) {
+                for (int k = g0; k < i; ++k) { gfirst[k] = g0; glast[k] = i; }
+                g0 = i;
+            }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {
+        const int m = i / nb, n = i % nb;
+        cd s = {0, 0};
+        for (int r = 0; r < nb; ++r) s = cadd(s, cmulc(E[(size_t)r * nb + m], Y[(size_t)r * nb + n]));
+        X[i] = s;
+    }
+    __syncthreads();
+    for (int n = threadIdx.x; n < nb; n += blockDim.x) {
+        double s = 0;
+        for (int m = gfirst[n]; m < glast[n]; ++m) s += X[(size_t)m * nb + m].x;
+        dl[n] = s / (glast[n] - gfirst[n]);
+        dw[n] = (w[n] > 1e-9) ? dl[n] / (2.0 * w[n]) : 0.0;
+        dom[(size_t)q * nb + n] = dw[n];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {
+        const int c = i / nb, n = i % nb;
+        cd s = {0, 0};
+        if (w[n] > 1e-9) {
+            const double ln = w[n] * fabs(w[n]);
+            for (int m = 0; m < nb; ++m) {
+                if (m >= gfirst[n] && m < glast[n]) continue;
+                const double lm = w[m] * fabs(w[m]);
+                s = cadd(s, cscale(cmul(E[(size_t)c * nb + m], X[(size_t)m * nb + n]), 1.0 / (ln - lm)));
+            }
+        }
+        dE[i] = s;
+    }
+    __syncthreads();
+    // group-averaged velocity derivative: one thread per (mode, direction) for the
+    // Hellmann-Feynman terms, then the mean over the group
+    __shared__ double dhf[3][128];
+    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) {
+        const int al = t / nb, n = t % nb;
+        const cd *G = gD + ((size_t)q * 3 + al) * nn, *dG = dgD + ((size_t)q * 3 + al) * nn;
+        double a = 0, b = 0;
+        for (int r = 0; r < nb; ++r) {
+            cd g = {0, 0}, h = {0, 0};
+            for (int c = 0; c < nb; ++c) {
+                g = cadd(g, cmul(dG[(size_t)r * nb + c], E[(size_t)c * nb + n]));
+                h = cadd(h, cmul(G[(size_t)r * nb + c], dE[(size_t)c * nb + n]));
+            }
+            const cd er = E[(size_t)r * nb + n];
+            a += er.x * g.x + er.y * g.y;
+            b += er.x * h.x + er.y * h.y;
+        }
+        dhf[al][n] = a + 2.0 * b;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) {
+        const int al = t / nb, n = t % nb;
+        double x = 0;
+        if (w[n] > 1e-9) {
+            double s = 0;
+            for (int m = gfirst[n]; m < glast[n]; ++m) s += dhf[al][m];
+            s /= (glast[n] - gfirst[n]);
+            x = s / (2.0 * w[n]) - vel[((size_t)q * nb + n) * 3 + al] * dw[n] / w[n];
+        }
+        dvel[((size_t)q * nb + n) * 3 + al] = x;
+    }
+}
+
 }  // namespace
 
 int run_harmonic(tc3_ctx *c) {
@@ -670,6 +772,61 @@ int run_harmonic(tc3_ctx *c) {
 }
 
 // Host copies of the full-grid frequencies, velocities and eigenvectors (on demand).
+// Derivatives of the frequencies, velocities and eigenvectors of the irreducible and the full
+// grid along a change dpm of the second-order force constants (9 per pair in the context's
+// sorted order, rows as p_m), into pool buffers: the dynamical matrices of the model and of
+// dpm (k_dynmat is linear in the force constants), the perturbation per irreducible point
+// (k_eig_jvp), and the symmetry unfolding of k_unfold applied to the derivatives (it is
+// linear in them).  The tables of the last harmonic step (q-points, pairs, operations) are
+// reused from the pool.
+int run_harmonic_jvp(tc3_ctx *c, const std::vector<double> &dpm, HarmonicJvp &out) {
+    const int na = c->cr.na, nb = 3 * na, nirr = c->mesh.nirr, nfull = c->mesh.nfull;
+    const int nsym = (int)c->ops.size(), npair = (int)c->p_a1.size();
+    const size_t nn = (size_t)nb * nb;
+    hipStream_t st = c->stream;
+    hipError_t e = hipSuccess;
+#define TRY(x) do { e = (x); if (e != hipSuccess) return device_error(e, #x, __FILE__, __LINE__); } while (0)
+    auto work = [&](const char *name, void **p, size_t bytes) { return c->pool_get(name, std::max<size_t>(bytes, 16), p); };
+    double *d_qv, *d_lv, *d_pm, *d_dpm, *d_ism, *d_opm, *d_v0, *d_qirr;
+    int *d_blk, *d_src, *d_opf;
+    cd *d_D, *d_gD, *d_dD, *d_dgD, *d_scr;
+    TRY(work("h_qv", (void **)&d_qv, sizeof(double) * 3 * nirr));
+    TRY(work("h_qirr", (void **)&d_qirr, sizeof(double) * 3 * nirr));
+    TRY(work("h_lv", (void **)&d_lv, sizeof(double) * 3 * std::max(npair, 1)));
+    TRY(work("h_pm", (void **)&d_pm, sizeof(double) * 9 * std::max(npair, 1)));
+    TRY(work("h_ism", (void **)&d_ism, sizeof(double) * na));
+    TRY(work("h_blk", (void **)&d_blk, sizeof(int) * (na * na + 1)));
+    TRY(work("h_opm", (void **)&d_opm, sizeof(double) * 9 * nsym));
+    TRY(work("h_v0", (void **)&d_v0, sizeof(double) * 3 * nsym * na));
+    TRY(work("h_src", (void **)&d_src, sizeof(int) * nsym * na));
+    TRY(work("h_opf", (void **)&d_opf, sizeof(int) * nfull));
+    TRY(work("j_dpm", (void **)&d_dpm, sizeof(double) * 9 * std::max(npair, 1)));
+    TRY(work("j_D", (void **)&d_D, sizeof(cd) * nn * nirr));
+    TRY(work("j_gD", (void **)&d_gD, sizeof(cd) * 3 * nn * nirr));
+    TRY(work("j_dD", (void **)&d_dD, sizeof(cd) * nn * nirr));
+    TRY(work("j_dgD", (void **)&d_dgD, sizeof(cd) * 3 * nn * nirr));
+    TRY(work("j_scr", (void **)&d_scr, sizeof(cd) * 2 * nn * nirr));
+    TRY(work("j_de_irr", (void **)&out.de_irr, sizeof(cd) * nn * nirr));
+    TRY(work("j_dom_irr", (void **)&out.dom_irr, sizeof(double) * nb * nirr));
+    TRY(work("j_dv_irr", (void **)&out.dv_irr, sizeof(double) * 3 * nb * nirr));
+    TRY(work("j_de_full", (void **)&out.de_full, sizeof(cd) * nn * nfull));
+    TRY(work("j_dom_full", (void **)&out.dom_full, sizeof(double) * nb * nfull));
+    TRY(work("j_dv_full", (void **)&out.dv_full, sizeof(double) * 3 * nb * nfull));
+    if (npair) {
+        TRY(hipMemcpyAsync(d_pm, c->p_m.data(), sizeof(double) * 9 * npair, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_dpm, dpm.data(), sizeof(double) * 9 * npair, hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(k_dynmat, dim3(nirr), dim3(kBlock), 0, st, na, d_qv, d_blk, d_lv, d_pm, d_ism, d_D, d_gD);
+    hipLaunchKernelGGL(k_dynmat, dim3(nirr), dim3(kBlock), 0, st, na, d_qv, d_blk, d_lv, d_dpm, d_ism, d_dD, d_dgD);
+    hipLaunchKernelGGL(k_eig_jvp, dim3(nirr), dim3(kBlock), 0, st, nb, c->d_egv_irr, c->d_om_irr, c->d_vel_irr, d_gD,
+                       d_dD, d_dgD, d_scr, out.de_irr, out.dom_irr, out.dv_irr);
+    hipLaunchKernelGGL(k_unfold, dim3(nfull), dim3(kBlock), 0, st, nb, na, c->d_irr_of_full, d_opf, d_opm, d_src, d_v0,
+                       d_qirr, out.de_irr, out.dv_irr, out.dom_irr, out.de_full, out.dv_full, out.dom_full);
+    TRY(hipGetLastError());
+#undef TRY
+    return 0;
+}
+
 int ensure_full_host(tc3_ctx *c) {
     if (c->full_host_valid) return 0;
     const int nb = c->nb, nfull = c->mesh.nfull;

@@ -834,6 +834,8 @@ void kappa_rta_cotangent(const tc3_ctx *c, int iT, const double *G, double *dlw)
         }
 }
 
+void symmetrize_kappa_tensor(const tc3_ctx *c, double *k) { symmetrize_kappa(c, k); }
+
 // Working memory of tc3_kappa_noalloc: every buffer of the iterative solve is carved from this
 // array (one copy per GPU, held until the process ends).  Host code never uses the name as an
 // address: bte_scratch_ptr() resolves the device address of the current GPU's copy once.

@@ -276,6 +276,21 @@ class Engine:
         self._check(self.lib.tc3_adjoint_kappa_rta(self.ctx, dptr(d), PREC[precision], dptr(out)))
         return out.reshape(self.nfc3, 3, 3, 3).transpose(0, 3, 2, 1).copy()
 
+    def jvp(self, dfc2=None, dfc3=None):
+        """Changes along dfc2 [npair, 3, 3] and dfc3 [ntrip, 3, 3, 3] (the layouts of set_fc2 and
+        set_fc3's m, Hartree/Bohr^n; None for no change) of the linewidths [nT, nirr, nb], the
+        frequencies [nirr, nb], the group velocities [nirr, nb, 3] and kappa_sma [nT, 3, 3] (as
+        kappa() returns it), by forward-mode differentiation on the device (tc3_jvp); needs
+        scattering_rates."""
+        nT, qb = len(self.temperatures), self.nirr * self.nb
+        d2 = None if dfc2 is None else np.ascontiguousarray(np.transpose(dfc2, (0, 2, 1)).reshape(-1))
+        d3 = None if dfc3 is None else np.ascontiguousarray(np.transpose(dfc3, (0, 3, 2, 1)).reshape(-1))
+        dlw, dom, dv, dk = np.zeros(nT * qb), np.zeros(qb), np.zeros(3 * qb), np.zeros(9 * nT)
+        self._check(self.lib.tc3_jvp(self.ctx, None if d2 is None else dptr(d2), None if d3 is None else dptr(d3),
+                                     dptr(dlw), dptr(dom), dptr(dv), dptr(dk)))
+        return dict(linewidth=dlw.reshape(nT, self.nirr, self.nb), omega=dom.reshape(self.nirr, self.nb),
+                    velocity=dv.reshape(self.nirr, self.nb, 3), kappa_sma=dk.reshape(nT, 3, 3).transpose(0, 2, 1).copy())
+
     def timings(self):
         t = np.zeros(8)
         self._check(self.lib.tc3_get_timings(self.ctx, dptr(t), 8))

@@ -48,6 +48,43 @@ class Morse:
         return D * (16 * a ** 4 * e1 * e1 - 2 * a ** 4 * e1)
 
 
+class MorseDerivative(Morse):
+    """Derivatives of (phi', phi'', phi''') of a Morse potential with respect to one of its
+    parameters, "D", "alpha" or "r0" (the D given to derivs includes the shell's jitter
+    factor, so the derivative with respect to D scales by it)."""
+
+    def __init__(self, base, which):
+        super().__init__(base.D, base.alpha, base.r0)
+        self.which = which
+
+    def derivs(self, r, D=None):
+        D = self.D if D is None else D
+        a, x = self.alpha, r - self.r0
+        e1 = np.exp(-a * x)
+        e2 = e1 * e1
+        if self.which == "D":
+            return tuple(v / self.D for v in Morse.derivs(self, r, D))
+        if self.which == "alpha":
+            return (D * (-2 * e2 + 2 * e1 + 4 * a * x * e2 - 2 * a * x * e1),
+                    D * (8 * a * e2 - 4 * a * e1 - 8 * a * a * x * e2 + 2 * a * a * x * e1),
+                    D * (-24 * a * a * e2 + 6 * a * a * e1 + 16 * a ** 3 * x * e2 - 2 * a ** 3 * x * e1))
+        if self.which == "r0":
+            return (D * (-4 * a * a * e2 + 2 * a * a * e1),
+                    D * (8 * a ** 3 * e2 - 2 * a ** 3 * e1),
+                    D * (-16 * a ** 4 * e2 + 2 * a ** 4 * e1))
+        raise KeyError(self.which)
+
+
+class ZeroPotential(Morse):
+    """A pair whose potential does not depend on the parameter (the other pair classes)."""
+
+    def __init__(self, base):
+        super().__init__(base.D, base.alpha, base.r0)
+
+    def derivs(self, r, D=None):
+        return 0.0, 0.0, 0.0
+
+
 def pair_tensors(R, d1, d2, d3):
     """Second (H_ij) and third (T_ijk) Cartesian derivatives of phi(|R|)."""
     r = np.linalg.norm(R)
@@ -287,6 +324,33 @@ CONFIGS = {
 }
 
 
+MORSE_FIELDS = ("D", "alpha", "r0")
+
+
+def morse_parameters(name):
+    """The Morse parameters of a family as a vector, (D, alpha, r0) per pair class (rocksalt:
+    unlike pairs, then like pairs), their labels, potential(v) -> potential(s1, s2), and
+    derivative(v, k) -> the potential(s1, s2) of the derivatives with respect to v[k]."""
+    if name in ("rocksalt", "rocksalt_adjoint"):
+        p = np.array([0.45, 1.30, 2.82, 0.10, 1.10, 3.99])
+        labels = [f"{c}_{f}" for c in ("unlike", "like") for f in MORSE_FIELDS]
+        cls = lambda s1, s2: 0 if s1 != s2 else 1
+    else:
+        p = np.array([MORSE_AL.D, MORSE_AL.alpha, MORSE_AL.r0])
+        labels = list(MORSE_FIELDS)
+        cls = lambda s1, s2: 0
+
+    def potential(v):
+        v = np.asarray(v, dtype=np.float64)
+        return lambda s1, s2: Morse(*v[3 * cls(s1, s2):3 * cls(s1, s2) + 3])
+
+    def derivative(v, k):
+        base = potential(v)
+        return lambda s1, s2: (MorseDerivative(base(s1, s2), MORSE_FIELDS[k % 3]) if cls(s1, s2) == k // 3
+                               else ZeroPotential(base(s1, s2)))
+    return p, labels, potential, derivative
+
+
 def make_crystal(name, seed=0):
     """Crystal, pair potential, cutoff (A) of a workload family."""
     if name == "fcc_al":
@@ -305,10 +369,13 @@ def make_crystal(name, seed=0):
     raise KeyError(name)
 
 
-def make_config(name, path, seed=0, fourth=False):
+def make_config(name, path, seed=0, fourth=False, params=None):
     """Write the four input files of a workload family into `path` (and with fourth=True
-    infile.forceconstant_fourthorder); returns its dict."""
+    infile.forceconstant_fourthorder); returns its dict.  params: the family's Morse
+    parameter vector (morse_parameters) in place of the default."""
     cr, pot, cutoff = make_crystal(name, seed)
+    if params is not None:
+        pot = morse_parameters(name)[2](params)
     if fourth:
         fc2, fc3, fc4 = morse_ifcs(cr, pot, cutoff, seed=seed, fourth=True)
     else:
@@ -317,3 +384,11 @@ def make_config(name, path, seed=0, fourth=False):
     if fourth:
         write_fourthorder(cr, fc4, cutoff, path)
     return dict(CONFIGS[name], name=name, natoms=cr.na)
+
+
+def morse_ifc_derivatives(name, params, k, seed=0):
+    """Derivatives of the force-constant lists of a family with respect to params[k], lists as
+    morse_ifcs returns them (eV/A^2, eV/A^3), entry for entry in the order of the files
+    make_config writes."""
+    cr, _, cutoff = make_crystal(name, seed)
+    return morse_ifcs(cr, morse_parameters(name)[3](params, k), cutoff, seed=seed)

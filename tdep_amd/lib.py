@@ -55,6 +55,7 @@ SIGNATURES = {
     "tc3_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_int_p]),
     "tc3_adjoint_gamma": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]),
     "tc3_adjoint_kappa_rta": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]),
+    "tc3_jvp": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
 }
 
 
