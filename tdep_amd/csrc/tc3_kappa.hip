@@ -718,10 +718,10 @@ __global__ void k_cv(int qb, int nb, int classical, const double *__restrict__ t
     cvw[T * sQ + i] = w;
 }
 
-// get_kappa of the final F (kappa.f90:52-70) on the device: per mode km[x] = (cv / V) sum_ba
-// S[9 x + 3 b + a] F_a v_b (S the operation average of op_average_map), then per temperature
-// kappa[x] = sum over the modes in order of km[x] w_q.  Products and sums unfused, in the
-// order of the host restatement get_kappa, so the results are its bits.
+// get_kappa (kappa.f90:52-70) on the device: per mode km[x] = (cv / V) sum_ba S[9 x + 3 b + a]
+// F_a v_b (S the operation average of op_average_map), products and sums unfused in the order
+// of the host restatement get_kappa (so the per-mode tensors are its bits), then per
+// temperature kappa[x] = sum over the modes of km[x] w_q in a fixed order.
 __global__ void k_kappa_mode(int qb, const double *__restrict__ S, const double *__restrict__ om,
                              const double *__restrict__ cv_, double volume, const double *__restrict__ Fn_,
                              const double *__restrict__ vel, double *__restrict__ km_, long sQ, long sF) {
@@ -743,29 +743,35 @@ __global__ void k_kappa_mode(int qb, const double *__restrict__ S, const double 
     }
 }
 
-// The sum in mode order: chunks of 256 modes staged in LDS (coalesced loads), then summed by
-// one thread per component.
+// The sum over the modes: each thread sums a contiguous range of modes in order, then the
+// ranges are added by a fixed tree (wave shuffles, then the waves in order), so the result does
+// not depend on scheduling.
 __global__ void __launch_bounds__(256) k_kappa_total(int nb, int qb, const double *__restrict__ om,
                                                      const double *__restrict__ km_, const double *__restrict__ wq,
                                                      double *__restrict__ kt) {
-    const int T = blockIdx.z, x = threadIdx.x;
+    const int T = blockIdx.z;
     const double *km = km_ + (size_t)T * qb * 9;
-    __shared__ double ck[256 * 9], cw[256];
-    double k = 0;
-    for (int i0 = 0; i0 < qb; i0 += 256) {
-        const int n = min(256, qb - i0);
-        for (int j = threadIdx.x; j < 9 * n; j += blockDim.x) ck[j] = km[(size_t)i0 * 9 + j];
-        if (threadIdx.x < n) {
-            const int i = i0 + threadIdx.x;
-            cw[threadIdx.x] = (om[i] < kFreqTol) ? -1.0 : wq[i / nb];
-        }
-        __syncthreads();
-        if (x < 9)
-            for (int j = 0; j < n; ++j)
-                if (cw[j] >= 0.0) k = __dadd_rn(k, __dmul_rn(ck[9 * j + x], cw[j]));
-        __syncthreads();
+    const int per = (qb + blockDim.x - 1) / blockDim.x;
+    const int i0 = threadIdx.x * per, i1 = min(qb, i0 + per);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = i0; i < i1; ++i) {
+        if (om[i] < kFreqTol) continue;
+        const double w = wq[i / nb];
+        for (int x = 0; x < 9; ++x) acc[x] = __dadd_rn(acc[x], __dmul_rn(km[(size_t)i * 9 + x], w));
     }
-    if (x < 9) kt[9 * T + x] = k;
+    __shared__ double red[9][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int x = 0; x < 9; ++x) {
+        double v = acc[x];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[x][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        double v = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[threadIdx.x][w];
+        kt[9 * T + threadIdx.x] = v;
+    }
 }
 
 __global__ void k_bte_check(int nirr, int nb, const double *__restrict__ g0part_, double *__restrict__ scf_,
