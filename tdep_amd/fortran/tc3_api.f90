@@ -220,6 +220,14 @@ module tc3_api
             integer(c_int), value :: precision
             real(c_double), intent(out) :: dL_dfc3(3, 3, 3, *)
         end function
+
+        integer(c_int) function tc3_jvp(ctx, dfc2, dfc3, dlinewidth, domega, dvel, dkappa_sma) &
+                bind(C, name="tc3_jvp")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: dfc2(3, 3, *), dfc3(3, 3, 3, *)
+            real(c_double), intent(out) :: dlinewidth(*), domega(*), dvel(3, *), dkappa_sma(3, 3, *)
+        end function
     end interface
 
 contains
