@@ -164,8 +164,39 @@ def main():
         err_sma = float(np.max(np.abs(ksma_mx - ksma_64)) / np.max(np.abs(ksma_64)))
         big = lw_64 > 1e-3 * lw_64.max(axis=(1, 2), keepdims=True)
         lw_rel = np.abs(lw_mx[big] / lw_64[big] - 1)
-        lw_err = dict(max=float(lw_rel.max()), median=float(np.median(lw_rel)),
-                      modes="linewidths above 1e-3 of the largest, all temperatures")
+        lw_err = {args.precision: dict(max=float(lw_rel.max()), median=float(np.median(lw_rel)))}
+        # the other reduced-precision mode's linewidths (one untimed pass)
+        other = "fp32" if args.precision == "bf16x3" else "bf16x3"
+        if args.precision != "fp64":
+            eng.scattering_rates(temps, precision=other)
+            lw_o = np.array([eng.linewidths(i)[0] for i in range(len(temps))])
+            r_o = np.abs(lw_o[big] / lw_64[big] - 1)
+            lw_err[other] = dict(max=float(r_o.max()), median=float(np.median(r_o)))
+        lw_err["modes"] = "linewidths above 1e-3 of the largest, all temperatures"
+        # |Psi|^2 of 32 triplets spread over the set, formed by the contraction kernels in each
+        # precision (tc3_get_psi2), against FP64: normwise per triplet, and per element above
+        # 1e-3 of its triplet's largest; elements with a frequency below lo_freqtol left out
+        psi_err = {}
+        pt = np.linspace(0, eng.ntriplets - 1, 32).astype(np.int64)
+        p64, pq = eng.psi2(pt, "fp64")
+        om_i, _, _ = eng.harmonic()
+        om_f, _, _ = eng.harmonic(full=True)
+        ftol = 1e-9   # lo_freqtol, Hartree
+        for prec in ("fp32", "bf16x3"):
+            pm, _ = eng.psi2(pt, prec)
+            nrm, el = [], []
+            for i, (a1, a2, a3) in enumerate(pq):
+                m = ((om_i[a1] >= ftol)[:, None, None] & (om_f[a2] >= ftol)[None, :, None]
+                     & (om_f[a3] >= ftol)[None, None, :])
+                x, y = pm[i][m], p64[i][m]
+                nrm.append(np.linalg.norm(x - y) / np.linalg.norm(y))
+                bg = y > 1e-3 * y.max()
+                el.append(np.abs(x[bg] / y[bg] - 1))
+            el = np.concatenate(el)
+            psi_err[prec] = dict(norm_max=float(max(nrm)), norm_median=float(np.median(nrm)),
+                                 element_max=float(el.max()), element_median=float(np.median(el)))
+        psi_err["sample"] = "32 triplets spread over the set; element errors over elements above 1e-3 of the triplet's largest"
+        del p64
         # end to end: a fresh context from the input files to outfile.thermal_conductivity and
         # the grid file (autotuner probes, graph instantiation and uploads included)
         e2e = {}
@@ -258,6 +289,7 @@ def main():
                              "stages_mixed": st_mx, "stages_fp64": st_64},
             "kappa_rel_err_vs_fp64": {"iterative": err, "sma": err_sma},
             "linewidth_rel_err_vs_fp64": lw_err,
+            "psi2_rel_err_vs_fp64": psi_err,
             "end_to_end_s": e2e,
             "kappa_xx_W_mK_300K": {"mixed": float(kit_mx[2][0, 0] * KAPPA_AU_TO_SI),
                                    "fp64": float(kit_64[2][0, 0] * KAPPA_AU_TO_SI)},

@@ -251,6 +251,17 @@ int tc3_adjoint_kappa_rta(tc3_ctx *ctx, const double *dL_dkappa, int precision, 
 int tc3_jvp(tc3_ctx *ctx, const double *dfc2, const double *dfc3, double *dlinewidth, double *domega, double *dvel,
             double *dkappa_sma);
 
+/* The three-phonon matrix elements of n irreducible triplets (indices into the
+ * 0 .. ntriplets-1 of tc3_mesh_info, the full-grid triplet set): psi2[t][b1][b2][b3] (n nb^3
+ * values) = threephonon_prefactor |Psi|^2 / N_q, the psisq of scattering_threephonon.f90:112
+ * without the permutation factor, for every band triple (those the reference skips
+ * included); qpoints[3t..3t+2] (may be null) receive q1 (irreducible index), q2 and q3
+ * (full-grid indices).  Formed by the production contraction kernels in the given precision
+ * (TC3_PREC_*), to measure the precision's |Psi|^2 error; the MFMA contraction only (48 and
+ * 96 bands, status 5 otherwise).  A diagnostic with no counterpart in the reference.  Needs
+ * tc3_harmonic and tc3_set_fc3. */
+int tc3_get_psi2(tc3_ctx *ctx, int precision, int n, const int64_t *triplets, double *psi2, int *qpoints);
+
 /* Probes of the contraction's batch autotuner since the context was created: up to maxn
  * records of (triplets per launch, best probe time in ms) in the order they ran; *n
  * receives the number of records available.  The chosen batch of the last tc3_scatter is

@@ -636,6 +636,15 @@ int tc3_jvp(tc3_ctx *ctx, const double *dfc2, const double *dfc3, double *dlinew
     });
 }
 
+int tc3_get_psi2(tc3_ctx *ctx, int precision, int n, const int64_t *triplets, double *psi2, int *qpoints) {
+    return guarded(ctx, [&]() -> int {
+        if (ctx->device < 0) return ctx->fail(7, "host-only context has no device");
+        if (!ctx->have_harmonic || !ctx->have_fc3) return ctx->fail(5, "run tc3_harmonic and set fc3 first");
+        if (precision < 0 || precision > 2) return ctx->fail(5, "unknown precision mode");
+        return run_psi2(ctx, precision, n, triplets, psi2, qpoints);
+    });
+}
+
 int tc3_get_timings(const tc3_ctx *ctx, double *t, int n) {
     if (!ctx || !t) return 5;
     for (int i = 0; i < n && i < 8; ++i) t[i] = ctx->timings[i];

@@ -193,6 +193,7 @@ struct HarmonicJvp {
 int run_harmonic_jvp(tc3_ctx *c, const std::vector<double> &dpm, HarmonicJvp &out);
 // tc3_scatter.hip
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma);
+int run_psi2(tc3_ctx *c, int precision, int n, const int64_t *trips, double *out, int *qpts);
 // tc3_kappa.hip
 int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch = nullptr, size_t scratch_bytes = 0);
 size_t bte_scratch_bytes();

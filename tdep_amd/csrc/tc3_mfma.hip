@@ -511,7 +511,9 @@ size_t psi_lds() {
     return s + sizeof(S) * 3 * (16 + 2 * NB);    // reduced velocities (integration type 6)
 }
 
-template <int PREC, int NB, int NTY>
+// DUMP: every tile is formed (no dead-tile skip) and |Psi|^2 with its prefactor is stored to
+// a.psi_out[t][b1][b2][b3] (diagnostic export; the production instantiations have DUMP off).
+template <int PREC, int NB, int NTY, bool DUMP = false>
 __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PREC == 2 ? 3 : 1))) k_psi(MfmaEpiArgs a, const void *E1re_, const void *E1im_,
                                                 const __bf16 *__restrict__ Et1, const void *W_) {
     typedef typename Traits<PREC>::T T;
@@ -677,7 +679,10 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     // the rounding of the element-wise test, decide per b2; the decision is wave-uniform.
     constexpr int NLW = (NB + 63) / 64;
     unsigned long long live[NLW];
-    {
+    if constexpr (DUMP) {
+#pragma unroll
+        for (int wd = 0; wd < NLW; ++wd) live[wd] = (NB - 64 * wd >= 64) ? ~0ull : ((1ull << (NB - 64 * wd)) - 1);
+    } else {
         double w1lo = 1e300, w1hi = -1e300, s1m = 0, w3lo = 1e300, w3hi = -1e300, s3m = 0, u1m = 0, u3m = 0;
         for (int m = 0; m < 16; ++m) {
             w1lo = fmin(w1lo, w1s[m]);
@@ -805,6 +810,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         for (int r = 0; r < 4; ++r) {
             const S re = (S)cr[r], im = (S)ci[r];
             const S psisq = sc * (re * re + im * im);
+            if constexpr (DUMP)
+                ((S *)a.psi_out)[(((size_t)t * NB + 16 * bb + c16) * NB + b2) * NB + b3w + out_row<PREC>(lane, r)] = psisq;
             S sig2 = itype2 ? sg12 + s3v[r] : sfix2;
             if (itype6) {
                 const S s6 = sigma_type6<S>(u6 + 3 * c16, u6 + 3 * (16 + b2), u6 + 3 * (16 + NB + b3w + out_row<PREC>(lane, r)),
@@ -921,7 +928,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     }
 }
 
-template <int PREC, int NB, int NTY>
+template <int PREC, int NB, int NTY, bool DUMP = false>
 int run_chain(const MfmaPlan &p, hipStream_t st) {
     constexpr int NW = NB / 16;
     UwArgs u;
@@ -934,7 +941,7 @@ int run_chain(const MfmaPlan &p, hipStream_t st) {
     hipError_t e = hipFuncSetAttribute((const void *)k_uw<PREC, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)sh_uw);
     if (e != hipSuccess) return device_error(e, "k_uw LDS size", __FILE__, __LINE__);
-    e = hipFuncSetAttribute((const void *)k_psi<PREC, NB, NTY>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    e = hipFuncSetAttribute((const void *)k_psi<PREC, NB, NTY, DUMP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sh_psi);
     if (e != hipSuccess) return device_error(e, "k_psi LDS size", __FILE__, __LINE__);
     MfmaEpiArgs ea = p.epi;
@@ -948,7 +955,7 @@ int run_chain(const MfmaPlan &p, hipStream_t st) {
         ea.t0 = (int)t0;
         ea.nt = nt;
         const int ngrp = (nt + 7) / 8;
-        hipLaunchKernelGGL((k_psi<PREC, NB, NTY>), dim3(ngrp * 8 * NW), dim3(64 * NW), sh_psi, st, ea, p.E1re, p.E1im,
+        hipLaunchKernelGGL((k_psi<PREC, NB, NTY, DUMP>), dim3(ngrp * 8 * NW), dim3(64 * NW), sh_psi, st, ea, p.E1re, p.E1im,
                            (const __bf16 *)p.Eti, (const void *)p.work);
         if (ev) (void)hipEventRecord(ev[2], st);
         e = hipGetLastError();
@@ -996,6 +1003,17 @@ int dispatch_nb(const MfmaPlan &p, hipStream_t st) {
     }
 }
 
+template <int PREC>
+int dispatch_psi2(const MfmaPlan &p, hipStream_t st) {
+    int r = prepare<PREC>(p, st);
+    if (r) return r;
+    switch (p.nb) {
+    case 48: return run_chain<PREC, 48, 4, true>(p, st);
+    case 96: return run_chain<PREC, 96, 4, true>(p, st);
+    default: return 5;
+    }
+}
+
 // LDS bytes of the two kernels for band count nb and temperature block nty.
 size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     const size_t es = (precision == TC3_PREC_FP64) ? 8 : 4;
@@ -1030,6 +1048,16 @@ int mfma_temperature_block(int nb, int nT, int precision) {
     for (int nty : {16, 8, 4})
         if (lds_need(nb, nty, precision, 0, 0) + 8 * 1024 <= 160 * 1024) return nty;
     return 0;
+}
+
+int run_mfma_psi2(const MfmaPlan &p, int precision, hipStream_t st) {
+    if (p.nty != 4 || p.epi.psi_out == nullptr) return 5;
+    if (lds_need(p.nb, 4, precision, p.maxblk, p.maxa1) > 160 * 1024) return 5;
+    switch (precision) {
+    case TC3_PREC_FP64: return dispatch_psi2<0>(p, st);
+    case TC3_PREC_FP32: return dispatch_psi2<1>(p, st);
+    default: return dispatch_psi2<2>(p, st);
+    }
 }
 
 int run_mfma_chain(const MfmaPlan &p, int precision, hipStream_t st) {

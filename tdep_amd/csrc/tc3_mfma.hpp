@@ -25,6 +25,7 @@ struct MfmaEpiArgs {
     const double *u_irr, *u_full;   // integration type 6: reduced group velocities [q][mode][3]
     double thresh6;                 // and the baseline width
     const float *tw;                // Monte-Carlo draw counts [t][b1], or null (all 1)
+    void *psi_out;                  // run_mfma_psi2 only: |Psi|^2 [t][b1][b2][b3], epilogue type
 };
 
 struct MfmaPlan {
@@ -49,5 +50,9 @@ int mfma_temperature_block(int nb, int nT, int precision);
 size_t mfma_elem_size(int precision);
 size_t mfma_split_elems(int nb);   // bf16 elements of one q-point's split planes
 int run_mfma_chain(const MfmaPlan &p, int precision, hipStream_t st);
+// |Psi|^2 with its prefactor (psi_scale, perm) of the plan's triplets, every element, into
+// p.epi.psi_out ([t][b1][b2][b3], FP64 in FP64 mode, FP32 otherwise), by the production
+// k_uw and the k_psi contraction; p.batch triplets per launch pair.
+int run_mfma_psi2(const MfmaPlan &p, int precision, hipStream_t st);
 
 }  // namespace tc3

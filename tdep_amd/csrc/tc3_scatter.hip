@@ -813,6 +813,136 @@ bool xi_source_map(const tc3::TripletSet &tr, const tc3::OrbitSet &orb, int nirr
 
 }  // namespace
 
+// Nonzero (a2, a3) IFC blocks of every atom a1, in (a1, a2, a3) order, with the offsets of
+// each a1 and each (a1, a2) and the most blocks of one a1 (the MFMA kernels' Phi blocks).
+static void mfma_block_lists(const tc3_ctx *c, std::vector<int4> &blist, std::vector<int> &a1_boff,
+                             std::vector<int> &a12_boff, int &maxblk) {
+    const int na = c->cr.na;
+    blist.clear();
+    a1_boff.assign(na + 1, 0);
+    a12_boff.assign((size_t)na * na + 1, 0);
+    maxblk = 0;
+    for (int a1 = 0; a1 < na; ++a1) {
+        a1_boff[a1] = (int)blist.size();
+        for (int a2 = 0; a2 < na; ++a2) {
+            a12_boff[(size_t)a1 * na + a2] = (int)blist.size();
+            for (int a3 = 0; a3 < na; ++a3) {
+                const size_t bk = ((size_t)a1 * na + a2) * na + a3;
+                if (c->blk3_off[bk + 1] > c->blk3_off[bk])
+                    blist.push_back(make_int4(a2, a3, c->blk3_off[bk], c->blk3_off[bk + 1]));
+            }
+        }
+        maxblk = std::max(maxblk, (int)blist.size() - a1_boff[a1]);
+    }
+    a1_boff[na] = (int)blist.size();
+    a12_boff[(size_t)na * na] = (int)blist.size();
+}
+
+int run_psi2(tc3_ctx *c, int precision, int n, const int64_t *trips, double *out, int *qpts) {
+    const int na = c->cr.na, nb = 3 * na;
+    const int nirr = c->mesh.nirr, nfull = c->mesh.nfull;
+    const TripletSet &tr = c->trip_full;
+    const long ntf = (long)tr.q1.size();
+    if (!mfma_supported(nb, precision))
+        return c->fail(5, "the |Psi|^2 export covers the MFMA contraction (48 and 96 bands)");
+    if (n <= 0 || !trips || !out) return c->fail(5, "need at least one triplet and an output buffer");
+    for (int i = 0; i < n; ++i)
+        if (trips[i] < 0 || trips[i] >= ntf) return c->fail(5, "triplet index out of range");
+    hipStream_t st = c->stream;
+    hipError_t e = hipSuccess;
+#define TRY(x) do { e = (x); if (e != hipSuccess) return device_error(e, #x, __FILE__, __LINE__); } while (0)
+    auto up = [&](const char *name, void **p, const void *src, size_t bytes) -> hipError_t {
+        hipError_t r = c->pool_get(name, bytes, p);
+        if (r == hipSuccess) r = hipMemcpyAsync(*p, src, bytes, hipMemcpyHostToDevice, st);
+        return r;
+    };
+    // the listed triplets, each with permutation factor 1 and no equivalent pairs (nothing
+    // of the scattering matrix is written: every row index is -1)
+    std::vector<int> tq1(n), tq2(n), tq3(n), eqo(n + 1, 0), rowof((size_t)nirr * nb, -1);
+    for (int i = 0; i < n; ++i) {
+        const long t = (long)trips[i];
+        tq1[i] = tr.q1[t];
+        tq2[i] = tr.q2[t];
+        tq3[i] = tr.q3[t];
+        if (qpts) {
+            qpts[3 * i] = tq1[i];
+            qpts[3 * i + 1] = tq2[i];
+            qpts[3 * i + 2] = tq3[i];
+        }
+    }
+    const std::vector<double> perm(n, 1.0), be((size_t)nirr * nb, 0.0), sg((size_t)nirr * nb, 1.0);
+    std::vector<int4> blist;
+    std::vector<int> a1_boff, a12_boff;
+    int maxblk = 0, maxa1 = 0;
+    mfma_block_lists(c, blist, a1_boff, a12_boff, maxblk);
+    for (int a = 0; a < na; ++a) maxa1 = std::max(maxa1, c->a1_off[a + 1] - c->a1_off[a]);
+    int *d_tq1, *d_tq2, *d_tq3, *d_eqo, *d_rowof, *d_a1off, *d_a1boff, *d_a12boff;
+    int4 *d_blist;
+    double *d_perm, *d_be, *d_sg, *d_qcart, *d_lv2, *d_lv3, *d_mwm, *d_gpart;
+    TRY(up("p2_tq1", (void **)&d_tq1, tq1.data(), sizeof(int) * n));
+    TRY(up("p2_tq2", (void **)&d_tq2, tq2.data(), sizeof(int) * n));
+    TRY(up("p2_tq3", (void **)&d_tq3, tq3.data(), sizeof(int) * n));
+    TRY(up("p2_eqo", (void **)&d_eqo, eqo.data(), sizeof(int) * (n + 1)));
+    TRY(up("p2_rowof", (void **)&d_rowof, rowof.data(), sizeof(int) * rowof.size()));
+    TRY(up("p2_perm", (void **)&d_perm, perm.data(), sizeof(double) * n));
+    TRY(up("p2_be", (void **)&d_be, be.data(), sizeof(double) * be.size()));
+    TRY(up("p2_sg", (void **)&d_sg, sg.data(), sizeof(double) * sg.size()));
+    TRY(up("p2_blist", (void **)&d_blist, blist.data(), sizeof(int4) * std::max<size_t>(blist.size(), 1)));
+    TRY(up("p2_a1boff", (void **)&d_a1boff, a1_boff.data(), sizeof(int) * a1_boff.size()));
+    TRY(up("p2_a12boff", (void **)&d_a12boff, a12_boff.data(), sizeof(int) * a12_boff.size()));
+    TRY(up("p2_a1off", (void **)&d_a1off, c->a1_off.data(), sizeof(int) * (na + 1)));
+    TRY(up("p2_qcart", (void **)&d_qcart, c->mesh.qcart.data(), sizeof(double) * 3 * nfull));
+    TRY(up("p2_lv2", (void **)&d_lv2, c->t_lv2.data(), sizeof(double) * 3 * c->nfc3));
+    TRY(up("p2_lv3", (void **)&d_lv3, c->t_lv3.data(), sizeof(double) * 3 * c->nfc3));
+    TRY(up("p2_mwm", (void **)&d_mwm, c->t_mwm.data(), sizeof(double) * 27 * c->nfc3));
+    TRY(c->pool_get("p2_gpart", sizeof(double) * (size_t)n * nb, (void **)&d_gpart));
+    TRY(hipStreamSynchronize(st));   // the host vectors above are sources of the uploads
+    const size_t es = mfma_elem_size(precision);
+    const long batch = std::min(n, 16);
+    MfmaPlan mp{};
+    mp.na = na; mp.nb = nb; mp.nirr = nirr; mp.nfull = nfull; mp.maxa1 = std::max(maxa1, 1);
+    mp.maxblk = std::max(maxblk, 1); mp.nty = 4; mp.ntrip = n; mp.batch = batch;
+    mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
+    TRY(c->pool_get("p2_E1re", es * (size_t)nirr * nb * nb, &mp.E1re));
+    TRY(c->pool_get("p2_E1im", es * (size_t)nirr * nb * nb, &mp.E1im));
+    TRY(c->pool_get("p2_Efre", es * (size_t)nfull * nb * nb, &mp.Efre));
+    TRY(c->pool_get("p2_Efim", es * (size_t)nfull * nb * nb, &mp.Efim));
+    if (precision == TC3_PREC_BF16X3) {
+        TRY(c->pool_get("p2_Eti", 2 * mfma_split_elems(nb) * nirr, &mp.Eti));
+        TRY(c->pool_get("p2_Etf", 2 * mfma_split_elems(nb) * nfull, &mp.Etf));
+    }
+    const size_t nb3 = (size_t)nb * nb * nb;
+    TRY(c->pool_get("p2_work", 2 * nb3 * es * batch, &mp.work));
+    mp.tq2 = d_tq2; mp.tq3 = d_tq3; mp.a1off = d_a1off;
+    mp.blist = d_blist; mp.a1_boff = d_a1boff; mp.a12_boff = d_a12boff;
+    mp.qcart = d_qcart; mp.lv2 = d_lv2; mp.lv3 = d_lv3; mp.mwm = d_mwm;
+    mp.ev = nullptr;
+    MfmaEpiArgs &me = mp.epi;
+    me.nb = nb; me.nT = 1; me.nirr = nirr; me.integrationtype = 1; me.gT = 1; me.T0 = 0;
+    me.sigma_fixed = 1.0;
+    // threephonon_prefactor * mcg%weight of the full grid (scattering_threephonon.f90:112)
+    me.psi_scale = (kPi / 16.0) / (double)nfull;
+    me.tq1 = d_tq1; me.tq2 = d_tq2; me.tq3 = d_tq3; me.eq_off = d_eqo; me.eq_q2 = d_eqo; me.eq_q3 = d_eqo;
+    me.irr_of_full = c->d_irr_of_full; me.row_of = d_rowof;
+    me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.be = d_be; me.sigsq = d_sg;
+    me.xi = nullptr; me.gpart = d_gpart; me.xi_row = 0; me.xi_T = 0;
+    me.comp = nullptr; me.comp_T = 0; me.u_irr = nullptr; me.u_full = nullptr; me.thresh6 = 0.0; me.tw = nullptr;
+    TRY(c->pool_get("p2_out", es * nb3 * n, &me.psi_out));
+    const int r = run_mfma_psi2(mp, precision, st);
+    if (r) return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r;
+    if (precision == TC3_PREC_FP64) {
+        TRY(hipMemcpyAsync(out, me.psi_out, sizeof(double) * nb3 * n, hipMemcpyDeviceToHost, st));
+        TRY(hipStreamSynchronize(st));
+    } else {
+        std::vector<float> h(nb3 * n);
+        TRY(hipMemcpyAsync(h.data(), me.psi_out, sizeof(float) * h.size(), hipMemcpyDeviceToHost, st));
+        TRY(hipStreamSynchronize(st));
+        for (size_t i = 0; i < h.size(); ++i) out[i] = (double)h[i];
+    }
+#undef TRY
+    return 0;
+}
+
 int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     const int na = c->cr.na, nb = 3 * na;
     const int nirr = c->mesh.nirr, nfull = c->mesh.nfull, nT = c->nT;
@@ -1022,24 +1152,10 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         TRY(hipMemsetAsync(c->d_xi, 0, (size_t)c->xi_es * xi_T * nT, st));
         c->batch = 0;
     } else if (use_mfma) {
-        // nonzero (a2, a3) IFC blocks of every atom a1, in (a1, a2, a3) order
         std::vector<int4> blist;
-        std::vector<int> a1_boff(na + 1, 0), a12_boff((size_t)na * na + 1, 0);
+        std::vector<int> a1_boff, a12_boff;
         int maxblk = 0;
-        for (int a1 = 0; a1 < na; ++a1) {
-            a1_boff[a1] = (int)blist.size();
-            for (int a2 = 0; a2 < na; ++a2) {
-                a12_boff[(size_t)a1 * na + a2] = (int)blist.size();
-                for (int a3 = 0; a3 < na; ++a3) {
-                    const size_t bk = ((size_t)a1 * na + a2) * na + a3;
-                    if (c->blk3_off[bk + 1] > c->blk3_off[bk])
-                        blist.push_back(make_int4(a2, a3, c->blk3_off[bk], c->blk3_off[bk + 1]));
-                }
-            }
-            maxblk = std::max(maxblk, (int)blist.size() - a1_boff[a1]);
-        }
-        a1_boff[na] = (int)blist.size();
-        a12_boff[(size_t)na * na] = (int)blist.size();
+        mfma_block_lists(c, blist, a1_boff, a12_boff, maxblk);
         int4 *d_blist;
         int *d_a1boff, *d_a12boff;
         TRY(table("blist", (void **)&d_blist, blist.data(), sizeof(int4) * blist.size()));

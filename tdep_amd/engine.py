@@ -291,6 +291,20 @@ class Engine:
         return dict(linewidth=dlw.reshape(nT, self.nirr, self.nb), omega=dom.reshape(self.nirr, self.nb),
                     velocity=dv.reshape(self.nirr, self.nb, 3), kappa_sma=dk.reshape(nT, 3, 3).transpose(0, 2, 1).copy())
 
+    def psi2(self, triplets, precision="fp64"):
+        """threephonon_prefactor |Psi|^2 / N_q [n, nb, nb, nb] (b1, b2, b3) of the irreducible
+        triplets with the given indices (0 .. ntriplets - 1 of the full grid), formed by the
+        contraction kernels in the given precision (tc3_get_psi2; 48 and 96 bands), and their
+        (q1 irreducible, q2, q3) indices [n, 3]."""
+        t = np.ascontiguousarray(np.asarray(triplets, dtype=np.int64).reshape(-1))
+        n, nb = len(t), self.nb
+        out = np.zeros(n * nb ** 3)
+        q = np.zeros(3 * n, dtype=np.int32)
+        self._check(self.lib.tc3_get_psi2(self.ctx, PREC[precision], n,
+                                          t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), dptr(out),
+                                          q.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out.reshape(n, nb, nb, nb), q.reshape(n, 3)
+
     def timings(self):
         t = np.zeros(8)
         self._check(self.lib.tc3_get_timings(self.ctx, dptr(t), 8))

@@ -228,6 +228,16 @@ module tc3_api
             real(c_double), intent(in) :: dfc2(3, 3, *), dfc3(3, 3, 3, *)
             real(c_double), intent(out) :: dlinewidth(*), domega(*), dvel(3, *), dkappa_sma(3, 3, *)
         end function
+
+        integer(c_int) function tc3_get_psi2(ctx, precision, n, triplets, psi2, qpoints) &
+                bind(C, name="tc3_get_psi2")
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: precision, n
+            integer(c_int64_t), intent(in) :: triplets(*)
+            real(c_double), intent(out) :: psi2(*)
+            type(c_ptr), value :: qpoints
+        end function
     end interface
 
 contains

@@ -56,6 +56,8 @@ SIGNATURES = {
     "tc3_adjoint_gamma": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]),
     "tc3_adjoint_kappa_rta": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]),
     "tc3_jvp": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+    "tc3_get_psi2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                    _c_dbl_p, _c_int_p]),
 }
 
 
