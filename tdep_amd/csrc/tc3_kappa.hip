@@ -192,6 +192,16 @@ void od_velocity_host(const tc3_ctx *c, int iq, zc *V) {
     }
 }
 
+// lo_harmonic_oscillator_cv (gottochblandat_physics.f90:331-367), as cv_host
+__device__ double cv_dev(double T, double om) {
+    const double x = om / T / kKbHartree;
+    if (x > 200.0) return 0.0;
+    if (x > 21.0) return kKbHartree * x * x * exp(-x);
+    if (x < 2e-2) return kKbHartree * (1.0 - x * x / 12.0 + x * x * x * x / 240.0);
+    const double ex = exp(x);
+    return kKbHartree * x * x * ex / ((ex - 1) * (ex - 1));
+}
+
 struct OdArgs {
     int nb, na;
     const double *qv, *qirr;        // [nirr][3]: 2 pi q (chopped) and q of the irreducible points
@@ -208,15 +218,21 @@ struct OdArgs {
 
 __device__ inline cd zmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-constexpr int kOdSlots = 9;   // nb^2 / 256 entries of V per thread (nb <= 48)
+constexpr int kOdSlots = 9;   // nb^2 / 256 entries of V per thread in LDS mode (nb <= 48)
 
-// Workgroup per (irreducible q, direction z): G_z, egw and G_z egw in LDS (3 nb^2 complex),
-// the sums over the little group in registers, in the order of od_velocity_host.
-__global__ void __launch_bounds__(256) k_od_velocity(OdArgs a) {
+// Workgroup per (irreducible q, direction z).  SH: G_z, egw and G_z egw in LDS (3 nb^2
+// complex) and the sums over the little group in registers; otherwise (any nb) the three
+// matrices in the workgroup's slice of a global scratch and the sums accumulated in V, each
+// entry by the thread that owns it.  The operations and their order are the same in both
+// forms, in the order of od_velocity_host.
+template <bool SH>
+__global__ void __launch_bounds__(256) k_od_velocity(OdArgs a, cd *__restrict__ scr) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nb = a.nb, na = a.na, nb2 = nb * nb;
     const int iq = blockIdx.x / 3, z = blockIdx.x % 3;
-    cd *G = (cd *)smem, *E = G + nb2, *Tm = E + nb2;
+    cd *G = SH ? (cd *)smem : scr + (size_t)blockIdx.x * 3 * nb2;
+    cd *E = G + nb2, *Tm = E + nb2;
+    cd *Vz = a.V + ((size_t)iq * 3 + z) * nb2;
     const double *qv = a.qv + 3 * iq, *q = a.qirr + 3 * iq;
     for (int x = threadIdx.x; x < nb2; x += blockDim.x) {
         const int r = x / nb, col = x % nb;
@@ -232,9 +248,10 @@ __global__ void __launch_bounds__(256) k_od_velocity(OdArgs a) {
             g.y += t.y * m;
         }
         G[x] = g;
+        if constexpr (!SH) Vz[x] = {0, 0};
     }
-    cd acc[kOdSlots];
-    for (int s = 0; s < kOdSlots; ++s) acc[s] = {0, 0};
+    cd acc[SH ? kOdSlots : 1];
+    for (int s = 0; s < (SH ? kOdSlots : 1); ++s) acc[s] = {0, 0};
     for (int l = a.lg_off[iq]; l < a.lg_off[iq + 1]; ++l) {
         const int o = a.lg[l];
         const double *m = a.opm + 9 * o, *tr = a.optr + 3 * o;
@@ -268,9 +285,9 @@ __global__ void __launch_bounds__(256) k_od_velocity(OdArgs a) {
             Tm[x] = s;
         }
         __syncthreads();
-        for (int sl = 0; sl < kOdSlots; ++sl) {
+        for (int sl = 0;; ++sl) {
             const int x = threadIdx.x + sl * blockDim.x;
-            if (x >= nb2) break;
+            if (x >= nb2 || (SH && sl >= kOdSlots)) break;
             const int i = x / nb, j = x % nb;
             cd s = {0, 0};
             for (int r = 0; r < nb; ++r) {
@@ -279,15 +296,73 @@ __global__ void __launch_bounds__(256) k_od_velocity(OdArgs a) {
                 s.x += t.x;
                 s.y += t.y;
             }
-            acc[sl].x += s.x;
-            acc[sl].y += s.y;
+            if constexpr (SH) {
+                acc[sl].x += s.x;
+                acc[sl].y += s.y;
+            } else {
+                cd v = Vz[x];
+                v.x += s.x;
+                v.y += s.y;
+                Vz[x] = v;
+            }
         }
         __syncthreads();
     }
-    for (int sl = 0; sl < kOdSlots; ++sl) {
-        const int x = threadIdx.x + sl * blockDim.x;
-        if (x < nb2) a.V[((size_t)iq * 3 + z) * nb2 + x] = acc[sl];
+    if constexpr (SH)
+        for (int sl = 0; sl < kOdSlots; ++sl) {
+            const int x = threadIdx.x + sl * blockDim.x;
+            if (x < nb2) Vz[x] = acc[sl];
+        }
+}
+
+// The coherence velocities of every mode pair on the device, vel[q][j][k][z] = Re V_z(q)[j][k]
+// / |lg(q)|, below vtol set to zero (kappa.f90:203-213), as od_real_velocities.
+__global__ void k_od_real(int nirr, int nb, const int *__restrict__ lg_off, const cd *__restrict__ V, double vtol,
+                          double *__restrict__ vel) {
+    const size_t nb2 = (size_t)nb * nb;
+    const size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (x >= (size_t)nirr * nb2 * 3) return;
+    const int z = (int)(x % 3);
+    const size_t jk = (x / 3) % nb2, iq = x / 3 / nb2;
+    const double ng = (double)(lg_off[iq + 1] - lg_off[iq]);
+    const double re = V[(iq * 3 + z) * nb2 + jk].x / ng;
+    vel[x] = fabs(re) < vtol ? 0.0 : re;
+}
+
+// M[T][q][9]: the sum over the mode pairs j != k of q, both above lo_freqtol in frequency and
+// linewidth, of w v_jk v_jk^T with w = f0 (G_j + G_k) / ((G_j + G_k)^2 + (om_j - om_k)^2),
+// f0 the mean heat capacity (kB classically) (kappa.f90:220-287, as kappa_offdiag); workgroup
+// per (q, T), per-thread sums over a fixed pair set added by a fixed tree.
+__global__ void __launch_bounds__(256) k_od_pairsum(int nirr, int nb, int classical, const double *__restrict__ temps,
+                                                    const double *__restrict__ om_, const double *__restrict__ lw_,
+                                                    const double *__restrict__ vel_, double *__restrict__ M) {
+    __shared__ double red[9][256];
+    const int iq = blockIdx.x, T = blockIdx.y;
+    const double tk = temps[T];
+    const double *om = om_ + (size_t)iq * nb, *lw = lw_ + ((size_t)T * nirr + iq) * nb;
+    const double *vel = vel_ + (size_t)iq * nb * nb * 3;
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < nb * nb; p += blockDim.x) {
+        const int j = p / nb, k = p % nb;
+        if (j == k) continue;
+        const double o1 = om[j], o2 = om[k], t1 = lw[j], t2 = lw[k];
+        if (o1 < kFreqTol || t1 < kFreqTol || o2 < kFreqTol || t2 < kFreqTol) continue;
+        const double c1 = classical ? 0.0 : cv_dev(tk, o1), c2 = classical ? 0.0 : cv_dev(tk, o2);
+        const double f0 = classical ? kKbHartree : 0.5 * (c1 + c2);
+        const double tau = (t1 + t2) / ((t1 + t2) * (t1 + t2) + (o1 - o2) * (o1 - o2));
+        const double w = tau * f0;
+        const double *v = vel + (size_t)p * 3;
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) m[3 * x + y] += w * v[x] * v[y];
     }
+    for (int i = 0; i < 9; ++i) red[i][threadIdx.x] = m[i];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int i = 0; i < 9; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) M[((size_t)T * nirr + iq) * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // The coherence velocities of every mode pair, vel[q][j][k][z] = Re V_z(q)[j][k] / |lg(q)|,
@@ -307,18 +382,24 @@ void od_real_velocities(const tc3_ctx *c, const std::vector<zc> &V, std::vector<
     }
 }
 
-// V of every irreducible point: on the device when 3 nb^2 complex values fit the LDS and nb^2
-// <= 256 kOdSlots, else (or with TC3_OD_HOST set) on the host.
-int od_velocities(tc3_ctx *c, std::vector<double> &vel) {
-    const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr, nsym = (int)c->ops.size();
+// Host form of the coherence velocities (TC3_OD_HOST; the check of the device form).
+void od_velocities_host(tc3_ctx *c, std::vector<double> &vel) {
+    const int nb = c->nb, nirr = c->mesh.nirr;
     const size_t nb2 = (size_t)nb * nb;
     std::vector<zc> V((size_t)nirr * 3 * nb2, zc(0, 0));
+    for (int iq = 0; iq < nirr; ++iq) od_velocity_host(c, iq, V.data() + (size_t)iq * 3 * nb2);
+    od_real_velocities(c, V, vel);
+}
+
+// The off-diagonal term on the device for every temperature: the coherence velocities V of
+// every irreducible point (k_od_velocity: LDS form when 3 nb^2 complex values fit and nb^2 <=
+// 256 kOdSlots, else the global-scratch form), their real parts (k_od_real) and the pair sums
+// M[T][q][9] (k_od_pairsum), downloaded for the star rotation.
+int od_device(tc3_ctx *c, std::vector<double> &M) {
+    const int na = c->cr.na, nb = c->nb, nirr = c->mesh.nirr, nsym = (int)c->ops.size(), nT = c->nT;
+    const size_t nb2 = (size_t)nb * nb;
     const size_t lds = 3 * nb2 * sizeof(cd);
-    if (nb2 > 256 * (size_t)kOdSlots || lds > 150 * 1024 || std::getenv("TC3_OD_HOST") != nullptr) {
-        for (int iq = 0; iq < nirr; ++iq) od_velocity_host(c, iq, V.data() + (size_t)iq * 3 * nb2);
-        od_real_velocities(c, V, vel);
-        return 0;
-    }
+    const bool sh = nb2 <= 256 * (size_t)kOdSlots && lds <= 150 * 1024;
     hipStream_t st = c->stream;
     hipError_t e;
 #define TRY(x) do { e = (x); if (e != hipSuccess) return device_error(e, #x, __FILE__, __LINE__); } while (0)
@@ -369,30 +450,71 @@ int od_velocities(tc3_ctx *c, std::vector<double> &vel) {
     TRY(up("od_mw", mw.data(), sizeof(double) * nb, &p)); a.mw = (const double *)p;
     a.egv = c->d_egv_irr;
     a.om = c->d_om_irr;
-    TRY(c->pool_get("od_V", sizeof(cd) * V.size(), &p));
+    TRY(c->pool_get("od_V", sizeof(cd) * (size_t)nirr * 3 * nb2, &p));
     a.V = (cd *)p;
-    TRY(hipFuncSetAttribute((const void *)k_od_velocity, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_od_velocity, dim3(3 * nirr), dim3(256), lds, st, a);
+    if (sh) {
+        TRY(hipFuncSetAttribute((const void *)k_od_velocity<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_od_velocity<true>, dim3(3 * nirr), dim3(256), lds, st, a, (cd *)nullptr);
+    } else {
+        cd *scr;
+        TRY(c->pool_get("od_scr", sizeof(cd) * 3 * nb2 * 3 * (size_t)nirr, (void **)&scr));
+        hipLaunchKernelGGL(k_od_velocity<false>, dim3(3 * nirr), dim3(256), 0, st, a, scr);
+    }
     TRY(hipGetLastError());
-    TRY(hipMemcpyAsync(V.data(), a.V, sizeof(cd) * V.size(), hipMemcpyDeviceToHost, st));
+    double *d_vel, *d_lw, *d_T, *d_M;
+    const size_t nvel = (size_t)nirr * nb2 * 3;
+    TRY(c->pool_get("od_vel", sizeof(double) * nvel, (void **)&d_vel));
+    const double vtol = 1e-10 / (kGroupvelAuToMs / 1000.0);
+    hipLaunchKernelGGL(k_od_real, dim3((unsigned)((nvel + 255) / 256)), dim3(256), 0, st, nirr, nb, a.lg_off, a.V,
+                       vtol, d_vel);
+    TRY(up("od_lw", c->linewidth.data(), sizeof(double) * (size_t)nT * nirr * nb, (void **)&d_lw));
+    TRY(up("od_T", c->temps.data(), sizeof(double) * nT, (void **)&d_T));
+    TRY(c->pool_get("od_M", sizeof(double) * 9 * (size_t)nT * nirr, (void **)&d_M));
+    hipLaunchKernelGGL(k_od_pairsum, dim3(nirr, nT), dim3(256), 0, st, nirr, nb, (int)c->classical, d_T, c->d_om_irr,
+                       d_lw, d_vel, d_M);
+    TRY(hipGetLastError());
+    M.assign(9 * (size_t)nT * nirr, 0.0);
+    TRY(hipMemcpyAsync(M.data(), d_M, sizeof(double) * M.size(), hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
 #undef TRY
-    od_real_velocities(c, V, vel);
     return 0;
 }
 
-// The temperature-dependent part of get_kappa_offdiag (kappa.f90:220-287) for the velocities
-// vel of od_velocities: the sum over the star of q of (m v)(m v)^T is linear in v v^T, so the
-// weighted v v^T of all mode pairs of a q is summed first and rotated through the star once.
+// The star rotation of get_kappa_offdiag (kappa.f90:220-287): the sum over the star of q of
+// (m v)(m v)^T is linear in v v^T, so the weighted v v^T of all mode pairs of a q (M[q][9]) is
+// rotated through the star once.
+void od_rotate(const tc3_ctx *c, const double *Mq, double *kod) {
+    const int nirr = c->mesh.nirr;
+    double k[9] = {0};
+    for (int iq = 0; iq < nirr; ++iq) {
+        const double *M = Mq + 9 * (size_t)iq;
+        const auto &so = c->mesh.star_ops[iq];
+        const double pref = c->mesh.weight_irr[iq] / c->cr.volume / (double)so.size();
+        for (int s : so) {
+            const double *m = c->ops[std::abs(s) - 1].m;
+            double mM[9];   // m M
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y)
+                    mM[3 * x + y] = m[3 * x] * M[y] + m[3 * x + 1] * M[3 + y] + m[3 * x + 2] * M[6 + y];
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y)
+                    k[3 * x + y] += pref * (mM[3 * x] * m[3 * y] + mM[3 * x + 1] * m[3 * y + 1] + mM[3 * x + 2] * m[3 * y + 2]);
+        }
+    }
+    chop_kappa(k);
+    std::memcpy(kod, k, sizeof(k));
+}
+
+// The temperature-dependent part of get_kappa_offdiag on the host (TC3_OD_HOST) for the
+// velocities of od_velocities_host: the pair sums of every q (as k_od_pairsum), then od_rotate.
 void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, const double *vel, double *kod) {
     const int nb = c->nb, nirr = c->mesh.nirr;
-    double k[9] = {0};
-    std::vector<double> cv(nb);
+    std::vector<double> Mq(9 * (size_t)nirr, 0.0), cv(nb);
     for (int iq = 0; iq < nirr; ++iq) {
         const double *om = &c->om_irr[(size_t)iq * nb];
         const double *lw = &c->linewidth[(size_t)iT * nirr * nb + (size_t)iq * nb];
         for (int j = 0; j < nb; ++j) cv[j] = (classical || om[j] < kFreqTol) ? 0.0 : cv_host(T, om[j]);
-        double M[9] = {0};
+        double *M = &Mq[9 * (size_t)iq];
         for (int j = 0; j < nb; ++j) {
             const double om1 = om[j];
             if (om1 < kFreqTol) continue;
@@ -412,21 +534,8 @@ void kappa_offdiag(const tc3_ctx *c, int iT, double T, bool classical, const dou
                     for (int y = 0; y < 3; ++y) M[3 * x + y] += w * v[x] * v[y];
             }
         }
-        const auto &so = c->mesh.star_ops[iq];
-        const double pref = c->mesh.weight_irr[iq] / c->cr.volume / (double)so.size();
-        for (int s : so) {
-            const double *m = c->ops[std::abs(s) - 1].m;
-            double mM[9];   // m M
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y)
-                    mM[3 * x + y] = m[3 * x] * M[y] + m[3 * x + 1] * M[3 + y] + m[3 * x + 2] * M[6 + y];
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y)
-                    k[3 * x + y] += pref * (mM[3 * x] * m[3 * y] + mM[3 * x + 1] * m[3 * y + 1] + mM[3 * x + 2] * m[3 * y + 2]);
-        }
     }
-    chop_kappa(k);
-    std::memcpy(kod, k, sizeof(k));
+    od_rotate(c, Mq.data(), kod);
 }
 
 constexpr int kBlock = 256;
@@ -690,15 +799,6 @@ __global__ void __launch_bounds__(256) k_kappa_sum(int nmode, int nsym, const do
 // scfcheck = mean over modes of |Fn - Fnb|^2 / |Fn|^2 (partials per point added in order),
 // stop when the last three add up to less than tol; after iteration 15 the mixing factor
 // shrinks by 2% per iteration while above 0.5.
-// lo_harmonic_oscillator_cv (gottochblandat_physics.f90:331-367), as cv_host
-__device__ double cv_dev(double T, double om) {
-    const double x = om / T / kKbHartree;
-    if (x > 200.0) return 0.0;
-    if (x > 21.0) return kKbHartree * x * x * exp(-x);
-    if (x < 2e-2) return kKbHartree * (1.0 - x * x / 12.0 + x * x * x * x / 240.0);
-    const double ex = exp(x);
-    return kKbHartree * x * x * ex / ((ex - 1) * (ex - 1));
-}
 
 // Heat capacity of every mode and temperature, and cv w_q / V, the weight of the kappa log
 // (zero below lo_freqtol or at T = 0).
@@ -1017,7 +1117,16 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(hipMemcpyAsync(d_star_off, star_off.data(), sizeof(int) * (nirr + 1), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_star, star.data(), sizeof(int) * star.size(), hipMemcpyHostToDevice, st));
     double t_bte = 0, t_od = 0;
-    std::vector<double> od_V;   // coherence velocities, formed on the first temperature that needs them
+    // off-diagonal term: on the device for all temperatures at once (the star rotation of the
+    // per-point sums on the host), or all on the host with TC3_OD_HOST
+    const bool od_host = std::getenv("TC3_OD_HOST") != nullptr;
+    std::vector<double> od_V, od_M;
+    if (offdiag && !od_host) {
+        const auto a = std::chrono::steady_clock::now();
+        const int r = od_device(c, od_M);
+        if (r) return r;
+        t_od += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    }
     // per temperature on the host: the off-diagonal term and the initial iteration state; the
     // heat capacities, the SMA kappa (get_kappa of F0) and the kappa of the final F come from
     // the device
@@ -1028,11 +1137,12 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         if (T < kTemperatureTol) continue;
         if (offdiag) {
             const auto a = std::chrono::steady_clock::now();
-            if (od_V.empty()) {
-                const int r = od_velocities(c, od_V);
-                if (r) return r;
+            if (od_host) {
+                if (od_V.empty()) od_velocities_host(c, od_V);
+                kappa_offdiag(c, iT, T, c->classical, od_V.data(), &c->kappa_od[9 * iT]);
+            } else {
+                od_rotate(c, &od_M[9 * (size_t)iT * nirr], &c->kappa_od[9 * iT]);
             }
-            kappa_offdiag(c, iT, T, c->classical, od_V.data(), &c->kappa_od[9 * iT]);
             t_od += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
         }
         if (maxiter <= 0) continue;

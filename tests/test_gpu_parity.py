@@ -57,10 +57,11 @@ def test_fp64_scattering_and_kappa_match_golden(inputs_dir, name, grid):
         assert e.niter[iT] == int(g[f"niter_{iT}"])
 
 
-@pytest.mark.parametrize("name,grid", [("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2))])
+@pytest.mark.parametrize("name,grid", [("rocksalt", (4, 4, 4)), ("sl16", (4, 4, 2)), ("sl32", (4, 4, 1))])
 def test_offdiag_device_velocities_match_host(inputs_dir, monkeypatch, name, grid):
-    """The coherence velocities formed on the device give the off-diagonal kappa of the host
-    loop (the path taken for more than 48 modes) to summation order."""
+    """The off-diagonal kappa formed on the device (coherence velocities in LDS up to 48 modes,
+    in global scratch at 96, pair sums per point) equals the host restatement's to summation
+    order."""
     e = _engine(inputs_dir, name, grid)
     e.scattering_rates([100.0, 300.0], precision="fp64")
     _, _, kd = e.kappa()
