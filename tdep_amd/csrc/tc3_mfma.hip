@@ -476,24 +476,77 @@ __device__ inline double gaussd(double x, double mu, double sigma) {
 }
 
 // d0 = g(w1; w3 - w2) - g(w1; w2 - w3), d1 = g(w1; w2 + w3) - g(w1; -w2 - w3) (get_dirac,
-// scattering_threephonon.f90:247-270) for sigma^2 = sig2, with g zero beyond 8 sigma.
-__device__ inline void delta_pair(double w1, double w2, double w3, double sig2, double &d0, double &d1) {
-    const double sig = sqrt(sig2);
-    d0 = gaussd(w1, -w2 + w3, sig) - gaussd(w1, w2 - w3, sig);
-    d1 = gaussd(w1, w2 + w3, sig) - gaussd(w1, -w2 - w3, sig);
+// scattering_threephonon.f90:247-270) for sigma^2 = sig2, with g zero beyond 8 sigma, for the
+// four elements (one b1, one b2, four b3) of a lane.  The four Gaussians are named by the
+// distance they test: A = w1 + w2 - w3, B = w1 - w2 + w3, C = w1 - w2 - w3, D = w1 + w2 + w3.
+// Bit k of the wave-uniform mask tm clear means term k lies beyond the 8 sigma cut for every
+// element of the wave's tile (interval bound in k_psi), where lo_gauss returns exactly 0: the
+// term is not evaluated, with bit-identical results.
+__device__ inline void delta4(double w1, double w2, const double *w3, const double *sig2, unsigned tm, double *d0,
+                              double *d1) {
+    double sig[4], gA[4] = {0, 0, 0, 0}, gB[4] = {0, 0, 0, 0}, gC[4] = {0, 0, 0, 0}, gD[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sig[r] = sqrt(sig2[r]);
+    if (tm & 1u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gA[r] = gaussd(w1, -w2 + w3[r], sig[r]);
+    }
+    if (tm & 2u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gB[r] = gaussd(w1, w2 - w3[r], sig[r]);
+    }
+    if (tm & 4u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gC[r] = gaussd(w1, w2 + w3[r], sig[r]);
+    }
+    if (tm & 8u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gD[r] = gaussd(w1, -w2 - w3[r], sig[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        d0[r] = gA[r] - gB[r];
+        d1[r] = gC[r] - gD[r];
+    }
 }
-__device__ inline void delta_pair(float x1, float x2, float x3, float sig2, float &d0, float &d1) {
-    const float inv = __builtin_amdgcn_rsqf(sig2);
-    const float ce = -0.72134752044448170f * inv * inv;  // -log2(e) / (2 sigma^2)
-    const float lim = 64.0f * sig2;                       // (8 sigma)^2
-    const float dd = x2 - x3, ss = x2 + x3;
-    auto g = [&](float d) {
+__device__ inline void delta4(float x1, float x2, const float *x3, const float *sig2, unsigned tm, float *d0,
+                              float *d1) {
+    float inv[4], ce[4], lim[4], dd[4], ss[4];
+    float gA[4] = {0, 0, 0, 0}, gB[4] = {0, 0, 0, 0}, gC[4] = {0, 0, 0, 0}, gD[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        inv[r] = __builtin_amdgcn_rsqf(sig2[r]);
+        ce[r] = -0.72134752044448170f * inv[r] * inv[r];   // -log2(e) / (2 sigma^2)
+        lim[r] = 64.0f * sig2[r];                           // (8 sigma)^2
+        dd[r] = x2 - x3[r];
+        ss[r] = x2 + x3[r];
+    }
+    auto g = [&](float d, int r) {
         const float d2 = d * d;
-        return d2 > lim ? 0.0f : __builtin_amdgcn_exp2f(ce * d2);
+        return d2 > lim[r] ? 0.0f : __builtin_amdgcn_exp2f(ce[r] * d2);
     };
-    const float nrm = 0.398942280401433f * inv;
-    d0 = nrm * (g(x1 + dd) - g(x1 - dd));
-    d1 = nrm * (g(x1 - ss) - g(x1 + ss));
+    if (tm & 1u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gA[r] = g(x1 + dd[r], r);
+    }
+    if (tm & 2u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gB[r] = g(x1 - dd[r], r);
+    }
+    if (tm & 4u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gC[r] = g(x1 - ss[r], r);
+    }
+    if (tm & 8u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gD[r] = g(x1 + ss[r], r);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float nrm = 0.398942280401433f * inv[r];
+        d0[r] = nrm * (gA[r] - gB[r]);
+        d1[r] = nrm * (gC[r] - gD[r]);
+    }
 }
 
 template <int NB> struct MaskWords { static constexpr int n = (NB + 31) / 32; };
@@ -679,9 +732,15 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     // the rounding of the element-wise test, decide per b2; the decision is wave-uniform.
     constexpr int NLW = (NB + 63) / 64;
     unsigned long long live[NLW];
+    // per Gaussian term (A, B, C, D of delta4) the b2 whose tile it reaches; live = their union
+    unsigned long long lterm[4][NLW];
     if constexpr (DUMP) {
 #pragma unroll
-        for (int wd = 0; wd < NLW; ++wd) live[wd] = (NB - 64 * wd >= 64) ? ~0ull : ((1ull << (NB - 64 * wd)) - 1);
+        for (int wd = 0; wd < NLW; ++wd) {
+            live[wd] = (NB - 64 * wd >= 64) ? ~0ull : ((1ull << (NB - 64 * wd)) - 1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lterm[k][wd] = live[wd];
+        }
     } else {
         double w1lo = 1e300, w1hi = -1e300, s1m = 0, w3lo = 1e300, w3hi = -1e300, s3m = 0, u1m = 0, u3m = 0;
         for (int m = 0; m < 16; ++m) {
@@ -700,7 +759,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
 #pragma unroll
         for (int wd = 0; wd < NLW; ++wd) {
             const int b2 = 64 * wd + lane;
-            bool lv = false;
+            bool lt[4] = {false, false, false, false};
             if (b2 < NB) {
                 const double w2 = w2s[b2];
                 // type 6: each pair width is at most sqrt(1/2) (|u_a|_inf + |u_b|_inf)
@@ -712,12 +771,19 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                                              : a.sigma_fixed;
                 const double L = 8.0 * smax * (1.0 + 1e-4) + 1e-6 * (fabs(w1hi) + fabs(w2) + fabs(w3hi));
                 auto hit = [&](double lo, double hi) { return lo <= L && hi >= -L; };
-                lv = hit(w1lo + w2 - w3hi, w1hi + w2 - w3lo) || hit(w1lo - w2 + w3lo, w1hi - w2 + w3hi) ||
-                     hit(w1lo - w2 - w3hi, w1hi - w2 - w3lo) || hit(w1lo + w2 + w3lo, w1hi + w2 + w3hi);
+                lt[0] = hit(w1lo + w2 - w3hi, w1hi + w2 - w3lo);   // A = w1 + w2 - w3
+                lt[1] = hit(w1lo - w2 + w3lo, w1hi - w2 + w3hi);   // B = w1 - w2 + w3
+                lt[2] = hit(w1lo - w2 - w3hi, w1hi - w2 - w3lo);   // C = w1 - w2 - w3
+                lt[3] = hit(w1lo + w2 + w3lo, w1hi + w2 + w3hi);   // D = w1 + w2 + w3
             }
-            const unsigned long long bl = __ballot(lv);
-            live[wd] = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(bl >> 32)) << 32) |
-                       (unsigned)__builtin_amdgcn_readfirstlane((unsigned)bl);
+            live[wd] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned long long bl = __ballot(lt[k]);
+                lterm[k][wd] = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(bl >> 32)) << 32) |
+                               (unsigned)__builtin_amdgcn_readfirstlane((unsigned)bl);
+                live[wd] |= lterm[k][wd];
+            }
         }
     }
     auto next_live = [&](int from) -> int {
@@ -805,25 +871,31 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
             s3v[0] = sv.x; s3v[1] = sv.y; s3v[2] = sv.z; s3v[3] = sv.w;
         }
         const S x2 = (PREC == 0) ? (S)w2s[b2] : (S)w2f[b2];
-        S P[4], M[4];
+        // Gaussian terms that reach this tile (wave-uniform)
+        const int wb = b2 >> 6, sb = b2 & 63;
+        const unsigned tm = (unsigned)((lterm[0][wb] >> sb) & 1ull) | ((unsigned)((lterm[1][wb] >> sb) & 1ull) << 1) |
+                            ((unsigned)((lterm[2][wb] >> sb) & 1ull) << 2) | ((unsigned)((lterm[3][wb] >> sb) & 1ull) << 3);
+        S P[4], M[4], psq[4], sgv[4], d0[4], d1[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const S re = (S)cr[r], im = (S)ci[r];
-            const S psisq = sc * (re * re + im * im);
+            psq[r] = sc * (re * re + im * im);
             if constexpr (DUMP)
-                ((S *)a.psi_out)[(((size_t)t * NB + 16 * bb + c16) * NB + b2) * NB + b3w + out_row<PREC>(lane, r)] = psisq;
-            S sig2 = itype2 ? sg12 + s3v[r] : sfix2;
+                ((S *)a.psi_out)[(((size_t)t * NB + 16 * bb + c16) * NB + b2) * NB + b3w + out_row<PREC>(lane, r)] = psq[r];
+            sgv[r] = itype2 ? sg12 + s3v[r] : sfix2;
             if (itype6) {
                 const S s6 = sigma_type6<S>(u6 + 3 * c16, u6 + 3 * (16 + b2), u6 + 3 * (16 + NB + b3w + out_row<PREC>(lane, r)),
                                             (S)a.thresh6);
-                sig2 = s6 * s6;
+                sgv[r] = s6 * s6;
             }
-            S d0, d1;
-            if constexpr (PREC == 0)
-                delta_pair(w1, x2, w3v[r], sig2, d0, d1);
-            else
-                delta_pair(x1, x2, w3v[r], sig2, d0, d1);
-            const S A = psisq * d0, B = psisq * d1;
+        }
+        if constexpr (PREC == 0)
+            delta4(w1, x2, w3v, sgv, tm, d0, d1);
+        else
+            delta4(x1, x2, w3v, sgv, tm, d0, d1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const S A = psq[r] * d0[r], B = psq[r] * d1[r];
             const bool on = (okm >> (sh3 + out_row<PREC>(lane, r))) & 1u;
             P[r] = on ? A + B : (S)0;
             M[r] = on ? B - A : (S)0;
