@@ -276,7 +276,7 @@ int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int 
  * compute_fourphonon_scattering (scattering_fourphonon.f90:1-328) then adds its linewidth
  * and scattering-matrix terms with the full q-grid for the quartets (-qg4ph equal to the
  * mesh unless tc3_set_montecarlo4 draws them), FP64 whatever the contraction precision;
- * cells of up to 2 atoms (6 bands, else tc3_scatter returns 5). */
+ * any cell of up to 32 atoms (96 bands; more: tc3_scatter returns 5). */
 int tc3_set_fc4(tc3_ctx *ctx, int nquart, const int *a1, const int *a2, const int *a3, const int *a4,
                 const double *lv2, const double *lv3, const double *lv4, const double *m);
 

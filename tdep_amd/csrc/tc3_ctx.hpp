@@ -41,6 +41,11 @@ struct tc3_ctx {
     // 5 ints each (q1, q2, q3, q4, multiplicity class), images 3 ints each from img_off
     std::vector<int> quart, quart_img_off, quart_img;
     long quart_gen = -1;
+    // the quartets the Monte-Carlo rows visit and their draw counts [quartet][b1] (rebuilt
+    // when the model or the draws change)
+    std::vector<int> mquart, mquart_img_off, mquart_img;
+    std::vector<float> mquart_tw;
+    long mquart_gen = -1;
     std::vector<int> t_src;                   // caller's index of each sorted triplet
 
     tc3::Mesh mesh;

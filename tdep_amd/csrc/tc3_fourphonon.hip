@@ -7,14 +7,18 @@
 // smeared-delta channels with the permutation multiplicities (:96-175) for every
 // temperature, summed over the other three bands per leg.  The host adds the terms of every
 // image to the row linewidths and to the off-diagonal scattering-matrix terms and averages
-// those over the little group (:143-205).  Bands up to 6 (phi4 of one quartet, 6^4 complex
-// values, sits in LDS twice).
+// those over the little group (:143-205).  Cells of one or two atoms (nb <= 6: phi4 of one
+// quartet, 6^4 complex values, sits in LDS twice) use k_fourphonon; larger cells the batched
+// kernels below (k_f4_phase, k_f4_phi, four k_f4_stage steps on FP64 MFMA, k_f4_epi), up to
+// 96 bands.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "tc3_chain.hpp"
@@ -193,6 +197,238 @@ __global__ void __launch_bounds__(kF4Block) k_fourphonon(F4Args a) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Cells of more than two atoms (nb > 6): quartets in batches, each step over the whole batch.
+//   k_f4_phase   phases exp(-2 pi i (q2.R2 + q3.R3 + q4.R4)) of every IFC entry, per quartet
+//   k_f4_phi     pretransform_phi4 (:459-506): phi4[ia][ib][ic][id], FP64 complex, dense (zero
+//                outside the atom blocks that hold IFCs)
+//   k_f4_stage   four contraction steps on FP64 MFMA, Y[r][b] = sum_k X[k][r] E[k][b] with X
+//                read as [nb][nb^3]: each step contracts the leading index with its leg's
+//                e / sqrt(w) and appends the band index last, so after ia, ib, ic, id the
+//                batch holds Psi[b1][b2][b3][b4] (unconjugated, :88-113 / :140-175)
+//   k_f4_epi     per (quartet, b1): the four occupation / smeared-delta channels with the
+//                permutation multiplicities for a block of temperatures and the sums per leg,
+//                in fixed order (the same outputs as k_fourphonon)
+constexpr int kF4Tb = 4;          // temperatures per epilogue pass
+constexpr int kF4EpiBlock = 256;
+
+__global__ void k_f4_phase(int nent, const int *__restrict__ fq, int t0, const double *__restrict__ qcart,
+                           const double *__restrict__ lv, cd *__restrict__ ph) {
+    const int tl = blockIdx.y, t = t0 + tl;
+    const double *v2 = qcart + 3 * (size_t)fq[5 * t + 1], *v3 = qcart + 3 * (size_t)fq[5 * t + 2],
+                 *v4 = qcart + 3 * (size_t)fq[5 * t + 3];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nent; p += gridDim.x * blockDim.x) {
+        const double *l = lv + 9 * (size_t)p;
+        const double iqr = -(v2[0] * l[0] + v2[1] * l[1] + v2[2] * l[2] + v3[0] * l[3] + v3[1] * l[4] + v3[2] * l[5] +
+                             v4[0] * l[6] + v4[1] * l[7] + v4[2] * l[8]) *
+                           kTwoPi;
+        ph[(size_t)tl * nent + p] = {cos(iqr), sin(iqr)};
+    }
+}
+
+__global__ void k_f4_phi(int nb, int na, int nent, const int *__restrict__ blk4, const double *__restrict__ mwm,
+                         const cd *__restrict__ ph, cd *__restrict__ X) {
+    const int tl = blockIdx.y;
+    const long nb2 = (long)nb * nb, nb3 = nb2 * nb, n4 = nb3 * nb;
+    const cd *pt = ph + (size_t)tl * nent;
+    cd *out = X + (size_t)tl * n4;
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n4; x += (long)gridDim.x * blockDim.x) {
+        const int ia = (int)(x / nb3), ib = (int)((x / nb2) % nb), ic = (int)((x / nb) % nb), id = (int)(x % nb);
+        const int blk = (((ia / 3) * na + ib / 3) * na + ic / 3) * na + id / 3;
+        const int cmp = 27 * (ia % 3) + 9 * (ib % 3) + 3 * (ic % 3) + id % 3;
+        double re = 0.0, im = 0.0;
+        for (int p = blk4[blk]; p < blk4[blk + 1]; ++p) {
+            const double m = mwm[81 * (size_t)p + cmp];
+            re += m * pt[p].x;
+            im += m * pt[p].y;
+        }
+        out[x] = {re, im};
+    }
+}
+
+typedef double v4d_f4 __attribute__((ext_vector_type(4)));
+
+// Y[tl][r][b] = sum_k X[tl][k][r] E_q[k][b], r < nb^3, k, b < nb, complex FP64 on
+// v_mfma_f64_16x16x4f64 (A[m = lane % 16][k = k0 + lane / 16], B[k][n = lane % 16], D rows
+// m = 4 reg + lane / 16).  Workgroup: 4 waves of 16 rows r each, every column tile of b.
+// E_q: the leg's table, E + q nb^2 with q = fq[5 t + leg].
+template <int NT>
+__global__ void __launch_bounds__(256) k_f4_stage(int nb, const int *__restrict__ fq, int t0, int leg,
+                                                  const cd *__restrict__ E, const cd *__restrict__ X,
+                                                  cd *__restrict__ Y) {
+    const int tl = blockIdx.y;
+    const long nb2 = (long)nb * nb, R = nb2 * nb;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    const long r0 = (long)blockIdx.x * 64 + 16 * w;
+    if (r0 >= R) return;
+    const cd *Xt = X + (size_t)tl * R * nb;
+    const cd *Eq = E + (size_t)fq[5 * (t0 + tl) + leg] * nb2;
+    v4d_f4 ar[NT], ai[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) ar[n] = ai[n] = (v4d_f4){0, 0, 0, 0};
+    const long rr = r0 + c16;
+    for (int k0 = 0; k0 < nb; k0 += 4) {
+        const int k = k0 + g;
+        cd a = {0.0, 0.0};
+        if (k < nb && rr < R) a = Xt[(size_t)k * R + rr];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int col = 16 * n + c16;
+            cd b = {0.0, 0.0};
+            if (k < nb && col < nb) b = Eq[(size_t)k * nb + col];
+            ar[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, ar[n], 0, 0, 0);
+            ar[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, ar[n], 0, 0, 0);
+            ai[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ai[n], 0, 0, 0);
+            ai[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ai[n], 0, 0, 0);
+        }
+    }
+    cd *Yt = Y + (size_t)tl * R * nb;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int col = 16 * n + c16;
+        if (col >= nb) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long row = r0 + 4 * i + g;
+            if (row < R) Yt[(size_t)row * nb + col] = {ar[n][i], ai[n][i]};
+        }
+    }
+}
+
+struct F4EpiArgs {
+    int nb, nT, nirr, t0, T0, nTb, tb, itype;   // nTb temperatures in this pass, tb <= kF4Tb LDS slots
+    const int *fq;
+    const double *om_irr, *om_full, *be, *sigsq;
+    const int *irr_of_full, *row_of;
+    const double *u_irr, *u_full;
+    double sigma_fixed, thresh, pref;
+    const float *tw;
+    const cd *psi;        // [tl][b1][b2][b3][b4]
+    double *g, *s;        // [t][T][nb], [t][T][3][nb][nb]
+};
+
+// fixed-order sum over a workgroup of 256 threads (wave sums, then the four waves in order)
+__device__ inline double f4_block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// Workgroup per (quartet of the batch, b1), temperatures T0 .. T0 + nTb - 1 (nTb <= kF4Tb).
+// Every thread keeps the (b3, b4) pairs p = tid + 256 j; their sums over b2 go to LDS
+// ([T][b3][b4], each slot written by its own thread), the sums over (b3, b4) at every b2
+// are workgroup sums.
+__global__ void __launch_bounds__(kF4EpiBlock) k_f4_epi(F4EpiArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nb = a.nb, nb2 = nb * nb, tl = blockIdx.y, t = a.t0 + tl, b1 = blockIdx.x;
+    const int tb = a.tb;
+    double *acc = (double *)smem;                          // [tb][nb][nb]
+    double *w234 = acc + (size_t)tb * nb2;                 // [3][nb] frequencies of legs 2, 3, 4
+    double *nn = w234 + 3 * nb;                            // [tb][3][nb] occupations of legs 2, 3, 4
+    double *sg = nn + tb * 3 * nb;                         // [3][nb] sigma^2 of legs 2, 3, 4
+    double *uu = sg + 3 * nb;                              // [3][nb][3] reduced velocities (type 6)
+    double *red = uu + 9 * nb;                             // [4]
+    double *s2 = red + 4;                                  // [tb][nb]
+    const int q1 = a.fq[5 * t], mc = a.fq[5 * t + 4];
+    const int qf[3] = {a.fq[5 * t + 1], a.fq[5 * t + 2], a.fq[5 * t + 3]};
+    const int qi[3] = {a.irr_of_full[qf[0]], a.irr_of_full[qf[1]], a.irr_of_full[qf[2]]};
+    for (int x = threadIdx.x; x < 3 * nb; x += blockDim.x) {
+        const int leg = x / nb, b = x % nb;
+        w234[x] = a.om_full[(size_t)qf[leg] * nb + b];
+        sg[x] = a.sigsq[(size_t)qi[leg] * nb + b];
+        if (a.itype == 6)
+            for (int k = 0; k < 3; ++k) uu[3 * x + k] = a.u_full[((size_t)qf[leg] * nb + b) * 3 + k];
+        for (int T = 0; T < a.nTb; ++T)
+            nn[(T * 3 + leg) * nb + b] = a.be[((size_t)(a.T0 + T) * a.nirr + qi[leg]) * nb + b];
+    }
+    for (int x = threadIdx.x; x < tb * nb2; x += blockDim.x) acc[x] = 0.0;
+    __syncthreads();
+    const double w1 = a.om_irr[(size_t)q1 * nb + b1];
+    const bool row_ok = a.row_of[(size_t)q1 * nb + b1] >= 0;
+    const double m0 = (mc == 0) ? 1.0 : (mc == 1 ? 3.0 : 6.0), m1 = (mc == 2) ? 2.0 : 1.0;
+    const double tw = a.tw ? (double)a.tw[(size_t)t * nb + b1] : 1.0;
+    const double sg1 = a.sigsq[(size_t)q1 * nb + b1];
+    const double *u1 = (a.itype == 6) ? a.u_irr + ((size_t)q1 * nb + b1) * 3 : nullptr;
+    const cd *ps = a.psi + ((size_t)tl * nb + b1) * (size_t)nb2 * nb;
+    for (int b2 = 0; b2 < nb; ++b2) {
+        double part[kF4Tb] = {0.0, 0.0, 0.0, 0.0};
+        const double w2 = w234[b2];
+        for (int p = threadIdx.x; p < nb2; p += blockDim.x) {
+            const int b3 = p / nb, b4 = p % nb;
+            const double w3 = w234[nb + b3], w4 = w234[2 * nb + b4];
+            if (!(row_ok && w2 >= kFreqTol && w3 >= kFreqTol && w4 >= kFreqTol)) continue;
+            const cd v = ps[(size_t)b2 * nb2 + p];
+            double sgm;
+            if (a.itype == 1) {
+                sgm = a.sigma_fixed;
+            } else if (a.itype == 2) {
+                sgm = sqrt(sg1 + sg[b2] + sg[nb + b3] + sg[2 * nb + b4]);
+            } else {
+                const double *u2 = uu + 3 * b2, *u3 = uu + 3 * (nb + b3), *u4 = uu + 3 * (2 * nb + b4);
+                sgm = 0.0;
+                sgm += pair_w6(u1, u2);
+                sgm += pair_w6(u1, u3);
+                sgm += pair_w6(u1, u4);
+                sgm += pair_w6(u2, u3);
+                sgm += pair_w6(u2, u4);
+                sgm += pair_w6(u3, u4);
+                sgm += a.thresh;
+            }
+            const double d0 = gauss(w1, w2 + w3 + w4, sgm) - gauss(w1, -w2 - w3 - w4, sgm);
+            const double d1 = gauss(w1, -w2 + w3 + w4, sgm) - gauss(w1, w2 - w3 - w4, sgm);
+            const double d2 = gauss(w1, w2 - w3 + w4, sgm) - gauss(w1, -w2 + w3 - w4, sgm);
+            const double d3 = gauss(w1, w2 + w3 - w4, sgm) - gauss(w1, -w2 - w3 + w4, sgm);
+            const double pq = a.pref * (v.x * v.x + v.y * v.y) * tw;
+#pragma unroll
+            for (int T = 0; T < kF4Tb; ++T) {
+                if (T >= a.nTb) break;
+                const double n2 = nn[(T * 3 + 0) * nb + b2], n3 = nn[(T * 3 + 1) * nb + b3], n4 = nn[(T * 3 + 2) * nb + b4];
+                const double n2p = n2 + 1.0, n3p = n3 + 1.0, n4p = n4 + 1.0;
+                const double plf0 = n2p * n3p * n4p - n2 * n3 * n4;
+                const double plf1 = 3.0 * n2 * n3p * n4p - n2p * n3 * n4;
+                const double plf2 = 3.0 * n3 * n2p * n4p - n3p * n2 * n4;
+                const double plf3 = 3.0 * n4 * n3p * n2p - n4p * n3 * n2;
+                const double f = m0 * pq * plf0 * d0 + m1 * pq * plf1 * d1 + m1 * pq * plf2 * d2 + m1 * pq * plf3 * d3;
+                part[T] += f;
+                acc[(size_t)T * nb2 + p] += f;
+            }
+        }
+#pragma unroll
+        for (int T = 0; T < kF4Tb; ++T) {
+            if (T >= a.nTb) break;
+            const double s = f4_block_sum(part[T], red);
+            if (threadIdx.x == 0) s2[T * nb + b2] = s;
+        }
+    }
+    __syncthreads();
+    for (int T = 0; T < a.nTb; ++T) {
+        double *g = a.g + ((size_t)t * a.nT + a.T0 + T) * nb;
+        double *s = a.s + ((size_t)t * a.nT + a.T0 + T) * 3 * (size_t)nb2;
+        if (threadIdx.x == 0) {
+            double tot = 0.0;
+            for (int b2 = 0; b2 < nb; ++b2) tot += s2[T * nb + b2];
+            g[b1] = tot;
+        }
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            double s3 = 0.0, s4 = 0.0;
+            for (int u = 0; u < nb; ++u) {
+                s3 += acc[(size_t)T * nb2 + b * nb + u];
+                s4 += acc[(size_t)T * nb2 + u * nb + b];
+            }
+            s[(size_t)b1 * nb + b] = s2[T * nb + b];
+            s[nb2 + (size_t)b1 * nb + b] = s3;
+            s[2 * (size_t)nb2 + (size_t)b1 * nb + b] = s4;
+        }
+    }
+}
+
+size_t f4_epi_lds(int nb, int tb) {
+    return sizeof(double) * ((size_t)tb * nb * nb + 3 * nb + (size_t)tb * 3 * nb + 3 * nb + 9 * nb + 4 + (size_t)tb * nb);
+}
+
 int fourth_grid_index(int i1, int i2, int i3, const int *dims) {
     const int n2 = dims[1], n3 = dims[2];
     int g[3] = {0, 0, 0};
@@ -267,9 +503,11 @@ void build_quartets_mc(const tc3_ctx *c, std::vector<int> &quart, std::vector<in
     img_off.assign(1, 0);
     img.clear();
     tw.clear();
-    std::vector<int> slot((size_t)np * np);
+    // (q2, q3) pairs seen for this q1: -1 reducible, else the quartet (at most 2 nb n4^2 draws
+    // per q1, so a hash map instead of an np^2 table)
+    std::unordered_map<long long, int> slot;
     for (int q1 = 0; q1 < m.nirr; ++q1) {
-        std::fill(slot.begin(), slot.end(), -2);   // -2 unseen, -1 reducible, else quartet
+        slot.clear();
         const int q1f = m.irr_full_index[q1];
         for (int b1 = 0; b1 < nb; ++b1) {
             if (c->row_of[(size_t)q1 * nb + b1] < 0) continue;
@@ -280,8 +518,9 @@ void build_quartets_mc(const tc3_ctx *c, std::vector<int> &quart, std::vector<in
                     if (q3 < q2) continue;
                     const int q4 = fourth_grid_index(q1f, q2, q3, m.dims);
                     if (q4 < q3) continue;
-                    int &s = slot[(size_t)q2 * np + q3];
-                    if (s == -2) {
+                    const auto ins = slot.emplace((long long)q2 * np + q3, -2);
+                    int &s = ins.first->second;
+                    if (ins.second) {
                         if (quartet_images(m, q1, q2, q3, q4, img)) {
                             s = (int)quart.size() / 5;
                             quart.insert(quart.end(), {q1, q2, q3, q4, multiplicity_class(q2, q3, q4)});
@@ -301,14 +540,16 @@ void build_quartets_mc(const tc3_ctx *c, std::vector<int> &quart, std::vector<in
 
 int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, std::vector<double> &od4) {
     const int nb = c->nb, na = c->cr.na, nT = in.nT, nirr = c->mesh.nirr, np = c->mesh.nfull, nrows = (int)c->row_q.size();
-    if (nb > kF4MaxNb) return c->fail(5, "four-phonon scattering handles at most 6 bands");
     const bool mc = c->mc4_on;
-    std::vector<int> mq, mio, mim;
-    std::vector<float> mtw;
-    if (mc) build_quartets_mc(c, mq, mio, mim, mtw);
-    else if (c->quart_gen != c->gen_model) build_quartets(c);
-    const std::vector<int> &quart = mc ? mq : c->quart, &qoff = mc ? mio : c->quart_img_off,
-                           &qimg = mc ? mim : c->quart_img;
+    if (mc && c->mquart_gen != c->gen_model) {
+        build_quartets_mc(c, c->mquart, c->mquart_img_off, c->mquart_img, c->mquart_tw);
+        c->mquart_gen = c->gen_model;
+    } else if (!mc && c->quart_gen != c->gen_model) {
+        build_quartets(c);
+    }
+    const std::vector<int> &quart = mc ? c->mquart : c->quart, &qoff = mc ? c->mquart_img_off : c->quart_img_off,
+                           &qimg = mc ? c->mquart_img : c->quart_img;
+    const std::vector<float> &mtw = c->mquart_tw;
     const int nqt = (int)quart.size() / 5;
     g4.assign((size_t)nT * nrows, 0.0);
     od4.assign((size_t)nT * nrows * np * nb, 0.0);
@@ -342,8 +583,82 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
     a.pref = (kPi / 96.0) / (npg * npg);
     TRY(c->pool_get("f4_g", sizeof(double) * (size_t)nqt * nT * nb, &p)); a.g = (double *)p;
     TRY(c->pool_get("f4_s", sizeof(double) * (size_t)nqt * nT * 3 * nb * nb, &p)); a.s = (double *)p;
-    hipLaunchKernelGGL(k_fourphonon, dim3(nqt), dim3(kF4Block), 0, st, a);
-    TRY(hipGetLastError());
+    // TC3_F4_BATCHED: the batched kernels also for cells of up to two atoms; TC3_F4_NQB: at
+    // most that many quartets per batch (both for tests)
+    if (nb <= kF4MaxNb && std::getenv("TC3_F4_BATCHED") == nullptr) {
+        hipLaunchKernelGGL(k_fourphonon, dim3(nqt), dim3(kF4Block), 0, st, a);
+        TRY(hipGetLastError());
+    } else {
+        // batched: phi4 and the contraction steps of up to nqb quartets at once, two buffers of
+        // nb^4 complex values per quartet (at most 1 GiB each)
+        const size_t n4 = (size_t)nb * nb * nb * nb, per = n4 * sizeof(cd);
+        int nqb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nqt, ((size_t)1 << 30) / per));
+        if (const char *env = std::getenv("TC3_F4_NQB")) nqb = std::max(1, std::min(nqb, std::atoi(env)));
+        const int nent = std::max(c->nfc4, 1);
+        void *pE1, *pEf, *pph, *pX, *pY;
+        TRY(c->pool_get("f4_E1", sizeof(cd) * (size_t)nirr * nb * nb, &pE1));
+        TRY(c->pool_get("f4_Ef", sizeof(cd) * (size_t)np * nb * nb, &pEf));
+        TRY(c->pool_get("f4_ph", sizeof(cd) * (size_t)nqb * nent, &pph));
+        TRY(c->pool_get("f4_X", per * nqb, &pX));
+        TRY(c->pool_get("f4_Y", per * nqb, &pY));
+        hipLaunchKernelGGL(k_scale<double>, dim3(256), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
+                           c->d_om_irr, (cplx<double> *)pE1);
+        hipLaunchKernelGGL(k_scale<double>, dim3(1024), dim3(256), 0, st, (long)np * nb * nb, nb, c->d_egv_full,
+                           c->d_om_full, (cplx<double> *)pEf);
+        TRY(hipGetLastError());
+        const cd *E1 = (const cd *)pE1, *Ef = (const cd *)pEf;
+        cd *X = (cd *)pX, *Y = (cd *)pY;
+        // epilogue: temperature slots that fit 100 KiB of LDS (at least one)
+        int tb = kF4Tb;
+        while (tb > 1 && f4_epi_lds(nb, tb) > 100 * 1024) --tb;
+        const size_t shm = f4_epi_lds(nb, tb);
+        if (shm > 160 * 1024) return c->fail(5, "band count too large for the four-phonon epilogue's LDS");
+        TRY(hipFuncSetAttribute((const void *)k_f4_epi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        const long R = (long)nb * nb * nb;
+        const unsigned sgx = (unsigned)((R + 63) / 64);
+        const int NT = (nb + 15) / 16;
+        auto stage = [&](int leg, const cd *src, cd *dst, int t0, int nq) -> hipError_t {
+            const dim3 gs(sgx, (unsigned)nq);
+            const cd *E = leg == 0 ? E1 : Ef;
+            switch (NT) {
+            case 1: hipLaunchKernelGGL(k_f4_stage<1>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+            case 2: hipLaunchKernelGGL(k_f4_stage<2>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+            case 3: hipLaunchKernelGGL(k_f4_stage<3>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+            case 4: hipLaunchKernelGGL(k_f4_stage<4>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+            case 5: hipLaunchKernelGGL(k_f4_stage<5>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+            case 6: hipLaunchKernelGGL(k_f4_stage<6>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+            default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        };
+        if (NT > 6) return c->fail(5, "four-phonon contraction handles at most 96 bands");
+        F4EpiArgs ep{};
+        ep.nb = nb; ep.nT = nT; ep.nirr = nirr; ep.tb = tb; ep.itype = a.itype;
+        ep.fq = a.fq; ep.om_irr = a.om_irr; ep.om_full = a.om_full; ep.be = a.be; ep.sigsq = a.sigsq;
+        ep.irr_of_full = a.irr_of_full; ep.row_of = a.row_of; ep.u_irr = a.u_irr; ep.u_full = a.u_full;
+        ep.sigma_fixed = a.sigma_fixed; ep.thresh = a.thresh; ep.pref = a.pref; ep.tw = a.tw;
+        ep.psi = X; ep.g = a.g; ep.s = a.s;
+        for (int t0 = 0; t0 < nqt; t0 += nqb) {
+            const int nq = std::min(nqb, nqt - t0);
+            hipLaunchKernelGGL(k_f4_phase, dim3((unsigned)((nent + 255) / 256), (unsigned)nq), dim3(256), 0, st, nent,
+                               a.fq, t0, a.qcart, a.lv, (cd *)pph);
+            hipLaunchKernelGGL(k_f4_phi, dim3((unsigned)std::min<size_t>((n4 + 255) / 256, 65535), (unsigned)nq),
+                               dim3(256), 0, st, nb, na, nent, a.blk4, a.mwm, (const cd *)pph, X);
+            TRY(hipGetLastError());
+            // ia with e1 (q1), ib with e2, ic with e3, id with e4: X -> Y -> X -> Y -> X
+            TRY(stage(0, X, Y, t0, nq));
+            TRY(stage(1, Y, X, t0, nq));
+            TRY(stage(2, X, Y, t0, nq));
+            TRY(stage(3, Y, X, t0, nq));
+            ep.t0 = t0;
+            for (int T0 = 0; T0 < nT; T0 += tb) {
+                ep.T0 = T0;
+                ep.nTb = std::min(tb, nT - T0);
+                hipLaunchKernelGGL(k_f4_epi, dim3((unsigned)nb, (unsigned)nq), dim3(kF4EpiBlock), shm, st, ep);
+                TRY(hipGetLastError());
+            }
+        }
+    }
     std::vector<double> hg((size_t)nqt * nT * nb), hs((size_t)nqt * nT * 3 * nb * nb);
     TRY(hipMemcpyAsync(hg.data(), a.g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, st));
     TRY(hipMemcpyAsync(hs.data(), a.s, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st));

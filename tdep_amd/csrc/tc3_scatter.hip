@@ -1467,7 +1467,6 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     std::vector<double> g4;
     double *d_od4 = nullptr;   // four-phonon terms for the symmetrisation below
     if (c->nfc4 > 0) {
-        if (use_mfma) { cleanup(); return c->fail(5, "four-phonon scattering handles at most 6 bands"); }
         F4Inputs fi;
         fi.nT = nT; fi.integrationtype = integrationtype; fi.sigma_fixed = ea.sigma_fixed; fi.thresh = c->thresh6;
         fi.d_be = d_be; fi.d_sigsq = d_sigsq; fi.d_uirr = d_uirr; fi.d_ufull = d_ufull; fi.d_rowof = d_rowof;

@@ -321,6 +321,9 @@ CONFIGS = {
     "rocksalt_adjoint": dict(qgrid=(16, 16, 16), temperatures=(300.0,)),
     # 32-atom FCC superlattice, 8x8x2, 300 K
     "sl32": dict(qgrid=(8, 8, 2), temperatures=(300.0,)),
+    # 4-atom cubic FCC cell, two atomic layers of 40 and 80 amu (12 bands): four-phonon tests
+    # beyond two atoms
+    "sl4": dict(qgrid=(3, 3, 3), temperatures=(300.0,)),
 }
 
 
@@ -362,6 +365,8 @@ def make_crystal(name, seed=0):
     if name == "sl16":
         lm = [40.0 if (l // 2) % 2 == 0 else 80.0 for l in range(8)]
         return fcc_superlattice(4.05, 4, lm), (lambda s1, s2: MORSE_AL), 5.0
+    if name == "sl4":
+        return fcc_superlattice(4.05, 1, [40.0, 80.0]), (lambda s1, s2: MORSE_AL), 5.0
     if name == "sl32":
         rng = np.random.default_rng(4)
         lm = [float(x) for x in rng.choice([28.0, 56.0, 84.0], size=16)]

@@ -1,7 +1,9 @@
 """Four-phonon scattering (--fourthorder) on the GPU against the oracle (oracle/fourphonon.py),
 through the C ABI: linewidths and the symmetrised scattering matrix, FP64, within 1e-9 of
-their largest value (1e-8 with velocity-difference widths); the four-phonon term changes the three-phonon linewidths by more than
-that."""
+their largest value (1e-8 with velocity-difference widths); the four-phonon term changes the
+three-phonon linewidths by more than that.  Cells of one and two atoms go through the
+one-workgroup-per-quartet kernel, four and sixteen atoms (12 and 48 bands) through the batched
+phi4 + MFMA contraction + epilogue kernels."""
 import os
 
 import numpy as np
@@ -14,16 +16,22 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("name,grid,it", [("fcc_al", (4, 4, 4), 2), ("fcc_al", (4, 4, 4), 1),
-                                          ("fcc_al", (4, 4, 4), 6), ("rocksalt", (3, 3, 3), 2)])
-def test_fourphonon_matches_oracle(tmp_path, name, grid, it):
+@pytest.mark.parametrize("name,grid,it,temps", [("fcc_al", (4, 4, 4), 2, (100.0, 300.0)),
+                                                ("fcc_al", (4, 4, 4), 1, (100.0, 300.0)),
+                                                ("fcc_al", (4, 4, 4), 6, (100.0, 300.0)),
+                                                ("rocksalt", (3, 3, 3), 2, (100.0, 300.0)),
+                                                # four atoms (12 bands): the batched MFMA contraction
+                                                ("sl4", (3, 3, 3), 2, (100.0, 300.0, 500.0, 700.0, 900.0)),
+                                                ("sl4", (3, 3, 3), 6, (300.0,)),
+                                                ("sl4", (3, 3, 3), 1, (300.0,))])
+def test_fourphonon_matches_oracle(tmp_path, name, grid, it, temps):
     from oracle import lattice as L, run as R, scattering as S
     from tdep_amd import inputs
     from tdep_amd.engine import Engine
-    print(f"[case] fourphonon {name} {grid} it={it}", flush=True)
+    print(f"[case] fourphonon {name} {grid} it={it} T={temps}", flush=True)
     d = str(tmp_path / name)
     inputs.make_config(name, d, fourth=True)
-    temps = [100.0, 300.0]
+    temps = list(temps)
     e = Engine.from_directory(d, grid, device=0)
     e.dispersions()
     e.scattering_rates(temps, precision="fp64", integrationtype=it)
@@ -70,7 +78,8 @@ def test_hosts_with_fourthorder(tmp_path):
 
 
 @pytest.mark.parametrize("name,grid,qg3,qg4", [("fcc_al", (4, 4, 4), None, (2, 2, 2)),
-                                               ("rocksalt", (3, 3, 3), (2, 2, 2), (2, 2, 2))])
+                                               ("rocksalt", (3, 3, 3), (2, 2, 2), (2, 2, 2)),
+                                               ("sl4", (3, 3, 3), (2, 2, 2), (2, 2, 2))])
 def test_fourphonon_montecarlo_matches_oracle(tmp_path, name, grid, qg3, qg4):
     """-qg4ph (and -qg3ph) with --seed 7: every row draws its three-phonon grid and two
     four-phonon grids in turn; linewidths and scattering matrix as the oracle's."""
@@ -89,6 +98,60 @@ def test_fourphonon_montecarlo_matches_oracle(tmp_path, name, grid, qg3, qg4):
     p, fc2, fc3, ops, mesh, disp = R.setup(d, grid)
     fc4 = L.read_fc4(p, os.path.join(d, "infile.forceconstant_fourthorder"))
     sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, fc4=fc4, qg3ph=qg3, qg4ph=qg4, seed=7)
+    assert _rel(e.linewidths(0)[0], sr.linewidth) < 1e-9
+    xi, rq = e.scattering_matrix(0)
+    assert [tuple(x) for x in rq] == list(sr.rows)
+    assert _rel(xi, sr.Xi) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_fourphonon_48_bands_matches_golden(tmp_path, prec):
+    """16-atom superlattice (48 bands) on a 3x3x1 mesh with --fourthorder: the three-phonon
+    term on the MFMA chain and the four-phonon term through the batched kernels; linewidths
+    and twelve scattering-matrix rows against the oracle's (tests/golden/make_golden.py,
+    FOUR_CASES; the oracle takes about a minute here).  FP64 within 1e-9 of the largest value;
+    with the split-BF16x3 three-phonon contraction (the four-phonon term stays FP64, the
+    scattering matrix is stored in FP32) within the mixed mode's 1e-4 budget."""
+    from tdep_amd import inputs
+    from tdep_amd.engine import Engine
+    root = os.path.dirname(os.path.abspath(__file__))
+    g = np.load(os.path.join(root, "golden", "oracle_fourth_sl16_3x3x1.npz"))
+    grid, temps = tuple(int(x) for x in g["grid"]), [float(x) for x in g["temperatures"]]
+    print(f"[case] fourphonon 48 bands sl16 {grid} {prec}", flush=True)
+    d = str(tmp_path / "sl16")
+    inputs.make_config("sl16", d, fourth=True)
+    e = Engine.from_directory(d, grid, device=0)
+    e.set_fourthorder(d)
+    e.dispersions()
+    e.scattering_rates(temps, precision=prec)
+    tol = 1e-9 if prec == "fp64" else 1e-4
+    for iT in range(len(temps)):
+        assert _rel(e.linewidths(iT)[0], g[f"linewidth_{iT}"]) < tol
+        xi, rq = e.scattering_matrix(iT)
+        assert [tuple(x) for x in rq] == [tuple(x) for x in g[f"rows_{iT}"]]
+        assert _rel(xi[g[f"xi_rows_{iT}"]], g[f"xi_{iT}"]) < tol
+
+
+@pytest.mark.parametrize("name,grid", [("fcc_al", (4, 4, 4)), ("rocksalt", (3, 3, 3))])
+def test_fourphonon_batched_kernels_on_small_cells(tmp_path, monkeypatch, name, grid):
+    """The batched four-phonon kernels forced on cells of one and two atoms (TC3_F4_BATCHED),
+    three quartets per batch (TC3_F4_NQB): linewidths and scattering matrix as the oracle's,
+    like the one-workgroup-per-quartet kernel's."""
+    from oracle import lattice as L, run as R, scattering as S
+    from tdep_amd import inputs
+    from tdep_amd.engine import Engine
+    print(f"[case] fourphonon batched kernels {name} {grid}", flush=True)
+    monkeypatch.setenv("TC3_F4_BATCHED", "1")
+    monkeypatch.setenv("TC3_F4_NQB", "3")
+    d = str(tmp_path / name)
+    inputs.make_config(name, d, fourth=True)
+    e = Engine.from_directory(d, grid, device=0)
+    e.set_fourthorder(d)
+    e.dispersions()
+    e.scattering_rates([300.0], precision="fp64")
+    p, fc2, fc3, ops, mesh, disp = R.setup(d, grid)
+    fc4 = L.read_fc4(p, os.path.join(d, "infile.forceconstant_fourthorder"))
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, fc4=fc4)
     assert _rel(e.linewidths(0)[0], sr.linewidth) < 1e-9
     xi, rq = e.scattering_matrix(0)
     assert [tuple(x) for x in rq] == list(sr.rows)
