@@ -337,16 +337,29 @@ __global__ void __launch_bounds__(256) k_small(EpiArgs a, int ntrip, int *__rest
     }
 }
 
-// Linewidth of every row: deterministic sum over the triplets of its q1.
-__global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_q, const int *__restrict__ row_b,
-                        const int *__restrict__ q1_off, const double *__restrict__ gpart, double *__restrict__ lw) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrows * nT) return;
-    const int T = i / nrows, r = i % nrows;
-    const int q = row_q[r], b = row_b[r];
+// Linewidth of every row: deterministic sum over the triplets of its q1.  Workgroup per
+// (irreducible q1, temperature), one thread per band; the triplets of q1 are summed in
+// order, eight loads in flight.
+__global__ void k_gamma(int nrows, int nT, int nb, const int *__restrict__ row_of, const int *__restrict__ q1_off,
+                        const double *__restrict__ gpart, double *__restrict__ lw) {
+    const int q = blockIdx.x, T = blockIdx.y, b = threadIdx.x;
+    if (b >= nb) return;
+    const int r = row_of[(size_t)q * nb + b];
+    if (r < 0) return;
+    const double *gp = gpart + (size_t)T * nb + b;
+    const size_t st = (size_t)nT * nb;
+    const int t0 = q1_off[q], t1 = q1_off[q + 1];
     double s = 0;
-    for (int t = q1_off[q]; t < q1_off[q + 1]; ++t) s += gpart[((size_t)t * nT + T) * nb + b];
-    lw[i] = s;
+    int t = t0;
+    for (; t + 8 <= t1; t += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = gp[(size_t)(t + k) * st];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; t < t1; ++t) s += gp[(size_t)t * st];
+    lw[(size_t)T * nrows + r] = s;
 }
 
 // Isotope scattering inputs (compute_isotope_scattering, scattering_isotope.f90:2-70): the
@@ -1478,8 +1491,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         TRY(hipStreamSynchronize(st));
     }
     hipEvent_t evs0 = c->ev_pool[2], evs1 = c->ev_pool[3];
-    hipLaunchKernelGGL(k_gamma, dim3((nrows * nT + 255) / 256), dim3(256), 0, st, nrows, nT, nb, d_rowq, d_rowb,
-                       d_q1off, d_gpart, d_lw);
+    hipLaunchKernelGGL(k_gamma, dim3((unsigned)nirr, (unsigned)nT), dim3((unsigned)((nb + 63) / 64 * 64)), 0, st, nrows,
+                       nT, nb, d_rowof, d_q1off, d_gpart, d_lw);
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs0, st));
     // isotope scattering (scattering.f90:229-232): on when requested and some atom has mass
