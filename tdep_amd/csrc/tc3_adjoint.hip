@@ -243,6 +243,7 @@ int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<do
     TRY(c->pool_get("be", sizeof(double) * nT * nirr * nb, (void **)&d_be));
     TRY(c->pool_get("sigsq", sizeof(double) * nirr * nb, (void **)&d_sigsq));
     TRY(c->pool_get("adj_g", sizeof(double) * (size_t)nT * nrows, (void **)&d_g));
+    host_occupations(c);
     TRY(hipMemcpyAsync(d_be, c->be.data(), sizeof(double) * nT * nirr * nb, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_sigsq, c->sigsq.data(), sizeof(double) * nirr * nb, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_g, g, sizeof(double) * (size_t)nT * nrows, hipMemcpyHostToDevice, st));

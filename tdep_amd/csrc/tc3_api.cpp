@@ -73,7 +73,27 @@ double planck(double T, double om) {
     return 1.0 / (std::exp(x) - 1.0);
 }
 
+
 }  // namespace
+
+namespace tc3 {
+// Bose-Einstein occupations of the irreducible modes at the pass's temperatures on the host
+// (scattering.f90:148-152; classical: k_B T / omega), zero below lo_freqtol.
+void host_occupations(tc3_ctx *ctx) {
+    if (ctx->be_host) return;
+    const int nb = ctx->nb, nirr = ctx->mesh.nirr, nT = ctx->nT;
+    ctx->be.assign((size_t)nT * nirr * nb, 0.0);
+    for (int q = 0; q < nirr; ++q)
+        for (int b = 0; b < nb; ++b) {
+            const double om = ctx->om_irr[(size_t)q * nb + b];
+            if (om < kFreqTol) continue;
+            for (int T = 0; T < nT; ++T)
+                ctx->be[((size_t)T * nirr + q) * nb + b] =
+                    ctx->classical ? kKbHartree * ctx->temps[T] / om : planck(ctx->temps[T], om);
+        }
+    ctx->be_host = true;
+}
+}  // namespace tc3
 
 extern "C" {
 
@@ -409,8 +429,9 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
         ctx->classical = classical != 0;
         ctx->integrationtype = integrationtype;
         ctx->sigma = sigma;
-        // occupations and adaptive smearing (scattering.f90:131-167)
-        ctx->be.assign((size_t)nT * nirr * nb, 0.0);
+        // adaptive smearing (scattering.f90:131-167); the occupations (:148-152) are formed on
+        // the device by the pass (k_occupations), on the host when a derivative needs them
+        ctx->be_host = false;
         ctx->sigsq.assign((size_t)nirr * nb, 0.0);
         std::vector<double> sigavg(nb, 0.0);
         const int *dims = ctx->mesh.dims;
@@ -418,9 +439,6 @@ int tc3_scatter(tc3_ctx *ctx, int nT, const double *temperatures, int precision,
             for (int b = 0; b < nb; ++b) {
                 const double om = ctx->om_irr[(size_t)q * nb + b];
                 if (om < kFreqTol) continue;
-                for (int T = 0; T < nT; ++T)
-                    ctx->be[((size_t)T * nirr + q) * nb + b] =
-                        ctx->classical ? kKbHartree * temperatures[T] / om : planck(temperatures[T], om);
                 const double *v = &ctx->vel_irr[((size_t)q * nb + b) * 3];
                 double sg = -1e300;
                 for (int j = 0; j < 3; ++j) {
@@ -647,12 +665,14 @@ int tc3_get_psi2(tc3_ctx *ctx, int precision, int n, const int64_t *triplets, do
 
 int tc3_get_timings(const tc3_ctx *ctx, double *t, int n) {
     if (!ctx || !t) return 5;
+    const_cast<tc3_ctx *>(ctx)->resolve_kstats();
     for (int i = 0; i < n && i < 8; ++i) t[i] = ctx->timings[i];
     return 0;
 }
 
 int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches, int64_t *batch) {
     if (!ctx || which < 0 || which > 4) return 5;
+    const_cast<tc3_ctx *>(ctx)->resolve_kstats();
     if (total_ms) *total_ms = ctx->kstat_ms[which];
     if (launches) *launches = ctx->kstat_n[which];
     if (batch) *batch = ctx->batch;

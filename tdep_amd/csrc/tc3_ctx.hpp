@@ -85,6 +85,9 @@ struct tc3_ctx {
     int integrationtype = 2;
     double sigma = 1.0;                       // fixed smearing (THz) of integrationtype 1
     std::vector<double> be, sigsq;            // [T][q][b], [q][b]
+    // be is formed on the device by the scattering pass; the host copy (adjoint, forward-mode
+    // derivatives) on first use after it (host_occupations)
+    bool be_host = false;
     std::vector<double> iso_g;                // mass-disorder parameter per atom
     bool iso_on = false;                      // isotope scattering requested
     double mfp_max = -1.0;                    // boundary scattering length (Bohr), off if <= 0
@@ -103,6 +106,29 @@ struct tc3_ctx {
     // 1 Psi + smeared-delta epilogue kernel, 2 scattering-matrix symmetrisation, 3 BTE Xi.F
     double kstat_ms[5] = {0, 0, 0, 0, 0};
     int kstat_n[5] = {0, 0, 0, 0, 0};
+    // events of the last scattering pass, read into kstat_ms / timings[5] when the statistics
+    // are asked for (querying them inside the pass costs host time after every pass)
+    std::vector<hipEvent_t> ks_kev;            // 3 per launch pair: before, between, after
+    hipEvent_t ks_ev0 = nullptr, ks_ev1 = nullptr, ks_evs0 = nullptr, ks_evs1 = nullptr;
+    bool ks_pending = false, ks_comp = false;
+    void resolve_kstats() {
+        if (!ks_pending) return;
+        ks_pending = false;
+        float ms = 0;
+        hipEventElapsedTime(&ms, ks_ev0, ks_ev1);
+        timings[5] = ms * 1e-3;
+        for (size_t b = 0; b + 2 < ks_kev.size(); b += 3) {
+            float m1 = 0, m2 = 0;
+            hipEventElapsedTime(&m1, ks_kev[b], ks_kev[b + 1]);
+            hipEventElapsedTime(&m2, ks_kev[b + 1], ks_kev[b + 2]);
+            kstat_ms[0] += m1; kstat_n[0] += 1;
+            kstat_ms[1] += m2; kstat_n[1] += 1;
+        }
+        float m3 = 0;
+        hipEventElapsedTime(&m3, ks_evs0, ks_evs1);
+        kstat_ms[2] += m3; kstat_n[2] += 1;
+        if (ks_comp) { kstat_ms[4] += m3; kstat_n[4] += 1; }
+    }
     long batch = 0;
 
     // device buffers
@@ -212,5 +238,7 @@ int run_jvp(tc3_ctx *c, const double *dfc2, const double *dfc3, std::vector<doub
 int run_adjoint_gamma(tc3_ctx *c, int precision, const double *g, std::vector<double> &grad);
 // tc3_hdf5.cpp
 int write_grid_hdf5(const tc3_ctx *c, int iT, const char *path, std::string &err);
+// tc3_api.cpp: the occupations of the last pass on the host (c->be), formed on first use
+void host_occupations(tc3_ctx *c);
 
 }  // namespace tc3

@@ -294,6 +294,7 @@ int run_jvp(tc3_ctx *c, const double *dfc2, const double *dfc3, std::vector<doub
     TRY(c->pool_get("j_part", sizeof(double) * std::max<long>(ntrip, 1) * nb * nT, (void **)&d_part));
     TRY(c->pool_get("j_dg", sizeof(double) * std::max(nrows, 1) * nT, (void **)&d_dg));
     TRY(hipMemcpyAsync(d_dmwm, dmwm.data(), sizeof(double) * dmwm.size(), hipMemcpyHostToDevice, st));
+    host_occupations(c);
     TRY(hipMemcpyAsync(d_be, c->be.data(), sizeof(double) * nT * qb, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_sigsq, c->sigsq.data(), sizeof(double) * qb, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_dsigsq, dsigsq.data(), sizeof(double) * qb, hipMemcpyHostToDevice, st));

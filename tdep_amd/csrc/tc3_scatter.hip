@@ -337,6 +337,27 @@ __global__ void __launch_bounds__(256) k_small(EpiArgs a, int ntrip, int *__rest
     }
 }
 
+// Bose-Einstein occupations be[T][q][b] of the irreducible modes (scattering.f90:148-152:
+// lo_planck, gottochblandat_physics.f90:160-188; classical: k_B T / omega), zero below
+// lo_freqtol.
+__global__ void k_occupations(int nT, int nqb, const double *__restrict__ temps, const double *__restrict__ om,
+                              int classical, double *__restrict__ be) {
+    const long n = (long)nT * nqb;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double T = temps[i / nqb], w = om[i % nqb];
+        double v = 0.0;
+        if (w >= kFreqTol) {
+            if (classical) {
+                v = kKbHartree * T / w;
+            } else if (T >= kTemperatureTol) {
+                const double x = w / (kKbHartree * T);
+                if (!(x > 1e2)) v = 1.0 / (exp(x) - 1.0);
+            }
+        }
+        be[i] = v;
+    }
+}
+
 // Linewidth of every row: deterministic sum over the triplets of its q1.  Workgroup per
 // (irreducible q1, temperature), one thread per band; the triplets of q1 are summed in
 // order, eight loads in flight.
@@ -1039,12 +1060,17 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     // per-call transfers go through the context's pinned staging buffer: occupations and
     // widths up, linewidths (and the isotope term) down
     const size_t n_be = (size_t)nT * nirr * nb, n_sg = (size_t)nirr * nb, n_lw = (size_t)nrows * nT;
-    TRY(c->pinned(sizeof(double) * (n_be + n_sg + n_lw + nrows)));
-    double *h_be = (double *)c->pin, *h_sg = h_be + n_be, *h_lw = h_sg + n_sg, *h_iso = h_lw + n_lw;
-    std::memcpy(h_be, c->be.data(), sizeof(double) * n_be);
+    TRY(c->pinned(sizeof(double) * (nT + n_sg + n_lw + nrows)));
+    double *h_T = (double *)c->pin, *h_sg = h_T + nT, *h_lw = h_sg + n_sg, *h_iso = h_lw + n_lw;
+    std::memcpy(h_T, c->temps.data(), sizeof(double) * nT);
     std::memcpy(h_sg, c->sigsq.data(), sizeof(double) * n_sg);
-    TRY(hipMemcpyAsync(d_be, h_be, sizeof(double) * n_be, hipMemcpyHostToDevice, st));
+    double *d_temps;
+    TRY(c->pool_get("temps", sizeof(double) * nT, (void **)&d_temps));
+    TRY(hipMemcpyAsync(d_temps, h_T, sizeof(double) * nT, hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_sigsq, h_sg, sizeof(double) * n_sg, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_occupations, dim3((unsigned)std::min<size_t>(1024, (n_be + 255) / 256)), dim3(256), 0, st, nT,
+                       nirr * nb, d_temps, c->d_om_irr, (int)c->classical, d_be);
+    TRY(hipGetLastError());
     // scattering matrix: every entry that any triplet owns is rewritten by every pass, so
     // the buffer is zeroed only when it is new or the model changed
     c->xi_row = (size_t)nfull * nb;
@@ -1155,6 +1181,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     TRY(pooled_events(4));
     hipEvent_t ev0 = c->ev_pool[0], ev1 = c->ev_pool[1];
     std::vector<hipEvent_t> kev;
+    c->ks_pending = false;
     for (double &x : c->kstat_ms) x = 0;
     for (int &x : c->kstat_n) x = 0;
     if (!c->thirdorder) {
@@ -1591,27 +1618,14 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             if (vn > kGroupVelTol)
                 for (int T = 0; T < nT; ++T) lw[(size_t)T * nrows + r] += vn / c->mfp_max;
         }
-    float ms = 0;
-    hipEventElapsedTime(&ms, ev0, ev1);
-
-    for (size_t b = 0; b + 2 < kev.size(); b += 3) {
-        float m1 = 0, m2 = 0;
-        hipEventElapsedTime(&m1, kev[b], kev[b + 1]);
-        hipEventElapsedTime(&m2, kev[b + 1], kev[b + 2]);
-        c->kstat_ms[0] += m1; c->kstat_n[0] += 1;
-        c->kstat_ms[1] += m2; c->kstat_n[1] += 1;
-    }
-
-    {
-        float m3 = 0;
-        hipEventElapsedTime(&m3, evs0, evs1);
-        c->kstat_ms[2] += m3; c->kstat_n[2] += 1;
-        if (d_comp) { c->kstat_ms[4] += m3; c->kstat_n[4] += 1; }
-    }
+    // kernel statistics and the pass's device time: read from these events on request
+    c->ks_kev = kev;
+    c->ks_ev0 = ev0; c->ks_ev1 = ev1; c->ks_evs0 = evs0; c->ks_evs1 = evs1;
+    c->ks_comp = d_comp != nullptr;
+    c->ks_pending = true;
 
 #undef TRY
     cleanup();
-    c->timings[5] = ms * 1e-3;
     // reduce + degeneracy average + F0 (scattering.f90:260-292)
     const size_t qb = (size_t)nirr * nb;
     c->linewidth.assign(qb * nT, 0.0);
