@@ -554,10 +554,10 @@ template <int NB> struct MaskWords { static constexpr int n = (NB + 31) / 32; };
 template <int PREC, int NB, int NTY>
 size_t psi_lds() {
     typedef typename Stage<PREC>::T S;
-    constexpr int NW = NB / 16, NMW = MaskWords<NB>::n, KS = KSteps<PREC, NB>::n;
+    constexpr int NW = NB / 16, NMW = MaskWords<NB>::n;
     size_t s = sizeof(S) * (2 * NTY * NB + 2 * NB + NW * kChunk * 16 * NTY) + sizeof(float) * NB;
     s = (s + 15) & ~(size_t)15;
-    s += (PREC == 2) ? (size_t)KS * kPlanes * 64 * 16 : 0;    // E1 fragments
+    s += (PREC == 2) ? (size_t)(NB / 16) * 4 * 64 * 16 : 0;   // E1 fragments (stacked K)
     s += sizeof(double) * (NB + 16 + 16 * NTY) + sizeof(float) * NB + sizeof(int) * 16 +
          sizeof(unsigned) * (size_t)(NB + 16) * NMW;
     s = (s + 15) & ~(size_t)15;
@@ -596,8 +596,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     float *w3f = (float *)(Xp + NW * kChunk * 16 * NTY);   // [NB] omega of q3 modes
     unsigned char *p8 = smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + 2 * NB + NW * kChunk * 16 * NTY) +
                                        sizeof(float) * NB);
-    v8bf *e1s = (v8bf *)p8;                // [KS][6 planes][64 lanes] (BF16X3)
-    if constexpr (PREC == 2) p8 += (size_t)KS * kPlanes * 64 * 16;
+    v8bf *e1s = (v8bf *)p8;                // [NB/16 K-steps][4 operands][64 lanes] (BF16X3)
+    if constexpr (PREC == 2) p8 += (size_t)(NB / 16) * 4 * 64 * 16;
     double *w2s = (double *)p8;            // [NB]
     double *w1s = w2s + NB;                // [16]
     double *gam = w1s + 16;                // [NTY][16] linewidth partial sums
@@ -657,14 +657,26 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         v12[x] = m;
     }
     if constexpr (PREC == 2) {
-        // E1 operand (B[k = ia][n = b1]) fragments, one 16-byte slot per lane
+        // E1 operands with real and imaginary parts stacked in K: K-step s covers the 16
+        // components ia = 16 s .. 16 s + 15, lane (g, c) supplies k = 8 g + j with ia = 16 s + 4 g
+        // + (j & 3), the real part of W for j < 4 and the imaginary part for j >= 4 (A), so that
+        // B_re = [E1r; -E1i] gives Re(Psi) and B_im = [E1i; E1r] gives Im(Psi) with no padding.
+        // Slots per K-step: B_re hi, B_re lo, B_im hi, B_im lo (planes of the split table: re hi,
+        // re lo, im hi, im lo, -im hi, -im lo).
         const size_t ps = (size_t)NB * KP;
-        const __bf16 *pe = Et1 + (size_t)q1 * kPlanes * ps + (size_t)(16 * bb + c16) * KP + 8 * g;
+        const __bf16 *pe = Et1 + (size_t)q1 * kPlanes * ps + (size_t)(16 * bb + c16) * KP + 4 * g;
         if (w == 0) {
+            auto cat = [&](int pa, int pb, int s) {
+                const uint2 lo = *(const uint2 *)(pe + pa * ps + 16 * s), hi = *(const uint2 *)(pe + pb * ps + 16 * s);
+                return __builtin_bit_cast(v8bf, make_uint4(lo.x, lo.y, hi.x, hi.y));
+            };
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-                for (int p = 0; p < kPlanes; ++p) e1s[(ks * kPlanes + p) * 64 + lane] = ld8(pe + p * ps + 32 * ks);
+            for (int s = 0; s < NB / 16; ++s) {
+                e1s[(4 * s + 0) * 64 + lane] = cat(0, 4, s);
+                e1s[(4 * s + 1) * 64 + lane] = cat(1, 5, s);
+                e1s[(4 * s + 2) * 64 + lane] = cat(2, 0, s);
+                e1s[(4 * s + 3) * 64 + lane] = cat(3, 1, s);
+            }
         }
     }
     __syncthreads();
@@ -691,20 +703,19 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     const T *Wt = (const T *)W_ + (size_t)tl * 2 * NB * NB2;
     const __amdgpu_buffer_rsrc_t wr = rsrc(Wt, sizeof(T) * NB * NB2);
     const __amdgpu_buffer_rsrc_t wi = rsrc(Wt + (size_t)NB * NB2, sizeof(T) * NB * NB2);
-    constexpr int RW = (PREC == 2) ? 8 : 1;
-    T rr[KS][RW], ri[KS][RW];
+    // BF16X3: per K-step of 16 components the lane's 4 (re and im) values of W; else one value
+    constexpr int KW = (PREC == 2) ? NB / 16 : KS, RW = (PREC == 2) ? 4 : 1;
+    T rr[KW][RW], ri[KW][RW];
     const T *E1r = (const T *)E1re_ + (size_t)q1 * NB2 + 16 * bb + c16;
     const T *E1i = (const T *)E1im_ + (size_t)q1 * NB2 + 16 * bb + c16;
     auto load_w = [&](int b2) {
         const int n = b2 * NB + b3w + c16;
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
+        for (int ks = 0; ks < KW; ++ks) {
             if constexpr (PREC == 2) {
-                // padded K rows (k >= NB) meet zero E1 fragments; read a valid row instead
-                const int kb = min(32 * ks + 8 * g, NB - 8);
-                const int vo = (int)sizeof(T) * (kb * NB2 + n);
+                const int vo = (int)sizeof(T) * ((16 * ks + 4 * g) * NB2 + n);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < 4; ++j) {
                     rr[ks][j] = bload(wr, vo, (int)sizeof(T) * j * NB2);
                     ri[ks][j] = bload(wi, vo, (int)sizeof(T) * j * NB2);
                 }
@@ -808,25 +819,22 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         const int b2 = cb + cs;
         Acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
         if constexpr (PREC == 2) {
-            Split ar[KS], ai[KS];
+            Split a8[KW];
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                split8(rr[ks], ar[ks]);
-                split8(ri[ks], ai[ks]);
+            for (int ks = 0; ks < KW; ++ks) {
+                const float x[8] = {rr[ks][0], rr[ks][1], rr[ks][2], rr[ks][3],
+                                    ri[ks][0], ri[ks][1], ri[ks][2], ri[ks][3]};
+                split8(x, a8[ks]);
             }
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                Split br, bi, bn;
-                br.h = e1s[(ks * kPlanes + 0) * 64 + lane];
-                br.l = e1s[(ks * kPlanes + 1) * 64 + lane];
-                bi.h = e1s[(ks * kPlanes + 2) * 64 + lane];
-                bi.l = e1s[(ks * kPlanes + 3) * 64 + lane];
-                bn.h = e1s[(ks * kPlanes + 4) * 64 + lane];
-                bn.l = e1s[(ks * kPlanes + 5) * 64 + lane];
-                mma3(cr, ar[ks], br);
-                mma3(cr, ai[ks], bn);
-                mma3(ci, ar[ks], bi);
-                mma3(ci, ai[ks], br);
+            for (int ks = 0; ks < KW; ++ks) {
+                Split bre, bim;
+                bre.h = e1s[(4 * ks + 0) * 64 + lane];
+                bre.l = e1s[(4 * ks + 1) * 64 + lane];
+                bim.h = e1s[(4 * ks + 2) * 64 + lane];
+                bim.l = e1s[(4 * ks + 3) * 64 + lane];
+                mma3(cr, a8[ks], bre);
+                mma3(ci, a8[ks], bim);
             }
             // next live tile's W is in flight during the epilogue
             nxt = next_live(b2 + 1);
@@ -1090,13 +1098,12 @@ int dispatch_psi2(const MfmaPlan &p, hipStream_t st) {
 size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
     const size_t es = (precision == TC3_PREC_FP64) ? 8 : 4;
     const size_t kp = kpad(nb), nw = nb / 16, nmw = (nb + 31) / 32;
-    const size_t ks = (precision == TC3_PREC_BF16X3) ? (nb + 31) / 32 : nb / 4;
     size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * (kp + 8) : es * 2 * nb * nb;
     uw += 2 * es * 27 * (size_t)maxblk + 16 * (size_t)maxblk;
     uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
     size_t ps = es * (2 * nty * nb + 2 * nb + nw * kChunk * 16 * nty) + 4 * nb;
     ps = ((ps + 15) & ~(size_t)15);
-    if (precision == TC3_PREC_BF16X3) ps += ks * kPlanes * 64 * 16;
+    if (precision == TC3_PREC_BF16X3) ps += (size_t)(nb / 16) * 4 * 64 * 16;
     ps += 8 * (nb + 16 + 16 * nty) + 4 * nb + 4 * 16 + 4 * (nb + 16) * nmw;
     ps = ((ps + 15) & ~(size_t)15) + es * 3 * (16 + 2 * (size_t)nb);
     return std::max(uw, ps);
