@@ -226,12 +226,14 @@ __global__ void k_f4_phase(int nent, const int *__restrict__ fq, int t0, const d
     }
 }
 
+// phi4 summed in FP64, stored in the contraction's type T (double, or float in the mixed modes)
+template <typename T>
 __global__ void k_f4_phi(int nb, int na, int nent, const int *__restrict__ blk4, const double *__restrict__ mwm,
-                         const cd *__restrict__ ph, cd *__restrict__ X) {
+                         const cd *__restrict__ ph, cplx<T> *__restrict__ X) {
     const int tl = blockIdx.y;
     const long nb2 = (long)nb * nb, nb3 = nb2 * nb, n4 = nb3 * nb;
     const cd *pt = ph + (size_t)tl * nent;
-    cd *out = X + (size_t)tl * n4;
+    cplx<T> *out = X + (size_t)tl * n4;
     for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n4; x += (long)gridDim.x * blockDim.x) {
         const int ia = (int)(x / nb3), ib = (int)((x / nb2) % nb), ic = (int)((x / nb) % nb), id = (int)(x % nb);
         const int blk = (((ia / 3) * na + ib / 3) * na + ic / 3) * na + id / 3;
@@ -242,54 +244,62 @@ __global__ void k_f4_phi(int nb, int na, int nent, const int *__restrict__ blk4,
             re += m * pt[p].x;
             im += m * pt[p].y;
         }
-        out[x] = {re, im};
+        out[x] = {(T)re, (T)im};
     }
 }
 
 typedef double v4d_f4 __attribute__((ext_vector_type(4)));
+typedef float v4f_f4 __attribute__((ext_vector_type(4)));
+template <typename T> struct F4Acc { typedef v4d_f4 V; };
+template <> struct F4Acc<float> { typedef v4f_f4 V; };
+__device__ inline void f4_mma(v4d_f4 &c, double a, double b) { c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline void f4_mma(v4f_f4 &c, float a, float b) { c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+// output row of accumulator register i in a 16 x 16 tile: m = 4 i + g (FP64), 4 g + i (FP32)
+template <typename T> __device__ inline int f4_row(int i, int g) { return sizeof(T) == 8 ? 4 * i + g : 4 * g + i; }
 
-// Y[tl][r][b] = sum_k X[tl][k][r] E_q[k][b], r < nb^3, k, b < nb, complex FP64 on
-// v_mfma_f64_16x16x4f64 (A[m = lane % 16][k = k0 + lane / 16], B[k][n = lane % 16], D rows
-// m = 4 reg + lane / 16).  Workgroup: 4 waves of 16 rows r each, every column tile of b.
-// E_q: the leg's table, E + q nb^2 with q = fq[5 t + leg].
-template <int NT>
+// Y[tl][r][b] = sum_k X[tl][k][r] E_q[k][b], r < nb^3, k, b < nb, complex, on
+// v_mfma_f64_16x16x4f64 (T = double) or v_mfma_f32_16x16x4f32 (T = float): A[m = lane % 16]
+// [k = k0 + lane / 16], B[k][n = lane % 16], D rows f4_row.  Workgroup: 4 waves of 16 rows r
+// each, every column tile of b.  E_q: the leg's table, E + q nb^2 with q = fq[5 t + leg].
+template <typename T, int NT>
 __global__ void __launch_bounds__(256) k_f4_stage(int nb, const int *__restrict__ fq, int t0, int leg,
-                                                  const cd *__restrict__ E, const cd *__restrict__ X,
-                                                  cd *__restrict__ Y) {
+                                                  const cplx<T> *__restrict__ E, const cplx<T> *__restrict__ X,
+                                                  cplx<T> *__restrict__ Y) {
+    typedef typename F4Acc<T>::V V;
     const int tl = blockIdx.y;
     const long nb2 = (long)nb * nb, R = nb2 * nb;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
     const long r0 = (long)blockIdx.x * 64 + 16 * w;
     if (r0 >= R) return;
-    const cd *Xt = X + (size_t)tl * R * nb;
-    const cd *Eq = E + (size_t)fq[5 * (t0 + tl) + leg] * nb2;
-    v4d_f4 ar[NT], ai[NT];
+    const cplx<T> *Xt = X + (size_t)tl * R * nb;
+    const cplx<T> *Eq = E + (size_t)fq[5 * (t0 + tl) + leg] * nb2;
+    V ar[NT], ai[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) ar[n] = ai[n] = (v4d_f4){0, 0, 0, 0};
+    for (int n = 0; n < NT; ++n) ar[n] = ai[n] = (V){0, 0, 0, 0};
     const long rr = r0 + c16;
     for (int k0 = 0; k0 < nb; k0 += 4) {
         const int k = k0 + g;
-        cd a = {0.0, 0.0};
+        cplx<T> a = {0, 0};
         if (k < nb && rr < R) a = Xt[(size_t)k * R + rr];
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
             const int col = 16 * n + c16;
-            cd b = {0.0, 0.0};
+            cplx<T> b = {0, 0};
             if (k < nb && col < nb) b = Eq[(size_t)k * nb + col];
-            ar[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, ar[n], 0, 0, 0);
-            ar[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, ar[n], 0, 0, 0);
-            ai[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ai[n], 0, 0, 0);
-            ai[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ai[n], 0, 0, 0);
+            f4_mma(ar[n], a.x, b.x);
+            f4_mma(ar[n], -a.y, b.y);
+            f4_mma(ai[n], a.x, b.y);
+            f4_mma(ai[n], a.y, b.x);
         }
     }
-    cd *Yt = Y + (size_t)tl * R * nb;
+    cplx<T> *Yt = Y + (size_t)tl * R * nb;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
         const int col = 16 * n + c16;
         if (col >= nb) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long row = r0 + 4 * i + g;
+            const long row = r0 + f4_row<T>(i, g);
             if (row < R) Yt[(size_t)row * nb + col] = {ar[n][i], ai[n][i]};
         }
     }
@@ -303,7 +313,7 @@ struct F4EpiArgs {
     const double *u_irr, *u_full;
     double sigma_fixed, thresh, pref;
     const float *tw;
-    const cd *psi;        // [tl][b1][b2][b3][b4]
+    const void *psi;      // [tl][b1][b2][b3][b4], cplx<T> of the contraction
     double *g, *s;        // [t][T][nb], [t][T][3][nb][nb]
 };
 
@@ -321,6 +331,7 @@ __device__ inline double f4_block_sum(double v, double *red) {
 // Every thread keeps the (b3, b4) pairs p = tid + 256 j; their sums over b2 go to LDS
 // ([T][b3][b4], each slot written by its own thread), the sums over (b3, b4) at every b2
 // are workgroup sums.
+template <typename CT>
 __global__ void __launch_bounds__(kF4EpiBlock) k_f4_epi(F4EpiArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nb = a.nb, nb2 = nb * nb, tl = blockIdx.y, t = a.t0 + tl, b1 = blockIdx.x;
@@ -352,7 +363,7 @@ __global__ void __launch_bounds__(kF4EpiBlock) k_f4_epi(F4EpiArgs a) {
     const double tw = a.tw ? (double)a.tw[(size_t)t * nb + b1] : 1.0;
     const double sg1 = a.sigsq[(size_t)q1 * nb + b1];
     const double *u1 = (a.itype == 6) ? a.u_irr + ((size_t)q1 * nb + b1) * 3 : nullptr;
-    const cd *ps = a.psi + ((size_t)tl * nb + b1) * (size_t)nb2 * nb;
+    const cplx<CT> *ps = (const cplx<CT> *)a.psi + ((size_t)tl * nb + b1) * (size_t)nb2 * nb;
     for (int b2 = 0; b2 < nb; ++b2) {
         double part[kF4Tb] = {0.0, 0.0, 0.0, 0.0};
         const double w2 = w234[b2];
@@ -360,7 +371,7 @@ __global__ void __launch_bounds__(kF4EpiBlock) k_f4_epi(F4EpiArgs a) {
             const int b3 = p / nb, b4 = p % nb;
             const double w3 = w234[nb + b3], w4 = w234[2 * nb + b4];
             if (!(row_ok && w2 >= kFreqTol && w3 >= kFreqTol && w4 >= kFreqTol)) continue;
-            const cd v = ps[(size_t)b2 * nb2 + p];
+            const cplx<CT> v = ps[(size_t)b2 * nb2 + p];
             double sgm;
             if (a.itype == 1) {
                 sgm = a.sigma_fixed;
@@ -381,7 +392,8 @@ __global__ void __launch_bounds__(kF4EpiBlock) k_f4_epi(F4EpiArgs a) {
             const double d1 = gauss(w1, -w2 + w3 + w4, sgm) - gauss(w1, w2 - w3 - w4, sgm);
             const double d2 = gauss(w1, w2 - w3 + w4, sgm) - gauss(w1, -w2 + w3 - w4, sgm);
             const double d3 = gauss(w1, w2 + w3 - w4, sgm) - gauss(w1, -w2 - w3 + w4, sgm);
-            const double pq = a.pref * (v.x * v.x + v.y * v.y) * tw;
+            const double vx = (double)v.x, vy = (double)v.y;
+            const double pq = a.pref * (vx * vx + vy * vy) * tw;
 #pragma unroll
             for (int T = 0; T < kF4Tb; ++T) {
                 if (T >= a.nTb) break;
@@ -590,74 +602,80 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
         TRY(hipGetLastError());
     } else {
         // batched: phi4 and the contraction steps of up to nqb quartets at once, two buffers of
-        // nb^4 complex values per quartet (at most 1 GiB each)
-        const size_t n4 = (size_t)nb * nb * nb * nb, per = n4 * sizeof(cd);
-        int nqb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nqt, ((size_t)1 << 30) / per));
-        if (const char *env = std::getenv("TC3_F4_NQB")) nqb = std::max(1, std::min(nqb, std::atoi(env)));
-        const int nent = std::max(c->nfc4, 1);
-        void *pE1, *pEf, *pph, *pX, *pY;
-        TRY(c->pool_get("f4_E1", sizeof(cd) * (size_t)nirr * nb * nb, &pE1));
-        TRY(c->pool_get("f4_Ef", sizeof(cd) * (size_t)np * nb * nb, &pEf));
-        TRY(c->pool_get("f4_ph", sizeof(cd) * (size_t)nqb * nent, &pph));
-        TRY(c->pool_get("f4_X", per * nqb, &pX));
-        TRY(c->pool_get("f4_Y", per * nqb, &pY));
-        hipLaunchKernelGGL(k_scale<double>, dim3(256), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
-                           c->d_om_irr, (cplx<double> *)pE1);
-        hipLaunchKernelGGL(k_scale<double>, dim3(1024), dim3(256), 0, st, (long)np * nb * nb, nb, c->d_egv_full,
-                           c->d_om_full, (cplx<double> *)pEf);
-        TRY(hipGetLastError());
-        const cd *E1 = (const cd *)pE1, *Ef = (const cd *)pEf;
-        cd *X = (cd *)pX, *Y = (cd *)pY;
-        // epilogue: temperature slots that fit 100 KiB of LDS (at least one)
-        int tb = kF4Tb;
-        while (tb > 1 && f4_epi_lds(nb, tb) > 100 * 1024) --tb;
-        const size_t shm = f4_epi_lds(nb, tb);
-        if (shm > 160 * 1024) return c->fail(5, "band count too large for the four-phonon epilogue's LDS");
-        TRY(hipFuncSetAttribute((const void *)k_f4_epi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        const long R = (long)nb * nb * nb;
-        const unsigned sgx = (unsigned)((R + 63) / 64);
-        const int NT = (nb + 15) / 16;
-        auto stage = [&](int leg, const cd *src, cd *dst, int t0, int nq) -> hipError_t {
-            const dim3 gs(sgx, (unsigned)nq);
-            const cd *E = leg == 0 ? E1 : Ef;
-            switch (NT) {
-            case 1: hipLaunchKernelGGL(k_f4_stage<1>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
-            case 2: hipLaunchKernelGGL(k_f4_stage<2>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
-            case 3: hipLaunchKernelGGL(k_f4_stage<3>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
-            case 4: hipLaunchKernelGGL(k_f4_stage<4>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
-            case 5: hipLaunchKernelGGL(k_f4_stage<5>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
-            case 6: hipLaunchKernelGGL(k_f4_stage<6>, gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
-            default: return hipErrorInvalidValue;
-            }
-            return hipGetLastError();
-        };
-        if (NT > 6) return c->fail(5, "four-phonon contraction handles at most 96 bands");
-        F4EpiArgs ep{};
-        ep.nb = nb; ep.nT = nT; ep.nirr = nirr; ep.tb = tb; ep.itype = a.itype;
-        ep.fq = a.fq; ep.om_irr = a.om_irr; ep.om_full = a.om_full; ep.be = a.be; ep.sigsq = a.sigsq;
-        ep.irr_of_full = a.irr_of_full; ep.row_of = a.row_of; ep.u_irr = a.u_irr; ep.u_full = a.u_full;
-        ep.sigma_fixed = a.sigma_fixed; ep.thresh = a.thresh; ep.pref = a.pref; ep.tw = a.tw;
-        ep.psi = X; ep.g = a.g; ep.s = a.s;
-        for (int t0 = 0; t0 < nqt; t0 += nqb) {
-            const int nq = std::min(nqb, nqt - t0);
-            hipLaunchKernelGGL(k_f4_phase, dim3((unsigned)((nent + 255) / 256), (unsigned)nq), dim3(256), 0, st, nent,
-                               a.fq, t0, a.qcart, a.lv, (cd *)pph);
-            hipLaunchKernelGGL(k_f4_phi, dim3((unsigned)std::min<size_t>((n4 + 255) / 256, 65535), (unsigned)nq),
-                               dim3(256), 0, st, nb, na, nent, a.blk4, a.mwm, (const cd *)pph, X);
+        // nb^4 complex values per quartet (at most 1 GiB each); FP64, or FP32 in the mixed modes
+        auto batched = [&](auto tag) -> int {
+            typedef decltype(tag) T;
+            const size_t n4 = (size_t)nb * nb * nb * nb, per = n4 * sizeof(cplx<T>);
+            int nqb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nqt, ((size_t)1 << 30) / per));
+            if (const char *env = std::getenv("TC3_F4_NQB")) nqb = std::max(1, std::min(nqb, std::atoi(env)));
+            const int nent = std::max(c->nfc4, 1);
+            void *pE1, *pEf, *pph, *pX, *pY;
+            const char *tg = sizeof(T) == 8 ? "d" : "f";
+            TRY(c->pool_get((std::string("f4_E1") + tg).c_str(), sizeof(cplx<T>) * (size_t)nirr * nb * nb, &pE1));
+            TRY(c->pool_get((std::string("f4_Ef") + tg).c_str(), sizeof(cplx<T>) * (size_t)np * nb * nb, &pEf));
+            TRY(c->pool_get("f4_ph", sizeof(cd) * (size_t)nqb * nent, &pph));
+            TRY(c->pool_get("f4_X", per * nqb, &pX));
+            TRY(c->pool_get("f4_Y", per * nqb, &pY));
+            hipLaunchKernelGGL(k_scale<T>, dim3(256), dim3(256), 0, st, (long)nirr * nb * nb, nb, c->d_egv_irr,
+                               c->d_om_irr, (cplx<T> *)pE1);
+            hipLaunchKernelGGL(k_scale<T>, dim3(1024), dim3(256), 0, st, (long)np * nb * nb, nb, c->d_egv_full,
+                               c->d_om_full, (cplx<T> *)pEf);
             TRY(hipGetLastError());
-            // ia with e1 (q1), ib with e2, ic with e3, id with e4: X -> Y -> X -> Y -> X
-            TRY(stage(0, X, Y, t0, nq));
-            TRY(stage(1, Y, X, t0, nq));
-            TRY(stage(2, X, Y, t0, nq));
-            TRY(stage(3, Y, X, t0, nq));
-            ep.t0 = t0;
-            for (int T0 = 0; T0 < nT; T0 += tb) {
-                ep.T0 = T0;
-                ep.nTb = std::min(tb, nT - T0);
-                hipLaunchKernelGGL(k_f4_epi, dim3((unsigned)nb, (unsigned)nq), dim3(kF4EpiBlock), shm, st, ep);
+            const cplx<T> *E1 = (const cplx<T> *)pE1, *Ef = (const cplx<T> *)pEf;
+            cplx<T> *X = (cplx<T> *)pX, *Y = (cplx<T> *)pY;
+            // epilogue: temperature slots that fit 100 KiB of LDS (at least one)
+            int tb = kF4Tb;
+            while (tb > 1 && f4_epi_lds(nb, tb) > 100 * 1024) --tb;
+            const size_t shm = f4_epi_lds(nb, tb);
+            if (shm > 160 * 1024) return c->fail(5, "band count too large for the four-phonon epilogue's LDS");
+            TRY(hipFuncSetAttribute((const void *)k_f4_epi<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            const long R = (long)nb * nb * nb;
+            const unsigned sgx = (unsigned)((R + 63) / 64);
+            const int NT = (nb + 15) / 16;
+            if (NT > 6) return c->fail(5, "four-phonon contraction handles at most 96 bands");
+            auto stage = [&](int leg, const cplx<T> *src, cplx<T> *dst, int t0, int nq) -> hipError_t {
+                const dim3 gs(sgx, (unsigned)nq);
+                const cplx<T> *E = leg == 0 ? E1 : Ef;
+                switch (NT) {
+                case 1: hipLaunchKernelGGL((k_f4_stage<T, 1>), gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+                case 2: hipLaunchKernelGGL((k_f4_stage<T, 2>), gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+                case 3: hipLaunchKernelGGL((k_f4_stage<T, 3>), gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+                case 4: hipLaunchKernelGGL((k_f4_stage<T, 4>), gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+                case 5: hipLaunchKernelGGL((k_f4_stage<T, 5>), gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+                default: hipLaunchKernelGGL((k_f4_stage<T, 6>), gs, dim3(256), 0, st, nb, a.fq, t0, leg, E, src, dst); break;
+                }
+                return hipGetLastError();
+            };
+            F4EpiArgs ep{};
+            ep.nb = nb; ep.nT = nT; ep.nirr = nirr; ep.tb = tb; ep.itype = a.itype;
+            ep.fq = a.fq; ep.om_irr = a.om_irr; ep.om_full = a.om_full; ep.be = a.be; ep.sigsq = a.sigsq;
+            ep.irr_of_full = a.irr_of_full; ep.row_of = a.row_of; ep.u_irr = a.u_irr; ep.u_full = a.u_full;
+            ep.sigma_fixed = a.sigma_fixed; ep.thresh = a.thresh; ep.pref = a.pref; ep.tw = a.tw;
+            ep.psi = X; ep.g = a.g; ep.s = a.s;
+            for (int t0 = 0; t0 < nqt; t0 += nqb) {
+                const int nq = std::min(nqb, nqt - t0);
+                hipLaunchKernelGGL(k_f4_phase, dim3((unsigned)((nent + 255) / 256), (unsigned)nq), dim3(256), 0, st,
+                                   nent, a.fq, t0, a.qcart, a.lv, (cd *)pph);
+                hipLaunchKernelGGL(k_f4_phi<T>, dim3((unsigned)std::min<size_t>((n4 + 255) / 256, 65535), (unsigned)nq),
+                                   dim3(256), 0, st, nb, na, nent, a.blk4, a.mwm, (const cd *)pph, X);
                 TRY(hipGetLastError());
+                // ia with e1 (q1), ib with e2, ic with e3, id with e4: X -> Y -> X -> Y -> X
+                TRY(stage(0, X, Y, t0, nq));
+                TRY(stage(1, Y, X, t0, nq));
+                TRY(stage(2, X, Y, t0, nq));
+                TRY(stage(3, Y, X, t0, nq));
+                ep.t0 = t0;
+                for (int T0 = 0; T0 < nT; T0 += tb) {
+                    ep.T0 = T0;
+                    ep.nTb = std::min(tb, nT - T0);
+                    hipLaunchKernelGGL(k_f4_epi<T>, dim3((unsigned)nb, (unsigned)nq), dim3(kF4EpiBlock), shm, st, ep);
+                    TRY(hipGetLastError());
+                }
             }
-        }
+            return 0;
+        };
+        const int rb = in.precision == 0 ? batched(double{}) : batched(float{});
+        if (rb) return rb;
     }
     std::vector<double> hg((size_t)nqt * nT * nb), hs((size_t)nqt * nT * 3 * nb * nb);
     TRY(hipMemcpyAsync(hg.data(), a.g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, st));

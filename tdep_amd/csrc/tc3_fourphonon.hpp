@@ -9,6 +9,7 @@ namespace tc3 {
 
 struct F4Inputs {
     int nT = 0, integrationtype = 2;
+    int precision = 0;   // contraction of cells above two atoms: 0 FP64, else FP32 (the mixed modes)
     double sigma_fixed = 0, thresh = 0;
     const double *d_be = nullptr, *d_sigsq = nullptr, *d_uirr = nullptr, *d_ufull = nullptr;
     const int *d_rowof = nullptr;

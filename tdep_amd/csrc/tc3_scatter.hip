@@ -1509,6 +1509,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     if (c->nfc4 > 0) {
         F4Inputs fi;
         fi.nT = nT; fi.integrationtype = integrationtype; fi.sigma_fixed = ea.sigma_fixed; fi.thresh = c->thresh6;
+        fi.precision = precision;
         fi.d_be = d_be; fi.d_sigsq = d_sigsq; fi.d_uirr = d_uirr; fi.d_ufull = d_ufull; fi.d_rowof = d_rowof;
         std::vector<double> od4;
         const int r4 = fourphonon_terms(c, fi, g4, od4);

@@ -110,8 +110,8 @@ def test_fourphonon_48_bands_matches_golden(tmp_path, prec):
     term on the MFMA chain and the four-phonon term through the batched kernels; linewidths
     and twelve scattering-matrix rows against the oracle's (tests/golden/make_golden.py,
     FOUR_CASES; the oracle takes about a minute here).  FP64 within 1e-9 of the largest value;
-    with the split-BF16x3 three-phonon contraction (the four-phonon term stays FP64, the
-    scattering matrix is stored in FP32) within the mixed mode's 1e-4 budget."""
+    with the split-BF16x3 three-phonon contraction (the four-phonon contraction then in FP32,
+    the scattering matrix stored in FP32) within the mixed mode's 1e-4 budget."""
     from tdep_amd import inputs
     from tdep_amd.engine import Engine
     root = os.path.dirname(os.path.abspath(__file__))
@@ -156,3 +156,28 @@ def test_fourphonon_batched_kernels_on_small_cells(tmp_path, monkeypatch, name, 
     xi, rq = e.scattering_matrix(0)
     assert [tuple(x) for x in rq] == list(sr.rows)
     assert _rel(xi, sr.Xi) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_fourphonon_mixed_precision_matches_oracle(tmp_path, prec):
+    """The mixed modes on the 4-atom cell (12 bands): three-phonon contraction in FP32 or
+    split-BF16x3, the batched four-phonon contraction in FP32 (FP64 epilogue); linewidths and
+    scattering matrix within 1e-5 of the FP64 oracle's largest value (both contractions carry
+    about 1e-6 relative error at this size)."""
+    from oracle import lattice as L, run as R, scattering as S
+    from tdep_amd import inputs
+    from tdep_amd.engine import Engine
+    print(f"[case] fourphonon mixed {prec} sl4 (3, 3, 3)", flush=True)
+    d = str(tmp_path / "sl4")
+    inputs.make_config("sl4", d, fourth=True)
+    grid = (3, 3, 3)
+    e = Engine.from_directory(d, grid, device=0)
+    e.set_fourthorder(d)
+    e.dispersions()
+    e.scattering_rates([300.0], precision=prec)
+    p, fc2, fc3, ops, mesh, disp = R.setup(d, grid)
+    fc4 = L.read_fc4(p, os.path.join(d, "infile.forceconstant_fourthorder"))
+    sr = S.scattering_rates(p, fc3, mesh, disp, ops, 300.0, fc4=fc4)
+    assert _rel(e.linewidths(0)[0], sr.linewidth) < 1e-5
+    xi, rq = e.scattering_matrix(0)
+    assert _rel(xi, sr.Xi) < 1e-5
