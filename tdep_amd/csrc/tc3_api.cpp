@@ -389,8 +389,8 @@ int tc3_harmonic(tc3_ctx *ctx, double *omega_min) {
 
 int tc3_get_harmonic(const tc3_ctx *ctx, int full, double *omega, double *vel, double *egv) {
     if (!ctx || !ctx->have_harmonic) return 5;
-    if (full) {
-        // full-grid results are copied from the device on first request
+    if (full || egv) {
+        // eigenvectors and full-grid results are copied from the device on first request
         tc3_ctx *mc = const_cast<tc3_ctx *>(ctx);
         const int r = guarded(mc, [&]() -> int { return ensure_full_host(mc); });
         if (r) return r;

@@ -68,7 +68,7 @@ struct tc3_ctx {
     // harmonic results (host copies); egv stored [q][component][mode]
     std::vector<double> om_irr, vel_irr, om_full, vel_full;
     std::vector<tc3::cd> egv_irr, egv_full;
-    bool full_host_valid = false;             // om/vel/egv_full hold the device results
+    bool full_host_valid = false;             // egv_irr and om/vel/egv_full hold the device results
     int harm_dims[3] = {-1, -1, -1};          // nb, nirr, nfull of the device harmonic arrays
     std::vector<std::vector<std::vector<int>>> degen;  // [q_irr][band] -> degenerate bands
     double omega_min = 0;

@@ -737,13 +737,11 @@ int run_harmonic(tc3_ctx *c) {
     TRY(hipMemcpyAsync(stat.data(), d_stat, sizeof(int) * nirr, hipMemcpyDeviceToHost, st));
     c->om_irr.resize((size_t)nb * nirr);
     c->vel_irr.resize((size_t)3 * nb * nirr);
-    c->egv_irr.resize(nn * nirr);
     TRY(hipMemcpyAsync(c->om_irr.data(), c->d_om_irr, sizeof(double) * nb * nirr, hipMemcpyDeviceToHost, st));
     TRY(hipMemcpyAsync(c->vel_irr.data(), c->d_vel_irr, sizeof(double) * 3 * nb * nirr, hipMemcpyDeviceToHost, st));
-    TRY(hipMemcpyAsync(c->egv_irr.data(), c->d_egv_irr, sizeof(cd) * nn * nirr, hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
 #undef TRY
-    // full-grid results stay on the device; host copies are made on first use
+    // eigenvectors and full-grid results stay on the device; host copies are made on first use
     c->full_host_valid = false;
     for (int k = 0; k < nirr; ++k)
         if (stat[k]) return c->fail(2, "Jacobi eigen-solver did not converge");
@@ -829,13 +827,17 @@ int run_harmonic_jvp(tc3_ctx *c, const std::vector<double> &dpm, HarmonicJvp &ou
 
 int ensure_full_host(tc3_ctx *c) {
     if (c->full_host_valid) return 0;
-    const int nb = c->nb, nfull = c->mesh.nfull;
+    const int nb = c->nb, nfull = c->mesh.nfull, nirr = c->mesh.nirr;
     const size_t nn = (size_t)nb * nb;
     c->om_full.resize((size_t)nb * nfull);
     c->vel_full.resize((size_t)3 * nb * nfull);
     c->egv_full.resize(nn * nfull);
-    hipError_t e = hipMemcpyAsync(c->om_full.data(), c->d_om_full, sizeof(double) * nb * nfull,
-                                  hipMemcpyDeviceToHost, c->stream);
+    c->egv_irr.resize(nn * nirr);
+    hipError_t e = hipMemcpyAsync(c->egv_irr.data(), c->d_egv_irr, sizeof(cd) * nn * nirr, hipMemcpyDeviceToHost,
+                                  c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->om_full.data(), c->d_om_full, sizeof(double) * nb * nfull, hipMemcpyDeviceToHost,
+                           c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->vel_full.data(), c->d_vel_full, sizeof(double) * 3 * nb * nfull, hipMemcpyDeviceToHost,
                            c->stream);

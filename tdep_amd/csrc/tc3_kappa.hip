@@ -384,6 +384,7 @@ void od_real_velocities(const tc3_ctx *c, const std::vector<zc> &V, std::vector<
 
 // Host form of the coherence velocities (TC3_OD_HOST; the check of the device form).
 void od_velocities_host(tc3_ctx *c, std::vector<double> &vel) {
+    if (ensure_full_host(c)) return;   // the irreducible eigenvectors on the host
     const int nb = c->nb, nirr = c->mesh.nirr;
     const size_t nb2 = (size_t)nb * nb;
     std::vector<zc> V((size_t)nirr * 3 * nb2, zc(0, 0));
