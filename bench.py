@@ -82,6 +82,10 @@ def kappa_wall(eng, temps, prec):
     tm = eng.timings()
     stages = {k: float(tm[k]) for k in ("harmonic", "scatter", "kappa", "bte")}
     stages["bte_iterations"] = [int(x) for x in eng.niter]
+    # the scattering matrix as the BTE product reads it: elements per temperature, bytes each,
+    # rows stored per orbit of the little group of their q-point or in full
+    n, es, per_orbit = eng.xi_storage()
+    stages["xi_elements_per_T"], stages["xi_entry_bytes"], stages["xi_per_orbit"] = n, es, per_orbit
     return wall, ks, ki, stages
 
 
@@ -257,8 +261,11 @@ def main():
             roof["traffic_per_triplet_vs_output"] = traffic / batch / out_trip
         # HBM-bound stages: rows of the scattering matrix assembled from the compact sums
         # (read once, rows written once) and the BTE (every iteration of every temperature
-        # streams its scattering matrix and the folded-out F)
-        xi_T = eng.nrows * eng.nfull * nb * es
+        # streams its scattering matrix and the folded-out F).  The rows are stored as the
+        # product reads them: one value per orbit of the little group of their q-point when
+        # nothing is added after the orbit average (xi_per_orbit), else nfull nb per row.
+        xi_T = st_mx["xi_elements_per_T"] * st_mx["xi_entry_bytes"]
+        xi_full_T = eng.nrows * eng.nfull * nb * es
         if n_asm:
             by = len(temps) * (eng.ntriplets * nb * 2 * nb * es + xi_T)
             roof["k_xi_asm"] = dict(bytes=by, ms=ms_asm, GBps=by / (ms_asm * 1e-3) / 1e9,
@@ -268,7 +275,10 @@ def main():
         roof["bte"] = dict(bytes=by_bte, ms=st_mx["bte"] * 1e3, GBps=by_bte / st_mx["bte"] / 1e9,
                            frac=by_bte / st_mx["bte"] / 1e9 / PEAK_HBM_GBPS,
                            temperature_iterations=int(sum(it_mx)),
-                           note="whole BTE (fold-out, Xi.F, update, kappa log, checks) against the Xi + F bytes")
+                           xi_bytes_per_T=xi_T, xi_full_bytes_per_T=xi_full_T,
+                           xi_per_orbit=st_mx["xi_per_orbit"],
+                           note="whole BTE (orbit sums of F, Xi.F, update, kappa log, checks) against the "
+                                "Xi bytes as stored + F bytes")
         res = {
             "metric": "3ph matrix elements/s + κ wall-time, mixed vs FP64 GPU @≤1e-4 rel κ err",
             "value": elements / t_mx,

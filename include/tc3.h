@@ -268,6 +268,13 @@ int tc3_get_psi2(tc3_ctx *ctx, int precision, int n, const int64_t *triplets, do
  * reported by tc3_get_kernel_stats.  No counterpart in the reference (a tuning aid). */
 int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int *n);
 
+/* Device storage of the scattering matrix of the last tc3_scatter: elements per temperature,
+ * bytes per element (8 in FP64 mode, 4 in the mixed modes) and whether the rows are stored per
+ * orbit of the little group of their q-point (1: one value per orbit and band, which
+ * tc3_get_xi expands to the full rows; 0: nfull*nb values per row).  No counterpart in the
+ * reference (a measurement aid: the bytes the BTE product reads per iteration). */
+int tc3_get_xi_storage(const tc3_ctx *ctx, int64_t *elems_per_T, int *entry_bytes, int *per_orbit);
+
 /* Fourth-order IFCs: replaces lo_forceconstant_fourthorder%readfromfile
  * (type_forceconstant_fourthorder.f90:259-340) and switches four-phonon scattering on in the
  * following tc3_scatter calls (--fourthorder; nquart = 0 switches it off).  lv2, lv3, lv4:

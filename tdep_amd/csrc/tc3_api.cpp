@@ -501,6 +501,34 @@ int tc3_get_xi(const tc3_ctx *ctx, int iT, int row0, int nrow, double *xi, int *
     if (xi && nrow > 0) {
         if (row0 < 0 || row0 + nrow > ctx->nrows) return 1;
         hipSetDevice(ctx->device);
+        if (ctx->xi_orb) {
+            // rows stored per orbit: expanded to every member of each orbit of the row's q1
+            const size_t e0 = (size_t)ctx->xo_row[row0];
+            const size_t e1 = (row0 + nrow < ctx->nrows) ? (size_t)ctx->xo_row[row0 + nrow] : ctx->xo_T;
+            std::vector<double> rows(e1 - e0);
+            const char *src = (const char *)ctx->d_xi + (size_t)ctx->xi_es * ((size_t)iT * ctx->xo_T + e0);
+            if (ctx->xi_es == 8) {
+                if (hipMemcpy(rows.data(), src, sizeof(double) * rows.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                    return 7;
+            } else {
+                std::vector<float> tmp(rows.size());
+                if (hipMemcpy(tmp.data(), src, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                    return 7;
+                std::copy(tmp.begin(), tmp.end(), rows.begin());
+            }
+            const int nb = ctx->nb;
+            const tc3::OrbitSet &o = ctx->orb;
+            for (int r = 0; r < nrow; ++r) {
+                const int q = ctx->row_q[row0 + r];
+                const double *x = rows.data() + ((size_t)ctx->xo_row[row0 + r] - e0);
+                double *out = xi + (size_t)r * ctx->xi_row;
+                for (int k = o.q1_off[q]; k < o.q1_off[q + 1]; ++k)
+                    for (int m = o.orb_off[k]; m < o.orb_off[k + 1]; ++m)
+                        std::copy(x + (size_t)(k - o.q1_off[q]) * nb, x + (size_t)(k - o.q1_off[q] + 1) * nb,
+                                  out + (size_t)o.members[m] * nb);
+            }
+            return 0;
+        }
         const size_t n = ctx->xi_row * (size_t)nrow;
         const char *src = (const char *)ctx->d_xi + (size_t)ctx->xi_es * ((size_t)iT * ctx->xi_row * ctx->nrows +
                                                                           (size_t)row0 * ctx->xi_row);
@@ -676,6 +704,14 @@ int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *l
     if (total_ms) *total_ms = ctx->kstat_ms[which];
     if (launches) *launches = ctx->kstat_n[which];
     if (batch) *batch = ctx->batch;
+    return 0;
+}
+
+int tc3_get_xi_storage(const tc3_ctx *ctx, int64_t *elems_per_T, int *entry_bytes, int *per_orbit) {
+    if (!ctx || !ctx->have_scatter) return 5;
+    if (elems_per_T) *elems_per_T = ctx->xi_orb ? (int64_t)ctx->xo_T : (int64_t)(ctx->xi_row * ctx->nrows);
+    if (entry_bytes) *entry_bytes = ctx->xi_es;
+    if (per_orbit) *per_orbit = ctx->xi_orb ? 1 : 0;
     return 0;
 }
 

@@ -156,6 +156,21 @@ struct tc3_ctx {
     std::vector<int> xi_src;
     bool xi_src_ok = false;
     long xi_src_gen = -1;
+    // Scattering matrix stored per orbit (xi_orb): once symmetrised, a row is constant over
+    // every orbit of the little group of its q1, so row r holds one value per orbit and band,
+    // o nb + b for the o-th orbit of q1, at xo_row[r] (elements per temperature: xo_T).  The
+    // irreducible points with the same orbit list share a group whose F orbit sums (3 planes
+    // of norb nb, at xo_goff[g] of the xo_F elements per temperature) the BTE product reads;
+    // group -1 where every orbit is the single point of its index (the folded-out F itself).
+    // Row groups of the product, at most 16 rows of one q1: (first row, rows, group, norb nb).
+    // Orbit entries of the sums: (first, end of its members in xo_mem, offset of the orbit's
+    // values in the sums, norb nb of its group).
+    bool xi_orb = false;
+    long xo_gen = -1;
+    std::vector<long long> xo_row, xo_goff;
+    std::vector<int4> xo_rg, xo_ent;
+    std::vector<int> xo_mem;
+    size_t xo_T = 0, xo_F = 0;
     // IFC block tables of the fused 48-band kernel (per model)
     std::vector<int4> fused_blocks, fused_pairs;
     std::vector<int> fused_pa1;

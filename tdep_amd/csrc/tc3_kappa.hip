@@ -648,6 +648,101 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv_w(int nrows, long ncol, cons
     }
 }
 
+// Rows stored per orbit (one value per orbit of the little group of the row's q1 and band):
+// the product needs, per orbit, the sum of the folded-out F over its members.  Entry e is
+// one orbit of one group: Fo[i][ent.z + b] (plane stride ent.w = norb nb of the group) =
+// sum over the members p of Fbb[i][p nb + b], in member order, in FP64 (rounded to FT once).
+template <typename FT>
+__global__ void k_orbsum(int nent, int nfull, int nb, const int4 *__restrict__ ent, const int *__restrict__ mem,
+                         const FT *__restrict__ Fbb_, FT *__restrict__ Fo_, const BteState *__restrict__ st_, long sFbb,
+                         long sFo) {
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const FT *Fbb = Fbb_ + T * sFbb;
+    FT *Fo = Fo_ + T * sFo;
+    const long ncol = (long)nfull * nb;
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < (long)nent * nb; x += (long)gridDim.x * blockDim.x) {
+        const int e = (int)(x / nb), b = (int)(x % nb);
+        const int4 en = ent[e];
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int m = en.x; m < en.y; ++m) {
+            const long j = (long)mem[m] * nb + b;
+            s0 += (double)Fbb[j];
+            s1 += (double)Fbb[ncol + j];
+            s2 += (double)Fbb[2 * ncol + j];
+        }
+        FT *o = Fo + en.z + b;
+        o[0] = (FT)s0;
+        o[en.w] = (FT)s1;
+        o[2 * (long)en.w] = (FT)s2;
+    }
+}
+
+// k_xi_gemv_w over rows stored per orbit: workgroup per (row group, column slice), a group
+// being up to 16 rows of one q1 (consecutive, norb nb elements each), against the orbit sums
+// of the group's orbit list (the folded-out F itself where every orbit is one point).
+static_assert(kRows * (kBlock / 64) >= 16, "a workgroup takes a whole row group");
+template <typename XT, int VW, typename FT>
+__global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, const int4 *__restrict__ rg_,
+                                                      const long long *__restrict__ xo_row,
+                                                      const long long *__restrict__ goff, const XT *__restrict__ xi_,
+                                                      const FT *__restrict__ Fbb_, const FT *__restrict__ Fo_,
+                                                      double *__restrict__ part_, const BteState *__restrict__ st_,
+                                                      size_t sXi, long sFbb, long sFo, long sPart) {
+    typedef XT XV __attribute__((ext_vector_type(VW)));
+    typedef FT DV __attribute__((ext_vector_type(VW)));
+    const int T = blockIdx.z;
+    if (st_[T].done) return;
+    const int4 rg = rg_[blockIdx.x];   // first row, rows, group, norb nb
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = rg.x + wave * kRows, sl = blockIdx.y;
+    const int nr = min(kRows, rg.x + rg.y - r0);
+    if (nr <= 0) return;
+    const long ncol = rg.w;
+    const XT *x = xi_ + T * sXi + xo_row[r0];
+    const FT *Fbb = rg.z < 0 ? Fbb_ + T * sFbb : Fo_ + T * sFo + goff[rg.z];
+    double *part = part_ + T * sPart;
+    const long nv = ncol / VW;
+    const long v0 = nv * sl / kSlices, v1 = nv * (sl + 1) / kSlices;
+    double s[kRows][3];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
+    for (long v = v0 + lane; v < v1; v += 64) {
+        const long j = v * VW;
+        const DV f0 = *(const DV *)(Fbb + j), f1 = *(const DV *)(Fbb + ncol + j), f2 = *(const DV *)(Fbb + 2 * ncol + j);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (r < nr) {
+                const XV xv = *(const XV *)(x + (size_t)r * ncol + j);
+#pragma unroll
+                for (int k = 0; k < VW; ++k) {
+                    const double e = (double)xv[k];
+                    s[r][0] += e * (double)f0[k];
+                    s[r][1] += e * (double)f1[k];
+                    s[r][2] += e * (double)f2[k];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double v = s[r][i];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            s[r][i] = v;
+        }
+    if (lane < 3 * nr) {
+        double v = 0;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                if (lane == 3 * r + i) v = s[r][i];
+        part[((size_t)sl * nrows + r0) * 3 + lane] = v;
+    }
+}
+
 // Fnb = -buf/qs; degeneracy average then +F0, band by band in order (kappa.f90:415-448);
 // convergence partial and mixing (kappa.f90:456-468).  Workgroup per irreducible point;
 // the ordered degeneracy pass runs over LDS copies: where the degenerate sets of q partition
@@ -1100,6 +1195,22 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(dalloc((void **)&d_cvw, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
     TRY(dalloc((void **)&d_klog, sizeof(double) * sLog * nT));
+    // rows stored per orbit: row groups, row offsets, the groups' orbit entries and the F
+    // orbit sums
+    const bool orb = c->xi_orb;
+    const long sFo = orb ? (long)std::max<size_t>(c->xo_F, 1) : 0;
+    int4 *d_xorg = nullptr, *d_xoent = nullptr;
+    long long *d_xorow = nullptr, *d_xogoff = nullptr;
+    int *d_xomem = nullptr;
+    double *d_Fo = nullptr;
+    if (orb) {
+        TRY(dalloc((void **)&d_xorg, sizeof(int4) * c->xo_rg.size()));
+        TRY(dalloc((void **)&d_xorow, sizeof(long long) * c->xo_row.size()));
+        TRY(dalloc((void **)&d_xogoff, sizeof(long long) * c->xo_goff.size()));
+        TRY(dalloc((void **)&d_xoent, sizeof(int4) * c->xo_ent.size()));
+        TRY(dalloc((void **)&d_xomem, sizeof(int) * c->xo_mem.size()));
+        TRY(dalloc((void **)&d_Fo, (size_t)c->xi_es * sFo * nT));
+    }
     double *d_part, *d_scf;
     BteState *d_st;
     TRY(dalloc((void **)&d_part, sizeof(double) * sPart * nT));
@@ -1117,6 +1228,14 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(hipMemcpyAsync(d_dgp, dgpart.data(), sizeof(int) * dgpart.size(), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_star_off, star_off.data(), sizeof(int) * (nirr + 1), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(d_star, star.data(), sizeof(int) * star.size(), hipMemcpyHostToDevice, st));
+    if (orb) {
+        auto up = [&](void *d, const void *h, size_t n) { return n ? hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st) : hipSuccess; };
+        TRY(up(d_xorg, c->xo_rg.data(), sizeof(int4) * c->xo_rg.size()));
+        TRY(up(d_xorow, c->xo_row.data(), sizeof(long long) * c->xo_row.size()));
+        TRY(up(d_xogoff, c->xo_goff.data(), sizeof(long long) * c->xo_goff.size()));
+        TRY(up(d_xoent, c->xo_ent.data(), sizeof(int4) * c->xo_ent.size()));
+        TRY(up(d_xomem, c->xo_mem.data(), sizeof(int) * c->xo_mem.size()));
+    }
     double t_bte = 0, t_od = 0;
     // off-diagonal term: on the device for all temperatures at once (the star rotation of the
     // per-point sums on the host), or all on the host with TC3_OD_HOST
@@ -1213,7 +1332,37 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         // one iteration for every temperature still running: Xi.F, update and fold-out, kappa
         // log, convergence test (kappa.f90:395-489)
         auto iteration = [&]() {
-            if (nrows > 0) {
+            if (nrows > 0 && orb) {
+                const int nent = (int)c->xo_ent.size(), nrg = (int)c->xo_rg.size();
+                const long ne = (long)nent * nb;
+                const dim3 gs((unsigned)std::max<long>(1, std::min<long>(1024, (ne + 255) / 256)), 1, nT);
+                const dim3 gw((unsigned)nrg, kSlices, nT);
+                if (f32) {
+                    if (nent > 0)
+                        hipLaunchKernelGGL(k_orbsum<float>, gs, dim3(256), 0, st, nent, nfull, nb, d_xoent, d_xomem,
+                                           (const float *)d_Fbb, (float *)d_Fo, d_st, sFbb, sFo);
+                    if (nb % 4 == 0)
+                        hipLaunchKernelGGL((k_xi_gemv_o<float, 4, float>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                                           d_xorow, d_xogoff, (const float *)c->d_xi, (const float *)d_Fbb,
+                                           (const float *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
+                    else
+                        hipLaunchKernelGGL((k_xi_gemv_o<float, 1, float>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                                           d_xorow, d_xogoff, (const float *)c->d_xi, (const float *)d_Fbb,
+                                           (const float *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
+                } else {
+                    if (nent > 0)
+                        hipLaunchKernelGGL(k_orbsum<double>, gs, dim3(256), 0, st, nent, nfull, nb, d_xoent, d_xomem,
+                                           (const double *)d_Fbb, d_Fo, d_st, sFbb, sFo);
+                    if (nb % 2 == 0)
+                        hipLaunchKernelGGL((k_xi_gemv_o<double, 2, double>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                                           d_xorow, d_xogoff, (const double *)c->d_xi, (const double *)d_Fbb,
+                                           (const double *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
+                    else
+                        hipLaunchKernelGGL((k_xi_gemv_o<double, 1, double>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                                           d_xorow, d_xogoff, (const double *)c->d_xi, (const double *)d_Fbb,
+                                           (const double *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
+                }
+            } else if (nrows > 0) {
                 const dim3 gw((nrows + 4 * kRows - 1) / (4 * kRows), kSlices, nT);
                 const long ncol = (long)c->xi_row;
                 if (!f32 && ncol % 2 == 0)
@@ -1254,7 +1403,10 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                                        (long)(uintptr_t)d_opf, (long)(uintptr_t)d_rowof, (long)(uintptr_t)d_dgoff, (long)(uintptr_t)d_dgp,
                                        (long)(uintptr_t)d_dg, (long)(uintptr_t)d_cvw, (long)(uintptr_t)d_vel,
                                        (long)(uintptr_t)d_klog, (long)(uintptr_t)d_part, (long)(uintptr_t)d_scf,
-                                       (long)(uintptr_t)d_st, sF, sFbb, sPart, sG, sScf, sLog};
+                                       (long)(uintptr_t)d_st, sF, sFbb, sPart, sG, sScf, sLog, (long)orb,
+                                       (long)c->xo_T, (long)c->xo_rg.size(), (long)c->xo_ent.size(), sFo,
+                                       (long)(uintptr_t)d_xorg, (long)(uintptr_t)d_xorow, (long)(uintptr_t)d_xogoff,
+                                       (long)(uintptr_t)d_xoent, (long)(uintptr_t)d_xomem, (long)(uintptr_t)d_Fo};
         if (!scratch && c->bte_exec && c->bte_key != key) {
             (void)hipGraphExecDestroy(c->bte_exec);
             c->bte_exec = nullptr;

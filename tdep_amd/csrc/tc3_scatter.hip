@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -645,6 +646,8 @@ constexpr int kOrbTemps = TC3_ORB_TEMPS;   // temperatures per workgroup
 
 // Workgroup per (row, block of kOrbTemps temperatures): the owner table and the frequencies
 // are read once for the block's temperatures, whose sums are independent loads in flight.
+// xo_row non-null (AVG only): the rows are stored per orbit (o nb + b for the o-th orbit of
+// q1, row r at xo_row[r]), each orbit's value once.
 template <typename XT, typename S, bool AVG>
 __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull, int nT, const int *__restrict__ row_q,
                                                    const int *__restrict__ row_b, const double *__restrict__ om_irr,
@@ -652,7 +655,7 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull,
                                                    const S *__restrict__ comp, size_t comp_T,
                                                    const int *__restrict__ orb_q1, const int *__restrict__ orb_off,
                                                    const int *__restrict__ orb_ns, XT *__restrict__ xi, size_t xi_row,
-                                                   size_t xi_T) {
+                                                   size_t xi_T, const long long *__restrict__ xo_row) {
     constexpr int TB = kOrbTemps;
     const int L = nb / 3;
     const int gs = blockDim.x / L, g0 = threadIdx.x / L, b = 3 * (threadIdx.x % L);
@@ -660,12 +663,15 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull,
     const int r = blockIdx.x % nrows, T0 = (blockIdx.x / nrows) * TB, nt = min(TB, nT - T0);
     const int q1 = row_q[r], b1 = row_b[r];
     const double iw1 = 2.0 / om_irr[(size_t)q1 * nb + b1];
+    const bool orbout = AVG && xo_row != nullptr;
+    const size_t rbase = orbout ? (size_t)xo_row[r] : (size_t)r * xi_row;
+    const size_t rlen = orbout ? (size_t)(orb_q1[q1 + 1] - orb_q1[q1]) * nb : xi_row;
     // byte offsets below 2^32 (checked by the caller)
     __amdgpu_buffer_rsrc_t rx[TB], rc[TB];
 #pragma unroll
     for (int t = 0; t < TB; ++t) {
         const int Tt = min(T0 + t, nT - 1);
-        rx[t] = xrsrc(xi + (size_t)Tt * xi_T + (size_t)r * xi_row, (unsigned)(xi_row * sizeof(XT)));
+        rx[t] = xrsrc(xi + (size_t)Tt * xi_T + rbase, (unsigned)(rlen * sizeof(XT)));
         rc[t] = xrsrc(comp + (size_t)Tt * comp_T, (unsigned)(comp_T * sizeof(S)));
     }
     const __amdgpu_buffer_rsrc_t ro = xrsrc(om_full, (unsigned)((size_t)nfull * nb * sizeof(double)));
@@ -723,7 +729,8 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull,
 #pragma unroll
                     for (int j = 0; j < 3; ++j) e[t][j] = (XT)(0.0 + (double)e[t][j]);
             }
-            store(e, ms[k].x);
+            // per orbit: these orbits of one member are the first orbits of q1, in member order
+            store(e, orbout ? m + k * gs - pa : ms[k].x);
         }
     }
     if constexpr (AVG) {
@@ -785,6 +792,12 @@ __global__ void __launch_bounds__(kBlock) k_xi_asm(int nrows, int nb, int nfull,
                     for (int m = m0; m < m1; ++m) sum += avg;
                 }
                 out[t] = (XT)(sum / (m1 - m0));
+            }
+            if (orbout) {
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    if (t < nt) xstore<XT>(out[t], rx[t], (unsigned)((o - orb_q1[q1]) * nb + b0) * (unsigned)sizeof(XT));
+                continue;
             }
             for (int m = m0; m < m1; m += kOrbDepth) {
                 int mm[kOrbDepth];
@@ -849,6 +862,65 @@ bool xi_source_map(const tc3::TripletSet &tr, const tc3::OrbitSet &orb, int nirr
 
 // Nonzero (a2, a3) IFC blocks of every atom a1, in (a1, a2, a3) order, with the offsets of
 // each a1 and each (a1, a2) and the most blocks of one a1 (the MFMA kernels' Phi blocks).
+// Tables of the scattering matrix stored per orbit (tc3_ctx::xi_orb), from the orbits of the
+// little group of every irreducible point.  False if an offset would not fit the tables' ints.
+static bool build_orbit_layout(tc3_ctx *c) {
+    if (c->xo_gen == c->gen_model) return true;
+    const int nirr = c->mesh.nirr, nfull = c->mesh.nfull, nb = c->nb, nrows = c->nrows;
+    const OrbitSet &o = c->orb;
+    std::vector<int> grp(nirr, -1);
+    c->xo_goff.clear();
+    c->xo_ent.clear();
+    c->xo_mem.clear();
+    std::map<std::vector<int>, int> groups;
+    long long foff = 0;
+    for (int q = 0; q < nirr; ++q) {
+        const int oa = o.q1_off[q], ob = o.q1_off[q + 1], norb = ob - oa;
+        // the orbit list as (size, members...) per orbit
+        std::vector<int> key;
+        bool ident = norb == nfull;
+        for (int x = oa; x < ob; ++x) {
+            key.push_back(o.orb_off[x + 1] - o.orb_off[x]);
+            for (int m = o.orb_off[x]; m < o.orb_off[x + 1]; ++m) key.push_back(o.members[m]);
+            ident = ident && o.orb_off[x + 1] - o.orb_off[x] == 1 && o.members[o.orb_off[x]] == x - oa;
+        }
+        if (ident) continue;
+        auto it = groups.find(key);
+        if (it == groups.end()) {
+            it = groups.emplace(key, (int)c->xo_goff.size()).first;
+            c->xo_goff.push_back(foff);
+            for (int x = oa; x < ob; ++x) {
+                const int m0 = (int)c->xo_mem.size();
+                for (int m = o.orb_off[x]; m < o.orb_off[x + 1]; ++m) c->xo_mem.push_back(o.members[m]);
+                c->xo_ent.push_back(make_int4(m0, (int)c->xo_mem.size(), (int)(foff + (long long)(x - oa) * nb),
+                                              norb * nb));
+            }
+            foff += 3ll * norb * nb;
+            if (foff >= (1ll << 31)) return false;
+        }
+        grp[q] = it->second;
+    }
+    c->xo_F = (size_t)foff;
+    c->xo_row.assign(nrows, 0);
+    c->xo_rg.clear();
+    long long off = 0;
+    for (int r = 0; r < nrows; ++r) {
+        c->xo_row[r] = off;
+        const int q = c->row_q[r];
+        off += (long long)(o.q1_off[q + 1] - o.q1_off[q]) * nb;
+    }
+    c->xo_T = (size_t)off;
+    for (int r = 0; r < nrows;) {
+        const int q = c->row_q[r];
+        int n = 0;
+        while (r + n < nrows && c->row_q[r + n] == q && n < 16) ++n;
+        c->xo_rg.push_back(make_int4(r, n, grp[q], (o.q1_off[q + 1] - o.q1_off[q]) * nb));
+        r += n;
+    }
+    c->xo_gen = c->gen_model;
+    return true;
+}
+
 static void mfma_block_lists(const tc3_ctx *c, std::vector<int4> &blist, std::vector<int> &a1_boff,
                              std::vector<int> &a12_boff, int &maxblk) {
     const int na = c->cr.na;
@@ -1071,11 +1143,34 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
     hipLaunchKernelGGL(k_occupations, dim3((unsigned)std::min<size_t>(1024, (n_be + 255) / 256)), dim3(256), 0, st, nT,
                        nirr * nb, d_temps, c->d_om_irr, (int)c->classical, d_be);
     TRY(hipGetLastError());
-    // scattering matrix: every entry that any triplet owns is rewritten by every pass, so
-    // the buffer is zeroed only when it is new or the model changed
+    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
     c->xi_row = (size_t)nfull * nb;
-    const size_t xi_T = c->xi_row * nrows;
     c->xi_es = (precision == TC3_PREC_FP64) ? 8 : 4;
+    // compact per-triplet sums, assembled into the rows by k_xi_asm, when every entry has one
+    // owner (TC3_XI_DIRECT: direct stores); k_xi_asm addresses one temperature's sums and one
+    // row with 32-bit byte offsets
+    const size_t comp_T = (size_t)ntrip * nb * 2 * nb;
+    bool comp_ok = false;
+    if (use_mfma && c->thirdorder) {
+        if (c->xi_src_gen != c->gen_model) {
+            c->xi_src_ok = xi_source_map(c->trip, c->orb, nirr, nfull, nb, c->xi_src);
+            c->xi_src_gen = c->gen_model;
+        }
+        comp_ok = c->xi_src_ok && (double)c->xi_es * comp_T < 4.0e9 && (double)c->xi_es * c->xi_row < 4.0e9 &&
+                  std::getenv("TC3_XI_DIRECT") == nullptr;
+    }
+    // isotope scattering (scattering.f90:229-232): on when requested and some atom has mass
+    // disorder
+    bool use_iso = false;
+    for (double g : c->iso_g) use_iso = use_iso || (c->iso_on && g > 0.0);
+    // rows stored per orbit when k_xi_asm writes the averaged rows and nothing is added to
+    // them afterwards (no isotope or four-phonon terms); TC3_XI_FULL keeps the full rows
+    c->xi_orb = comp_ok && !use_iso && c->nfc4 == 0 && std::getenv("TC3_XI_FULL") == nullptr &&
+                build_orbit_layout(c);
+    // scattering matrix: every entry that any triplet owns is rewritten by every pass, so
+    // the full rows are zeroed only when they are new or the model changed; k_xi_asm writes
+    // every element of the rows stored per orbit
+    const size_t xi_T = c->xi_orb ? c->xo_T : c->xi_row * nrows;
     if (c->xi_bytes < (size_t)c->xi_es * xi_T * nT) {
         hipFree(c->d_xi);
         c->d_xi = nullptr;
@@ -1084,14 +1179,15 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         c->xi_bytes = (size_t)c->xi_es * xi_T * nT;
         c->xi_gen = -1;
     }
-    if (c->xi_gen != c->gen_model || c->xi_es_zeroed != c->xi_es) {
+    if (c->xi_orb) {
+        c->xi_gen = -1;
+    } else if (c->xi_gen != c->gen_model || c->xi_es_zeroed != c->xi_es) {
         TRY(hipMemsetAsync(c->d_xi, 0, c->xi_bytes, st));
         c->xi_gen = c->gen_model;
         c->xi_es_zeroed = c->xi_es;
     }
     c->scatter_tables_gen = c->gen_model;
 
-    const bool use_mfma = mfma_supported(nb, precision) && std::getenv("TC3_NO_MFMA") == nullptr;
     // generic path: batch of intermediates of 4 * nb^3 complex per triplet
     const long nb3 = (long)nb * nb * nb;
     const size_t elem = (precision == TC3_PREC_FP64) ? sizeof(cd) : sizeof(cf);
@@ -1218,13 +1314,6 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
         // the fused kernel (48 bands, split-BF16x3, compact sums) needs neither W nor the
         // chain's operand planes
-        if (c->xi_src_gen != c->gen_model) {
-            c->xi_src_ok = xi_source_map(c->trip, c->orb, nirr, nfull, nb, c->xi_src);
-            c->xi_src_gen = c->gen_model;
-        }
-        const size_t comp_T = (size_t)ntrip * nb * 2 * nb;
-        const bool comp_ok = c->xi_src_ok && (double)c->xi_es * comp_T < 4.0e9 &&
-                             (double)c->xi_es * c->xi_row < 4.0e9 && std::getenv("TC3_XI_DIRECT") == nullptr;
         // IFC blocks (a1, a2, a3) in entry order and the (a2, a3) pairs with their blocks, for the
         // fused kernel: its Phi slot holds, per pair, per (j, k), per block, the three i values
         if (c->fused_gen != c->gen_model) {
@@ -1304,11 +1393,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         me.tw = ea.tw;
         me.perm = d_perm; me.om_irr = c->d_om_irr; me.om_full = c->d_om_full; me.sigsq = d_sigsq;
         me.gpart = d_gpart; me.xi_row = c->xi_row; me.xi_T = xi_T;
-        // compact per-triplet sums, assembled into the rows by k_xi_asm, when every entry
-        // has one owner (TC3_XI_DIRECT: direct stores)
         me.comp = nullptr;
         me.comp_T = comp_T;
-        // k_xi_asm addresses one temperature's sums and one row with 32-bit byte offsets
         if (comp_ok) {
             TRY(table("xisrc", (void **)&d_xisrc, c->xi_src.data(), sizeof(int4) * (c->xi_src.size() / 4)));
             TRY(c->pool_get("xi_comp", (size_t)c->xi_es * nT * me.comp_T, &me.comp));
@@ -1523,11 +1609,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
                        nT, nb, d_rowof, d_q1off, d_gpart, d_lw);
     TRY(hipGetLastError());
     TRY(hipEventRecord(evs0, st));
-    // isotope scattering (scattering.f90:229-232): on when requested and some atom has mass
-    // disorder
     IsoArgs iso{};
-    bool use_iso = false;
-    for (double g : c->iso_g) use_iso = use_iso || (c->iso_on && g > 0.0);
     if (use_iso) {
         double *d_g;
         TRY(c->pool_get("iso_g", sizeof(double) * na, (void **)&d_g));
@@ -1552,9 +1634,12 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             const int L = nb / 3;
             const dim3 bo(L * (kBlock / L)), go(nrows * ((nT + kOrbTemps - 1) / kOrbTemps));
             const bool avg = !(use_iso || d_od4);
+            long long *d_xorow = nullptr;
+            if (c->xi_orb) TRY(table("xo_row", (void **)&d_xorow, c->xo_row.data(), sizeof(long long) * nrows));
             auto launch = [&](auto kf, auto *cp, auto *xp) {
                 hipLaunchKernelGGL(kf, go, bo, 0, st, nrows, nb, nfull, nT, d_rowq, d_rowb, c->d_om_irr, c->d_om_full,
-                                   (const int4 *)d_xisrc, cp, cT, d_orbq1, d_orboff, d_orbns, xp, c->xi_row, xi_T);
+                                   (const int4 *)d_xisrc, cp, cT, d_orbq1, d_orboff, d_orbns, xp, c->xi_row, xi_T,
+                                   d_xorow);
             };
             if (c->xi_es == 8)
                 launch(avg ? k_xi_asm<double, double, true> : k_xi_asm<double, double, false>, (const double *)d_comp,

@@ -320,6 +320,13 @@ class Engine:
         self._check(self.lib.tc3_get_tuning(self.ctx, n.value, dptr(b), dptr(ms), ctypes.byref(n)))
         return [(int(b[i]), float(ms[i])) for i in range(n.value)]
 
+    def xi_storage(self):
+        """Device storage of the scattering matrix of the last scattering_rates call: (elements
+        per temperature, bytes per element, rows stored per orbit)."""
+        n, es, po = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.tc3_get_xi_storage(self.ctx, ctypes.byref(n), ctypes.byref(es), ctypes.byref(po)))
+        return int(n.value), int(es.value), bool(po.value)
+
     def write_grid_hdf5(self, path, iT=0):
         self._check(self.lib.tc3_write_grid_hdf5(self.ctx, iT, path.encode()))
 

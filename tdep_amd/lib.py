@@ -53,6 +53,7 @@ SIGNATURES = {
     "tc3_set_montecarlo": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, ctypes.c_int]),
     "tc3_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "tc3_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_int_p]),
+    "tc3_get_xi_storage": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _c_int_p, _c_int_p]),
     "tc3_adjoint_gamma": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]),
     "tc3_adjoint_kappa_rta": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]),
     "tc3_jvp": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),

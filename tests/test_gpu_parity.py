@@ -265,3 +265,37 @@ def test_fused_contraction_matches_two_kernel_path(inputs_dir, monkeypatch):
     assert _rel(lwf, lwc) < 1e-4
     for a, b in zip(kf, k64):
         assert _rel(a, b) < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_rows_stored_per_orbit_match_full_rows(inputs_dir, monkeypatch, prec):
+    """The scattering matrix stored per orbit of the little group of each row's q-point (one
+    value per orbit and band, the BTE product against the orbit sums of F) against the full
+    rows (TC3_XI_FULL=1) on the 16-atom superlattice: fewer elements stored, every row the same
+    bits once expanded, the same iteration counts, and kappa within 1e-12 relative in FP64 (the
+    products are summed per orbit first) and 1e-9 in split-BF16x3 (the orbit sums of the FP32
+    folded-out F are rounded to FP32)."""
+    e = _engine(inputs_dir, "sl16", (4, 4, 2))
+    T = [100.0, 300.0, 700.0]
+    out = {}
+    for mode in ("full", "orbit", "full again"):
+        if mode != "orbit":
+            monkeypatch.setenv("TC3_XI_FULL", "1")
+        else:
+            monkeypatch.delenv("TC3_XI_FULL", raising=False)
+        e.scattering_rates(T, precision=prec)
+        n, es, per_orbit = e.xi_storage()
+        xi = np.array([e.scattering_matrix(i)[0] for i in range(len(T))])
+        k = e.kappa(offdiag=False)
+        out[mode] = (n, es, per_orbit, xi, k, np.array(e.niter[:len(T)]))
+    nf, esf, pof, xif, kf, itf = out["full"]
+    no, eso, poo, xio, ko, ito = out["orbit"]
+    assert not pof and poo and esf == eso == (8 if prec == "fp64" else 4)
+    assert nf == e.nrows * e.nfull * e.nb and no < nf
+    assert np.array_equal(xif, xio)
+    assert np.array_equal(itf, ito)
+    # the full rows after the per-orbit ones: zeroed again where no triplet writes
+    assert np.array_equal(out["full again"][3], xif)
+    tol = 1e-12 if prec == "fp64" else 1e-9
+    for a, b in zip(ko, kf):
+        assert _rel(a, b) < tol
