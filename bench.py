@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed mixed-precision pass (for profiler runs); prints no JSON")
+    ap.add_argument("--kappa-walls", type=int, default=0, metavar="N",
+                    help="only N kappa walls (harmonic step, scattering pass, iterative BTE) in the "
+                         "given precision, after one untimed wall (for profiler runs); prints no JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (the reference kernel for scaling the HBM "
                          "byte counters); prints no JSON")
@@ -145,6 +148,12 @@ def main():
         nb = eng.nb
         elements = eng.ntriplets * nb ** 3
         log(f"[bench] nirr={eng.nirr} triplets={eng.ntriplets} nb={nb} elements/step={elements:.3e}")
+        if args.kappa_walls:
+            kappa_wall(eng, temps, args.precision)
+            for i in range(args.kappa_walls):
+                kw, _, _, st = kappa_wall(eng, temps, args.precision)
+                log(f"[bench] kappa wall {i} {args.precision}: {kw * 1e3:.2f} ms, stages {st}")
+            return
         # mixed (timed) pass
         t_mx = scatter_pass(eng, temps, args.precision, args.steps, args.warmup)
         ks_mx = kernel_stats(eng)

@@ -165,6 +165,7 @@ struct tc3_ctx {
     // Row groups of the product, at most 16 rows of one q1: (first row, rows, group, norb nb).
     // Orbit entries of the sums: (first, end of its members in xo_mem, offset of the orbit's
     // values in the sums, norb nb of its group).
+    long harm_tables_gen = -1;  // model generation the harmonic step's tables were uploaded for
     bool xi_orb = false;
     long xo_gen = -1;
     std::vector<long long> xo_row, xo_goff;

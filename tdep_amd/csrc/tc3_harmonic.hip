@@ -656,8 +656,15 @@ int run_harmonic(tc3_ctx *c) {
     auto cleanup = [&]() {};
     hipError_t e = hipSuccess;
 #define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return device_error(e, #x, __FILE__, __LINE__); } } while (0)
-    // work buffers from the context's pool (kept across calls)
-    auto work = [&](const char *name, void **p, size_t bytes) { return c->pool_get(name, std::max<size_t>(bytes, 16), p); };
+    // work buffers from the context's pool (kept across calls); the model's tables are uploaded
+    // when the model changed since the last upload or a buffer is new
+    bool fresh_any = false;
+    auto work = [&](const char *name, void **p, size_t bytes) {
+        bool fresh = false;
+        const hipError_t r = c->pool_get(name, std::max<size_t>(bytes, 16), p, &fresh);
+        fresh_any = fresh_any || fresh;
+        return r;
+    };
     TRY(work("h_qv", (void **)&d_qv, sizeof(double) * 3 * nirr));
     TRY(work("h_qirr", (void **)&d_qirr, sizeof(double) * 3 * nirr));
     TRY(work("h_lv", (void **)&d_lv, sizeof(double) * 3 * std::max(npair, 1)));
@@ -691,21 +698,26 @@ int run_harmonic(tc3_ctx *c) {
         TRY(hipMalloc(&c->d_egv_full, sizeof(cd) * nn * nfull));
         TRY(hipMalloc(&c->d_irr_of_full, sizeof(int) * nfull));
         c->harm_dims[0] = nb; c->harm_dims[1] = nirr; c->harm_dims[2] = nfull;
+        fresh_any = true;
     }
-    TRY(hipMemcpyAsync(d_qv, qv.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_qirr, qirr.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
-    if (npair) {
-        TRY(hipMemcpyAsync(d_lv, c->p_lv.data(), sizeof(double) * 3 * npair, hipMemcpyHostToDevice, st));
-        TRY(hipMemcpyAsync(d_pm, c->p_m.data(), sizeof(double) * 9 * npair, hipMemcpyHostToDevice, st));
+    if (fresh_any || c->harm_tables_gen != c->gen_model) {
+        c->harm_tables_gen = -1;
+        TRY(hipMemcpyAsync(d_qv, qv.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_qirr, qirr.data(), sizeof(double) * 3 * nirr, hipMemcpyHostToDevice, st));
+        if (npair) {
+            TRY(hipMemcpyAsync(d_lv, c->p_lv.data(), sizeof(double) * 3 * npair, hipMemcpyHostToDevice, st));
+            TRY(hipMemcpyAsync(d_pm, c->p_m.data(), sizeof(double) * 9 * npair, hipMemcpyHostToDevice, st));
+        }
+        TRY(hipMemcpyAsync(d_ism, c->cr.invsqrtmass.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_blk, c->blk2_off.data(), sizeof(int) * (na * na + 1), hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_atg, atg.data(), sizeof(int) * nirr, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_v0, v0tab.data(), sizeof(double) * 3 * nsym * na, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_src, src.data(), sizeof(int) * nsym * na, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(c->d_irr_of_full, c->mesh.irr_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
+        c->harm_tables_gen = c->gen_model;
     }
-    TRY(hipMemcpyAsync(d_ism, c->cr.invsqrtmass.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_blk, c->blk2_off.data(), sizeof(int) * (na * na + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_atg, atg.data(), sizeof(int) * nirr, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_v0, v0tab.data(), sizeof(double) * 3 * nsym * na, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_src, src.data(), sizeof(int) * nsym * na, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(c->d_irr_of_full, c->mesh.irr_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
 
     hipLaunchKernelGGL(k_dynmat, dim3(nirr), dim3(kBlock), 0, st, na, d_qv, d_blk, d_lv, d_pm, d_ism, d_D, d_gD);
     TRY(hipGetLastError());
@@ -733,14 +745,19 @@ int run_harmonic(tc3_ctx *c) {
                        d_src, d_v0, d_qirr, c->d_egv_irr, c->d_vel_irr, c->d_om_irr, c->d_egv_full,
                        c->d_vel_full, c->d_om_full);
     TRY(hipGetLastError());
-    std::vector<int> stat(nirr);
-    TRY(hipMemcpyAsync(stat.data(), d_stat, sizeof(int) * nirr, hipMemcpyDeviceToHost, st));
-    c->om_irr.resize((size_t)nb * nirr);
-    c->vel_irr.resize((size_t)3 * nb * nirr);
-    TRY(hipMemcpyAsync(c->om_irr.data(), c->d_om_irr, sizeof(double) * nb * nirr, hipMemcpyDeviceToHost, st));
-    TRY(hipMemcpyAsync(c->vel_irr.data(), c->d_vel_irr, sizeof(double) * 3 * nb * nirr, hipMemcpyDeviceToHost, st));
+    // solver status, frequencies and velocities down through the pinned buffer
+    const size_t b_om = sizeof(double) * nb * nirr, b_vel = 3 * b_om;
+    TRY(c->pinned(b_om + b_vel + sizeof(int) * nirr));
+    double *h_om = (double *)c->pin, *h_vel = h_om + (size_t)nb * nirr;
+    int *h_stat = (int *)(h_vel + (size_t)3 * nb * nirr);
+    TRY(hipMemcpyAsync(h_stat, d_stat, sizeof(int) * nirr, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(h_om, c->d_om_irr, b_om, hipMemcpyDeviceToHost, st));
+    TRY(hipMemcpyAsync(h_vel, c->d_vel_irr, b_vel, hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
 #undef TRY
+    std::vector<int> stat(h_stat, h_stat + nirr);
+    c->om_irr.assign(h_om, h_om + (size_t)nb * nirr);
+    c->vel_irr.assign(h_vel, h_vel + (size_t)3 * nb * nirr);
     // eigenvectors and full-grid results stay on the device; host copies are made on first use
     c->full_host_valid = false;
     for (int k = 0; k < nirr; ++k)

@@ -1154,6 +1154,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     TRY(dalloc((void **)&d_dg, sizeof(int) * dg.size()));
     TRY(dalloc((void **)&d_dgp, sizeof(int) * dgpart.size()));
     int *d_fz;
+    std::vector<int> fzero(nirr, 0);
     {
         // points where symmetry fixes no vector, so that F vanishes: F(q) = R F(q) for R q = q
         // (the scattering-matrix rows are averaged over these R, so the reference's iteration
@@ -1163,7 +1164,6 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         // Monte-Carlo grid draws q2 and -q2 independently, the reference keeps the F of
         // Monte-Carlo-noise size there, and so only the proper operations count then.
         const bool tr_fix = c->mesh.timereversal && !c->mc_on && !(c->nfc4 > 0 && c->mc4_on);
-        std::vector<int> fzero(nirr, 0);
         auto matvec3 = [](const double *m, const double *x, double *y) {
             for (int i = 0; i < 3; ++i) y[i] = m[3 * i] * x[0] + m[3 * i + 1] * x[1] + m[3 * i + 2] * x[2];
         };
@@ -1190,7 +1190,6 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
             fzero[q] = (cnt > 0 && mx < 1e-8 * (double)cnt) ? 1 : 0;
         }
         TRY(dalloc((void **)&d_fz, sizeof(int) * std::max(nirr, 1)));
-        if (!overflow) TRY(hipMemcpy(d_fz, fzero.data(), sizeof(int) * nirr, hipMemcpyHostToDevice));
     }
     TRY(dalloc((void **)&d_cvw, sizeof(double) * sQ * nT));
     TRY(dalloc((void **)&d_vel, sizeof(double) * qb * 3));
@@ -1219,23 +1218,58 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     if (overflow)
         return c->fail(1, "the iterative solution needs " + std::to_string(used >> 20) + "+ MiB, more than the " +
                               std::to_string(scratch_bytes >> 20) + " MiB scratch array holds");
-    TRY(hipMemcpyAsync(d_vel, c->vel_irr.data(), sizeof(double) * qb * 3, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_opm, opm.data(), sizeof(double) * 9 * nsym, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_rowof, c->row_of.data(), sizeof(int) * qb, hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_dgoff, dgoff.data(), sizeof(int) * (qb + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_dg, dg.data(), sizeof(int) * dg.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_dgp, dgpart.data(), sizeof(int) * dgpart.size(), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_star_off, star_off.data(), sizeof(int) * (nirr + 1), hipMemcpyHostToDevice, st));
-    TRY(hipMemcpyAsync(d_star, star.data(), sizeof(int) * star.size(), hipMemcpyHostToDevice, st));
+    // transfers staged through the context's pinned buffer (a host vector without device
+    // allocation in scratch mode), 256-byte slots: the model's tables first, then the per-call
+    // inputs below; the downloads reuse it after the uploads, which the stream has finished by
+    // then
+    struct Upload {
+        void *dst;
+        const void *src;
+        size_t n;
+    };
+    std::vector<Upload> tables = {{d_vel, c->vel_irr.data(), sizeof(double) * qb * 3},
+                                  {d_opm, opm.data(), sizeof(double) * 9 * nsym},
+                                  {d_opf, c->mesh.op_of_full.data(), sizeof(int) * nfull},
+                                  {d_rowof, c->row_of.data(), sizeof(int) * qb},
+                                  {d_dgoff, dgoff.data(), sizeof(int) * (qb + 1)},
+                                  {d_dg, dg.data(), sizeof(int) * dg.size()},
+                                  {d_dgp, dgpart.data(), sizeof(int) * dgpart.size()},
+                                  {d_star_off, star_off.data(), sizeof(int) * (nirr + 1)},
+                                  {d_star, star.data(), sizeof(int) * star.size()},
+                                  {d_fz, fzero.data(), sizeof(int) * nirr}};
     if (orb) {
-        auto up = [&](void *d, const void *h, size_t n) { return n ? hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st) : hipSuccess; };
-        TRY(up(d_xorg, c->xo_rg.data(), sizeof(int4) * c->xo_rg.size()));
-        TRY(up(d_xorow, c->xo_row.data(), sizeof(long long) * c->xo_row.size()));
-        TRY(up(d_xogoff, c->xo_goff.data(), sizeof(long long) * c->xo_goff.size()));
-        TRY(up(d_xoent, c->xo_ent.data(), sizeof(int4) * c->xo_ent.size()));
-        TRY(up(d_xomem, c->xo_mem.data(), sizeof(int) * c->xo_mem.size()));
+        tables.push_back({d_xorg, c->xo_rg.data(), sizeof(int4) * c->xo_rg.size()});
+        tables.push_back({d_xorow, c->xo_row.data(), sizeof(long long) * c->xo_row.size()});
+        tables.push_back({d_xogoff, c->xo_goff.data(), sizeof(long long) * c->xo_goff.size()});
+        tables.push_back({d_xoent, c->xo_ent.data(), sizeof(int4) * c->xo_ent.size()});
+        tables.push_back({d_xomem, c->xo_mem.data(), sizeof(int) * c->xo_mem.size()});
     }
+    auto slot = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    size_t b_tab = 0;
+    for (const Upload &u : tables) b_tab += slot(u.n);
+    const size_t b_qT = sizeof(double) * qb * nT, b_F = sizeof(double) * qb * 3 * nT;
+    const size_t b_up = slot(b_qT) + slot(b_F) + slot(sizeof(BteState) * nT) + slot(sizeof(double) * 81) +
+                        slot(sizeof(double) * std::max(nirr, 1)) + slot(sizeof(double) * nT);
+    const size_t b_down = slot(sizeof(double) * sScf * nT) + slot(sizeof(double) * sLog * nT) + slot(b_F) +
+                          2 * slot(sizeof(double) * 9 * nT) + slot(sizeof(double) * qb * 9 * nT);
+    std::vector<char> stage_host;
+    char *stage = nullptr;
+    if (scratch) {
+        stage_host.resize(std::max(b_tab + b_up, b_down));
+        stage = stage_host.data();
+    } else {
+        TRY(c->pinned(std::max(b_tab + b_up, b_down)));
+        stage = (char *)c->pin;
+    }
+    char *h = stage;
+    auto up = [&](void *dst, const void *src, size_t n) -> hipError_t {
+        if (n == 0) return hipSuccess;
+        std::memcpy(h, src, n);
+        const hipError_t r = hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, st);
+        h += slot(n);
+        return r;
+    };
+    for (const Upload &u : tables) TRY(up(u.dst, u.src, u.n));
     double t_bte = 0, t_od = 0;
     // off-diagonal term: on the device for all temperatures at once (the star rotation of the
     // per-point sums on the host), or all on the host with TC3_OD_HOST
@@ -1271,32 +1305,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     const auto a = std::chrono::steady_clock::now();
     bool any = false;
     for (int iT = 0; iT < nT; ++iT) any = any || !hs[iT].done;
-    // transfers staged through the context's pinned buffer (a host vector without device
-    // allocation in scratch mode), 256-byte slots; the downloads reuse it after the uploads,
-    // which the stream has finished by then
-    auto slot = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    const size_t b_qT = sizeof(double) * qb * nT, b_F = sizeof(double) * qb * 3 * nT;
-    const size_t b_up = slot(b_qT) + slot(b_F) + slot(sizeof(BteState) * nT) + slot(sizeof(double) * 81) +
-                        slot(sizeof(double) * std::max(nirr, 1)) + slot(sizeof(double) * nT);
-    const size_t b_down = slot(sizeof(double) * sScf * nT) + slot(sizeof(double) * sLog * nT) + slot(b_F) +
-                          2 * slot(sizeof(double) * 9 * nT) + slot(sizeof(double) * qb * 9 * nT);
-    std::vector<char> stage_host;
-    char *stage = nullptr;
-    if (scratch) {
-        stage_host.resize(std::max(b_up, b_down));
-        stage = stage_host.data();
-    } else {
-        TRY(c->pinned(std::max(b_up, b_down)));
-        stage = (char *)c->pin;
-    }
     {
-        char *h = stage;
-        auto up = [&](void *dst, const void *src, size_t n) -> hipError_t {
-            std::memcpy(h, src, n);
-            const hipError_t r = hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, st);
-            h += slot(n);
-            return r;
-        };
         double S[81];
         op_average_map(c, S);
         TRY(up(d_F0, c->F0.data(), b_F));
