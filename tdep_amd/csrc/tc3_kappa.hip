@@ -680,10 +680,13 @@ __global__ void k_orbsum(int nent, int nfull, int nb, const int4 *__restrict__ e
 
 // k_xi_gemv_w over rows stored per orbit: workgroup per (row group, column slice), a group
 // being up to 16 rows of one q1 (consecutive, norb nb elements each), against the orbit sums
-// of the group's orbit list (the folded-out F itself where every orbit is one point).
+// of the group's orbit list (the folded-out F itself where every orbit is one point).  A row
+// of ncol elements is cut into ceil(kSlices ncol / ncol_max) slices, so that every slice
+// holds about as many columns as one of the longest rows' (the workgroups of the remaining
+// slices write zero partial sums).
 static_assert(kRows * (kBlock / 64) >= 16, "a workgroup takes a whole row group");
 template <typename XT, int VW, typename FT>
-__global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, const int4 *__restrict__ rg_,
+__global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, long ncol_max, const int4 *__restrict__ rg_,
                                                       const long long *__restrict__ xo_row,
                                                       const long long *__restrict__ goff, const XT *__restrict__ xi_,
                                                       const FT *__restrict__ Fbb_, const FT *__restrict__ Fo_,
@@ -699,11 +702,16 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, const int4 *__r
     const int nr = min(kRows, rg.x + rg.y - r0);
     if (nr <= 0) return;
     const long ncol = rg.w;
+    double *part = part_ + T * sPart;
+    const int ns = (int)min((long)kSlices, (kSlices * ncol + ncol_max - 1) / ncol_max);
+    if (sl >= ns) {
+        if (lane < 3 * nr) part[((size_t)sl * nrows + r0) * 3 + lane] = 0.0;
+        return;
+    }
     const XT *x = xi_ + T * sXi + xo_row[r0];
     const FT *Fbb = rg.z < 0 ? Fbb_ + T * sFbb : Fo_ + T * sFo + goff[rg.z];
-    double *part = part_ + T * sPart;
     const long nv = ncol / VW;
-    const long v0 = nv * sl / kSlices, v1 = nv * (sl + 1) / kSlices;
+    const long v0 = nv * sl / ns, v1 = nv * (sl + 1) / ns;
     double s[kRows][3];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) s[r][0] = s[r][1] = s[r][2] = 0;
@@ -1351,11 +1359,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                         hipLaunchKernelGGL(k_orbsum<float>, gs, dim3(256), 0, st, nent, nfull, nb, d_xoent, d_xomem,
                                            (const float *)d_Fbb, (float *)d_Fo, d_st, sFbb, sFo);
                     if (nb % 4 == 0)
-                        hipLaunchKernelGGL((k_xi_gemv_o<float, 4, float>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<float, 4, float>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
                                            d_xorow, d_xogoff, (const float *)c->d_xi, (const float *)d_Fbb,
                                            (const float *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                     else
-                        hipLaunchKernelGGL((k_xi_gemv_o<float, 1, float>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<float, 1, float>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
                                            d_xorow, d_xogoff, (const float *)c->d_xi, (const float *)d_Fbb,
                                            (const float *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                 } else {
@@ -1363,11 +1371,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                         hipLaunchKernelGGL(k_orbsum<double>, gs, dim3(256), 0, st, nent, nfull, nb, d_xoent, d_xomem,
                                            (const double *)d_Fbb, d_Fo, d_st, sFbb, sFo);
                     if (nb % 2 == 0)
-                        hipLaunchKernelGGL((k_xi_gemv_o<double, 2, double>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<double, 2, double>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
                                            d_xorow, d_xogoff, (const double *)c->d_xi, (const double *)d_Fbb,
                                            (const double *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                     else
-                        hipLaunchKernelGGL((k_xi_gemv_o<double, 1, double>), gw, dim3(kBlock), 0, st, nrows, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<double, 1, double>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
                                            d_xorow, d_xogoff, (const double *)c->d_xi, (const double *)d_Fbb,
                                            (const double *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                 }
