@@ -678,15 +678,17 @@ __global__ void k_orbsum(int nent, int nfull, int nb, const int4 *__restrict__ e
     }
 }
 
-// k_xi_gemv_w over rows stored per orbit: workgroup per (row group, column slice), a group
-// being up to 16 rows of one q1 (consecutive, norb nb elements each), against the orbit sums
-// of the group's orbit list (the folded-out F itself where every orbit is one point).  A row
-// of ncol elements is cut into ceil(kSlices ncol / ncol_max) slices, so that every slice
-// holds about as many columns as one of the longest rows' (the workgroups of the remaining
-// slices write zero partial sums).
+// k_xi_gemv_w over rows stored per orbit: workgroup per (work unit, temperature), a unit being
+// one column slice of a row group (up to 16 rows of one q1, consecutive, norb nb elements
+// each) against the orbit sums of the group's orbit list (the folded-out F itself where every
+// orbit is one point).  A row of ncol elements is cut into ns = ceil(kSlices ncol / ncol_max)
+// slices, so that every slice holds about as many columns as one of the longest rows'; the
+// last slice's workgroup writes zero partial sums into the slots after it.  Units come in
+// decreasing size with the temperature fastest, so the short ones fill the end of the launch.
 static_assert(kRows * (kBlock / 64) >= 16, "a workgroup takes a whole row group");
 template <typename XT, int VW, typename FT>
-__global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, long ncol_max, const int4 *__restrict__ rg_,
+__global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, int nT, const int4 *__restrict__ units,
+                                                      const int4 *__restrict__ rg_,
                                                       const long long *__restrict__ xo_row,
                                                       const long long *__restrict__ goff, const XT *__restrict__ xi_,
                                                       const FT *__restrict__ Fbb_, const FT *__restrict__ Fo_,
@@ -694,20 +696,18 @@ __global__ void __launch_bounds__(kBlock) k_xi_gemv_o(int nrows, long ncol_max, 
                                                       size_t sXi, long sFbb, long sFo, long sPart) {
     typedef XT XV __attribute__((ext_vector_type(VW)));
     typedef FT DV __attribute__((ext_vector_type(VW)));
-    const int T = blockIdx.z;
+    const int T = blockIdx.x % nT;
     if (st_[T].done) return;
-    const int4 rg = rg_[blockIdx.x];   // first row, rows, group, norb nb
+    const int4 un = units[blockIdx.x / nT];   // row group, slice, slices
+    const int4 rg = rg_[un.x];                // first row, rows, group, norb nb
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = rg.x + wave * kRows, sl = blockIdx.y;
+    const int r0 = rg.x + wave * kRows, sl = un.y, ns = un.z;
     const int nr = min(kRows, rg.x + rg.y - r0);
     if (nr <= 0) return;
     const long ncol = rg.w;
     double *part = part_ + T * sPart;
-    const int ns = (int)min((long)kSlices, (kSlices * ncol + ncol_max - 1) / ncol_max);
-    if (sl >= ns) {
-        if (lane < 3 * nr) part[((size_t)sl * nrows + r0) * 3 + lane] = 0.0;
-        return;
-    }
+    if (sl == ns - 1 && lane < 3 * nr)
+        for (int z = ns; z < kSlices; ++z) part[((size_t)z * nrows + r0) * 3 + lane] = 0.0;
     const XT *x = xi_ + T * sXi + xo_row[r0];
     const FT *Fbb = rg.z < 0 ? Fbb_ + T * sFbb : Fo_ + T * sFo + goff[rg.z];
     const long nv = ncol / VW;
@@ -1210,6 +1210,24 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
     long long *d_xorow = nullptr, *d_xogoff = nullptr;
     int *d_xomem = nullptr;
     double *d_Fo = nullptr;
+    // work units of the product over rows stored per orbit: (row group, slice, slices), in
+    // decreasing size
+    std::vector<int4> units;
+    int4 *d_units = nullptr;
+    if (orb) {
+        const long ncol_max = (long)nfull * nb;
+        std::vector<std::pair<long, int4>> wu;
+        for (size_t g = 0; g < c->xo_rg.size(); ++g) {
+            const int4 rg = c->xo_rg[g];
+            const int ns = (int)std::min<long>(kSlices, (kSlices * (long)rg.w + ncol_max - 1) / ncol_max);
+            for (int sl = 0; sl < ns; ++sl) wu.push_back({(long)rg.y * ((long)rg.w / ns), make_int4((int)g, sl, ns, 0)});
+        }
+        std::stable_sort(wu.begin(), wu.end(), [](const std::pair<long, int4> &a, const std::pair<long, int4> &b) {
+            return a.first > b.first;
+        });
+        for (const auto &w : wu) units.push_back(w.second);
+        TRY(dalloc((void **)&d_units, sizeof(int4) * units.size()));
+    }
     if (orb) {
         TRY(dalloc((void **)&d_xorg, sizeof(int4) * c->xo_rg.size()));
         TRY(dalloc((void **)&d_xorow, sizeof(long long) * c->xo_row.size()));
@@ -1246,6 +1264,7 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                                   {d_star, star.data(), sizeof(int) * star.size()},
                                   {d_fz, fzero.data(), sizeof(int) * nirr}};
     if (orb) {
+        tables.push_back({d_units, units.data(), sizeof(int4) * units.size()});
         tables.push_back({d_xorg, c->xo_rg.data(), sizeof(int4) * c->xo_rg.size()});
         tables.push_back({d_xorow, c->xo_row.data(), sizeof(long long) * c->xo_row.size()});
         tables.push_back({d_xogoff, c->xo_goff.data(), sizeof(long long) * c->xo_goff.size()});
@@ -1350,20 +1369,20 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         // log, convergence test (kappa.f90:395-489)
         auto iteration = [&]() {
             if (nrows > 0 && orb) {
-                const int nent = (int)c->xo_ent.size(), nrg = (int)c->xo_rg.size();
+                const int nent = (int)c->xo_ent.size();
                 const long ne = (long)nent * nb;
                 const dim3 gs((unsigned)std::max<long>(1, std::min<long>(1024, (ne + 255) / 256)), 1, nT);
-                const dim3 gw((unsigned)nrg, kSlices, nT);
+                const dim3 gw((unsigned)(units.size() * nT));
                 if (f32) {
                     if (nent > 0)
                         hipLaunchKernelGGL(k_orbsum<float>, gs, dim3(256), 0, st, nent, nfull, nb, d_xoent, d_xomem,
                                            (const float *)d_Fbb, (float *)d_Fo, d_st, sFbb, sFo);
                     if (nb % 4 == 0)
-                        hipLaunchKernelGGL((k_xi_gemv_o<float, 4, float>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<float, 4, float>), gw, dim3(kBlock), 0, st, nrows, nT, d_units, d_xorg,
                                            d_xorow, d_xogoff, (const float *)c->d_xi, (const float *)d_Fbb,
                                            (const float *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                     else
-                        hipLaunchKernelGGL((k_xi_gemv_o<float, 1, float>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<float, 1, float>), gw, dim3(kBlock), 0, st, nrows, nT, d_units, d_xorg,
                                            d_xorow, d_xogoff, (const float *)c->d_xi, (const float *)d_Fbb,
                                            (const float *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                 } else {
@@ -1371,11 +1390,11 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                         hipLaunchKernelGGL(k_orbsum<double>, gs, dim3(256), 0, st, nent, nfull, nb, d_xoent, d_xomem,
                                            (const double *)d_Fbb, d_Fo, d_st, sFbb, sFo);
                     if (nb % 2 == 0)
-                        hipLaunchKernelGGL((k_xi_gemv_o<double, 2, double>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<double, 2, double>), gw, dim3(kBlock), 0, st, nrows, nT, d_units, d_xorg,
                                            d_xorow, d_xogoff, (const double *)c->d_xi, (const double *)d_Fbb,
                                            (const double *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                     else
-                        hipLaunchKernelGGL((k_xi_gemv_o<double, 1, double>), gw, dim3(kBlock), 0, st, nrows, nj, d_xorg,
+                        hipLaunchKernelGGL((k_xi_gemv_o<double, 1, double>), gw, dim3(kBlock), 0, st, nrows, nT, d_units, d_xorg,
                                            d_xorow, d_xogoff, (const double *)c->d_xi, (const double *)d_Fbb,
                                            (const double *)d_Fo, d_part, d_st, c->xo_T, sFbb, sFo, sPart);
                 }
@@ -1423,7 +1442,8 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                                        (long)(uintptr_t)d_st, sF, sFbb, sPart, sG, sScf, sLog, (long)orb,
                                        (long)c->xo_T, (long)c->xo_rg.size(), (long)c->xo_ent.size(), sFo,
                                        (long)(uintptr_t)d_xorg, (long)(uintptr_t)d_xorow, (long)(uintptr_t)d_xogoff,
-                                       (long)(uintptr_t)d_xoent, (long)(uintptr_t)d_xomem, (long)(uintptr_t)d_Fo};
+                                       (long)(uintptr_t)d_xoent, (long)(uintptr_t)d_xomem, (long)(uintptr_t)d_Fo,
+                                       (long)units.size(), (long)(uintptr_t)d_units};
         if (!scratch && c->bte_exec && c->bte_key != key) {
             (void)hipGraphExecDestroy(c->bte_exec);
             c->bte_exec = nullptr;
