@@ -550,7 +550,7 @@ void build_quartets_mc(const tc3_ctx *c, std::vector<int> &quart, std::vector<in
 
 }  // namespace
 
-int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, std::vector<double> &od4) {
+int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, double *d_od4) {
     const int nb = c->nb, na = c->cr.na, nT = in.nT, nirr = c->mesh.nirr, np = c->mesh.nfull, nrows = (int)c->row_q.size();
     const bool mc = c->mc4_on;
     if (mc && c->mquart_gen != c->gen_model) {
@@ -564,11 +564,14 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
     const std::vector<float> &mtw = c->mquart_tw;
     const int nqt = (int)quart.size() / 5;
     g4.assign((size_t)nT * nrows, 0.0);
-    od4.assign((size_t)nT * nrows * np * nb, 0.0);
-    if (nqt == 0) return 0;
     hipStream_t st = c->stream;
     hipError_t e;
 #define TRY(x) do { e = (x); if (e != hipSuccess) return device_error(e, #x, __FILE__, __LINE__); } while (0)
+    const size_t ncol = (size_t)np * nb;
+    if (nqt == 0) {
+        TRY(hipMemsetAsync(d_od4, 0, sizeof(double) * nT * nrows * ncol, st));
+        return 0;
+    }
     auto up = [&](const char *name, const void *src, size_t bytes, void **dst) -> hipError_t {
         hipError_t r = c->pool_get(name, std::max<size_t>(bytes, 16), dst);
         if (r == hipSuccess && bytes) r = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, st);
@@ -681,21 +684,25 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
     TRY(hipMemcpyAsync(hg.data(), a.g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, st));
     TRY(hipMemcpyAsync(hs.data(), a.s, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st));
     TRY(hipStreamSynchronize(st));
-#undef TRY
-    // the terms of every image (scattering_fourphonon.f90:176-187)
+    // one temperature at a time on the host (rows x nfull nb), then to its slice of d_od4
     std::vector<int> row_of((size_t)nirr * nb, -1);
     for (int r = 0; r < nrows; ++r) row_of[(size_t)c->row_q[r] * nb + c->row_b[r]] = r;
-    const size_t ncol = (size_t)np * nb;
-    for (int t = 0; t < nqt; ++t) {
-        const int q1 = quart[5 * t], qq[3] = {quart[5 * t + 1], quart[5 * t + 2], quart[5 * t + 3]};
-        for (int T = 0; T < nT; ++T) {
+    std::vector<double> odT((size_t)nrows * ncol);
+    std::vector<double> buf(nb);
+    std::vector<int> nn(nb), imgs;
+    const long nlist = mc ? c->mc4_npoints : np;
+    for (int T = 0; T < nT; ++T) {
+        std::fill(odT.begin(), odT.end(), 0.0);
+        // the terms of every image (scattering_fourphonon.f90:176-187)
+        for (int t = 0; t < nqt; ++t) {
+            const int q1 = quart[5 * t], qq[3] = {quart[5 * t + 1], quart[5 * t + 2], quart[5 * t + 3]};
             const double *gt = &hg[((size_t)t * nT + T) * nb];
             const double *stt = &hs[((size_t)t * nT + T) * 3 * nb * nb];
             for (int b1 = 0; b1 < nb; ++b1) {
                 const int r = row_of[(size_t)q1 * nb + b1];
                 if (r < 0) continue;
                 const double w1 = c->om_irr[(size_t)q1 * nb + b1];
-                double *od = &od4[((size_t)T * nrows + r) * ncol];
+                double *od = &odT[(size_t)r * ncol];
                 for (int i = qoff[t]; i < qoff[t + 1]; ++i) {
                     g4[(size_t)T * nrows + r] += gt[b1];
                     for (int leg = 0; leg < 3; ++leg) {
@@ -709,17 +716,12 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
                 }
             }
         }
-    }
-    // symmetrize_and_distribute (scattering_fourphonon.f90:143-205), over the row's first
-    // grid in drawn order (every point on the full mesh)
-    std::vector<double> buf(nb);
-    std::vector<int> nn(nb), imgs;
-    const long nlist = mc ? c->mc4_npoints : np;
-    for (int T = 0; T < nT; ++T)
+        // symmetrize_and_distribute (scattering_fourphonon.f90:143-205), over the row's first
+        // grid in drawn order (every point on the full mesh)
         for (int r = 0; r < nrows; ++r) {
             const int q1 = c->row_q[r];
             const std::vector<int> &lg = c->mesh.littlegroup[q1];
-            double *od = &od4[((size_t)T * nrows + r) * ncol];
+            double *od = &odT[(size_t)r * ncol];
             const long *ga = mc ? &c->g4[((size_t)q1 * nb + c->row_b[r]) * 2 * c->mc4_npoints] : nullptr;
             for (long qi = 0; qi < nlist; ++qi) {
                 const int q2 = mc ? (int)ga[qi] : (int)qi;
@@ -745,6 +747,9 @@ int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, st
                         if (nn[b]) od[(size_t)x * nb + b] = buf[b] / (double)nn[b];
             }
         }
+        TRY(hipMemcpy(d_od4 + (size_t)T * nrows * ncol, odT.data(), sizeof(double) * odT.size(), hipMemcpyHostToDevice));
+    }
+#undef TRY
     return 0;
 }
 

@@ -17,6 +17,6 @@ struct F4Inputs {
 
 // Linewidth terms g4[T][row] and off-diagonal scattering-matrix terms od4[T][row][q][b],
 // already averaged over the little group of each row's q1.
-int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, std::vector<double> &od4);
+int fourphonon_terms(tc3_ctx *c, const F4Inputs &in, std::vector<double> &g4, double *d_od4);
 
 }  // namespace tc3

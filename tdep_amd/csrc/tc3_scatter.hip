@@ -1597,12 +1597,11 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         fi.nT = nT; fi.integrationtype = integrationtype; fi.sigma_fixed = ea.sigma_fixed; fi.thresh = c->thresh6;
         fi.precision = precision;
         fi.d_be = d_be; fi.d_sigsq = d_sigsq; fi.d_uirr = d_uirr; fi.d_ufull = d_ufull; fi.d_rowof = d_rowof;
-        std::vector<double> od4;
-        const int r4 = fourphonon_terms(c, fi, g4, od4);
+        // the off-diagonal terms [T][row][q nb + b], formed a temperature at a time on the host
+        TRY(c->pool_get("f4_od", sizeof(double) * std::max<size_t>((size_t)nT * nrows * nfull * nb, 1),
+                        (void **)&d_od4));
+        const int r4 = fourphonon_terms(c, fi, g4, d_od4);
         if (r4) { cleanup(); return r4; }
-        TRY(c->pool_get("f4_od", sizeof(double) * std::max<size_t>(od4.size(), 1), (void **)&d_od4));
-        TRY(hipMemcpyAsync(d_od4, od4.data(), sizeof(double) * od4.size(), hipMemcpyHostToDevice, st));
-        TRY(hipStreamSynchronize(st));
     }
     hipEvent_t evs0 = c->ev_pool[2], evs1 = c->ev_pool[3];
     hipLaunchKernelGGL(k_gamma, dim3((unsigned)nirr, (unsigned)nT), dim3((unsigned)((nb + 63) / 64 * 64)), 0, st, nrows,
