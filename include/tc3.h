@@ -208,7 +208,10 @@ int tc3_get_timings(const tc3_ctx *ctx, double *t, int n);
  * 1 last contraction + smeared-delta epilogue, 2 scattering-matrix assembly and
  * symmetrisation, 3 BTE scattering-matrix product, 4 the same as 2 when the rows were
  * assembled from compact per-triplet sums (no launch counted when the contraction kernels
- * stored the entries directly).  Also the triplet batch of the last scatter call.
+ * stored the entries directly).  Also the triplet batch of the last scatter call.  A pass
+ * whose contraction was replayed from its HIP graph (every call after the first with the same
+ * arguments, unless TC3_NO_GRAPH is set) counts no launches in classes 0 and 1: the graph
+ * records no events between its kernels.
  * No counterpart in the reference (its timers are host wall-clock buckets). */
 int tc3_get_kernel_stats(const tc3_ctx *ctx, int which, double *total_ms, int *launches,
                          int64_t *batch);

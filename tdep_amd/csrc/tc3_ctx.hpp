@@ -199,6 +199,10 @@ struct tc3_ctx {
     // instantiated BTE iteration graph and the arguments it was captured with
     hipGraphExec_t bte_exec = nullptr;
     std::vector<long> bte_key;
+    // instantiated graph of the scattering pass's contraction launches and the arguments it
+    // was captured with (sc_key_failed: the arguments a capture failed for, not retried)
+    hipGraphExec_t sc_exec = nullptr;
+    std::vector<long> sc_key, sc_key_failed;
     hipError_t pool_get(const std::string &name, size_t bytes, void **p, bool *fresh = nullptr) {
         DevBuf &b = pool[name];
         if (fresh) *fresh = false;

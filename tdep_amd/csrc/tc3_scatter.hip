@@ -921,6 +921,31 @@ static bool build_orbit_layout(tc3_ctx *c) {
     return true;
 }
 
+// Every argument the contraction's launches take (the plan, its epilogue arguments and the
+// events of each launch pair): the key of the pass's HIP graph.
+static void chain_key(std::vector<long> &k, const MfmaPlan &p, int precision, int nev) {
+    auto P = [&](const void *x) { k.push_back((long)(uintptr_t)x); };
+    auto I = [&](long x) { k.push_back(x); };
+    auto D = [&](double x) {
+        long v;
+        std::memcpy(&v, &x, sizeof(v));
+        k.push_back(v);
+    };
+    I(precision); I(p.na); I(p.nb); I(p.nirr); I(p.nfull); I(p.maxa1); I(p.maxblk); I(p.nty); I(p.ntrip);
+    I(p.batch);
+    P(p.egv_irr); P(p.egv_full); P(p.om_irr); P(p.om_full); P(p.E1re); P(p.E1im); P(p.Efre); P(p.Efim);
+    P(p.work); P(p.Eti); P(p.Etf); P(p.tq2); P(p.tq3); P(p.a1off); P(p.blist); P(p.a1_boff); P(p.a12_boff);
+    P(p.qcart); P(p.lv2); P(p.lv3); P(p.mwm);
+    const MfmaEpiArgs &e = p.epi;
+    I(e.nb); I(e.nT); I(e.t0); I(e.nt); I(e.nirr); I(e.integrationtype); I(e.gT); I(e.T0);
+    D(e.sigma_fixed); D(e.psi_scale);
+    P(e.tq1); P(e.tq2); P(e.tq3); P(e.eq_off); P(e.eq_q2); P(e.eq_q3); P(e.irr_of_full); P(e.row_of);
+    P(e.perm); P(e.om_irr); P(e.om_full); P(e.be); P(e.sigsq); P(e.xi); P(e.gpart); I((long)e.xi_row);
+    I((long)e.xi_T); P(e.comp); I((long)e.comp_T); P(e.u_irr); P(e.u_full); D(e.thresh6); P(e.tw);
+    P(e.psi_out);
+    for (int i = 0; i < nev; ++i) P(p.ev ? (const void *)p.ev[i] : nullptr);
+}
+
 static void mfma_block_lists(const tc3_ctx *c, std::vector<int4> &blist, std::vector<int> &a1_boff,
                              std::vector<int> &a12_boff, int &maxblk) {
     const int na = c->cr.na;
@@ -1499,7 +1524,7 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         TRY(take_events(kev, 3 * nbatch * ntchunk));
         TRY(hipEventRecord(ev0, st));
         // temperatures in blocks of mp.nty; each block recomputes the contraction
-        for (int tc = 0; tc < ntchunk; ++tc) {
+        auto chunk = [&](int tc) {
             const int T0 = tc * mp.nty;
             me.T0 = T0;
             me.nT = std::min(mp.nty, nT - T0);
@@ -1507,7 +1532,59 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
             me.xi = (char *)c->d_xi + (size_t)c->xi_es * T0 * xi_T;
             if (d_comp) me.comp = (char *)d_comp + (size_t)c->xi_es * T0 * me.comp_T;
             mp.ev = kev.data() + 3 * nbatch * tc;
-            int r = run_mfma_chain(mp, precision, st);
+        };
+        auto chain = [&]() -> int {
+            for (int tc = 0; tc < ntchunk; ++tc) {
+                chunk(tc);
+                const int r = run_mfma_chain(mp, precision, st);
+                if (r) return r;
+            }
+            return 0;
+        };
+        auto chain_noev = [&]() -> int {
+            for (int tc = 0; tc < ntchunk; ++tc) {
+                chunk(tc);
+                mp.ev = nullptr;
+                const int r = run_mfma_chain(mp, precision, st);
+                if (r) return r;
+            }
+            return 0;
+        };
+        // the launches of the pass (operand preparation, every launch pair with its events) are
+        // captured once as a HIP graph and replayed while every argument they take stays the
+        // same; direct launches where capture fails (TC3_NO_GRAPH: always)
+        std::vector<long> gkey;
+        for (int tc = 0; tc < ntchunk; ++tc) {
+            chunk(tc);
+            mp.ev = nullptr;
+            chain_key(gkey, mp, precision, 0);
+        }
+        const bool use_graph = std::getenv("TC3_NO_GRAPH") == nullptr;
+        if (c->sc_exec && c->sc_key != gkey) {
+            (void)hipGraphExecDestroy(c->sc_exec);
+            c->sc_exec = nullptr;
+        }
+        if (use_graph && !c->sc_exec && gkey != c->sc_key_failed &&
+            hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            const int rc = chain_noev();
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (rc == 0 && ec == hipSuccess && graph &&
+                hipGraphInstantiate(&c->sc_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                c->sc_key = gkey;
+            } else {
+                c->sc_exec = nullptr;
+                c->sc_key_failed = gkey;
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+        }
+        if (use_graph && c->sc_exec) {
+            // the graph records no per-launch events: no per-kernel statistics for this pass
+            TRY(hipGraphLaunch(c->sc_exec, st));
+            kev.clear();
+        } else {
+            const int r = chain();
             if (r) { cleanup(); return r == 5 ? c->fail(5, "IFC blocks per atom exceed the MFMA kernels' LDS") : r; }
         }
         c->batch = mbatch;

@@ -299,3 +299,25 @@ def test_rows_stored_per_orbit_match_full_rows(inputs_dir, monkeypatch, prec):
     tol = 1e-12 if prec == "fp64" else 1e-9
     for a, b in zip(ko, kf):
         assert _rel(a, b) < tol
+
+
+def test_contraction_graph_replay_matches_direct_launches(inputs_dir, monkeypatch):
+    """The scattering pass replays its contraction launches from a HIP graph after the first
+    call with the same arguments; the replays (three in a row, with other work between them)
+    give the same linewidths and scattering-matrix rows, bit for bit, as direct launches
+    (TC3_NO_GRAPH=1), in the split-BF16x3 and FP64 modes, and the graph is rebuilt when the
+    temperatures change."""
+    e = _engine(inputs_dir, "sl16", (4, 4, 2))
+    for prec in ("bf16x3", "fp64"):
+        for T in ([100.0, 300.0], [200.0, 400.0, 600.0]):
+            monkeypatch.setenv("TC3_NO_GRAPH", "1")
+            e.scattering_rates(T, precision=prec)
+            lw0 = np.array([e.linewidths(i)[0] for i in range(len(T))])
+            xi0 = np.array([e.scattering_matrix(i, 0, 32)[0] for i in range(len(T))])
+            monkeypatch.delenv("TC3_NO_GRAPH")
+            for _ in range(3):
+                e.scattering_rates(T, precision=prec)
+                e.kappa(offdiag=False)
+                lw = np.array([e.linewidths(i)[0] for i in range(len(T))])
+                xi = np.array([e.scattering_matrix(i, 0, 32)[0] for i in range(len(T))])
+                assert np.array_equal(lw, lw0) and np.array_equal(xi, xi0)
