@@ -322,7 +322,8 @@ def main():
                                    "fp64": float(kit_64[2][0, 0] * KAPPA_AU_TO_SI)},
             "roofline": roof,
             "autotune": {"chosen_batch": batch,
-                         "probes_ms": [[b, ms_] for b, ms_ in eng.tuning()]},
+                         "probes_ms": [[b, ms_] for b, ms_ in eng.tuning()[:4]],
+                         "whole_triplet_k_psi_probe_ms": [[b, ms_] for b, ms_ in eng.tuning()[4:5]]},
         }
         if not args.no_cpu_baseline:
             try:

@@ -267,8 +267,11 @@ int tc3_get_psi2(tc3_ctx *ctx, int precision, int n, const int64_t *triplets, do
 
 /* Probes of the contraction's batch autotuner since the context was created: up to maxn
  * records of (triplets per launch, best probe time in ms) in the order they ran; *n
- * receives the number of records available.  The chosen batch of the last tc3_scatter is
- * reported by tc3_get_kernel_stats.  No counterpart in the reference (a tuning aid). */
+ * receives the number of records available.  For split-BF16x3 at 48 bands the batch probes
+ * of a shape are followed by one more record: the chosen batch with the last contraction
+ * kernel run as one workgroup per triplet instead of one per 16 b1 (kept if faster).  The
+ * chosen batch of the last tc3_scatter is reported by tc3_get_kernel_stats.  No
+ * counterpart in the reference (a tuning aid). */
 int tc3_get_tuning(const tc3_ctx *ctx, int maxn, double *batch, double *ms, int *n);
 
 /* Device storage of the scattering matrix of the last tc3_scatter: elements per temperature,

@@ -152,6 +152,7 @@ struct tc3_ctx {
     long adjoint_tables_gen = -1;
     // autotuned triplets per launch of the MFMA contraction, by shape, and the probes timed
     std::map<std::string, long> tuned_batch;
+    std::map<std::string, int> tuned_bb;   // b1 blocks per k_psi workgroup, by shape
     // owner of every scattering-matrix entry of the rows of each irreducible q1 (k_xi_asm)
     std::vector<int> xi_src;
     bool xi_src_ok = false;

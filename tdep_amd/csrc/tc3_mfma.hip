@@ -551,23 +551,25 @@ __device__ inline void delta4(float x1, float x2, const float *x3, const float *
 
 template <int NB> struct MaskWords { static constexpr int n = (NB + 31) / 32; };
 
-template <int PREC, int NB, int NTY>
+template <int PREC, int NB, int NTY, int BB = 1>
 size_t psi_lds() {
     typedef typename Stage<PREC>::T S;
     constexpr int NW = NB / 16, NMW = MaskWords<NB>::n;
-    size_t s = sizeof(S) * (2 * NTY * NB + 2 * NB + NW * kChunk * 16 * NTY) + sizeof(float) * NB;
+    size_t s = sizeof(S) * (2 * NTY * NB + 2 * NB + BB * NW * kChunk * 16 * NTY) + sizeof(float) * NB;
     s = (s + 15) & ~(size_t)15;
-    s += (PREC == 2) ? (size_t)(NB / 16) * 4 * 64 * 16 : 0;   // E1 fragments (stacked K)
-    s += sizeof(double) * (NB + 16 + 16 * NTY) + sizeof(float) * NB + sizeof(int) * 16 +
-         sizeof(unsigned) * (size_t)(NB + 16) * NMW;
+    s += (PREC == 2) ? (size_t)BB * (NB / 16) * 4 * 64 * 16 : 0;   // E1 fragments (stacked K)
+    s += sizeof(double) * (NB + BB * 16 + BB * 16 * NTY) + sizeof(float) * NB + sizeof(int) * BB * 16 +
+         sizeof(unsigned) * (size_t)(NB + BB * 16) * NMW;
     s = (s + 15) & ~(size_t)15;
-    return s + sizeof(S) * 3 * (16 + 2 * NB);    // reduced velocities (integration type 6)
+    return s + sizeof(S) * 3 * (BB * 16 + 2 * NB);    // reduced velocities (integration type 6)
 }
 
 // DUMP: every tile is formed (no dead-tile skip) and |Psi|^2 with its prefactor is stored to
 // a.psi_out[t][b1][b2][b3] (diagnostic export; the production instantiations have DUMP off).
-template <int PREC, int NB, int NTY, bool DUMP = false>
-__global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PREC == 2 ? 3 : 1))) k_psi(MfmaEpiArgs a, const void *E1re_, const void *E1im_,
+// BB: blocks of 16 b1 per workgroup (NW waves each, one per 16 b3): with BB = NB/16 one
+// workgroup takes a whole triplet and W is fetched once for all its b1.
+template <int PREC, int NB, int NTY, bool DUMP = false, int BB = 1>
+__global__ void __launch_bounds__(4 * NB * BB) __attribute__((amdgpu_waves_per_eu(PREC == 2 ? 3 : 1))) k_psi(MfmaEpiArgs a, const void *E1re_, const void *E1im_,
                                                 const __bf16 *__restrict__ Et1, const void *W_) {
     typedef typename Traits<PREC>::T T;
     typedef typename Traits<PREC>::Acc Acc;
@@ -577,13 +579,17 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     constexpr int NW = NB / 16, NB2 = NB * NB, KS = KSteps<PREC, NB>::n, KP = kpad(NB), NMW = MaskWords<NB>::n;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nT = a.nT;
-    // XCD-aware order: the b1 blocks of one triplet share blockIdx % 8
+    // XCD-aware order: the workgroups (b1 block groups) of one triplet share blockIdx % 8
+    constexpr int NG = NW / BB;
+    static_assert(NW % BB == 0, "b1 blocks per workgroup divide the b1 blocks");
     const int gid = blockIdx.x;
-    const int tl = (gid / (8 * NW)) * 8 + (gid % 8);
-    const int bb = (gid / 8) % NW;
+    const int tl = (gid / (8 * NG)) * 8 + (gid % 8);
+    const int lane = threadIdx.x & 63, wg = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    // this wave: b1 block bl of the workgroup (block bb of the triplet), b3 block w
+    const int bl = wg / NW, w = wg % NW;
+    const int bb0 = ((gid / 8) % NG) * BB, bb = bb0 + bl;
     if (tl >= a.nt) return;
     const int t = a.t0 + tl;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
     const int q1 = a.tq1[t], q2 = a.tq2[t], q3 = a.tq3[t];
     const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
     const int e0 = a.eq_off[t], e1 = a.eq_off[t + 1], neq = e1 - e0;
@@ -592,28 +598,30 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     S *n3h = n2h + NTY * NB;               // [NTY][NB]     n3 + 1/2
     S *sg2s = n3h + NTY * NB;              // [NB]          sigma^2 of q2 modes
     S *sg3s = sg2s + NB;                   // [NB]          sigma^2 of q3 modes
-    S *Xp = sg3s + NB;                     // [NW][kChunk][16][NTY] per-wave partial sums over b3
-    float *w3f = (float *)(Xp + NW * kChunk * 16 * NTY);   // [NB] omega of q3 modes
-    unsigned char *p8 = smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + 2 * NB + NW * kChunk * 16 * NTY) +
+    S *Xp = sg3s + NB;                     // [BB NW][kChunk][16][NTY] per-wave partial sums over b3
+    float *w3f = (float *)(Xp + BB * NW * kChunk * 16 * NTY);   // [NB] omega of q3 modes
+    unsigned char *p8 = smem + align16(sizeof(S) * (size_t)(2 * NTY * NB + 2 * NB + BB * NW * kChunk * 16 * NTY) +
                                        sizeof(float) * NB);
-    v8bf *e1s = (v8bf *)p8;                // [NB/16 K-steps][4 operands][64 lanes] (BF16X3)
-    if constexpr (PREC == 2) p8 += (size_t)(NB / 16) * 4 * 64 * 16;
+    v8bf *e1sa = (v8bf *)p8;               // [BB][NB/16 K-steps][4 operands][64 lanes] (BF16X3)
+    if constexpr (PREC == 2) p8 += (size_t)BB * (NB / 16) * 4 * 64 * 16;
+    v8bf *e1s = e1sa + (size_t)bl * (NB / 16) * 4 * 64;   // this wave's b1 block
     double *w2s = (double *)p8;            // [NB]
-    double *w1s = w2s + NB;                // [16]
-    double *gam = w1s + 16;                // [NTY][16] linewidth partial sums
-    float *w2f = (float *)(gam + 16 * NTY);   // [NB]
-    int *rows = (int *)(w2f + NB);         // [16]
-    unsigned *v23 = (unsigned *)(rows + 16);   // [NB][NMW]  |w2 - w3| >= tol, by b3
-    unsigned *v12 = v23 + NB * NMW;            // [16][NMW]  b2 allowed for b1, by b2
-    // [16 b1, NB b2, NB b3][3] reduced group velocities (integration type 6)
-    S *u6 = (S *)(smem + align16((size_t)((unsigned char *)(v12 + 16 * NMW) - smem)));
+    double *w1s = w2s + NB;                // [BB 16]
+    double *gam = w1s + BB * 16;           // [BB][NTY][16] linewidth partial sums
+    float *w2f = (float *)(gam + BB * 16 * NTY);   // [NB]
+    int *rows = (int *)(w2f + NB);         // [BB 16]
+    unsigned *v23 = (unsigned *)(rows + BB * 16);   // [NB][NMW]  |w2 - w3| >= tol, by b3
+    unsigned *v12 = v23 + NB * NMW;                 // [BB 16][NMW]  b2 allowed for b1, by b2
+    // [BB 16 b1, NB b2, NB b3][3] reduced group velocities (integration type 6)
+    S *u6 = (S *)(smem + align16((size_t)((unsigned char *)(v12 + BB * 16 * NMW) - smem)));
+    constexpr int M1 = BB * 16;            // b1 of the workgroup
     const bool itype6 = a.integrationtype == 6;
     if (itype6)
-        for (int x = threadIdx.x; x < 3 * (16 + 2 * NB); x += blockDim.x) {
+        for (int x = threadIdx.x; x < 3 * (M1 + 2 * NB); x += blockDim.x) {
             const int m = x / 3, cc = x % 3;
-            const double v = (m < 16)        ? a.u_irr[((size_t)q1 * NB + 16 * bb + m) * 3 + cc]
-                             : (m < 16 + NB) ? a.u_full[((size_t)q2 * NB + m - 16) * 3 + cc]
-                                             : a.u_full[((size_t)q3 * NB + m - 16 - NB) * 3 + cc];
+            const double v = (m < M1)        ? a.u_irr[((size_t)q1 * NB + 16 * bb0 + m) * 3 + cc]
+                             : (m < M1 + NB) ? a.u_full[((size_t)q2 * NB + m - M1) * 3 + cc]
+                                             : a.u_full[((size_t)q3 * NB + m - M1 - NB) * 3 + cc];
             u6[x] = (S)v;
         }
     const double *om2 = a.om_full + (size_t)q2 * NB, *om3 = a.om_full + (size_t)q3 * NB;
@@ -629,12 +637,12 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         sg3s[b] = (S)a.sigsq[(size_t)i3 * NB + b];
         w3f[b] = (float)om3[b];
     }
-    if (threadIdx.x < 16) {
-        const int b1 = 16 * bb + threadIdx.x;
+    if (threadIdx.x < M1) {
+        const int b1 = 16 * bb0 + threadIdx.x;
         w1s[threadIdx.x] = a.om_irr[(size_t)q1 * NB + b1];
         rows[threadIdx.x] = a.row_of[(size_t)q1 * NB + b1];
     }
-    for (int x = threadIdx.x; x < 16 * NTY; x += blockDim.x) gam[x] = 0.0;
+    for (int x = threadIdx.x; x < M1 * NTY; x += blockDim.x) gam[x] = 0.0;
     // pair masks (scattering_threephonon.f90:90-91, 100-101)
     for (int x = threadIdx.x; x < NB * NMW; x += blockDim.x) {
         const int b2 = x / NMW, wd = x % NMW;
@@ -644,9 +652,9 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
             if (!(fabs(w2 - om3[32 * wd + k]) < kTol)) m |= 1u << k;
         v23[x] = m;
     }
-    for (int x = threadIdx.x; x < 16 * NMW; x += blockDim.x) {
+    for (int x = threadIdx.x; x < M1 * NMW; x += blockDim.x) {
         const int m1 = x / NMW, wd = x % NMW;
-        const int b1 = 16 * bb + m1;
+        const int b1 = 16 * bb0 + m1;
         const double w1 = a.om_irr[(size_t)q1 * NB + b1];
         const bool r_ok = a.row_of[(size_t)q1 * NB + b1] >= 0;
         unsigned m = 0;
@@ -665,7 +673,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         // re lo, im hi, im lo, -im hi, -im lo).
         const size_t ps = (size_t)NB * KP;
         const __bf16 *pe = Et1 + (size_t)q1 * kPlanes * ps + (size_t)(16 * bb + c16) * KP + 4 * g;
-        if (w == 0) {
+        if (w == 0) {   // the first wave of each b1 block
             auto cat = [&](int pa, int pb, int s) {
                 const uint2 lo = *(const uint2 *)(pe + pa * ps + 16 * s), hi = *(const uint2 *)(pe + pb * ps + 16 * s);
                 return __builtin_bit_cast(v8bf, make_uint4(lo.x, lo.y, hi.x, hi.y));
@@ -681,13 +689,14 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     }
     __syncthreads();
     // per-lane constants: b1 = 16 bb + c16 and four values of b3
-    const double w1 = w1s[c16];
+    const int m1l = 16 * bl + c16;         // the lane's b1 in the workgroup
+    const double w1 = w1s[m1l];
     const float x1 = (float)w1;
-    const int row1 = rows[c16];
+    const int row1 = rows[m1l];
     const S sg1 = (S)a.sigsq[(size_t)q1 * NB + 16 * bb + c16];
     unsigned v12w[NMW];
 #pragma unroll
-    for (int k = 0; k < NMW; ++k) v12w[k] = v12[c16 * NMW + k];
+    for (int k = 0; k < NMW; ++k) v12w[k] = v12[m1l * NMW + k];
     const int b3w = 16 * w;                         // first b3 of this wave
     const int sh3 = (b3w & 31);                     // bit of b3w in its mask word
     unsigned ok13 = 0;
@@ -755,8 +764,8 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     } else {
         double w1lo = 1e300, w1hi = -1e300, s1m = 0, w3lo = 1e300, w3hi = -1e300, s3m = 0, u1m = 0, u3m = 0;
         for (int m = 0; m < 16; ++m) {
-            w1lo = fmin(w1lo, w1s[m]);
-            w1hi = fmax(w1hi, w1s[m]);
+            w1lo = fmin(w1lo, w1s[16 * bl + m]);
+            w1hi = fmax(w1hi, w1s[16 * bl + m]);
             s1m = fmax(s1m, a.sigsq[(size_t)q1 * NB + 16 * bb + m]);
             w3lo = fmin(w3lo, om3[b3w + m]);
             w3hi = fmax(w3hi, om3[b3w + m]);
@@ -811,7 +820,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     if (nxt < NB) load_w(nxt);
     for (int cb = 0; cb < NB; cb += kChunk) {
     // this wave's partial sums of the chunk start at zero; live b2 overwrite their slots
-    for (int x = lane; x < kChunk * 16 * NTY; x += 64) Xp[w * kChunk * 16 * NTY + x] = (S)0;
+    for (int x = lane; x < kChunk * 16 * NTY; x += 64) Xp[wg * kChunk * 16 * NTY + x] = (S)0;
     unsigned cm = (unsigned)((live[cb >> 6] >> (cb & 63)) & ((1ull << kChunk) - 1));
     while (cm) {
         const int cs = __builtin_ctz(cm);
@@ -892,7 +901,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
                 ((S *)a.psi_out)[(((size_t)t * NB + 16 * bb + c16) * NB + b2) * NB + b3w + out_row<PREC>(lane, r)] = psq[r];
             sgv[r] = itype2 ? sg12 + s3v[r] : sfix2;
             if (itype6) {
-                const S s6 = sigma_type6<S>(u6 + 3 * c16, u6 + 3 * (16 + b2), u6 + 3 * (16 + NB + b3w + out_row<PREC>(lane, r)),
+                const S s6 = sigma_type6<S>(u6 + 3 * m1l, u6 + 3 * (M1 + b2), u6 + 3 * (M1 + NB + b3w + out_row<PREC>(lane, r)),
                                             (S)a.thresh6);
                 sgv[r] = s6 * s6;
             }
@@ -919,7 +928,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         }
         if (c16 < nT) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Xp[((w * kChunk + cs) * 16 + out_row<PREC>(lane, r)) * NTY + c16] = (S)xa[r];
+            for (int r = 0; r < 4; ++r) Xp[((wg * kChunk + cs) * 16 + out_row<PREC>(lane, r)) * NTY + c16] = (S)xa[r];
         }
         // sums over b2, kept per lane
 #pragma unroll
@@ -938,28 +947,28 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
             // scattering matrix (:126-133).  Every target (row, q', band) belongs to one
             // triplet and, inside it, to one equivalent pair, so values are stored.
             __syncthreads();
-            for (int x = threadIdx.x; x < 16 * nT; x += blockDim.x) {
-                const int m = x % 16, T_ = x / 16;
+            for (int x = threadIdx.x; x < M1 * nT; x += blockDim.x) {
+                const int mb = x % M1, T_ = x / M1, m = mb % 16, xb = mb / 16;
                 S v[kChunk];
                 double gs = 0;
 #pragma unroll
                 for (int c = 0; c < kChunk; ++c) {
                     S s = 0;
 #pragma unroll
-                    for (int ww = 0; ww < NW; ++ww) s += Xp[((ww * kChunk + c) * 16 + m) * NTY + T_];
+                    for (int ww = 0; ww < NW; ++ww) s += Xp[(((xb * NW + ww) * kChunk + c) * 16 + m) * NTY + T_];
                     v[c] = s;
                     gs += (double)s;
                 }
-                gam[T_ * 16 + m] += gs;
-                const int r = rows[m];
+                gam[(xb * NTY + T_) * 16 + m] += gs;
+                const int r = rows[mb];
                 if (r < 0) continue;
                 if (a.comp) {
-                    S *cp = (S *)a.comp + (size_t)T_ * a.comp_T + ((size_t)t * NB + 16 * bb + m) * 2 * NB + cb;
+                    S *cp = (S *)a.comp + (size_t)T_ * a.comp_T + ((size_t)t * NB + 16 * bb0 + mb) * 2 * NB + cb;
 #pragma unroll
                     for (int c = 0; c < kChunk; ++c) cp[c] = v[c];
                     continue;
                 }
-                const double iw1 = 2.0 / w1s[m];
+                const double iw1 = 2.0 / w1s[mb];
                 for (int e = e0; e < e1; ++e) {
                     XT *xr = xi + (size_t)T_ * a.xi_T + (size_t)r * a.xi_row + (size_t)a.eq_q2[e] * NB + cb;
 #pragma unroll
@@ -970,9 +979,10 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
         }
     }
     // linewidth partials (g0 += f0 + f1 for every equivalent triplet, :126-128)
-    for (int x = threadIdx.x; x < nT * 16; x += blockDim.x) {
-        const int T_ = x / 16, m = x % 16;
-        a.gpart[((size_t)t * a.gT + a.T0 + T_) * NB + 16 * bb + m] = (rows[m] >= 0) ? gam[x] * (double)neq : 0.0;
+    for (int x = threadIdx.x; x < nT * M1; x += blockDim.x) {
+        const int T_ = x / M1, mb = x % M1, m = mb % 16, xb = mb / 16;
+        a.gpart[((size_t)t * a.gT + a.T0 + T_) * NB + 16 * bb0 + mb] =
+            (rows[mb] >= 0) ? gam[(xb * NTY + T_) * 16 + m] * (double)neq : 0.0;
     }
     // q3' entries (:134) from this lane's registers; where q3' = q2' the entry already
     // holds the q2' term written above by this workgroup, and the q3' term is added.
@@ -1008,7 +1018,7 @@ __global__ void __launch_bounds__(4 * NB) __attribute__((amdgpu_waves_per_eu(PRE
     }
 }
 
-template <int PREC, int NB, int NTY, bool DUMP = false>
+template <int PREC, int NB, int NTY, bool DUMP = false, int BB = 1>
 int run_chain(const MfmaPlan &p, hipStream_t st) {
     constexpr int NW = NB / 16;
     UwArgs u;
@@ -1017,11 +1027,11 @@ int run_chain(const MfmaPlan &p, hipStream_t st) {
     u.a1_off = p.a1off; u.blist = p.blist; u.a1_boff = p.a1_boff; u.a12_boff = p.a12_boff;
     u.Ef_re = p.Efre; u.Ef_im = p.Efim; u.Et = (const __bf16 *)p.Etf; u.W = p.work;
     const size_t sh_uw = uw_lds<PREC, NB>(u.maxblk, p.maxa1);
-    const size_t sh_psi = psi_lds<PREC, NB, NTY>();
+    const size_t sh_psi = psi_lds<PREC, NB, NTY, BB>();
     hipError_t e = hipFuncSetAttribute((const void *)k_uw<PREC, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)sh_uw);
     if (e != hipSuccess) return device_error(e, "k_uw LDS size", __FILE__, __LINE__);
-    e = hipFuncSetAttribute((const void *)k_psi<PREC, NB, NTY, DUMP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    e = hipFuncSetAttribute((const void *)k_psi<PREC, NB, NTY, DUMP, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sh_psi);
     if (e != hipSuccess) return device_error(e, "k_psi LDS size", __FILE__, __LINE__);
     MfmaEpiArgs ea = p.epi;
@@ -1035,7 +1045,8 @@ int run_chain(const MfmaPlan &p, hipStream_t st) {
         ea.t0 = (int)t0;
         ea.nt = nt;
         const int ngrp = (nt + 7) / 8;
-        hipLaunchKernelGGL((k_psi<PREC, NB, NTY, DUMP>), dim3(ngrp * 8 * NW), dim3(64 * NW), sh_psi, st, ea, p.E1re, p.E1im,
+        hipLaunchKernelGGL((k_psi<PREC, NB, NTY, DUMP, BB>), dim3(ngrp * 8 * (NW / BB)), dim3(64 * NW * BB), sh_psi, st,
+                           ea, p.E1re, p.E1im,
                            (const __bf16 *)p.Eti, (const void *)p.work);
         if (ev) (void)hipEventRecord(ev[2], st);
         e = hipGetLastError();
@@ -1065,6 +1076,16 @@ int prepare(const MfmaPlan &p, hipStream_t st) {
 
 template <int PREC, int NB>
 int dispatch_nty(const MfmaPlan &p, hipStream_t st) {
+    // one workgroup per triplet in k_psi: split-BF16x3 at 48 bands
+    if constexpr (PREC == 2 && NB == 48) {
+        if (p.psi_bb == NB / 16) {
+            switch (p.nty) {
+            case 4: return run_chain<PREC, NB, 4, false, NB / 16>(p, st);
+            case 8: return run_chain<PREC, NB, 8, false, NB / 16>(p, st);
+            default: return run_chain<PREC, NB, 16, false, NB / 16>(p, st);
+            }
+        }
+    }
     switch (p.nty) {
     case 4: return run_chain<PREC, NB, 4>(p, st);
     case 8: return run_chain<PREC, NB, 8>(p, st);
@@ -1094,18 +1115,19 @@ int dispatch_psi2(const MfmaPlan &p, hipStream_t st) {
     }
 }
 
-// LDS bytes of the two kernels for band count nb and temperature block nty.
-size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1) {
+// LDS bytes of the two kernels for band count nb and temperature block nty (k_psi with
+// bb blocks of 16 b1 per workgroup).
+size_t lds_need(int nb, int nty, int precision, int maxblk, int maxa1, int bb = 1) {
     const size_t es = (precision == TC3_PREC_FP64) ? 8 : 4;
     const size_t kp = kpad(nb), nw = nb / 16, nmw = (nb + 31) / 32;
     size_t uw = (precision == TC3_PREC_BF16X3) ? 2 * 4 * nb * (kp + 8) : es * 2 * nb * nb;
     uw += 2 * es * 27 * (size_t)maxblk + 16 * (size_t)maxblk;
     uw = ((uw + 15) & ~(size_t)15) + 16 * (size_t)std::max(maxa1, 1);
-    size_t ps = es * (2 * nty * nb + 2 * nb + nw * kChunk * 16 * nty) + 4 * nb;
+    size_t ps = es * (2 * nty * nb + 2 * nb + bb * nw * kChunk * 16 * nty) + 4 * nb;
     ps = ((ps + 15) & ~(size_t)15);
-    if (precision == TC3_PREC_BF16X3) ps += (size_t)(nb / 16) * 4 * 64 * 16;
-    ps += 8 * (nb + 16 + 16 * nty) + 4 * nb + 4 * 16 + 4 * (nb + 16) * nmw;
-    ps = ((ps + 15) & ~(size_t)15) + es * 3 * (16 + 2 * (size_t)nb);
+    if (precision == TC3_PREC_BF16X3) ps += (size_t)bb * (nb / 16) * 4 * 64 * 16;
+    ps += 8 * (nb + 16 * bb + 16 * bb * nty) + 4 * nb + 4 * 16 * bb + 4 * (nb + 16 * bb) * nmw;
+    ps = ((ps + 15) & ~(size_t)15) + es * 3 * (16 * bb + 2 * (size_t)nb);
     return std::max(uw, ps);
 }
 
@@ -1140,7 +1162,7 @@ int run_mfma_psi2(const MfmaPlan &p, int precision, hipStream_t st) {
 }
 
 int run_mfma_chain(const MfmaPlan &p, int precision, hipStream_t st) {
-    if (lds_need(p.nb, p.nty, precision, p.maxblk, p.maxa1) > 160 * 1024) return 5;
+    if (lds_need(p.nb, p.nty, precision, p.maxblk, p.maxa1, p.psi_bb) > 160 * 1024) return 5;
     switch (precision) {
     case TC3_PREC_FP64: return dispatch_nb<0>(p, st);
     case TC3_PREC_FP32: return dispatch_nb<1>(p, st);

@@ -41,6 +41,7 @@ struct MfmaPlan {
     const double *qcart, *lv2, *lv3, *mwm;
     MfmaEpiArgs epi;
     hipEvent_t *ev;   // 3 events per launch pair (before k_uw, between, after k_psi), or null
+    int psi_bb = 1;   // blocks of 16 b1 per k_psi workgroup: 1, or nb/16 (split-BF16x3, 48 bands)
 };
 
 // Whether the MFMA kernels handle this band count and precision, and the temperature

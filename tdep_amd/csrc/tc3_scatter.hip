@@ -932,7 +932,7 @@ static void chain_key(std::vector<long> &k, const MfmaPlan &p, int precision, in
         k.push_back(v);
     };
     I(precision); I(p.na); I(p.nb); I(p.nirr); I(p.nfull); I(p.maxa1); I(p.maxblk); I(p.nty); I(p.ntrip);
-    I(p.batch);
+    I(p.batch); I(p.psi_bb);
     P(p.egv_irr); P(p.egv_full); P(p.om_irr); P(p.om_full); P(p.E1re); P(p.E1im); P(p.Efre); P(p.Efim);
     P(p.work); P(p.Eti); P(p.Etf); P(p.tq2); P(p.tq3); P(p.a1off); P(p.blist); P(p.a1_boff); P(p.a12_boff);
     P(p.qcart); P(p.lv2); P(p.lv3); P(p.mwm);
@@ -1337,6 +1337,8 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         if (mp.nty == 0) { cleanup(); return c->fail(5, "band count too large for the MFMA kernels' LDS"); }
         mp.ntrip = ntrip; mp.batch = mbatch;
         mp.egv_irr = c->d_egv_irr; mp.egv_full = c->d_egv_full; mp.om_irr = c->d_om_irr; mp.om_full = c->d_om_full;
+        if (const char *eb = std::getenv("TC3_PSI_BB"))
+            mp.psi_bb = (precision == TC3_PREC_BF16X3 && nb == 48 && std::atoi(eb) == nb / 16) ? nb / 16 : 1;
         // the fused kernel (48 bands, split-BF16x3, compact sums) needs neither W nor the
         // chain's operand planes
         // IFC blocks (a1, a2, a3) in entry order and the (a2, a3) pairs with their blocks, for the
@@ -1470,9 +1472,13 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
         if (!env_mb) {
             const std::string key = std::to_string(nb) + "/" + std::to_string(precision) + "/" +
                                     std::to_string(mp.nty) + "/" + std::to_string(ntrip);
+            // the workgroup shape of k_psi: one b1 block (the default) or the whole triplet
+            // (split-BF16x3, 48 bands), timed at the best batch unless TC3_PSI_BB forces one
+            const bool try_bb = precision == TC3_PREC_BF16X3 && nb == 48 && std::getenv("TC3_PSI_BB") == nullptr;
             auto it = c->tuned_batch.find(key);
             if (it != c->tuned_batch.end()) {
                 mbatch = std::min<long>(it->second, bmax);
+                if (try_bb) mp.psi_bb = c->tuned_bb[key];
             } else {
                 std::vector<long> cand;
                 for (long b = bmax; cand.size() < 4 && (b >= 64 || cand.empty()); b /= 2) cand.push_back(b);
@@ -1512,6 +1518,30 @@ int run_scatter(tc3_ctx *c, int precision, int integrationtype, double sigma) {
                     }
                     c->tune_log.push_back({key, (double)b, (double)msb});
                     if (msb < best) { best = msb; mbatch = b; }
+                }
+                if (try_bb) {
+                    MfmaPlan pb = pp;
+                    pb.psi_bb = nb / 16;
+                    // untimed first launch of the other kernel, then best of two at the best batch
+                    pb.ntrip = std::min<long>(ntrip, 64);
+                    pb.batch = pb.ntrip;
+                    int r = run_mfma_chain(pb, precision, st);
+                    pb.ntrip = bmax;
+                    pb.batch = mbatch;
+                    float msb = 1e30f;
+                    for (int rep = 0; r == 0 && rep < 2; ++rep) {
+                        TRY(hipEventRecord(p0, st));
+                        r = run_mfma_chain(pb, precision, st);
+                        TRY(hipEventRecord(p1, st));
+                        TRY(hipEventSynchronize(p1));
+                        float ms = 0;
+                        hipEventElapsedTime(&ms, p0, p1);
+                        msb = std::min(msb, ms);
+                    }
+                    if (r) { hipEventDestroy(p0); hipEventDestroy(p1); cleanup(); return r; }
+                    c->tune_log.push_back({key + "/whole-triplet k_psi", (double)mbatch, (double)msb});
+                    c->tuned_bb[key] = (msb < best) ? nb / 16 : 1;
+                    mp.psi_bb = c->tuned_bb[key];
                 }
                 hipEventDestroy(p0);
                 hipEventDestroy(p1);

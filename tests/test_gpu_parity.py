@@ -321,3 +321,19 @@ def test_contraction_graph_replay_matches_direct_launches(inputs_dir, monkeypatc
                 lw = np.array([e.linewidths(i)[0] for i in range(len(T))])
                 xi = np.array([e.scattering_matrix(i, 0, 32)[0] for i in range(len(T))])
                 assert np.array_equal(lw, lw0) and np.array_equal(xi, xi0)
+
+
+def test_whole_triplet_psi_workgroup_matches(inputs_dir, monkeypatch):
+    """The last contraction kernel with one workgroup per triplet (TC3_PSI_BB=3: the three
+    blocks of 16 b1 in one workgroup, the shape the autotuner also times) gives the same
+    linewidths and scattering-matrix rows, bit for bit, as one workgroup per block of 16 b1,
+    on the 16-atom superlattice in split-BF16x3 with 7 temperatures."""
+    e = _engine(inputs_dir, "sl16", (4, 4, 2))
+    T = [100.0 * i for i in range(1, 8)]
+    out = []
+    for bb in ("1", "3"):
+        monkeypatch.setenv("TC3_PSI_BB", bb)
+        e.scattering_rates(T, precision="bf16x3")
+        out.append((np.array([e.linewidths(i)[0] for i in range(7)]),
+                    np.array([e.scattering_matrix(i, 0, 48)[0] for i in range(7)])))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
