@@ -29,6 +29,7 @@ int device_error(hipError_t e, const char *what, const char *file, int line) {
 
 tc3_ctx::~tc3_ctx() {
     if (bte_exec) hipGraphExecDestroy(bte_exec);
+    if (bte_exec2) hipGraphExecDestroy(bte_exec2);
     if (sc_exec) hipGraphExecDestroy(sc_exec);
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     if (pin) hipHostFree(pin);

@@ -197,8 +197,8 @@ struct tc3_ctx {
     }
     struct TuneProbe { std::string key; double batch, ms; };
     std::vector<TuneProbe> tune_log;
-    // instantiated BTE iteration graph and the arguments it was captured with
-    hipGraphExec_t bte_exec = nullptr;
+    // instantiated BTE iteration graphs and the arguments they were captured with
+    hipGraphExec_t bte_exec = nullptr, bte_exec2 = nullptr;   // first chunk, later chunks
     std::vector<long> bte_key;
     // instantiated graph of the scattering pass's contraction launches and the arguments it
     // was captured with (sc_key_failed: the arguments a capture failed for, not retried)

@@ -1426,11 +1426,12 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                                d_vel, d_opm, d_klog, d_st, sQ, sF, sLog);
             hipLaunchKernelGGL(k_bte_check, dim3(1, 1, nT), dim3(64), 0, st, nirr, nb, d_g0, d_scf, d_st, sG, sScf);
         };
-        // chunks of kChunkIt iterations captured once as a graph and replayed until the
-        // device reports every temperature done; direct launches if capture is not available.
-        // The instantiated graph is kept in the context while every argument it captured
+        // a chunk of kChunkIt iterations, then chunks of kChunkIt2 until the device reports
+        // every temperature done (iterations past convergence are no-ops), each chunk captured
+        // once as a graph and replayed; direct launches if capture is not available.  The
+        // instantiated graphs are kept in the context while every argument they captured
         // (buffers, sizes, strides) stays the same.
-        constexpr int kChunkIt = 8;
+        constexpr int kChunkIt = 8, kChunkIt2 = 4;
         const std::vector<long> key = {(long)(uintptr_t)c->d_xi, (long)c->xi_es, (long)c->xi_row, nrows, nfull, nb,
                                        nirr, nT, nsym, (long)(uintptr_t)c->d_irr_of_full, (long)(uintptr_t)d_Fn,
                                        (long)(uintptr_t)d_F0, (long)(uintptr_t)d_qs, (long)(uintptr_t)d_Fbb,
@@ -1444,30 +1445,36 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
                                        (long)(uintptr_t)d_xorg, (long)(uintptr_t)d_xorow, (long)(uintptr_t)d_xogoff,
                                        (long)(uintptr_t)d_xoent, (long)(uintptr_t)d_xomem, (long)(uintptr_t)d_Fo,
                                        (long)units.size(), (long)(uintptr_t)d_units};
-        if (!scratch && c->bte_exec && c->bte_key != key) {
-            (void)hipGraphExecDestroy(c->bte_exec);
-            c->bte_exec = nullptr;
+        if (!scratch && c->bte_key != key) {
+            for (hipGraphExec_t *g : {&c->bte_exec, &c->bte_exec2})
+                if (*g) {
+                    (void)hipGraphExecDestroy(*g);
+                    *g = nullptr;
+                }
         }
-        if (!scratch && !c->bte_exec && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        auto capture = [&](hipGraphExec_t *exec, int nit) {
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
             hipGraph_t graph = nullptr;
-            for (int k = 0; k < kChunkIt; ++k) iteration();
-            if (hipStreamEndCapture(st, &graph) == hipSuccess &&
-                hipGraphInstantiate(&c->bte_exec, graph, nullptr, nullptr, 0) == hipSuccess)
-                c->bte_key = key;
-            else
-                c->bte_exec = nullptr;
+            for (int k = 0; k < nit; ++k) iteration();
+            if (hipStreamEndCapture(st, &graph) != hipSuccess || hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                *exec = nullptr;
             if (graph) (void)hipGraphDestroy(graph);
+        };
+        if (!scratch && (!c->bte_exec || !c->bte_exec2)) {
+            if (!c->bte_exec) capture(&c->bte_exec, kChunkIt);
+            if (!c->bte_exec2) capture(&c->bte_exec2, kChunkIt2);
+            c->bte_key = key;
         }
         (void)hipGetLastError();
-        hipGraphExec_t gexec = scratch ? nullptr : c->bte_exec;
-        for (bool done = false; !done;) {
+        for (int chunk = 0, done = 0; !done; ++chunk) {
+            const hipGraphExec_t gexec = scratch ? nullptr : (chunk == 0 ? c->bte_exec : c->bte_exec2);
             if (gexec) TRY(hipGraphLaunch(gexec, st));
             else
-                for (int k = 0; k < kChunkIt; ++k) iteration();
+                for (int k = 0; k < (chunk == 0 ? kChunkIt : kChunkIt2); ++k) iteration();
             TRY(hipGetLastError());
             TRY(hipMemcpyAsync(hs.data(), d_st, sizeof(BteState) * nT, hipMemcpyDeviceToHost, st));
             TRY(hipStreamSynchronize(st));
-            done = true;
+            done = 1;
             for (int iT = 0; iT < nT; ++iT) done = done && hs[iT].done;
         }
     }
