@@ -579,16 +579,18 @@ __global__ void __launch_bounds__(4 * NB * BB) __attribute__((amdgpu_waves_per_e
     constexpr int NW = NB / 16, NB2 = NB * NB, KS = KSteps<PREC, NB>::n, KP = kpad(NB), NMW = MaskWords<NB>::n;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nT = a.nT;
-    // XCD-aware order: the workgroups (b1 block groups) of one triplet share blockIdx % 8
+    // XCD-aware order: the workgroups (b1 block groups) of one triplet share blockIdx % 8.
+    // The batch's triplets are taken last first: the W that k_uw wrote last is the part still
+    // in the last-level cache when this launch starts.
     constexpr int NG = NW / BB;
     static_assert(NW % BB == 0, "b1 blocks per workgroup divide the b1 blocks");
     const int gid = blockIdx.x;
-    const int tl = (gid / (8 * NG)) * 8 + (gid % 8);
+    const int tl = a.nt - 1 - ((gid / (8 * NG)) * 8 + (gid % 8));
     const int lane = threadIdx.x & 63, wg = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
     // this wave: b1 block bl of the workgroup (block bb of the triplet), b3 block w
     const int bl = wg / NW, w = wg % NW;
     const int bb0 = ((gid / 8) % NG) * BB, bb = bb0 + bl;
-    if (tl >= a.nt) return;
+    if (tl < 0) return;
     const int t = a.t0 + tl;
     const int q1 = a.tq1[t], q2 = a.tq2[t], q3 = a.tq3[t];
     const int i2 = a.irr_of_full[q2], i3 = a.irr_of_full[q3];
