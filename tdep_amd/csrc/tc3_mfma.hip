@@ -551,6 +551,17 @@ __device__ inline void delta4(float x1, float x2, const float *x3, const float *
 
 template <int NB> struct MaskWords { static constexpr int n = (NB + 31) / 32; };
 
+// The value v holds in lane l of the wave (every lane active).
+__device__ inline float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline double lane_value(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 template <int PREC, int NB, int NTY, int BB = 1>
 size_t psi_lds() {
     typedef typename Stage<PREC>::T S;
@@ -818,6 +829,25 @@ __global__ void __launch_bounds__(4 * NB * BB) __attribute__((amdgpu_waves_per_e
         }
         return NB;
     };
+    // per-lane values of this lane's four b3, the same for every b2: frequencies and sigma^2
+    // (4 consecutive values outside FP64) and the occupations n3 + 1/2 of the lane's
+    // temperature column
+    S w3v[4], s3v[4], bmv[4];
+    if constexpr (PREC == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int b3 = b3w + out_row<PREC>(lane, r);
+            w3v[r] = om3[b3];
+            s3v[r] = sg3s[b3];
+        }
+    } else {
+        const int b30 = b3w + out_row<PREC>(lane, 0);
+        const float4 wv = *(const float4 *)(w3f + b30), sv = *(const float4 *)(sg3s + b30);
+        w3v[0] = wv.x; w3v[1] = wv.y; w3v[2] = wv.z; w3v[3] = wv.w;
+        s3v[0] = sv.x; s3v[1] = sv.y; s3v[2] = sv.z; s3v[3] = sv.w;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bmv[r] = (c16 < nT) ? n3h[tcol * NB + b3w + out_row<PREC>(lane, r)] : (S)0;
     int nxt = next_live(0);
     if (nxt < NB) load_w(nxt);
     for (int cb = 0; cb < NB; cb += kChunk) {
@@ -874,21 +904,6 @@ __global__ void __launch_bounds__(4 * NB * BB) __attribute__((amdgpu_waves_per_e
         const unsigned ok2 = (v12w[b2 >> 5] >> (b2 & 31)) & 1u;
         const unsigned okm = ok2 ? (ok13 & v23[b2 * NMW + (b3w >> 5)]) : 0u;
         const S sg12 = sg1 + sg2s[b2];
-        // frequencies and sigma^2 of this lane's b3 (4 consecutive values outside FP64)
-        S w3v[4], s3v[4];
-        if constexpr (PREC == 0) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int b3 = b3w + out_row<PREC>(lane, r);
-                w3v[r] = om3[b3];
-                s3v[r] = sg3s[b3];
-            }
-        } else {
-            const int b30 = b3w + out_row<PREC>(lane, 0);
-            const float4 wv = *(const float4 *)(w3f + b30), sv = *(const float4 *)(sg3s + b30);
-            w3v[0] = wv.x; w3v[1] = wv.y; w3v[2] = wv.z; w3v[3] = wv.w;
-            s3v[0] = sv.x; s3v[1] = sv.y; s3v[2] = sv.z; s3v[3] = sv.w;
-        }
         const S x2 = (PREC == 0) ? (S)w2s[b2] : (S)w2f[b2];
         // Gaussian terms that reach this tile (wave-uniform)
         const int wb = b2 >> 6, sb = b2 & 63;
@@ -924,19 +939,18 @@ __global__ void __launch_bounds__(4 * NB * BB) __attribute__((amdgpu_waves_per_e
         Acc xa = {0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const S bm = (c16 < nT) ? n3h[tcol * NB + b3w + out_row<PREC>(lane, r)] : (S)0;
-            mma1(xa, M[r], bm);
+            mma1(xa, M[r], bmv[r]);
             mma1(xa, P[r], bp);
         }
         if (c16 < nT) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Xp[((wg * kChunk + cs) * 16 + out_row<PREC>(lane, r)) * NTY + c16] = (S)xa[r];
         }
-        // sums over b2, kept per lane
+        // sums over b2, kept per lane; n2 + 1/2 of temperature T at this b2 is bp of lane T
 #pragma unroll
         for (int T_ = 0; T_ < NTY; ++T_) {
             if (T_ < nT) {
-                const S nh = n2h[T_ * NB + b2];
+                const S nh = lane_value(bp, T_);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) y[T_][r] += P[r] * nh;
             }
