@@ -681,9 +681,10 @@ __global__ void k_orbsum(int nent, int nfull, int nb, const int4 *__restrict__ e
 // k_xi_gemv_w over rows stored per orbit: workgroup per (work unit, temperature), a unit being
 // one column slice of a row group (up to 16 rows of one q1, consecutive, norb nb elements
 // each) against the orbit sums of the group's orbit list (the folded-out F itself where every
-// orbit is one point).  A row of ncol elements is cut into ns = ceil(kSlices ncol / ncol_max)
-// slices, so that every slice holds about as many columns as one of the longest rows'; the
-// last slice's workgroup writes zero partial sums into the slots after it.  Units come in
+// orbit is one point).  A row of ncol elements is cut into ns = ceil(S ncol / ncol_max)
+// slices (S = 4 for FP32 rows, kSlices for FP64 rows), so that every slice holds about as many
+// columns as one of the longest rows'; the last slice's workgroup writes zero partial sums into
+// the slots after it, up to kSlices.  Units come in
 // decreasing size with the temperature fastest, so the short ones fill the end of the launch.
 static_assert(kRows * (kBlock / 64) >= 16, "a workgroup takes a whole row group");
 template <typename XT, int VW, typename FT>
@@ -1219,7 +1220,8 @@ int run_kappa(tc3_ctx *c, int maxiter, double tol, bool offdiag, void *scratch, 
         std::vector<std::pair<long, int4>> wu;
         for (size_t g = 0; g < c->xo_rg.size(); ++g) {
             const int4 rg = c->xo_rg[g];
-            const int ns = (int)std::min<long>(kSlices, (kSlices * (long)rg.w + ncol_max - 1) / ncol_max);
+            const long smax = (c->xi_es == 4) ? std::min(4, kSlices) : kSlices;
+            const int ns = (int)std::min<long>(smax, (smax * (long)rg.w + ncol_max - 1) / ncol_max);
             for (int sl = 0; sl < ns; ++sl) wu.push_back({(long)rg.y * ((long)rg.w / ns), make_int4((int)g, sl, ns, 0)});
         }
         std::stable_sort(wu.begin(), wu.end(), [](const std::pair<long, int4> &a, const std::pair<long, int4> &b) {
